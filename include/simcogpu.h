@@ -1,0 +1,141 @@
+/*
+ * simcogpu -- MI355X (gfx950) projection step and interface transport for
+ * simcoflow-style 2D cut-cell two-phase flow.  Plain C ABI: plain pointers
+ * and sizes, no C++ or torch types, so a Fortran host binds it with
+ * iso_c_binding and Python with ctypes.
+ *
+ * Array layout is the reference's Fortran layout (column-major, x fastest):
+ *   cell array      (1:ni,1:nj)        ni*nj doubles, element (i,j) at (i-1)+(j-1)*ni
+ *   ghosted array   (0:ni+1,0:nj+1)    (ni+2)*(nj+2) doubles, element (i,j) at i+j*(ni+2)
+ * Host-pointer entry points copy in/out and return after the device work has
+ * finished; the *_resident entry points work on the context's device fields.
+ *
+ * Return codes: 0 OK; negative = argument / HIP / allocation error;
+ * positive = numerical (1 no convergence within max_iter, 2 NaN found,
+ * 3 value guard tripped).  sg_last_error() gives the message.
+ */
+#ifndef SIMCOGPU_H
+#define SIMCOGPU_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SG_OK 0
+#define SG_ERR_ARG -1
+#define SG_ERR_HIP -2
+#define SG_ERR_ALLOC -3
+#define SG_ERR_STATE -4
+#define SG_NOT_CONVERGED 1
+#define SG_NAN 2
+#define SG_GUARD 3
+
+#define SG_PREC_FP64 0  /* every solver kernel in FP64 */
+#define SG_PREC_MIXED 1 /* FP32 V-cycle, FP64 outer residual / dots / coarsest solve */
+
+typedef struct sg_ctx sg_ctx;
+
+typedef struct sg_params {
+    /* fluids: liquid/gas density and the reference density
+     * (StateVariables.f90:26-27, Main.f90:148: Roref = row) */
+    double rho_l, rho_g, rho_ref;
+    /* kinematic viscosities, nu_ref, Reynolds and Froude numbers (predictor) */
+    double nu_l, nu_g, nu_ref, Re, Fr;
+    /* pressure boundary conditions W,E,S,N: 1 Neumann, 0 Dirichlet
+     * (ProjectionP_Mod.f90:55,64 use 1,1,1,0) */
+    int bc_w, bc_e, bc_s, bc_n;
+    int precision;   /* SG_PREC_FP64 or SG_PREC_MIXED */
+    int max_iter;    /* PCG iteration cap (ProjectionP_Mod.f90:108 uses 50) */
+    double tol;      /* stop at ||b-Ap||_2 <= tol*||b||_2 */
+    int mg_pre, mg_post; /* smoothing sweeps per level (ProjectionP_Mod.f90:126: 1) */
+    int warm_start;  /* 0: x0 = 0 (ProjectionP_Mod.f90:319); 1: x0 = previous solution */
+    int check_every; /* PCG iterations between host convergence checks (>= 1) */
+} sg_params;
+
+typedef struct sg_solve_info {
+    int iterations;
+    int status;
+    double relres;           /* ||b-Ap||_2/||b||_2, recomputed in FP64 after the solve */
+    double max_row_residual; /* max |(Ap-b)_i| over fluid rows (ProjectionP_Mod.f90:70-93) */
+    double ms;               /* device time of the whole call */
+    double setup_ms;         /* coefficient build + multigrid setup */
+} sg_solve_info;
+
+/* fields held on the device by a context (sg_field_ptr / upload / download) */
+enum sg_field {
+    /* P-cell arrays (ni*nj) */
+    SG_F_VOF = 0, SG_F_PHI, SG_F_VOFS, SG_F_NYS, SG_F_WLLH, SG_F_NXS, SG_F_PHIS,
+    SG_F_NX, SG_F_NY, SG_F_MRES, SG_F_VOFSU, SG_F_VOFSV,
+    /* ghosted arrays ((ni+2)*(nj+2)) */
+    SG_F_U, SG_F_V, SG_F_P, SG_F_UPRED, SG_F_VPRED, SG_F_PP, SG_F_DPU, SG_F_DPV,
+    SG_F_AE, SG_F_AW, SG_F_AN, SG_F_AS,
+    /* int32 cell arrays (ni*nj): UCell/VCell Posnu/=-1 .and. MoExCell/=1 */
+    SG_F_UFLAG, SG_F_VFLAG,
+    SG_F_COUNT
+};
+
+const char* sg_last_error(void);
+int sg_params_default(sg_params* p);
+
+/* Context: grid spacings of the tensor-product grid of Mesh_Mod.f90:54-175,
+ * dx[ni], dy[nj] of the P cells, udx[ni] of the U cells and vdy[nj] of the V
+ * cells.  Allocates every device buffer once (no allocation afterwards). */
+int sg_ctx_create(int ni, int nj, const double* dx, const double* dy,
+                  const double* udx, const double* vdy, const sg_params* p,
+                  sg_ctx** out);
+int sg_ctx_destroy(sg_ctx* ctx);
+int sg_set_params(sg_ctx* ctx, const sg_params* p);
+
+/* device fields */
+void* sg_field_ptr(sg_ctx* ctx, int field);
+int sg_upload(sg_ctx* ctx, int field, const void* host);
+int sg_download(sg_ctx* ctx, int field, void* host);
+int sg_synchronize(sg_ctx* ctx);
+
+/* Cut-cell geometry computed by Cutcell_Mod (Grid_Preprocess,
+ * Cutcell_Mod.f90:18-32): P-cell vofS, nyS, WlLh, the ghosted P apertures
+ * E/W/N/S, and the U/V flags and solid fractions used by the correction. */
+int sg_set_geometry(sg_ctx* ctx, const double* vofS, const double* nyS,
+                    const double* WlLh, const double* aE, const double* aW,
+                    const double* aN, const double* aS, const int* uflag,
+                    const int* vflag, const double* vofSU, const double* vofSV);
+
+/* ---- projection (replaces ProjectionP_Mod.f90:26-99) --------------------- */
+/* Builds the operator from the device fields VOF, PHI, DPU, DPV and the
+ * geometry, and the RHS from UPRED, VPRED (ComputePossionMatrixCoefficient
+ * :395-580, SetPoissonMatrix :137-273, SetPoissonVectors :275-354). */
+int sg_poisson_build_resident(sg_ctx* ctx, double vb);
+/* PCG + multigrid solve of the built system, then scatter/ghost fill into the
+ * device field PP (DeltaPressureGetValues/BoundaryCondition :356-393,582-595). */
+int sg_poisson_solve_resident(sg_ctx* ctx, sg_solve_info* info);
+/* Drop-in body of PoissonEquationSolver: host in, host out.  u_star/v_star =
+ * TPred%u/v, DpU/DpV = PU%Dp/PV%Dp (ghosted), vof/phi = PCell%vof/phi,
+ * Pp_out = Proj%Pp (ghosted). */
+int sg_poisson_solve(sg_ctx* ctx, const double* u_star, const double* v_star,
+                     const double* DpU, const double* DpV, const double* vof,
+                     const double* phi, double vb, double* Pp_out,
+                     sg_solve_info* info);
+
+/* test hooks: the assembled rows in the reference's matr layout
+ * (ni,nj,5: S,W,diag,E,N; ProjectionP_Mod.f90:183-252), the RHS, and
+ * y = A x with the device operator (x, y cell arrays, host pointers). */
+int sg_poisson_matrix(sg_ctx* ctx, double* matr_out);
+int sg_poisson_rhs(sg_ctx* ctx, double* rhs_out);
+int sg_poisson_apply(sg_ctx* ctx, const double* x, double* y);
+
+/* ---- velocity correction (replaces ComputePUV_Mod.f90:59-217,227-245) ---- */
+/* device fields in: UPRED, VPRED, PP, DPU, DPV, VOF, PHI + geometry;
+ * in/out: U, V, P; out: MRES. */
+int sg_velocity_correct_resident(sg_ctx* ctx, double vb);
+
+/* ---- measurement ----------------------------------------------------------
+ * Average device time (HIP events on the context's stream) of `reps`
+ * back-to-back launches of one kernel on the built operator: which = 0 is
+ * the fine-level matrix-free apply y = A x (5 FP64 words per cell).  No
+ * counterpart in the reference (it has no timers). */
+int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,162 @@
+"""ctypes front end of the CPU FP64 oracle (test infrastructure only).
+
+Only tests/, the smoke check and the CPU-baseline leg of bench.py import this
+module.  Arrays follow the reference's Fortran layout: a cell array is a
+numpy array of shape (nj, ni) in C order (so that x is the fastest index, as
+in ``PCell%vof(1:Isize,1:Jsize)``), a ghosted array has shape (nj+2, ni+2).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libsg_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE],
+                       check=True, stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_is_wet.argtypes = [ctypes.c_double] * 3
+        L.orc_is_wet.restype = ctypes.c_int
+        L.orc_poisson_coef.argtypes = [ctypes.c_int, ctypes.c_int] + [_D] * 9 + [ctypes.c_double] * 3 + [_D]
+        L.orc_poisson_matrix.argtypes = [ctypes.c_int, ctypes.c_int] + [_D] * 6 + [ctypes.c_int] * 4 + [_D]
+        L.orc_poisson_rhs.argtypes = [ctypes.c_int, ctypes.c_int] + [_D] * 11 + [ctypes.c_double, _D]
+        L.orc_matr_apply.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
+        L.orc_max_row_residual.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]
+        L.orc_max_row_residual.restype = ctypes.c_double
+        L.orc_pressure_scatter_bc.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D] + [ctypes.c_int] * 4 + [_D]
+        L.orc_velocity_correct.argtypes = ([ctypes.c_int, ctypes.c_int] + [_D] * 13 + [_I, _I] + [_D] * 7 +
+                                           [ctypes.c_double] * 4 + [_D] * 4)
+        L.orc_pcg_mg_solve.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
+        L.orc_pcg_mg_solve.restype = ctypes.c_int
+        L.orc_pcg_jacobi_solve.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, ctypes.c_double,
+                                           ctypes.c_int, _D]
+        L.orc_pcg_jacobi_solve.restype = ctypes.c_int
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_I)
+
+
+def poisson_coef(g, f, DpU, DpV, fl):
+    """ComputePossionMatrixCoefficient -> PoCoef of shape (4, nj, ni) (S,W,E,N)."""
+    ni, nj = g.ni, g.nj
+    out = np.zeros((4, nj, ni))
+    lib().orc_poisson_coef(ni, nj, _d(g.dx), _d(g.dy), _d(g.udx), _d(g.vdy),
+                           _d(f.vof), _d(f.phi), _d(f.vofS), _d(DpU), _d(DpV),
+                           fl.rho_l, fl.rho_g, fl.rho_ref, _d(out))
+    return out
+
+
+def poisson_matrix(g, geo, PoCoef, bc=(1, 1, 1, 0)):
+    """SetPoissonMatrix -> matr of shape (5, nj, ni) (S,W,diag,E,N)."""
+    ni, nj = g.ni, g.nj
+    out = np.zeros((5, nj, ni))
+    lib().orc_poisson_matrix(ni, nj, _d(PoCoef), _d(geo.aE), _d(geo.aW), _d(geo.aN), _d(geo.aS),
+                             _d(geo.vofS), bc[0], bc[1], bc[2], bc[3], _d(out))
+    return out
+
+
+def poisson_rhs(g, geo, u, v, vb=0.0):
+    ni, nj = g.ni, g.nj
+    out = np.zeros((nj, ni))
+    lib().orc_poisson_rhs(ni, nj, _d(g.dx), _d(g.dy), _d(u), _d(v), _d(geo.aE), _d(geo.aW),
+                          _d(geo.aN), _d(geo.aS), _d(geo.nyS), _d(geo.WlLh), _d(geo.vofS), vb, _d(out))
+    return out
+
+
+def matr_apply(g, matr, x):
+    y = np.zeros((g.nj, g.ni))
+    lib().orc_matr_apply(g.ni, g.nj, _d(matr), _d(np.ascontiguousarray(x)), _d(y))
+    return y
+
+
+def max_row_residual(g, geo, matr, x, b):
+    return lib().orc_max_row_residual(g.ni, g.nj, _d(matr), _d(np.ascontiguousarray(x)),
+                                      _d(np.ascontiguousarray(b)), _d(geo.vofS))
+
+
+def pressure_scatter_bc(g, geo, x, bc=(1, 1, 1, 0)):
+    Pp = np.zeros((g.nj + 2, g.ni + 2))
+    lib().orc_pressure_scatter_bc(g.ni, g.nj, _d(np.ascontiguousarray(x)), _d(geo.vofS),
+                                  bc[0], bc[1], bc[2], bc[3], _d(Pp))
+    return Pp
+
+
+def velocity_correct(g, geo, f, fl, DpU, DpV, upred, vpred, Pp, u, v, p, vb=0.0):
+    """UpdatePUV's correction; returns (u, v, p, mres) (copies of the inputs updated)."""
+    u = u.copy(); v = v.copy(); p = p.copy()
+    mres = np.zeros((g.nj, g.ni))
+    lib().orc_velocity_correct(g.ni, g.nj, _d(g.dx), _d(g.dy), _d(g.udx), _d(g.vdy),
+                               _d(f.vof), _d(f.phi), _d(geo.vofS), _d(geo.nyS), _d(geo.WlLh),
+                               _d(geo.aE), _d(geo.aW), _d(geo.aN), _d(geo.aS),
+                               _i(geo.uflag), _i(geo.vflag), _d(geo.vofSU), _d(geo.vofSV),
+                               _d(DpU), _d(DpV), _d(upred), _d(vpred), _d(Pp), vb,
+                               fl.rho_l, fl.rho_g, fl.rho_ref, _d(u), _d(v), _d(p), _d(mres))
+    return u, v, p, mres
+
+
+def pcg_mg_solve(g, matr, b, tol=1e-10, max_iter=500, pre=1, post=1, x0=None):
+    x = np.zeros((g.nj, g.ni)) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+    rr = ctypes.c_double(0.0)
+    it = lib().orc_pcg_mg_solve(g.ni, g.nj, _d(matr), _d(np.ascontiguousarray(b)), _d(x), tol,
+                                max_iter, pre, post, ctypes.byref(rr))
+    return x, it, rr.value
+
+
+def pcg_jacobi_solve(g, matr, b, tol=1e-10, max_iter=100000):
+    x = np.zeros((g.nj, g.ni))
+    rr = ctypes.c_double(0.0)
+    it = lib().orc_pcg_jacobi_solve(g.ni, g.nj, _d(matr), _d(np.ascontiguousarray(b)), _d(x), tol,
+                                    max_iter, ctypes.byref(rr))
+    return x, it, rr.value
+
+
+def matr_to_scipy(g, geo, matr):
+    """Assemble the oracle's row-wise matrix over fluid rows as scipy CSR
+    (index = i + j*ni, x fastest); solid rows become identity rows."""
+    import scipy.sparse as sp
+    ni, nj = g.ni, g.nj
+    N = ni * nj
+    idx = np.arange(N).reshape(nj, ni)
+    fluid = (geo.vofS < 1.0 - 1e-3)
+    rows, cols, vals = [], [], []
+    offs = {0: (-1, 0), 1: (0, -1), 3: (0, 1), 4: (1, 0)}  # S, W, E, N as (dj, di)
+    for k, (dj, di) in offs.items():
+        m = matr[k]
+        jj, ii = np.nonzero(m)
+        rows.append(idx[jj, ii]); cols.append(idx[jj + dj, ii + di]); vals.append(m[jj, ii])
+    jj, ii = np.nonzero(fluid)
+    rows.append(idx[jj, ii]); cols.append(idx[jj, ii]); vals.append(matr[2][jj, ii])
+    jj, ii = np.nonzero(~fluid)
+    rows.append(idx[jj, ii]); cols.append(idx[jj, ii]); vals.append(np.ones(len(jj)))
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+    return A

@@ -1,0 +1,111 @@
+/*
+ * CPU FP64 oracle for the simcoflow projection step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product path may link or call
+ * this library: it is the checker used by tests/, the smoke check and the
+ * CPU-baseline leg of bench.py.
+ *
+ * Layout (identical to the reference's Fortran arrays, column-major, x fastest):
+ *   cell arrays     (1:ni,1:nj)        a[(i-1) + (j-1)*ni]
+ *   ghosted arrays  (0:ni+1,0:nj+1)    a[i + j*(ni+2)]
+ *   PoCoef          (1:ni,1:nj,1:4)    faces S,W,E,N   (ProjectionP_Mod.f90:45)
+ *   matr            (1:ni,1:nj,1:5)    S,W,diag,E,N    (ProjectionP_Mod.f90:183-252)
+ *
+ * Each function names the reference lines it restates.
+ */
+#ifndef SG_ORACLE_H
+#define SG_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Wet/dry classification used by ProjectionP_Mod.f90:409-410,493-494 and
+ * ComputePUV_Mod.f90:63-64,80-81.  Returns 1 wet, 0 dry. */
+int orc_is_wet(double vof, double phi, double vofS);
+
+/* ComputePossionMatrixCoefficient, ProjectionP_Mod.f90:395-580. */
+void orc_poisson_coef(int ni, int nj, const double* dx, const double* dy,
+                      const double* udx, const double* vdy,
+                      const double* vof, const double* phi, const double* vofS,
+                      const double* DpU, const double* DpV,
+                      double rho_l, double rho_g, double rho_ref,
+                      double* PoCoef);
+
+/* SetPoissonMatrix, ProjectionP_Mod.f90:137-273 (row values only; bc flags
+ * 1 = Neumann, 0 = Dirichlet).  Solid rows (vofS >= 1-epsi) are left zero. */
+void orc_poisson_matrix(int ni, int nj, const double* PoCoef,
+                        const double* aE, const double* aW,
+                        const double* aN, const double* aS,
+                        const double* vofS, int bcW, int bcE, int bcS, int bcN,
+                        double* matr);
+
+/* SetPoissonVectors, ProjectionP_Mod.f90:275-354.  rhs on fluid rows, 0 on
+ * solid rows. */
+void orc_poisson_rhs(int ni, int nj, const double* dx, const double* dy,
+                     const double* u, const double* v,
+                     const double* aE, const double* aW,
+                     const double* aN, const double* aS,
+                     const double* nyS, const double* WlLh, const double* vofS,
+                     double vb, double* rhs);
+
+/* Row-wise product y = A x exactly as the residual check of
+ * ProjectionP_Mod.f90:70-85 forms it (x is a cell array). */
+void orc_matr_apply(int ni, int nj, const double* matr, const double* x,
+                    double* y);
+
+/* max over fluid rows of |(A x)_i - b_i|, ProjectionP_Mod.f90:70-93. */
+double orc_max_row_residual(int ni, int nj, const double* matr,
+                            const double* x, const double* b,
+                            const double* vofS);
+
+/* DeltaPressureGetValues + DeltaPressureBoundaryCondition,
+ * ProjectionP_Mod.f90:356-393,582-595: scatter the cell solution into the
+ * ghosted Pp (solid cells -> 0) and fill ghosts. */
+void orc_pressure_scatter_bc(int ni, int nj, const double* x,
+                             const double* vofS, int bcW, int bcE, int bcS,
+                             int bcN, double* Pp);
+
+/* Velocity correction + top-row continuity + mass residual + internal-cell
+ * condition, ComputePUV_Mod.f90:59-217,227-245.  u,v,p are the ghosted
+ * TVar%u/v/p (in/out), upred/vpred the ghosted predictor velocities,
+ * Pp the ghosted projection pressure.  uflag/vflag: 1 where
+ * UCell%Posnu/=-1 and UCell%MoExCell/=1 (resp. V), else 0. */
+void orc_velocity_correct(int ni, int nj, const double* dx, const double* dy,
+                          const double* udx, const double* vdy,
+                          const double* vof, const double* phi,
+                          const double* vofS, const double* nyS,
+                          const double* WlLh,
+                          const double* aE, const double* aW,
+                          const double* aN, const double* aS,
+                          const int* uflag, const int* vflag,
+                          const double* vofSU, const double* vofSV,
+                          const double* DpU, const double* DpV,
+                          const double* upred, const double* vpred,
+                          const double* Pp, double vb,
+                          double rho_l, double rho_g, double rho_ref,
+                          double* u, double* v, double* p, double* mres);
+
+/* ---- CPU restatement of the solver (the CPU baseline) -----------------
+ * PCG preconditioned by the same structured geometric multigrid V-cycle the
+ * GPU path runs (2x2 aggregation, Galerkin coarse operators, red-black
+ * Gauss-Seidel, direct coarsest solve).  The operator is the symmetric form
+ * of matr: cE(i,j) = -matr(i,j,E), cN(i,j) = -matr(i,j,N), diag = matr(.,3)
+ * (0 on solid rows).  Stops when ||b-Ax||_2 <= tol*||b||_2.
+ * Returns the iteration count (negative if max_iter was reached). */
+int orc_pcg_mg_solve(int ni, int nj, const double* matr, const double* b,
+                     double* x, double tol, int max_iter, int pre, int post,
+                     double* relres_out);
+
+/* Plain Jacobi-PCG on the same operator (reference iteration for tests). */
+int orc_pcg_jacobi_solve(int ni, int nj, const double* matr, const double* b,
+                         double* x, double tol, int max_iter,
+                         double* relres_out);
+
+/* Number of OpenMP threads the solver uses (1 when built without OpenMP). */
+int orc_num_threads(void);
+void orc_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+"""ctypes binding of libsimcogpu.so (the C ABI declared in include/simcogpu.h).
+
+The library is loaded from the package's in-tree build directory.  There is
+no fallback: if it is missing the import fails with the build command.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libsimcogpu.so")
+
+SG_OK = 0
+SG_NOT_CONVERGED = 1
+SG_NAN = 2
+SG_GUARD = 3
+SG_PREC_FP64 = 0
+SG_PREC_MIXED = 1
+
+# enum sg_field
+FIELDS = ["VOF", "PHI", "VOFS", "NYS", "WLLH", "NXS", "PHIS", "NX", "NY", "MRES", "VOFSU", "VOFSV",
+          "U", "V", "P", "UPRED", "VPRED", "PP", "DPU", "DPV", "AE", "AW", "AN", "AS",
+          "UFLAG", "VFLAG"]
+F = {name: k for k, name in enumerate(FIELDS)}
+CELL_FIELDS = set(FIELDS[:12])
+GHOST_FIELDS = set(FIELDS[12:24])
+INT_FIELDS = {"UFLAG", "VFLAG"}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("rho_l", ctypes.c_double), ("rho_g", ctypes.c_double), ("rho_ref", ctypes.c_double),
+        ("nu_l", ctypes.c_double), ("nu_g", ctypes.c_double), ("nu_ref", ctypes.c_double),
+        ("Re", ctypes.c_double), ("Fr", ctypes.c_double),
+        ("bc_w", ctypes.c_int), ("bc_e", ctypes.c_int), ("bc_s", ctypes.c_int), ("bc_n", ctypes.c_int),
+        ("precision", ctypes.c_int), ("max_iter", ctypes.c_int), ("tol", ctypes.c_double),
+        ("mg_pre", ctypes.c_int), ("mg_post", ctypes.c_int), ("warm_start", ctypes.c_int),
+        ("check_every", ctypes.c_int),
+    ]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int), ("status", ctypes.c_int), ("relres", ctypes.c_double),
+        ("max_row_residual", ctypes.c_double), ("ms", ctypes.c_double), ("setup_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# exported symbol -> (argtypes, restype); kept in step with include/simcogpu.h
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_V = ctypes.c_void_p
+_CTX = ctypes.c_void_p
+PROTOTYPES = {
+    "sg_last_error": ([], ctypes.c_char_p),
+    "sg_params_default": ([ctypes.POINTER(Params)], ctypes.c_int),
+    "sg_ctx_create": ([ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.POINTER(Params),
+                       ctypes.POINTER(_CTX)], ctypes.c_int),
+    "sg_ctx_destroy": ([_CTX], ctypes.c_int),
+    "sg_set_params": ([_CTX, ctypes.POINTER(Params)], ctypes.c_int),
+    "sg_field_ptr": ([_CTX, ctypes.c_int], ctypes.c_void_p),
+    "sg_upload": ([_CTX, ctypes.c_int, _V], ctypes.c_int),
+    "sg_download": ([_CTX, ctypes.c_int, _V], ctypes.c_int),
+    "sg_synchronize": ([_CTX], ctypes.c_int),
+    "sg_set_geometry": ([_CTX, _D, _D, _D, _D, _D, _D, _D, _I, _I, _D, _D], ctypes.c_int),
+    "sg_poisson_build_resident": ([_CTX, ctypes.c_double], ctypes.c_int),
+    "sg_poisson_solve_resident": ([_CTX, ctypes.POINTER(SolveInfo)], ctypes.c_int),
+    "sg_poisson_solve": ([_CTX, _D, _D, _D, _D, _D, _D, ctypes.c_double, _D, ctypes.POINTER(SolveInfo)],
+                         ctypes.c_int),
+    "sg_poisson_matrix": ([_CTX, _D], ctypes.c_int),
+    "sg_poisson_rhs": ([_CTX, _D], ctypes.c_int),
+    "sg_poisson_apply": ([_CTX, _D, _D], ctypes.c_int),
+    "sg_velocity_correct_resident": ([_CTX, ctypes.c_double], ctypes.c_int),
+}
+
+_lib = None
+
+
+class SimcoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"simcogpu error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libsimcogpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m simcoflow_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (argt, rest) in PROTOTYPES.items():
+        fn = getattr(L, name)
+        fn.argtypes = argt
+        fn.restype = rest
+    _lib = L
+    return L
+
+
+def last_error():
+    return load().sg_last_error().decode(errors="replace")
+
+
+def check(rc, allow=()):
+    if rc != SG_OK and rc not in allow:
+        raise SimcoError(rc, last_error())
+    return rc

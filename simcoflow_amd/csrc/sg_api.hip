@@ -1,0 +1,396 @@
+// C ABI of the simcogpu library: context, device fields, projection entry
+// points.  Every device buffer is allocated in sg_ctx_create; the solve and
+// correction entry points allocate nothing.
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "sg_internal.hpp"
+
+namespace sg {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+    g_err = std::string("HIP error '") + hipGetErrorString(e) + "' in " + what;
+    return SG_ERR_HIP;
+}
+
+void* dev_alloc(Ctx* c, size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, bytes ? bytes : 16) != hipSuccess) {
+        (void)hipFree(p);
+        return nullptr;
+    }
+    c->allocations.push_back(p);
+    return p;
+}
+
+static bool is_cell_field(int f) { return f >= SG_F_VOF && f <= SG_F_VOFSV; }
+static bool is_ghost_field(int f) { return f >= SG_F_U && f <= SG_F_AS; }
+static bool is_int_field(int f) { return f == SG_F_UFLAG || f == SG_F_VFLAG; }
+
+static void destroy(Ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocations) (void)hipFree(p);
+    if (c->sc_host) (void)hipHostFree(c->sc_host);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int check_params(const sg_params* p) {
+    if (!p) return SG_ERR_ARG;
+    if (!(p->rho_l > 0.0) || !(p->rho_g > 0.0) || !(p->rho_ref > 0.0)) {
+        set_error("densities must be positive");
+        return SG_ERR_ARG;
+    }
+    if (p->precision != SG_PREC_FP64 && p->precision != SG_PREC_MIXED) {
+        set_error("precision must be SG_PREC_FP64 or SG_PREC_MIXED");
+        return SG_ERR_ARG;
+    }
+    if (!(p->tol > 0.0)) {
+        set_error("tol must be positive");
+        return SG_ERR_ARG;
+    }
+    return SG_OK;
+}
+
+}  // namespace sg
+
+using namespace sg;
+
+extern "C" {
+
+const char* sg_last_error(void) { return g_err.c_str(); }
+
+int sg_params_default(sg_params* p) {
+    if (!p) return SG_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->rho_l = 998.2;  // StateVariables.f90:27
+    p->rho_g = 1.225;
+    p->rho_ref = 998.2;  // Main.f90:148
+    p->nu_l = 1.0034e-6;
+    p->nu_g = 1.506e-5;
+    p->nu_ref = 1.0034e-6;
+    p->Re = 1.0;
+    p->Fr = 1.0;
+    p->bc_w = 1;
+    p->bc_e = 1;
+    p->bc_s = 1;
+    p->bc_n = 0;
+    p->precision = SG_PREC_FP64;
+    p->max_iter = 500;
+    p->tol = 1e-10;
+    p->mg_pre = 1;
+    p->mg_post = 1;
+    p->warm_start = 0;
+    p->check_every = 1;
+    return SG_OK;
+}
+
+int sg_ctx_create(int ni, int nj, const double* dx, const double* dy, const double* udx, const double* vdy,
+                  const sg_params* prm, sg_ctx** out) {
+    if (!out || ni < 2 || nj < 2 || !dx || !dy || !udx || !vdy) {
+        set_error("sg_ctx_create: need ni, nj >= 2 and all spacing arrays");
+        return SG_ERR_ARG;
+    }
+    *out = nullptr;
+    int rc = check_params(prm);
+    if (rc) return rc;
+    if ((long long)ni * (long long)nj > (1ll << 31)) {
+        set_error("sg_ctx_create: grid too large");
+        return SG_ERR_ARG;
+    }
+    Ctx* c = new Ctx();
+    c->ni = ni;
+    c->nj = nj;
+    c->prm = *prm;
+#define CK(call)                    \
+    do {                            \
+        hipError_t _e = (call);     \
+        if (_e != hipSuccess) {     \
+            hip_fail(_e, #call);    \
+            destroy(c);             \
+            return SG_ERR_HIP;      \
+        }                           \
+    } while (0)
+    CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CK(hipEventCreate(&c->ev0));
+    CK(hipEventCreate(&c->ev1));
+    CK(hipEventCreate(&c->ev2));
+    auto fail_alloc = [&]() {
+        set_error("sg_ctx_create: device allocation failed");
+        destroy(c);
+        return SG_ERR_ALLOC;
+    };
+    c->dx = (double*)dev_alloc(c, ni * sizeof(double));
+    c->dy = (double*)dev_alloc(c, nj * sizeof(double));
+    c->udx = (double*)dev_alloc(c, ni * sizeof(double));
+    c->vdy = (double*)dev_alloc(c, nj * sizeof(double));
+    if (!c->dx || !c->dy || !c->udx || !c->vdy) return fail_alloc();
+    CK(hipMemcpy(c->dx, dx, ni * sizeof(double), hipMemcpyHostToDevice));
+    CK(hipMemcpy(c->dy, dy, nj * sizeof(double), hipMemcpyHostToDevice));
+    CK(hipMemcpy(c->udx, udx, ni * sizeof(double), hipMemcpyHostToDevice));
+    CK(hipMemcpy(c->vdy, vdy, nj * sizeof(double), hipMemcpyHostToDevice));
+    const size_t ncell = (size_t)ni * nj, nghost = (size_t)(ni + 2) * (nj + 2);
+    for (int f = 0; f < SG_F_COUNT; ++f) {
+        size_t bytes = 0;
+        if (is_cell_field(f)) bytes = ncell * sizeof(double);
+        else if (is_ghost_field(f)) bytes = nghost * sizeof(double);
+        else if (is_int_field(f)) bytes = ncell * sizeof(int);
+        c->field_bytes[f] = bytes;
+        c->field[f] = dev_alloc(c, bytes);
+        if (!c->field[f]) return fail_alloc();
+    }
+    if ((rc = mg_allocate(c)) != SG_OK) return fail_alloc();
+    const size_t n0 = c->lev[0].d.alloc_elems();
+    double** vecs[] = {&c->x, &c->r, &c->z, &c->p0, &c->p1, &c->q, &c->rhs};
+    for (double** v : vecs) {
+        *v = (double*)dev_alloc(c, n0 * sizeof(double));
+        if (!*v) return fail_alloc();
+    }
+    // cell-layout staging for the host-pointer test hooks (5 planes)
+    c->stage = (double*)dev_alloc(c, 5 * ncell * sizeof(double));
+    if (!c->stage) return fail_alloc();
+    const dim3 g = grid2d(ni, nj);
+    c->red.max_blocks = (int)(g.x * g.y);
+    c->red.partials = (double*)dev_alloc(c, sizeof(double) * 8 * NRED * (size_t)c->red.max_blocks);
+    c->red.tickets = (unsigned int*)dev_alloc(c, sizeof(unsigned int) * 8);
+    c->sc = (PcgScalars*)dev_alloc(c, sizeof(PcgScalars));
+    if (!c->red.partials || !c->red.tickets || !c->sc) return fail_alloc();
+    CK(hipHostMalloc((void**)&c->sc_host, sizeof(PcgScalars), hipHostMallocDefault));
+    std::memset(c->sc_host, 0, sizeof(PcgScalars));
+    // a geometry with no solid: open apertures, fluid everywhere
+    {
+        std::vector<double> ones(nghost, 1.0);
+        for (int f : {SG_F_AE, SG_F_AW, SG_F_AN, SG_F_AS}) CK(hipMemcpy(c->field[f], ones.data(), nghost * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<int> flags(ncell, 1);
+        for (int f : {SG_F_UFLAG, SG_F_VFLAG}) CK(hipMemcpy(c->field[f], flags.data(), ncell * sizeof(int), hipMemcpyHostToDevice));
+    }
+    CK(hipDeviceSynchronize());
+#undef CK
+    *out = reinterpret_cast<sg_ctx*>(c);
+    return SG_OK;
+}
+
+int sg_ctx_destroy(sg_ctx* ctx) {
+    destroy(reinterpret_cast<Ctx*>(ctx));
+    return SG_OK;
+}
+
+int sg_set_params(sg_ctx* ctx, const sg_params* p) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    int rc = check_params(p);
+    if (rc) return rc;
+    const bool prec_changed = p->precision != c->prm.precision;
+    const bool coef_changed = p->rho_l != c->prm.rho_l || p->rho_g != c->prm.rho_g || p->rho_ref != c->prm.rho_ref ||
+                              p->bc_w != c->prm.bc_w || p->bc_e != c->prm.bc_e || p->bc_s != c->prm.bc_s ||
+                              p->bc_n != c->prm.bc_n;
+    c->prm = *p;
+    if (prec_changed || coef_changed) {
+        c->built = 0;
+        c->mg_ready = 0;
+    }
+    return SG_OK;
+}
+
+void* sg_field_ptr(sg_ctx* ctx, int field) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || field < 0 || field >= SG_F_COUNT) return nullptr;
+    return c->field[field];
+}
+
+int sg_upload(sg_ctx* ctx, int field, const void* host) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !host || field < 0 || field >= SG_F_COUNT) {
+        set_error("sg_upload: bad argument");
+        return SG_ERR_ARG;
+    }
+    SG_HIP(hipMemcpyAsync(c->field[field], host, c->field_bytes[field], hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    c->built = 0;
+    return SG_OK;
+}
+
+int sg_download(sg_ctx* ctx, int field, void* host) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !host || field < 0 || field >= SG_F_COUNT) {
+        set_error("sg_download: bad argument");
+        return SG_ERR_ARG;
+    }
+    SG_HIP(hipMemcpyAsync(host, c->field[field], c->field_bytes[field], hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_synchronize(sg_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_set_geometry(sg_ctx* ctx, const double* vofS, const double* nyS, const double* WlLh, const double* aE,
+                    const double* aW, const double* aN, const double* aS, const int* uflag, const int* vflag,
+                    const double* vofSU, const double* vofSV) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !vofS || !nyS || !WlLh || !aE || !aW || !aN || !aS || !uflag || !vflag || !vofSU || !vofSV) {
+        set_error("sg_set_geometry: null argument");
+        return SG_ERR_ARG;
+    }
+    const void* src[] = {vofS, nyS, WlLh, aE, aW, aN, aS, uflag, vflag, vofSU, vofSV};
+    const int dst[] = {SG_F_VOFS, SG_F_NYS, SG_F_WLLH, SG_F_AE, SG_F_AW, SG_F_AN, SG_F_AS,
+                       SG_F_UFLAG, SG_F_VFLAG, SG_F_VOFSU, SG_F_VOFSV};
+    for (int k = 0; k < 11; ++k)
+        SG_HIP(hipMemcpyAsync(c->field[dst[k]], src[k], c->field_bytes[dst[k]], hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    c->built = 0;
+    c->mg_ready = 0;
+    return SG_OK;
+}
+
+int sg_poisson_build_resident(sg_ctx* ctx, double vb) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    c->vb = vb;
+    return poisson_build(c);
+}
+
+int sg_poisson_solve_resident(sg_ctx* ctx, sg_solve_info* info) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    if (!c->built) {
+        set_error("sg_poisson_solve_resident: call sg_poisson_build_resident first");
+        return SG_ERR_STATE;
+    }
+    int guard = 0;
+    SG_HIP(hipMemcpyAsync(&guard, &c->sc->guard, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    int rc = pcg_solve(c, info);
+    if (guard && rc == SG_OK) {
+        rc = (guard & 1) ? SG_NAN : SG_GUARD;
+        if (info) info->status = rc;
+        set_error("projection: NaN or out-of-range matrix/RHS value (ProjectionP_Mod.f90:209-213,322-325)");
+    }
+    // clear the guard bits for the next build
+    const int zero = 0;
+    SG_HIP(hipMemcpyAsync(&c->sc->guard, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    if (rc == SG_NOT_CONVERGED) set_error("projection: PCG did not reach the tolerance within max_iter");
+    return rc;
+}
+
+int sg_poisson_solve(sg_ctx* ctx, const double* u_star, const double* v_star, const double* DpU, const double* DpV,
+                     const double* vof, const double* phi, double vb, double* Pp_out, sg_solve_info* info) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !u_star || !v_star || !DpU || !DpV || !vof || !phi || !Pp_out) {
+        set_error("sg_poisson_solve: null argument");
+        return SG_ERR_ARG;
+    }
+    const void* src[] = {u_star, v_star, DpU, DpV, vof, phi};
+    const int dst[] = {SG_F_UPRED, SG_F_VPRED, SG_F_DPU, SG_F_DPV, SG_F_VOF, SG_F_PHI};
+    for (int k = 0; k < 6; ++k)
+        SG_HIP(hipMemcpyAsync(c->field[dst[k]], src[k], c->field_bytes[dst[k]], hipMemcpyHostToDevice, c->stream));
+    int rc = sg_poisson_build_resident(ctx, vb);
+    if (rc) return rc;
+    rc = sg_poisson_solve_resident(ctx, info);
+    if (rc < 0) return rc;
+    SG_HIP(hipMemcpyAsync(Pp_out, c->field[SG_F_PP], c->field_bytes[SG_F_PP], hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+int sg_poisson_matrix(sg_ctx* ctx, double* matr_out) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !matr_out) return SG_ERR_ARG;
+    const size_t ncell = (size_t)c->ni * c->nj;
+    int rc = poisson_matrix_rows(c, c->stage);
+    if (rc) return rc;
+    SG_HIP(hipMemcpyAsync(matr_out, c->stage, 5 * ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_poisson_rhs(sg_ctx* ctx, double* rhs_out) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !rhs_out) return SG_ERR_ARG;
+    if (!c->built) {
+        set_error("sg_poisson_rhs: call sg_poisson_build_resident first");
+        return SG_ERR_STATE;
+    }
+    int rc = padded_to_cell(c, c->rhs, c->stage);
+    if (rc) return rc;
+    SG_HIP(hipMemcpyAsync(rhs_out, c->stage, (size_t)c->ni * c->nj * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_poisson_apply(sg_ctx* ctx, const double* x, double* y) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !x || !y) return SG_ERR_ARG;
+    if (!c->built) {
+        set_error("sg_poisson_apply: call sg_poisson_build_resident first");
+        return SG_ERR_STATE;
+    }
+    const size_t ncell = (size_t)c->ni * c->nj;
+    // padded staging vectors: level-0 x and r (interior writes only)
+    double* xp = c->lev[0].v.x;
+    double* yp = c->lev[0].v.r;
+    SG_HIP(hipMemcpyAsync(c->stage, x, ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = cell_to_padded(c, c->stage, xp);
+    if (rc) return rc;
+    rc = poisson_apply_fine(c, xp, yp);
+    if (rc) return rc;
+    rc = padded_to_cell(c, yp, c->stage);
+    if (rc) return rc;
+    SG_HIP(hipMemcpyAsync(y, c->stage, ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_velocity_correct_resident(sg_ctx* ctx, double vb) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    c->vb = vb;
+    int rc = velocity_correct(c);
+    if (rc) return rc;
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !avg_ms || reps <= 0) return SG_ERR_ARG;
+    if (!c->built) {
+        set_error("sg_time_kernel: build the operator first");
+        return SG_ERR_STATE;
+    }
+    if (which != 0) {
+        set_error("sg_time_kernel: unknown kernel id");
+        return SG_ERR_ARG;
+    }
+    // fine operator apply y = A x on the PCG vectors x -> level-0 residual
+    int rc = poisson_apply_fine(c, c->x, c->lev[0].v.r);
+    if (rc) return rc;
+    SG_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int k = 0; k < reps; ++k) {
+        rc = poisson_apply_fine(c, c->x, c->lev[0].v.r);
+        if (rc) return rc;
+    }
+    SG_HIP(hipEventRecord(c->ev1, c->stream));
+    SG_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    SG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = (double)ms / reps;
+    return SG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+// Internal declarations of the simcogpu library (gfx950).
+//
+// Solver arrays use a padded layout with a one-cell zero ring: element (i,j)
+// with i in [-1,ni], j in [-1,nj] lives at (j+2)*ld + i.  Rows start on a
+// 256-byte boundary (ld is a multiple of 32 elements), the left ring cell of
+// row j is the right padding of the storage row before it, and two extra
+// storage rows hold the bottom ring.  Every kernel reads neighbours without
+// bounds tests; the ring and padding stay zero because no kernel writes
+// outside [0,ni)x[0,nj).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/simcogpu.h"
+
+namespace sg {
+
+constexpr double EPSI = 1.0e-3;  // StateVariables.f90:28
+constexpr double TOLB = 1.0e-24; // ProjectionP_Mod.f90:404
+constexpr int COARSEST_MAX = 64; // unknowns of the dense coarsest solve
+constexpr int MAX_LEVELS = 24;
+constexpr int NRED = 4;          // values a reduction slot can carry
+
+struct Dims {
+    int ni = 0, nj = 0, ld = 0;
+    size_t alloc_elems() const { return (size_t)(nj + 3) * (size_t)ld; }
+    __host__ __device__ size_t off(int i, int j) const {
+        return (size_t)(j + 2) * (size_t)ld + (size_t)(i);
+    }
+    size_t interior_cells() const { return (size_t)ni * (size_t)nj; }
+};
+
+inline Dims make_dims(int ni, int nj) {
+    Dims d;
+    d.ni = ni;
+    d.nj = nj;
+    d.ld = ((ni + 1 + 31) / 32) * 32;
+    return d;
+}
+
+// Operator of one level: symmetric 9-point stencil stored as positive
+// couplings (matrix entry = -c).  Level 0 is 5-point: cNE/cNW are null.
+template <typename T>
+struct Op {
+    T* diag = nullptr;
+    T* cE = nullptr;  // (i,j)-(i+1,j)
+    T* cN = nullptr;  // (i,j)-(i,j+1)
+    T* cNE = nullptr; // (i,j)-(i+1,j+1)
+    T* cNW = nullptr; // (i,j)-(i-1,j+1)
+};
+
+// Prolongation from level l+1 to level l, stored at the coarse resolution
+// (coarse dims): for coarse point (I,J)
+//   inj : fine (2I,2J)     <- (I,J)
+//   xW,xE: fine (2I+1,2J)  <- (I,J),(I+1,J)
+//   yS,yN: fine (2I,2J+1)  <- (I,J),(I,J+1)
+//   SW,SE,NW,NE: fine (2I+1,2J+1) <- (I,J),(I+1,J),(I,J+1),(I+1,J+1)
+template <typename T>
+struct Prol {
+    T* w[9] = {nullptr};
+};
+enum { W_INJ = 0, W_XW, W_XE, W_YS, W_YN, W_SW, W_SE, W_NW, W_NE };
+
+template <typename T>
+struct LevelVecs {
+    T* x = nullptr;
+    T* b = nullptr;
+    T* r = nullptr;
+};
+
+struct Level {
+    Dims d;
+    Op<double> A;       // FP64 operator (setup is always FP64)
+    Op<float> Af;       // FP32 copy (mixed precision)
+    Prol<double> P;     // prolongation to this level from the next (if any)
+    Prol<float> Pf;
+    LevelVecs<double> v;
+    LevelVecs<float> vf;
+};
+
+// Device-side scalars of the PCG iteration.
+struct PcgScalars {
+    double bb;      // ||b||^2 over fluid rows
+    double rz;      // (r,z) of the current iteration
+    double rz_old;  // (r,z) of the previous iteration
+    double zq;      // (z_new, q_old) for the flexible beta
+    double pq;      // (p,q)
+    double rr;      // (r,r)
+    double tol2;    // tol^2 * bb
+    int active;     // 1 while not converged
+    int iters;      // completed iterations
+    int first;      // 1 before the first direction update
+    int flexible;   // Polak-Ribiere beta (mixed precision)
+    int guard;      // bit 0: NaN, bit 1: diag/rhs guard tripped
+};
+
+struct Reducer {
+    double* partials = nullptr;  // [NRED][max_blocks]
+    unsigned int* tickets = nullptr;
+    int max_blocks = 0;
+};
+
+struct Ctx {
+    int ni = 0, nj = 0;
+    sg_params prm{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    // grid spacing
+    double *dx = nullptr, *dy = nullptr, *udx = nullptr, *vdy = nullptr;
+    // device fields in the reference layout
+    void* field[SG_F_COUNT] = {nullptr};
+    size_t field_bytes[SG_F_COUNT] = {0};
+    // solver
+    int nlev = 0;
+    Level lev[MAX_LEVELS];
+    double* coarse_inv = nullptr;  // dense inverse of the coarsest operator
+    float* coarse_inv_f = nullptr;
+    int coarse_n = 0;
+    // PCG vectors on the fine level (FP64)
+    double *x = nullptr, *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
+    double* rhs = nullptr;
+    double* stage = nullptr;  // cell-layout staging, 5*ni*nj
+    int pcur = 0;
+    PcgScalars* sc = nullptr;      // device
+    PcgScalars* sc_host = nullptr; // pinned host mirror
+    Reducer red;
+    int built = 0;      // operator + rhs built
+    int mg_ready = 0;   // hierarchy set up for the current operator
+    double vb = 0.0;
+    std::vector<void*> allocations;
+};
+
+// ---- launch helpers -------------------------------------------------------
+constexpr int TX = 64;  // one wavefront along x
+constexpr int TY = 4;
+inline dim3 grid2d(int ni, int nj) { return dim3((ni + TX - 1) / TX, (nj + TY - 1) / TY); }
+inline dim3 block2d() { return dim3(TX, TY); }
+
+// error handling
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+#define SG_HIP(call)                                   \
+    do {                                               \
+        hipError_t _e = (call);                        \
+        if (_e != hipSuccess) return hip_fail(_e, #call); \
+    } while (0)
+
+// ---- entry points of the kernel files (host wrappers) --------------------
+// projection set-up, apply, scatter, correction (sg_poisson.hip)
+int poisson_build(Ctx* c);
+int poisson_rhs_only(Ctx* c);
+int poisson_apply_fine(Ctx* c, const double* x, double* y);  // padded layout
+int poisson_scatter(Ctx* c);
+int poisson_matrix_rows(Ctx* c, double* d_matr);
+int velocity_correct(Ctx* c);
+int padded_to_cell(Ctx* c, const double* src, double* dst_cell);
+int cell_to_padded(Ctx* c, const double* src_cell, double* dst);
+// multigrid (sg_mg.hip)
+int mg_allocate(Ctx* c);
+int mg_setup(Ctx* c);
+int mg_vcycle(Ctx* c, const double* r, double* z);  // fine padded vectors
+// PCG (sg_pcg.hip)
+int pcg_solve(Ctx* c, sg_solve_info* info);
+int fine_residual_norms(Ctx* c, double* relres, double* maxrow);
+
+void* dev_alloc(Ctx* c, size_t bytes);
+
+}  // namespace sg

@@ -1,0 +1,169 @@
+"""Host side of the projection step, over the simcogpu C ABI.
+
+``Projection`` mirrors the reference's ``PoissonEquationSolver``
+(ProjectionP_Mod.f90:26-99) and the correction part of ``UpdatePUV``
+(ComputePUV_Mod.f90:17-225): same inputs (predictor velocities, PU%Dp /
+PV%Dp, the P-cell vof / phi, cut-cell apertures), same outputs (Proj%Pp with
+its ghost cells; corrected TVar%u / v / p and the mass residual).  Arrays are
+numpy arrays in the reference's Fortran layout: a cell array has shape
+(nj, ni) in C order, a ghosted array (nj+2, ni+2).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import F, Params, SolveInfo, check
+
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_DP)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_IP)
+
+
+def default_params(**kw):
+    p = Params()
+    check(_lib.load().sg_params_default(ctypes.byref(p)))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def params_for(fluids, tol=1e-10, precision=_lib.SG_PREC_FP64, bc=(1, 1, 1, 0), **kw):
+    """Parameters for a cases.Fluids description."""
+    return default_params(rho_l=fluids.rho_l, rho_g=fluids.rho_g, rho_ref=fluids.rho_ref,
+                          nu_l=fluids.mu_l / fluids.rho_l, nu_g=fluids.mu_g / fluids.rho_g,
+                          nu_ref=fluids.mu_l / fluids.rho_l, bc_w=bc[0], bc_e=bc[1], bc_s=bc[2], bc_n=bc[3],
+                          tol=tol, precision=precision, **kw)
+
+
+class Projection:
+    """A device context for one grid (all device memory allocated here)."""
+
+    def __init__(self, grid, params):
+        self.lib = _lib.load()
+        self.grid = grid
+        self.ni, self.nj = grid.ni, grid.nj
+        self.params = params
+        ctx = ctypes.c_void_p()
+        keep = [_d(grid.dx), _d(grid.dy), _d(grid.udx), _d(grid.vdy)]
+        check(self.lib.sg_ctx_create(self.ni, self.nj, keep[0][1], keep[1][1], keep[2][1], keep[3][1],
+                                     ctypes.byref(params), ctypes.byref(ctx)))
+        self.ctx = ctx
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if self.ctx:
+            self.lib.sg_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, params):
+        self.params = params
+        check(self.lib.sg_set_params(self.ctx, ctypes.byref(params)))
+
+    # -- fields ----------------------------------------------------------
+    def _shape(self, name):
+        if name in _lib.GHOST_FIELDS:
+            return (self.nj + 2, self.ni + 2)
+        return (self.nj, self.ni)
+
+    def upload(self, name, arr):
+        shape = self._shape(name)
+        if name in _lib.INT_FIELDS:
+            a = np.ascontiguousarray(arr, dtype=np.int32)
+        else:
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"{name}: expected shape {shape}, got {a.shape}")
+        check(self.lib.sg_upload(self.ctx, F[name], a.ctypes.data_as(ctypes.c_void_p)))
+
+    def download(self, name):
+        dt = np.int32 if name in _lib.INT_FIELDS else np.float64
+        a = np.empty(self._shape(name), dtype=dt)
+        check(self.lib.sg_download(self.ctx, F[name], a.ctypes.data_as(ctypes.c_void_p)))
+        return a
+
+    def field_ptr(self, name):
+        return self.lib.sg_field_ptr(self.ctx, F[name])
+
+    def set_geometry(self, geo):
+        args = [_d(geo.vofS), _d(geo.nyS), _d(geo.WlLh), _d(geo.aE), _d(geo.aW), _d(geo.aN), _d(geo.aS)]
+        ints = [_i(geo.uflag), _i(geo.vflag)]
+        tail = [_d(geo.vofSU), _d(geo.vofSV)]
+        check(self.lib.sg_set_geometry(self.ctx, *[a[1] for a in args], *[a[1] for a in ints],
+                                       *[a[1] for a in tail]))
+
+    def set_interface(self, vof, phi):
+        self.upload("VOF", vof)
+        self.upload("PHI", phi)
+
+    def set_predictor(self, upred, vpred, DpU, DpV):
+        self.upload("UPRED", upred)
+        self.upload("VPRED", vpred)
+        self.upload("DPU", DpU)
+        self.upload("DPV", DpV)
+
+    # -- projection --------------------------------------------------------
+    def build(self, vb=0.0):
+        check(self.lib.sg_poisson_build_resident(self.ctx, float(vb)))
+
+    def solve(self, allow_not_converged=False):
+        info = SolveInfo()
+        rc = self.lib.sg_poisson_solve_resident(self.ctx, ctypes.byref(info))
+        check(rc, allow=(_lib.SG_NOT_CONVERGED,) if allow_not_converged else ())
+        return info
+
+    def poisson_solve(self, u_star, v_star, DpU, DpV, vof, phi, vb=0.0):
+        """Drop-in PoissonEquationSolver: host arrays in, Proj%Pp out."""
+        ins = [_d(a) for a in (u_star, v_star, DpU, DpV, vof, phi)]
+        Pp = np.zeros((self.nj + 2, self.ni + 2))
+        info = SolveInfo()
+        rc = self.lib.sg_poisson_solve(self.ctx, *[a[1] for a in ins], float(vb), Pp.ctypes.data_as(_DP),
+                                       ctypes.byref(info))
+        check(rc)
+        return Pp, info
+
+    def matrix(self):
+        """Assembled rows in the reference's matr layout, shape (5, nj, ni)."""
+        m = np.zeros((5, self.nj, self.ni))
+        check(self.lib.sg_poisson_matrix(self.ctx, m.ctypes.data_as(_DP)))
+        return m
+
+    def rhs(self):
+        b = np.zeros((self.nj, self.ni))
+        check(self.lib.sg_poisson_rhs(self.ctx, b.ctypes.data_as(_DP)))
+        return b
+
+    def apply(self, x):
+        x, xp = _d(x)
+        y = np.zeros((self.nj, self.ni))
+        check(self.lib.sg_poisson_apply(self.ctx, xp, y.ctypes.data_as(_DP)))
+        return y
+
+    def velocity_correct(self, vb=0.0):
+        check(self.lib.sg_velocity_correct_resident(self.ctx, float(vb)))
+
+    def synchronize(self):
+        check(self.lib.sg_synchronize(self.ctx))
