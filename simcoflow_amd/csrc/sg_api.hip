@@ -2,6 +2,7 @@
 // points.  Every device buffer is allocated in sg_ctx_create; the solve and
 // correction entry points allocate nothing.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -36,6 +37,8 @@ static bool is_int_field(int f) { return f == SG_F_UFLAG || f == SG_F_VFLAG; }
 static void destroy(Ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipGraphExec_t g : c->iter_graph)
+        if (g) (void)hipGraphExecDestroy(g);
     for (void* p : c->allocations) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -91,7 +94,7 @@ int sg_params_default(sg_params* p) {
     p->mg_pre = 1;
     p->mg_post = 1;
     p->warm_start = 0;
-    p->check_every = 1;
+    p->check_every = 2;
     return SG_OK;
 }
 
@@ -109,6 +112,7 @@ int sg_ctx_create(int ni, int nj, const double* dx, const double* dy, const doub
         return SG_ERR_ARG;
     }
     Ctx* c = new Ctx();
+    if (const char* ev = std::getenv("SG_NO_GRAPH")) c->use_graph = (ev[0] == '0');
     c->ni = ni;
     c->nj = nj;
     c->prm = *prm;

@@ -21,7 +21,7 @@ namespace sg {
 
 constexpr double EPSI = 1.0e-3;  // StateVariables.f90:28
 constexpr double TOLB = 1.0e-24; // ProjectionP_Mod.f90:404
-constexpr int COARSEST_MAX = 64; // unknowns of the dense coarsest solve
+constexpr int COARSEST_MAX = 16; // unknowns of the dense coarsest solve
 constexpr int MAX_LEVELS = 24;
 constexpr int NRED = 4;          // values a reduction slot can carry
 
@@ -91,6 +91,10 @@ struct PcgScalars {
     double pq;      // (p,q)
     double rr;      // (r,r)
     double tol2;    // tol^2 * bb
+    double alpha;   // step length of the current iteration
+    double beta;    // direction update of the current iteration
+    double true_rr; // ||b - A x||^2 after the solve
+    double true_max;// max |b - A x| over fluid rows after the solve
     int active;     // 1 while not converged
     int iters;      // completed iterations
     int first;      // 1 before the first direction update
@@ -131,6 +135,11 @@ struct Ctx {
     int built = 0;      // operator + rhs built
     int mg_ready = 0;   // hierarchy set up for the current operator
     double vb = 0.0;
+    // captured pair of PCG iterations, per precision (kernel arguments are
+    // fixed for the life of the context, so one capture serves every solve)
+    hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
+    int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0};
+    int use_graph = 1;
     std::vector<void*> allocations;
 };
 

@@ -12,14 +12,20 @@ namespace sg {
 namespace {
 
 constexpr int NT = TX * TY;
+constexpr int NTF = 1024;  // threads of the finalize kernels
 
-__device__ inline double* slot(const Reducer& R, int s) { return R.partials + (size_t)s * NRED * R.max_blocks; }
+enum FinMode { FIN_INIT = 0, FIN_REPLACE, FIN_RZ, FIN_PQ, FIN_RR, FIN_TRUE };
 
-// r = b - A x (or r = b for a zero start), ||b||^2, ||r||^2
+// one reduction slot: NRED planes of per-block partials
+struct Slots {
+    double* part;
+    int max_blocks;
+};
+
+// r = b - A x (or r = b for a zero start); partials of ||b||^2, ||r||^2
 __global__ void __launch_bounds__(256) k_init(Dims D, const double* __restrict__ diag, const double* __restrict__ cE,
                                               const double* __restrict__ cN, const double* __restrict__ b,
-                                              double* __restrict__ x, double* __restrict__ r, int warm,
-                                              int replace, Reducer R, PcgScalars* sc, double tol) {
+                                              double* __restrict__ x, double* __restrict__ r, int warm, Slots S) {
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     double v[2] = {0.0, 0.0};
@@ -40,28 +46,12 @@ __global__ void __launch_bounds__(256) k_init(Dims D, const double* __restrict__
         v[0] = bp * bp;
         v[1] = rp * rp;
     }
-    double out[2];
-    if (grid_reduce<2, NT>(v, slot(R, 0), R.max_blocks, R.tickets + 0, out)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            // replace: the recursive residual met the tolerance; restart from
-            // the true residual b - A x if it did not
-            if (!replace) {
-                sc->bb = out[0];
-                sc->tol2 = tol * tol * out[0];
-                sc->iters = 0;
-            }
-            sc->rr = out[1];
-            sc->first = 1;
-            sc->active = (out[0] > 0.0 && out[1] > sc->tol2) ? 1 : 0;
-            sc->rz = sc->rz_old = sc->zq = sc->pq = 0.0;
-        }
-    }
+    block_partials<2, NT>(v, S.part, S.max_blocks);
 }
 
-// rz = (r,z), zq = (z,q_old); last block forms beta
+// partials of rz = (r,z), zq = (z,q_old)
 __global__ void __launch_bounds__(256) k_dots_rz(Dims D, const double* __restrict__ r, const double* __restrict__ z,
-                                                 const double* __restrict__ q, Reducer R, PcgScalars* sc,
-                                                 double* beta_out) {
+                                                 const double* __restrict__ q, Slots S, const PcgScalars* sc) {
     if (!sc->active) return;
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
@@ -72,31 +62,17 @@ __global__ void __launch_bounds__(256) k_dots_rz(Dims D, const double* __restric
         v[0] = r[p] * zp;
         v[1] = zp * q[p];
     }
-    double out[2];
-    if (grid_reduce<2, NT>(v, slot(R, 1), R.max_blocks, R.tickets + 1, out)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            double beta = 0.0;
-            if (!sc->first) {
-                if (sc->flexible)
-                    beta = -beta_out[1] * out[1] / sc->rz_old;  // beta_out[1] holds alpha
-                else
-                    beta = out[0] / sc->rz_old;
-            }
-            sc->rz = out[0];
-            sc->zq = out[1];
-            beta_out[0] = beta;
-        }
-    }
+    block_partials<2, NT>(v, S.part, S.max_blocks);
 }
 
-// p_new = z + beta p_old ; q = A p_new ; pq = (p_new, q); last block: alpha
+// p_new = z + beta p_old ; q = A p_new ; partials of (p_new, q)
 __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restrict__ diag,
                                                    const double* __restrict__ cE, const double* __restrict__ cN,
                                                    const double* __restrict__ z, const double* __restrict__ pold,
-                                                   double* __restrict__ pnew, double* __restrict__ q, Reducer R,
-                                                   PcgScalars* sc, double* ab) {
+                                                   double* __restrict__ pnew, double* __restrict__ q, Slots S,
+                                                   const PcgScalars* sc) {
     if (!sc->active) return;
-    const double beta = ab[0];
+    const double beta = sc->beta;
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     double v[1] = {0.0};
@@ -111,22 +87,15 @@ __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restr
         q[p] = qv;
         v[0] = pc * qv;
     }
-    double out[1];
-    if (grid_reduce<1, NT>(v, slot(R, 2), R.max_blocks, R.tickets + 2, out)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            sc->pq = out[0];
-            ab[1] = sc->rz / out[0];  // alpha
-            sc->first = 0;
-        }
-    }
+    block_partials<1, NT>(v, S.part, S.max_blocks);
 }
 
-// x += alpha p ; r -= alpha q ; rr = (r,r); last block: convergence test
+// x += alpha p ; r -= alpha q ; partials of (r,r)
 __global__ void __launch_bounds__(256) k_update(Dims D, const double* __restrict__ p, const double* __restrict__ q,
-                                                double* __restrict__ x, double* __restrict__ r, Reducer R,
-                                                PcgScalars* sc, const double* ab) {
+                                                double* __restrict__ x, double* __restrict__ r, Slots S,
+                                                const PcgScalars* sc) {
     if (!sc->active) return;
-    const double alpha = ab[1];
+    const double alpha = sc->alpha;
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     double v[1] = {0.0};
@@ -137,120 +106,198 @@ __global__ void __launch_bounds__(256) k_update(Dims D, const double* __restrict
         r[k] = rn;
         v[0] = rn * rn;
     }
-    double out[1];
-    if (grid_reduce<1, NT>(v, slot(R, 3), R.max_blocks, R.tickets + 3, out)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            sc->rr = out[0];
-            sc->rz_old = sc->rz;
-            sc->iters += 1;
-            if (!(out[0] > sc->tol2)) sc->active = 0;
-            if (isnan(out[0])) {
-                sc->active = 0;
-                sc->guard |= 1;
-            }
-        }
-    }
+    block_partials<1, NT>(v, S.part, S.max_blocks);
 }
 
-// true residual of the fine system: sum of squares and max |.| over fluid rows
+// partials of the true residual: sum of squares and max |.| over fluid rows
 __global__ void __launch_bounds__(256) k_true_resid(Dims D, const double* __restrict__ diag,
                                                     const double* __restrict__ cE, const double* __restrict__ cN,
                                                     const double* __restrict__ b, const double* __restrict__ x,
-                                                    double* partials, int max_blocks, unsigned int* ticket,
-                                                    double* out) {
-    __shared__ double ssum[NT / 64], smax[NT / 64];
-    __shared__ bool is_last;
+                                                    Slots S) {
+    __shared__ double smax[NT / 64];
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
-    double s = 0.0, m = 0.0;
+    double v[1] = {0.0};
+    double m = 0.0;
     if (i < D.ni && j < D.nj) {
         const size_t p = D.off(i, j);
         const size_t ld = D.ld;
         if (diag[p] > 0.0) {
             const double rr = b[p] - (diag[p] * x[p] - cE[p - 1] * x[p - 1] - cE[p] * x[p + 1] -
                                       cN[p - ld] * x[p - ld] - cN[p] * x[p + ld]);
-            s = rr * rr;
+            v[0] = rr * rr;
             m = fabs(rr);
         }
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-        s += __shfl_xor(s, o, 64);
-        m = fmax(m, __shfl_xor(m, o, 64));
-    }
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    if ((tid & 63) == 0) {
-        ssum[tid >> 6] = s;
-        smax[tid >> 6] = m;
-    }
-    __syncthreads();
-    const int nblocks = gridDim.x * gridDim.y;
-    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+    if ((tid & 63) == 0) smax[tid >> 6] = m;
+    block_partials<1, NT>(v, S.part, S.max_blocks);  // synchronises the block
     if (tid == 0) {
-        double a = 0.0, c = 0.0;
-        for (int w = 0; w < NT / 64; ++w) {
-            a += ssum[w];
-            c = fmax(c, smax[w]);
+        double c = 0.0;
+        for (int w = 0; w < NT / 64; ++w) c = fmax(c, smax[w]);
+        S.part[(size_t)S.max_blocks + blockIdx.x + blockIdx.y * gridDim.x] = c;
+    }
+}
+
+// One block: reduce the partials in block order and update the scalars.
+__global__ void __launch_bounds__(NTF) k_finalize(Slots S, int nblocks, int mode, PcgScalars* sc, double tol) {
+    if (mode != FIN_INIT && mode != FIN_REPLACE && mode != FIN_TRUE && !sc->active) return;
+    if (mode == FIN_TRUE) {
+        __shared__ double mx[NTF / 64];
+        double s[1];
+        finalize_sum<1, NTF>(S.part, S.max_blocks, nblocks, s);
+        double m = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += NTF) m = fmax(m, S.part[(size_t)S.max_blocks + b]);
+        for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0) mx[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double c = 0.0;
+            for (int w = 0; w < NTF / 64; ++w) c = fmax(c, mx[w]);
+            sc->true_rr = s[0];
+            sc->true_max = c;
         }
-        partials[bid] = a;
-        partials[max_blocks + bid] = c;
-        __threadfence();
-        const unsigned int t = atomicAdd(ticket, 1u);
-        is_last = (t == (unsigned int)(nblocks - 1));
+        return;
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double a = 0.0, c = 0.0;
-    for (int bb = tid; bb < nblocks; bb += NT) {
-        a += partials[bb];
-        c = fmax(c, partials[max_blocks + bb]);
-    }
-    for (int o = 32; o >= 1; o >>= 1) {
-        a += __shfl_xor(a, o, 64);
-        c = fmax(c, __shfl_xor(c, o, 64));
-    }
-    if ((tid & 63) == 0) {
-        ssum[tid >> 6] = a;
-        smax[tid >> 6] = c;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double aa = 0.0, cc = 0.0;
-        for (int w = 0; w < NT / 64; ++w) {
-            aa += ssum[w];
-            cc = fmax(cc, smax[w]);
+    if (mode == FIN_INIT || mode == FIN_REPLACE || mode == FIN_RZ) {
+        double s[2];
+        finalize_sum<2, NTF>(S.part, S.max_blocks, nblocks, s);
+        if (threadIdx.x != 0) return;
+        if (mode == FIN_RZ) {
+            double beta = 0.0;
+            if (!sc->first) beta = sc->flexible ? -sc->alpha * s[1] / sc->rz_old : s[0] / sc->rz_old;
+            sc->rz = s[0];
+            sc->zq = s[1];
+            sc->beta = beta;
+            return;
         }
-        out[0] = aa;
-        out[1] = cc;
-        *ticket = 0u;
-        __threadfence();
+        // FIN_INIT / FIN_REPLACE: s[0] = ||b||^2, s[1] = ||r||^2
+        if (mode == FIN_INIT) {
+            sc->bb = s[0];
+            sc->tol2 = tol * tol * s[0];
+            sc->iters = 0;
+        }
+        sc->rr = s[1];
+        sc->first = 1;
+        sc->beta = 0.0;
+        sc->active = (s[0] > 0.0 && s[1] > sc->tol2) ? 1 : 0;
+        sc->rz = sc->rz_old = sc->zq = sc->pq = 0.0;
+        return;
     }
+    double s[1];
+    finalize_sum<1, NTF>(S.part, S.max_blocks, nblocks, s);
+    if (threadIdx.x != 0) return;
+    if (mode == FIN_PQ) {
+        sc->pq = s[0];
+        sc->alpha = sc->rz / s[0];
+        sc->first = 0;
+    } else {  // FIN_RR
+        sc->rr = s[0];
+        sc->rz_old = sc->rz;
+        sc->iters += 1;
+        if (!(s[0] > sc->tol2)) sc->active = 0;
+        if (isnan(s[0])) {
+            sc->active = 0;
+            sc->guard |= 1;
+        }
+    }
+}
+
+Slots slots(Ctx* c, int s) {
+    return Slots{c->red.partials + (size_t)s * NRED * c->red.max_blocks, c->red.max_blocks};
+}
+
+// one PCG iteration (V-cycle + direction + update), enqueued on the stream
+int enqueue_iteration(Ctx* c, int parity) {
+    Level& L0 = c->lev[0];
+    const Dims D = L0.d;
+    hipStream_t s = c->stream;
+    const dim3 g = grid2d(c->ni, c->nj), b = block2d();
+    const int nb = (int)(g.x * g.y);
+    double* pold = parity ? c->p1 : c->p0;
+    double* pnew = parity ? c->p0 : c->p1;
+    int rc = mg_vcycle(c, c->r, c->z);
+    if (rc) return rc;
+    Slots S1 = slots(c, 1), S2 = slots(c, 2), S3 = slots(c, 3);
+    hipLaunchKernelGGL(k_dots_rz, g, b, 0, s, D, (const double*)c->r, (const double*)c->z, (const double*)c->q, S1,
+                       (const PcgScalars*)c->sc);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S1, nb, (int)FIN_RZ, c->sc, 0.0);
+    hipLaunchKernelGGL(k_dir_apply, g, b, 0, s, D, L0.A.diag, L0.A.cE, L0.A.cN, (const double*)c->z,
+                       (const double*)pold, pnew, c->q, S2, (const PcgScalars*)c->sc);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S2, nb, (int)FIN_PQ, c->sc, 0.0);
+    hipLaunchKernelGGL(k_update, g, b, 0, s, D, (const double*)pnew, (const double*)c->q, c->x, c->r, S3,
+                       (const PcgScalars*)c->sc);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S3, nb, (int)FIN_RR, c->sc, 0.0);
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+int enqueue_init(Ctx* c, int warm, int mode) {
+    Level& L0 = c->lev[0];
+    const dim3 g = grid2d(c->ni, c->nj);
+    const int nb = (int)(g.x * g.y);
+    Slots S0 = slots(c, 0);
+    hipLaunchKernelGGL(k_init, g, block2d(), 0, c->stream, L0.d, L0.A.diag, L0.A.cE, L0.A.cN,
+                       (const double*)c->rhs, c->x, c->r, warm, S0);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, c->stream, S0, nb, mode, c->sc, c->prm.tol);
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+// Two PCG iterations (parities 0 and 1) captured once into a graph.
+int iteration_graph(Ctx* c, hipGraphExec_t* out) {
+    const int k = c->prm.precision == SG_PREC_MIXED ? 1 : 0;
+    if (c->iter_graph[k] && c->graph_pre[k] == c->prm.mg_pre && c->graph_post[k] == c->prm.mg_post) {
+        *out = c->iter_graph[k];
+        return SG_OK;
+    }
+    if (c->iter_graph[k]) {
+        (void)hipGraphExecDestroy(c->iter_graph[k]);
+        c->iter_graph[k] = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    SG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_iteration(c, 0);
+    if (rc == SG_OK) rc = enqueue_iteration(c, 1);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    c->iter_graph[k] = exec;
+    c->graph_pre[k] = c->prm.mg_pre;
+    c->graph_post[k] = c->prm.mg_post;
+    *out = exec;
+    return SG_OK;
 }
 
 }  // namespace
 
 int fine_residual_norms(Ctx* c, double* relres, double* maxrow) {
     Level& L0 = c->lev[0];
-    double* out = c->red.partials + (size_t)7 * NRED * c->red.max_blocks;  // spare slot
-    double* parts = c->red.partials + (size_t)5 * NRED * c->red.max_blocks;
-    hipLaunchKernelGGL(k_true_resid, grid2d(c->ni, c->nj), block2d(), 0, c->stream, L0.d, L0.A.diag, L0.A.cE,
-                       L0.A.cN, c->rhs, c->x, parts, c->red.max_blocks, c->red.tickets + 5, out);
+    const dim3 g = grid2d(c->ni, c->nj);
+    const int nb = (int)(g.x * g.y);
+    Slots S = slots(c, 4);
+    hipLaunchKernelGGL(k_true_resid, g, block2d(), 0, c->stream, L0.d, L0.A.diag, L0.A.cE, L0.A.cN,
+                       (const double*)c->rhs, (const double*)c->x, S);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, c->stream, S, nb, (int)FIN_TRUE, c->sc, 0.0);
     SG_HIP(hipGetLastError());
-    double h[2];
-    SG_HIP(hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
     const double bb = c->sc_host->bb;
-    *relres = bb > 0.0 ? sqrt(h[0] / bb) : 0.0;
-    *maxrow = h[1];
+    *relres = bb > 0.0 ? sqrt(c->sc_host->true_rr / bb) : 0.0;
+    *maxrow = c->sc_host->true_max;
     return SG_OK;
 }
 
 int pcg_solve(Ctx* c, sg_solve_info* info) {
-    Level& L0 = c->lev[0];
-    const Dims D = L0.d;
     hipStream_t s = c->stream;
-    const dim3 g = grid2d(c->ni, c->nj), b = block2d();
-    double* ab = c->red.partials + (size_t)6 * NRED * c->red.max_blocks;  // beta, alpha
     SG_HIP(hipEventRecord(c->ev0, s));
     if (!c->mg_ready) {
         int rc = mg_setup(c);
@@ -258,50 +305,56 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
     }
     SG_HIP(hipEventRecord(c->ev1, s));
     // flexible beta in mixed precision
-    const int flexible = c->prm.precision == SG_PREC_MIXED ? 1 : 0;
-    SG_HIP(hipMemcpyAsync(&c->sc->flexible, &flexible, sizeof(int), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_init, g, b, 0, s, D, L0.A.diag, L0.A.cE, L0.A.cN, (const double*)c->rhs, c->x, c->r,
-                       c->prm.warm_start, 0, c->red, c->sc, c->prm.tol);
-    SG_HIP(hipGetLastError());
+    c->sc_host->flexible = c->prm.precision == SG_PREC_MIXED ? 1 : 0;
+    SG_HIP(hipMemcpyAsync(&c->sc->flexible, &c->sc_host->flexible, sizeof(int), hipMemcpyHostToDevice, s));
+    int rc = enqueue_init(c, c->prm.warm_start, FIN_INIT);
+    if (rc) return rc;
     SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));
     const int max_iter = c->prm.max_iter > 0 ? c->prm.max_iter : 1000;
     const int every = c->prm.check_every > 0 ? c->prm.check_every : 1;
+    hipGraphExec_t graph = nullptr;
+    if (c->use_graph) {
+        c->pcur = 0;
+        rc = iteration_graph(c, &graph);
+        if (rc) return rc;
+    }
     int it = 0;
     int replacements = 0;
     for (;;) {
-    while (c->sc_host->active && it < max_iter) {
-        const int nb = (max_iter - it) < every ? (max_iter - it) : every;
-        for (int k = 0; k < nb; ++k) {
-            double* pold = c->pcur ? c->p1 : c->p0;
-            double* pnew = c->pcur ? c->p0 : c->p1;
-            int rc = mg_vcycle(c, c->r, c->z);
-            if (rc) return rc;
-            hipLaunchKernelGGL(k_dots_rz, g, b, 0, s, D, (const double*)c->r, (const double*)c->z,
-                               (const double*)c->q, c->red, c->sc, ab);
-            hipLaunchKernelGGL(k_dir_apply, g, b, 0, s, D, L0.A.diag, L0.A.cE, L0.A.cN, (const double*)c->z,
-                               (const double*)pold, pnew, c->q, c->red, c->sc, ab);
-            hipLaunchKernelGGL(k_update, g, b, 0, s, D, (const double*)pnew, (const double*)c->q, c->x, c->r, c->red,
-                               c->sc, (const double*)ab);
-            SG_HIP(hipGetLastError());
-            c->pcur ^= 1;
+        while (c->sc_host->active && it < max_iter) {
+            if (graph) {
+                // whole graph launches of two iterations; iterations past
+                // convergence leave x and r untouched (kernels test sc->active)
+                int pairs = (every + 1) / 2;
+                const int left = (max_iter - it + 1) / 2;
+                if (pairs > left) pairs = left;
+                for (int k = 0; k < pairs; ++k) SG_HIP(hipGraphLaunch(graph, s));
+                it += 2 * pairs;
+            } else {
+                const int nb = (max_iter - it) < every ? (max_iter - it) : every;
+                for (int k = 0; k < nb; ++k) {
+                    rc = enqueue_iteration(c, c->pcur);
+                    if (rc) return rc;
+                    c->pcur ^= 1;
+                }
+                it += nb;
+            }
+            SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+            SG_HIP(hipStreamSynchronize(s));
         }
-        it += nb;
+        // FP64 residual replacement: the recursive residual drifts from
+        // b - A x by rounding; restart CG from the true residual until it
+        // meets tol
+        if (c->sc_host->active || (c->sc_host->guard & 1) || replacements >= 4 || it >= max_iter) break;
+        rc = enqueue_init(c, 1, FIN_REPLACE);
+        if (rc) return rc;
         SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
         SG_HIP(hipStreamSynchronize(s));
+        ++replacements;
+        if (!c->sc_host->active) break;
     }
-    // FP64 residual replacement: the recursive residual drifts from b - A x
-    // by rounding; restart CG from the true residual until it meets tol
-    if (c->sc_host->active || (c->sc_host->guard & 1) || replacements >= 4 || it >= max_iter) break;
-    hipLaunchKernelGGL(k_init, g, b, 0, s, D, L0.A.diag, L0.A.cE, L0.A.cN, (const double*)c->rhs, c->x, c->r, 1, 1,
-                       c->red, c->sc, c->prm.tol);
-    SG_HIP(hipGetLastError());
-    SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipStreamSynchronize(s));
-    ++replacements;
-    if (!c->sc_host->active) break;
-    }
-    int rc = poisson_scatter(c);
+    rc = poisson_scatter(c);
     if (rc) return rc;
     SG_HIP(hipEventRecord(c->ev2, s));
     double relres = 0.0, maxrow = 0.0;

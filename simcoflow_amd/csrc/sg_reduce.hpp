@@ -1,8 +1,13 @@
-// Deterministic FP64 reductions for gfx950: 64-lane wavefront butterfly
-// (__shfl_xor, lowered to DPP/ds_swizzle), then across the block's
-// wavefronts through LDS, then across blocks by a last-block-done ticket.
-// The last block adds the per-block partials in block order, so a sum does
-// not depend on the order in which blocks finish.
+// Deterministic FP64 reductions for gfx950 in two phases:
+//   1. every block reduces its values -- 64-lane wavefront butterfly
+//      (__shfl_xor, lowered to DPP/ds_swizzle), then across the block's
+//      wavefronts through LDS -- and stores one partial per block with a
+//      plain store;
+//   2. a one-block finalize kernel adds the partials in block order.
+// The kernel boundary between the phases makes the partials visible to the
+// finalize block without any device-scope fence inside the producer (a fence
+// per block forces an L2 write-back on gfx950 and costs far more than the
+// extra launch).  Sums do not depend on the order in which blocks finish.
 #pragma once
 #include "sg_internal.hpp"
 
@@ -40,44 +45,29 @@ __device__ inline void block_reduce(double (&v)[NV]) {
     __syncthreads();
 }
 
-// Grid-wide sum.  Every block calls it once with its per-thread values; the
-// block that finishes last writes the sums to out[0..NV).  Returns true in
-// the threads of that last block (thread 0 holds nothing special).
+// Phase 1: store this block's sums at partials[k*max_blocks + block].
 template <int NV, int NT>
-__device__ inline bool grid_reduce(double (&v)[NV], double* partials, int max_blocks,
-                                   unsigned int* ticket, double* out) {
-    __shared__ bool is_last;
+__device__ inline void block_partials(double (&v)[NV], double* partials, int max_blocks) {
     block_reduce<NV, NT>(v);
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    const int nblocks = gridDim.x * gridDim.y;
-    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
     if (tid == 0) {
+        const int bid = blockIdx.x + blockIdx.y * gridDim.x;
 #pragma unroll
         for (int k = 0; k < NV; ++k) partials[(size_t)k * max_blocks + bid] = v[k];
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        is_last = (t == (unsigned int)(nblocks - 1));
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double s[NV];
+}
+
+// Phase 2 (inside a one-block kernel of NT threads): sums of the first
+// nblocks partials of each of the NV planes; valid in thread 0.
+template <int NV, int NT>
+__device__ inline void finalize_sum(const double* partials, int max_blocks, int nblocks, double (&s)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) s[k] = 0.0;
-    for (int b = tid; b < nblocks; b += NT) {
+    for (int b = threadIdx.x; b < nblocks; b += NT) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            s[k] += partials[(size_t)k * max_blocks + b];
-        }
+        for (int k = 0; k < NV; ++k) s[k] += partials[(size_t)k * max_blocks + b];
     }
     block_reduce<NV, NT>(s);
-    if (tid == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) out[k] = s[k];
-        *ticket = 0u;
-        __threadfence();
-    }
-    return true;
 }
 
 }  // namespace sg
