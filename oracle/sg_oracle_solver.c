@@ -1,10 +1,12 @@
 /*
- * CPU FP64 restatement of the projection solver: PCG preconditioned by a
- * structured geometric multigrid V-cycle (2x2 aggregation with Galerkin
- * coarse operators, red-black Gauss-Seidel smoothing, direct solve on the
- * coarsest level).  It stands where HYPRE ParCSR PCG + BoomerAMG stands in
- * the reference (ProjectionP_Mod.f90:52-58,101-135): the converged pressure
- * is fixed by A and b, not by the solver's iterates.
+ * CPU FP64 restatement of the projection solver: PCG preconditioned by the
+ * structured multigrid V-cycle the GPU path runs -- coarse points on every
+ * other cell, operator-dependent (black-box) prolongation, Galerkin 9-point
+ * coarse operators P^T A P, red-black Gauss-Seidel on the 5-point fine
+ * level, multicolour Gauss-Seidel on the coarse levels, dense solve on the
+ * coarsest level.  It stands where HYPRE ParCSR PCG + BoomerAMG stands in the
+ * reference (ProjectionP_Mod.f90:52-58,101-135): the converged pressure is
+ * fixed by A and b, not by the solver's iterates.
  *
  * TEST INFRASTRUCTURE ONLY: the checker and the timed CPU baseline.
  */
@@ -16,12 +18,11 @@
 #include <omp.h>
 #endif
 
-#define COARSEST_MAX 64
+#define COARSEST_MAX 16
 
-static double g_cscale = 1.0;
-void orc_set_coarse_scale(double s) { g_cscale = s; }
-static int g_gamma = 1;
-void orc_set_cycle(int g) { g_gamma = g; }
+/* colours of the coarse-level Gauss-Seidel: 4 ((i&1)+2(j&1)) or 2 (i+j&1) */
+static int g_coarse_colours = 4;
+void orc_set_coarse_colours(int n) { g_coarse_colours = (n == 2) ? 2 : 4; }
 
 int orc_num_threads(void) {
 #ifdef _OPENMP
@@ -38,90 +39,177 @@ void orc_set_num_threads(int n) {
 #endif
 }
 
+/* 9-point symmetric operator stored as positive couplings (entry = -c) */
 typedef struct {
     int ni, nj;
-    double *cE, *cN, *diag, *x, *b, *r;
+    double *diag, *cE, *cN, *cNE, *cNW;
+    double *x, *b, *r;
+    /* prolongation from the next level, at its resolution */
+    double* w[9];
 } level_t;
+
+enum { W_INJ = 0, W_XW, W_XE, W_YS, W_YN, W_SW, W_SE, W_NW, W_NE };
 
 typedef struct {
     int nl;
     level_t lv[32];
-    double* cinv; /* dense inverse of the coarsest operator, n x n */
+    double* cinv;
     int cn;
 } hier_t;
 
-#define IX(l, i, j) ((size_t)(i) + (size_t)(j) * (size_t)(l)->ni)
-
 static double* dalloc(size_t n) { return (double*)calloc(n ? n : 1, sizeof(double)); }
 
-/* Galerkin coarse operator for piecewise-constant prolongation over 2x2
- * aggregates (solid cells have diag == 0 and zero faces, so they drop out). */
-static void galerkin(const level_t* f, level_t* c) {
-#pragma omp parallel for
-    for (int J = 0; J < c->nj; ++J) {
-        for (int I = 0; I < c->ni; ++I) {
-            double d = 0.0, e = 0.0, n = 0.0;
-            const int i0 = 2 * I, j0 = 2 * J;
-            const int i1 = i0 + 1 < f->ni ? i0 + 1 : -1;
-            const int j1 = j0 + 1 < f->nj ? j0 + 1 : -1;
-            d += f->diag[IX(f, i0, j0)];
-            if (i1 >= 0) d += f->diag[IX(f, i1, j0)];
-            if (j1 >= 0) d += f->diag[IX(f, i0, j1)];
-            if (i1 >= 0 && j1 >= 0) d += f->diag[IX(f, i1, j1)];
-            if (i1 >= 0) {
-                d -= 2.0 * f->cE[IX(f, i0, j0)];
-                if (j1 >= 0) d -= 2.0 * f->cE[IX(f, i0, j1)];
-            }
-            if (j1 >= 0) {
-                d -= 2.0 * f->cN[IX(f, i0, j0)];
-                if (i1 >= 0) d -= 2.0 * f->cN[IX(f, i1, j0)];
-            }
-            const int ie = i1 >= 0 ? i1 : i0;
-            e += f->cE[IX(f, ie, j0)];
-            if (j1 >= 0) e += f->cE[IX(f, ie, j1)];
-            const int jn = j1 >= 0 ? j1 : j0;
-            n += f->cN[IX(f, i0, jn)];
-            if (i1 >= 0) n += f->cN[IX(f, i1, jn)];
-            if (I == c->ni - 1) e = 0.0;
-            if (J == c->nj - 1) n = 0.0;
-            c->diag[IX(c, I, J)] = d > 0.0 ? g_cscale * d : 0.0;
-            c->cE[IX(c, I, J)] = g_cscale * e;
-            c->cN[IX(c, I, J)] = g_cscale * n;
-        }
-    }
+#define K(L, i, j) ((size_t)(i) + (size_t)(j) * (size_t)(L)->ni)
+static int inb(const level_t* L, int i, int j) { return i >= 0 && j >= 0 && i < L->ni && j < L->nj; }
+static double get(const level_t* L, const double* a, int i, int j) { return (a && inb(L, i, j)) ? a[K(L, i, j)] : 0.0; }
+
+/* matrix entry A((i,j),(i+di,j+dj)) */
+static double aent(const level_t* L, int i, int j, int di, int dj) {
+    if (!inb(L, i, j) || !inb(L, i + di, j + dj)) return 0.0;
+    if (di == 0 && dj == 0) return L->diag[K(L, i, j)];
+    if (dj == 0) return di > 0 ? -get(L, L->cE, i, j) : -get(L, L->cE, i - 1, j);
+    if (di == 0) return dj > 0 ? -get(L, L->cN, i, j) : -get(L, L->cN, i, j - 1);
+    if (!L->cNE) return 0.0;
+    if (di > 0 && dj > 0) return -get(L, L->cNE, i, j);
+    if (di < 0 && dj < 0) return -get(L, L->cNE, i - 1, j - 1);
+    if (di < 0 && dj > 0) return -get(L, L->cNW, i, j);
+    return -get(L, L->cNW, i + 1, j - 1);
 }
 
-/* Dense inverse of the coarsest operator by Gauss-Jordan with a zero-pivot
- * guard (a zero pivot -- a solid cell, or the constant mode of an all-Neumann
- * problem -- maps to a zero row/column). */
+/* coupling of (i,j) to (i+di,j+dj) as a positive number */
+static double cpl(const level_t* L, int i, int j, int di, int dj) { return -aent(L, i, j, di, dj); }
+
+/* P(g, D): weight of coarse point D on fine cell g; C = coarse level holding w */
+static double pw(const level_t* F, const level_t* C, int gi, int gj, int DI, int DJ) {
+    if (gi < 0 || gj < 0 || gi >= F->ni || gj >= F->nj) return 0.0;
+    const int pi = gi & 1, pj = gj & 1;
+    double* const* w = F->w;
+    if (!pi && !pj) return (gi / 2 == DI && gj / 2 == DJ) ? get(C, w[W_INJ], DI, DJ) : 0.0;
+    if (pi && !pj) {
+        const int I0 = (gi - 1) / 2, J0 = gj / 2;
+        if (DJ != J0) return 0.0;
+        if (DI == I0) return get(C, w[W_XW], I0, J0);
+        if (DI == I0 + 1) return get(C, w[W_XE], I0, J0);
+        return 0.0;
+    }
+    if (!pi && pj) {
+        const int I0 = gi / 2, J0 = (gj - 1) / 2;
+        if (DI != I0) return 0.0;
+        if (DJ == J0) return get(C, w[W_YS], I0, J0);
+        if (DJ == J0 + 1) return get(C, w[W_YN], I0, J0);
+        return 0.0;
+    }
+    const int I0 = (gi - 1) / 2, J0 = (gj - 1) / 2;
+    const int a = DI - I0, b = DJ - J0;
+    if (a < 0 || a > 1 || b < 0 || b > 1) return 0.0;
+    return get(C, w[W_SW + a + 2 * b], I0, J0);
+}
+
+/* black-box prolongation weights of fine level F, stored on C's grid */
+static void prolongation(level_t* F, const level_t* C) {
+    for (int k = 0; k < 9; ++k) F->w[k] = dalloc((size_t)C->ni * C->nj);
+#pragma omp parallel for
+    for (int J = 0; J < C->nj; ++J)
+        for (int I = 0; I < C->ni; ++I) {
+            const size_t c = K(C, I, J);
+            const int fi = 2 * I, fj = 2 * J;
+            F->w[W_INJ][c] = F->diag[K(F, fi, fj)] > 0.0 ? 1.0 : 0.0;
+            if (fi + 1 < F->ni) {
+                const int i = fi + 1, j = fj;
+                const double d = F->diag[K(F, i, j)];
+                if (d > 0.0) {
+                    const double den = d - cpl(F, i, j, 0, 1) - cpl(F, i, j, 0, -1);
+                    const double wc = cpl(F, i, j, -1, 0) + cpl(F, i, j, -1, 1) + cpl(F, i, j, -1, -1);
+                    const double ec = cpl(F, i, j, 1, 0) + cpl(F, i, j, 1, 1) + cpl(F, i, j, 1, -1);
+                    if (den > 0.0) {
+                        F->w[W_XW][c] = wc / den;
+                        F->w[W_XE][c] = ec / den;
+                    }
+                }
+            }
+            if (fj + 1 < F->nj) {
+                const int i = fi, j = fj + 1;
+                const double d = F->diag[K(F, i, j)];
+                if (d > 0.0) {
+                    const double den = d - cpl(F, i, j, 1, 0) - cpl(F, i, j, -1, 0);
+                    const double sr = cpl(F, i, j, 0, -1) + cpl(F, i, j, -1, -1) + cpl(F, i, j, 1, -1);
+                    const double nr = cpl(F, i, j, 0, 1) + cpl(F, i, j, -1, 1) + cpl(F, i, j, 1, 1);
+                    if (den > 0.0) {
+                        F->w[W_YS][c] = sr / den;
+                        F->w[W_YN][c] = nr / den;
+                    }
+                }
+            }
+        }
+#pragma omp parallel for
+    for (int J = 0; J < C->nj; ++J)
+        for (int I = 0; I < C->ni; ++I) {
+            const size_t c = K(C, I, J);
+            const int i = 2 * I + 1, j = 2 * J + 1;
+            if (i >= F->ni || j >= F->nj) continue;
+            const double d = F->diag[K(F, i, j)];
+            if (!(d > 0.0)) continue;
+            const double cW = cpl(F, i, j, -1, 0), cE = cpl(F, i, j, 1, 0);
+            const double cS = cpl(F, i, j, 0, -1), cN = cpl(F, i, j, 0, 1);
+            const double cSW = cpl(F, i, j, -1, -1), cSE = cpl(F, i, j, 1, -1);
+            const double cNW = cpl(F, i, j, -1, 1), cNE = cpl(F, i, j, 1, 1);
+            double* const* w = F->w;
+#define WG(k, a, b) get(C, w[k], a, b)
+            F->w[W_SW][c] = (cW * WG(W_YS, I, J) + cS * WG(W_XW, I, J) + cSW * WG(W_INJ, I, J)) / d;
+            F->w[W_SE][c] = (cE * WG(W_YS, I + 1, J) + cS * WG(W_XE, I, J) + cSE * WG(W_INJ, I + 1, J)) / d;
+            F->w[W_NW][c] = (cW * WG(W_YN, I, J) + cN * WG(W_XW, I, J + 1) + cNW * WG(W_INJ, I, J + 1)) / d;
+            F->w[W_NE][c] = (cE * WG(W_YN, I + 1, J) + cN * WG(W_XE, I, J + 1) + cNE * WG(W_INJ, I + 1, J + 1)) / d;
+#undef WG
+        }
+}
+
+/* C->op = P^T F->op P */
+static void galerkin(const level_t* F, level_t* C) {
+#pragma omp parallel for
+    for (int J = 0; J < C->nj; ++J)
+        for (int I = 0; I < C->ni; ++I) {
+            const int DI[5] = {I, I + 1, I, I + 1, I - 1};
+            const int DJ[5] = {J, J, J + 1, J + 1, J + 1};
+            double acc[5] = {0, 0, 0, 0, 0};
+            for (int b = -1; b <= 1; ++b)
+                for (int a = -1; a <= 1; ++a) {
+                    const int fi = 2 * I + a, fj = 2 * J + b;
+                    const double pc = pw(F, C, fi, fj, I, J);
+                    if (pc == 0.0) continue;
+                    for (int dj = -1; dj <= 1; ++dj)
+                        for (int di = -1; di <= 1; ++di) {
+                            const double afg = aent(F, fi, fj, di, dj);
+                            if (afg == 0.0) continue;
+                            for (int k = 0; k < 5; ++k) acc[k] += pc * afg * pw(F, C, fi + di, fj + dj, DI[k], DJ[k]);
+                        }
+                }
+            const size_t c = K(C, I, J);
+            C->diag[c] = acc[0] > 0.0 ? acc[0] : 0.0;
+            C->cE[c] = (I + 1 < C->ni) ? -acc[1] : 0.0;
+            C->cN[c] = (J + 1 < C->nj) ? -acc[2] : 0.0;
+            C->cNE[c] = (I + 1 < C->ni && J + 1 < C->nj) ? -acc[3] : 0.0;
+            C->cNW[c] = (I > 0 && J + 1 < C->nj) ? -acc[4] : 0.0;
+        }
+}
+
 static void coarse_inverse(const level_t* c, double* inv) {
     const int n = c->ni * c->nj;
     double* a = dalloc((size_t)n * n);
-    memset(inv, 0, sizeof(double) * (size_t)n * n);
-    for (int j = 0; j < c->nj; ++j)
-        for (int i = 0; i < c->ni; ++i) {
-            const int k = i + j * c->ni;
-            a[(size_t)k * n + k] = c->diag[k];
-            if (i + 1 < c->ni) {
-                a[(size_t)k * n + k + 1] = -c->cE[k];
-                a[(size_t)(k + 1) * n + k] = -c->cE[k];
-            }
-            if (j + 1 < c->nj) {
-                a[(size_t)k * n + k + c->ni] = -c->cN[k];
-                a[(size_t)(k + c->ni) * n + k] = -c->cN[k];
-            }
+    for (int r = 0; r < n; ++r)
+        for (int q = 0; q < n; ++q) {
+            const int ri = r % c->ni, rj = r / c->ni, qi = q % c->ni, qj = q / c->ni;
+            const int di = qi - ri, dj = qj - rj;
+            a[(size_t)r * n + q] = (abs(di) <= 1 && abs(dj) <= 1) ? aent(c, ri, rj, di, dj) : 0.0;
+            inv[(size_t)r * n + q] = r == q ? 1.0 : 0.0;
         }
-    for (int k = 0; k < n; ++k) inv[(size_t)k * n + k] = 1.0;
     double amax = 0.0;
     for (int k = 0; k < n; ++k) amax = fmax(amax, fabs(a[(size_t)k * n + k]));
     for (int k = 0; k < n; ++k) {
         const double piv = a[(size_t)k * n + k];
         if (!(fabs(piv) > 1e-13 * amax)) {
             for (int m = 0; m < n; ++m) {
-                a[(size_t)k * n + m] = 0.0;
-                a[(size_t)m * n + k] = 0.0;
-                inv[(size_t)k * n + m] = 0.0;
-                inv[(size_t)m * n + k] = 0.0;
+                a[(size_t)k * n + m] = a[(size_t)m * n + k] = 0.0;
+                inv[(size_t)k * n + m] = inv[(size_t)m * n + k] = 0.0;
             }
             continue;
         }
@@ -143,19 +231,28 @@ static void coarse_inverse(const level_t* c, double* inv) {
     free(a);
 }
 
+static void alloc_level(level_t* L, int ni, int nj, int nine) {
+    memset(L, 0, sizeof(*L));
+    L->ni = ni;
+    L->nj = nj;
+    const size_t n = (size_t)ni * nj;
+    L->diag = dalloc(n);
+    L->cE = dalloc(n);
+    L->cN = dalloc(n);
+    if (nine) {
+        L->cNE = dalloc(n);
+        L->cNW = dalloc(n);
+    }
+    L->x = dalloc(n);
+    L->b = dalloc(n);
+    L->r = dalloc(n);
+}
+
 static void build_hier(hier_t* h, int ni, int nj, const double* matr) {
     memset(h, 0, sizeof(*h));
     const size_t N = (size_t)ni * nj;
     level_t* f = &h->lv[0];
-    f->ni = ni;
-    f->nj = nj;
-    f->cE = dalloc(N);
-    f->cN = dalloc(N);
-    f->diag = dalloc(N);
-    f->x = dalloc(N);
-    f->b = dalloc(N);
-    f->r = dalloc(N);
-    /* symmetric form: the E/N entry of each row is the face coefficient */
+    alloc_level(f, ni, nj, 0);
     for (size_t k = 0; k < N; ++k) {
         f->diag[k] = matr[k + 2 * N] > 0.0 ? matr[k + 2 * N] : 0.0;
         f->cE[k] = -matr[k + 3 * N];
@@ -164,15 +261,8 @@ static void build_hier(hier_t* h, int ni, int nj, const double* matr) {
     int l = 0;
     while ((size_t)h->lv[l].ni * h->lv[l].nj > COARSEST_MAX && l < 30) {
         level_t* c = &h->lv[l + 1];
-        c->ni = (h->lv[l].ni + 1) / 2;
-        c->nj = (h->lv[l].nj + 1) / 2;
-        const size_t M = (size_t)c->ni * c->nj;
-        c->cE = dalloc(M);
-        c->cN = dalloc(M);
-        c->diag = dalloc(M);
-        c->x = dalloc(M);
-        c->b = dalloc(M);
-        c->r = dalloc(M);
+        alloc_level(c, (h->lv[l].ni + 1) / 2, (h->lv[l].nj + 1) / 2, 1);
+        prolongation(&h->lv[l], c);
         galerkin(&h->lv[l], c);
         ++l;
     }
@@ -185,45 +275,61 @@ static void build_hier(hier_t* h, int ni, int nj, const double* matr) {
 static void free_hier(hier_t* h) {
     for (int l = 0; l < h->nl; ++l) {
         level_t* v = &h->lv[l];
-        free(v->cE); free(v->cN); free(v->diag); free(v->x); free(v->b); free(v->r);
+        free(v->diag); free(v->cE); free(v->cN); free(v->cNE); free(v->cNW);
+        free(v->x); free(v->b); free(v->r);
+        for (int k = 0; k < 9; ++k) free(v->w[k]);
     }
     free(h->cinv);
 }
 
-/* one half-sweep of red-black Gauss-Seidel on colour col ((i+j)&1 == col) */
-static void rb_half(level_t* L, int col) {
+/* one Gauss-Seidel pass over the points of one colour (Jacobi within the
+ * colour for couplings between points of the same colour) */
+static void gs_colour(level_t* L, int ncol, int col) {
     const int ni = L->ni, nj = L->nj;
+    double* xn = L->r; /* new values of this colour, then copied */
 #pragma omp parallel for
-    for (int j = 0; j < nj; ++j) {
-        for (int i = ((j + col) & 1); i < ni; i += 2) {
-            const size_t k = IX(L, i, j);
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            const int c = ncol == 2 ? ((i + j) & 1) : ((i & 1) + 2 * (j & 1));
+            if (c != col) continue;
+            const size_t k = K(L, i, j);
             const double d = L->diag[k];
-            if (!(d > 0.0)) { L->x[k] = 0.0; continue; }
+            if (!(d > 0.0)) { xn[k] = 0.0; continue; }
             double s = L->b[k];
-            if (i > 0) s += L->cE[k - 1] * L->x[k - 1];
-            if (i + 1 < ni) s += L->cE[k] * L->x[k + 1];
-            if (j > 0) s += L->cN[k - ni] * L->x[k - ni];
-            if (j + 1 < nj) s += L->cN[k] * L->x[k + ni];
-            L->x[k] = s / d;
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int di = -1; di <= 1; ++di) {
+                    if (!di && !dj) continue;
+                    const double a = aent(L, i, j, di, dj);
+                    if (a != 0.0) s -= a * L->x[K(L, i + di, j + dj)];
+                }
+            xn[k] = s / d;
         }
-    }
+#pragma omp parallel for
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni; ++i) {
+            const int c = ncol == 2 ? ((i + j) & 1) : ((i & 1) + 2 * (j & 1));
+            if (c == col) L->x[K(L, i, j)] = xn[K(L, i, j)];
+        }
 }
 
-/* y = A x for a level operator */
+static void smooth(level_t* L, int level, int sweeps, int reverse) {
+    const int ncol = (level == 0) ? 2 : g_coarse_colours;
+    for (int s = 0; s < sweeps; ++s)
+        for (int k = 0; k < ncol; ++k) gs_colour(L, ncol, reverse ? ncol - 1 - k : k);
+}
+
 static void lv_apply(const level_t* L, const double* x, double* y) {
-    const int ni = L->ni, nj = L->nj;
 #pragma omp parallel for
-    for (int j = 0; j < nj; ++j) {
-        for (int i = 0; i < ni; ++i) {
-            const size_t k = IX(L, i, j);
-            double s = L->diag[k] * x[k];
-            if (i > 0) s -= L->cE[k - 1] * x[k - 1];
-            if (i + 1 < ni) s -= L->cE[k] * x[k + 1];
-            if (j > 0) s -= L->cN[k - ni] * x[k - ni];
-            if (j + 1 < nj) s -= L->cN[k] * x[k + ni];
-            y[k] = s;
+    for (int j = 0; j < L->nj; ++j)
+        for (int i = 0; i < L->ni; ++i) {
+            double s = 0.0;
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int di = -1; di <= 1; ++di) {
+                    const double a = aent(L, i, j, di, dj);
+                    if (a != 0.0) s += a * x[K(L, i + di, j + dj)];
+                }
+            y[K(L, i, j)] = s;
         }
-    }
 }
 
 static void vcycle(hier_t* h, int l, int pre, int post) {
@@ -238,45 +344,40 @@ static void vcycle(hier_t* h, int l, int pre, int post) {
         }
         return;
     }
+    level_t* C = &h->lv[l + 1];
     memset(L->x, 0, N * sizeof(double));
-    for (int s = 0; s < pre; ++s) { rb_half(L, 0); rb_half(L, 1); }
+    smooth(L, l, pre, 0);
     lv_apply(L, L->x, L->r);
-    level_t* c = &h->lv[l + 1];
+    for (size_t k = 0; k < N; ++k) L->r[k] = L->b[k] - L->r[k];
+    /* restriction P^T r */
 #pragma omp parallel for
-    for (int J = 0; J < c->nj; ++J)
-        for (int I = 0; I < c->ni; ++I) {
+    for (int J = 0; J < C->nj; ++J)
+        for (int I = 0; I < C->ni; ++I) {
             double s = 0.0;
-            for (int b = 0; b < 2; ++b)
-                for (int a = 0; a < 2; ++a) {
-                    const int i = 2 * I + a, j = 2 * J + b;
-                    if (i < L->ni && j < L->nj) {
-                        const size_t k = IX(L, i, j);
-                        s += L->b[k] - L->r[k];
-                    }
+            for (int b = -1; b <= 1; ++b)
+                for (int a = -1; a <= 1; ++a) {
+                    const int fi = 2 * I + a, fj = 2 * J + b;
+                    if (!inb(L, fi, fj)) continue;
+                    const double w = pw(L, C, fi, fj, I, J);
+                    if (w != 0.0) s += w * L->r[K(L, fi, fj)];
                 }
-            c->b[IX(c, I, J)] = s;
+            C->b[K(C, I, J)] = s;
         }
     vcycle(h, l + 1, pre, post);
-    for (int gg = 1; gg < g_gamma && l + 1 < h->nl - 1; ++gg) {
-        /* extra coarse-grid iterations: residual of the coarse problem */
-        level_t* cc = &h->lv[l + 1];
-        const size_t M = (size_t)cc->ni * cc->nj;
-        double* save_b = dalloc(M); double* save_x = dalloc(M);
-        memcpy(save_b, cc->b, M * sizeof(double)); memcpy(save_x, cc->x, M * sizeof(double));
-        lv_apply(cc, cc->x, cc->r);
-        for (size_t k = 0; k < M; ++k) cc->b[k] = save_b[k] - cc->r[k];
-        vcycle(h, l + 1, pre, post);
-        for (size_t k = 0; k < M; ++k) cc->x[k] += save_x[k];
-        memcpy(cc->b, save_b, M * sizeof(double));
-        free(save_b); free(save_x);
-    }
+    /* x += P e */
 #pragma omp parallel for
     for (int j = 0; j < L->nj; ++j)
         for (int i = 0; i < L->ni; ++i) {
-            const size_t k = IX(L, i, j);
-            if (L->diag[k] > 0.0) L->x[k] += c->x[IX(c, i / 2, j / 2)];
+            double s = 0.0;
+            const int I0 = (i - 1) / 2 < 0 ? 0 : (i - 1) / 2;
+            for (int J = (j - 1) / 2 < 0 ? 0 : (j - 1) / 2; J <= (j + 1) / 2 && J < C->nj; ++J)
+                for (int I = I0; I <= (i + 1) / 2 && I < C->ni; ++I) {
+                    const double w = pw(L, C, i, j, I, J);
+                    if (w != 0.0) s += w * C->x[K(C, I, J)];
+                }
+            L->x[K(L, i, j)] += s;
         }
-    for (int s = 0; s < post; ++s) { rb_half(L, 1); rb_half(L, 0); }
+    smooth(L, l, post, 1);
 }
 
 static double dot(const double* a, const double* b, size_t n) {
@@ -294,7 +395,6 @@ int orc_pcg_mg_solve(int ni, int nj, const double* matr, const double* b,
     level_t* F = &h.lv[0];
     const size_t N = (size_t)ni * nj;
     double *r = dalloc(N), *z = dalloc(N), *p = dalloc(N), *q = dalloc(N);
-    /* solid rows carry no unknown */
     for (size_t k = 0; k < N; ++k) if (!(F->diag[k] > 0.0)) x[k] = 0.0;
     lv_apply(F, x, q);
     for (size_t k = 0; k < N; ++k) r[k] = (F->diag[k] > 0.0) ? b[k] - q[k] : 0.0;
@@ -310,32 +410,38 @@ int orc_pcg_mg_solve(int ni, int nj, const double* matr, const double* b,
         return 0;
     }
     double rz = 0.0;
-    while (rn > tol * bn && it < max_iter) {
-        memcpy(F->b, r, N * sizeof(double));
-        vcycle(&h, 0, pre, post);
-        memcpy(z, F->x, N * sizeof(double));
-        const double rz_new = dot(r, z, N);
-        if (it == 0) {
-            memcpy(p, z, N * sizeof(double));
-        } else {
-            const double beta = rz_new / rz;
-            for (size_t k = 0; k < N; ++k) p[k] = z[k] + beta * p[k];
+    int first = 1;
+    for (int rep = 0; rep < 5; ++rep) {
+        while (rn > tol * bn && it < max_iter) {
+            memcpy(F->b, r, N * sizeof(double));
+            vcycle(&h, 0, pre, post);
+            memcpy(z, F->x, N * sizeof(double));
+            const double rz_new = dot(r, z, N);
+            if (first) {
+                memcpy(p, z, N * sizeof(double));
+                first = 0;
+            } else {
+                const double beta = rz_new / rz;
+                for (size_t k = 0; k < N; ++k) p[k] = z[k] + beta * p[k];
+            }
+            rz = rz_new;
+            lv_apply(F, p, q);
+            const double alpha = rz / dot(p, q, N);
+            for (size_t k = 0; k < N; ++k) {
+                x[k] += alpha * p[k];
+                r[k] -= alpha * q[k];
+            }
+            rn = sqrt(dot(r, r, N));
+            ++it;
         }
-        rz = rz_new;
-        lv_apply(F, p, q);
-        const double alpha = rz / dot(p, q, N);
-        for (size_t k = 0; k < N; ++k) {
-            x[k] += alpha * p[k];
-            r[k] -= alpha * q[k];
-        }
+        /* residual replacement: restart from the true residual */
+        lv_apply(F, x, q);
+        for (size_t k = 0; k < N; ++k) r[k] = (F->diag[k] > 0.0) ? b[k] - q[k] : 0.0;
         rn = sqrt(dot(r, r, N));
-        ++it;
+        first = 1;
+        if (rn <= tol * bn || it >= max_iter) break;
     }
-    /* report the true residual */
-    lv_apply(F, x, q);
-    double tr = 0.0;
-    for (size_t k = 0; k < N; ++k) if (F->diag[k] > 0.0) tr += (b[k] - q[k]) * (b[k] - q[k]);
-    if (relres_out) *relres_out = sqrt(tr) / bn;
+    if (relres_out) *relres_out = rn / bn;
     free_hier(&h); free(r); free(z); free(p); free(q);
     return (rn > tol * bn) ? -it : it;
 }
@@ -344,8 +450,7 @@ int orc_pcg_jacobi_solve(int ni, int nj, const double* matr, const double* b,
                          double* x, double tol, int max_iter, double* relres_out) {
     level_t F;
     const size_t N = (size_t)ni * nj;
-    F.ni = ni; F.nj = nj;
-    F.cE = dalloc(N); F.cN = dalloc(N); F.diag = dalloc(N);
+    alloc_level(&F, ni, nj, 0);
     for (size_t k = 0; k < N; ++k) {
         F.diag[k] = matr[k + 2 * N] > 0.0 ? matr[k + 2 * N] : 0.0;
         F.cE[k] = -matr[k + 3 * N];
@@ -378,6 +483,7 @@ int orc_pcg_jacobi_solve(int ni, int nj, const double* matr, const double* b,
         ++it;
     }
     if (relres_out) *relres_out = bn > 0.0 ? rn / bn : 0.0;
-    free(F.cE); free(F.cN); free(F.diag); free(r); free(z); free(p); free(q);
+    free(F.diag); free(F.cE); free(F.cN); free(F.x); free(F.b); free(F.r);
+    free(r); free(z); free(p); free(q);
     return (bn > 0.0 && rn > tol * bn) ? -it : it;
 }

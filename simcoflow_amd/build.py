@@ -22,6 +22,7 @@ SOURCES = [
     "sg_poisson.hip",
     "sg_mg.hip",
     "sg_pcg.hip",
+    "sg_cycle.hip",
 ]
 EXTRA = {
     "sg_poisson.hip": ["-ffp-contract=off"],

@@ -1,0 +1,796 @@
+// Multigrid V-cycle kernels: one fused kernel per level on the way down
+// (red-black pre-smoothing from a zero guess + residual + restriction),
+// one on the way up (prolongation + red-black post-smoothing), and one
+// single-workgroup kernel for all levels of at most TAIL_MAX points.
+//
+// Every kernel stages its operands (b, x, the operator) for a tile plus the
+// halo its passes need in LDS with one unrolled load phase, then runs the
+// colour passes out of LDS: a sweep reads each coefficient once from HBM
+// and pays one memory latency instead of one per pass.  On the 9-point
+// coarse levels the red-black passes treat couplings between cells of the
+// same colour (the diagonal ones) Jacobi-wise: new values are staged in
+// registers and written after a barrier, which keeps the update independent
+// of thread order and the cycle symmetric (post-smoothing runs the colours
+// in reverse order).
+#include "sg_internal.hpp"
+#include "sg_mg_common.hpp"
+#include "sg_reduce.hpp"
+
+namespace sg {
+
+namespace {
+
+constexpr int NTB = 256;
+
+constexpr int CW = 32, CH = 16;  // coarse-level (9-point) tile
+constexpr int TAIL_MAX = 256;    // levels with at most this many points run in the tail kernel
+constexpr int TAIL_THREADS = 1024;
+constexpr int MAX_TAIL = 10;
+constexpr size_t TAIL_LDS_MAX = 64 * 1024;
+
+struct Slots {
+    double* part;
+    int max_blocks;
+};
+
+__device__ inline bool in_dom(const Dims& D, int i, int j) { return i >= 0 && j >= 0 && i < D.ni && j < D.nj; }
+
+// operator of a tile (+halo) in LDS; local (li,lj) <-> cell (i0+li-h, j0+lj-h)
+template <typename T, int RW, int RH, bool NINE>
+struct Stage {
+    T d[RH][RW], e[RH][RW], n[RH][RW];
+    T ne[NINE ? RH : 1][NINE ? RW : 1], nw[NINE ? RH : 1][NINE ? RW : 1];
+    __device__ void load(const OpV<T>& A, const Dims& F, int i0, int j0, int h, int tid) {
+        constexpr int NL = (RW * RH + NTB - 1) / NTB;
+#pragma unroll
+        for (int m = 0; m < NL; ++m) {
+            const int k = tid + m * NTB;
+            if (k >= RW * RH) continue;
+            const int li = k % RW, lj = k / RW;
+            const int gi = i0 + li - h, gj = j0 + lj - h;
+            T a = T(0), b = T(0), c = T(0), dd = T(0), f = T(0);
+            if (in_dom(F, gi, gj)) {
+                const size_t p = F.off(gi, gj);
+                a = A.diag[p];
+                b = A.cE[p];
+                c = A.cN[p];
+                if (NINE) {
+                    dd = A.cNE[p];
+                    f = A.cNW[p];
+                }
+            }
+            d[lj][li] = a;
+            e[lj][li] = b;
+            n[lj][li] = c;
+            if (NINE) {
+                ne[lj][li] = dd;
+                nw[lj][li] = f;
+            }
+        }
+    }
+    template <typename XF>
+    __device__ T coupled(int li, int lj, XF X) const {
+        T s = e[lj][li - 1] * X(-1, 0) + e[lj][li] * X(1, 0) + n[lj - 1][li] * X(0, -1) + n[lj][li] * X(0, 1);
+        if (NINE)
+            s += ne[lj][li] * X(1, 1) + ne[lj - 1][li - 1] * X(-1, -1) + nw[lj][li] * X(-1, 1) +
+                 nw[lj - 1][li + 1] * X(1, -1);
+        return s;
+    }
+};
+
+// prolongation weights (and optionally the coarse vector) of a coarse window
+// [I0, I0+WW) x [J0, J0+WH); ring cells read as zero
+template <typename T, int WW, int WH>
+struct StageW {
+    T w[9][WH][WW];
+    T ev[WH][WW];
+    __device__ void load(const Prol<T>& P, const T* e, const Dims& C, int I0, int J0, int tid) {
+        for (int k = tid; k < WW * WH; k += NTB) {
+            const int a = k % WW, b = k / WW;
+            const int I = I0 + a, J = J0 + b;
+            const bool ok = I >= -1 && J >= -1 && I <= C.ni && J <= C.nj;
+            const size_t c = ok ? C.off(I, J) : 0;
+#pragma unroll
+            for (int m = 0; m < 9; ++m) w[m][b][a] = ok ? P.w[m][c] : T(0);
+            if (e) ev[b][a] = ok ? e[c] : T(0);
+        }
+    }
+    // P^T r at coarse (I,J) sitting at window (wa,wb); R(a,b) = r at fine (2I+a,2J+b)
+    template <typename RF>
+    __device__ T restrict_at(int wa, int wb, RF R) const {
+        T s = w[W_INJ][wb][wa] * R(0, 0);
+        s += w[W_XW][wb][wa] * R(1, 0) + w[W_XE][wb][wa - 1] * R(-1, 0);
+        s += w[W_YS][wb][wa] * R(0, 1) + w[W_YN][wb - 1][wa] * R(0, -1);
+        s += w[W_SW][wb][wa] * R(1, 1) + w[W_SE][wb][wa - 1] * R(-1, 1) + w[W_NW][wb - 1][wa] * R(1, -1) +
+             w[W_NE][wb - 1][wa - 1] * R(-1, -1);
+        return s;
+    }
+    // P e at a fine cell of parity (pi,pj) whose coarse parent sits at (a,c)
+    __device__ T prolong_at(int pi, int pj, int a, int c) const {
+        if (!pi && !pj) return w[W_INJ][c][a] * ev[c][a];
+        if (pi && !pj) return w[W_XW][c][a] * ev[c][a] + w[W_XE][c][a] * ev[c][a + 1];
+        if (!pi && pj) return w[W_YS][c][a] * ev[c][a] + w[W_YN][c][a] * ev[c + 1][a];
+        return w[W_SW][c][a] * ev[c][a] + w[W_SE][c][a] * ev[c][a + 1] + w[W_NW][c][a] * ev[c + 1][a] +
+               w[W_NE][c][a] * ev[c + 1][a + 1];
+    }
+};
+
+// one colour pass over the local rectangle [lo, RW-lo) x [lo, RH-lo);
+// new values staged in registers, written after a barrier
+template <typename T, int RW, int RH, bool NINE, int LO>
+__device__ inline void colour_pass(const Stage<T, RW, RH, NINE>& S, T (&sb)[RH][RW], T (&sx)[RH][RW], int i0, int j0,
+                                   int h, int col, int tid) {
+    constexpr int W = RW - 2 * LO, H = RH - 2 * LO;
+    constexpr int NB = (W * H + NTB - 1) / NTB;
+    T nv[NB];
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        const int k = tid + m * NTB;
+        nv[m] = T(0);
+        if (k >= W * H) continue;
+        const int li = k % W + LO, lj = k / W + LO;
+        const int gi = i0 + li - h, gj = j0 + lj - h;
+        T v = sx[lj][li];
+        if (((gi + gj) & 1) == col) {
+            const T d = S.d[lj][li];
+            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
+            v = d > T(0) ? (sb[lj][li] + S.coupled(li, lj, X)) / d : T(0);
+        }
+        nv[m] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= W * H) continue;
+        sx[k / W + LO][k % W + LO] = nv[m];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// down: x = S(0; b) (red from zero, then black), r = b - A x, bc = P^T r
+template <typename T, typename TB, bool NINE, int TW, int TH>
+__global__ void __launch_bounds__(NTB) k_down(Dims F, OpV<T> A, const TB* __restrict__ b, T* __restrict__ xo, Dims C,
+                                              Prol<T> P, T* __restrict__ bc) {
+    constexpr int H = 3;
+    constexpr int RW = TW + 5, RH = TH + 5;  // cells [-3, T+2)
+    constexpr int WW = TW / 2 + 1, WH = TH / 2 + 1;
+    constexpr int NL = (RW * RH + NTB - 1) / NTB;
+    __shared__ Stage<T, RW, RH, NINE> S;
+    __shared__ StageW<T, WW, WH> Wt;
+    __shared__ T sb[RH][RW];
+    __shared__ T sx[RH][RW];
+    const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
+    const int I0 = i0 / 2, J0 = j0 / 2;
+    const int tid = threadIdx.x;
+    S.load(A, F, i0, j0, H, tid);
+    Wt.load(P, nullptr, C, I0 - 1, J0 - 1, tid);
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= RW * RH) continue;
+        const int li = k % RW, lj = k / RW;
+        const int gi = i0 + li - H, gj = j0 + lj - H;
+        sb[lj][li] = in_dom(F, gi, gj) ? (T)b[F.off(gi, gj)] : T(0);
+    }
+    __syncthreads();
+    // red from a zero guess
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= RW * RH) continue;
+        const int li = k % RW, lj = k / RW;
+        const int gi = i0 + li - H, gj = j0 + lj - H;
+        const T d = S.d[lj][li];
+        sx[lj][li] = (((gi + gj) & 1) == 0 && d > T(0)) ? sb[lj][li] / d : T(0);
+    }
+    __syncthreads();
+    colour_pass<T, RW, RH, NINE, 1>(S, sb, sx, i0, j0, H, 1, tid);
+    // residual on cells [-1,TW) x [-1,TH), written over b
+    {
+        constexpr int W = TW + 1, HH = TH + 1;
+        constexpr int NR = (W * HH + NTB - 1) / NTB;
+        T rv[NR];
+#pragma unroll
+        for (int m = 0; m < NR; ++m) {
+            const int k = tid + m * NTB;
+            rv[m] = T(0);
+            if (k >= W * HH) continue;
+            const int li = k % W + 2, lj = k / W + 2;
+            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
+            rv[m] = sb[lj][li] - S.d[lj][li] * sx[lj][li] + S.coupled(li, lj, X);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NR; ++m) {
+            const int k = tid + m * NTB;
+            if (k >= W * HH) continue;
+            sb[k / W + 2][k % W + 2] = rv[m];
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < (TW / 2) * (TH / 2); k += NTB) {
+        const int a = k % (TW / 2), bq = k / (TW / 2);
+        const int I = I0 + a, J = J0 + bq;
+        if (I < C.ni && J < C.nj) {
+            const int li = 2 * a + H, lj = 2 * bq + H;
+            auto R = [&](int aa, int bb) { return sb[lj + bb][li + aa]; };
+            bc[C.off(I, J)] = Wt.restrict_at(a + 1, bq + 1, R);
+        }
+    }
+    for (int k = tid; k < TW * TH; k += NTB) {
+        const int li = k % TW + H, lj = k / TW + H;
+        const int gi = i0 + li - H, gj = j0 + lj - H;
+        if (in_dom(F, gi, gj)) xo[F.off(gi, gj)] = sx[lj][li];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// up: x = xin + P e, then black and red passes; writes xo (TZ).  DOTS: the
+// fine level of the PCG -- partials of (b, x) and (x, q), b being the PCG
+// residual the cycle was applied to.
+template <typename T, typename TB, typename TZ, bool NINE, bool DOTS, int TW, int TH>
+__global__ void __launch_bounds__(NTB) k_up(Dims F, OpV<T> A, const TB* __restrict__ b, const T* __restrict__ xin,
+                                            Dims C, Prol<T> P, const T* __restrict__ e, TZ* __restrict__ xo,
+                                            const double* __restrict__ q, Slots Sl) {
+    constexpr int H = 2;
+    constexpr int RW = TW + 4, RH = TH + 4;  // cells [-2, T+2)
+    constexpr int WW = TW / 2 + 3, WH = TH / 2 + 3;
+    constexpr int NL = (RW * RH + NTB - 1) / NTB;
+    constexpr int NR = (TW * TH + NTB - 1) / NTB;
+    __shared__ Stage<T, RW, RH, NINE> S;
+    __shared__ StageW<T, WW, WH> Wt;
+    __shared__ T sb[RH][RW];
+    __shared__ T sx[RH][RW];
+    const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
+    const int I0 = i0 / 2, J0 = j0 / 2;
+    const int tid = threadIdx.x;
+    S.load(A, F, i0, j0, H, tid);
+    Wt.load(P, e, C, I0 - 1, J0 - 1, tid);
+    T xr[NL];
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        const int k = tid + m * NTB;
+        xr[m] = T(0);
+        if (k >= RW * RH) continue;
+        const int li = k % RW, lj = k / RW;
+        const int gi = i0 + li - H, gj = j0 + lj - H;
+        T bv = T(0);
+        if (in_dom(F, gi, gj)) {
+            const size_t p = F.off(gi, gj);
+            bv = (T)b[p];
+            xr[m] = xin[p];
+        }
+        sb[lj][li] = bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= RW * RH) continue;
+        const int li = k % RW, lj = k / RW;
+        const int gi = i0 + li - H, gj = j0 + lj - H;
+        T xv = T(0);
+        if (in_dom(F, gi, gj)) xv = xr[m] + Wt.prolong_at(gi & 1, gj & 1, (gi >> 1) - I0 + 1, (gj >> 1) - J0 + 1);
+        sx[lj][li] = xv;
+    }
+    __syncthreads();
+    colour_pass<T, RW, RH, NINE, 1>(S, sb, sx, i0, j0, H, 1, tid);
+    // red on the tile, written straight out
+    double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= TW * TH) continue;
+        const int li = k % TW + H, lj = k / TW + H;
+        const int gi = i0 + li - H, gj = j0 + lj - H;
+        if (!in_dom(F, gi, gj)) continue;
+        T v = sx[lj][li];
+        if (((gi + gj) & 1) == 0) {
+            const T d = S.d[lj][li];
+            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
+            v = d > T(0) ? (sb[lj][li] + S.coupled(li, lj, X)) / d : T(0);
+        }
+        const size_t p = F.off(gi, gj);
+        xo[p] = (TZ)v;
+        if (DOTS) {
+            const double zv = (double)(TZ)v;
+            v0 += (double)b[p] * zv;
+            v1 += zv * q[p];
+        }
+    }
+    if (DOTS) {
+        double vv[2] = {v0, v1};
+        block_partials<2, NTB>(vv, Sl.part, Sl.max_blocks);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fine (5-point) level at full size: many tiles hide the memory latency, so
+// the operator is read straight from global memory (through the caches)
+// rather than staged, which keeps LDS small and occupancy high.
+constexpr int GW = 64, GH = 16;
+
+template <typename T, typename TB>
+__global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __restrict__ b, T* __restrict__ xo,
+                                               Dims C, Prol<T> P, T* __restrict__ bc) {
+    constexpr int RW = GW + 5, RH = GH + 5;  // local (li,lj) <-> cell (i0+li-3, j0+lj-3)
+    constexpr int NB = ((RW - 2) * (RH - 2) + NTB - 1) / NTB;
+    __shared__ T sb[RH][RW];
+    __shared__ T sx[RH][RW];
+    __shared__ T sr[RH][RW];
+    const int i0 = blockIdx.x * GW, j0 = blockIdx.y * GH;
+    const int tid = threadIdx.x;
+    const size_t ld = F.ld;
+    constexpr int NL = (RW * RH + NTB - 1) / NTB;
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= RW * RH) continue;
+        const int li = k % RW, lj = k / RW;
+        const int gi = i0 + li - 3, gj = j0 + lj - 3;
+        T bv = T(0), xv = T(0);
+        if (in_dom(F, gi, gj)) {
+            const size_t p = F.off(gi, gj);
+            bv = (T)b[p];
+            if (((gi + gj) & 1) == 0) {
+                const T d = A.diag[p];
+                xv = d > T(0) ? bv / d : T(0);
+            }
+        }
+        sb[lj][li] = bv;
+        sx[lj][li] = xv;
+    }
+    __syncthreads();
+    // black cells on [-2,GW+1) x [-2,GH+1); no black-black coupling on 5 points
+    T nv[NB];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+        const int k = tid + n * NTB;
+        nv[n] = T(0);
+        if (k >= (RW - 2) * (RH - 2)) continue;
+        const int li = k % (RW - 2) + 1, lj = k / (RW - 2) + 1;
+        const int gi = i0 + li - 3, gj = j0 + lj - 3;
+        T v = sx[lj][li];
+        if (((gi + gj) & 1) == 1 && in_dom(F, gi, gj)) {
+            const size_t p = F.off(gi, gj);
+            const T d = A.diag[p];
+            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
+            v = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+        }
+        nv[n] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+        const int k = tid + n * NTB;
+        if (k >= (RW - 2) * (RH - 2)) continue;
+        sx[k / (RW - 2) + 1][k % (RW - 2) + 1] = nv[n];
+    }
+    __syncthreads();
+    constexpr int NRS = ((GW + 1) * (GH + 1) + NTB - 1) / NTB;
+#pragma unroll
+    for (int m = 0; m < NRS; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= (GW + 1) * (GH + 1)) continue;
+        const int li = k % (GW + 1) + 2, lj = k / (GW + 1) + 2;
+        const int gi = i0 + li - 3, gj = j0 + lj - 3;
+        T r = T(0);
+        if (in_dom(F, gi, gj)) {
+            const size_t p = F.off(gi, gj);
+            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
+            r = sb[lj][li] - A.diag[p] * sx[lj][li] + coupled_sum<T>(A, p, ld, X);
+        }
+        sr[lj][li] = r;
+    }
+    __syncthreads();
+    {
+        const int I = i0 / 2 + tid % (GW / 2), J = j0 / 2 + tid / (GW / 2);
+        if (I < C.ni && J < C.nj) {
+            const int li = 2 * I - i0 + 3, lj = 2 * J - j0 + 3;
+            auto R = [&](int a, int bb) { return sr[lj + bb][li + a]; };
+            bc[C.off(I, J)] = restrict_point<T>(C, P, I, J, R);
+        }
+    }
+    for (int k = tid; k < GW * GH; k += NTB) {
+        const int li = k % GW + 3, lj = k / GW + 3;
+        const int gi = i0 + li - 3, gj = j0 + lj - 3;
+        if (in_dom(F, gi, gj)) xo[F.off(gi, gj)] = sx[lj][li];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __restrict__ b,
+                                             const T* __restrict__ xin, Dims C, Prol<T> P, const T* __restrict__ e,
+                                             double* __restrict__ z, const double* __restrict__ q, Slots Sl) {
+    constexpr int RW = GW + 4, RH = GH + 4;  // local (li,lj) <-> cell (i0+li-2, j0+lj-2)
+    constexpr int NR = (GW * GH + NTB - 1) / NTB;
+    __shared__ T sb[RH][RW];
+    __shared__ T sx[RH][RW];
+    const int i0 = blockIdx.x * GW, j0 = blockIdx.y * GH;
+    const int tid = threadIdx.x;
+    const size_t ld = F.ld;
+    constexpr int NL = (RW * RH + NTB - 1) / NTB;
+    constexpr int NBK = ((RW - 2) * (RH - 2) + NTB - 1) / NTB;
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= RW * RH) continue;
+        const int li = k % RW, lj = k / RW;
+        const int gi = i0 + li - 2, gj = j0 + lj - 2;
+        T bv = T(0), xv = T(0);
+        if (in_dom(F, gi, gj)) {
+            const size_t p = F.off(gi, gj);
+            bv = (T)b[p];
+            xv = xin[p] + prolong_point<T>(C, P, e, gi, gj);
+        }
+        sb[lj][li] = bv;
+        sx[lj][li] = xv;
+    }
+    __syncthreads();
+    // black on [-1,GW+1) x [-1,GH+1) in place (no black-black coupling)
+#pragma unroll
+    for (int m = 0; m < NBK; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= (RW - 2) * (RH - 2)) continue;
+        const int li = k % (RW - 2) + 1, lj = k / (RW - 2) + 1;
+        const int gi = i0 + li - 2, gj = j0 + lj - 2;
+        if (((gi + gj) & 1) == 1 && in_dom(F, gi, gj)) {
+            const size_t p = F.off(gi, gj);
+            const T d = A.diag[p];
+            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
+            sx[lj][li] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+        }
+    }
+    __syncthreads();
+    double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+    for (int n = 0; n < NR; ++n) {
+        const int k = tid + n * NTB;
+        if (k >= GW * GH) continue;
+        const int li = k % GW + 2, lj = k / GW + 2;
+        const int gi = i0 + li - 2, gj = j0 + lj - 2;
+        if (!in_dom(F, gi, gj)) continue;
+        const size_t p = F.off(gi, gj);
+        T v = sx[lj][li];
+        if (((gi + gj) & 1) == 0) {
+            const T d = A.diag[p];
+            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
+            v = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+        }
+        const double zv = (double)v;
+        z[p] = zv;
+        v0 += b[p] * zv;
+        v1 += zv * q[p];
+    }
+    double vv[2] = {v0, v1};
+    block_partials<2, NTB>(vv, Sl.part, Sl.max_blocks);
+}
+
+// ---------------------------------------------------------------------------
+// tail: the V-cycle on levels lt..nlev-1 in one workgroup, all of it in LDS
+// (compact layout cd: ld = ni+1; the arrays of a level follow each other)
+enum { TA_D = 0, TA_E, TA_N, TA_NE, TA_NW, TA_X, TA_B, TA_R, TA_COUNT };
+
+template <typename T>
+struct TailLevel {
+    Dims d;   // global layout
+    Dims cd;  // LDS layout
+    int base;  // LDS offset (elements) of this level's TA_COUNT arrays
+    int wbase; // LDS offset of its 9 prolongation-weight arrays (next level's cd)
+    OpV<T> A;
+    Prol<T> P;  // to the next level
+    T* x;       // global x of the level (written back for the first tail level)
+    const T* b; // global b (read for the first tail level)
+};
+
+template <typename T>
+struct TailArgs {
+    int n;  // number of tail levels (the last is the coarsest)
+    TailLevel<T> L[MAX_TAIL];
+    const T* inv;  // dense inverse of the coarsest operator
+};
+
+template <typename T>
+__device__ inline T* tarr(T* lds, const TailLevel<T>& L, int k) {
+    return lds + L.base + (size_t)k * L.cd.alloc_elems();
+}
+
+template <typename T>
+__device__ void tail_colour(T* lds, const TailLevel<T>& L, int col, bool zero_start) {
+    constexpr int NP = (TAIL_MAX + TAIL_THREADS - 1) / TAIL_THREADS;
+    const Dims& D = L.cd;
+    const int n = D.ni * D.nj;
+    const size_t ld = D.ld;
+    const T* d = tarr(lds, L, TA_D);
+    const T* e = tarr(lds, L, TA_E);
+    const T* nn = tarr(lds, L, TA_N);
+    const T* ne = tarr(lds, L, TA_NE);
+    const T* nw = tarr(lds, L, TA_NW);
+    T* x = tarr(lds, L, TA_X);
+    const T* b = tarr(lds, L, TA_B);
+    T nv[NP];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        const int k = threadIdx.x + m * TAIL_THREADS;
+        nv[m] = T(0);
+        if (k >= n) continue;
+        const int i = k % D.ni, j = k / D.ni;
+        const size_t p = D.off(i, j);
+        T v = zero_start ? T(0) : x[p];
+        if (((i + j) & 1) == col) {
+            const T dd = d[p];
+            if (zero_start) {
+                v = dd > T(0) ? b[p] / dd : T(0);
+            } else {
+                const T s = e[p - 1] * x[p - 1] + e[p] * x[p + 1] + nn[p - ld] * x[p - ld] + nn[p] * x[p + ld] +
+                            ne[p] * x[p + ld + 1] + ne[p - ld - 1] * x[p - ld - 1] + nw[p] * x[p + ld - 1] +
+                            nw[p - ld + 1] * x[p - ld + 1];
+                v = dd > T(0) ? (b[p] + s) / dd : T(0);
+            }
+        }
+        nv[m] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        const int k = threadIdx.x + m * TAIL_THREADS;
+        if (k >= n) continue;
+        x[D.off(k % D.ni, k / D.ni)] = nv[m];
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_elems) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    const int tid = threadIdx.x;
+    for (int k = tid; k < lds_elems; k += TAIL_THREADS) lds[k] = T(0);
+    __syncthreads();
+    // stage operators, weights and the first level's b
+    for (int l = 0; l < a.n; ++l) {
+        const TailLevel<T>& L = a.L[l];
+        const int n = L.d.ni * L.d.nj;
+        const bool nine = L.A.cNE != nullptr;
+        for (int k = tid; k < n; k += TAIL_THREADS) {
+            const int i = k % L.d.ni, j = k / L.d.ni;
+            const size_t g = L.d.off(i, j), c = L.cd.off(i, j);
+            tarr(lds, L, TA_D)[c] = L.A.diag[g];
+            tarr(lds, L, TA_E)[c] = L.A.cE[g];
+            tarr(lds, L, TA_N)[c] = L.A.cN[g];
+            tarr(lds, L, TA_NE)[c] = nine ? L.A.cNE[g] : T(0);
+            tarr(lds, L, TA_NW)[c] = nine ? L.A.cNW[g] : T(0);
+            if (l == 0) tarr(lds, L, TA_B)[c] = L.b[g];
+        }
+        if (l + 1 < a.n) {
+            const TailLevel<T>& Cl = a.L[l + 1];
+            const int nc = Cl.d.ni * Cl.d.nj;
+            for (int k = tid; k < nc; k += TAIL_THREADS) {
+                const int I = k % Cl.d.ni, J = k / Cl.d.ni;
+                const size_t g = Cl.d.off(I, J), c = Cl.cd.off(I, J);
+#pragma unroll
+                for (int m = 0; m < 9; ++m) lds[L.wbase + (size_t)m * Cl.cd.alloc_elems() + c] = L.P.w[m][g];
+            }
+        }
+    }
+    __syncthreads();
+    const int last = a.n - 1;
+    for (int l = 0; l < last; ++l) {
+        const TailLevel<T>& L = a.L[l];
+        const TailLevel<T>& Cl = a.L[l + 1];
+        const Dims& D = L.cd;
+        const size_t ld = D.ld;
+        tail_colour<T>(lds, L, 0, true);
+        tail_colour<T>(lds, L, 1, false);
+        const T* d = tarr(lds, L, TA_D);
+        const T* e = tarr(lds, L, TA_E);
+        const T* nn = tarr(lds, L, TA_N);
+        const T* ne = tarr(lds, L, TA_NE);
+        const T* nw = tarr(lds, L, TA_NW);
+        const T* x = tarr(lds, L, TA_X);
+        const T* b = tarr(lds, L, TA_B);
+        T* r = tarr(lds, L, TA_R);
+        for (int k = tid; k < D.ni * D.nj; k += TAIL_THREADS) {
+            const size_t p = D.off(k % D.ni, k / D.ni);
+            const T s = e[p - 1] * x[p - 1] + e[p] * x[p + 1] + nn[p - ld] * x[p - ld] + nn[p] * x[p + ld] +
+                        ne[p] * x[p + ld + 1] + ne[p - ld - 1] * x[p - ld - 1] + nw[p] * x[p + ld - 1] +
+                        nw[p - ld + 1] * x[p - ld + 1];
+            r[p] = b[p] - d[p] * x[p] + s;
+        }
+        __syncthreads();
+        T* bc = tarr(lds, Cl, TA_B);
+        const size_t wstride = Cl.cd.alloc_elems();
+        T* const w0 = lds + L.wbase;
+        for (int k = tid; k < Cl.cd.ni * Cl.cd.nj; k += TAIL_THREADS) {
+            const int I = k % Cl.cd.ni, J = k / Cl.cd.ni;
+            const Dims& Cd = Cl.cd;
+            auto W = [&](int m, int ii, int jj) { return w0[(size_t)m * wstride + Cd.off(ii, jj)]; };
+            auto R = [&](int aa, int bb) { return r[D.off(2 * I + aa, 2 * J + bb)]; };
+            T s = W(W_INJ, I, J) * R(0, 0);
+            s += W(W_XW, I, J) * R(1, 0) + W(W_XE, I - 1, J) * R(-1, 0);
+            s += W(W_YS, I, J) * R(0, 1) + W(W_YN, I, J - 1) * R(0, -1);
+            s += W(W_SW, I, J) * R(1, 1) + W(W_SE, I - 1, J) * R(-1, 1) + W(W_NW, I, J - 1) * R(1, -1) +
+                 W(W_NE, I - 1, J - 1) * R(-1, -1);
+            bc[Cd.off(I, J)] = s;
+        }
+        __syncthreads();
+    }
+    {
+        const TailLevel<T>& L = a.L[last];
+        const Dims& D = L.cd;
+        const int n = D.ni * D.nj;
+        const T* b = tarr(lds, L, TA_B);
+        T* x = tarr(lds, L, TA_X);
+        for (int r = tid; r < n; r += TAIL_THREADS) {
+            T s = T(0);
+            for (int q = 0; q < n; ++q) s += a.inv[(size_t)r * n + q] * b[D.off(q % D.ni, q / D.ni)];
+            x[D.off(r % D.ni, r / D.ni)] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = last - 1; l >= 0; --l) {
+        const TailLevel<T>& L = a.L[l];
+        const TailLevel<T>& Cl = a.L[l + 1];
+        const Dims& D = L.cd;
+        const Dims& Cd = Cl.cd;
+        const size_t wstride = Cd.alloc_elems();
+        const T* w0 = lds + L.wbase;
+        const T* ec = tarr(lds, Cl, TA_X);
+        T* x = tarr(lds, L, TA_X);
+        for (int k = tid; k < D.ni * D.nj; k += TAIL_THREADS) {
+            const int i = k % D.ni, j = k / D.ni;
+            const int I = i >> 1, J = j >> 1;
+            const size_t c = Cd.off(I, J);
+            auto W = [&](int m) { return w0[(size_t)m * wstride + c]; };
+            const int pi = i & 1, pj = j & 1;
+            T v;
+            if (!pi && !pj) v = W(W_INJ) * ec[c];
+            else if (pi && !pj) v = W(W_XW) * ec[c] + W(W_XE) * ec[c + 1];
+            else if (!pi && pj) v = W(W_YS) * ec[c] + W(W_YN) * ec[c + Cd.ld];
+            else v = W(W_SW) * ec[c] + W(W_SE) * ec[c + 1] + W(W_NW) * ec[c + Cd.ld] + W(W_NE) * ec[c + Cd.ld + 1];
+            x[D.off(i, j)] += v;
+        }
+        __syncthreads();
+        tail_colour<T>(lds, L, 1, false);
+        tail_colour<T>(lds, L, 0, false);
+    }
+    {
+        const TailLevel<T>& L = a.L[0];
+        const T* x = tarr(lds, L, TA_X);
+        for (int k = tid; k < L.d.ni * L.d.nj; k += TAIL_THREADS) {
+            const int i = k % L.d.ni, j = k / L.d.ni;
+            L.x[L.d.off(i, j)] = x[L.cd.off(i, j)];
+        }
+    }
+}
+
+// coarsest-only hierarchy (grids of at most COARSEST_MAX cells)
+template <typename T, typename TB, typename TZ>
+__global__ void __launch_bounds__(256) k_direct(Dims C, const T* __restrict__ inv, const TB* __restrict__ b,
+                                                TZ* __restrict__ x) {
+    const int n = C.ni * C.nj;
+    const int r = threadIdx.x;
+    if (r >= n) return;
+    T s = T(0);
+    for (int q = 0; q < n; ++q) s += inv[(size_t)r * n + q] * (T)b[C.off(q % C.ni, q / C.ni)];
+    x[C.off(r % C.ni, r / C.ni)] = (TZ)s;
+}
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+    static const Op<double>& op(const Level& L) { return L.A; }
+    static const Prol<double>& pr(const Level& L) { return L.P; }
+    static const LevelVecs<double>& vec(const Level& L) { return L.v; }
+    static const double* inv(const Ctx* c) { return c->coarse_inv; }
+};
+template <>
+struct Acc<float> {
+    static const Op<float>& op(const Level& L) { return L.Af; }
+    static const Prol<float>& pr(const Level& L) { return L.Pf; }
+    static const LevelVecs<float>& vec(const Level& L) { return L.vf; }
+    static const float* inv(const Ctx* c) { return c->coarse_inv_f; }
+};
+
+inline dim3 tiles(const Dims& d, int tw, int th) { return dim3((d.ni + tw - 1) / tw, (d.nj + th - 1) / th); }
+
+inline Dims compact(const Dims& d) {
+    Dims c = d;
+    c.ld = d.ni + 1;
+    return c;
+}
+
+// z = M^{-1} r ; fine-level dots (r,z), (z,q) go to the partials slot S
+template <typename T>
+int vcycle(Ctx* c, const double* r, double* z, const double* q, Slots S, int* dots_done) {
+    hipStream_t s = c->stream;
+    const int nl = c->nlev;
+    *dots_done = 0;
+    if (nl == 1) {
+        const Level& L = c->lev[0];
+        hipLaunchKernelGGL((k_direct<T, double, double>), dim3(1), dim3(256), 0, s, L.d, Acc<T>::inv(c), r, z);
+        SG_HIP(hipGetLastError());
+        return SG_OK;
+    }
+    *dots_done = 1;
+    int lt = 1;  // first level of the tail
+    while (lt < nl - 1 && (size_t)c->lev[lt].d.ni * c->lev[lt].d.nj > (size_t)TAIL_MAX) ++lt;
+    // down
+    for (int l = 0; l < lt; ++l) {
+        const Level& L = c->lev[l];
+        const Level& Cl = c->lev[l + 1];
+        if (l == 0)
+            hipLaunchKernelGGL((k_down5<T, double>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
+                               Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), Acc<T>::vec(Cl).b);
+        else
+            hipLaunchKernelGGL((k_down<T, T, true, CW, CH>), tiles(L.d, CW, CH), dim3(NTB), 0, s, L.d,
+                               opv(Acc<T>::op(L)), (const T*)Acc<T>::vec(L).b, Acc<T>::vec(L).x, Cl.d,
+                               Acc<T>::pr(L), Acc<T>::vec(Cl).b);
+    }
+    // tail
+    TailArgs<T> ta{};
+    ta.n = nl - lt;
+    if (ta.n > MAX_TAIL) {
+        set_error("multigrid: too many tail levels");
+        return SG_ERR_STATE;
+    }
+    int off = 0;
+    for (int k = 0; k < ta.n; ++k) {
+        const Level& L = c->lev[lt + k];
+        TailLevel<T>& t = ta.L[k];
+        t.d = L.d;
+        t.cd = compact(L.d);
+        t.base = off;
+        off += TA_COUNT * (int)t.cd.alloc_elems();
+        t.A = opv(Acc<T>::op(L));
+        t.P = Acc<T>::pr(L);
+        t.x = Acc<T>::vec(L).x;
+        t.b = Acc<T>::vec(L).b;
+        if (k + 1 < ta.n) {
+            t.wbase = off;
+            off += 9 * (int)compact(c->lev[lt + k + 1].d).alloc_elems();
+        }
+    }
+    const size_t lds_bytes = (size_t)off * sizeof(T);
+    if (lds_bytes > TAIL_LDS_MAX) {
+        set_error("multigrid: tail levels exceed the LDS budget");
+        return SG_ERR_STATE;
+    }
+    ta.inv = Acc<T>::inv(c);
+    hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(TAIL_THREADS), lds_bytes, s, ta, off);
+    // up: level l reads x_l from the down pass and e_{l+1}; writes its own
+    // correction into r_l (its solution for the level above)
+    for (int l = lt - 1; l >= 0; --l) {
+        const Level& L = c->lev[l];
+        const Level& Cl = c->lev[l + 1];
+        const T* e = (l + 1 == lt) ? Acc<T>::vec(Cl).x : Acc<T>::vec(Cl).r;
+        if (l == 0)
+            hipLaunchKernelGGL((k_up5<T>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
+                               (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e, z, q, S);
+        else
+            hipLaunchKernelGGL((k_up<T, T, T, true, false, CW, CH>), tiles(L.d, CW, CH), dim3(NTB), 0, s, L.d,
+                               opv(Acc<T>::op(L)), (const T*)Acc<T>::vec(L).b, (const T*)Acc<T>::vec(L).x, Cl.d,
+                               Acc<T>::pr(L), e, Acc<T>::vec(L).r, q, S);
+    }
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+}  // namespace
+
+int mg_vcycle_dots(Ctx* c, const double* r, double* z, const double* q, double* partials, int max_blocks,
+                   int* dots_done) {
+    Slots S{partials, max_blocks};
+    if (c->prm.precision == SG_PREC_MIXED) return vcycle<float>(c, r, z, q, S, dots_done);
+    return vcycle<double>(c, r, z, q, S, dots_done);
+}
+
+int mg_partial_blocks(Ctx* c) {
+    const dim3 t = tiles(c->lev[0].d, GW, GH);
+    return (int)(t.x * t.y);
+}
+
+}  // namespace sg

@@ -27,7 +27,7 @@ constexpr int NRED = 4;          // values a reduction slot can carry
 
 struct Dims {
     int ni = 0, nj = 0, ld = 0;
-    size_t alloc_elems() const { return (size_t)(nj + 3) * (size_t)ld; }
+    __host__ __device__ size_t alloc_elems() const { return (size_t)(nj + 3) * (size_t)ld; }
     __host__ __device__ size_t off(int i, int j) const {
         return (size_t)(j + 2) * (size_t)ld + (size_t)(i);
     }
@@ -171,7 +171,12 @@ int cell_to_padded(Ctx* c, const double* src_cell, double* dst);
 // multigrid (sg_mg.hip)
 int mg_allocate(Ctx* c);
 int mg_setup(Ctx* c);
-int mg_vcycle(Ctx* c, const double* r, double* z);  // fine padded vectors
+// z = M^{-1} r on the fine level (padded FP64 vectors); the per-block
+// partials of (r,z) and (z,q) land in `partials` (2 planes of max_blocks),
+// *dots_done says whether they did (a one-level hierarchy has no fused dots)
+int mg_vcycle_dots(Ctx* c, const double* r, double* z, const double* q, double* partials, int max_blocks,
+                   int* dots_done);
+int mg_partial_blocks(Ctx* c);  // blocks that wrote partials in mg_vcycle_dots
 // PCG (sg_pcg.hip)
 int pcg_solve(Ctx* c, sg_solve_info* info);
 int fine_residual_norms(Ctx* c, double* relres, double* maxrow);

@@ -10,68 +10,16 @@
 //    apertures shape the coarse basis;
 //  * coarse operators: Galerkin P^T A P (9-point, symmetric), built in FP64
 //    every solve; FP32 copies drive the mixed-precision cycle;
-//  * smoothing: red-black Gauss-Seidel on the 5-point fine level, four-colour
-//    Gauss-Seidel on the 9-point coarse levels, reversed order after the
-//    coarse-grid correction so the V-cycle is a symmetric preconditioner;
+//  * smoothing (sg_cycle.hip): red-black Gauss-Seidel on every level,
+//    reversed order after the coarse-grid correction so the V-cycle is a
+//    symmetric preconditioner;
 //  * coarsest level: dense inverse in FP64 (Gauss-Jordan in LDS).
 #include "sg_internal.hpp"
+#include "sg_mg_common.hpp"
 
 namespace sg {
 
 namespace {
-
-template <typename T>
-struct OpV {
-    const T *diag, *cE, *cN, *cNE, *cNW;  // cNE/cNW null on the 5-point level
-};
-
-template <typename T>
-__device__ inline T opget(const T* a, size_t p) {
-    return a ? a[p] : T(0);
-}
-
-// prolongation weight P(g, D) for fine cell g and coarse point D
-template <typename T>
-__device__ inline T pweight(const Dims& C, T* const* w, int gi, int gj, int DI, int DJ) {
-    const int pi = gi & 1, pj = gj & 1;
-    if (!pi && !pj) {
-        return (gi >> 1) == DI && (gj >> 1) == DJ ? w[W_INJ][C.off(DI, DJ)] : T(0);
-    }
-    if (pi && !pj) {
-        const int I0 = (gi - 1) >> 1, J0 = gj >> 1;
-        if (DJ != J0) return T(0);
-        if (DI == I0) return w[W_XW][C.off(I0, J0)];
-        if (DI == I0 + 1) return w[W_XE][C.off(I0, J0)];
-        return T(0);
-    }
-    if (!pi && pj) {
-        const int I0 = gi >> 1, J0 = (gj - 1) >> 1;
-        if (DI != I0) return T(0);
-        if (DJ == J0) return w[W_YS][C.off(I0, J0)];
-        if (DJ == J0 + 1) return w[W_YN][C.off(I0, J0)];
-        return T(0);
-    }
-    const int I0 = (gi - 1) >> 1, J0 = (gj - 1) >> 1;
-    const int a = DI - I0, b = DJ - J0;
-    if (a < 0 || a > 1 || b < 0 || b > 1) return T(0);
-    const int k = W_SW + a + 2 * b;  // SW, SE, NW, NE
-    return w[k][C.off(I0, J0)];
-}
-
-// coupling A(f, f+(di,dj)) as a matrix entry (negative of the stored c)
-template <typename T>
-__device__ inline T aentry(const Dims& F, const OpV<T>& A, int i, int j, int di, int dj) {
-    const size_t p = F.off(i, j);
-    const size_t ld = F.ld;
-    if (di == 0 && dj == 0) return A.diag[p];
-    if (dj == 0) return di > 0 ? -A.cE[p] : -A.cE[p - 1];
-    if (di == 0) return dj > 0 ? -A.cN[p] : -A.cN[p - ld];
-    if (!A.cNE) return T(0);
-    if (di > 0 && dj > 0) return -A.cNE[p];
-    if (di < 0 && dj < 0) return -A.cNE[p - ld - 1];
-    if (di < 0 && dj > 0) return -A.cNW[p];
-    return -A.cNW[p - ld + 1];  // (di > 0 && dj < 0)
-}
 
 // ---- prolongation weights --------------------------------------------------
 __global__ void __launch_bounds__(256) k_prol_edges(Dims F, OpV<double> A, Dims C, Prol<double> P) {
@@ -253,235 +201,6 @@ __global__ void __launch_bounds__(256) k_coarse_inverse(Dims C, OpV<double> A, d
     }
 }
 
-// ---- cycle kernels ----------------------------------------------------------
-// red-black Gauss-Seidel half sweep on the 5-point level; FIRST: the initial
-// guess is zero (colour 0 takes b/diag, colour 1 is cleared)
-template <typename T, typename TB, bool FIRST>
-__global__ void __launch_bounds__(256) k_rb5(Dims D, OpV<T> A, const TB* __restrict__ b, T* __restrict__ x, int col) {
-    const int t = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    const int i = 2 * t + ((j + col) & 1);
-    if (FIRST) {
-        // cover both colours of this pair of cells
-        const int i0 = 2 * t, i1 = 2 * t + 1;
-        if (j >= D.nj) return;
-        if (i0 < D.ni) {
-            const size_t p = D.off(i0, j);
-            const bool mine = ((i0 + j) & 1) == col;
-            const T d = A.diag[p];
-            x[p] = (mine && d > T(0)) ? (T)b[p] / d : T(0);
-        }
-        if (i1 < D.ni) {
-            const size_t p = D.off(i1, j);
-            const bool mine = ((i1 + j) & 1) == col;
-            const T d = A.diag[p];
-            x[p] = (mine && d > T(0)) ? (T)b[p] / d : T(0);
-        }
-        return;
-    }
-    if (i >= D.ni || j >= D.nj) return;
-    const size_t p = D.off(i, j);
-    const size_t ld = D.ld;
-    const T d = A.diag[p];
-    T s = (T)b[p] + A.cE[p - 1] * x[p - 1] + A.cE[p] * x[p + 1] + A.cN[p - ld] * x[p - ld] + A.cN[p] * x[p + ld];
-    x[p] = d > T(0) ? s / d : T(0);
-}
-
-// four-colour Gauss-Seidel on a 9-point level, colour = (i&1) + 2*(j&1)
-template <typename T, bool FIRST>
-__global__ void __launch_bounds__(256) k_gs9(Dims D, OpV<T> A, const T* __restrict__ b, T* __restrict__ x, int col) {
-    const int ci = col & 1, cj = col >> 1;
-    const int i = 2 * (blockIdx.x * TX + threadIdx.x) + ci;
-    const int j = 2 * (blockIdx.y * TY + threadIdx.y) + cj;
-    if (FIRST) {
-        // zero initial guess: this colour takes b/diag, the other three are cleared
-        const int i0 = i - ci, j0 = j - cj;
-        for (int bj = 0; bj < 2; ++bj)
-            for (int bi = 0; bi < 2; ++bi) {
-                const int ii = i0 + bi, jj = j0 + bj;
-                if (ii >= D.ni || jj >= D.nj) continue;
-                const size_t p = D.off(ii, jj);
-                const T d = A.diag[p];
-                x[p] = (bi == ci && bj == cj && d > T(0)) ? b[p] / d : T(0);
-            }
-        return;
-    }
-    if (i >= D.ni || j >= D.nj) return;
-    const size_t p = D.off(i, j);
-    const size_t ld = D.ld;
-    const T d = A.diag[p];
-    T s = b[p] + A.cE[p - 1] * x[p - 1] + A.cE[p] * x[p + 1] + A.cN[p - ld] * x[p - ld] + A.cN[p] * x[p + ld] +
-          A.cNE[p] * x[p + ld + 1] + A.cNE[p - ld - 1] * x[p - ld - 1] + A.cNW[p] * x[p + ld - 1] +
-          A.cNW[p - ld + 1] * x[p - ld + 1];
-    x[p] = d > T(0) ? s / d : T(0);
-}
-
-// r = b - A x
-template <typename T, typename TB, bool NINE>
-__global__ void __launch_bounds__(256) k_resid(Dims D, OpV<T> A, const TB* __restrict__ b, const T* __restrict__ x,
-                                               T* __restrict__ r) {
-    const int i = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    if (i >= D.ni || j >= D.nj) return;
-    const size_t p = D.off(i, j);
-    const size_t ld = D.ld;
-    T s = (T)b[p] - A.diag[p] * x[p] + A.cE[p - 1] * x[p - 1] + A.cE[p] * x[p + 1] + A.cN[p - ld] * x[p - ld] +
-          A.cN[p] * x[p + ld];
-    if (NINE) {
-        s += A.cNE[p] * x[p + ld + 1] + A.cNE[p - ld - 1] * x[p - ld - 1] + A.cNW[p] * x[p + ld - 1] +
-             A.cNW[p - ld + 1] * x[p - ld + 1];
-    }
-    r[p] = s;
-}
-
-// coarse b = P^T r
-template <typename T>
-__global__ void __launch_bounds__(256) k_restrict(Dims F, Dims C, Prol<T> P, const T* __restrict__ r, T* __restrict__ bc) {
-    const int I = blockIdx.x * TX + threadIdx.x;
-    const int J = blockIdx.y * TY + threadIdx.y;
-    if (I >= C.ni || J >= C.nj) return;
-    const int fi = 2 * I, fj = 2 * J;
-    const size_t ld = F.ld;
-    const size_t f = F.off(fi, fj);
-    T* const* w = P.w;
-    const size_t c = C.off(I, J), cw = C.off(I - 1, J), cs = C.off(I, J - 1), csw = C.off(I - 1, J - 1);
-    T s = w[W_INJ][c] * r[f];
-    s += w[W_XW][c] * r[f + 1] + w[W_XE][cw] * r[f - 1];
-    s += w[W_YS][c] * r[f + ld] + w[W_YN][cs] * r[f - ld];
-    s += w[W_SW][c] * r[f + ld + 1] + w[W_SE][cw] * r[f + ld - 1] + w[W_NW][cs] * r[f - ld + 1] +
-         w[W_NE][csw] * r[f - ld - 1];
-    bc[c] = s;
-}
-
-// x += P e (fine cells)
-template <typename T>
-__global__ void __launch_bounds__(256) k_prolong(Dims F, Dims C, Prol<T> P, const T* __restrict__ e, T* __restrict__ x) {
-    const int i = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    if (i >= F.ni || j >= F.nj) return;
-    T* const* w = P.w;
-    const int pi = i & 1, pj = j & 1;
-    const int I = i >> 1, J = j >> 1;
-    const size_t c = C.off(I, J);
-    T v;
-    if (!pi && !pj) {
-        v = w[W_INJ][c] * e[c];
-    } else if (pi && !pj) {
-        v = w[W_XW][c] * e[c] + w[W_XE][c] * e[c + 1];
-    } else if (!pi && pj) {
-        v = w[W_YS][c] * e[c] + w[W_YN][c] * e[c + C.ld];
-    } else {
-        v = w[W_SW][c] * e[c] + w[W_SE][c] * e[c + 1] + w[W_NW][c] * e[c + C.ld] + w[W_NE][c] * e[c + C.ld + 1];
-    }
-    x[F.off(i, j)] += v;
-}
-
-template <typename T, typename TB>
-__global__ void __launch_bounds__(256) k_coarse_solve(Dims C, const T* __restrict__ inv, const TB* __restrict__ b,
-                                                      T* __restrict__ x) {
-    const int n = C.ni * C.nj;
-    const int r = threadIdx.x;
-    if (r >= n) return;
-    T s = T(0);
-    for (int q = 0; q < n; ++q) s += inv[(size_t)r * n + q] * (T)b[C.off(q % C.ni, q / C.ni)];
-    x[C.off(r % C.ni, r / C.ni)] = s;
-}
-
-template <typename TD, typename TS>
-__global__ void __launch_bounds__(256) k_convert(Dims D, const TS* __restrict__ a, TD* __restrict__ b) {
-    const int i = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    if (i >= D.ni || j >= D.nj) return;
-    const size_t p = D.off(i, j);
-    b[p] = (TD)a[p];
-}
-
-template <typename T>
-OpV<T> opv(const Op<T>& o) {
-    return OpV<T>{o.diag, o.cE, o.cN, o.cNE, o.cNW};
-}
-
-inline dim3 grid_half(int ni, int nj) { return dim3(((ni + 1) / 2 + TX - 1) / TX, (nj + TY - 1) / TY); }
-inline dim3 grid_quarter(int ni, int nj) { return dim3(((ni + 1) / 2 + TX - 1) / TX, ((nj + 1) / 2 + TY - 1) / TY); }
-
-template <typename T>
-struct LvAccess;
-template <>
-struct LvAccess<double> {
-    static Op<double>& op(Level& L) { return L.A; }
-    static Prol<double>& pr(Level& L) { return L.P; }
-    static LevelVecs<double>& vec(Level& L) { return L.v; }
-    static double* cinv(Ctx* c) { return c->coarse_inv; }
-};
-template <>
-struct LvAccess<float> {
-    static Op<float>& op(Level& L) { return L.Af; }
-    static Prol<float>& pr(Level& L) { return L.Pf; }
-    static LevelVecs<float>& vec(Level& L) { return L.vf; }
-    static float* cinv(Ctx* c) { return c->coarse_inv_f; }
-};
-
-// smoothing sweeps on level l; FIRST_ZERO: x starts at zero
-template <typename T, typename TB>
-void smooth(Ctx* c, int l, const TB* b, T* x, int sweeps, bool first_zero, bool reverse) {
-    Level& L = c->lev[l];
-    const Dims D = L.d;
-    OpV<T> A = opv(LvAccess<T>::op(L));
-    hipStream_t s = c->stream;
-    for (int sw = 0; sw < sweeps; ++sw) {
-        const bool z = first_zero && sw == 0;
-        if (l == 0) {
-            const int c0 = reverse ? 1 : 0, c1 = reverse ? 0 : 1;
-            if (z)
-                hipLaunchKernelGGL((k_rb5<T, TB, true>), grid_half(D.ni, D.nj), block2d(), 0, s, D, A, b, x, c0);
-            else
-                hipLaunchKernelGGL((k_rb5<T, TB, false>), grid_half(D.ni, D.nj), block2d(), 0, s, D, A, b, x, c0);
-            hipLaunchKernelGGL((k_rb5<T, TB, false>), grid_half(D.ni, D.nj), block2d(), 0, s, D, A, b, x, c1);
-        } else {
-            for (int k = 0; k < 4; ++k) {
-                const int col = reverse ? 3 - k : k;
-                if (z && k == 0)
-                    hipLaunchKernelGGL((k_gs9<T, true>), grid_quarter(D.ni, D.nj), block2d(), 0, s, D, A,
-                                       (const T*)b, x, col);
-                else
-                    hipLaunchKernelGGL((k_gs9<T, false>), grid_quarter(D.ni, D.nj), block2d(), 0, s, D, A,
-                                       (const T*)b, x, col);
-            }
-        }
-    }
-}
-
-template <typename T, typename TB>
-int vcycle_level(Ctx* c, int l, const TB* b, T* x) {
-    Level& L = c->lev[l];
-    hipStream_t s = c->stream;
-    if (l == c->nlev - 1) {
-        hipLaunchKernelGGL((k_coarse_solve<T, TB>), dim3(1), dim3(256), 0, s, L.d, LvAccess<T>::cinv(c), b, x);
-        SG_HIP(hipGetLastError());
-        return SG_OK;
-    }
-    Level& Lc = c->lev[l + 1];
-    const int pre = c->prm.mg_pre > 0 ? c->prm.mg_pre : 1;
-    const int post = c->prm.mg_post > 0 ? c->prm.mg_post : 1;
-    smooth<T, TB>(c, l, b, x, pre, true, false);
-    T* r = LvAccess<T>::vec(L).r;
-    OpV<T> A = opv(LvAccess<T>::op(L));
-    if (l == 0)
-        hipLaunchKernelGGL((k_resid<T, TB, false>), grid2d(L.d.ni, L.d.nj), block2d(), 0, s, L.d, A, b, x, r);
-    else
-        hipLaunchKernelGGL((k_resid<T, TB, true>), grid2d(L.d.ni, L.d.nj), block2d(), 0, s, L.d, A, b, x, r);
-    LevelVecs<T>& vc = LvAccess<T>::vec(Lc);
-    hipLaunchKernelGGL((k_restrict<T>), grid2d(Lc.d.ni, Lc.d.nj), block2d(), 0, s, L.d, Lc.d, LvAccess<T>::pr(L),
-                       (const T*)r, vc.b);
-    int rc = vcycle_level<T, T>(c, l + 1, vc.b, vc.x);
-    if (rc) return rc;
-    hipLaunchKernelGGL((k_prolong<T>), grid2d(L.d.ni, L.d.nj), block2d(), 0, s, L.d, Lc.d, LvAccess<T>::pr(L),
-                       (const T*)vc.x, x);
-    smooth<T, TB>(c, l, b, x, post, false, true);
-    SG_HIP(hipGetLastError());
-    return SG_OK;
-}
-
 }  // namespace
 
 int mg_allocate(Ctx* c) {
@@ -562,20 +281,6 @@ int mg_setup(Ctx* c) {
     SG_HIP(hipGetLastError());
     c->mg_ready = 1;
     return SG_OK;
-}
-
-// z = M^{-1} r on the fine level (padded FP64 vectors)
-int mg_vcycle(Ctx* c, const double* r, double* z) {
-    if (c->prm.precision == SG_PREC_MIXED) {
-        Level& L0 = c->lev[0];
-        int rc = vcycle_level<float, double>(c, 0, r, L0.vf.x);
-        if (rc) return rc;
-        hipLaunchKernelGGL((k_convert<double, float>), grid2d(L0.d.ni, L0.d.nj), block2d(), 0, c->stream, L0.d,
-                           (const float*)L0.vf.x, z);
-        SG_HIP(hipGetLastError());
-        return SG_OK;
-    }
-    return vcycle_level<double, double>(c, 0, r, z);
 }
 
 }  // namespace sg

@@ -217,12 +217,18 @@ int enqueue_iteration(Ctx* c, int parity) {
     const int nb = (int)(g.x * g.y);
     double* pold = parity ? c->p1 : c->p0;
     double* pnew = parity ? c->p0 : c->p1;
-    int rc = mg_vcycle(c, c->r, c->z);
-    if (rc) return rc;
     Slots S1 = slots(c, 1), S2 = slots(c, 2), S3 = slots(c, 3);
-    hipLaunchKernelGGL(k_dots_rz, g, b, 0, s, D, (const double*)c->r, (const double*)c->z, (const double*)c->q, S1,
-                       (const PcgScalars*)c->sc);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S1, nb, (int)FIN_RZ, c->sc, 0.0);
+    // the cycle's last kernel also leaves the partials of (r,z) and (z,q)
+    int dots_done = 0;
+    int rc = mg_vcycle_dots(c, c->r, c->z, c->q, S1.part, S1.max_blocks, &dots_done);
+    if (rc) return rc;
+    int nb_rz = mg_partial_blocks(c);
+    if (!dots_done) {
+        hipLaunchKernelGGL(k_dots_rz, g, b, 0, s, D, (const double*)c->r, (const double*)c->z,
+                           (const double*)c->q, S1, (const PcgScalars*)c->sc);
+        nb_rz = nb;
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S1, nb_rz, (int)FIN_RZ, c->sc, 0.0);
     hipLaunchKernelGGL(k_dir_apply, g, b, 0, s, D, L0.A.diag, L0.A.cE, L0.A.cN, (const double*)c->z,
                        (const double*)pold, pnew, c->q, S2, (const PcgScalars*)c->sc);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S2, nb, (int)FIN_PQ, c->sc, 0.0);
