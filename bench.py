@@ -66,7 +66,7 @@ def run_path(case, fields, precision, steps, warmup, tol):
     t1 = time.perf_counter()
     Pp = pj.download("PP")
     mres = pj.download("MRES")
-    kern = kernel_roofline(pj, case)
+    kern = kernel_roofline(pj, case, precision)
     pj.close()
     ms = (t1 - t0) * 1e3 / steps
     return {
@@ -82,24 +82,57 @@ def run_path(case, fields, precision, steps, warmup, tol):
     }
 
 
-def kernel_roofline(pj, case):
-    """Average duration of the fine-level operator apply (HIP events on the
-    library's stream) and its algorithmic bytes: x (read once), diag, cE, cN
-    (read) and y (write) -- 5 FP64 words per cell."""
+# Algorithmic bytes per fine cell of the fine-level up-sweep k_up5 (the
+# kernel with the largest share of a solve): reads b = PCG residual (FP64),
+# x after pre-smoothing, diag/cE/cN, the 9 prolongation weights and the
+# coarse correction (both at 1/4 resolution) and q (FP64, for the fused
+# dot); writes z (FP64).  w = word size of the multigrid data.
+def up_sweep_bytes_per_cell(w):
+    return 8 + w + 3 * w + 9 * w / 4 + w / 4 + 8 + 8
+
+
+def kernel_roofline(pj, case, precision):
+    """Average duration of one kernel, timed with HIP events on the
+    library's stream by sg_time_kernel, against its algorithmic bytes."""
     import ctypes
     from simcoflow_amd import _lib
     L = _lib.load()
-    if not hasattr(L, "sg_time_kernel"):
-        return None
     fn = L.sg_time_kernel
     fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     fn.restype = ctypes.c_int
+    out = {}
+    n = case.grid.ncells
+    w = 4 if precision == 1 else 8
+    for which, name, bpc in [(2, f"k_up5 (fine multigrid up-sweep + PCG dots, {'FP32' if w == 4 else 'FP64'})",
+                              up_sweep_bytes_per_cell(w)),
+                             (0, "k_apply5 (fine operator apply, FP64)", 5 * 8)]:
+        ms = ctypes.c_double(0.0)
+        if fn(pj.ctx, which, 100, ctypes.byref(ms)) != 0:
+            continue
+        b = bpc * n
+        out[which] = {"name": name, "avg_ms": ms.value, "bytes": b, "GBs": b / (ms.value * 1e-3) / 1e9}
+    # a plain copy of one padded fine-level FP64 vector: the calibration
+    # kernel for the FETCH_SIZE / WRITE_SIZE counters
+    ld = ((case.grid.ni + 1 + 31) // 32) * 32
+    ncopy = (case.grid.nj + 3) * ld
     ms = ctypes.c_double(0.0)
-    if fn(pj.ctx, 0, 200, ctypes.byref(ms)) != 0:
+    if fn(pj.ctx, 3, 20, ctypes.byref(ms)) == 0:
+        out[3] = {"name": "k_copy", "avg_ms": ms.value, "bytes": 16 * ncopy,
+                  "GBs": 16 * ncopy / (ms.value * 1e-3) / 1e9}
+    return out
+
+
+def measured_traffic(kernel_name):
+    """HBM bytes per launch of the roofline kernel from the committed rocprof
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel),
+    or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", "traffic_up_sweep_swarm2048.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("bytes_per_launch") if d.get("kernel", "") in kernel_name else None
+    except (OSError, ValueError):
         return None
-    bytes_alg = 5 * 8 * case.grid.ncells
-    gbs = bytes_alg / (ms.value * 1e-3) / 1e9
-    return {"name": "k_apply5 (fine operator apply, FP64)", "avg_ms": ms.value, "bytes": bytes_alg, "GBs": gbs}
 
 
 def cpu_baseline(case, fields, threads):
@@ -172,11 +205,16 @@ def main():
         "pressure_rel_l2_mixed_vs_fp64": err,
         "max_mass_residual": {"fp64": fp64["max_mres"], "mixed": mixed["max_mres"]},
     }
-    k = fp64["kernel"]
+    k = mixed["kernel"].get(2)
     if k:
         out["roofline"] = {"bound": "hbm", "achieved": round(k["GBs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                           "frac": round(k["GBs"] / PEAK_HBM_GBS, 4), "traffic": None, "kernel": k["name"],
-                           "avg_ms": k["avg_ms"]}
+                           "frac": round(k["GBs"] / PEAK_HBM_GBS, 4), "traffic": measured_traffic(k["name"]),
+                           "kernel": k["name"], "avg_ms": round(k["avg_ms"], 5),
+                           "bytes_alg": int(k["bytes"])}
+        out["kernels"] = {
+            "fp64_up_sweep": fp64["kernel"].get(2),
+            "fp64_apply": fp64["kernel"].get(0),
+        }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, fields, a.cpu_threads)
     print(json.dumps(out))

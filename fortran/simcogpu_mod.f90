@@ -1,0 +1,165 @@
+! iso_c_binding interface to libsimcogpu (include/simcogpu.h).
+!
+! A Fortran host (the reference's Solver_Mod / ProjectionP_Mod callers) uses
+! this module to call the MI355X projection path.  Arrays are passed as
+! c_loc of their first element; the layouts are the reference's own:
+! cell arrays (1:Isize,1:Jsize), ghosted arrays (0:Isize+1,0:Jsize+1).
+module simcogpu
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: SG_OK = 0, SG_NOT_CONVERGED = 1, SG_NAN = 2, SG_GUARD = 3
+  integer(c_int), parameter, public :: SG_PREC_FP64 = 0, SG_PREC_MIXED = 1
+  ! enum sg_field
+  integer(c_int), parameter, public :: SG_F_VOF = 0, SG_F_PHI = 1, SG_F_VOFS = 2, SG_F_NYS = 3, &
+       SG_F_WLLH = 4, SG_F_NXS = 5, SG_F_PHIS = 6, SG_F_NX = 7, SG_F_NY = 8, SG_F_MRES = 9, &
+       SG_F_VOFSU = 10, SG_F_VOFSV = 11, SG_F_U = 12, SG_F_V = 13, SG_F_P = 14, SG_F_UPRED = 15, &
+       SG_F_VPRED = 16, SG_F_PP = 17, SG_F_DPU = 18, SG_F_DPV = 19, SG_F_AE = 20, SG_F_AW = 21, &
+       SG_F_AN = 22, SG_F_AS = 23, SG_F_UFLAG = 24, SG_F_VFLAG = 25
+
+  type, bind(C), public :: sg_params
+    real(c_double) :: rho_l, rho_g, rho_ref
+    real(c_double) :: nu_l, nu_g, nu_ref, Re, Fr
+    integer(c_int) :: bc_w, bc_e, bc_s, bc_n
+    integer(c_int) :: precision
+    integer(c_int) :: max_iter
+    real(c_double) :: tol
+    integer(c_int) :: mg_pre, mg_post
+    integer(c_int) :: warm_start
+    integer(c_int) :: check_every
+  end type sg_params
+
+  type, bind(C), public :: sg_solve_info
+    integer(c_int) :: iterations
+    integer(c_int) :: status
+    real(c_double) :: relres
+    real(c_double) :: max_row_residual
+    real(c_double) :: ms
+    real(c_double) :: setup_ms
+  end type sg_solve_info
+
+  public :: sg_last_error, sg_params_default, sg_ctx_create, sg_ctx_destroy, sg_set_params
+  public :: sg_upload, sg_download, sg_synchronize, sg_set_geometry
+  public :: sg_poisson_build_resident, sg_poisson_solve_resident, sg_poisson_solve
+  public :: sg_poisson_matrix, sg_poisson_rhs, sg_poisson_apply, sg_velocity_correct_resident
+  public :: sg_error_message
+
+  interface
+    function sg_last_error() bind(C, name="sg_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    function sg_params_default(p) bind(C, name="sg_params_default") result(rc)
+      import :: c_int, sg_params
+      type(sg_params), intent(out) :: p
+      integer(c_int) :: rc
+    end function
+    function sg_ctx_create(ni, nj, dx, dy, udx, vdy, p, ctx) bind(C, name="sg_ctx_create") result(rc)
+      import :: c_int, c_ptr, sg_params
+      integer(c_int), value :: ni, nj
+      type(c_ptr), value :: dx, dy, udx, vdy
+      type(sg_params), intent(in) :: p
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: rc
+    end function
+    function sg_ctx_destroy(ctx) bind(C, name="sg_ctx_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function sg_set_params(ctx, p) bind(C, name="sg_set_params") result(rc)
+      import :: c_int, c_ptr, sg_params
+      type(c_ptr), value :: ctx
+      type(sg_params), intent(in) :: p
+      integer(c_int) :: rc
+    end function
+    function sg_upload(ctx, field, host) bind(C, name="sg_upload") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, host
+      integer(c_int), value :: field
+      integer(c_int) :: rc
+    end function
+    function sg_download(ctx, field, host) bind(C, name="sg_download") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, host
+      integer(c_int), value :: field
+      integer(c_int) :: rc
+    end function
+    function sg_synchronize(ctx) bind(C, name="sg_synchronize") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function sg_set_geometry(ctx, vofS, nyS, WlLh, aE, aW, aN, aS, uflag, vflag, vofSU, vofSV) &
+        bind(C, name="sg_set_geometry") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, vofS, nyS, WlLh, aE, aW, aN, aS, uflag, vflag, vofSU, vofSV
+      integer(c_int) :: rc
+    end function
+    function sg_poisson_build_resident(ctx, vb) bind(C, name="sg_poisson_build_resident") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: vb
+      integer(c_int) :: rc
+    end function
+    function sg_poisson_solve_resident(ctx, info) bind(C, name="sg_poisson_solve_resident") result(rc)
+      import :: c_int, c_ptr, sg_solve_info
+      type(c_ptr), value :: ctx
+      type(sg_solve_info), intent(out) :: info
+      integer(c_int) :: rc
+    end function
+    function sg_poisson_solve(ctx, u_star, v_star, DpU, DpV, vof, phi, vb, Pp, info) &
+        bind(C, name="sg_poisson_solve") result(rc)
+      import :: c_int, c_ptr, c_double, sg_solve_info
+      type(c_ptr), value :: ctx, u_star, v_star, DpU, DpV, vof, phi, Pp
+      real(c_double), value :: vb
+      type(sg_solve_info), intent(out) :: info
+      integer(c_int) :: rc
+    end function
+    function sg_poisson_matrix(ctx, matr) bind(C, name="sg_poisson_matrix") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, matr
+      integer(c_int) :: rc
+    end function
+    function sg_poisson_rhs(ctx, rhs) bind(C, name="sg_poisson_rhs") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, rhs
+      integer(c_int) :: rc
+    end function
+    function sg_poisson_apply(ctx, x, y) bind(C, name="sg_poisson_apply") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, x, y
+      integer(c_int) :: rc
+    end function
+    function sg_velocity_correct_resident(ctx, vb) bind(C, name="sg_velocity_correct_resident") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: vb
+      integer(c_int) :: rc
+    end function
+  end interface
+
+contains
+
+  ! the library's last error message as a Fortran string
+  function sg_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    p = sg_last_error()
+    if (.not. c_associated(p)) then
+      msg = ''
+      return
+    end if
+    call c_f_pointer(p, s, [4096])
+    n = 0
+    do while (s(n + 1) /= c_null_char .and. n < 4096)
+      n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function sg_error_message
+
+end module simcogpu

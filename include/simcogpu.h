@@ -130,9 +130,11 @@ int sg_velocity_correct_resident(sg_ctx* ctx, double vb);
 
 /* ---- measurement ----------------------------------------------------------
  * Average device time (HIP events on the context's stream) of `reps`
- * back-to-back launches of one kernel on the built operator: which = 0 is
- * the fine-level matrix-free apply y = A x (5 FP64 words per cell).  No
- * counterpart in the reference (it has no timers). */
+ * back-to-back launches of one kernel on the current data: which = 0 the
+ * fine-level matrix-free apply y = A x, 1 the fine-level multigrid
+ * down-sweep, 2 the fine-level up-sweep (with the fused PCG dots), 3 a plain
+ * copy of one padded fine-level vector (counter calibration).  1-3 need one
+ * solve first.  No counterpart in the reference (it has no timers). */
 int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms);
 
 #ifdef __cplusplus

@@ -62,7 +62,42 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FORTRAN_DIR = os.path.join(os.path.dirname(HERE), "fortran")
+FORTRAN_OUT = os.path.join(FORTRAN_DIR, "_build")
+FORTRAN_TEST = os.path.join(FORTRAN_OUT, "test_projection")
+
+
+def amdflang():
+    for cand in (os.environ.get("AMDFLANG"), "/opt/rocm/bin/amdflang", "/opt/rocm/lib/llvm/bin/amdflang"):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose=False):
+    """The Fortran host binding (iso_c_binding module, drop-in bodies) and
+    its test driver, linked against libsimcogpu.so; skipped when no Fortran
+    compiler is installed."""
+    fc = amdflang()
+    if fc is None:
+        if verbose:
+            print("amdflang not found: Fortran host binding not built")
+        return None
+    os.makedirs(FORTRAN_OUT, exist_ok=True)
+    srcs = [os.path.join(FORTRAN_DIR, f) for f in ("simcogpu_mod.f90", "sg_projection_dropin.f90",
+                                                   "test_projection.f90")]
+    cmd = [fc, "-O2", "-module-dir", FORTRAN_OUT] + srcs + [
+        "-L" + OUT_DIR, "-lsimcogpu", "-Wl,-rpath," + OUT_DIR, "-o", FORTRAN_TEST]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=FORTRAN_OUT)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stderr}")
+    if verbose:
+        print("built", FORTRAN_TEST)
+    return FORTRAN_TEST
 
 
 if __name__ == "__main__":

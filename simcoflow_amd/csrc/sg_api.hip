@@ -377,16 +377,22 @@ int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms) {
         set_error("sg_time_kernel: build the operator first");
         return SG_ERR_STATE;
     }
-    if (which != 0) {
+    if (which < 0 || which > 3) {
         set_error("sg_time_kernel: unknown kernel id");
         return SG_ERR_ARG;
     }
-    // fine operator apply y = A x on the PCG vectors x -> level-0 residual
-    int rc = poisson_apply_fine(c, c->x, c->lev[0].v.r);
+    if (which > 0 && !c->mg_ready) {
+        set_error("sg_time_kernel: solve once before timing the cycle kernels");
+        return SG_ERR_STATE;
+    }
+    // 0: fine operator apply y = A x (PCG x -> level-0 residual buffer);
+    // 1/2: fine down/up sweep of the V-cycle; 3: plain copy (calibration)
+    auto launch = [&]() { return which == 0 ? poisson_apply_fine(c, c->x, c->lev[0].v.r) : mg_launch_fine(c, which); };
+    int rc = launch();
     if (rc) return rc;
     SG_HIP(hipEventRecord(c->ev0, c->stream));
     for (int k = 0; k < reps; ++k) {
-        rc = poisson_apply_fine(c, c->x, c->lev[0].v.r);
+        rc = launch();
         if (rc) return rc;
     }
     SG_HIP(hipEventRecord(c->ev1, c->stream));

@@ -343,31 +343,28 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
         sx[lj][li] = xv;
     }
     __syncthreads();
-    // black cells on [-2,GW+1) x [-2,GH+1); no black-black coupling on 5 points
-    T nv[NB];
+    // black cells on [-2,GW+1) x [-2,GH+1), one thread per black cell; the
+    // 5-point stencil couples black only to red, so the update is in place.
+    // Local parity: black <=> (li + lj) odd.
+    {
+        constexpr int PR = (RW - 1) / 2;  // black slots per row
+        constexpr int NBK = (PR * (RH - 2) + NTB - 1) / NTB;
 #pragma unroll
-    for (int n = 0; n < NB; ++n) {
-        const int k = tid + n * NTB;
-        nv[n] = T(0);
-        if (k >= (RW - 2) * (RH - 2)) continue;
-        const int li = k % (RW - 2) + 1, lj = k / (RW - 2) + 1;
-        const int gi = i0 + li - 3, gj = j0 + lj - 3;
-        T v = sx[lj][li];
-        if (((gi + gj) & 1) == 1 && in_dom(F, gi, gj)) {
+        for (int n = 0; n < NBK; ++n) {
+            const int k = tid + n * NTB;
+            if (k >= PR * (RH - 2)) continue;
+            const int lj = k / PR + 1;
+            const int li = 2 * (k % PR) + 1 + (lj & 1);
+            if (li >= RW - 1) continue;
+            const int gi = i0 + li - 3, gj = j0 + lj - 3;
+            if (!in_dom(F, gi, gj)) continue;
             const size_t p = F.off(gi, gj);
             const T d = A.diag[p];
             auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            v = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+            sx[lj][li] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
         }
-        nv[n] = v;
     }
-    __syncthreads();
-#pragma unroll
-    for (int n = 0; n < NB; ++n) {
-        const int k = tid + n * NTB;
-        if (k >= (RW - 2) * (RH - 2)) continue;
-        sx[k / (RW - 2) + 1][k % (RW - 2) + 1] = nv[n];
-    }
+    (void)NB;
     __syncthreads();
     constexpr int NRS = ((GW + 1) * (GH + 1) + NTB - 1) / NTB;
 #pragma unroll
@@ -411,32 +408,81 @@ __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __r
     const int i0 = blockIdx.x * GW, j0 = blockIdx.y * GH;
     const int tid = threadIdx.x;
     const size_t ld = F.ld;
-    constexpr int NL = (RW * RH + NTB - 1) / NTB;
-    constexpr int NBK = ((RW - 2) * (RH - 2) + NTB - 1) / NTB;
+    // load b and x + P e per 2x2 block of cells sharing one coarse parent
+    // (the region starts on an even cell, so blocks align with coarse cells)
+    {
+        constexpr int BW = RW / 2, BH = RH / 2;
+        constexpr int NL2 = (BW * BH + NTB - 1) / NTB;
+        const int Ib = i0 / 2 - 1, Jb = j0 / 2 - 1;
 #pragma unroll
-    for (int m = 0; m < NL; ++m) {
-        const int k = tid + m * NTB;
-        if (k >= RW * RH) continue;
-        const int li = k % RW, lj = k / RW;
-        const int gi = i0 + li - 2, gj = j0 + lj - 2;
-        T bv = T(0), xv = T(0);
-        if (in_dom(F, gi, gj)) {
-            const size_t p = F.off(gi, gj);
-            bv = (T)b[p];
-            xv = xin[p] + prolong_point<T>(C, P, e, gi, gj);
+        for (int m = 0; m < NL2; ++m) {
+            const int k = tid + m * NTB;
+            if (k >= BW * BH) continue;
+            const int a = k % BW, c = k / BW;
+            const int I = Ib + a, J = Jb + c;
+            T xv[2][2] = {{T(0), T(0)}, {T(0), T(0)}}, bv[2][2] = {{T(0), T(0)}, {T(0), T(0)}};
+            if (I >= 0 && J >= 0 && I < C.ni && J < C.nj) {
+                const size_t cc = C.off(I, J);
+                T* const* w = P.w;
+                const T e00 = e[cc], e10 = e[cc + 1], e01 = e[cc + C.ld], e11 = e[cc + C.ld + 1];
+                const T pe[2][2] = {
+                    {w[W_INJ][cc] * e00, w[W_XW][cc] * e00 + w[W_XE][cc] * e10},
+                    {w[W_YS][cc] * e00 + w[W_YN][cc] * e01,
+                     w[W_SW][cc] * e00 + w[W_SE][cc] * e10 + w[W_NW][cc] * e01 + w[W_NE][cc] * e11}};
+#pragma unroll
+                for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                    for (int di = 0; di < 2; ++di) {
+                        const int gi = 2 * I + di, gj = 2 * J + dj;
+                        if (gi < F.ni && gj < F.nj) {
+                            const size_t p = F.off(gi, gj);
+                            bv[dj][di] = (T)b[p];
+                            xv[dj][di] = xin[p] + pe[dj][di];
+                        }
+                    }
+            }
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    sb[2 * c + dj][2 * a + di] = bv[dj][di];
+                    sx[2 * c + dj][2 * a + di] = xv[dj][di];
+                }
         }
-        sb[lj][li] = bv;
-        sx[lj][li] = xv;
     }
     __syncthreads();
-    // black on [-1,GW+1) x [-1,GH+1) in place (no black-black coupling)
+    // black on [-1,GW+1) x [-1,GH+1), one thread per black cell, in place
+    // (no black-black coupling).  Local parity: black <=> (li + lj) odd.
+    {
+        constexpr int PR = (RW - 2) / 2;  // black cells per row
+        constexpr int NBK = (PR * (RH - 2) + NTB - 1) / NTB;
 #pragma unroll
-    for (int m = 0; m < NBK; ++m) {
-        const int k = tid + m * NTB;
-        if (k >= (RW - 2) * (RH - 2)) continue;
-        const int li = k % (RW - 2) + 1, lj = k / (RW - 2) + 1;
-        const int gi = i0 + li - 2, gj = j0 + lj - 2;
-        if (((gi + gj) & 1) == 1 && in_dom(F, gi, gj)) {
+        for (int m = 0; m < NBK; ++m) {
+            const int k = tid + m * NTB;
+            if (k >= PR * (RH - 2)) continue;
+            const int lj = k / PR + 1;
+            const int li = 2 * (k % PR) + 1 + (lj & 1);
+            const int gi = i0 + li - 2, gj = j0 + lj - 2;
+            if (!in_dom(F, gi, gj)) continue;
+            const size_t p = F.off(gi, gj);
+            const T d = A.diag[p];
+            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
+            sx[lj][li] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+        }
+    }
+    __syncthreads();
+    // red on the tile, in place
+    {
+        constexpr int PR = GW / 2;
+        constexpr int NRD = (PR * GH + NTB - 1) / NTB;
+#pragma unroll
+        for (int m = 0; m < NRD; ++m) {
+            const int k = tid + m * NTB;
+            if (k >= PR * GH) continue;
+            const int lj = k / PR + 2;
+            const int li = 2 * (k % PR) + 2 + (lj & 1);
+            const int gi = i0 + li - 2, gj = j0 + lj - 2;
+            if (!in_dom(F, gi, gj)) continue;
             const size_t p = F.off(gi, gj);
             const T d = A.diag[p];
             auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
@@ -453,13 +499,7 @@ __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __r
         const int gi = i0 + li - 2, gj = j0 + lj - 2;
         if (!in_dom(F, gi, gj)) continue;
         const size_t p = F.off(gi, gj);
-        T v = sx[lj][li];
-        if (((gi + gj) & 1) == 0) {
-            const T d = A.diag[p];
-            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            v = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
-        }
-        const double zv = (double)v;
+        const double zv = (double)sx[lj][li];
         z[p] = zv;
         v0 += b[p] * zv;
         v1 += zv * q[p];
@@ -786,6 +826,51 @@ int mg_vcycle_dots(Ctx* c, const double* r, double* z, const double* q, double* 
     Slots S{partials, max_blocks};
     if (c->prm.precision == SG_PREC_MIXED) return vcycle<float>(c, r, z, q, S, dots_done);
     return vcycle<double>(c, r, z, q, S, dots_done);
+}
+
+namespace {
+__global__ void __launch_bounds__(256) k_copy(size_t n, const double* __restrict__ a, double* __restrict__ b) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+}  // namespace
+
+// One launch of a fine-level kernel on the current data (measurement):
+// 1 = down-sweep, 2 = up-sweep, 3 = plain copy of the level-0 padded vector.
+int mg_launch_fine(Ctx* c, int which) {
+    hipStream_t s = c->stream;
+    const Level& L = c->lev[0];
+    if (c->nlev < 2) return SG_ERR_STATE;
+    const Level& Cl = c->lev[1];
+    Slots S{c->red.partials + (size_t)7 * NRED * c->red.max_blocks, c->red.max_blocks};
+    const bool mixed = c->prm.precision == SG_PREC_MIXED;
+    if (which == 1) {
+        if (mixed)
+            hipLaunchKernelGGL((k_down5<float, double>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->r,
+                               L.vf.x, Cl.d, L.Pf, Cl.vf.b);
+        else
+            hipLaunchKernelGGL((k_down5<double, double>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
+                               L.v.x, Cl.d, L.P, Cl.v.b);
+    } else if (which == 2) {
+        // the coarse correction sits in x when level 1 is the tail's first level, else in r
+        const bool tail1 = c->nlev == 2 || (size_t)Cl.d.ni * Cl.d.nj <= (size_t)TAIL_MAX;
+        if (mixed)
+            hipLaunchKernelGGL((k_up5<float>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->r,
+                               (const float*)L.vf.x, Cl.d, L.Pf, (const float*)(tail1 ? Cl.vf.x : Cl.vf.r), c->z,
+                               c->q, S);
+        else
+            hipLaunchKernelGGL((k_up5<double>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
+                               (const double*)L.v.x, Cl.d, L.P, (const double*)(tail1 ? Cl.v.x : Cl.v.r), c->z,
+                               c->q, S);
+    } else if (which == 3) {
+        const size_t n = L.d.alloc_elems();
+        hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, (const double*)L.v.b,
+                           L.v.r);
+    } else {
+        return SG_ERR_ARG;
+    }
+    SG_HIP(hipGetLastError());
+    return SG_OK;
 }
 
 int mg_partial_blocks(Ctx* c) {

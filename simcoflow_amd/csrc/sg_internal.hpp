@@ -177,6 +177,7 @@ int mg_setup(Ctx* c);
 int mg_vcycle_dots(Ctx* c, const double* r, double* z, const double* q, double* partials, int max_blocks,
                    int* dots_done);
 int mg_partial_blocks(Ctx* c);  // blocks that wrote partials in mg_vcycle_dots
+int mg_launch_fine(Ctx* c, int which);  // measurement: 1 down, 2 up, 3 copy
 // PCG (sg_pcg.hip)
 int pcg_solve(Ctx* c, sg_solve_info* info);
 int fine_residual_norms(Ctx* c, double* relres, double* maxrow);
