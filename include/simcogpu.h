@@ -128,6 +128,34 @@ int sg_poisson_apply(sg_ctx* ctx, const double* x, double* y);
  * in/out: U, V, P; out: MRES. */
 int sg_velocity_correct_resident(sg_ctx* ctx, double vb);
 
+/* ---- interface transport (replaces Coupled_LS_VOF, Clsvof_Mod.f90:730-959) --
+ * nsub sub-steps of alternating x/y PLIC volume sweeps (X_Sweep/Y_Sweep
+ * :987-1245) with level-set transport, reconstruction (Interface_Reconstruct
+ * :1250-1321) and re-distancing (Redistance :1525-1708), for solids at rest
+ * (the geometry of sg_set_geometry plus the solid line NXS/NYS/PHIS).
+ * Needs ni, nj >= 4. */
+typedef struct sg_clsvof_params {
+    int nsub; /* sub-steps per call (nv = 6, Clsvof_Mod.f90:754) */
+    /* west inflow of X_Sweep (:1057-1059): row j takes in liquid through its
+     * west face while y_j - dy_j/2 < depth + amp*sin(kw*xw - omega*(time +
+     * k*dt/nsub)) at sub-step k, xw the west edge of the grid
+     * (Depthw, Amp0, kw, omew of StateVariables.f90:24-25) */
+    double depth, amp, kw, omega;
+} sg_clsvof_params;
+
+int sg_clsvof_params_default(sg_clsvof_params* p);
+/* P-cell centres PGrid%x(:,1), PGrid%y(1,:) (default: faces from 0 by the
+ * spacings given to sg_ctx_create). */
+int sg_set_centres(sg_ctx* ctx, const double* xc, const double* yc);
+/* device fields in: U, V (TVar%u/v, masked by UFLAG/VFLAG as :778-797),
+ * geometry; in/out: VOF, PHI; out: NX, NY (PCell%nx/ny). */
+int sg_clsvof_advect_resident(sg_ctx* ctx, double dt, double time, const sg_clsvof_params* p);
+/* Drop-in body of Coupled_LS_VOF: host in/out.  vof/phi = PCell%vof/phi
+ * (in/out), nx/ny = PCell%nx/ny (out, may be NULL), u/v = TVar%u/v
+ * (ghosted), time = NondiT. */
+int sg_clsvof_advect(sg_ctx* ctx, double* vof, double* phi, double* nx, double* ny, const double* u,
+                     const double* v, double dt, double time, const sg_clsvof_params* p);
+
 /* ---- measurement ----------------------------------------------------------
  * Average device time (HIP events on the context's stream) of `reps`
  * back-to-back launches of one kernel on the current data: which = 0 the

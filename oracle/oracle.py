@@ -160,3 +160,148 @@ def matr_to_scipy(g, geo, matr):
     rows.append(idx[jj, ii]); cols.append(idx[jj, ii]); vals.append(np.ones(len(jj)))
     A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
     return A
+
+
+# ---------------------------------------------------------------------------
+# Coupled level-set / VOF transport (Clsvof_Mod.f90)
+class _ClsGeom(ctypes.Structure):
+    _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int),
+                ("dx", _D), ("dy", _D), ("xc", _D), ("yc", _D),
+                ("vofS", _D), ("nxS", _D), ("nyS", _D), ("phiS", _D),
+                ("aE", _D), ("aW", _D), ("aN", _D), ("aS", _D),
+                ("uflag", _I), ("vflag", _I)]
+
+
+class _Wave(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_double), ("amp", ctypes.c_double),
+                ("kw", ctypes.c_double), ("omega", ctypes.c_double)]
+
+
+_cls_declared = False
+
+
+def _cls_lib():
+    global _cls_declared
+    L = lib()
+    if not _cls_declared:
+        dbl = ctypes.c_double
+        G = ctypes.POINTER(_ClsGeom)
+        L.orc_frac.argtypes = [dbl] * 5
+        L.orc_frac.restype = dbl
+        L.orc_plic_distance.argtypes = [dbl] * 5
+        L.orc_plic_distance.restype = dbl
+        L.orc_cellgeocal.argtypes = [dbl] * 8 + [_D, _D]
+        L.orc_distance_fluid.argtypes = [dbl] * 8 + [_D]
+        L.orc_distance_fluid.restype = ctypes.c_int
+        L.orc_normal_irre.argtypes = [G, _D, _D, ctypes.c_int, ctypes.c_int, _D, _D]
+        L.orc_reconstruct.argtypes = [G, _D, _D, _D, _D, _D]
+        L.orc_masked_velocities.argtypes = [G, _D, _D, _D, _D]
+        L.orc_x_sweep.argtypes = [G, _D, _D, _D, _D, _D, _D, dbl, dbl, _D, _D]
+        L.orc_y_sweep.argtypes = [G, _D, _D, _D, _D, _D, _D, dbl, _D, _D]
+        L.orc_boundary_copy.argtypes = [ctypes.c_int, ctypes.c_int, _D]
+        L.orc_redistance.argtypes = [G, _D, _D, _D, _D, _D]
+        L.orc_inflow_level.argtypes = [G, ctypes.POINTER(_Wave), dbl, ctypes.c_int, dbl]
+        L.orc_inflow_level.restype = dbl
+        L.orc_clsvof_advect.argtypes = [G, _D, _D, _D, _D, _D, _D, dbl, dbl, ctypes.c_int,
+                                        ctypes.POINTER(_Wave)]
+        _cls_declared = True
+    return L
+
+
+class ClsGeom:
+    """Grid + static solid geometry for the transport oracle (keeps the
+    numpy arrays alive for the C struct)."""
+
+    def __init__(self, g, geo):
+        c = np.ascontiguousarray
+        self.ni, self.nj = g.ni, g.nj
+        self._keep = dict(dx=c(g.dx, dtype=np.float64), dy=c(g.dy, dtype=np.float64),
+                          xc=c(g.xc, dtype=np.float64), yc=c(g.yc, dtype=np.float64),
+                          vofS=c(geo.vofS), nxS=c(geo.nxS), nyS=c(geo.nyS), phiS=c(geo.phiS),
+                          aE=c(geo.aE), aW=c(geo.aW), aN=c(geo.aN), aS=c(geo.aS),
+                          uflag=c(geo.uflag, dtype=np.int32), vflag=c(geo.vflag, dtype=np.int32))
+        k = self._keep
+        self.s = _ClsGeom(g.ni, g.nj, _d(k["dx"]), _d(k["dy"]), _d(k["xc"]), _d(k["yc"]),
+                          _d(k["vofS"]), _d(k["nxS"]), _d(k["nyS"]), _d(k["phiS"]),
+                          _d(k["aE"]), _d(k["aW"]), _d(k["aN"]), _d(k["aS"]),
+                          _i(k["uflag"]), _i(k["vflag"]))
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.s)
+
+
+def frac1(nx, ny, d, dx, dy):
+    return _cls_lib().orc_frac(nx, ny, d, dx, dy)
+
+
+def plic_distance(vf, nx, ny, dx, dy):
+    return _cls_lib().orc_plic_distance(vf, nx, ny, dx, dy)
+
+
+def cellgeocal(nxs, nys, phis, nxl, nyl, phil, dx, dy):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    _cls_lib().orc_cellgeocal(nxs, nys, phis, nxl, nyl, phil, dx, dy, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def distance_fluid(nxs, nys, phis, nxf, nyf, volf, dx, dy, init=0.0):
+    out = ctypes.c_double(init)
+    ok = _cls_lib().orc_distance_fluid(nxs, nys, phis, nxf, nyf, volf, dx, dy, ctypes.byref(out))
+    return out.value, bool(ok)
+
+
+def normal_irre(cg, phi, i, j):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    _cls_lib().orc_normal_irre(cg.ref, _d(phi), _d(cg._keep["vofS"]), i, j, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def reconstruct(cg, vof, phi):
+    nx = np.zeros((cg.nj, cg.ni)); ny = np.zeros_like(nx); dis = np.zeros_like(nx)
+    _cls_lib().orc_reconstruct(cg.ref, _d(np.ascontiguousarray(vof)), _d(np.ascontiguousarray(phi)),
+                               _d(nx), _d(ny), _d(dis))
+    return nx, ny, dis
+
+
+def masked_velocities(cg, u, v):
+    ue = np.zeros((cg.nj, cg.ni + 1)); ve = np.zeros((cg.nj + 1, cg.ni))
+    _cls_lib().orc_masked_velocities(cg.ref, _d(u), _d(v), _d(ue), _d(ve))
+    return ue, ve
+
+
+def x_sweep(cg, vof, phi, nx, ny, dis, ue, dtv, level, temvf, temls):
+    temvf = temvf.copy(); temls = temls.copy()
+    _cls_lib().orc_x_sweep(cg.ref, _d(vof), _d(phi), _d(nx), _d(ny), _d(dis), _d(ue), dtv, level,
+                           _d(temvf), _d(temls))
+    return temvf, temls
+
+
+def y_sweep(cg, vof, phi, nx, ny, dis, ve, dtv, temvf, temls):
+    temvf = temvf.copy(); temls = temls.copy()
+    _cls_lib().orc_y_sweep(cg.ref, _d(vof), _d(phi), _d(nx), _d(ny), _d(dis), _d(ve), dtv,
+                           _d(temvf), _d(temls))
+    return temvf, temls
+
+
+def boundary_copy(a):
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    _cls_lib().orc_boundary_copy(a.shape[1], a.shape[0], _d(a))
+    return a
+
+
+def redistance(cg, vof, nx, ny, dis, phi):
+    phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    _cls_lib().orc_redistance(cg.ref, _d(np.ascontiguousarray(vof)), _d(nx), _d(ny), _d(dis), _d(phi))
+    return phi
+
+
+def clsvof_advect(cg, vof, phi, u, v, dt, time=0.0, nsub=6, wave=(1.0e30, 0.0, 0.0, 0.0)):
+    """Coupled_LS_VOF: returns (vof, phi, nx, ny) after nsub sub-steps."""
+    vof = np.ascontiguousarray(vof, dtype=np.float64).copy()
+    phi = np.ascontiguousarray(phi, dtype=np.float64).copy()
+    nx = np.zeros_like(vof); ny = np.zeros_like(vof)
+    w = _Wave(*wave)
+    _cls_lib().orc_clsvof_advect(cg.ref, _d(vof), _d(phi), _d(nx), _d(ny), _d(np.ascontiguousarray(u)),
+                                 _d(np.ascontiguousarray(v)), dt, time, nsub, ctypes.byref(w))
+    return vof, phi, nx, ny

@@ -105,6 +105,52 @@ int orc_pcg_jacobi_solve(int ni, int nj, const double* matr, const double* b,
 int orc_num_threads(void);
 void orc_set_num_threads(int n);
 
+/* ---- coupled level-set / VOF transport (Clsvof_Mod.f90) ------------------ */
+
+/* Grid and static solid geometry seen by Coupled_LS_VOF: dx[ni], dy[nj]
+ * spacings and xc[ni], yc[nj] centres of the P cells (PGrid%dx/x), the
+ * P-cell solid fraction and solid line (vofS, nxS, nyS, phiS; cell arrays),
+ * the ghosted apertures E/W/N/S and the U/V face flags (cell arrays, 1 where
+ * Posnu/=-1 .and. MoExCell/=1). */
+typedef struct orc_cls_geom {
+    int ni, nj;
+    const double *dx, *dy, *xc, *yc;
+    const double *vofS, *nxS, *nyS, *phiS;
+    const double *aE, *aW, *aN, *aS;
+    const int *uflag, *vflag;
+} orc_cls_geom;
+
+/* West inflow wave of X_Sweep (:1057-1059): liquid enters row j while
+ * yc_j - dy_j/2 < depth + amp*sin(kw*xw - omega*t), xw the west edge. */
+typedef struct orc_wave {
+    double depth, amp, kw, omega;
+} orc_wave;
+
+double orc_frac(double nx, double ny, double diss, double dx, double dy);
+double orc_plic_distance(double vf, double nxx, double nyy, double dx, double dy);
+void orc_cellgeocal(double nxs, double nys, double phis, double nxl, double nyl, double phil,
+                    double dx, double dy, double* vols, double* volf);
+int orc_distance_fluid(double nxss, double nyss, double phiss, double nxf, double nyf, double volf,
+                       double dx, double dy, double* phif);
+void orc_normal_irre(const orc_cls_geom* g, const double* phi, const double* vofS, int i, int j,
+                     double* nx, double* ny);
+void orc_reconstruct(const orc_cls_geom* g, const double* vof, const double* phi, double* nx, double* ny,
+                     double* dis);
+void orc_masked_velocities(const orc_cls_geom* g, const double* u, const double* v, double* ue, double* ve);
+void orc_x_sweep(const orc_cls_geom* g, const double* vfl, const double* phi, const double* nx, const double* ny,
+                 const double* dis, const double* ue, double dtv, double inflow_level, double* temvf,
+                 double* temls);
+void orc_y_sweep(const orc_cls_geom* g, const double* vfl, const double* phi, const double* nx, const double* ny,
+                 const double* dis, const double* ve, double dtv, double* temvf, double* temls);
+void orc_boundary_copy(int ni, int nj, double* a);
+void orc_redistance(const orc_cls_geom* g, const double* vfl, const double* nxx1, const double* nyy1,
+                    const double* diss1, double* phi);
+double orc_inflow_level(const orc_cls_geom* g, const orc_wave* w, double time, int k, double dtv);
+/* nv sub-steps of Coupled_LS_VOF over dt; u, v ghosted TVar%u/v;
+ * vof, phi in/out; nx, ny out. */
+void orc_clsvof_advect(const orc_cls_geom* g, double* vof, double* phi, double* nx, double* ny,
+                       const double* u, const double* v, double dt, double time, int nv, const orc_wave* w);
+
 #ifdef __cplusplus
 }
 #endif

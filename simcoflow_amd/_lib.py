@@ -48,6 +48,11 @@ class SolveInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ClsvofParams(ctypes.Structure):
+    _fields_ = [("nsub", ctypes.c_int), ("depth", ctypes.c_double), ("amp", ctypes.c_double),
+                ("kw", ctypes.c_double), ("omega", ctypes.c_double)]
+
+
 # exported symbol -> (argtypes, restype); kept in step with include/simcogpu.h
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -73,6 +78,13 @@ PROTOTYPES = {
     "sg_poisson_rhs": ([_CTX, _D], ctypes.c_int),
     "sg_poisson_apply": ([_CTX, _D, _D], ctypes.c_int),
     "sg_velocity_correct_resident": ([_CTX, ctypes.c_double], ctypes.c_int),
+    "sg_clsvof_params_default": ([ctypes.POINTER(ClsvofParams)], ctypes.c_int),
+    "sg_set_centres": ([_CTX, _D, _D], ctypes.c_int),
+    "sg_clsvof_advect_resident": ([_CTX, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ClsvofParams)],
+                                  ctypes.c_int),
+    "sg_clsvof_advect": ([_CTX, _D, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double,
+                          ctypes.POINTER(ClsvofParams)], ctypes.c_int),
+    "sg_time_kernel": ([_CTX, ctypes.c_int, ctypes.c_int, _D], ctypes.c_int),
 }
 
 _lib = None

@@ -3,8 +3,9 @@
     python -m simcoflow_amd.build
 
 Every HIP source is compiled for gfx950 only, with packed-FP32 instruction
-selection disabled; the projection-coefficient file is compiled with FP
-contraction off so its arithmetic rounds exactly as written.
+selection disabled; the projection-coefficient, interface-transport and API
+files are compiled with FP contraction off so their arithmetic rounds
+exactly as written.
 """
 import os
 import subprocess
@@ -23,9 +24,12 @@ SOURCES = [
     "sg_mg.hip",
     "sg_pcg.hip",
     "sg_cycle.hip",
+    "sg_clsvof.hip",
 ]
 EXTRA = {
     "sg_poisson.hip": ["-ffp-contract=off"],
+    "sg_clsvof.hip": ["-ffp-contract=off"],
+    "sg_api.hip": ["-ffp-contract=off"],
 }
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -549,3 +549,55 @@ def predictor_fields(case: Case, seed=1, amp=0.1):
     DpV = case.dt / (1.0 + 0.05 * (1 + np.cos(2 * np.pi * (0.7 * X - Y))))
     return (np.ascontiguousarray(u), np.ascontiguousarray(v),
             np.ascontiguousarray(DpU), np.ascontiguousarray(DpV))
+
+
+# ---------------------------------------------------------------------------
+def transport_velocity(g: Grid, kind="rotation", amp=1.0, seed=0, walls=True):
+    """Face velocities TVar%u / TVar%v (ghosted, (nj+2, ni+2)) for the
+    interface transport.  u(i,j) sits on the east face of P cell (i,j),
+    v(i,j) on its north face (Mesh_Mod.f90:151,165).
+
+    kind: "rotation" solid-body rotation about the domain centre;
+          "uniform"  constant (amp, amp/2);
+          "vortex"   seeded sum of modes of a node stream function, so the
+                     discrete divergence (u_e-u_w)/dx + (v_n-v_s)/dy vanishes;
+          "random"   seeded white noise (not divergence-free).
+    walls: zero normal velocity on the four boundary faces."""
+    ni, nj = g.ni, g.nj
+    xf = g.xc[0] - g.dx[0] / 2 + np.concatenate([[0.0], np.cumsum(g.dx)])  # faces x_{i+1/2}, i=0..ni
+    yf = g.yc[0] - g.dy[0] / 2 + np.concatenate([[0.0], np.cumsum(g.dy)])
+    u = np.zeros((nj + 2, ni + 2))
+    v = np.zeros((nj + 2, ni + 2))
+    xm, ym = 0.5 * (xf[0] + xf[-1]), 0.5 * (yf[0] + yf[-1])
+    if kind == "rotation":
+        Yc, Xf = np.meshgrid(g.yc, xf, indexing="ij")       # (nj, ni+1)
+        u[1:nj + 1, 0:ni + 1] = -amp * (Yc - ym)
+        Yf, Xc = np.meshgrid(yf, g.xc, indexing="ij")       # (nj+1, ni)
+        v[0:nj + 1, 1:ni + 1] = amp * (Xc - xm)
+    elif kind == "uniform":
+        u[1:nj + 1, 0:ni + 1] = amp
+        v[0:nj + 1, 1:ni + 1] = 0.5 * amp
+    elif kind == "vortex":
+        rng = np.random.default_rng(seed)
+        Lx, Ly = xf[-1] - xf[0], yf[-1] - yf[0]
+        Yn, Xn = np.meshgrid(yf, xf, indexing="ij")         # nodes (nj+1, ni+1)
+        psi = np.zeros(Xn.shape)
+        for _ in range(4):
+            kx, ky = rng.integers(1, 4, size=2)
+            a = rng.uniform(-1, 1)
+            psi += a * np.sin(kx * np.pi * (Xn - xf[0]) / Lx) * np.sin(ky * np.pi * (Yn - yf[0]) / Ly)
+        psi *= amp * min(Lx, Ly) / np.pi
+        u[1:nj + 1, 0:ni + 1] = (psi[1:, :] - psi[:-1, :]) / g.dy[:, None]
+        v[0:nj + 1, 1:ni + 1] = -(psi[:, 1:] - psi[:, :-1]) / g.dx[None, :]
+    elif kind == "random":
+        rng = np.random.default_rng(seed)
+        u[:, :] = amp * rng.uniform(-1, 1, u.shape)
+        v[:, :] = amp * rng.uniform(-1, 1, v.shape)
+    else:
+        raise ValueError(kind)
+    if walls:
+        u[:, 0] = 0.0
+        u[:, ni] = 0.0
+        v[0, :] = 0.0
+        v[nj, :] = 0.0
+    return np.ascontiguousarray(u), np.ascontiguousarray(v)

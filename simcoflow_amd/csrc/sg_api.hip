@@ -1,6 +1,7 @@
 // C ABI of the simcogpu library: context, device fields, projection entry
 // points.  Every device buffer is allocated in sg_ctx_create; the solve and
 // correction entry points allocate nothing.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -159,6 +160,25 @@ int sg_ctx_create(int ni, int nj, const double* dx, const double* dy, const doub
     for (double** v : vecs) {
         *v = (double*)dev_alloc(c, n0 * sizeof(double));
         if (!*v) return fail_alloc();
+    }
+    if (clsvof_allocate(c) != SG_OK) return fail_alloc();
+    {
+        // default cell centres: faces from 0 by the spacings (Mesh_Mod.f90:123-135)
+        std::vector<double> xc(ni), yc(nj);
+        double f = 0.0;
+        for (int i = 0; i < ni; ++i) {
+            xc[i] = 0.5 * ((f + dx[i]) + f);
+            f = f + dx[i];
+        }
+        f = 0.0;
+        for (int j = 0; j < nj; ++j) {
+            yc[j] = 0.5 * ((f + dy[j]) + f);
+            f = f + dy[j];
+        }
+        CK(hipMemcpy(c->cls.xc, xc.data(), ni * sizeof(double), hipMemcpyHostToDevice));
+        CK(hipMemcpy(c->cls.yc, yc.data(), nj * sizeof(double), hipMemcpyHostToDevice));
+        c->cls.xw = xc[0] - dx[0] / 2.0;
+        c->cls.dx0 = dx[0];
     }
     // cell-layout staging for the host-pointer test hooks (5 planes)
     c->stage = (double*)dev_alloc(c, 5 * ncell * sizeof(double));
@@ -366,6 +386,77 @@ int sg_velocity_correct_resident(sg_ctx* ctx, double vb) {
     c->vb = vb;
     int rc = velocity_correct(c);
     if (rc) return rc;
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_clsvof_params_default(sg_clsvof_params* p) {
+    if (!p) return SG_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->nsub = 6;          // Clsvof_Mod.f90:754
+    p->depth = 1.0e300;   // inflow open on every row
+    return SG_OK;
+}
+
+int sg_set_centres(sg_ctx* ctx, const double* xc, const double* yc) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !xc || !yc) {
+        set_error("sg_set_centres: null argument");
+        return SG_ERR_ARG;
+    }
+    SG_HIP(hipMemcpyAsync(c->cls.xc, xc, c->ni * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(c->cls.yc, yc, c->nj * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    c->cls.xw = xc[0] - c->cls.dx0 / 2.0;
+    return SG_OK;
+}
+
+int sg_clsvof_advect_resident(sg_ctx* ctx, double dt, double time, const sg_clsvof_params* p) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !p) return SG_ERR_ARG;
+    if (c->ni < 4 || c->nj < 4) {
+        set_error("sg_clsvof_advect: the transport needs ni, nj >= 4");
+        return SG_ERR_ARG;
+    }
+    if (p->nsub < 1 || p->nsub > 1024) {
+        set_error("sg_clsvof_advect: nsub must be in [1, 1024]");
+        return SG_ERR_ARG;
+    }
+    if (!(dt >= 0.0)) {
+        set_error("sg_clsvof_advect: dt must be >= 0");
+        return SG_ERR_ARG;
+    }
+    // Depthw + etau of each sub-step (X_Sweep, Clsvof_Mod.f90:1057-1059)
+    std::vector<double> levels(p->nsub);
+    const double dtv = dt / (double)p->nsub;
+    for (int k = 1; k <= p->nsub; ++k) {
+        const double etau = p->amp * std::sin(p->kw * c->cls.xw - p->omega * (time + (double)k * dtv));
+        levels[k - 1] = p->depth + etau;
+    }
+    int rc = clsvof_advect(c, dt, p->nsub, levels.data());
+    if (rc) return rc;
+    SG_HIP(hipStreamSynchronize(c->stream));
+    c->built = 0;
+    return SG_OK;
+}
+
+int sg_clsvof_advect(sg_ctx* ctx, double* vof, double* phi, double* nx, double* ny, const double* u, const double* v,
+                     double dt, double time, const sg_clsvof_params* p) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !vof || !phi || !u || !v || !p) {
+        set_error("sg_clsvof_advect: null argument");
+        return SG_ERR_ARG;
+    }
+    const void* src[] = {vof, phi, u, v};
+    const int dst[] = {SG_F_VOF, SG_F_PHI, SG_F_U, SG_F_V};
+    for (int k = 0; k < 4; ++k)
+        SG_HIP(hipMemcpyAsync(c->field[dst[k]], src[k], c->field_bytes[dst[k]], hipMemcpyHostToDevice, c->stream));
+    int rc = sg_clsvof_advect_resident(ctx, dt, time, p);
+    if (rc) return rc;
+    SG_HIP(hipMemcpyAsync(vof, c->field[SG_F_VOF], c->field_bytes[SG_F_VOF], hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipMemcpyAsync(phi, c->field[SG_F_PHI], c->field_bytes[SG_F_PHI], hipMemcpyDeviceToHost, c->stream));
+    if (nx) SG_HIP(hipMemcpyAsync(nx, c->field[SG_F_NX], c->field_bytes[SG_F_NX], hipMemcpyDeviceToHost, c->stream));
+    if (ny) SG_HIP(hipMemcpyAsync(ny, c->field[SG_F_NY], c->field_bytes[SG_F_NY], hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
     return SG_OK;
 }

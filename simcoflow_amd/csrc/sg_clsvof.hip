@@ -1,0 +1,1138 @@
+// Coupled level-set / volume-of-fluid transport on gfx950 (Clsvof_Mod.f90).
+//
+// One call advances vof and phi over dt in nsub sub-steps (Coupled_LS_VOF,
+// Clsvof_Mod.f90:730-959).  Each sub-step is nine launches:
+//
+//   k_recon      PLIC reconstruction (Interface_Reconstruct :1250-1321)
+//   k_dis_fix    cells whose plane constant the reference leaves stale
+//   k_sweep      first sweep, implicit divergence (:798-823 / :862-888)
+//   k_bc         boundary copy (Boundary_Condition :961-973)
+//   k_recon, k_dis_fix
+//   k_sweep      second sweep, explicit divergence (:834-858 / :898-929)
+//   k_bc, k_recon (+ interface bit mask), k_dis_fix
+//   k_redistance level set from the PLIC segments (Redistance :1525-1708)
+//
+// The reference's sweeps walk each row in order and update cells in place;
+// every face flux they use depends only on data from before the sweep, so a
+// sweep here is one thread per cell that recomputes the fluxes of its two
+// faces and applies them in the reference's order.  Redistance scatters
+// from interface cells into a 9x9 window with order-dependent ties; here
+// each target gathers over its window in the reference's source order,
+// which gives the same result bit for bit.
+//
+// This file is compiled with FP contraction off: every sum and product
+// rounds as the reference writes it.
+#include "sg_internal.hpp"
+
+namespace sg {
+
+namespace {
+
+constexpr double VOFEPS = 1.0e-14;   // Clsvof_Mod.f90:10
+constexpr double TOLDENO = 1.0e-24;  // Clsvof_Mod.f90:10
+constexpr int BAND = 4;              // Clsvof_Mod.f90:9 band_width
+
+struct ClsGeo {
+    int ni, nj;
+    const double *dx, *dy, *xc, *yc;
+    const double *vofS, *nxS, *nyS, *phiS;
+    const double *aE, *aW, *aN, *aS;  // ghosted
+    const int *uflag, *vflag;
+    const double *u, *v;  // ghosted TVar%u, TVar%v
+};
+
+// 1-based cell index (the reference's (i,j)) -> cell array offset
+__device__ __forceinline__ size_t cix(const ClsGeo& G, int i, int j) {
+    return (size_t)(i - 1) + (size_t)(j - 1) * (size_t)G.ni;
+}
+__device__ __forceinline__ size_t gix(const ClsGeo& G, int i, int j) {
+    return (size_t)i + (size_t)j * (size_t)(G.ni + 2);
+}
+__device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+// ---- PLIC geometry --------------------------------------------------------
+
+// frac (:1371-1404): liquid fraction of a dx*dy box below |nx|x+|ny|y+d=0
+__device__ double plic_frac(double nx, double ny, double d, double dx, double dy) {
+    const double tnx = fabs(nx), tny = fabs(ny);
+    if (d + 0.5 * (tnx * dx + tny * dy) <= 0.0) return 1.0;
+    if (d - 0.5 * (tnx * dx + tny * dy) >= 0.0) return 0.0;
+    if (tnx <= 1.0e-20) return 0.5 - d / dy;
+    if (tny <= 1.0e-20) return 0.5 - d / dx;
+    const double xx = (0.5 * tny * dy - d) / dx / tnx + 0.5;
+    const double yy = (0.5 * tnx * dx - d) / dy / tny + 0.5;
+    const double area = 0.5 * xx * yy;
+    double top = 0.0, right = 0.0;
+    if (yy > 1.0) {
+        const double t = (yy - 1.0) / yy;
+        top = t * t;
+    }
+    if (xx > 1.0) {
+        const double t = (xx - 1.0) / xx;
+        right = t * t;
+    }
+    return area * (1.0 - top - right);
+}
+
+// distance (:1323-1369): plane constant for volume fraction vf
+__device__ double plic_distance(double vf, double nxx, double nyy, double dx, double dy) {
+    const double nx = fabs(nxx), ny = fabs(nyy);
+    if (nx < 1.0e-20) return (0.5 - vf) * dy;
+    if (ny < 1.0e-20) return (0.5 - vf) * dx;
+    const double dll = 0.5 * (nx * dx + ny * dy);
+    double dul = 0.5 * (nx * dx - ny * dy);
+    double dlr = -dul;
+    const double dur = -dll;
+    double vul = plic_frac(nx, ny, dul, dx, dy);
+    double vlr = plic_frac(nx, ny, dlr, dx, dy);
+    if (vf <= vul && vf <= vlr) {
+        if (vul > vlr) {
+            double t = vul; vul = vlr; vlr = t;
+            t = dul; dul = dlr; dlr = t;
+        }
+        const double e = dul - dll;
+        return dll - sqrt((e * e) * vf / vul);
+    }
+    if (vf >= vul && vf >= vlr) {
+        if (vlr < vul) {
+            double t = vul; vul = vlr; vlr = t;
+            t = dul; dul = dlr; dlr = t;
+        }
+        const double e = dlr - dur;
+        return dur + sqrt(e * e * (1.0 - vf) / (1.0 - vlr));
+    }
+    if (fabs(vlr - vul) < 1.0e-20) return 0.5 * (dul + dlr);
+    const double th = (vf - vul) / (vlr - vul);
+    return dlr * th + dul * (1.0 - th);
+}
+
+// Fluid part of a cell cut by the solid line, counter-clockwise from the SE
+// corner (CellGeoCal :2170-2219, DistanceFluidCalculate :1966-2015).
+// Returns the number of vertices (<= 5); nd[n] repeats nd[0].
+struct Poly {
+    double x[6], y[6];
+    int n;
+};
+
+__device__ void solid_poly(double nxs, double nys, double phis, double dx, double dy, Poly& P) {
+    const double dx2 = dx / 2.0, dy2 = dy / 2.0;
+    const double d0 = dx2 * nxs - dy2 * nys + phis;
+    const double d1 = -dx2 * nxs - dy2 * nys + phis;
+    const double d2 = -dx2 * nxs + dy2 * nys + phis;
+    const double d3 = dx2 * nxs + dy2 * nys + phis;
+    int n = 0;
+    for (int k = 0; k < 6; ++k) P.x[k] = P.y[k] = 0.0;
+    auto put = [&](double a, double b) {
+        if (n < 5) {
+            P.x[n] = a;
+            P.y[n] = b;
+            ++n;
+        }
+    };
+    if (d0 >= 0.0) put(dx2, -dy2);
+    if (d0 * d1 < 0.0) put((dy2 * nys - phis) / nxs, -dy2);
+    if (d1 >= 0.0) put(-dx2, -dy2);
+    if (d1 * d2 < 0.0) put(-dx2, (dx2 * nxs - phis) / nys);
+    if (d2 >= 0.0) put(-dx2, dy2);
+    if (d2 * d3 < 0.0) put((-dy2 * nys - phis) / nxs, dy2);
+    if (d3 >= 0.0) put(dx2, dy2);
+    if (d3 * d0 < 0.0) put(dx2, (-dx2 * nxs - phis) / nys);
+    P.x[n] = P.x[0];
+    P.y[n] = P.y[0];
+    P.n = n;
+}
+
+// signed shoelace area of the part of polygon P where pdt <= 0
+// (pdt[n] = pdt[0]); CellGeoCal :2227-2251
+__device__ double liquid_area(const Poly& P, const double* pdt) {
+    double lx[7], ly[7];
+    int m = 0;
+    for (int k = 0; k < P.n; ++k) {
+        if (pdt[k] <= 0.0) {
+            lx[m] = P.x[k];
+            ly[m] = P.y[k];
+            ++m;
+        }
+        if (pdt[k] * pdt[k + 1] < 0.0) {
+            const double a = fabs(pdt[k]), b = fabs(pdt[k + 1]);
+            lx[m] = P.x[k] + (P.x[k + 1] - P.x[k]) * a / (a + b);
+            ly[m] = P.y[k] + (P.y[k + 1] - P.y[k]) * a / (a + b);
+            ++m;
+        }
+    }
+    if (m == 0) return 0.0;
+    lx[m] = lx[0];
+    ly[m] = ly[0];
+    double vol = 0.0;
+    for (int k = 0; k < m; ++k) vol = vol + 0.5 * (lx[k] * ly[k + 1] - lx[k + 1] * ly[k]);
+    return vol;
+}
+
+// CellGeoCal (:2159-2253)
+__device__ void cell_geo(double nxs, double nys, double phis, double nxl, double nyl, double phil, double dx,
+                         double dy, double& vols, double& volf) {
+    Poly P;
+    solid_poly(nxs, nys, phis, dx, dy, P);
+    double pdt[7];
+    double vol = 0.0;
+    for (int k = 0; k < P.n; ++k) {
+        vol = vol + 0.5 * (P.x[k] * P.y[k + 1] - P.x[k + 1] * P.y[k]);
+        pdt[k] = P.x[k] * nxl + P.y[k] * nyl + phil;
+    }
+    vols = 1.0 - fabs(vol / (dx * dy));
+    pdt[P.n] = P.n > 0 ? pdt[0] : 0.0;
+    volf = fabs(liquid_area(P, pdt) / (dx * dy));
+}
+
+// LiqFuncVol (:2107-2155) on the fluid polygon
+__device__ double liq_func(double nxf, double nyf, double volf, double dx, double dy, const Poly& P, double xf,
+                           double yf) {
+    const double phil = -(nxf * xf + nyf * yf);
+    // the reference counts vertices as the last non-zero node
+    int nz = 0;
+    for (int k = 0; k < 6; ++k)
+        if (fabs(P.x[k]) > 1.0e-14 || fabs(P.y[k]) > 1.0e-14) nz = k + 1;
+    if (nz == 0) return volf;
+    Poly Q = P;
+    Q.n = nz - 1;
+    double pdt[7];
+    for (int k = 0; k < Q.n; ++k) pdt[k] = Q.x[k] * nxf + Q.y[k] * nyf + phil;
+    pdt[Q.n] = pdt[0];
+    return volf - fabs(liquid_area(Q, pdt) / (dx * dy));
+}
+
+// DistanceFluidCalculate + zriddr (:1957-2105).  Returns false when the
+// reference leaves the plane constant unassigned.
+__device__ bool distance_fluid(double nxs, double nys, double phis, double nxf, double nyf, double volf, double dx,
+                               double dy, double& phif) {
+    Poly P;
+    solid_poly(nxs, nys, phis, dx, dy, P);
+    const double diag = sqrt(dx * dx + dy * dy);
+    double maxd = -2.0 * diag, mind = 2.0 * diag;
+    double hx = 0.0, hy = 0.0, lx = 0.0, ly = 0.0;
+    for (int k = 0; k < P.n; ++k) {
+        const double t = nxf * P.x[k] + nyf * P.y[k];
+        if (t > maxd) {
+            maxd = t;
+            hx = P.x[k];
+            hy = P.y[k];
+        }
+        if (t < mind) {
+            mind = t;
+            lx = P.x[k];
+            ly = P.y[k];
+        }
+    }
+    hx = hx + hx * nxf * EPSI;
+    hy = hy + hy * nyf * EPSI;
+    lx = lx - lx * nxf * EPSI;
+    ly = ly - ly * nyf * EPSI;
+    // Ridders' method on the segment (lx,ly) -> (hx,hy), tol 1e-14, 1000 its
+    const double tol = 1.0e-14;
+    double fl = liq_func(nxf, nyf, volf, dx, dy, P, lx, ly);
+    double fh = liq_func(nxf, nyf, volf, dx, dy, P, hx, hy);
+    if ((fl > tol && fh < -tol) || (fl < -tol && fh > tol)) {
+        double xlo = lx, ylo = ly, xhi = hx, yhi = hy;
+        double ax = -1.11e-30, ay = -1.11e-30;
+        for (int it = 0; it < 1000; ++it) {
+            const double mx = 0.5 * (xlo + xhi), my = 0.5 * (ylo + yhi);
+            const double fm = liq_func(nxf, nyf, volf, dx, dy, P, mx, my);
+            const double s = sqrt(fm * fm - fl * fh);
+            const double sg = fsign(1.0, fl - fh);
+            const double nxw = mx + (mx - xlo) * sg * fm / s;
+            const double nyw = my + (my - ylo) * sg * fm / s;
+            const double e0 = nxw - ax, e1 = nyw - ay;
+            if (sqrt(e0 * e0 + e1 * e1) <= tol) {
+                phif = -(nxf * ax + nyf * ay);
+                return true;
+            }
+            ax = nxw;
+            ay = nyw;
+            const double fn = liq_func(nxf, nyf, volf, dx, dy, P, ax, ay);
+            if (fabs(fn) <= tol) {
+                phif = -(nxf * ax + nyf * ay);
+                return true;
+            }
+            if (fsign(fm, fn) != fm) {
+                xlo = mx; ylo = my; fl = fm;
+                xhi = ax; yhi = ay; fh = fn;
+            } else if (fsign(fl, fn) != fl) {
+                xhi = ax; yhi = ay; fh = fn;
+            } else if (fsign(fh, fn) != fh) {
+                xlo = ax; ylo = ay; fl = fn;
+            }
+            const double h0 = xhi - xlo, h1 = yhi - ylo;
+            if (sqrt(h0 * h0 + h1 * h1) <= tol) {
+                phif = -(nxf * ax + nyf * ay);
+                return true;
+            }
+        }
+        phif = -(nxf * ax + nyf * ay);
+        return true;
+    }
+    if (fl < tol && fl > -tol) {
+        phif = -(nxf * lx + nyf * ly);
+        return true;
+    }
+    if (fh < tol && fh > -tol) {
+        phif = -(nxf * hx + nyf * hy);
+        return true;
+    }
+    return false;
+}
+
+// ---- normals ----------------------------------------------------------------
+
+struct PhiView {
+    const ClsGeo& G;
+    const double* phi;
+    __device__ double operator()(int i, int j) const { return phi[cix(G, i, j)]; }
+    __device__ double X(int i) const { return G.xc[i - 1]; }
+    __device__ double Y(int j) const { return G.yc[j - 1]; }
+};
+
+__device__ __forceinline__ double qual(double vx, double vy) { return fabs(1.0 - sqrt(vx * vx + vy * vy)); }
+
+// The 8 candidate directions of Coupled_LS_VOF (:746-753) as the reference
+// leaves them: Vset(1,2) is overwritten with +1 and Vset(2,2) never set, so
+// direction 2 is (0,0) and drops out (its cosine is 0/0).
+__device__ __forceinline__ void vset(int k, double& a, double& b) {
+    const double A[8] = {0.0, 0.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0};
+    const double B[8] = {1.0, 0.0, 0.0, -1.0, 1.0, 0.0, -1.0, 1.0};
+    a = A[k];
+    b = B[k];
+}
+
+// one-sided pair chosen by the direction most parallel to grad phi
+// (Normal_Vector_Irre :1768-1827 / :1855-1905)
+__device__ void nv_directional(const PhiView& p, int i, int j, int imv0, double& nx, double& ny) {
+    const double eta = 0.075, nuy = eta / 8.0;
+    const double vx = (p(i + 1, j) - p(i - 1, j)) / (p.X(i + 1) - p.X(i - 1));
+    const double vy = (p(i, j + 1) - p(i, j - 1)) / (p.Y(j + 1) - p.Y(j - 1));
+    double maxv = 0.0;
+    int imv = imv0;
+    for (int k = 1; k <= 8; ++k) {
+        double a, b;
+        vset(k - 1, a, b);
+        const double cp = fabs(a * vx + b * vy) / (sqrt(a * a + b * b) * sqrt(vx * vx + vy * vy));
+        if (cp >= maxv) {
+            maxv = cp;
+            imv = k;
+        }
+    }
+    double v1x, v1y;
+    if (imv == 0) {  // zero gradient: outside Vset in the reference
+        v1x = -1.0;
+        v1y = 0.0;
+    } else {
+        double a, b;
+        vset(imv - 1, a, b);
+        v1x = -b;
+        v1y = a;
+    }
+    double vx1, vx2, vy1, vy2;
+    if (v1x == -1.0) {
+        vx1 = (p(i, j) - p(i - 1, j)) / (p.X(i) - p.X(i - 1));
+        vx2 = (p(i + 1, j) - p(i, j)) / (p.X(i + 1) - p.X(i));
+    } else if (v1x == 0.0) {
+        vx1 = (p(i + 1, j) - p(i - 1, j)) / (p.X(i + 1) - p.X(i - 1));
+        vx2 = vx1;
+    } else {
+        vx1 = (p(i + 1, j) - p(i, j)) / (p.X(i + 1) - p.X(i));
+        vx2 = (p(i, j) - p(i - 1, j)) / (p.X(i) - p.X(i - 1));
+    }
+    if (v1y == -1.0) {
+        vy1 = (p(i, j) - p(i, j - 1)) / (p.Y(j) - p.Y(j - 1));
+        vy2 = (p(i, j + 1) - p(i, j)) / (p.Y(j + 1) - p.Y(j));
+    } else if (v1y == 0.0) {
+        vy1 = (p(i, j + 1) - p(i, j - 1)) / (p.Y(j + 1) - p.Y(j - 1));
+        vy2 = vy1;
+    } else {
+        vy1 = (p(i, j + 1) - p(i, j)) / (p.Y(j + 1) - p.Y(j));
+        vy2 = (p(i, j) - p(i, j - 1)) / (p.Y(j) - p.Y(j - 1));
+    }
+    if (qual(vx1, vy1) < qual(vx2, vy2) + nuy) {
+        nx = vx1;
+        ny = vy1;
+    } else {
+        nx = vx2;
+        ny = vy2;
+    }
+}
+
+// Normal_Vector_Irre (:1710-1919), interior cells
+__device__ void normal_irre(const ClsGeo& G, const double* phi, int i, int j, double& nx, double& ny) {
+    const PhiView p{G, phi};
+    const double eta = 0.075;
+    const double vs = G.vofS[cix(G, i, j)];
+    if (vs >= EPSI) {
+        const double se = G.vofS[cix(G, i + 1, j)], sw = G.vofS[cix(G, i - 1, j)];
+        const double sn = G.vofS[cix(G, i, j + 1)], ss = G.vofS[cix(G, i, j - 1)];
+        if (se < 1.0 - EPSI && sw < 1.0 - EPSI)
+            nx = (p(i + 1, j) - p(i - 1, j)) / (p.X(i + 1) - p.X(i - 1));
+        else if (se >= 1.0 - EPSI)
+            nx = (p(i, j) - p(i - 1, j)) / (p.X(i) - p.X(i - 1));
+        else
+            nx = (p(i + 1, j) - p(i, j)) / (p.X(i + 1) - p.X(i));
+        if (sn < 1.0 - EPSI && ss < 1.0 - EPSI)
+            ny = (p(i, j + 1) - p(i, j - 1)) / (p.Y(j + 1) - p.Y(j - 1));
+        else if (sn > 1.0 - EPSI)
+            ny = (p(i, j) - p(i, j - 1)) / (p.Y(j) - p.Y(j - 1));
+        else
+            ny = (p(i, j + 1) - p(i, j)) / (p.Y(j + 1) - p.Y(j));
+        return;
+    }
+    const double cx = (p(i + 1, j) - p(i - 1, j)) / (p.X(i + 1) - p.X(i - 1));
+    const double cy = (p(i, j + 1) - p(i, j - 1)) / (p.Y(j + 1) - p.Y(j - 1));
+    const double q0 = qual(cx, cy);
+    double vx, vy;
+    vx = i >= 3 ? (p(i, j) - p(i - 2, j)) / (p.X(i) - p.X(i - 2)) : (p(i, j) - p(i - 1, j)) / (p.X(i) - p.X(i - 1));
+    vy = (p(i - 1, j + 1) - p(i - 1, j - 1)) / (p.Y(j + 1) - p.Y(j - 1));
+    const double qm = qual(vx, vy);
+    vx = i <= G.ni - 2 ? (p(i + 2, j) - p(i, j)) / (p.X(i + 2) - p.X(i))
+                       : (p(i + 1, j) - p(i, j)) / (p.X(i + 1) - p.X(i));
+    vy = (p(i + 1, j + 1) - p(i + 1, j - 1)) / (p.Y(j + 1) - p.Y(j - 1));
+    const double qp = qual(vx, vy);
+    int sx;
+    if (qm < eta && qp >= eta) sx = -1;
+    else if (qm >= eta && qp < eta) sx = 1;
+    else if (qm < eta && q0 < eta && qp < eta) sx = 0;
+    else if (qm >= eta && q0 >= eta && qp >= eta) sx = 0;
+    else {
+        nv_directional(p, i, j, 0, nx, ny);
+        return;
+    }
+    vy = j >= 3 ? (p(i, j) - p(i, j - 2)) / (p.Y(j) - p.Y(j - 2)) : (p(i, j) - p(i, j - 1)) / (p.Y(j) - p.Y(j - 1));
+    vx = (p(i + 1, j - 1) - p(i - 1, j - 1)) / (p.X(i + 1) - p.X(i - 1));
+    const double rm = qual(vx, vy);
+    vx = (p(i + 1, j + 1) - p(i - 1, j + 1)) / (p.X(i + 1) - p.X(i - 1));
+    vy = j <= G.nj - 2 ? (p(i, j + 2) - p(i, j)) / (p.Y(j + 2) - p.Y(j))
+                       : (p(i, j + 1) - p(i, j)) / (p.Y(j + 1) - p.Y(j));
+    const double rp = qual(vx, vy);
+    int sy;
+    if (rm < eta && rp >= eta) sy = -1;
+    else if (rm >= eta && rp < eta) sy = 1;
+    else if (rm < eta && q0 < eta && rp < eta) sy = 0;
+    else if (rm >= eta && q0 >= eta && rp >= eta) sy = 0;
+    else {
+        nv_directional(p, i, j, 1, nx, ny);
+        return;
+    }
+    if (sx == -1) nx = (p(i, j) - p(i - 1, j)) / (p.X(i) - p.X(i - 1));
+    else if (sx == 1) nx = (p(i + 1, j) - p(i, j)) / (p.X(i + 1) - p.X(i));
+    else nx = cx;
+    if (sy == -1) ny = (p(i, j) - p(i, j - 1)) / (p.Y(j) - p.Y(j - 1));
+    else if (sy == 1) ny = (p(i, j + 1) - p(i, j)) / (p.Y(j + 1) - p.Y(j));
+    else ny = cy;
+}
+
+// interface class of Interface_Reconstruct (:1258-1266): 0 none, 1 two
+// phases, 2 three phases
+__device__ __forceinline__ int recon_class(double vf, double vs) {
+    if (vf >= VOFEPS && vf < (1.0 - VOFEPS - vs)) {
+        if (vs < EPSI) return 1;
+        if (vs < 1.0 - EPSI) return 2;
+    }
+    return 0;
+}
+
+__device__ __forceinline__ void normalise(double& nx, double& ny) {
+    const double t = sqrt(nx * nx + ny * ny);
+    if (t < 1.0e-14) {
+        nx = 0.0;
+        ny = 1.0;
+    } else {
+        nx = nx / t;
+        ny = ny / t;
+    }
+}
+
+// one-sided differences the reference uses on the domain edge (:1272-1283)
+__device__ __forceinline__ double edge_nx(const PhiView& p, int i, int j, int ni) {
+    return i == 1 ? (p(2, j) - p(1, j)) / (p.X(2) - p.X(1)) : (p(ni, j) - p(ni - 1, j)) / (p.X(ni) - p.X(ni - 1));
+}
+__device__ __forceinline__ double edge_ny(const PhiView& p, int i, int j, int nj) {
+    return j == 1 ? (p(i, 2) - p(i, 1)) / (p.Y(2) - p.Y(1)) : (p(i, nj) - p(i, nj - 1)) / (p.Y(nj) - p.Y(nj - 1));
+}
+
+__device__ __forceinline__ int cls_of(const ClsGeo& G, const double* vof, int i, int j) {
+    const size_t c = cix(G, i, j);
+    return recon_class(vof[c], G.vofS[c]);
+}
+
+// final normal of an interior cell
+__device__ void final_interior(const ClsGeo& G, const double* vof, const double* phi, int i, int j, double& nx,
+                               double& ny) {
+    if (cls_of(G, vof, i, j)) {
+        normal_irre(G, phi, i, j, nx, ny);
+        normalise(nx, ny);
+    } else {
+        nx = 0.0;
+        ny = 1.0;
+    }
+}
+
+// final normal of a cell on the west or east edge (i = 1 or ni).  Away from
+// the corners the reference's y component is the previous cell's (i,j-1)
+// final value (a stale local, :1271-1283), so a run of interface cells up the
+// edge is a chain; it is walked from the cell below its start.
+__device__ void final_side(const ClsGeo& G, const double* vof, const double* phi, int i, int j, double& nx,
+                           double& ny) {
+    const PhiView p{G, phi};
+    const int nj = G.nj;
+    if (!cls_of(G, vof, i, j)) {
+        nx = 0.0;
+        ny = 1.0;
+        return;
+    }
+    if (j == 1 || j == nj) {
+        nx = edge_nx(p, i, j, G.ni);
+        ny = edge_ny(p, i, j, nj);
+        normalise(nx, ny);
+        return;
+    }
+    int js = j - 1;
+    while (js > 1 && cls_of(G, vof, i, js)) --js;
+    double nyp;
+    if (js == 1 && cls_of(G, vof, i, 1)) {
+        double a = edge_nx(p, i, 1, G.ni), b = edge_ny(p, i, 1, nj);
+        normalise(a, b);
+        nyp = b;
+    } else {
+        nyp = 1.0;
+    }
+    for (int k = js + 1; k <= j; ++k) {
+        double a = edge_nx(p, i, k, G.ni), b = nyp;
+        normalise(a, b);
+        nyp = b;
+        if (k == j) {
+            nx = a;
+            ny = b;
+        }
+    }
+}
+
+// final normal of a cell on the south or north edge (1 < i < ni).  Its x
+// component is the previous cell's in the reference's loop order: (i,nj-1)
+// for the north edge, (i-1,nj) for the south edge.
+__device__ void final_topbottom(const ClsGeo& G, const double* vof, const double* phi, int i, int j, double& nx,
+                                double& ny) {
+    const PhiView p{G, phi};
+    const int ni = G.ni, nj = G.nj;
+    if (!cls_of(G, vof, i, j)) {
+        nx = 0.0;
+        ny = 1.0;
+        return;
+    }
+    double snx, t;
+    if (j == nj) {
+        final_interior(G, vof, phi, i, nj - 1, snx, t);
+    } else {  // j == 1: previous cell is (i-1, nj)
+        const int ip = i - 1;
+        if (ip == 1) {
+            final_side(G, vof, phi, 1, nj, snx, t);
+        } else if (cls_of(G, vof, ip, nj)) {
+            double a, b;
+            final_interior(G, vof, phi, ip, nj - 1, a, b);
+            b = edge_ny(p, ip, nj, nj);
+            normalise(a, b);
+            snx = a;
+        } else {
+            snx = 0.0;
+        }
+    }
+    nx = snx;
+    ny = edge_ny(p, i, j, nj);
+    normalise(nx, ny);
+}
+
+// ---- kernels ------------------------------------------------------------------
+
+// Interface_Reconstruct.  stale[c] = 1 where DistanceFluidCalculate leaves the
+// plane constant unassigned (the reference then keeps the previous cell's).
+// With mask != null it also writes the redistance source mask (one bit per
+// cell: vofeps < vof < 1-vofeps-vofS), one 64-bit word per wavefront.
+__global__ void __launch_bounds__(256) k_recon(ClsGeo G, const double* __restrict__ vof,
+                                               const double* __restrict__ phi, double* __restrict__ nxo,
+                                               double* __restrict__ nyo, double* __restrict__ dis,
+                                               unsigned char* __restrict__ stale,
+                                               unsigned long long* __restrict__ mask, int mw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const bool in = i <= G.ni && j <= G.nj;
+    bool src = false;
+    if (in) {
+        const size_t c = cix(G, i, j);
+        const double vf = vof[c], vs = G.vofS[c];
+        src = (0.0 + VOFEPS) < vf && vf < (1.0 - VOFEPS - vs);
+        const int cl = recon_class(vf, vs);
+        double nx, ny, d = 0.0;
+        unsigned char st = 0;
+        if (cl) {
+            if (i > 1 && i < G.ni && j > 1 && j < G.nj)
+                final_interior(G, vof, phi, i, j, nx, ny);
+            else if (i == 1 || i == G.ni)
+                final_side(G, vof, phi, i, j, nx, ny);
+            else
+                final_topbottom(G, vof, phi, i, j, nx, ny);
+            const double hx = G.dx[i - 1], hy = G.dy[j - 1];
+            if (cl == 1)
+                d = plic_distance(vf, nx, ny, hx, hy);
+            else if (!distance_fluid(G.nxS[c], G.nyS[c], G.phiS[c], nx, ny, vf, hx, hy, d))
+                st = 1;
+        } else {
+            nx = 0.0;
+            ny = 1.0;
+            if (1.0 - vs > VOFEPS)
+                d = (0.5 - vf / (1.0 - vs)) * G.dy[j - 1];
+            else
+                d = (0.0 - vf) * G.dy[j - 1];
+        }
+        nxo[c] = nx;
+        nyo[c] = ny;
+        dis[c] = d;
+        stale[c] = st;
+    }
+    if (mask) {
+        // blockDim.x == 64: one wavefront covers 64 consecutive i of row j
+        const unsigned long long bits = __ballot(src);
+        if (threadIdx.x == 0 && j <= G.nj) mask[(size_t)(j - 1) * mw + blockIdx.x] = bits;
+    }
+}
+
+// Cells whose plane constant the reference leaves stale take the value of
+// the nearest earlier cell in its loop order (i outer, j inner) that has one.
+__global__ void __launch_bounds__(256) k_dis_fix(ClsGeo G, const unsigned char* __restrict__ stale,
+                                                 double* __restrict__ dis) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > G.ni || j > G.nj) return;
+    if (!stale[cix(G, i, j)]) return;
+    int pi = i, pj = j;
+    for (;;) {
+        if (pj > 1) {
+            --pj;
+        } else if (pi > 1) {
+            --pi;
+            pj = G.nj;
+        } else {
+            dis[cix(G, i, j)] = 0.0;
+            return;
+        }
+        if (!stale[cix(G, pi, pj)]) {
+            dis[cix(G, i, j)] = dis[cix(G, pi, pj)];
+            return;
+        }
+    }
+}
+
+// Boundary_Condition (:961-973) on two arrays: every ring cell takes its
+// value from an interior cell, so the copy is race-free in place.
+__global__ void k_bc(int ni, int nj, double* __restrict__ a, double* __restrict__ b) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, si, sj;
+    if (t < ni) {
+        i = t + 1; j = 1;
+    } else if (t < 2 * ni) {
+        i = t - ni + 1; j = nj;
+    } else if (t < 2 * ni + (nj - 2)) {
+        i = 1; j = t - 2 * ni + 2;
+    } else if (t < 2 * ni + 2 * (nj - 2)) {
+        i = ni; j = t - 2 * ni - (nj - 2) + 2;
+    } else {
+        return;
+    }
+    si = i == 1 ? 2 : (i == ni ? ni - 1 : i);
+    sj = j == 1 ? 2 : (j == nj ? nj - 1 : j);
+    const size_t d = (size_t)(i - 1) + (size_t)(j - 1) * ni, s = (size_t)(si - 1) + (size_t)(sj - 1) * ni;
+    a[d] = a[s];
+    b[d] = b[s];
+}
+
+// ---- sweeps -------------------------------------------------------------------
+// A sweep along x (X=true) walks i in row b=j; along y it walks j in column
+// b=i.  Index a is the position along the sweep, na its extent.
+template <bool X>
+struct Line {
+    const ClsGeo& G;
+    int b;
+    __device__ int na() const { return X ? G.ni : G.nj; }
+    __device__ size_t c(int a) const { return X ? cix(G, a, b) : cix(G, b, a); }
+    // masked face velocity ue(a,j) / ve(i,a) (Coupled_LS_VOF :778-797)
+    __device__ double vel(int a) const {
+        if (X) {
+            if (a == 0) return G.u[gix(G, 0, b)];
+            return G.uflag[cix(G, a, b)] ? G.u[gix(G, a, b)] : 0.0;
+        }
+        if (a == 0) return G.v[gix(G, b, 0)];
+        return G.vflag[cix(G, b, a)] ? G.v[gix(G, b, a)] : 0.0;
+    }
+    __device__ double ap(int a) const { return X ? G.aE[gix(G, a, b)] : G.aN[gix(G, b, a)]; }
+    __device__ double am(int a) const { return X ? G.aW[gix(G, a, b)] : G.aS[gix(G, b, a)]; }
+    __device__ double h(int a) const { return X ? G.dx[a - 1] : G.dy[a - 1]; }
+    __device__ double hc() const { return X ? G.dy[b - 1] : G.dx[b - 1]; }
+    __device__ double pos(int a) const { return X ? G.xc[a - 1] : G.yc[a - 1]; }
+};
+
+// rightflux/leftflux/topflux/bottomflux (:1416-1522): liquid volume swept
+// through the far (side=+1) or near (side=-1) face over a width w
+template <bool X>
+__device__ double plic_face_flux(int side, double nxx, double nyy, double diss, double volf, double nxs, double nys,
+                                 double phis, double vols, double hx, double hy, double w) {
+    if (w == 0.0) return 0.0;
+    const double nl = X ? nxx : nyy, ns = X ? nxs : nys, len = X ? hx : hy;
+    const double sh = 0.5 * (w - len);
+    const double dl = side > 0 ? diss - sh * nl : diss + sh * nl;
+    double flux;
+    if (vols < EPSI) {
+        flux = X ? plic_frac(nxx, nyy, dl, w, hy) : plic_frac(nxx, nyy, dl, hx, w);
+    } else if (vols >= 1.0 - EPSI) {
+        return volf;
+    } else {
+        const double ps = side > 0 ? phis - sh * ns : phis + sh * ns;
+        double volss;
+        if (X)
+            cell_geo(nxs, nys, ps, nxx, nyy, dl, w, hy, volss, flux);
+        else
+            cell_geo(nxs, nys, ps, nxx, nyy, dl, hx, w, volss, flux);
+        if (volf / (1.0 - vols) >= 1.0 - VOFEPS) flux = 1.0 - volss;
+        flux = flux / (1.0 - volss + TOLDENO);
+        if (volf + vols > 1.0 - 1.0e-14) flux = 1.0;
+    }
+    return flux * w;
+}
+
+struct SweepIn {
+    const double* vof;
+    const double* phi;
+    const double* nx;
+    const double* ny;
+    const double* dis;
+};
+
+// volume flux through face a (between cells a and a+1), X_Sweep :1014-1051
+template <bool X>
+__device__ double vof_flux(const Line<X>& L, const SweepIn& s, int a, double dtv) {
+    const ClsGeo& G = L.G;
+    const double w = L.vel(a);
+    const int n = L.na();
+    if (X ? (w >= 0.0) : (w > 0.0)) {
+        const size_t c = L.c(a);
+        const double f = s.vof[c];
+        double flux;
+        if (f >= (1.0 - VOFEPS) || f <= VOFEPS) {
+            flux = f * w * dtv;
+            if (f >= 1.0) flux = w * dtv;
+            if (f < 0.0) flux = 0.0;
+        } else {
+            const double hx = X ? G.dx[a - 1] : G.dx[L.b - 1], hy = X ? G.dy[L.b - 1] : G.dy[a - 1];
+            flux = plic_face_flux<X>(1, s.nx[c], s.ny[c], s.dis[c], f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c], hx,
+                                     hy, w * dtv);
+        }
+        if (L.ap(a) > 1.0 - VOFEPS && G.vofS[c] > 1.0 - VOFEPS) flux = 1.0 * w * dtv;
+        return flux;
+    }
+    if (a < n) {
+        const size_t c = L.c(a + 1);
+        const double f = s.vof[c];
+        double flux;
+        if (f >= (1.0 - VOFEPS) || f <= VOFEPS) {
+            flux = f * w * dtv;
+            if (f >= 1.0) flux = w * dtv;
+            if (f < 0.0) flux = 0.0;
+        } else {
+            const double hx = X ? G.dx[a] : G.dx[L.b - 1], hy = X ? G.dy[L.b - 1] : G.dy[a];
+            flux = -plic_face_flux<X>(-1, s.nx[c], s.ny[c], s.dis[c], f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c],
+                                      hx, hy, -w * dtv);
+        }
+        if (L.am(a + 1) > 1.0 - VOFEPS && G.vofS[c] > 1.0 - VOFEPS) flux = 1.0 * w * dtv;
+        return flux;
+    }
+    return s.vof[L.c(a)] * w * dtv;
+}
+
+// level-set flux through interior face a, 2 <= a <= na-1 (:1073-1096)
+template <bool X>
+__device__ double ls_flux(const Line<X>& L, const double* phi, int a, double dtv) {
+    const double w = L.vel(a);
+    double l;
+    if (w >= 0.0)
+        l = (phi[L.c(a + 1)] + phi[L.c(a - 1)]) / 2.0;
+    else if (a <= L.na() - 2)
+        l = (phi[L.c(a + 2)] + phi[L.c(a)]) / 2.0;
+    else
+        l = (phi[L.c(a + 1)] + phi[L.c(a - 1)]) / 2.0;
+    return w * l * dtv;
+}
+
+// level-set flux through face 1 (:1098-1108 / :1218-1227)
+template <bool X>
+__device__ double ls_flux_first(const Line<X>& L, const double* phi, double dtv) {
+    const double w = L.vel(1);
+    const double p1 = phi[L.c(1)], p2 = phi[L.c(2)], p3 = phi[L.c(3)];
+    double l;
+    if (w >= 0.0)
+        l = p1 + L.h(1) / 2.0 * (1.0 - w * dtv / L.h(1)) * (p2 - p1) / (L.pos(2) - L.pos(1));
+    else
+        l = p2 - L.h(2) / 2.0 * (1.0 + w * dtv / L.h(2)) * (p3 - p1) / (X ? 2.0 * L.h(2) : L.pos(3) - L.pos(1));
+    return l * w * dtv;
+}
+
+// level-set flux through the last face (:1111-1123 / :1231-1243)
+template <bool X>
+__device__ double ls_flux_last(const Line<X>& L, const double* phi, double dtv) {
+    const int n = L.na();
+    const double w = L.vel(n);
+    const double pn = phi[L.c(n)], pm = phi[L.c(n - 1)];
+    double l;
+    if (w > 0.0)
+        l = pn + L.h(n) / 2.0 * (1.0 - w * dtv / L.h(n)) * (pn - pm) / (L.pos(n) - L.pos(n - 1));
+    else
+        l = pn - L.h(n - 1) / 2.0 * (1.0 + w * dtv / L.h(n - 1)) * (pn - pm) / (L.pos(n) - L.pos(n - 1));
+    return l * w * dtv;
+}
+
+// One sweep.  IMPL: first sweep of the sub-step (implicit divergence, clip);
+// otherwise second sweep (explicit divergence with the first sweep's
+// tv/tl).  level: Depthw+etau of the west inflow (x sweeps).
+template <bool X, bool IMPL>
+__global__ void __launch_bounds__(256) k_sweep(ClsGeo G, SweepIn s, double dtv, double level,
+                                               const double* __restrict__ tv_in, const double* __restrict__ tl_in,
+                                               double* __restrict__ tv_out, double* __restrict__ tl_out,
+                                               double* __restrict__ vof_out, double* __restrict__ phi_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > G.ni || j > G.nj) return;
+    const Line<X> L{G, X ? j : i};
+    const int a = X ? i : j;
+    const int n = L.na();
+    const size_t c = L.c(a);
+    const double vs = G.vofS[c];
+    // volume of fluid: temvf(a) after the sweep's in-place updates
+    double t = s.vof[c];
+    if (vs < 1.0 - VOFEPS) t = t / (1.0 - vs);
+    if (a >= 2) t = t * (1.0 - vs) + vof_flux<X>(L, s, a - 1, dtv) * L.am(a) / L.h(a);
+    t = t - vof_flux<X>(L, s, a, dtv) * L.ap(a) / L.h(a);
+    if (a == 1) {
+        double fin = 0.0;
+        if (X) {
+            if (G.yc[j - 1] - G.dy[j - 1] / 2.0 < level) fin = s.vof[c] * L.vel(0) * dtv;
+        } else {
+            fin = s.vof[c] * L.vel(0) * dtv;
+        }
+        t = t + fin * L.am(1) / L.h(1);
+    }
+    // level set: temls(a)
+    double q = s.phi[c];
+    if (a >= 3) q = q * (1.0 - vs) + ls_flux<X>(L, s.phi, a - 1, dtv) * L.am(a) / L.h(a);
+    if (a >= 2 && a <= n - 1) q = q - ls_flux<X>(L, s.phi, a, dtv) * L.ap(a) / L.h(a);
+    if (a == 1) {
+        q = q - ls_flux_first<X>(L, s.phi, dtv) * L.ap(1) / L.h(1);
+        q = q + s.phi[c] * L.vel(0) * dtv * L.am(1) / L.h(1);
+    }
+    if (a == 2) q = q + ls_flux_first<X>(L, s.phi, dtv) * L.ap(1) / L.h(2);
+    if (a == n) {
+        const double fl = ls_flux_last<X>(L, s.phi, dtv);
+        q = X ? q - fl * L.ap(n) / L.h(n) : q - fl / L.h(n);
+    }
+    const double dvel = L.vel(a) * L.ap(a) - L.vel(a - 1) * L.am(a);
+    if (IMPL) {
+        const double den = (1.0 - vs + TOLDENO) - dtv / L.h(a) * dvel;
+        t = t / den;
+        q = q / den;
+        tv_out[c] = t;
+        tl_out[c] = q;
+        double f = t * (1.0 - vs);
+        if (f >= 1.0 - VOFEPS - vs) f = 1.0 - vs;
+        if (f < VOFEPS) f = 0.0;
+        vof_out[c] = f;
+        phi_out[c] = q;
+    } else {
+        t = t + dtv / L.h(a) * tv_in[c] * dvel;
+        q = q + dtv / L.h(a) * tl_in[c] * dvel;
+        vof_out[c] = t;
+        phi_out[c] = vs < 1.0 - EPSI ? q / (1.0 - vs) : s.phi[c];
+    }
+}
+
+// ---- redistance -----------------------------------------------------------------
+
+// isinsiderect (:1921-1955)
+__device__ bool inside_rect(double xp, double yp, double dx, double dy) {
+    const double x1 = -0.5 * dx, y1 = 0.5 * dy, x2 = 0.5 * dx, y2 = 0.5 * dy;
+    const double x3 = 0.5 * dx, y3 = -0.5 * dy, x4 = -0.5 * dx, y4 = -0.5 * dy;
+    const double b1 = sqrt((x1 - xp) * (x1 - xp) + (y1 - yp) * (y1 - yp));
+    const double b2 = sqrt((x2 - xp) * (x2 - xp) + (y2 - yp) * (y2 - yp));
+    const double b3 = sqrt((x3 - xp) * (x3 - xp) + (y3 - yp) * (y3 - yp));
+    const double b4 = sqrt((x4 - xp) * (x4 - xp) + (y4 - yp) * (y4 - yp));
+    const double u1 = 0.5 * (dx + b1 + b2), u2 = 0.5 * (dy + b2 + b3);
+    const double u3 = 0.5 * (dx + b3 + b4), u4 = 0.5 * (dy + b4 + b1);
+    const double a1 = sqrt(u1 * (u1 - dx) * (u1 - b1) * (u1 - b2));
+    const double a2 = sqrt(u2 * (u2 - dy) * (u2 - b2) * (u2 - b3));
+    const double a3 = sqrt(u3 * (u3 - dx) * (u3 - b3) * (u3 - b4));
+    const double a4 = sqrt(u4 * (u4 - dy) * (u4 - b4) * (u4 - b1));
+    return fabs(a1 + a2 + a3 + a4 - dx * dy) < 1.0e-10;
+}
+
+struct Target {
+    double v, s, phi, x, y;  // vof, vofS, phi, centre of the target cell
+};
+
+// 3-phase update of a target (:1598-1603): nearer distance, sign by the
+// target's own state
+__device__ __forceinline__ void near3(double deuc, const Target& T, double& pa) {
+    if (fabs(deuc) < fabs(pa) && T.v >= 1.0 - VOFEPS - T.s) pa = -fabs(deuc);
+    if (fabs(deuc) < fabs(pa) && T.v < VOFEPS) pa = fabs(deuc);
+}
+
+// the visit of target T by interface source (i,j) (:1550-1686)
+__device__ void visit(const ClsGeo& G, const double* __restrict__ nxa, const double* __restrict__ nya,
+                      const double* __restrict__ disa, int i, int j, int ii, int jj, const Target& T, double& pa) {
+    const size_t c = cix(G, i, j);
+    const double dxs = G.dx[i - 1], dys = G.dy[j - 1];
+    const double sxn = G.nxS[c], syn = G.nyS[c], sph = G.phiS[c];
+    const double n1 = nxa[c], n2 = nya[c], d1 = disa[c];
+    const int l = ii < -1 ? -1 : (ii > 1 ? 1 : ii);
+    const int m = jj < -1 ? -1 : (jj > 1 ? 1 : jj);
+    double xv = dxs * (double)l / 2.0;
+    double yv = dys * (double)m / 2.0;
+    double dvs = sxn * xv + syn * yv + sph;
+    if (dvs < 0.0) {
+        if (fabs(yv) < VOFEPS) {
+            yv = dys / 2.0;
+            dvs = sxn * xv + syn * yv + sph;
+            if (dvs < 0.0) {
+                yv = -dys / 2.0;
+                dvs = sxn * xv + syn * yv + sph;
+                if (dvs < 0.0) {
+                    xv = dxs * (0.0 - (double)l) / 2.0;
+                    dvs = sxn * xv + syn * yv + sph;
+                    if (dvs < 0.0) yv = dys / 2.0;
+                }
+            }
+        } else if (fabs(xv) < VOFEPS) {
+            xv = dxs / 2.0;
+            dvs = sxn * xv + syn * yv + sph;
+            if (dvs < 0.0) {
+                xv = -dxs / 2.0;
+                dvs = sxn * xv + syn * yv + sph;
+                if (dvs < 0.0) {
+                    yv = dys * (0.0 - (double)m) / 2.0;
+                    dvs = sxn * xv + syn * yv + sph;
+                    if (dvs < 0.0) xv = dxs / 2.0;
+                }
+            }
+        } else {
+            yv = dys * (0.0 - (double)m) / 2.0;
+            dvs = sxn * xv + syn * yv + sph;
+            if (dvs < 0.0) {
+                xv = dxs * (0.0 - (double)l) / 2.0;
+                yv = dys * (double)m / 2.0;
+                dvs = sxn * xv + syn * yv + sph;
+                if (dvs < 0.0) yv = dys * (0.0 - (double)m) / 2.0;
+            }
+        }
+    }
+    const double ddx = T.x - G.xc[i - 1], ddy = T.y - G.yc[j - 1];
+    const double dv = n1 * xv + n2 * yv + d1;
+    if (dv * fsign(1.0, T.phi) <= 0.0) {
+        const double ax = ddx - xv, ay = ddy - yv;
+        const double deuc = sqrt(ax * ax + ay * ay);
+        if (T.s < EPSI)
+            pa = fsign(1.0, 0.5 - T.v) * fmin(deuc, fabs(pa));
+        else
+            near3(deuc, T, pa);
+        return;
+    }
+    const double dij = n1 * ddx + n2 * ddy + d1;
+    const double xp = ddx - dij * n1, yp = ddy - dij * n2;
+    const bool inside = inside_rect(xp, yp, dxs, dys);
+    if (inside && T.s < EPSI) {
+        const double ax = ddx - xp, ay = ddy - yp;
+        const double deuc = sqrt(ax * ax + ay * ay);
+        pa = fsign(1.0, 0.5 - T.v) * fmin(deuc, fabs(pa));
+    } else if (inside && T.s >= EPSI && T.s < 1.0 - EPSI) {
+        const double ax = ddx - xp, ay = ddy - yp;
+        near3(sqrt(ax * ax + ay * ay), T, pa);
+    } else if (T.s >= 1.0 - EPSI) {
+        if (G.vofS[c] < EPSI) {
+            const double ax = ddx - xp, ay = ddy - yp;
+            const double deuc = sqrt(ax * ax + ay * ay);
+            if (fabs(deuc) < fabs(pa)) pa = fsign(1.0, dij) * deuc;
+        }
+    } else {
+        const double tol = 1.0e-24;
+        const double xoff = fmax(fabs(xp) - 0.5 * dxs, 0.0);
+        const double yoff = fmax(fabs(yp) - 0.5 * dys, 0.0);
+        const double xfc = fsign(1.0, xp) * 0.5 * dxs;
+        const double yfc = fsign(1.0, yp) * 0.5 * dys;
+        double xs, ys;
+        if (xoff * fabs(n1 + tol) >= yoff * fabs(n2 + tol)) {
+            xs = xfc;
+            ys = (d1 + n1 * xs) / (-n2 + tol);
+        } else {
+            ys = yfc;
+            xs = (d1 + n2 * ys) / (-n1 + tol);
+        }
+        const double ax = ddx - xs, ay = ddy - ys;
+        const double deuc = sqrt(ax * ax + ay * ay);
+        if (sxn * xs + syn * ys + sph >= 0.0) {
+            if (T.s < EPSI)
+                pa = fsign(1.0, 0.5 - T.v) * fmin(deuc, fabs(pa));
+            else
+                near3(deuc, T, pa);
+        }
+    }
+}
+
+// Redistance, gathered: each cell visits the interface cells of its 9x9
+// window in the reference's source order (i ascending, then j ascending).
+__global__ void __launch_bounds__(256) k_redistance(ClsGeo G, const double* __restrict__ vof,
+                                                    const double* __restrict__ nxa, const double* __restrict__ nya,
+                                                    const double* __restrict__ disa,
+                                                    const unsigned long long* __restrict__ mask, int mw,
+                                                    double* __restrict__ phi) {
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int tj = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (ti > G.ni || tj > G.nj) return;
+    const size_t c = cix(G, ti, tj);
+    Target T;
+    T.v = vof[c];
+    T.s = G.vofS[c];
+    if ((0.0 + VOFEPS) < T.v && T.v < (1.0 - VOFEPS - T.s)) {
+        phi[c] = disa[c];  // an interface cell keeps its own plane constant
+        return;
+    }
+    const bool target = T.v < 0.0 + VOFEPS || T.v >= 1.0 - VOFEPS - T.s;
+    double pa = 1.0e4;
+    bool visited = false;
+    if (target) {
+        const int ilo = max(ti - BAND, 1), ihi = min(ti + BAND, G.ni);
+        const int jlo = max(tj - BAND, 1), jhi = min(tj + BAND, G.nj);
+        // window bits per row: bit k <-> source column ilo+k
+        unsigned rows[2 * BAND + 1];
+        unsigned any = 0;
+        for (int sj = jlo; sj <= jhi; ++sj) {
+            const unsigned long long* r = mask + (size_t)(sj - 1) * mw;
+            const int b0 = ilo - 1, w0 = b0 >> 6, w1 = (ihi - 1) >> 6, sh = b0 & 63;
+            unsigned long long bits = r[w0] >> sh;
+            if (w1 != w0 && sh) bits |= r[w1] << (64 - sh);
+            const unsigned win = (unsigned)(bits & ((1ull << (ihi - ilo + 1)) - 1ull));
+            rows[sj - jlo] = win;
+            any |= win;
+        }
+        if (any) {
+            T.phi = phi[c];
+            T.x = G.xc[ti - 1];
+            T.y = G.yc[tj - 1];
+            for (int si = ilo; si <= ihi; ++si) {
+                for (int sj = jlo; sj <= jhi; ++sj) {
+                    if (!((rows[sj - jlo] >> (si - ilo)) & 1u)) continue;
+                    visited = true;
+                    visit(G, nxa, nya, disa, si, sj, ti - si, tj - sj, T, pa);
+                }
+            }
+        }
+    }
+    if (visited && !(pa > 9.999e3)) {
+        phi[c] = pa;
+        return;
+    }
+    const double hx = G.dx[ti - 1], hy = G.dy[tj - 1];
+    if (T.v <= VOFEPS)
+        phi[c] = (double)(BAND + 1) * sqrt(hx * hx + hy * hy);
+    else if (T.v >= (1.0 - VOFEPS - T.s))
+        phi[c] = -(double)(BAND + 1) * sqrt(hx * hx + hy * hy);
+}
+
+ClsGeo make_geo(Ctx* c) {
+    ClsGeo G;
+    G.ni = c->ni;
+    G.nj = c->nj;
+    G.dx = c->dx;
+    G.dy = c->dy;
+    G.xc = c->cls.xc;
+    G.yc = c->cls.yc;
+    G.vofS = (const double*)c->field[SG_F_VOFS];
+    G.nxS = (const double*)c->field[SG_F_NXS];
+    G.nyS = (const double*)c->field[SG_F_NYS];
+    G.phiS = (const double*)c->field[SG_F_PHIS];
+    G.aE = (const double*)c->field[SG_F_AE];
+    G.aW = (const double*)c->field[SG_F_AW];
+    G.aN = (const double*)c->field[SG_F_AN];
+    G.aS = (const double*)c->field[SG_F_AS];
+    G.uflag = (const int*)c->field[SG_F_UFLAG];
+    G.vflag = (const int*)c->field[SG_F_VFLAG];
+    G.u = (const double*)c->field[SG_F_U];
+    G.v = (const double*)c->field[SG_F_V];
+    return G;
+}
+
+}  // namespace
+
+int clsvof_allocate(Ctx* c) {
+    const size_t n = (size_t)c->ni * c->nj;
+    ClsBuffers& B = c->cls;
+    B.mw = (c->ni + 63) / 64;
+    B.xc = (double*)dev_alloc(c, c->ni * sizeof(double));
+    B.yc = (double*)dev_alloc(c, c->nj * sizeof(double));
+    B.dis = (double*)dev_alloc(c, n * sizeof(double));
+    B.vt = (double*)dev_alloc(c, n * sizeof(double));
+    B.pt = (double*)dev_alloc(c, n * sizeof(double));
+    B.tv = (double*)dev_alloc(c, n * sizeof(double));
+    B.tl = (double*)dev_alloc(c, n * sizeof(double));
+    B.stale = (unsigned char*)dev_alloc(c, n);
+    B.mask = (unsigned long long*)dev_alloc(c, (size_t)B.mw * c->nj * sizeof(unsigned long long));
+    if (!B.xc || !B.yc || !B.dis || !B.vt || !B.pt || !B.tv || !B.tl || !B.stale || !B.mask) return SG_ERR_ALLOC;
+    return SG_OK;
+}
+
+// Coupled_LS_VOF on the device fields VOF, PHI (in/out), NX, NY (out),
+// U, V (in).  `levels` holds Depthw+etau of each sub-step.
+int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
+    const ClsGeo G = make_geo(c);
+    ClsBuffers& B = c->cls;
+    double* vof = (double*)c->field[SG_F_VOF];
+    double* phi = (double*)c->field[SG_F_PHI];
+    double* nx = (double*)c->field[SG_F_NX];
+    double* ny = (double*)c->field[SG_F_NY];
+    const dim3 blk(64, 4), grd((c->ni + 63) / 64, (c->nj + 3) / 4);
+    const int ring = 2 * c->ni + 2 * (c->nj - 2);
+    const dim3 rblk(256), rgrd((ring + 255) / 256);
+    const double dtv = dt / (double)nsub;
+    hipStream_t st = c->stream;
+    auto recon = [&](const double* v, const double* p, bool with_mask) {
+        hipLaunchKernelGGL(k_recon, grd, blk, 0, st, G, v, p, nx, ny, B.dis, B.stale,
+                           with_mask ? B.mask : (unsigned long long*)nullptr, B.mw);
+        hipLaunchKernelGGL(k_dis_fix, grd, blk, 0, st, G, B.stale, B.dis);
+    };
+    for (int k = 1; k <= nsub; ++k) {
+        const double level = levels[k - 1];
+        // first sweep: (vof, phi) -> (vt, pt), implicit divergence
+        recon(vof, phi, false);
+        const SweepIn s1{vof, phi, nx, ny, B.dis};
+        if (k % 2 == 1)
+            hipLaunchKernelGGL((k_sweep<true, true>), grd, blk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
+                               B.tl, B.vt, B.pt);
+        else
+            hipLaunchKernelGGL((k_sweep<false, true>), grd, blk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
+                               B.tl, B.vt, B.pt);
+        hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, B.vt, B.pt);
+        // second sweep: (vt, pt) -> (vof, phi), explicit divergence
+        recon(B.vt, B.pt, false);
+        const SweepIn s2{B.vt, B.pt, nx, ny, B.dis};
+        if (k % 2 == 1)
+            hipLaunchKernelGGL((k_sweep<false, false>), grd, blk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
+                               nullptr, vof, phi);
+        else
+            hipLaunchKernelGGL((k_sweep<true, false>), grd, blk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
+                               nullptr, vof, phi);
+        hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, vof, phi);
+        recon(vof, phi, true);
+        hipLaunchKernelGGL(k_redistance, grd, blk, 0, st, G, vof, nx, ny, B.dis, B.mask, B.mw, phi);
+    }
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+}  // namespace sg

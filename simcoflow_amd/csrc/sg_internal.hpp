@@ -108,6 +108,22 @@ struct Reducer {
     int max_blocks = 0;
 };
 
+// Work arrays of the interface transport (cell layout).
+struct ClsBuffers {
+    double* xc = nullptr;   // P-cell centres PGrid%x(:,1), PGrid%y(1,:)
+    double* yc = nullptr;
+    double* dis = nullptr;  // PLIC plane constants of the last reconstruction
+    double* vt = nullptr;   // vof, phi after the first sweep of a sub-step
+    double* pt = nullptr;
+    double* tv = nullptr;   // temvf, temls of the first sweep (explicit term)
+    double* tl = nullptr;
+    unsigned char* stale = nullptr;    // plane constant left unassigned
+    unsigned long long* mask = nullptr;  // interface cells, 1 bit per cell
+    int mw = 0;                        // mask words per row
+    double xw = 0.0;                   // west edge x of the grid (host copy)
+    double dx0 = 0.0;                  // width of the first column (host copy)
+};
+
 struct Ctx {
     int ni = 0, nj = 0;
     sg_params prm{};
@@ -140,6 +156,7 @@ struct Ctx {
     hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
     int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0};
     int use_graph = 1;
+    ClsBuffers cls;
     std::vector<void*> allocations;
 };
 
@@ -181,6 +198,10 @@ int mg_launch_fine(Ctx* c, int which);  // measurement: 1 down, 2 up, 3 copy
 // PCG (sg_pcg.hip)
 int pcg_solve(Ctx* c, sg_solve_info* info);
 int fine_residual_norms(Ctx* c, double* relres, double* maxrow);
+
+// interface transport (sg_clsvof.hip)
+int clsvof_allocate(Ctx* c);
+int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels);
 
 void* dev_alloc(Ctx* c, size_t bytes);
 

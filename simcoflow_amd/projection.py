@@ -1,10 +1,13 @@
-"""Host side of the projection step, over the simcogpu C ABI.
+"""Host side of the projection step and the interface transport, over the
+simcogpu C ABI.
 
 ``Projection`` mirrors the reference's ``PoissonEquationSolver``
-(ProjectionP_Mod.f90:26-99) and the correction part of ``UpdatePUV``
-(ComputePUV_Mod.f90:17-225): same inputs (predictor velocities, PU%Dp /
-PV%Dp, the P-cell vof / phi, cut-cell apertures), same outputs (Proj%Pp with
-its ghost cells; corrected TVar%u / v / p and the mass residual).  Arrays are
+(ProjectionP_Mod.f90:26-99), the correction part of ``UpdatePUV``
+(ComputePUV_Mod.f90:17-225) and ``Coupled_LS_VOF`` (Clsvof_Mod.f90:730-959):
+same inputs (predictor velocities, PU%Dp / PV%Dp, the P-cell vof / phi,
+cut-cell apertures, TVar%u / v), same outputs (Proj%Pp with its ghost cells;
+corrected TVar%u / v / p and the mass residual; advected PCell%vof / phi /
+nx / ny).  Arrays are
 numpy arrays in the reference's Fortran layout: a cell array has shape
 (nj, ni) in C order, a ghosted array (nj+2, ni+2).
 """
@@ -60,6 +63,9 @@ class Projection:
         check(self.lib.sg_ctx_create(self.ni, self.nj, keep[0][1], keep[1][1], keep[2][1], keep[3][1],
                                      ctypes.byref(params), ctypes.byref(ctx)))
         self.ctx = ctx
+        if getattr(grid, "xc", None) is not None:
+            xc, yc = _d(grid.xc), _d(grid.yc)
+            check(self.lib.sg_set_centres(self.ctx, xc[1], yc[1]))
 
     # -- lifetime --------------------------------------------------------
     def close(self):
@@ -114,6 +120,10 @@ class Projection:
         tail = [_d(geo.vofSU), _d(geo.vofSV)]
         check(self.lib.sg_set_geometry(self.ctx, *[a[1] for a in args], *[a[1] for a in ints],
                                        *[a[1] for a in tail]))
+        # the solid line of the three-phase cells (interface transport)
+        if getattr(geo, "nxS", None) is not None:
+            self.upload("NXS", geo.nxS)
+            self.upload("PHIS", geo.phiS)
 
     def set_interface(self, vof, phi):
         self.upload("VOF", vof)
@@ -167,3 +177,33 @@ class Projection:
 
     def synchronize(self):
         check(self.lib.sg_synchronize(self.ctx))
+
+    # -- interface transport -------------------------------------------------
+    def clsvof_params(self, nsub=6, wave=None):
+        p = _lib.ClsvofParams()
+        check(self.lib.sg_clsvof_params_default(ctypes.byref(p)))
+        p.nsub = nsub
+        if wave is not None:
+            p.depth, p.amp, p.kw, p.omega = wave
+        return p
+
+    def clsvof_advect_resident(self, dt, time=0.0, nsub=6, wave=None):
+        """Coupled_LS_VOF on the device fields VOF, PHI (in/out), NX, NY (out)."""
+        p = self.clsvof_params(nsub, wave)
+        check(self.lib.sg_clsvof_advect_resident(self.ctx, float(dt), float(time), ctypes.byref(p)))
+
+    def clsvof_advect(self, vof, phi, u, v, dt, time=0.0, nsub=6, wave=None):
+        """Drop-in Coupled_LS_VOF: host arrays in, (vof, phi, nx, ny) out."""
+        vof = np.array(vof, dtype=np.float64, order="C")
+        phi = np.array(phi, dtype=np.float64, order="C")
+        nx = np.zeros_like(vof)
+        ny = np.zeros_like(vof)
+        u, up = _d(u)
+        v, vp = _d(v)
+        if vof.shape != (self.nj, self.ni) or u.shape != (self.nj + 2, self.ni + 2) or v.shape != u.shape:
+            raise ValueError("clsvof_advect: vof/phi must be (nj, ni), u/v (nj+2, ni+2)")
+        p = self.clsvof_params(nsub, wave)
+        check(self.lib.sg_clsvof_advect(self.ctx, vof.ctypes.data_as(_DP), phi.ctypes.data_as(_DP),
+                                        nx.ctypes.data_as(_DP), ny.ctypes.data_as(_DP), up, vp, float(dt),
+                                        float(time), ctypes.byref(p)))
+        return vof, phi, nx, ny
