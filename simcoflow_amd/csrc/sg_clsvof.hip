@@ -1,7 +1,7 @@
 // Coupled level-set / volume-of-fluid transport on gfx950 (Clsvof_Mod.f90).
 //
 // One call advances vof and phi over dt in nsub sub-steps (Coupled_LS_VOF,
-// Clsvof_Mod.f90:730-959).  Each sub-step is nine launches:
+// Clsvof_Mod.f90:730-959).  Each sub-step runs these launches:
 //
 //   k_recon      PLIC reconstruction (Interface_Reconstruct :1250-1321)
 //   k_dis_fix    cells whose plane constant the reference leaves stale
@@ -9,16 +9,20 @@
 //   k_bc         boundary copy (Boundary_Condition :961-973)
 //   k_recon, k_dis_fix
 //   k_sweep      second sweep, explicit divergence (:834-858 / :898-929)
-//   k_bc, k_recon (+ interface bit mask), k_dis_fix
-//   k_redistance level set from the PLIC segments (Redistance :1525-1708)
+//   k_bc, k_recon (+ interface bit mask and source list), k_dis_fix
+//   k_rd_reset, k_rd_scatter, k_redistance: level set from the PLIC
+//                segments (Redistance :1525-1708)
 //
 // The reference's sweeps walk each row in order and update cells in place;
 // every face flux they use depends only on data from before the sweep, so a
 // sweep here is one thread per cell that recomputes the fluxes of its two
 // faces and applies them in the reference's order.  Redistance scatters
-// from interface cells into a 9x9 window with order-dependent ties; here
-// each target gathers over its window in the reference's source order,
-// which gives the same result bit for bit.
+// from interface cells into a 9x9 window.  For two- and three-phase targets
+// the result is a minimum over the visits, taken here with one lane per
+// (source, window offset) pair and an integer atomic minimum on the
+// distance's bit pattern; full-solid targets, whose sign depends on the
+// order of equal distances, replay their visits in the reference's order.
+// Both give the reference's result bit for bit.
 //
 // This file is compiled with FP contraction off: every sum and product
 // rounds as the reference writes it.
@@ -548,27 +552,73 @@ __device__ void final_topbottom(const ClsGeo& G, const double* vof, const double
 
 // ---- kernels ------------------------------------------------------------------
 
-// Interface_Reconstruct.  stale[c] = 1 where DistanceFluidCalculate leaves the
-// plane constant unassigned (the reference then keeps the previous cell's).
-// With mask != null it also writes the redistance source mask (one bit per
-// cell: vofeps < vof < 1-vofeps-vofS), one 64-bit word per wavefront.
+// PLIC planes of the last reconstruction.  They are stored only for interface
+// cells (class 1 or 2); every other cell has n = (0,1) and its plane constant
+// follows from its volume fraction (Interface_Reconstruct :1307-1314), so
+// readers rebuild those on the fly instead of reading them.
+struct Planes {
+    double* nx;
+    double* ny;
+    double* dis;
+};
+
+__device__ __forceinline__ double default_dis(double vf, double vs, double hy) {
+    return 1.0 - vs > VOFEPS ? (0.5 - vf / (1.0 - vs)) * hy : (0.0 - vf) * hy;
+}
+
+__device__ __forceinline__ void plane_of(const ClsGeo& G, const Planes& P, size_t c, double vf, double hy, double& nx,
+                                         double& ny, double& d) {
+    const double vs = G.vofS[c];
+    if (recon_class(vf, vs)) {
+        nx = P.nx[c];
+        ny = P.ny[c];
+        d = P.dis[c];
+    } else {
+        nx = 0.0;
+        ny = 1.0;
+        d = default_dis(vf, vs, hy);
+    }
+}
+
+// wavefront-aggregated append of `value` to list where pred holds (one
+// atomic per wavefront); every lane of the wavefront must call it
+__device__ __forceinline__ void append(bool pred, unsigned* count, int* list, int value) {
+    const unsigned long long b = __ballot(pred);
+    if (!b) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)b) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(b));
+    base = __shfl(base, leader);
+    if (pred) list[base + (unsigned)__popcll(b & ((1ull << lane) - 1ull))] = value;
+}
+
+// Interface_Reconstruct.  Interface cells get their normal and plane
+// constant; three-phase cells whose Ridders search leaves the plane constant
+// unassigned (the reference then keeps the previous cell's) are flagged in
+// `stale` and listed for k_dis_fix.  full_out: also write n = (0,1) for the
+// other cells (the PCell%nx/ny output).  With mask != null the kernel also
+// writes the re-distancing source mask (vofeps < vof < 1-vofeps-vofS, one bit
+// per cell, one 64-bit word per wavefront) and lists those cells.
+// cnt[0]: sources listed, cnt[1]: stale cells listed.
 __global__ void __launch_bounds__(256) k_recon(ClsGeo G, const double* __restrict__ vof,
-                                               const double* __restrict__ phi, double* __restrict__ nxo,
-                                               double* __restrict__ nyo, double* __restrict__ dis,
-                                               unsigned char* __restrict__ stale,
-                                               unsigned long long* __restrict__ mask, int mw) {
+                                               const double* __restrict__ phi, Planes P,
+                                               unsigned char* __restrict__ stale, int full_out,
+                                               unsigned long long* __restrict__ mask, int mw,
+                                               int* __restrict__ srcs, int* __restrict__ stales,
+                                               unsigned* __restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const bool in = i <= G.ni && j <= G.nj;
-    bool src = false;
+    bool src = false, st = false;
+    size_t c = 0;
     if (in) {
-        const size_t c = cix(G, i, j);
+        c = cix(G, i, j);
         const double vf = vof[c], vs = G.vofS[c];
         src = (0.0 + VOFEPS) < vf && vf < (1.0 - VOFEPS - vs);
         const int cl = recon_class(vf, vs);
-        double nx, ny, d = 0.0;
-        unsigned char st = 0;
         if (cl) {
+            double nx, ny, d = 0.0;
             if (i > 1 && i < G.ni && j > 1 && j < G.nj)
                 final_interior(G, vof, phi, i, j, nx, ny);
             else if (i == 1 || i == G.ni)
@@ -578,51 +628,54 @@ __global__ void __launch_bounds__(256) k_recon(ClsGeo G, const double* __restric
             const double hx = G.dx[i - 1], hy = G.dy[j - 1];
             if (cl == 1)
                 d = plic_distance(vf, nx, ny, hx, hy);
-            else if (!distance_fluid(G.nxS[c], G.nyS[c], G.phiS[c], nx, ny, vf, hx, hy, d))
-                st = 1;
-        } else {
-            nx = 0.0;
-            ny = 1.0;
-            if (1.0 - vs > VOFEPS)
-                d = (0.5 - vf / (1.0 - vs)) * G.dy[j - 1];
             else
-                d = (0.0 - vf) * G.dy[j - 1];
+                st = !distance_fluid(G.nxS[c], G.nyS[c], G.phiS[c], nx, ny, vf, hx, hy, d);
+            P.nx[c] = nx;
+            P.ny[c] = ny;
+            P.dis[c] = d;
+            if (cl == 2) stale[c] = st ? 1 : 0;
+        } else if (full_out) {
+            P.nx[c] = 0.0;
+            P.ny[c] = 1.0;
         }
-        nxo[c] = nx;
-        nyo[c] = ny;
-        dis[c] = d;
-        stale[c] = st;
     }
+    append(st, cnt + 1, stales, (int)c);
     if (mask) {
         // blockDim.x == 64: one wavefront covers 64 consecutive i of row j
         const unsigned long long bits = __ballot(src);
         if (threadIdx.x == 0 && j <= G.nj) mask[(size_t)(j - 1) * mw + blockIdx.x] = bits;
+        append(src, cnt, srcs, (int)c);
     }
 }
 
 // Cells whose plane constant the reference leaves stale take the value of
 // the nearest earlier cell in its loop order (i outer, j inner) that has one.
-__global__ void __launch_bounds__(256) k_dis_fix(ClsGeo G, const unsigned char* __restrict__ stale,
+__global__ void __launch_bounds__(256) k_dis_fix(ClsGeo G, const double* __restrict__ vof,
+                                                 const unsigned char* __restrict__ stale,
+                                                 const int* __restrict__ list, const unsigned* __restrict__ cnt,
                                                  double* __restrict__ dis) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > G.ni || j > G.nj) return;
-    if (!stale[cix(G, i, j)]) return;
-    int pi = i, pj = j;
-    for (;;) {
-        if (pj > 1) {
-            --pj;
-        } else if (pi > 1) {
-            --pi;
-            pj = G.nj;
-        } else {
-            dis[cix(G, i, j)] = 0.0;
-            return;
+    const unsigned n = cnt[1];
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int c0 = list[t];
+        int pi = c0 % G.ni + 1, pj = c0 / G.ni + 1;
+        double d = 0.0;
+        for (;;) {
+            if (pj > 1) {
+                --pj;
+            } else if (pi > 1) {
+                --pi;
+                pj = G.nj;
+            } else {
+                break;  // before the first cell: the reference's uninitialised local
+            }
+            const size_t p = cix(G, pi, pj);
+            const double vf = vof[p], vs = G.vofS[p];
+            const int cl = recon_class(vf, vs);
+            if (cl == 2 && stale[p]) continue;
+            d = cl ? dis[p] : default_dis(vf, vs, G.dy[pj - 1]);
+            break;
         }
-        if (!stale[cix(G, pi, pj)]) {
-            dis[cix(G, i, j)] = dis[cix(G, pi, pj)];
-            return;
-        }
+        dis[c0] = d;
     }
 }
 
@@ -705,9 +758,7 @@ __device__ double plic_face_flux(int side, double nxx, double nyy, double diss, 
 struct SweepIn {
     const double* vof;
     const double* phi;
-    const double* nx;
-    const double* ny;
-    const double* dis;
+    Planes pl;
 };
 
 // volume flux through face a (between cells a and a+1), X_Sweep :1014-1051
@@ -726,8 +777,9 @@ __device__ double vof_flux(const Line<X>& L, const SweepIn& s, int a, double dtv
             if (f < 0.0) flux = 0.0;
         } else {
             const double hx = X ? G.dx[a - 1] : G.dx[L.b - 1], hy = X ? G.dy[L.b - 1] : G.dy[a - 1];
-            flux = plic_face_flux<X>(1, s.nx[c], s.ny[c], s.dis[c], f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c], hx,
-                                     hy, w * dtv);
+            double nx, ny, d;
+            plane_of(G, s.pl, c, f, hy, nx, ny, d);
+            flux = plic_face_flux<X>(1, nx, ny, d, f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c], hx, hy, w * dtv);
         }
         if (L.ap(a) > 1.0 - VOFEPS && G.vofS[c] > 1.0 - VOFEPS) flux = 1.0 * w * dtv;
         return flux;
@@ -742,8 +794,9 @@ __device__ double vof_flux(const Line<X>& L, const SweepIn& s, int a, double dtv
             if (f < 0.0) flux = 0.0;
         } else {
             const double hx = X ? G.dx[a] : G.dx[L.b - 1], hy = X ? G.dy[L.b - 1] : G.dy[a];
-            flux = -plic_face_flux<X>(-1, s.nx[c], s.ny[c], s.dis[c], f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c],
-                                      hx, hy, -w * dtv);
+            double nx, ny, d;
+            plane_of(G, s.pl, c, f, hy, nx, ny, d);
+            flux = -plic_face_flux<X>(-1, nx, ny, d, f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c], hx, hy, -w * dtv);
         }
         if (L.am(a + 1) > 1.0 - VOFEPS && G.vofS[c] > 1.0 - VOFEPS) flux = 1.0 * w * dtv;
         return flux;
@@ -878,22 +931,27 @@ struct Target {
     double v, s, phi, x, y;  // vof, vofS, phi, centre of the target cell
 };
 
-// 3-phase update of a target (:1598-1603): nearer distance, sign by the
-// target's own state
-__device__ __forceinline__ void near3(double deuc, const Target& T, double& pa) {
-    if (fabs(deuc) < fabs(pa) && T.v >= 1.0 - VOFEPS - T.s) pa = -fabs(deuc);
-    if (fabs(deuc) < fabs(pa) && T.v < VOFEPS) pa = fabs(deuc);
-}
+// What one visit of a target by an interface source does (:1550-1686):
+//   R2  two-phase target:   pa = sign(0.5-vof) * min(d, |pa|)
+//   R3  three-phase target: if d < |pa|: pa = -d (liquid-full) or +d (empty)
+//   RS  full-solid target:  if d < |pa|: pa = sign(dij) * d
+// R2 and R3 reduce to a minimum over the visits in any order; RS depends on
+// which source comes first among equal distances.
+enum { V_NONE = 0, V_R2, V_R3, V_RS };
+struct Visit {
+    int kind;
+    double d;
+    double sg;  // RS sign
+};
 
-// the visit of target T by interface source (i,j) (:1550-1686)
-__device__ void visit(const ClsGeo& G, const double* __restrict__ nxa, const double* __restrict__ nya,
-                      const double* __restrict__ disa, int i, int j, int ii, int jj, const Target& T, double& pa) {
+__device__ Visit propose(const ClsGeo& G, int i, int j, int ii, int jj, double n1, double n2, double d1,
+                         const Target& T) {
     const size_t c = cix(G, i, j);
     const double dxs = G.dx[i - 1], dys = G.dy[j - 1];
     const double sxn = G.nxS[c], syn = G.nyS[c], sph = G.phiS[c];
-    const double n1 = nxa[c], n2 = nya[c], d1 = disa[c];
     const int l = ii < -1 ? -1 : (ii > 1 ? 1 : ii);
     const int m = jj < -1 ? -1 : (jj > 1 ? 1 : jj);
+    // point of the source cell's edge facing the target, moved off the solid
     double xv = dxs * (double)l / 2.0;
     double yv = dys * (double)m / 2.0;
     double dvs = sxn * xv + syn * yv + sph;
@@ -934,63 +992,127 @@ __device__ void visit(const ClsGeo& G, const double* __restrict__ nxa, const dou
         }
     }
     const double ddx = T.x - G.xc[i - 1], ddy = T.y - G.yc[j - 1];
+    const int rule = T.s < EPSI ? V_R2 : V_R3;
     const double dv = n1 * xv + n2 * yv + d1;
     if (dv * fsign(1.0, T.phi) <= 0.0) {
         const double ax = ddx - xv, ay = ddy - yv;
-        const double deuc = sqrt(ax * ax + ay * ay);
-        if (T.s < EPSI)
-            pa = fsign(1.0, 0.5 - T.v) * fmin(deuc, fabs(pa));
-        else
-            near3(deuc, T, pa);
-        return;
+        return {rule, sqrt(ax * ax + ay * ay), 0.0};
     }
+    // projection of the target centre onto the source's PLIC line
     const double dij = n1 * ddx + n2 * ddy + d1;
     const double xp = ddx - dij * n1, yp = ddy - dij * n2;
     const bool inside = inside_rect(xp, yp, dxs, dys);
-    if (inside && T.s < EPSI) {
+    if (inside && T.s < 1.0 - EPSI) {
         const double ax = ddx - xp, ay = ddy - yp;
-        const double deuc = sqrt(ax * ax + ay * ay);
-        pa = fsign(1.0, 0.5 - T.v) * fmin(deuc, fabs(pa));
-    } else if (inside && T.s >= EPSI && T.s < 1.0 - EPSI) {
-        const double ax = ddx - xp, ay = ddy - yp;
-        near3(sqrt(ax * ax + ay * ay), T, pa);
-    } else if (T.s >= 1.0 - EPSI) {
+        return {rule, sqrt(ax * ax + ay * ay), 0.0};
+    }
+    if (T.s >= 1.0 - EPSI) {
         if (G.vofS[c] < EPSI) {
             const double ax = ddx - xp, ay = ddy - yp;
-            const double deuc = sqrt(ax * ax + ay * ay);
-            if (fabs(deuc) < fabs(pa)) pa = fsign(1.0, dij) * deuc;
+            return {V_RS, sqrt(ax * ax + ay * ay), fsign(1.0, dij)};
         }
+        return {V_NONE, 0.0, 0.0};
+    }
+    // nearest end of the segment
+    const double tol = 1.0e-24;
+    const double xoff = fmax(fabs(xp) - 0.5 * dxs, 0.0);
+    const double yoff = fmax(fabs(yp) - 0.5 * dys, 0.0);
+    const double xfc = fsign(1.0, xp) * 0.5 * dxs;
+    const double yfc = fsign(1.0, yp) * 0.5 * dys;
+    double xs, ys;
+    if (xoff * fabs(n1 + tol) >= yoff * fabs(n2 + tol)) {
+        xs = xfc;
+        ys = (d1 + n1 * xs) / (-n2 + tol);
     } else {
-        const double tol = 1.0e-24;
-        const double xoff = fmax(fabs(xp) - 0.5 * dxs, 0.0);
-        const double yoff = fmax(fabs(yp) - 0.5 * dys, 0.0);
-        const double xfc = fsign(1.0, xp) * 0.5 * dxs;
-        const double yfc = fsign(1.0, yp) * 0.5 * dys;
-        double xs, ys;
-        if (xoff * fabs(n1 + tol) >= yoff * fabs(n2 + tol)) {
-            xs = xfc;
-            ys = (d1 + n1 * xs) / (-n2 + tol);
-        } else {
-            ys = yfc;
-            xs = (d1 + n2 * ys) / (-n1 + tol);
-        }
-        const double ax = ddx - xs, ay = ddy - ys;
-        const double deuc = sqrt(ax * ax + ay * ay);
-        if (sxn * xs + syn * ys + sph >= 0.0) {
-            if (T.s < EPSI)
-                pa = fsign(1.0, 0.5 - T.v) * fmin(deuc, fabs(pa));
-            else
-                near3(deuc, T, pa);
-        }
+        ys = yfc;
+        xs = (d1 + n2 * ys) / (-n1 + tol);
+    }
+    const double ax = ddx - xs, ay = ddy - ys;
+    const double deuc = sqrt(ax * ax + ay * ay);
+    if (sxn * xs + syn * ys + sph >= 0.0) return {rule, deuc, 0.0};
+    return {V_NONE, 0.0, 0.0};
+}
+
+__device__ __forceinline__ void apply(const Visit& w, const Target& T, double& pa) {
+    if (w.kind == V_R2) {
+        pa = fsign(1.0, 0.5 - T.v) * fmin(w.d, fabs(pa));
+    } else if (w.kind == V_R3) {
+        if (fabs(w.d) < fabs(pa) && T.v >= 1.0 - VOFEPS - T.s) pa = -fabs(w.d);
+        if (fabs(w.d) < fabs(pa) && T.v < VOFEPS) pa = fabs(w.d);
+    } else if (w.kind == V_RS) {
+        if (fabs(w.d) < fabs(pa)) pa = w.sg * w.d;
     }
 }
 
-// Redistance, gathered: each cell visits the interface cells of its 9x9
-// window in the reference's source order (i ascending, then j ascending).
-__global__ void __launch_bounds__(256) k_redistance(ClsGeo G, const double* __restrict__ vof,
-                                                    const double* __restrict__ nxa, const double* __restrict__ nya,
-                                                    const double* __restrict__ disa,
+__device__ __forceinline__ bool rd_source(double v, double s) { return (0.0 + VOFEPS) < v && v < (1.0 - VOFEPS - s); }
+__device__ __forceinline__ bool rd_target(double v, double s) { return v < 0.0 + VOFEPS || v >= 1.0 - VOFEPS - s; }
+
+// (source, offset) pairs of the listed interface cells; pair p is source
+// p/81 and window offset p%81
+struct PairView {
+    const int* srcs;
+    const unsigned* cnt;
+};
+
+// Pass 1 over the pairs: reset the minimum of every scattered-to target.
+__global__ void __launch_bounds__(256) k_rd_reset(ClsGeo G, const double* __restrict__ vof, PairView pv,
+                                                  unsigned long long* __restrict__ best,
+                                                  unsigned char* __restrict__ hit) {
+    const unsigned long long np = (unsigned long long)pv.cnt[0] * 81ull;
+    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; p < np;
+         p += (unsigned long long)gridDim.x * blockDim.x) {
+        const int c0 = pv.srcs[p / 81];
+        const int o = (int)(p % 81);
+        const int ti = c0 % G.ni + 1 + o / 9 - BAND, tj = c0 / G.ni + 1 + o % 9 - BAND;
+        if (ti < 1 || ti > G.ni || tj < 1 || tj > G.nj) continue;
+        const size_t t = cix(G, ti, tj);
+        best[t] = 0x7FF0000000000000ull;  // +inf
+        hit[t] = 0;
+    }
+}
+
+// Pass 2: each pair proposes a distance to its target; two- and
+// three-phase targets keep the minimum (the bit pattern of a non-negative
+// double orders like the value).  Full-solid targets are left to the
+// ordered gather of k_redistance.
+__global__ void __launch_bounds__(256) k_rd_scatter(ClsGeo G, const double* __restrict__ vof,
+                                                    const double* __restrict__ phi, Planes P, PairView pv,
+                                                    unsigned long long* __restrict__ best,
+                                                    unsigned char* __restrict__ hit) {
+    const unsigned long long np = (unsigned long long)pv.cnt[0] * 81ull;
+    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; p < np;
+         p += (unsigned long long)gridDim.x * blockDim.x) {
+        const int c0 = pv.srcs[p / 81];
+        const int o = (int)(p % 81);
+        const int si = c0 % G.ni + 1, sj = c0 / G.ni + 1;
+        const int ii = o / 9 - BAND, jj = o % 9 - BAND;
+        const int ti = si + ii, tj = sj + jj;
+        if (ti < 1 || ti > G.ni || tj < 1 || tj > G.nj) continue;
+        const size_t t = cix(G, ti, tj);
+        Target T;
+        T.v = vof[t];
+        T.s = G.vofS[t];
+        if (!rd_target(T.v, T.s) || T.s >= 1.0 - EPSI) continue;
+        T.phi = phi[t];
+        T.x = G.xc[ti - 1];
+        T.y = G.yc[tj - 1];
+        double n1, n2, d1;
+        plane_of(G, P, (size_t)c0, vof[c0], G.dy[sj - 1], n1, n2, d1);
+        const Visit w = propose(G, si, sj, ii, jj, n1, n2, d1, T);
+        if (w.kind == V_NONE) continue;
+        if (w.kind == V_R2) hit[t] = 1;
+        if (!(w.d != w.d)) atomicMin(best + t, (unsigned long long)__double_as_longlong(w.d));
+    }
+}
+
+// Redistance, per cell: interface cells take their plane constant, cells
+// reached from an interface cell the nearest distance, the rest the band
+// value.  Full-solid targets replay their visits in the reference's source
+// order (i ascending, then j ascending).
+__global__ void __launch_bounds__(256) k_redistance(ClsGeo G, const double* __restrict__ vof, Planes P,
                                                     const unsigned long long* __restrict__ mask, int mw,
+                                                    const unsigned long long* __restrict__ best,
+                                                    const unsigned char* __restrict__ hit,
                                                     double* __restrict__ phi) {
     const int ti = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int tj = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -999,14 +1121,16 @@ __global__ void __launch_bounds__(256) k_redistance(ClsGeo G, const double* __re
     Target T;
     T.v = vof[c];
     T.s = G.vofS[c];
-    if ((0.0 + VOFEPS) < T.v && T.v < (1.0 - VOFEPS - T.s)) {
-        phi[c] = disa[c];  // an interface cell keeps its own plane constant
+    const double hy = G.dy[tj - 1];
+    if (rd_source(T.v, T.s)) {
+        double n1, n2, d1;
+        plane_of(G, P, c, T.v, hy, n1, n2, d1);
+        phi[c] = d1;  // an interface cell keeps its own plane constant
         return;
     }
-    const bool target = T.v < 0.0 + VOFEPS || T.v >= 1.0 - VOFEPS - T.s;
     double pa = 1.0e4;
     bool visited = false;
-    if (target) {
+    if (rd_target(T.v, T.s)) {
         const int ilo = max(ti - BAND, 1), ihi = min(ti + BAND, G.ni);
         const int jlo = max(tj - BAND, 1), jhi = min(tj + BAND, G.nj);
         // window bits per row: bit k <-> source column ilo+k
@@ -1021,15 +1145,27 @@ __global__ void __launch_bounds__(256) k_redistance(ClsGeo G, const double* __re
             rows[sj - jlo] = win;
             any |= win;
         }
-        if (any) {
-            T.phi = phi[c];
-            T.x = G.xc[ti - 1];
-            T.y = G.yc[tj - 1];
-            for (int si = ilo; si <= ihi; ++si) {
-                for (int sj = jlo; sj <= jhi; ++sj) {
-                    if (!((rows[sj - jlo] >> (si - ilo)) & 1u)) continue;
-                    visited = true;
-                    visit(G, nxa, nya, disa, si, sj, ti - si, tj - sj, T, pa);
+        visited = any != 0;
+        if (visited) {
+            if (T.s >= 1.0 - EPSI) {
+                T.phi = phi[c];
+                T.x = G.xc[ti - 1];
+                T.y = G.yc[tj - 1];
+                for (int si = ilo; si <= ihi; ++si) {
+                    for (int sj = jlo; sj <= jhi; ++sj) {
+                        if (!((rows[sj - jlo] >> (si - ilo)) & 1u)) continue;
+                        const size_t sc = cix(G, si, sj);
+                        double n1, n2, d1;
+                        plane_of(G, P, sc, vof[sc], G.dy[sj - 1], n1, n2, d1);
+                        apply(propose(G, si, sj, ti - si, tj - sj, n1, n2, d1, T), T, pa);
+                    }
+                }
+            } else {
+                const double b = __longlong_as_double((long long)best[c]);
+                if (T.s < EPSI) {
+                    if (hit[c]) pa = fsign(1.0, 0.5 - T.v) * fmin(b, 1.0e4);
+                } else if (b < 1.0e4) {
+                    pa = T.v >= 1.0 - VOFEPS - T.s ? -b : b;
                 }
             }
         }
@@ -1038,11 +1174,16 @@ __global__ void __launch_bounds__(256) k_redistance(ClsGeo G, const double* __re
         phi[c] = pa;
         return;
     }
-    const double hx = G.dx[ti - 1], hy = G.dy[tj - 1];
+    const double hx = G.dx[ti - 1];
     if (T.v <= VOFEPS)
         phi[c] = (double)(BAND + 1) * sqrt(hx * hx + hy * hy);
     else if (T.v >= (1.0 - VOFEPS - T.s))
         phi[c] = -(double)(BAND + 1) * sqrt(hx * hx + hy * hy);
+}
+
+__global__ void k_zero(unsigned* __restrict__ a, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = 0u;
 }
 
 ClsGeo make_geo(Ctx* c) {
@@ -1082,8 +1223,15 @@ int clsvof_allocate(Ctx* c) {
     B.tv = (double*)dev_alloc(c, n * sizeof(double));
     B.tl = (double*)dev_alloc(c, n * sizeof(double));
     B.stale = (unsigned char*)dev_alloc(c, n);
+    B.hit = (unsigned char*)dev_alloc(c, n);
+    B.best = (unsigned long long*)dev_alloc(c, n * sizeof(unsigned long long));
+    B.srcs = (int*)dev_alloc(c, n * sizeof(int));
+    B.stales = (int*)dev_alloc(c, n * sizeof(int));
+    B.cnt = (unsigned*)dev_alloc(c, ClsBuffers::CNT_SLOTS * sizeof(unsigned));
     B.mask = (unsigned long long*)dev_alloc(c, (size_t)B.mw * c->nj * sizeof(unsigned long long));
-    if (!B.xc || !B.yc || !B.dis || !B.vt || !B.pt || !B.tv || !B.tl || !B.stale || !B.mask) return SG_ERR_ALLOC;
+    if (!B.xc || !B.yc || !B.dis || !B.vt || !B.pt || !B.tv || !B.tl || !B.stale || !B.hit || !B.best || !B.srcs ||
+        !B.stales || !B.cnt || !B.mask)
+        return SG_ERR_ALLOC;
     return SG_OK;
 }
 
@@ -1094,23 +1242,34 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
     ClsBuffers& B = c->cls;
     double* vof = (double*)c->field[SG_F_VOF];
     double* phi = (double*)c->field[SG_F_PHI];
-    double* nx = (double*)c->field[SG_F_NX];
-    double* ny = (double*)c->field[SG_F_NY];
+    const Planes P{(double*)c->field[SG_F_NX], (double*)c->field[SG_F_NY], B.dis};
     const dim3 blk(64, 4), grd((c->ni + 63) / 64, (c->nj + 3) / 4);
     const int ring = 2 * c->ni + 2 * (c->nj - 2);
     const dim3 rblk(256), rgrd((ring + 255) / 256);
+    const dim3 pblk(256), pgrd(1024);  // grid-stride loops over the listed pairs / cells
     const double dtv = dt / (double)nsub;
     hipStream_t st = c->stream;
-    auto recon = [&](const double* v, const double* p, bool with_mask) {
-        hipLaunchKernelGGL(k_recon, grd, blk, 0, st, G, v, p, nx, ny, B.dis, B.stale,
-                           with_mask ? B.mask : (unsigned long long*)nullptr, B.mw);
-        hipLaunchKernelGGL(k_dis_fix, grd, blk, 0, st, G, B.stale, B.dis);
+    // two counters per reconstruction, three reconstructions per sub-step
+    const int slots = 6 * nsub;
+    if (slots > ClsBuffers::CNT_SLOTS) {
+        set_error("clsvof_advect: too many sub-steps");
+        return SG_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_zero, dim3((slots + 255) / 256), dim3(256), 0, st, B.cnt, slots);
+    int slot = 0;
+    auto recon = [&](const double* v, const double* p, bool last) {
+        unsigned* cnt = B.cnt + slot;
+        slot += 2;
+        hipLaunchKernelGGL(k_recon, grd, blk, 0, st, G, v, p, P, B.stale, last ? 1 : 0,
+                           last ? B.mask : (unsigned long long*)nullptr, B.mw, B.srcs, B.stales, cnt);
+        hipLaunchKernelGGL(k_dis_fix, dim3(64), pblk, 0, st, G, v, B.stale, B.stales, cnt, B.dis);
+        return cnt;
     };
     for (int k = 1; k <= nsub; ++k) {
         const double level = levels[k - 1];
         // first sweep: (vof, phi) -> (vt, pt), implicit divergence
         recon(vof, phi, false);
-        const SweepIn s1{vof, phi, nx, ny, B.dis};
+        const SweepIn s1{vof, phi, P};
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_sweep<true, true>), grd, blk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
                                B.tl, B.vt, B.pt);
@@ -1120,7 +1279,7 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, B.vt, B.pt);
         // second sweep: (vt, pt) -> (vof, phi), explicit divergence
         recon(B.vt, B.pt, false);
-        const SweepIn s2{B.vt, B.pt, nx, ny, B.dis};
+        const SweepIn s2{B.vt, B.pt, P};
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_sweep<false, false>), grd, blk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
                                nullptr, vof, phi);
@@ -1128,8 +1287,12 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
             hipLaunchKernelGGL((k_sweep<true, false>), grd, blk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
                                nullptr, vof, phi);
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, vof, phi);
-        recon(vof, phi, true);
-        hipLaunchKernelGGL(k_redistance, grd, blk, 0, st, G, vof, nx, ny, B.dis, B.mask, B.mw, phi);
+        // reconstruction (full normals out) and re-distancing
+        unsigned* cnt = recon(vof, phi, true);
+        const PairView pv{B.srcs, cnt};
+        hipLaunchKernelGGL(k_rd_reset, pgrd, pblk, 0, st, G, vof, pv, B.best, B.hit);
+        hipLaunchKernelGGL(k_rd_scatter, pgrd, pblk, 0, st, G, vof, phi, P, pv, B.best, B.hit);
+        hipLaunchKernelGGL(k_redistance, grd, blk, 0, st, G, vof, P, B.mask, B.mw, B.best, B.hit, phi);
     }
     SG_HIP(hipGetLastError());
     return SG_OK;

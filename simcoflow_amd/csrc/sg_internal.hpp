@@ -118,6 +118,12 @@ struct ClsBuffers {
     double* tv = nullptr;   // temvf, temls of the first sweep (explicit term)
     double* tl = nullptr;
     unsigned char* stale = nullptr;    // plane constant left unassigned
+    unsigned char* hit = nullptr;      // re-distancing: two-phase rule applied
+    unsigned long long* best = nullptr;  // re-distancing: nearest distance (bits)
+    int* srcs = nullptr;               // listed interface cells (re-distancing sources)
+    int* stales = nullptr;             // listed cells with a stale plane constant
+    unsigned* cnt = nullptr;           // list lengths, two per reconstruction
+    static constexpr int CNT_SLOTS = 6 * 1024;
     unsigned long long* mask = nullptr;  // interface cells, 1 bit per cell
     int mw = 0;                        // mask words per row
     double xw = 0.0;                   // west edge x of the grid (host copy)
