@@ -71,6 +71,12 @@ enum sg_field {
     SG_F_AE, SG_F_AW, SG_F_AN, SG_F_AS,
     /* int32 cell arrays (ni*nj): UCell/VCell Posnu/=-1 .and. MoExCell/=1 */
     SG_F_UFLAG, SG_F_VFLAG,
+    /* momentum predictor, P-cell arrays (ni*nj): the P interface at time n
+     * (PCellO%vof/phi/nx/ny), the U/V liquid fractions after the transport
+     * (UCell/VCell%vof) and at time n (UCellO/VCellO%vof) */
+    SG_F_VOFO, SG_F_PHIO, SG_F_NXO, SG_F_NYO, SG_F_UVOF, SG_F_VVOF, SG_F_UVOFO, SG_F_VVOFO,
+    /* ghosted: Flux_n1(:,:,1) and Flux_n1(:,:,2) (Solver_Mod.f90:49-50) */
+    SG_F_FLUXN1U, SG_F_FLUXN1V,
     SG_F_COUNT
 };
 
@@ -127,6 +133,39 @@ int sg_poisson_apply(sg_ctx* ctx, const double* x, double* y);
 /* device fields in: UPRED, VPRED, PP, DPU, DPV, VOF, PHI + geometry;
  * in/out: U, V, P; out: MRES. */
 int sg_velocity_correct_resident(sg_ctx* ctx, double vb);
+
+/* ---- momentum predictor (replaces Predictor_UV, PredictorUV_mod.f90:28-497) --
+ * Geometry of the U (which = 0) or V (which = 1) cells, computed once by the
+ * host's cut-cell set-up (NewCellFace, Cutcell_Mod.f90:399-725).  Cell arrays
+ * (ni*nj): vofS, nxS, nyS, phiS, WlLh, delh, Cell_Cent x/y (ccx, ccy),
+ * MoExCell (moex), active (1 where Posnu/=-1: a row of the momentum system;
+ * the U cells of column ni and the V cells of row nj are never active).
+ * Ghosted arrays ((ni+2)*(nj+2)): the four edge apertures, EtaE, EtaN, AlE,
+ * AlN, DAlE, DAlN, SxE, SyN and FCE / FCN (x, y). */
+typedef struct sg_stag_geometry {
+    const double *vofS, *nxS, *nyS, *phiS, *WlLh, *delh, *ccx, *ccy;
+    const int *moex, *active;
+    const double *aE, *aW, *aN, *aS, *EtaE, *EtaN, *AlE, *AlN, *DAlE, *DAlN, *SxE, *SyN;
+    const double *fcex, *fcey, *fcnx, *fcny;
+} sg_stag_geometry;
+
+typedef struct sg_predictor_info {
+    int iterations_u, iterations_v;  /* BiCGStab iterations of the two systems */
+    double relres_u, relres_v;       /* final ||D^-1(b-Ax)||/||D^-1 b||, D = diag(A) */
+    double ms;                       /* device time of the whole call */
+} sg_predictor_info;
+
+int sg_set_staggered_geometry(sg_ctx* ctx, int which, const sg_stag_geometry* g);
+/* device fields in: U, V, VOFO, PHIO, NXO, NYO, UVOF, VVOF, UVOFO, VVOFO,
+ * VOFS, NXS, NYS, PHIS + staggered geometry; in/out: FLUXN1U/V, DPU, DPV
+ * (active cells and ghosts), UPRED, VPRED (rows 1..ni-1 / 1..nj-1 and the
+ * open top row of v).  first_step: itt == 1 (first-order convective
+ * fluxes, no Adams-Bashforth).  The two momentum systems are solved to
+ * ||D^-1(b-Ax)|| <= 1e-13 ||D^-1 b|| (the reference runs 50 PCG iterations). */
+int sg_predictor_resident(sg_ctx* ctx, double dt, double vb, int first_step, sg_predictor_info* info);
+/* test hook: the assembled U and V rows (matr layout ni*nj*5: S,W,diag,E,N)
+ * and right-hand sides of the last sg_predictor_resident call */
+int sg_predictor_rows(sg_ctx* ctx, double* matr_u, double* rhs_u, double* matr_v, double* rhs_v);
 
 /* ---- interface transport (replaces Coupled_LS_VOF, Clsvof_Mod.f90:730-959) --
  * nsub sub-steps of alternating x/y PLIC volume sweeps (X_Sweep/Y_Sweep

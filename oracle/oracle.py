@@ -305,3 +305,117 @@ def clsvof_advect(cg, vof, phi, u, v, dt, time=0.0, nsub=6, wave=(1.0e30, 0.0, 0
     _cls_lib().orc_clsvof_advect(cg.ref, _d(vof), _d(phi), _d(nx), _d(ny), _d(np.ascontiguousarray(u)),
                                  _d(np.ascontiguousarray(v)), dt, time, nsub, ctypes.byref(w))
     return vof, phi, nx, ny
+
+
+# ---------------------------------------------------------------------------
+# Momentum predictor (PredictorUV_mod.f90)
+_STAG_D1 = ["vofS", "nxS", "nyS", "phiS", "WlLh", "delh", "ccx", "ccy"]
+_STAG_I = ["moex", "active"]
+_STAG_D2 = ["vof_old", "vof_new", "aE", "aW", "aN", "aS", "EtaE", "EtaN", "AlE", "AlN", "DAlE", "DAlN",
+            "SxE", "SyN", "fcex", "fcey", "fcnx", "fcny"]
+
+
+class _Stag(ctypes.Structure):
+    _fields_ = ([(k, _D) for k in _STAG_D1] + [("moex", _I), ("flag", _I)] +
+                [(k, _D) for k in _STAG_D2])
+
+
+class _PredIn(ctypes.Structure):
+    _fields_ = ([("ni", ctypes.c_int), ("nj", ctypes.c_int)] +
+                [(k, _D) for k in ("dx", "dy", "udx", "vdy")] +
+                [("U", _Stag), ("V", _Stag)] +
+                [(k, _D) for k in ("pvof", "pphi", "pnx", "pny", "pvofS", "pnxS", "pnyS", "pphiS", "u", "v")] +
+                [(k, ctypes.c_double) for k in ("rho_l", "rho_g", "rho_ref", "nu_l", "nu_g", "nu_ref", "Re", "Fr",
+                                                "dt", "vb")] +
+                [("first_step", ctypes.c_int)])
+
+
+_pred_declared = False
+
+
+def _pred_lib():
+    global _pred_declared
+    L = lib()
+    if not _pred_declared:
+        L.orc_predictor_assemble.argtypes = [ctypes.POINTER(_PredIn)] + [_D] * 7
+        L.orc_predictor_finish.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
+        _pred_declared = True
+    return L
+
+
+def _stag_struct(st, vof_old, vof_new, keep):
+    c = np.ascontiguousarray
+    args = {}
+    for k in _STAG_D1:
+        a = c(getattr(st, k), dtype=np.float64); keep.append(a); args[k] = _d(a)
+    mo = c(st.moex, dtype=np.int32); fl = c(st.active, dtype=np.int32)
+    keep += [mo, fl]
+    vo = c(vof_old, dtype=np.float64); vn = c(vof_new, dtype=np.float64)
+    keep += [vo, vn]
+    rest = {}
+    for k in _STAG_D2[2:]:
+        a = c(getattr(st, k), dtype=np.float64); keep.append(a); rest[k] = _d(a)
+    return _Stag(*[args[k] for k in _STAG_D1], _i(mo), _i(fl), _d(vo), _d(vn), *[rest[k] for k in _STAG_D2[2:]])
+
+
+def predictor_assemble(g, Ust, Vst, uvof_old, uvof_new, vvof_old, vvof_new, pcell, u, v, phys, dt, vb=0.0,
+                       first_step=False, fluxn1=None, DpU=None, DpV=None):
+    """Predictor_UV up to the linear systems.  pcell: dict with the P-cell
+    arrays at time n (vof, phi, nx, ny, vofS, nxS, nyS, phiS); phys: dict
+    rho_l, rho_g, rho_ref, nu_l, nu_g, nu_ref, Re, Fr.  Returns dict with
+    fluxn1 (2, nj+2, ni+2), matr_u / matr_v (5, nj, ni), rhs_u / rhs_v,
+    DpU / DpV (ghosted; active cells updated)."""
+    c = np.ascontiguousarray
+    ni, nj = g.ni, g.nj
+    keep = []
+    gx = [c(a, dtype=np.float64) for a in (g.dx, g.dy, g.udx, g.vdy)]
+    keep += gx
+    Us = _stag_struct(Ust, uvof_old, uvof_new, keep)
+    Vs = _stag_struct(Vst, vvof_old, vvof_new, keep)
+    pk = [c(pcell[k], dtype=np.float64) for k in ("vof", "phi", "nx", "ny", "vofS", "nxS", "nyS", "phiS")]
+    uv = [c(u, dtype=np.float64), c(v, dtype=np.float64)]
+    keep += pk + uv
+    s = _PredIn(ni, nj, *[_d(a) for a in gx], Us, Vs, *[_d(a) for a in pk], *[_d(a) for a in uv],
+                *[float(phys[k]) for k in ("rho_l", "rho_g", "rho_ref", "nu_l", "nu_g", "nu_ref", "Re", "Fr")],
+                float(dt), float(vb), int(bool(first_step)))
+    f1 = np.zeros((2, nj + 2, ni + 2)) if fluxn1 is None else np.array(fluxn1, dtype=np.float64, order="C")
+    DU = np.zeros((nj + 2, ni + 2)) if DpU is None else np.array(DpU, dtype=np.float64, order="C")
+    DV = np.zeros((nj + 2, ni + 2)) if DpV is None else np.array(DpV, dtype=np.float64, order="C")
+    mu = np.zeros((5, nj, ni)); ru = np.zeros((nj, ni))
+    mv = np.zeros((5, nj, ni)); rv = np.zeros((nj, ni))
+    _pred_lib().orc_predictor_assemble(ctypes.byref(s), _d(f1), _d(mu), _d(ru), _d(mv), _d(rv), _d(DU), _d(DV))
+    return dict(fluxn1=f1, matr_u=mu, rhs_u=ru, matr_v=mv, rhs_v=rv, DpU=DU, DpV=DV)
+
+
+def predictor_finish(g, DpU, DpV, vpred):
+    DU = np.array(DpU, dtype=np.float64, order="C")
+    DV = np.array(DpV, dtype=np.float64, order="C")
+    vp = np.array(vpred, dtype=np.float64, order="C")
+    _pred_lib().orc_predictor_finish(g.ni, g.nj, _d(DU), _d(DV), _d(vp))
+    return DU, DV, vp
+
+
+def momentum_solve(matr, rhs, active):
+    """Direct sparse solve of one momentum system in the matr layout
+    (S,W,diag,E,N); inactive cells are 0 (DeltaGetValues,
+    PredictorUV_mod.f90:847-885)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    nj, ni = rhs.shape
+    idx = -np.ones((nj, ni), dtype=np.int64)
+    jj, ii = np.nonzero(active)
+    idx[jj, ii] = np.arange(len(jj))
+    rows, cols, vals = [], [], []
+    offs = {0: (-1, 0), 1: (0, -1), 3: (0, 1), 4: (1, 0)}
+    for k, (dj, di) in offs.items():
+        m = matr[k]
+        sel = (m != 0) & (active != 0)
+        j2, i2 = np.nonzero(sel)
+        rows.append(idx[j2, i2]); cols.append(idx[j2 + dj, i2 + di]); vals.append(m[j2, i2])
+    rows.append(idx[jj, ii]); cols.append(idx[jj, ii]); vals.append(matr[2][jj, ii])
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(len(jj), len(jj)))
+    x = spl.spsolve(A.tocsc(), rhs[jj, ii])
+    out = np.zeros((nj, ni))
+    out[jj, ii] = x
+    return out

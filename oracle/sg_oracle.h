@@ -151,6 +151,45 @@ double orc_inflow_level(const orc_cls_geom* g, const orc_wave* w, double time, i
 void orc_clsvof_advect(const orc_cls_geom* g, double* vof, double* phi, double* nx, double* ny,
                        const double* u, const double* v, double dt, double time, int nv, const orc_wave* w);
 
+/* ---- momentum predictor (PredictorUV_mod.f90) ------------------------------ */
+
+/* One staggered grid (U or V cells).  Cell arrays (1:ni,1:nj): vofS, nxS,
+ * nyS, phiS, WlLh, delh, Cell_Cent (ccx, ccy), MoExCell (moex), the active
+ * flag (Posnu/=-1: a row of the momentum system), the vof at time n (the
+ * U/VCellO%vof) and after the transport (U/VCell%vof).  Ghosted arrays
+ * (0:ni+1,0:nj+1): edge apertures, EtaE/N, AlE/N, DAlE/N, SxE, SyN and the
+ * face centres FCE (fcex, fcey) and FCN (fcnx, fcny). */
+typedef struct orc_stag {
+    const double *vofS, *nxS, *nyS, *phiS, *WlLh, *delh, *ccx, *ccy;
+    const int *moex, *flag;
+    const double *vof_old, *vof_new;
+    const double *aE, *aW, *aN, *aS;
+    const double *EtaE, *EtaN, *AlE, *AlN, *DAlE, *DAlN, *SxE, *SyN;
+    const double *fcex, *fcey, *fcnx, *fcny;
+} orc_stag;
+
+typedef struct orc_pred_in {
+    int ni, nj;
+    const double *dx, *dy, *udx, *vdy;  /* P spacings, U-cell dx, V-cell dy */
+    orc_stag U, V;
+    /* P cells at time n (PCellO) */
+    const double *pvof, *pphi, *pnx, *pny, *pvofS, *pnxS, *pnyS, *pphiS;
+    const double *u, *v;                /* ghosted TVar%u, TVar%v */
+    double rho_l, rho_g, rho_ref, nu_l, nu_g, nu_ref, Re, Fr;
+    double dt, vb;
+    int first_step;                     /* itt == 1 */
+} orc_pred_in;
+
+/* Predictor_UV up to the two linear systems (PredictorUV_mod.f90:28-497):
+ * U and V momentum rows (matr layout, 5 planes S,W,diag,E,N of ni*nj),
+ * right-hand sides (ni*nj), PU%Dp / PV%Dp on active cells (ghosted, in/out)
+ * and Flux_n1 (ghosted, 2 planes, in/out). */
+void orc_predictor_assemble(const orc_pred_in* in, double* fluxn1, double* matr_u, double* rhs_u,
+                            double* matr_v, double* rhs_v, double* DpU, double* DpV);
+/* ghost copies of PU%Dp / PV%Dp (:530-540) and Pred%v(:,nj) = Pred%v(:,nj-1)
+ * (PredictorVelocityBoundaryCondition :2199-2213) */
+void orc_predictor_finish(int ni, int nj, double* DpU, double* DpV, double* vpred);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,10 +19,12 @@ SG_PREC_MIXED = 1
 # enum sg_field
 FIELDS = ["VOF", "PHI", "VOFS", "NYS", "WLLH", "NXS", "PHIS", "NX", "NY", "MRES", "VOFSU", "VOFSV",
           "U", "V", "P", "UPRED", "VPRED", "PP", "DPU", "DPV", "AE", "AW", "AN", "AS",
-          "UFLAG", "VFLAG"]
+          "UFLAG", "VFLAG",
+          "VOFO", "PHIO", "NXO", "NYO", "UVOF", "VVOF", "UVOFO", "VVOFO",
+          "FLUXN1U", "FLUXN1V"]
 F = {name: k for k, name in enumerate(FIELDS)}
-CELL_FIELDS = set(FIELDS[:12])
-GHOST_FIELDS = set(FIELDS[12:24])
+CELL_FIELDS = set(FIELDS[:12]) | set(FIELDS[26:34])
+GHOST_FIELDS = set(FIELDS[12:24]) | {"FLUXN1U", "FLUXN1V"}
 INT_FIELDS = {"UFLAG", "VFLAG"}
 
 
@@ -43,6 +45,26 @@ class SolveInfo(ctypes.Structure):
         ("iterations", ctypes.c_int), ("status", ctypes.c_int), ("relres", ctypes.c_double),
         ("max_row_residual", ctypes.c_double), ("ms", ctypes.c_double), ("setup_ms", ctypes.c_double),
     ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+STAG_CELL = ["vofS", "nxS", "nyS", "phiS", "WlLh", "delh", "ccx", "ccy"]
+STAG_INT = ["moex", "active"]
+STAG_GHOST = ["aE", "aW", "aN", "aS", "EtaE", "EtaN", "AlE", "AlN", "DAlE", "DAlN", "SxE", "SyN",
+              "fcex", "fcey", "fcnx", "fcny"]
+
+
+class StagGeometry(ctypes.Structure):
+    _fields_ = ([(k, ctypes.POINTER(ctypes.c_double)) for k in STAG_CELL] +
+                [(k, ctypes.POINTER(ctypes.c_int)) for k in STAG_INT] +
+                [(k, ctypes.POINTER(ctypes.c_double)) for k in STAG_GHOST])
+
+
+class PredictorInfo(ctypes.Structure):
+    _fields_ = [("iterations_u", ctypes.c_int), ("iterations_v", ctypes.c_int),
+                ("relres_u", ctypes.c_double), ("relres_v", ctypes.c_double), ("ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -85,6 +107,10 @@ PROTOTYPES = {
     "sg_clsvof_advect": ([_CTX, _D, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double,
                           ctypes.POINTER(ClsvofParams)], ctypes.c_int),
     "sg_time_kernel": ([_CTX, ctypes.c_int, ctypes.c_int, _D], ctypes.c_int),
+    "sg_set_staggered_geometry": ([_CTX, ctypes.c_int, ctypes.POINTER(StagGeometry)], ctypes.c_int),
+    "sg_predictor_resident": ([_CTX, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                               ctypes.POINTER(PredictorInfo)], ctypes.c_int),
+    "sg_predictor_rows": ([_CTX, _D, _D, _D, _D], ctypes.c_int),
 }
 
 _lib = None

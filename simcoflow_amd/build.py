@@ -25,10 +25,13 @@ SOURCES = [
     "sg_pcg.hip",
     "sg_cycle.hip",
     "sg_clsvof.hip",
+    "sg_predictor.hip",
+    "sg_momentum.hip",
 ]
 EXTRA = {
     "sg_poisson.hip": ["-ffp-contract=off"],
     "sg_clsvof.hip": ["-ffp-contract=off"],
+    "sg_predictor.hip": ["-ffp-contract=off"],
     "sg_api.hip": ["-ffp-contract=off"],
 }
 COMMON = [

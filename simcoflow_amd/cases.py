@@ -415,6 +415,8 @@ class Case:
     dt: float
     tol: float
     bc: tuple = (1, 1, 1, 0)   # W,E,S,N: Neumann W/E/S, Dirichlet N (ProjectionP_Mod.f90:55)
+    solid: object = None       # solid(X, Y) -> (phiS, nxS, nyS), or None
+    levelset: object = None    # levelset(X, Y) -> (phi, nx, ny), liquid where phi < 0
 
 
 def _gas_bubble(cx, cy, r):
@@ -430,7 +432,7 @@ def case_bubble(ni=128, nj=256):
     geo = build_geometry(g, None)
     itf = interface_from_levelset(g, geo, _gas_bubble(0.5, 0.5, 0.25))
     h = g.dx[0]
-    return Case("bubble", g, Fluids(), geo, itf, dt=0.25 * h, tol=1e-10)
+    return Case("bubble", g, Fluids(), geo, itf, dt=0.25 * h, tol=1e-10, levelset=_gas_bubble(0.5, 0.5, 0.25))
 
 
 def case_dambreak(ni=512, nj=256):
@@ -444,7 +446,7 @@ def case_dambreak(ni=512, nj=256):
     def ls(X, Y):
         return sdf_box(X, Y, 0.5, 0.5, 0.5, 0.5)  # > 0 outside the water column
     itf = interface_from_levelset(g, geo, ls)
-    return Case("dambreak", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-10)
+    return Case("dambreak", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-10, solid=solid, levelset=ls)
 
 
 def swarm_bubbles(seed=0, n=64, depth=0.8, rmin=0.01, rmax=0.03, ymin=0.1, ymax=0.6):
@@ -488,7 +490,7 @@ def case_swarm(ni=2048, nj=2048, seed=0):
             ny[sl] = np.where(sel, b, ny[sl])
         return phi, nx, ny
     itf = interface_from_levelset(g, geo, ls)
-    return Case("swarm", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-8)
+    return Case("swarm", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-8, levelset=ls)
 
 
 def case_waves(ni=1024, nj=1024, seed=0):
@@ -518,7 +520,7 @@ def case_waves(ni=1024, nj=1024, seed=0):
         nrm = np.sqrt(1.0 + ds * ds)
         return (Y - s) / nrm, np.broadcast_to(-ds / nrm, X.shape).copy(), np.broadcast_to(1.0 / nrm, X.shape).copy()
     itf = interface_from_levelset(g, geo, ls)
-    return Case("waves", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-8)
+    return Case("waves", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-8, solid=solid, levelset=ls)
 
 
 CASES = {"bubble": case_bubble, "dambreak": case_dambreak, "swarm": case_swarm, "waves": case_waves}
@@ -601,3 +603,187 @@ def transport_velocity(g: Grid, kind="rotation", amp=1.0, seed=0, walls=True):
         v[0, :] = 0.0
         v[nj, :] = 0.0
     return np.ascontiguousarray(u), np.ascontiguousarray(v)
+
+
+# ---------------------------------------------------------------------------
+# Staggered (U / V) cell geometry for the momentum predictor.
+@dataclass
+class StagGeometry:
+    """Cut-cell geometry of one staggered grid, in the arrays the predictor
+    reads (PredictorUV_mod.f90).  Cell arrays (nj, ni): vofS, nxS, nyS, phiS,
+    WlLh, delh, ccx, ccy (fluid centroid offset from the cell centre), moex,
+    active (a row of the momentum system).  Ghosted arrays (nj+2, ni+2):
+    apertures aE..aS, EtaE, EtaN, AlE, AlN, DAlE, DAlN, SxE, SyN and the
+    open-face centres FCE (fcex, fcey) and FCN (fcnx, fcny), offsets from
+    the cell centre."""
+    vofS: np.ndarray
+    nxS: np.ndarray
+    nyS: np.ndarray
+    phiS: np.ndarray
+    WlLh: np.ndarray
+    delh: np.ndarray
+    ccx: np.ndarray
+    ccy: np.ndarray
+    moex: np.ndarray
+    active: np.ndarray
+    aE: np.ndarray
+    aW: np.ndarray
+    aN: np.ndarray
+    aS: np.ndarray
+    EtaE: np.ndarray
+    EtaN: np.ndarray
+    AlE: np.ndarray
+    AlN: np.ndarray
+    DAlE: np.ndarray
+    DAlN: np.ndarray
+    SxE: np.ndarray
+    SyN: np.ndarray
+    fcex: np.ndarray
+    fcey: np.ndarray
+    fcnx: np.ndarray
+    fcny: np.ndarray
+
+
+def _fluid_centroid(nxS, nyS, phiS, hx, hy):
+    """Centroid of {nxS x + nyS y + phiS >= 0} inside the hx*hy box centred
+    at 0 (polygon clipping); (0,0) when the box is all fluid or all solid."""
+    poly = [(-hx / 2, -hy / 2), (hx / 2, -hy / 2), (hx / 2, hy / 2), (-hx / 2, hy / 2)]
+    out = []
+    for k in range(4):
+        p, q = poly[k], poly[(k + 1) % 4]
+        fp = nxS * p[0] + nyS * p[1] + phiS
+        fq = nxS * q[0] + nyS * q[1] + phiS
+        if fp >= 0:
+            out.append(p)
+        if (fp < 0 <= fq) or (fq < 0 <= fp):
+            t = fp / (fp - fq)
+            out.append((p[0] + t * (q[0] - p[0]), p[1] + t * (q[1] - p[1])))
+    if len(out) < 3:
+        return 0.0, 0.0
+    a = cx = cy = 0.0
+    for k in range(len(out)):
+        p, q = out[k], out[(k + 1) % len(out)]
+        cr = p[0] * q[1] - q[0] * p[1]
+        a += cr
+        cx += (p[0] + q[0]) * cr
+        cy += (p[1] + q[1]) * cr
+    if abs(a) < 1e-300:
+        return 0.0, 0.0
+    return cx / (3.0 * a), cy / (3.0 * a)
+
+
+def _stag(g, hx, hy, phiS, nxS, nyS, vofS, moex, which, spacing_next):
+    """One staggered grid.  hx, hy: (nj, ni) cell sizes; spacing_next: the
+    centre distance to the next cell along x (for SxE) and along y (SyN)."""
+    nj, ni = phiS.shape
+    aE, aW, aN, aS, WlLh, vofS2 = _cell_geo(phiS, nxS, nyS, vofS, hx, hy, closed_rule=False)
+    cut = (vofS2 > EPSI) & (vofS2 < 1.0 - EPSI)
+    ccx = np.zeros((nj, ni))
+    ccy = np.zeros((nj, ni))
+    for j, i in zip(*np.nonzero(cut)):
+        ccx[j, i], ccy[j, i] = _fluid_centroid(nxS[j, i], nyS[j, i], phiS[j, i], hx[j, i], hy[j, i])
+    G = lambda: np.zeros((nj + 2, ni + 2))
+    fcex, fcey, fcnx, fcny = G(), G(), G(), G()
+    fcex[1:-1, 1:-1] = hx / 2
+    fcny[1:-1, 1:-1] = hy / 2
+    # open segment of a cut east / north edge: its centre sits at half the open
+    # length from the fluid end
+    e_open = aE[1:-1, 1:-1]
+    n_open = aN[1:-1, 1:-1]
+    fcey[1:-1, 1:-1] = np.where(cut, (0.5 - e_open / 2.0) * np.sign(nyS) * hy, 0.0)
+    fcnx[1:-1, 1:-1] = np.where(cut, (0.5 - n_open / 2.0) * np.sign(nxS) * hx, 0.0)
+    for a in (fcex, fcey, fcnx, fcny):
+        a[0, :] = a[1, :]
+        a[-1, :] = a[-2, :]
+        a[:, 0] = a[:, 1]
+        a[:, -1] = a[:, -2]
+    active = (vofS2 < 1.0 - EPSI).astype(np.int32)
+    if which == 0:
+        active[:, -1] = 0
+    else:
+        active[-1, :] = 0
+    SxE, EtaE, AlE, DAlE = G(), G(), G(), G()
+    SyN, EtaN, AlN, DAlN = G(), G(), G(), G()
+    sx_next, sy_next = spacing_next
+    SxE[1:-1, 1:-1] = sx_next
+    SyN[1:-1, 1:-1] = sy_next
+    EtaE[:] = 0.5
+    EtaN[:] = 0.5
+    AlE[:] = 1.0
+    AlN[:] = 1.0
+    # cut faces: interpolation factor from the centroids, the shift of the
+    # interpolation point to the face centre (EastFaceInterpolationInf,
+    # NorthFaceInterpolationInf of Cutcell_Mod.f90 in simplified form)
+    for j in range(nj):
+        for i in range(ni):
+            if i + 1 < ni and active[j, i] and active[j, i + 1] and (cut[j, i] or cut[j, i + 1]):
+                sx = ccx[j, i + 1] + sx_next[j, i] - ccx[j, i]
+                sy = ccy[j, i + 1] - ccy[j, i]
+                SxE[j + 1, i + 1] = sx
+                eta = abs(fcex[j + 1, i + 1] - ccx[j, i]) / sx
+                EtaE[j + 1, i + 1] = 0.5 if eta >= 1.0 else eta
+                if abs(sy) > 1e-4 * hy[j, i]:
+                    AlE[j + 1, i + 1] = min(2.0, 1.0 + abs(sy) / max(hy[j, i], 1e-300))
+                    DAlE[j + 1, i + 1] = sy * 0.5 * (nyS[j, i] + nyS[j, i + 1]) / (sx * hy[j, i])
+            if j + 1 < nj and active[j, i] and active[j + 1, i] and (cut[j, i] or cut[j + 1, i]):
+                sy = ccy[j + 1, i] + sy_next[j, i] - ccy[j, i]
+                sx = ccx[j + 1, i] - ccx[j, i]
+                SyN[j + 1, i + 1] = sy
+                eta = abs(fcny[j + 1, i + 1] - ccy[j, i]) / sy
+                EtaN[j + 1, i + 1] = 0.5 if eta >= 1.0 else eta
+                if abs(sx) >= 1e-4 * hx[j, i]:
+                    AlN[j + 1, i + 1] = min(2.0, 1.0 + abs(sx) / max(hx[j, i], 1e-300))
+                    DAlN[j + 1, i + 1] = sx * 0.5 * (nxS[j, i] + nxS[j + 1, i]) / (sy * hx[j, i])
+    for a in (SxE, EtaE, AlE, DAlE, SyN, EtaN, AlN, DAlN):
+        a[0, :] = a[1, :]
+        a[-1, :] = a[-2, :]
+        a[:, 0] = a[:, 1]
+        a[:, -1] = a[:, -2]
+    delh = np.abs(ccx * nxS + ccy * nyS + phiS) + 1.0e-24
+    delh = np.where(moex == 1, 1.0e-20, delh)
+    c = np.ascontiguousarray
+    return StagGeometry(vofS=c(vofS2), nxS=c(nxS), nyS=c(nyS), phiS=c(phiS), WlLh=c(WlLh), delh=c(delh),
+                        ccx=c(ccx), ccy=c(ccy), moex=c(moex.astype(np.int32)), active=c(active),
+                        aE=aE, aW=aW, aN=aN, aS=aS, EtaE=EtaE, EtaN=EtaN, AlE=AlE, AlN=AlN, DAlE=DAlE,
+                        DAlN=DAlN, SxE=SxE, SyN=SyN, fcex=fcex, fcey=fcey, fcnx=fcnx, fcny=fcny)
+
+
+def staggered_geometry(g: Grid, geo: Geometry, solid=None):
+    """U and V cell geometry for the predictor; `solid` as in build_geometry."""
+    ni, nj = g.ni, g.nj
+    DX = np.broadcast_to(g.dx[None, :], (nj, ni))
+    DY = np.broadcast_to(g.dy[:, None], (nj, ni))
+    UDX = np.ascontiguousarray(np.broadcast_to(g.udx[None, :], (nj, ni)))
+    VDY = np.ascontiguousarray(np.broadcast_to(g.vdy[:, None], (nj, ni)))
+    Xp, Yp = np.meshgrid(g.xc, g.yc)
+    Xu, Yv = Xp + 0.5 * DX, Yp + 0.5 * DY
+    if solid is None:
+        zero = np.zeros((nj, ni))
+        planes = [(np.full((nj, ni), 1.0e3), zero.copy(), np.ones((nj, ni)))] * 2
+    else:
+        planes = [solid(Xu, Yp), solid(Xp, Yv)]
+    # centre distance to the next cell: U cells along x = the P spacing of
+    # cell i+1, along y = dy; V cells along x = the U spacing, along y = dy
+    # of cell j+1 (Cutcell_Mod.f90 EastFaceInterpolationInf BGrid%dx)
+    dxn = np.append(g.dx[1:], g.dx[-1])[None, :].repeat(nj, 0)
+    dyn = np.append(g.dy[1:], g.dy[-1])[:, None].repeat(ni, 1)
+    U = _stag(g, UDX, np.ascontiguousarray(DY), planes[0][0], planes[0][1], planes[0][2], geo.vofSU.copy(),
+              geo.moexU, 0, (dxn, np.ascontiguousarray(VDY)))
+    V = _stag(g, np.ascontiguousarray(DX), VDY, planes[1][0], planes[1][1], planes[1][2], geo.vofSV.copy(),
+              geo.moexV, 1, (UDX, dyn))
+    return U, V
+
+
+def staggered_vof(g: Grid, stag: StagGeometry, which, levelset):
+    """Liquid fraction of U (which=0) or V cells from a level set (liquid
+    where phi < 0), scaled by the fluid fraction."""
+    ni, nj = g.ni, g.nj
+    DX = np.broadcast_to(g.dx[None, :], (nj, ni))
+    DY = np.broadcast_to(g.dy[:, None], (nj, ni))
+    Xp, Yp = np.meshgrid(g.xc, g.yc)
+    if which == 0:
+        X, Y, hx, hy = Xp + 0.5 * DX, Yp, np.broadcast_to(g.udx[None, :], (nj, ni)), DY
+    else:
+        X, Y, hx, hy = Xp, Yp + 0.5 * DY, DX, np.broadcast_to(g.vdy[:, None], (nj, ni))
+    phi, nx, ny = levelset(X, Y)
+    return np.ascontiguousarray(frac(nx, ny, phi, hx, hy) * (1.0 - stag.vofS))

@@ -31,8 +31,10 @@ void* dev_alloc(Ctx* c, size_t bytes) {
     return p;
 }
 
-static bool is_cell_field(int f) { return f >= SG_F_VOF && f <= SG_F_VOFSV; }
-static bool is_ghost_field(int f) { return f >= SG_F_U && f <= SG_F_AS; }
+static bool is_cell_field(int f) {
+    return (f >= SG_F_VOF && f <= SG_F_VOFSV) || (f >= SG_F_VOFO && f <= SG_F_VVOFO);
+}
+static bool is_ghost_field(int f) { return (f >= SG_F_U && f <= SG_F_AS) || f == SG_F_FLUXN1U || f == SG_F_FLUXN1V; }
 static bool is_int_field(int f) { return f == SG_F_UFLAG || f == SG_F_VFLAG; }
 
 static void destroy(Ctx* c) {
@@ -42,6 +44,7 @@ static void destroy(Ctx* c) {
         if (g) (void)hipGraphExecDestroy(g);
     for (void* p : c->allocations) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
+    if (c->pred.sc_host) (void)hipHostFree(c->pred.sc_host);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -162,6 +165,7 @@ int sg_ctx_create(int ni, int nj, const double* dx, const double* dy, const doub
         if (!*v) return fail_alloc();
     }
     if (clsvof_allocate(c) != SG_OK) return fail_alloc();
+    if (predictor_allocate(c) != SG_OK) return fail_alloc();
     {
         // default cell centres: faces from 0 by the spacings (Mesh_Mod.f90:123-135)
         std::vector<double> xc(ni), yc(nj);
@@ -386,6 +390,89 @@ int sg_velocity_correct_resident(sg_ctx* ctx, double vb) {
     c->vb = vb;
     int rc = velocity_correct(c);
     if (rc) return rc;
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_set_staggered_geometry(sg_ctx* ctx, int which, const sg_stag_geometry* g) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !g || (which != 0 && which != 1)) {
+        set_error("sg_set_staggered_geometry: bad argument");
+        return SG_ERR_ARG;
+    }
+    const double* src[STG_COUNT] = {g->vofS, g->nxS, g->nyS, g->phiS, g->WlLh, g->delh, g->ccx, g->ccy,
+                                    g->aE, g->aW, g->aN, g->aS, g->EtaE, g->EtaN, g->AlE, g->AlN,
+                                    g->DAlE, g->DAlN, g->SxE, g->SyN, g->fcex, g->fcey, g->fcnx, g->fcny};
+    for (int k = 0; k < STG_COUNT; ++k)
+        if (!src[k]) {
+            set_error("sg_set_staggered_geometry: null array");
+            return SG_ERR_ARG;
+        }
+    if (!g->moex || !g->active) {
+        set_error("sg_set_staggered_geometry: null array");
+        return SG_ERR_ARG;
+    }
+    const size_t n = (size_t)c->ni * c->nj, ng = (size_t)(c->ni + 2) * (c->nj + 2);
+    StagDev& S = c->stag[which];
+    for (int k = 0; k < STG_COUNT; ++k)
+        SG_HIP(hipMemcpyAsync(S.a[k], src[k], (k < STG_NCELL ? n : ng) * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    SG_HIP(hipMemcpyAsync(S.moex, g->moex, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // the reference numbers U cells of columns 1..ni-1 and V cells of rows
+    // 1..nj-1 only (NumberExternalCell, Cutcell_Mod.f90:340-358)
+    std::vector<int> act(g->active, g->active + n);
+    for (int j = 0; j < c->nj; ++j)
+        for (int i = 0; i < c->ni; ++i)
+            if ((which == 0 && i == c->ni - 1) || (which == 1 && j == c->nj - 1)) act[(size_t)i + (size_t)j * c->ni] = 0;
+    SG_HIP(hipMemcpyAsync(S.act, act.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_predictor_resident(sg_ctx* ctx, double dt, double vb, int first_step, sg_predictor_info* info) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    if (!(dt > 0.0)) {
+        set_error("sg_predictor_resident: dt must be positive");
+        return SG_ERR_ARG;
+    }
+    if (c->ni < 4 || c->nj < 4) {
+        set_error("sg_predictor_resident: needs ni, nj >= 4");
+        return SG_ERR_ARG;
+    }
+    SG_HIP(hipEventRecord(c->ev0, c->stream));
+    int rc = predictor_assemble(c, dt, vb, first_step);
+    if (rc) return rc;
+    sg_predictor_info loc{};
+    const double tol = 1.0e-13;
+    rc = momentum_solve(c, c->pred.mu, c->pred.ru, c->pred.xu, tol, 500, &loc.iterations_u, &loc.relres_u);
+    if (rc < 0) return rc;
+    int rc2 = momentum_solve(c, c->pred.mv, c->pred.rv, c->pred.xv, tol, 500, &loc.iterations_v, &loc.relres_v);
+    if (rc2 < 0) return rc2;
+    int rc3 = predictor_finish(c);
+    if (rc3) return rc3;
+    SG_HIP(hipEventRecord(c->ev1, c->stream));
+    SG_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    SG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    loc.ms = ms;
+    if (info) *info = loc;
+    c->built = 0;
+    if (rc || rc2) {
+        set_error("sg_predictor_resident: a momentum system did not converge");
+        return SG_NOT_CONVERGED;
+    }
+    return SG_OK;
+}
+
+int sg_predictor_rows(sg_ctx* ctx, double* matr_u, double* rhs_u, double* matr_v, double* rhs_v) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !matr_u || !rhs_u || !matr_v || !rhs_v) return SG_ERR_ARG;
+    const size_t n = (size_t)c->ni * c->nj;
+    SG_HIP(hipMemcpyAsync(matr_u, c->pred.mu, 5 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipMemcpyAsync(rhs_u, c->pred.ru, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipMemcpyAsync(matr_v, c->pred.mv, 5 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipMemcpyAsync(rhs_v, c->pred.rv, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
     return SG_OK;
 }

@@ -130,6 +130,41 @@ struct ClsBuffers {
     double dx0 = 0.0;                  // width of the first column (host copy)
 };
 
+// Staggered-cell geometry on the device: cell arrays first, then ghosted.
+enum {
+    STG_VOFS = 0, STG_NXS, STG_NYS, STG_PHIS, STG_WLLH, STG_DELH, STG_CCX, STG_CCY,
+    STG_NCELL,
+    STG_AE = STG_NCELL, STG_AW, STG_AN, STG_AS, STG_ETAE, STG_ETAN, STG_ALE, STG_ALN,
+    STG_DALE, STG_DALN, STG_SXE, STG_SYN, STG_FCEX, STG_FCEY, STG_FCNX, STG_FCNY,
+    STG_COUNT
+};
+struct StagDev {
+    double* a[STG_COUNT] = {nullptr};
+    int* moex = nullptr;
+    int* act = nullptr;  // row of the momentum system
+};
+
+// Device scalars of the momentum BiCGStab.
+struct BiScalars {
+    double rho, rho_old, alpha, omega, beta, rv, ts, tt, rr, bb;
+};
+
+// Work arrays of the momentum predictor.
+struct PredBuffers {
+    static constexpr int NFACE = 22;
+    double* face[NFACE] = {nullptr};  // face fluxes (ghosted planes)
+    double *gu = nullptr, *gv = nullptr;   // FluxDiv before mixing
+    double *fdu = nullptr, *fdv = nullptr; // FluxDiv
+    double *mu = nullptr, *mv = nullptr;   // momentum rows (5 planes)
+    double *ru = nullptr, *rv = nullptr;   // right-hand sides
+    double *xu = nullptr, *xv = nullptr;   // solutions
+    double* vec[8] = {nullptr};            // BiCGStab vectors
+    double* partials = nullptr;            // 4 planes of BI_BLOCKS
+    BiScalars* sc = nullptr;               // device
+    BiScalars* sc_host = nullptr;          // pinned
+    static constexpr int BI_BLOCKS = 2048;
+};
+
 struct Ctx {
     int ni = 0, nj = 0;
     sg_params prm{};
@@ -163,6 +198,8 @@ struct Ctx {
     int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0};
     int use_graph = 1;
     ClsBuffers cls;
+    StagDev stag[2];  // U, V cells
+    PredBuffers pred;
     std::vector<void*> allocations;
 };
 
@@ -205,6 +242,12 @@ int mg_launch_fine(Ctx* c, int which);  // measurement: 1 down, 2 up, 3 copy
 int pcg_solve(Ctx* c, sg_solve_info* info);
 int fine_residual_norms(Ctx* c, double* relres, double* maxrow);
 
+// momentum predictor (sg_predictor.hip, sg_momentum.hip)
+int predictor_allocate(Ctx* c);
+int predictor_assemble(Ctx* c, double dt, double vb, int first_step);
+int predictor_finish(Ctx* c);
+int momentum_solve(Ctx* c, const double* m, const double* b, double* x, double tol, int max_iter, int* iters,
+                   double* relres);
 // interface transport (sg_clsvof.hip)
 int clsvof_allocate(Ctx* c);
 int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels);

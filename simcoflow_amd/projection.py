@@ -207,3 +207,38 @@ class Projection:
                                         nx.ctypes.data_as(_DP), ny.ctypes.data_as(_DP), up, vp, float(dt),
                                         float(time), ctypes.byref(p)))
         return vof, phi, nx, ny
+
+    # -- momentum predictor ----------------------------------------------------
+    def set_staggered_geometry(self, which, st):
+        """U (which=0) or V (which=1) cell geometry (a cases.StagGeometry)."""
+        keep = []
+        cell = []
+        for k in _lib.STAG_CELL:
+            a, ptr = _d(getattr(st, k))
+            keep.append(a)
+            cell.append(ptr)
+        ints = []
+        for k in _lib.STAG_INT:
+            a, ptr = _i(getattr(st, k))
+            keep.append(a)
+            ints.append(ptr)
+        gh = []
+        for k in _lib.STAG_GHOST:
+            a, ptr = _d(getattr(st, k))
+            keep.append(a)
+            gh.append(ptr)
+        g = _lib.StagGeometry(*cell, *ints, *gh)
+        check(self.lib.sg_set_staggered_geometry(self.ctx, int(which), ctypes.byref(g)))
+
+    def predictor_resident(self, dt, vb=0.0, first_step=False):
+        info = _lib.PredictorInfo()
+        check(self.lib.sg_predictor_resident(self.ctx, float(dt), float(vb), int(bool(first_step)),
+                                             ctypes.byref(info)))
+        return info
+
+    def predictor_rows(self):
+        mu = np.zeros((5, self.nj, self.ni)); ru = np.zeros((self.nj, self.ni))
+        mv = np.zeros((5, self.nj, self.ni)); rv = np.zeros((self.nj, self.ni))
+        check(self.lib.sg_predictor_rows(self.ctx, mu.ctypes.data_as(_DP), ru.ctypes.data_as(_DP),
+                                         mv.ctypes.data_as(_DP), rv.ctypes.data_as(_DP)))
+        return mu, ru, mv, rv

@@ -1,0 +1,103 @@
+"""Inputs of the momentum predictor for tests: a case's P-cell state at time
+n, U/V cell geometry and liquid fractions, face velocities, Flux_n1 and the
+previous Dp, all seeded."""
+import numpy as np
+
+from simcoflow_amd import cases as C
+
+
+def phys_of(fl, Re=None, Fr=None):
+    """Non-dimensional groups with Uref = 1 m/s, Lref = 1 m
+    (StateVariables.f90: Re = Uref Lref / nuref, Fr = Uref / sqrt(g Lref))."""
+    nu_l = fl.mu_l / fl.rho_l
+    return dict(rho_l=fl.rho_l, rho_g=fl.rho_g, rho_ref=fl.rho_ref, nu_l=nu_l, nu_g=fl.mu_g / fl.rho_g,
+                nu_ref=nu_l, Re=1.0 / nu_l if Re is None else Re, Fr=1.0 / np.sqrt(fl.g) if Fr is None else Fr)
+
+
+def predictor_inputs(case, seed=0, amp=0.4, Re=None, shift=0.3):
+    g = case.grid
+    rng = np.random.default_rng(seed)
+    U, V = C.staggered_geometry(g, case.geo, case.solid)
+    # vof at time n and after a small shift of the interface (the transport)
+    ls = case.levelset
+
+    def shifted(X, Y):
+        return ls(X - shift * g.dx[0], Y + 0.5 * shift * g.dy[0])
+    uvo = C.staggered_vof(g, U, 0, ls)
+    vvo = C.staggered_vof(g, V, 1, ls)
+    uvn = C.staggered_vof(g, U, 0, shifted)
+    vvn = C.staggered_vof(g, V, 1, shifted)
+    u, v = C.transport_velocity(g, "vortex", amp=amp, seed=seed + 1)
+    u += 0.05 * amp * rng.standard_normal(u.shape)
+    v += 0.05 * amp * rng.standard_normal(v.shape)
+    itf = case.itf
+    pcell = dict(vof=itf.vof, phi=itf.phi, nx=itf.nx, ny=itf.ny, vofS=case.geo.vofS, nxS=case.geo.nxS,
+                 nyS=case.geo.nyS, phiS=case.geo.phiS)
+    ng = (g.nj + 2, g.ni + 2)
+    fluxn1 = 1e-3 * rng.standard_normal((2,) + ng)
+    DpU = case.dt * (1.0 + 0.1 * rng.uniform(size=ng))
+    DpV = case.dt * (1.0 + 0.1 * rng.uniform(size=ng))
+    upred = rng.standard_normal(ng)
+    vpred = rng.standard_normal(ng)
+    return dict(grid=g, U=U, V=V, uvo=uvo, uvn=uvn, vvo=vvo, vvn=vvn, pcell=pcell, u=np.ascontiguousarray(u),
+                v=np.ascontiguousarray(v), phys=phys_of(case.fluids, Re=Re), dt=case.dt, fluxn1=fluxn1,
+                DpU=DpU, DpV=DpV, upred=upred, vpred=vpred, geo=case.geo)
+
+
+def oracle_predictor(O, d, first_step=False, vb=0.0):
+    """The oracle's step: rows, direct solves, scatter and ghost copies."""
+    g = d["grid"]
+    r = O.predictor_assemble(g, d["U"], d["V"], d["uvo"], d["uvn"], d["vvo"], d["vvn"], d["pcell"], d["u"], d["v"],
+                             d["phys"], d["dt"], vb=vb, first_step=first_step, fluxn1=d["fluxn1"], DpU=d["DpU"],
+                             DpV=d["DpV"])
+    au = d["U"].active.copy()
+    au[:, -1] = 0
+    av = d["V"].active.copy()
+    av[-1, :] = 0
+    xu = O.momentum_solve(r["matr_u"], r["rhs_u"], au)
+    xv = O.momentum_solve(r["matr_v"], r["rhs_v"], av)
+    up = d["upred"].copy()
+    vp = d["vpred"].copy()
+    up[1:-1, 1:-1][:, :-1] = np.where(au[:, :-1] == 1, xu[:, :-1], 0.0)
+    vp[1:-1, 1:-1][:-1, :] = np.where(av[:-1, :] == 1, xv[:-1, :], 0.0)
+    DU, DV, vp = O.predictor_finish(g, r["DpU"], r["DpV"], vp)
+    r.update(upred=up, vpred=vp, DpU=DU, DpV=DV, xu=xu, xv=xv)
+    return r
+
+
+def device_predictor(d, first_step=False, vb=0.0):
+    """The same step through the C ABI; returns the device results."""
+    from simcoflow_amd.projection import Projection, default_params
+    g = d["grid"]
+    ph = d["phys"]
+    prm = default_params(rho_l=ph["rho_l"], rho_g=ph["rho_g"], rho_ref=ph["rho_ref"], nu_l=ph["nu_l"],
+                         nu_g=ph["nu_g"], nu_ref=ph["nu_ref"], Re=ph["Re"], Fr=ph["Fr"])
+    pj = Projection(g, prm)
+    try:
+        pj.set_geometry(d["geo"])
+        pj.set_staggered_geometry(0, d["U"])
+        pj.set_staggered_geometry(1, d["V"])
+        pc = d["pcell"]
+        for name, key in (("VOFO", "vof"), ("PHIO", "phi"), ("NXO", "nx"), ("NYO", "ny")):
+            pj.upload(name, pc[key])
+        pj.upload("UVOFO", d["uvo"])
+        pj.upload("UVOF", d["uvn"])
+        pj.upload("VVOFO", d["vvo"])
+        pj.upload("VVOF", d["vvn"])
+        pj.upload("U", d["u"])
+        pj.upload("V", d["v"])
+        pj.upload("FLUXN1U", d["fluxn1"][0])
+        pj.upload("FLUXN1V", d["fluxn1"][1])
+        pj.upload("DPU", d["DpU"])
+        pj.upload("DPV", d["DpV"])
+        pj.upload("UPRED", d["upred"])
+        pj.upload("VPRED", d["vpred"])
+        info = pj.predictor_resident(d["dt"], vb=vb, first_step=first_step)
+        mu, ru, mv, rv = pj.predictor_rows()
+        out = dict(info=info.as_dict(), matr_u=mu, rhs_u=ru, matr_v=mv, rhs_v=rv,
+                   fluxn1=np.stack([pj.download("FLUXN1U"), pj.download("FLUXN1V")]),
+                   DpU=pj.download("DPU"), DpV=pj.download("DPV"), upred=pj.download("UPRED"),
+                   vpred=pj.download("VPRED"))
+    finally:
+        pj.close()
+    return out
