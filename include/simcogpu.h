@@ -77,6 +77,8 @@ enum sg_field {
     SG_F_VOFO, SG_F_PHIO, SG_F_NXO, SG_F_NYO, SG_F_UVOF, SG_F_VVOF, SG_F_UVOFO, SG_F_VVOFO,
     /* ghosted: Flux_n1(:,:,1) and Flux_n1(:,:,2) (Solver_Mod.f90:49-50) */
     SG_F_FLUXN1U, SG_F_FLUXN1V,
+    /* U / V cell level set and normal (UCell/VCell%phi, nx, ny), ni*nj */
+    SG_F_UPHI, SG_F_UNX, SG_F_UNY, SG_F_VPHI, SG_F_VNX, SG_F_VNY,
     SG_F_COUNT
 };
 
@@ -166,6 +168,13 @@ int sg_predictor_resident(sg_ctx* ctx, double dt, double vb, int first_step, sg_
 /* test hook: the assembled U and V rows (matr layout ni*nj*5: S,W,diag,E,N)
  * and right-hand sides of the last sg_predictor_resident call */
 int sg_predictor_rows(sg_ctx* ctx, double* matr_u, double* rhs_u, double* matr_v, double* rhs_v);
+
+/* ---- interface on the U / V cells (replaces Initial_ClsVofUV,
+ * Clsvof_Mod.f90:402-710): device fields in VOF, PHI, NX, NY, VOFS, NXS, NYS,
+ * PHIS and the staggered geometry (vofS, phiS of sg_set_staggered_geometry);
+ * out UVOF, UPHI, UNX, UNY, VVOF, VPHI, VNX, VNY.  (The particle volume and
+ * source sums of that routine belong to the particle model, not kept.) */
+int sg_interface_to_uv_resident(sg_ctx* ctx);
 
 /* ---- interface transport (replaces Coupled_LS_VOF, Clsvof_Mod.f90:730-959) --
  * nsub sub-steps of alternating x/y PLIC volume sweeps (X_Sweep/Y_Sweep

@@ -204,6 +204,7 @@ def _cls_lib():
         L.orc_inflow_level.restype = dbl
         L.orc_clsvof_advect.argtypes = [G, _D, _D, _D, _D, _D, _D, dbl, dbl, ctypes.c_int,
                                         ctypes.POINTER(_Wave)]
+        L.orc_interface_to_uv.argtypes = [G, ctypes.c_int] + [_D] * 10
         _cls_declared = True
     return L
 
@@ -305,6 +306,16 @@ def clsvof_advect(cg, vof, phi, u, v, dt, time=0.0, nsub=6, wave=(1.0e30, 0.0, 0
     _cls_lib().orc_clsvof_advect(cg.ref, _d(vof), _d(phi), _d(nx), _d(ny), _d(np.ascontiguousarray(u)),
                                  _d(np.ascontiguousarray(v)), dt, time, nsub, ctypes.byref(w))
     return vof, phi, nx, ny
+
+
+def interface_to_uv(cg, which, vof, phi, nx, ny, vofST, phiST):
+    """Initial_ClsVofUV (Clsvof_Mod.f90:402-710) for U (which=0) or V
+    (which=1) cells; returns (vof, phi, nx, ny) of the staggered cells."""
+    c = np.ascontiguousarray
+    ins = [c(a, dtype=np.float64) for a in (vof, phi, nx, ny, vofST, phiST)]
+    outs = [np.zeros((cg.nj, cg.ni)) for _ in range(4)]
+    _cls_lib().orc_interface_to_uv(cg.ref, which, *[_d(a) for a in ins], *[_d(a) for a in outs])
+    return tuple(outs)
 
 
 # ---------------------------------------------------------------------------

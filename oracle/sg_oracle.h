@@ -146,6 +146,12 @@ void orc_boundary_copy(int ni, int nj, double* a);
 void orc_redistance(const orc_cls_geom* g, const double* vfl, const double* nxx1, const double* nyy1,
                     const double* diss1, double* phi);
 double orc_inflow_level(const orc_cls_geom* g, const orc_wave* w, double time, int k, double dtv);
+/* Initial_ClsVofUV (Clsvof_Mod.f90:402-710) for U (which=0) or V (which=1)
+ * cells from the P-cell vof, phi, nx, ny; vofST / phiST the staggered
+ * cells' solid fraction and solid level set; outputs are cell arrays. */
+void orc_interface_to_uv(const orc_cls_geom* g, int which, const double* vof, const double* phi, const double* nx,
+                         const double* ny, const double* vofST, const double* phiST, double* tvof, double* tphi,
+                         double* tnx, double* tny);
 /* nv sub-steps of Coupled_LS_VOF over dt; u, v ghosted TVar%u/v;
  * vof, phi in/out; nx, ny out. */
 void orc_clsvof_advect(const orc_cls_geom* g, double* vof, double* phi, double* nx, double* ny,

@@ -1042,3 +1042,111 @@ void orc_clsvof_advect(const orc_cls_geom* g, double* vfl, double* phi, double* 
     free(ue); free(ve); free(nx); free(ny); free(dis);
     free(tvx); free(tvy); free(tlx); free(tly);
 }
+
+/* LagrangePoly, Clsvof_Mod.f90:712-726, at PosTar = 0 */
+static double lagrange4(const double posi[4], const double valu[4])
+{
+    double val = 0.0;
+    for (int i = 0; i < 4; i++) {
+        double mult = valu[i];
+        for (int j = 0; j < 4; j++)
+            if (j != i) mult = mult * (0.0 - posi[j]) / (posi[i] - posi[j]);
+        val = val + mult;
+    }
+    return val;
+}
+
+/* Liquid fraction of the half of P cell (i,j) on the U side (which=0: the
+ * east half when `right` is 0, the west half of cell i+1 when 1) or the V
+ * side, with the state branches of Initial_ClsVofUV :448-536 / :591-672.
+ * Returns 0 when no branch assigns it (the reference keeps the previous
+ * value). */
+static int half_volf(const orc_cls_geom* g, const double* vof, const double* phi, const double* nx,
+                     const double* ny, int which, int right, int i, int j, double phisT, double* out)
+{
+    double vf = C(vof, i, j), vs = C(g->vofS, i, j);
+    double dx = g->dx[i - 1], dy = g->dy[j - 1];
+    double ph;
+    if (which == 0)
+        ph = right ? C(phi, i, j) - dx / 4.0 * C(nx, i, j) : C(phi, i, j) + dx / 4.0 * C(nx, i, j);
+    else
+        ph = right ? C(phi, i, j) - dy / 4.0 * C(ny, i, j) : C(phi, i, j) + dy / 4.0 * C(ny, i, j);
+    double hx = which == 0 ? dx / 2.0 : dx, hy = which == 0 ? dy : dy / 2.0;
+    int set = 0;
+    double v = 0.0, vols = 0.0;
+    if (vf >= 1.0 - VOFEPS && vs < EPSI) { v = 1.0; set = 1; }
+    if (vf < 1.0 - VOFEPS && vf >= VOFEPS && vs < EPSI) { v = orc_frac(C(nx, i, j), C(ny, i, j), ph, hx, hy); set = 1; }
+    if (vf < VOFEPS && vs < EPSI) { v = 0.0; set = 1; }
+    if (vf < VOFEPS && vs >= 1.0 - EPSI) { v = 0.0; set = 1; }
+    if (vf < VOFEPS && vs < (which == 0 ? 1.0 - VOFEPS : 1.0 - EPSI) && vs >= EPSI) { v = 0.0; set = 1; }
+    if (vf >= VOFEPS && vf < 1.0 - VOFEPS && vs >= EPSI && vs < 1.0 - EPSI) {
+        orc_cellgeocal(C(g->nxS, i, j), C(g->nyS, i, j), phisT, C(nx, i, j), C(ny, i, j), ph, hx, hy, &vols, &v);
+        set = 1;
+    }
+    if (set) *out = v;
+    return set;
+}
+
+/* Initial_ClsVofUV, Clsvof_Mod.f90:402-710, for U (which=0) or V (which=1)
+ * cells: level set and normal interpolated from the P cells (4-point
+ * Lagrange, 2-point average near the edges), liquid fraction from the two
+ * half P cells.  vofST, phiST: the staggered cells' solid fraction and solid
+ * level set.  Outputs are cell arrays of the staggered grid. */
+void orc_interface_to_uv(const orc_cls_geom* g, int which, const double* vof, const double* phi, const double* nx,
+                         const double* ny, const double* vofST, const double* phiST, double* tvof, double* tphi,
+                         double* tnx, double* tny)
+{
+    int ni = g->ni, nj = g->nj;
+    double volfl = 0.0, volfr = 0.0; /* kept across cells, as in the reference */
+    int nout = which == 0 ? nj : ni, nin = which == 0 ? ni - 1 : nj - 1;
+    for (int o = 1; o <= nout; o++) {
+        for (int k = 1; k <= nin; k++) {
+            int i = which == 0 ? k : o, j = which == 0 ? o : k;
+            int ip = which == 0 ? i + 1 : i, jp = which == 0 ? j : j + 1;
+            int lagr = which == 0 ? (i >= 3 && i <= ni - 2) : (j >= 3 && j <= nj - 2);
+            double p, a, b;
+            if (lagr) {
+                double posi[4], v1[4], v2[4], v3[4];
+                for (int m = 0; m < 4; m++) {
+                    int ii = which == 0 ? i - 1 + m : i, jj = which == 0 ? j : j - 1 + m;
+                    if (which == 0)
+                        posi[m] = (XC(i) + 0.5 * g->dx[i - 1]) - XC(ii);
+                    else
+                        posi[m] = (YC(j) + 0.5 * g->dy[j - 1]) - YC(jj);
+                    v1[m] = C(phi, ii, jj);
+                    v2[m] = C(nx, ii, jj);
+                    v3[m] = C(ny, ii, jj);
+                }
+                p = lagrange4(posi, v1);
+                a = lagrange4(posi, v2);
+                b = lagrange4(posi, v3);
+            } else {
+                p = 0.5 * (C(phi, ip, jp) + C(phi, i, j));
+                a = 0.5 * (C(nx, ip, jp) + C(nx, i, j));
+                b = 0.5 * (C(ny, ip, jp) + C(ny, i, j));
+            }
+            double temn = which == 0 ? sqrt(a * a + b * b) + 1.0e-14 : sqrt(a * a + b * b + 1.0e-30);
+            C(tphi, i, j) = p;
+            C(tnx, i, j) = a / temn;
+            C(tny, i, j) = b / temn;
+            double phisT = C(phiST, i, j);
+            half_volf(g, vof, phi, nx, ny, which, 0, i, j, phisT, &volfl);
+            half_volf(g, vof, phi, nx, ny, which, 1, ip, jp, phisT, &volfr);
+            double t = 0.5 * (volfl + volfr);
+            if (t < VOFEPS) t = 0.0;
+            if (t > 1.0 - VOFEPS - C(vofST, i, j)) t = 1.0 - C(vofST, i, j);
+            int im = which == 0 ? i - 1 : i, jm = which == 0 ? j : j - 1;
+            if ((which == 0 ? i > 1 : j > 1) && C(vof, im, jm) >= 1.0 - VOFEPS - C(g->vofS, im, jm) &&
+                C(vof, ip, jp) >= 1.0 - VOFEPS - C(g->vofS, ip, jp))
+                t = 1.0 - C(vofST, i, j);
+            C(tvof, i, j) = t;
+        }
+        /* the last staggered cell copies its neighbour (:555-558 / :700-703) */
+        int i = which == 0 ? ni : o, j = which == 0 ? o : nj;
+        int im = which == 0 ? ni - 1 : o, jm = which == 0 ? o : nj - 1;
+        C(tphi, i, j) = C(tphi, im, jm);
+        C(tnx, i, j) = C(tnx, im, jm);
+        C(tny, i, j) = C(tny, im, jm);
+        C(tvof, i, j) = C(tvof, im, jm);
+    }
+}

@@ -21,9 +21,10 @@ FIELDS = ["VOF", "PHI", "VOFS", "NYS", "WLLH", "NXS", "PHIS", "NX", "NY", "MRES"
           "U", "V", "P", "UPRED", "VPRED", "PP", "DPU", "DPV", "AE", "AW", "AN", "AS",
           "UFLAG", "VFLAG",
           "VOFO", "PHIO", "NXO", "NYO", "UVOF", "VVOF", "UVOFO", "VVOFO",
-          "FLUXN1U", "FLUXN1V"]
+          "FLUXN1U", "FLUXN1V",
+          "UPHI", "UNX", "UNY", "VPHI", "VNX", "VNY"]
 F = {name: k for k, name in enumerate(FIELDS)}
-CELL_FIELDS = set(FIELDS[:12]) | set(FIELDS[26:34])
+CELL_FIELDS = set(FIELDS[:12]) | set(FIELDS[26:34]) | set(FIELDS[36:42])
 GHOST_FIELDS = set(FIELDS[12:24]) | {"FLUXN1U", "FLUXN1V"}
 INT_FIELDS = {"UFLAG", "VFLAG"}
 
@@ -111,6 +112,7 @@ PROTOTYPES = {
     "sg_predictor_resident": ([_CTX, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                ctypes.POINTER(PredictorInfo)], ctypes.c_int),
     "sg_predictor_rows": ([_CTX, _D, _D, _D, _D], ctypes.c_int),
+    "sg_interface_to_uv_resident": ([_CTX], ctypes.c_int),
 }
 
 _lib = None

@@ -32,7 +32,8 @@ void* dev_alloc(Ctx* c, size_t bytes) {
 }
 
 static bool is_cell_field(int f) {
-    return (f >= SG_F_VOF && f <= SG_F_VOFSV) || (f >= SG_F_VOFO && f <= SG_F_VVOFO);
+    return (f >= SG_F_VOF && f <= SG_F_VOFSV) || (f >= SG_F_VOFO && f <= SG_F_VVOFO) ||
+           (f >= SG_F_UPHI && f <= SG_F_VNY);
 }
 static bool is_ghost_field(int f) { return (f >= SG_F_U && f <= SG_F_AS) || f == SG_F_FLUXN1U || f == SG_F_FLUXN1V; }
 static bool is_int_field(int f) { return f == SG_F_UFLAG || f == SG_F_VFLAG; }
@@ -473,6 +474,19 @@ int sg_predictor_rows(sg_ctx* ctx, double* matr_u, double* rhs_u, double* matr_v
     SG_HIP(hipMemcpyAsync(rhs_u, c->pred.ru, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipMemcpyAsync(matr_v, c->pred.mv, 5 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipMemcpyAsync(rhs_v, c->pred.rv, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_interface_to_uv_resident(sg_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    if (c->ni < 4 || c->nj < 4) {
+        set_error("sg_interface_to_uv_resident: needs ni, nj >= 4");
+        return SG_ERR_ARG;
+    }
+    int rc = interface_to_uv(c);
+    if (rc) return rc;
     SG_HIP(hipStreamSynchronize(c->stream));
     return SG_OK;
 }

@@ -1186,6 +1186,144 @@ __global__ void k_zero(unsigned* __restrict__ a, int n) {
     if (t < n) a[t] = 0u;
 }
 
+// ---- interface on the U / V cells (Initial_ClsVofUV :402-710) ----------------
+
+// LagrangePoly (:712-726) at PosTar = 0
+__device__ double lagrange4(const double (&posi)[4], const double (&valu)[4]) {
+    double val = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        double mult = valu[i];
+        for (int j = 0; j < 4; ++j)
+            if (j != i) mult = mult * (0.0 - posi[j]) / (posi[i] - posi[j]);
+        val = val + mult;
+    }
+    return val;
+}
+
+struct UVIn {
+    const double *vof, *phi, *nx, *ny;   // P cells
+    const double *vofST, *phiST;         // staggered solid fraction / level set
+};
+
+// liquid fraction of the half of P cell (i,j) facing the staggered cell
+// (right: the half of the next cell); false when no branch of :448-536 /
+// :591-672 assigns it
+template <int W>
+__device__ bool half_volf(const ClsGeo& G, const UVIn& in, bool right, int i, int j, double phisT, double& out) {
+    const size_t c = cix(G, i, j);
+    const double vf = in.vof[c], vs = G.vofS[c];
+    const double dx = G.dx[i - 1], dy = G.dy[j - 1];
+    double ph;
+    if (W == 0)
+        ph = right ? in.phi[c] - dx / 4.0 * in.nx[c] : in.phi[c] + dx / 4.0 * in.nx[c];
+    else
+        ph = right ? in.phi[c] - dy / 4.0 * in.ny[c] : in.phi[c] + dy / 4.0 * in.ny[c];
+    const double hx = W == 0 ? dx / 2.0 : dx, hy = W == 0 ? dy : dy / 2.0;
+    bool set = false;
+    double v = 0.0;
+    if (vf >= 1.0 - VOFEPS && vs < EPSI) { v = 1.0; set = true; }
+    if (vf < 1.0 - VOFEPS && vf >= VOFEPS && vs < EPSI) { v = plic_frac(in.nx[c], in.ny[c], ph, hx, hy); set = true; }
+    if (vf < VOFEPS && vs < EPSI) { v = 0.0; set = true; }
+    if (vf < VOFEPS && vs >= 1.0 - EPSI) { v = 0.0; set = true; }
+    if (vf < VOFEPS && vs < (W == 0 ? 1.0 - VOFEPS : 1.0 - EPSI) && vs >= EPSI) { v = 0.0; set = true; }
+    if (vf >= VOFEPS && vf < 1.0 - VOFEPS && vs >= EPSI && vs < 1.0 - EPSI) {
+        double vols;
+        cell_geo(G.nxS[c], G.nyS[c], phisT, in.nx[c], in.ny[c], ph, hx, hy, vols, v);
+        set = true;
+    }
+    if (set) out = v;
+    return set;
+}
+
+// previous staggered cell in the reference's loop order (U: j outer, i
+// inner over i < ni; V: i outer, j inner over j < nj); false before the first
+template <int W>
+__device__ __forceinline__ bool prev_cell(const ClsGeo& G, int& i, int& j) {
+    if (W == 0) {
+        if (i > 1) { --i; return true; }
+        if (j > 1) { --j; i = G.ni - 1; return true; }
+        return false;
+    }
+    if (j > 1) { --j; return true; }
+    if (i > 1) { --i; j = G.nj - 1; return true; }
+    return false;
+}
+
+// volfl / volfr of staggered cell (i,j), walking back to the last cell that
+// assigned it when this one does not (stale locals of the reference)
+template <int W>
+__device__ double half_stale(const ClsGeo& G, const UVIn& in, bool right, int i, int j) {
+    double v = 0.0;
+    for (;;) {
+        const int ip = W == 0 ? i + 1 : i, jp = W == 0 ? j : j + 1;
+        const double phisT = in.phiST[cix(G, i, j)];
+        if (right ? half_volf<W>(G, in, true, ip, jp, phisT, v) : half_volf<W>(G, in, false, i, j, phisT, v)) return v;
+        if (!prev_cell<W>(G, i, j)) return 0.0;
+    }
+}
+
+template <int W>
+__device__ void uv_cell(const ClsGeo& G, const UVIn& in, int i, int j, double& tv, double& tp, double& tx, double& ty) {
+    const int ip = W == 0 ? i + 1 : i, jp = W == 0 ? j : j + 1;
+    const bool lagr = W == 0 ? (i >= 3 && i <= G.ni - 2) : (j >= 3 && j <= G.nj - 2);
+    double p, a, b;
+    if (lagr) {
+        double posi[4], v1[4], v2[4], v3[4];
+        for (int m = 0; m < 4; ++m) {
+            const int ii = W == 0 ? i - 1 + m : i, jj = W == 0 ? j : j - 1 + m;
+            posi[m] = W == 0 ? (G.xc[i - 1] + 0.5 * G.dx[i - 1]) - G.xc[ii - 1]
+                             : (G.yc[j - 1] + 0.5 * G.dy[j - 1]) - G.yc[jj - 1];
+            const size_t c = cix(G, ii, jj);
+            v1[m] = in.phi[c];
+            v2[m] = in.nx[c];
+            v3[m] = in.ny[c];
+        }
+        p = lagrange4(posi, v1);
+        a = lagrange4(posi, v2);
+        b = lagrange4(posi, v3);
+    } else {
+        const size_t c0 = cix(G, i, j), c1 = cix(G, ip, jp);
+        p = 0.5 * (in.phi[c1] + in.phi[c0]);
+        a = 0.5 * (in.nx[c1] + in.nx[c0]);
+        b = 0.5 * (in.ny[c1] + in.ny[c0]);
+    }
+    const double temn = W == 0 ? sqrt(a * a + b * b) + 1.0e-14 : sqrt(a * a + b * b + 1.0e-30);
+    tp = p;
+    tx = a / temn;
+    ty = b / temn;
+    const double vl = half_stale<W>(G, in, false, i, j);
+    const double vr = half_stale<W>(G, in, true, i, j);
+    double t = 0.5 * (vl + vr);
+    const double vsT = in.vofST[cix(G, i, j)];
+    if (t < VOFEPS) t = 0.0;
+    if (t > 1.0 - VOFEPS - vsT) t = 1.0 - vsT;
+    const int im = W == 0 ? i - 1 : i, jm = W == 0 ? j : j - 1;
+    if ((W == 0 ? i > 1 : j > 1) && in.vof[cix(G, im, jm)] >= 1.0 - VOFEPS - G.vofS[cix(G, im, jm)] &&
+        in.vof[cix(G, ip, jp)] >= 1.0 - VOFEPS - G.vofS[cix(G, ip, jp)])
+        t = 1.0 - vsT;
+    tv = t;
+}
+
+// one thread per staggered cell; the last column (U) / row (V) copies its
+// neighbour (:555-558 / :700-703)
+template <int W>
+__global__ void __launch_bounds__(256) k_to_uv(ClsGeo G, UVIn in, double* __restrict__ tvof,
+                                               double* __restrict__ tphi, double* __restrict__ tnx,
+                                               double* __restrict__ tny) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > G.ni || j > G.nj) return;
+    const bool last = W == 0 ? i == G.ni : j == G.nj;
+    const int si = last && W == 0 ? i - 1 : i, sj = last && W == 1 ? j - 1 : j;
+    double tv, tp, tx, ty;
+    uv_cell<W>(G, in, si, sj, tv, tp, tx, ty);
+    const size_t c = cix(G, i, j);
+    tvof[c] = tv;
+    tphi[c] = tp;
+    tnx[c] = tx;
+    tny[c] = ty;
+}
+
 ClsGeo make_geo(Ctx* c) {
     ClsGeo G;
     G.ni = c->ni;
@@ -1294,6 +1432,27 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         hipLaunchKernelGGL(k_rd_scatter, pgrd, pblk, 0, st, G, vof, phi, P, pv, B.best, B.hit);
         hipLaunchKernelGGL(k_redistance, grd, blk, 0, st, G, vof, P, B.mask, B.mw, B.best, B.hit, phi);
     }
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+}  // namespace sg
+
+namespace sg {
+
+// Initial_ClsVofUV for both staggered grids: device fields VOF, PHI, NX, NY
+// (+ the solid geometry) -> UVOF/UPHI/UNX/UNY and VVOF/VPHI/VNX/VNY.
+int interface_to_uv(Ctx* c) {
+    const ClsGeo G = make_geo(c);
+    const dim3 blk(64, 4), grd((c->ni + 63) / 64, (c->nj + 3) / 4);
+    auto f = [&](int k) { return (double*)c->field[k]; };
+    UVIn in{f(SG_F_VOF), f(SG_F_PHI), f(SG_F_NX), f(SG_F_NY), c->stag[0].a[STG_VOFS], c->stag[0].a[STG_PHIS]};
+    hipLaunchKernelGGL(k_to_uv<0>, grd, blk, 0, c->stream, G, in, f(SG_F_UVOF), f(SG_F_UPHI), f(SG_F_UNX),
+                       f(SG_F_UNY));
+    in.vofST = c->stag[1].a[STG_VOFS];
+    in.phiST = c->stag[1].a[STG_PHIS];
+    hipLaunchKernelGGL(k_to_uv<1>, grd, blk, 0, c->stream, G, in, f(SG_F_VVOF), f(SG_F_VPHI), f(SG_F_VNX),
+                       f(SG_F_VNY));
     SG_HIP(hipGetLastError());
     return SG_OK;
 }

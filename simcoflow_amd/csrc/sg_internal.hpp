@@ -251,6 +251,7 @@ int momentum_solve(Ctx* c, const double* m, const double* b, double* x, double t
 // interface transport (sg_clsvof.hip)
 int clsvof_allocate(Ctx* c);
 int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels);
+int interface_to_uv(Ctx* c);
 
 void* dev_alloc(Ctx* c, size_t bytes);
 

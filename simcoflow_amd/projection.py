@@ -242,3 +242,10 @@ class Projection:
         check(self.lib.sg_predictor_rows(self.ctx, mu.ctypes.data_as(_DP), ru.ctypes.data_as(_DP),
                                          mv.ctypes.data_as(_DP), rv.ctypes.data_as(_DP)))
         return mu, ru, mv, rv
+
+    def interface_to_uv(self):
+        """Initial_ClsVofUV on the device fields; returns the U and V
+        (vof, phi, nx, ny)."""
+        check(self.lib.sg_interface_to_uv_resident(self.ctx))
+        return (tuple(self.download(k) for k in ("UVOF", "UPHI", "UNX", "UNY")),
+                tuple(self.download(k) for k in ("VVOF", "VPHI", "VNX", "VNY")))

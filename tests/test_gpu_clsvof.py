@@ -204,3 +204,71 @@ def test_full_size_swarm_bit_exact(oracle_lib):
     # round-off plus the split scheme's O(dt^2) imbalance
     dv = abs(vof.sum() - case.itf.vof.sum()) / case.itf.vof.sum()
     assert dv < 1e-5, dv
+
+
+# ---- interface on the U / V cells (Initial_ClsVofUV) -------------------------
+def _uv_both(O, g, geo, stags, vof, phi, nx, ny):
+    cg = O.ClsGeom(g, geo)
+    pj = _ctx(g, geo)
+    try:
+        for w in (0, 1):
+            pj.set_staggered_geometry(w, stags[w])
+        for name, a in (("VOF", vof), ("PHI", phi), ("NX", nx), ("NY", ny)):
+            pj.upload(name, a)
+        got = pj.interface_to_uv()
+    finally:
+        pj.close()
+    for w in (0, 1):
+        ref = O.interface_to_uv(cg, w, vof, phi, nx, ny, stags[w].vofS, stags[w].phiS)
+        for name, a, b in zip(("vof", "phi", "nx", "ny"), got[w], ref):
+            _equal(f"{'UV'[w]} {name}", a, b)
+
+
+UV_CASES = [
+    ("bubble", lambda: C.case_bubble(64, 128)),
+    ("dambreak", lambda: C.case_dambreak(128, 64)),
+    ("waves", lambda: C.case_waves(96, 48, seed=2)),
+]
+
+
+@pytest.mark.parametrize("name,mk", UV_CASES, ids=[c[0] for c in UV_CASES])
+def test_interface_to_uv_bit_exact(oracle_lib, name, mk):
+    O = oracle_lib
+    case = mk()
+    g = case.grid
+    print(f"[to_uv] {name} {g.ni}x{g.nj}", flush=True)
+    nx, ny, _ = O.reconstruct(O.ClsGeom(g, case.geo), case.itf.vof, case.itf.phi)
+    stags = C.staggered_geometry(g, case.geo, case.solid)
+    _uv_both(O, g, case.geo, stags, case.itf.vof, case.itf.phi, nx, ny)
+
+
+@pytest.mark.parametrize("seed,ni,nj", [(0, 40, 36), (1, 37, 41), (2, 40, 36), (3, 4, 4)])
+def test_interface_to_uv_random_bit_exact(oracle_lib, seed, ni, nj):
+    """Random cut-cell states, including full-liquid cells that are cut by
+    solids (no branch assigns the half fraction: the value of the previous
+    staggered cell in loop order is kept) and a non-uniform grid."""
+    import dataclasses
+    g, geo, vof, phi = _random_state(ni, nj, seed, nonuniform=(seed == 2))
+    rng = np.random.default_rng(100 + seed)
+    th = rng.uniform(0, 2 * np.pi, g.shape)
+    nx, ny = np.cos(th), np.sin(th)
+    # full liquid where the solid fraction is small but above the threshold
+    vof = np.where(rng.uniform(size=g.shape) < 0.1, 1.0, vof)
+    stags = []
+    for st in C.staggered_geometry(g, geo, None):
+        r = rng.uniform(size=g.shape)
+        vs = np.where(r < 0.7, 0.0, np.where(r < 0.9, rng.uniform(1e-3, 1 - 1e-3, g.shape), 1.0))
+        stags.append(dataclasses.replace(st, vofS=np.ascontiguousarray(vs),
+                                         phiS=np.ascontiguousarray(rng.uniform(-0.6, 0.6, g.shape) * g.dx[0])))
+    print(f"[to_uv] random seed {seed} {ni}x{nj}", flush=True)
+    _uv_both(oracle_lib, g, geo, stags, np.ascontiguousarray(vof), phi, nx, ny)
+
+
+def test_interface_to_uv_full_size(oracle_lib):
+    O = oracle_lib
+    case = C.case_swarm(2048, 2048, seed=0)
+    g = case.grid
+    print("[to_uv] swarm 2048x2048", flush=True)
+    nx, ny, _ = O.reconstruct(O.ClsGeom(g, case.geo), case.itf.vof, case.itf.phi)
+    stags = C.staggered_geometry(g, case.geo, case.solid)
+    _uv_both(O, g, case.geo, stags, case.itf.vof, case.itf.phi, nx, ny)

@@ -293,3 +293,61 @@ def test_boundary_copy(oracle_lib):
     exp[:, 0] = exp[:, 1]
     exp[:, -1] = exp[:, -2]
     assert np.array_equal(b, exp)
+
+
+# ---- interface on the U / V cells (Initial_ClsVofUV) -------------------------
+@pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("th", [0.3, 1.1, 2.5, 4.0, 5.9])
+def test_interface_to_uv_straight_line(oracle_lib, th, which):
+    """A straight interface: the two PLIC half cells make the exact liquid
+    fraction of the staggered cell, the 4-point Lagrange interpolation and
+    the two-point averages are exact for the linear level set, and the
+    normal is the line's (the last column / row copies its neighbour)."""
+    O = oracle_lib
+    g, geo, n, state = _line_state(40, 36, th)
+    vof, phi = state()
+    nx = np.broadcast_to(n[0], g.shape)
+    ny = np.broadcast_to(n[1], g.shape)
+    st = C.staggered_geometry(g, geo, None)[which]
+    tv, tp, tx, ty = O.interface_to_uv(O.ClsGeom(g, geo), which, vof, phi, nx, ny, st.vofS, st.phiS)
+    X, Y = np.meshgrid(g.xc, g.yc)
+    h = g.dx[0] if which == 0 else g.dy[0]
+    Xs, Ys = (X + 0.5 * h, Y) if which == 0 else (X, Y + 0.5 * h)
+    exact_phi = n[0] * Xs + n[1] * Ys + (phi - n[0] * X - n[1] * Y)
+    exact_vof = C.frac(n[0], n[1], exact_phi, g.dx[0], g.dy[0])
+    body = (slice(None), slice(0, -1)) if which == 0 else (slice(0, -1), slice(None))
+    assert np.abs(tp[body] - exact_phi[body]).max() < 1e-12
+    assert np.abs(tx[body] - n[0]).max() < 1e-12
+    assert np.abs(ty[body] - n[1]).max() < 1e-12
+    ev = np.where(exact_vof < 1e-14, 0.0, np.where(exact_vof > 1 - 1e-14, 1.0, exact_vof))
+    assert np.abs(tv[body] - ev[body]).max() < 1e-12
+    assert 0.05 < ev[body].mean() < 0.95
+    last = (slice(None), -1) if which == 0 else (-1, slice(None))
+    prev = (slice(None), -2) if which == 0 else (-2, slice(None))
+    for a in (tv, tp, tx, ty):
+        assert np.array_equal(a[last], a[prev])
+
+
+def test_interface_to_uv_solid_bounds(oracle_lib):
+    """Over solids: the staggered liquid fraction stays within the fluid
+    part, submerged cells away from the interface are full of liquid and cells
+    high above it are empty."""
+    O = oracle_lib
+    case = C.case_dambreak(96, 48)
+    g = case.grid
+    cg = O.ClsGeom(g, case.geo)
+    nx, ny, _ = O.reconstruct(cg, case.itf.vof, case.itf.phi)
+    U, V = C.staggered_geometry(g, case.geo, case.solid)
+    X, Y = np.meshgrid(g.xc, g.yc)
+    for which, st in ((0, U), (1, V)):
+        tv, tp, _, _ = O.interface_to_uv(cg, which, case.itf.vof, case.itf.phi, nx, ny, st.vofS, st.phiS)
+        assert np.all(tv >= 0.0)
+        assert np.all(tv <= 1.0 - st.vofS + 1e-12)
+        h = g.dx[0]
+        deep = (case.itf.phi < -3 * h) & (st.vofS < 1 - 1e-3)
+        deep[:, -2:] = False
+        deep[-2:, :] = False
+        assert deep.sum() > 50
+        assert np.abs(tv[deep] - (1.0 - st.vofS[deep])).max() < 1e-12
+        dry = case.itf.phi > 3 * h
+        assert dry.sum() > 50 and np.all(tv[dry] == 0.0)
