@@ -16,7 +16,10 @@ module simcogpu
        SG_F_WLLH = 4, SG_F_NXS = 5, SG_F_PHIS = 6, SG_F_NX = 7, SG_F_NY = 8, SG_F_MRES = 9, &
        SG_F_VOFSU = 10, SG_F_VOFSV = 11, SG_F_U = 12, SG_F_V = 13, SG_F_P = 14, SG_F_UPRED = 15, &
        SG_F_VPRED = 16, SG_F_PP = 17, SG_F_DPU = 18, SG_F_DPV = 19, SG_F_AE = 20, SG_F_AW = 21, &
-       SG_F_AN = 22, SG_F_AS = 23, SG_F_UFLAG = 24, SG_F_VFLAG = 25
+       SG_F_AN = 22, SG_F_AS = 23, SG_F_UFLAG = 24, SG_F_VFLAG = 25, SG_F_VOFO = 26, SG_F_PHIO = 27, &
+       SG_F_NXO = 28, SG_F_NYO = 29, SG_F_UVOF = 30, SG_F_VVOF = 31, SG_F_UVOFO = 32, SG_F_VVOFO = 33, &
+       SG_F_FLUXN1U = 34, SG_F_FLUXN1V = 35, SG_F_UPHI = 36, SG_F_UNX = 37, SG_F_UNY = 38, &
+       SG_F_VPHI = 39, SG_F_VNX = 40, SG_F_VNY = 41
 
   type, bind(C), public :: sg_params
     real(c_double) :: rho_l, rho_g, rho_ref
@@ -37,8 +40,33 @@ module simcogpu
     real(c_double) :: max_row_residual
     real(c_double) :: ms
     real(c_double) :: setup_ms
+    real(c_double) :: warm_ms
+    real(c_double) :: warm_relres
+    real(c_double) :: warm_energy
   end type sg_solve_info
 
+  ! U (which = 0) or V (which = 1) cell geometry: c_loc of each array, cell
+  ! arrays (1:Isize,1:Jsize) then ghosted arrays (0:Isize+1,0:Jsize+1)
+  type, bind(C), public :: sg_stag_geometry
+    type(c_ptr) :: vofS, nxS, nyS, phiS, WlLh, delh, ccx, ccy
+    type(c_ptr) :: moex, active
+    type(c_ptr) :: aE, aW, aN, aS, EtaE, EtaN, AlE, AlN, DAlE, DAlN, SxE, SyN
+    type(c_ptr) :: fcex, fcey, fcnx, fcny
+  end type sg_stag_geometry
+
+  type, bind(C), public :: sg_predictor_info
+    integer(c_int) :: iterations_u, iterations_v
+    real(c_double) :: relres_u, relres_v
+    real(c_double) :: ms
+  end type sg_predictor_info
+
+  type, bind(C), public :: sg_clsvof_params
+    integer(c_int) :: nsub
+    real(c_double) :: depth, amp, kw, omega
+  end type sg_clsvof_params
+
+  public :: sg_set_staggered_geometry, sg_predictor_resident, sg_predictor_rows, sg_interface_to_uv_resident
+  public :: sg_clsvof_params_default, sg_set_centres, sg_clsvof_advect_resident, sg_clsvof_advect
   public :: sg_last_error, sg_params_default, sg_ctx_create, sg_ctx_destroy, sg_set_params
   public :: sg_upload, sg_download, sg_synchronize, sg_set_geometry
   public :: sg_poisson_build_resident, sg_poisson_solve_resident, sg_poisson_solve
@@ -136,6 +164,55 @@ module simcogpu
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: vb
+      integer(c_int) :: rc
+    end function
+    function sg_set_staggered_geometry(ctx, which, g) bind(C, name="sg_set_staggered_geometry") result(rc)
+      import :: c_int, c_ptr, sg_stag_geometry
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
+      type(sg_stag_geometry), intent(in) :: g
+      integer(c_int) :: rc
+    end function
+    function sg_predictor_resident(ctx, dt, vb, first_step, info) bind(C, name="sg_predictor_resident") result(rc)
+      import :: c_int, c_ptr, c_double, sg_predictor_info
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt, vb
+      integer(c_int), value :: first_step
+      type(sg_predictor_info), intent(out) :: info
+      integer(c_int) :: rc
+    end function
+    function sg_predictor_rows(ctx, matr_u, rhs_u, matr_v, rhs_v) bind(C, name="sg_predictor_rows") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, matr_u, rhs_u, matr_v, rhs_v
+      integer(c_int) :: rc
+    end function
+    function sg_interface_to_uv_resident(ctx) bind(C, name="sg_interface_to_uv_resident") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function sg_clsvof_params_default(p) bind(C, name="sg_clsvof_params_default") result(rc)
+      import :: c_int, sg_clsvof_params
+      type(sg_clsvof_params), intent(out) :: p
+      integer(c_int) :: rc
+    end function
+    function sg_set_centres(ctx, xc, yc) bind(C, name="sg_set_centres") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, xc, yc
+      integer(c_int) :: rc
+    end function
+    function sg_clsvof_advect_resident(ctx, dt, time, p) bind(C, name="sg_clsvof_advect_resident") result(rc)
+      import :: c_int, c_ptr, c_double, sg_clsvof_params
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt, time
+      type(sg_clsvof_params), intent(in) :: p
+      integer(c_int) :: rc
+    end function
+    function sg_clsvof_advect(ctx, vof, phi, nx, ny, u, v, dt, time, p) bind(C, name="sg_clsvof_advect") result(rc)
+      import :: c_int, c_ptr, c_double, sg_clsvof_params
+      type(c_ptr), value :: ctx, vof, phi, nx, ny, u, v
+      real(c_double), value :: dt, time
+      type(sg_clsvof_params), intent(in) :: p
       integer(c_int) :: rc
     end function
   end interface

@@ -48,7 +48,8 @@ typedef struct sg_params {
     int max_iter;    /* PCG iteration cap (ProjectionP_Mod.f90:108 uses 50) */
     double tol;      /* stop at ||b-Ap||_2 <= tol*||b||_2 */
     int mg_pre, mg_post; /* smoothing sweeps per level (ProjectionP_Mod.f90:126: 1) */
-    int warm_start;  /* 0: x0 = 0 (ProjectionP_Mod.f90:319); 1: x0 = previous solution */
+    int warm_start;  /* 0: x0 = 0 (ProjectionP_Mod.f90:319); 1: x0 = previous solution;
+                        2: learned warm start + deflation vector (sg_set_learned) */
     int check_every; /* PCG iterations between host convergence checks (>= 1) */
 } sg_params;
 
@@ -59,6 +60,10 @@ typedef struct sg_solve_info {
     double max_row_residual; /* max |(Ap-b)_i| over fluid rows (ProjectionP_Mod.f90:70-93) */
     double ms;               /* device time of the whole call */
     double setup_ms;         /* coefficient build + multigrid setup */
+    double warm_ms;          /* learned warm start (net + prolongation + scaling), 0 without it */
+    double warm_relres;      /* ||b-A x0||_2/||b||_2 of the starting guess */
+    double warm_energy;      /* (w,b)^2/(w,Aw): the share x0'A x0 of the solution's energy x'A x = x'b
+                                that the learned start captures (0 without it) */
 } sg_solve_info;
 
 /* fields held on the device by a context (sg_field_ptr / upload / download) */
@@ -98,6 +103,9 @@ int sg_set_params(sg_ctx* ctx, const sg_params* p);
 void* sg_field_ptr(sg_ctx* ctx, int field);
 int sg_upload(sg_ctx* ctx, int field, const void* host);
 int sg_download(sg_ctx* ctx, int field, void* host);
+/* the same from / to a device buffer of the caller on the same GPU */
+int sg_upload_device(sg_ctx* ctx, int field, const void* dev);
+int sg_download_device(sg_ctx* ctx, int field, void* dev);
 int sg_synchronize(sg_ctx* ctx);
 
 /* Cut-cell geometry computed by Cutcell_Mod (Grid_Preprocess,
@@ -203,6 +211,47 @@ int sg_clsvof_advect_resident(sg_ctx* ctx, double dt, double time, const sg_clsv
  * (ghosted), time = NondiT. */
 int sg_clsvof_advect(sg_ctx* ctx, double* vof, double* phi, double* nx, double* ny, const double* u,
                      const double* v, double dt, double time, const sg_clsvof_params* p);
+
+/* ---- learned warm start (no counterpart in the reference: it starts PCG
+ * from x0 = 0, ProjectionP_Mod.f90:319) --------------------------------------
+ * A 4-level convolutional encoder-decoder (16/32/64/128 channels, 3x3
+ * kernels, additive skips, ReLU; BF16 weights and activations, FP32
+ * accumulation on the matrix cores) runs on level L of the solve's
+ * multigrid hierarchy.  Input channels at each level-L point: the
+ * restricted right-hand side / its rms, the Galerkin couplings E, N, NE, NW
+ * / the diagonal, the liquid fraction of the fine block, y and x in [0,1].
+ * Output: the level-L pressure / (out_scale * rms).  PCG starts from the
+ * prolongated, Galerkin-scaled prediction w and (deflate = 1) keeps its
+ * directions A-orthogonal to w; the stopping test is unchanged. */
+typedef struct sg_nn sg_nn;
+size_t sg_nn_num_weights(void);
+/* He-normal initial weights (host array of sg_nn_num_weights floats) */
+int sg_nn_init_weights(float* w, unsigned long long seed);
+/* a net for level grids of H x W (H % 8 == 0, W % 128 == 0), batches of up
+ * to `batch` samples, on its own stream */
+int sg_nn_create(int H, int W, int batch, sg_nn** out);
+int sg_nn_destroy(sg_nn* n);
+int sg_nn_set_weights(sg_nn* n, const float* w);  /* host FP32; resets Adam */
+int sg_nn_get_weights(sg_nn* n, float* w);
+/* x: device BF16 [nb][H+2][W+2][8] with a zero border; y: device FP32 [nb][H][W] */
+int sg_nn_forward(sg_nn* n, const void* x, float* y, int nb);
+/* loss = mean over the nb*H*W points of mask*(y - t)^2 (t, mask: device FP32
+ * [nb][H][W]); sg_nn_gradient returns its gradient w.r.t. the weights
+ * (host, sg_nn_num_weights floats), sg_nn_train_step applies one Adam step
+ * (beta 0.9 / 0.999) with learning rate lr.  loss may be NULL. */
+int sg_nn_gradient(sg_nn* n, const void* x, const float* t, const float* mask, int nb, float* grad, float* loss);
+int sg_nn_train_step(sg_nn* n, const void* x, const float* t, const float* mask, int nb, float lr, float* loss);
+/* grid of multigrid level `level` of a context (0 = the fine grid) */
+int sg_nn_level_dims(sg_ctx* ctx, int level, int* H, int* W);
+/* net input of the last solved system at `level` (device BF16 [H+2][W+2][8],
+ * border left as is) and, when t != NULL, the training target t = solution
+ * at the level's points / scale and the fluid mask (device FP32 [H][W]);
+ * scale = rms of the restricted right-hand side. */
+int sg_nn_features(sg_ctx* ctx, int level, void* x, float* t, float* mask, double* scale);
+/* enable the learned warm start for solves with warm_start = 2: weights
+ * (host FP32), the level it runs on, out_scale, deflate 0/1; w = NULL
+ * disables it */
+int sg_set_learned(sg_ctx* ctx, const float* w, int level, double out_scale, int deflate);
 
 /* ---- measurement ----------------------------------------------------------
  * Average device time (HIP events on the context's stream) of `reps`

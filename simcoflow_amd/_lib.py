@@ -45,6 +45,7 @@ class SolveInfo(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int), ("status", ctypes.c_int), ("relres", ctypes.c_double),
         ("max_row_residual", ctypes.c_double), ("ms", ctypes.c_double), ("setup_ms", ctypes.c_double),
+        ("warm_ms", ctypes.c_double), ("warm_relres", ctypes.c_double), ("warm_energy", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -81,6 +82,7 @@ _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 _V = ctypes.c_void_p
 _CTX = ctypes.c_void_p
+_F = ctypes.POINTER(ctypes.c_float)
 PROTOTYPES = {
     "sg_last_error": ([], ctypes.c_char_p),
     "sg_params_default": ([ctypes.POINTER(Params)], ctypes.c_int),
@@ -113,6 +115,21 @@ PROTOTYPES = {
                                ctypes.POINTER(PredictorInfo)], ctypes.c_int),
     "sg_predictor_rows": ([_CTX, _D, _D, _D, _D], ctypes.c_int),
     "sg_interface_to_uv_resident": ([_CTX], ctypes.c_int),
+    "sg_upload_device": ([_CTX, ctypes.c_int, _V], ctypes.c_int),
+    "sg_download_device": ([_CTX, ctypes.c_int, _V], ctypes.c_int),
+    "sg_nn_num_weights": ([], ctypes.c_size_t),
+    "sg_nn_init_weights": ([_F, ctypes.c_ulonglong], ctypes.c_int),
+    "sg_nn_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "sg_nn_destroy": ([_V], ctypes.c_int),
+    "sg_nn_set_weights": ([_V, _F], ctypes.c_int),
+    "sg_nn_get_weights": ([_V, _F], ctypes.c_int),
+    "sg_nn_forward": ([_V, _V, _V, ctypes.c_int], ctypes.c_int),
+    "sg_nn_gradient": ([_V, _V, _V, _V, ctypes.c_int, _F, _F], ctypes.c_int),
+    "sg_nn_train_step": ([_V, _V, _V, _V, ctypes.c_int, ctypes.c_float, _F], ctypes.c_int),
+    "sg_nn_level_dims": ([_CTX, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+                         ctypes.c_int),
+    "sg_nn_features": ([_CTX, ctypes.c_int, _V, _V, _V, _D], ctypes.c_int),
+    "sg_set_learned": ([_CTX, _F, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
 }
 
 _lib = None

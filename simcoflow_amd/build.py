@@ -27,12 +27,17 @@ SOURCES = [
     "sg_clsvof.hip",
     "sg_predictor.hip",
     "sg_momentum.hip",
+    "sg_nn.hip",
+    "sg_learned.hip",
 ]
 EXTRA = {
     "sg_poisson.hip": ["-ffp-contract=off"],
     "sg_clsvof.hip": ["-ffp-contract=off"],
     "sg_predictor.hip": ["-ffp-contract=off"],
     "sg_api.hip": ["-ffp-contract=off"],
+    # FP32 atomic adds of the weight gradients as hardware atomics (device
+    # memory, coarse-grained)
+    "sg_nn.hip": ["-munsafe-fp-atomics"],
 }
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -76,6 +81,7 @@ def build(verbose=False):
 FORTRAN_DIR = os.path.join(os.path.dirname(HERE), "fortran")
 FORTRAN_OUT = os.path.join(FORTRAN_DIR, "_build")
 FORTRAN_TEST = os.path.join(FORTRAN_OUT, "test_projection")
+FORTRAN_TESTS = ("test_projection", "test_transport")
 
 
 def amdflang():
@@ -95,15 +101,16 @@ def build_fortran(verbose=False):
             print("amdflang not found: Fortran host binding not built")
         return None
     os.makedirs(FORTRAN_OUT, exist_ok=True)
-    srcs = [os.path.join(FORTRAN_DIR, f) for f in ("simcogpu_mod.f90", "sg_projection_dropin.f90",
-                                                   "test_projection.f90")]
-    cmd = [fc, "-O2", "-module-dir", FORTRAN_OUT] + srcs + [
-        "-L" + OUT_DIR, "-lsimcogpu", "-Wl,-rpath," + OUT_DIR, "-o", FORTRAN_TEST]
-    r = subprocess.run(cmd, capture_output=True, text=True, cwd=FORTRAN_OUT)
-    if r.returncode != 0:
-        raise RuntimeError(f"amdflang failed:\n{r.stderr}")
-    if verbose:
-        print("built", FORTRAN_TEST)
+    lib_srcs = [os.path.join(FORTRAN_DIR, f) for f in ("simcogpu_mod.f90", "sg_projection_dropin.f90")]
+    for t in FORTRAN_TESTS:
+        exe = os.path.join(FORTRAN_OUT, t)
+        cmd = [fc, "-O2", "-module-dir", FORTRAN_OUT] + lib_srcs + [os.path.join(FORTRAN_DIR, t + ".f90")] + [
+            "-L" + OUT_DIR, "-lsimcogpu", "-Wl,-rpath," + OUT_DIR, "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=FORTRAN_OUT)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed:\n{r.stderr}")
+        if verbose:
+            print("built", exe)
     return FORTRAN_TEST
 
 

@@ -8,6 +8,7 @@
 #include <mutex>
 
 #include "sg_internal.hpp"
+#include "sg_nn.hpp"
 
 namespace sg {
 
@@ -43,6 +44,9 @@ static void destroy(Ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipGraphExec_t g : c->iter_graph)
         if (g) (void)hipGraphExecDestroy(g);
+    learned_release(c);
+    if (c->ev_nn0) (void)hipEventDestroy(c->ev_nn0);
+    if (c->ev_nn1) (void)hipEventDestroy(c->ev_nn1);
     for (void* p : c->allocations) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
     if (c->pred.sc_host) (void)hipHostFree(c->pred.sc_host);
@@ -256,6 +260,31 @@ int sg_download(sg_ctx* ctx, int field, void* host) {
         return SG_ERR_ARG;
     }
     SG_HIP(hipMemcpyAsync(host, c->field[field], c->field_bytes[field], hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+// device-to-device variants: the other buffer lives on the same GPU (a
+// caller's own allocation); synchronous on return
+int sg_upload_device(sg_ctx* ctx, int field, const void* dev) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !dev || field < 0 || field >= SG_F_COUNT) {
+        set_error("sg_upload_device: bad argument");
+        return SG_ERR_ARG;
+    }
+    SG_HIP(hipMemcpyAsync(c->field[field], dev, c->field_bytes[field], hipMemcpyDeviceToDevice, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    c->built = 0;
+    return SG_OK;
+}
+
+int sg_download_device(sg_ctx* ctx, int field, void* dev) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !dev || field < 0 || field >= SG_F_COUNT) {
+        set_error("sg_download_device: bad argument");
+        return SG_ERR_ARG;
+    }
+    SG_HIP(hipMemcpyAsync(dev, c->field[field], c->field_bytes[field], hipMemcpyDeviceToDevice, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
     return SG_OK;
 }
@@ -593,6 +622,153 @@ int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms) {
     SG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *avg_ms = (double)ms / reps;
     return SG_OK;
+}
+
+}  // extern "C"
+
+// ---- learned warm start ---------------------------------------------------
+struct sg_nn {
+    sg::nn::Net net;
+    hipStream_t stream = nullptr;
+    float* host_loss = nullptr;
+};
+
+extern "C" {
+
+size_t sg_nn_num_weights(void) { return sg::nn::num_weights(); }
+
+int sg_nn_init_weights(float* w, unsigned long long seed) {
+    using namespace sg::nn;
+    if (!w) return SG_ERR_ARG;
+    // xorshift64* + Box-Muller; He-normal std sqrt(2 / (9 cin)) on the first
+    // 9*cin columns of each row, zero padding; the output layer starts small
+    // and only its row 0 is nonzero
+    unsigned long long st = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+    auto uni = [&]() {
+        st ^= st >> 12;
+        st ^= st << 25;
+        st ^= st >> 27;
+        return ((st * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    size_t o = 0;
+    for (int l = 0; l < NL; ++l) {
+        const int cin = SPEC[l].cin, cout = SPEC[l].cout, kp = kpad(cin);
+        const double sd = (l == NL - 1 ? 0.1 : 1.0) * std::sqrt(2.0 / (9.0 * cin));
+        for (int co = 0; co < cout; ++co)
+            for (int k = 0; k < kp; ++k) {
+                double v = 0.0;
+                if (k < 9 * cin && (l < NL - 1 || co == 0)) {
+                    const double u1 = uni() + 1e-300, u2 = uni();
+                    v = sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+                }
+                w[o++] = (float)v;
+            }
+    }
+    return SG_OK;
+}
+
+int sg_nn_create(int H, int W, int batch, sg_nn** out) {
+    if (!out) return SG_ERR_ARG;
+    *out = nullptr;
+    sg_nn* n = new sg_nn();
+    if (hipStreamCreate(&n->stream) != hipSuccess || hipHostMalloc(&n->host_loss, sizeof(float)) != hipSuccess) {
+        if (n->stream) (void)hipStreamDestroy(n->stream);
+        delete n;
+        sg::set_error("sg_nn_create: HIP stream / pinned memory");
+        return SG_ERR_HIP;
+    }
+    int rc = sg::nn::create(&n->net, H, W, batch, n->stream);
+    if (rc) {
+        (void)hipStreamDestroy(n->stream);
+        (void)hipHostFree(n->host_loss);
+        delete n;
+        return rc;
+    }
+    *out = n;
+    return SG_OK;
+}
+
+int sg_nn_destroy(sg_nn* n) {
+    if (!n) return SG_OK;
+    (void)hipStreamSynchronize(n->stream);
+    sg::nn::destroy(&n->net);
+    (void)hipHostFree(n->host_loss);
+    (void)hipStreamDestroy(n->stream);
+    delete n;
+    return SG_OK;
+}
+
+int sg_nn_set_weights(sg_nn* n, const float* w) {
+    if (!n || !w) return SG_ERR_ARG;
+    return sg::nn::set_weights(&n->net, w);
+}
+
+int sg_nn_get_weights(sg_nn* n, float* w) {
+    if (!n || !w) return SG_ERR_ARG;
+    return sg::nn::get_weights(&n->net, w);
+}
+
+int sg_nn_forward(sg_nn* n, const void* x, float* y, int nb) {
+    if (!n || !x || !y) return SG_ERR_ARG;
+    int rc = sg::nn::forward(&n->net, (const sg::nn::bf16*)x, nb);
+    if (rc) return rc;
+    SG_HIP(hipMemcpyAsync(y, n->net.y, (size_t)nb * n->net.H * n->net.W * sizeof(float), hipMemcpyDeviceToDevice,
+                          n->stream));
+    SG_HIP(hipStreamSynchronize(n->stream));
+    return SG_OK;
+}
+
+static int nn_loss_grad(sg_nn* n, const void* x, const float* t, const float* mask, int nb, float* loss) {
+    if (!n || !x || !t || !mask) return SG_ERR_ARG;
+    int rc = sg::nn::backward(&n->net, (const sg::nn::bf16*)x, t, mask, nb);
+    if (rc) return rc;
+    if (loss) {
+        SG_HIP(hipMemcpyAsync(n->host_loss, n->net.loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    }
+    return SG_OK;
+}
+
+int sg_nn_gradient(sg_nn* n, const void* x, const float* t, const float* mask, int nb, float* grad, float* loss) {
+    int rc = nn_loss_grad(n, x, t, mask, nb, loss);
+    if (rc) return rc;
+    if (grad) SG_HIP(hipMemcpyAsync(grad, n->net.g, n->net.nw * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    SG_HIP(hipStreamSynchronize(n->stream));
+    if (loss) *loss = *n->host_loss;
+    return SG_OK;
+}
+
+int sg_nn_train_step(sg_nn* n, const void* x, const float* t, const float* mask, int nb, float lr, float* loss) {
+    int rc = nn_loss_grad(n, x, t, mask, nb, loss);
+    if (rc) return rc;
+    rc = sg::nn::adam_step(&n->net, lr);
+    if (rc) return rc;
+    if (loss) {
+        SG_HIP(hipStreamSynchronize(n->stream));
+        *loss = *n->host_loss;
+    }
+    return SG_OK;
+}
+
+int sg_nn_level_dims(sg_ctx* ctx, int level, int* H, int* W) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !H || !W || level < 0 || level >= c->nlev) return SG_ERR_ARG;
+    *H = c->lev[level].d.nj;
+    *W = c->lev[level].d.ni;
+    return SG_OK;
+}
+
+int sg_nn_features(sg_ctx* ctx, int level, void* x, float* t, float* mask, double* scale) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !x || (t && !mask)) return SG_ERR_ARG;
+    return sg::learned_features(c, level, x, t, mask, scale);
+}
+
+int sg_set_learned(sg_ctx* ctx, const float* w, int level, double out_scale, int deflate) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    int rc = sg::learned_enable(c, w, level, out_scale, deflate);
+    if (rc == SG_OK) (void)hipStreamSynchronize(c->stream);
+    return rc;
 }
 
 }  // extern "C"

@@ -95,6 +95,11 @@ struct PcgScalars {
     double beta;    // direction update of the current iteration
     double true_rr; // ||b - A x||^2 after the solve
     double true_max;// max |b - A x| over fluid rows after the solve
+    double mu;      // deflation: (A w, z) / (w, A w)
+    double wq;      // deflation: (w, A w)
+    double wb;      // learned warm start: (w, b)
+    double walpha;  // learned warm start: x0 = walpha * w
+    double nn_scale;// rms of the restricted right-hand side (net input scale)
     int active;     // 1 while not converged
     int iters;      // completed iterations
     int first;      // 1 before the first direction update
@@ -165,6 +170,23 @@ struct PredBuffers {
     static constexpr int BI_BLOCKS = 2048;
 };
 
+namespace nn {
+struct Net;
+}
+
+// Learned warm start / deflation vector of the projection PCG.
+struct NnState {
+    nn::Net* net = nullptr;  // inference net (batch 1) on level `level`
+    int level = 0;
+    int on = 0;
+    int deflate = 0;
+    double out_scale = 1.0;    // pressure = out_scale * rhs scale * net output
+    void* X = nullptr;         // BF16 input [H+2][W+2][8]
+    double* w = nullptr;       // fine deflation vector (padded FP64)
+    double* aw = nullptr;      // A w
+    float ms = 0.f;            // device time of the last warm start
+};
+
 struct Ctx {
     int ni = 0, nj = 0;
     sg_params prm{};
@@ -195,7 +217,9 @@ struct Ctx {
     // captured pair of PCG iterations, per precision (kernel arguments are
     // fixed for the life of the context, so one capture serves every solve)
     hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
-    int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0};
+    int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0}, graph_defl[2] = {0, 0};
+    NnState nn;
+    hipEvent_t ev_nn0 = nullptr, ev_nn1 = nullptr;
     int use_graph = 1;
     ClsBuffers cls;
     StagDev stag[2];  // U, V cells
@@ -241,6 +265,11 @@ int mg_launch_fine(Ctx* c, int which);  // measurement: 1 down, 2 up, 3 copy
 // PCG (sg_pcg.hip)
 int pcg_solve(Ctx* c, sg_solve_info* info);
 int fine_residual_norms(Ctx* c, double* relres, double* maxrow);
+// learned warm start (sg_learned.hip)
+int learned_warm_start(Ctx* c);   // x = Galerkin-scaled prolongated net output
+int learned_features(Ctx* c, int level, void* X, float* t, float* mask, double* scale);
+int learned_enable(Ctx* c, const float* w, int level, double out_scale, int deflate);
+void learned_release(Ctx* c);
 
 // momentum predictor (sg_predictor.hip, sg_momentum.hip)
 int predictor_allocate(Ctx* c);

@@ -14,7 +14,7 @@ namespace {
 constexpr int NT = TX * TY;
 constexpr int NTF = 1024;  // threads of the finalize kernels
 
-enum FinMode { FIN_INIT = 0, FIN_REPLACE, FIN_RZ, FIN_PQ, FIN_RR, FIN_TRUE };
+enum FinMode { FIN_INIT = 0, FIN_REPLACE, FIN_RZ, FIN_PQ, FIN_RR, FIN_TRUE, FIN_MU };
 
 // one reduction slot: NRED planes of per-block partials
 struct Slots {
@@ -65,21 +65,37 @@ __global__ void __launch_bounds__(256) k_dots_rz(Dims D, const double* __restric
     block_partials<2, NT>(v, S.part, S.max_blocks);
 }
 
-// p_new = z + beta p_old ; q = A p_new ; partials of (p_new, q)
+// partials of (A w, z): the deflation coefficient mu = (A w, z) / (w, A w)
+__global__ void __launch_bounds__(256) k_dot_mu(Dims D, const double* __restrict__ aw, const double* __restrict__ z,
+                                                Slots S, const PcgScalars* sc) {
+    if (!sc->active) return;
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    double v[1] = {0.0};
+    if (i < D.ni && j < D.nj) {
+        const size_t p = D.off(i, j);
+        v[0] = aw[p] * z[p];
+    }
+    block_partials<1, NT>(v, S.part, S.max_blocks);
+}
+
+// p_new = z + beta p_old (- mu w with deflation) ; q = A p_new ;
+// partials of (p_new, q)
 __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restrict__ diag,
                                                    const double* __restrict__ cE, const double* __restrict__ cN,
                                                    const double* __restrict__ z, const double* __restrict__ pold,
-                                                   double* __restrict__ pnew, double* __restrict__ q, Slots S,
-                                                   const PcgScalars* sc) {
+                                                   const double* __restrict__ w, double* __restrict__ pnew,
+                                                   double* __restrict__ q, Slots S, const PcgScalars* sc) {
     if (!sc->active) return;
     const double beta = sc->beta;
+    const double mu = w ? sc->mu : 0.0;
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     double v[1] = {0.0};
     if (i < D.ni && j < D.nj) {
         const size_t p = D.off(i, j);
         const size_t ld = D.ld;
-        auto pn = [&](size_t k) { return z[k] + beta * pold[k]; };
+        auto pn = [&](size_t k) { return w ? z[k] + beta * pold[k] - mu * w[k] : z[k] + beta * pold[k]; };
         const double pc = pn(p);
         const double qv = diag[p] * pc - cE[p - 1] * pn(p - 1) - cE[p] * pn(p + 1) - cN[p - ld] * pn(p - ld) -
                           cN[p] * pn(p + ld);
@@ -188,7 +204,9 @@ __global__ void __launch_bounds__(NTF) k_finalize(Slots S, int nblocks, int mode
     double s[1];
     finalize_sum<1, NTF>(S.part, S.max_blocks, nblocks, s);
     if (threadIdx.x != 0) return;
-    if (mode == FIN_PQ) {
+    if (mode == FIN_MU) {
+        sc->mu = sc->wq > 0.0 ? s[0] / sc->wq : 0.0;
+    } else if (mode == FIN_PQ) {
         sc->pq = s[0];
         sc->alpha = sc->rz / s[0];
         sc->first = 0;
@@ -207,6 +225,9 @@ __global__ void __launch_bounds__(NTF) k_finalize(Slots S, int nblocks, int mode
 Slots slots(Ctx* c, int s) {
     return Slots{c->red.partials + (size_t)s * NRED * c->red.max_blocks, c->red.max_blocks};
 }
+
+bool learned(Ctx* c) { return c->prm.warm_start == 2 && c->nn.on; }
+bool deflating(Ctx* c) { return learned(c) && c->nn.deflate; }
 
 // one PCG iteration (V-cycle + direction + update), enqueued on the stream
 int enqueue_iteration(Ctx* c, int parity) {
@@ -229,8 +250,16 @@ int enqueue_iteration(Ctx* c, int parity) {
         nb_rz = nb;
     }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S1, nb_rz, (int)FIN_RZ, c->sc, 0.0);
+    const double* w = nullptr;
+    if (deflating(c)) {
+        Slots S6 = slots(c, 6);
+        hipLaunchKernelGGL(k_dot_mu, g, b, 0, s, D, (const double*)c->nn.aw, (const double*)c->z, S6,
+                           (const PcgScalars*)c->sc);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S6, nb, (int)FIN_MU, c->sc, 0.0);
+        w = c->nn.w;
+    }
     hipLaunchKernelGGL(k_dir_apply, g, b, 0, s, D, L0.A.diag, L0.A.cE, L0.A.cN, (const double*)c->z,
-                       (const double*)pold, pnew, c->q, S2, (const PcgScalars*)c->sc);
+                       (const double*)pold, w, pnew, c->q, S2, (const PcgScalars*)c->sc);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S2, nb, (int)FIN_PQ, c->sc, 0.0);
     hipLaunchKernelGGL(k_update, g, b, 0, s, D, (const double*)pnew, (const double*)c->q, c->x, c->r, S3,
                        (const PcgScalars*)c->sc);
@@ -254,7 +283,9 @@ int enqueue_init(Ctx* c, int warm, int mode) {
 // Two PCG iterations (parities 0 and 1) captured once into a graph.
 int iteration_graph(Ctx* c, hipGraphExec_t* out) {
     const int k = c->prm.precision == SG_PREC_MIXED ? 1 : 0;
-    if (c->iter_graph[k] && c->graph_pre[k] == c->prm.mg_pre && c->graph_post[k] == c->prm.mg_post) {
+    const int defl = deflating(c) ? 1 : 0;
+    if (c->iter_graph[k] && c->graph_pre[k] == c->prm.mg_pre && c->graph_post[k] == c->prm.mg_post &&
+        c->graph_defl[k] == defl) {
         *out = c->iter_graph[k];
         return SG_OK;
     }
@@ -279,6 +310,7 @@ int iteration_graph(Ctx* c, hipGraphExec_t* out) {
     c->iter_graph[k] = exec;
     c->graph_pre[k] = c->prm.mg_pre;
     c->graph_post[k] = c->prm.mg_post;
+    c->graph_defl[k] = defl;
     *out = exec;
     return SG_OK;
 }
@@ -313,10 +345,28 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
     // flexible beta in mixed precision
     c->sc_host->flexible = c->prm.precision == SG_PREC_MIXED ? 1 : 0;
     SG_HIP(hipMemcpyAsync(&c->sc->flexible, &c->sc_host->flexible, sizeof(int), hipMemcpyHostToDevice, s));
-    int rc = enqueue_init(c, c->prm.warm_start, FIN_INIT);
+    int rc;
+    if (c->prm.warm_start == 2) {
+        if (!c->nn.on) {
+            set_error("warm_start = 2 needs the learned warm start (sg_set_learned)");
+            return SG_ERR_ARG;
+        }
+        rc = learned_warm_start(c);
+        if (rc) return rc;
+        rc = enqueue_init(c, 1, FIN_INIT);
+    } else {
+        rc = enqueue_init(c, c->prm.warm_start, FIN_INIT);
+    }
     if (rc) return rc;
     SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));
+    const double warm_relres = c->sc_host->bb > 0.0 ? sqrt(c->sc_host->rr / c->sc_host->bb) : 0.0;
+    float warm_ms = 0.f;
+    double warm_energy = 0.0;
+    if (c->prm.warm_start == 2) {
+        SG_HIP(hipEventElapsedTime(&warm_ms, c->ev_nn0, c->ev_nn1));
+        if (c->sc_host->wq > 0.0) warm_energy = c->sc_host->wb * c->sc_host->wb / c->sc_host->wq;
+    }
     const int max_iter = c->prm.max_iter > 0 ? c->prm.max_iter : 1000;
     const int every = c->prm.check_every > 0 ? c->prm.check_every : 1;
     hipGraphExec_t graph = nullptr;
@@ -379,6 +429,9 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
         info->ms = ms_all;
         info->setup_ms = ms_setup;
         info->status = status;
+        info->warm_ms = warm_ms;
+        info->warm_relres = warm_relres;
+        info->warm_energy = warm_energy;
     }
     return status;
 }

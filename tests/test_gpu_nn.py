@@ -1,0 +1,186 @@
+"""The learned warm start on the GPU: the library's BF16 matrix-core forward
+and backward kernels against the CPU FP64 torch reference (with the same BF16
+roundings), Adam training, and the ablation -- PCG iterations of the
+projection solve with and without the trained network.
+
+Tolerance of the forward output and of every layer's weight gradient: the
+BF16 roundings make the result sensitive to the order and precision of the
+sums (a flipped rounding changes a ReLU mask or an activation by one BF16
+ulp), so the bound is set by the reference itself -- the GPU must be within
+3x (+1e-3) the relative L2 distance between the reference evaluated with
+FP32 sums and with FP64 sums, and within 5e-2 absolutely."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(H, W, nb, seed):
+    from oracle.nn_reference import bf16_round
+    rng = np.random.default_rng(seed)
+    x = np.zeros((nb, H + 2, W + 2, 8))
+    x[:, 1:-1, 1:-1, :] = bf16_round(rng.standard_normal((nb, H, W, 8)))
+    t = rng.standard_normal((nb, H, W)).astype(np.float32)
+    m = (rng.uniform(size=(nb, H, W)) < 0.85).astype(np.float32)
+    return x, t, m
+
+
+def _to_dev(x, t, m):
+    xd = torch.from_numpy(x).to(torch.bfloat16).cuda().contiguous()
+    out = xd, torch.from_numpy(t).cuda().contiguous(), torch.from_numpy(m).cuda().contiguous()
+    torch.cuda.synchronize()
+    return out
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("H,W,nb", [(16, 128, 2), (8, 256, 1)])
+def test_forward_and_gradients_match_reference(H, W, nb):
+    from oracle import nn_reference as R
+    from simcoflow_amd.learned import Trainer, init_weights, split_weights
+    w = init_weights(11)
+    x, t, m = _batch(H, W, nb, 5)
+    print(f"[nn] forward/gradient H={H} W={W} nb={nb}", flush=True)
+    tr = Trainer(H, W, nb, weights=w)
+    try:
+        xd, td, md = _to_dev(x, t, m)
+        y = torch.zeros((nb, H, W), device="cuda", dtype=torch.float32)
+        torch.cuda.synchronize()
+        tr.forward(xd, y, nb)
+        g, loss = tr.gradient(xd, td, md, nb)
+        y = y.cpu().numpy().astype(np.float64)
+    finally:
+        tr.close()
+    def ref(dt):
+        Ws = [torch.from_numpy(R.bf16_round(a)).to(dt) for a in split_weights(w)]
+        xr = torch.from_numpy(x[:, 1:-1, 1:-1, :].transpose(0, 3, 1, 2).copy()).to(dt)
+        return R.loss_and_grads(xr, torch.from_numpy(t.astype(np.float64)).to(dt),
+                                torch.from_numpy(m.astype(np.float64)).to(dt), Ws, bf16=True)
+    lr, yr, gr = ref(torch.float64)
+    _, y32, g32 = ref(torch.float32)
+    noise = _rel(y32.double().numpy(), yr.numpy())
+    e = _rel(y, yr.numpy())
+    print(f"forward rel err {e:.2e} (reference fp32 vs fp64: {noise:.2e}), loss {loss} vs {lr}")
+    assert e <= min(3 * noise + 1e-3, 5e-2)
+    assert abs(loss - lr) <= 1e-2 * abs(lr)
+    gl = split_weights(g)
+    for l, (a, b, b32) in enumerate(zip(gl, gr, g32)):
+        b = b.numpy()
+        b32 = b32.double().numpy()
+        if l == 14:  # only row 0 of the output layer carries gradient
+            assert np.all(a[1:] == 0.0)
+            a, b, b32 = a[:1], b[:1], b32[:1]
+        e = _rel(a, b)
+        noise = _rel(b32, b)
+        print(f"layer {l}: grad rel err {e:.2e} (reference fp32 vs fp64: {noise:.2e})")
+        assert e <= min(3 * noise + 1e-3, 5e-2), (l, e, noise)
+
+
+def test_training_fits_a_smooth_map():
+    """Adam on one batch of a smooth, learnable target (a blurred input
+    channel): the loss falls by more than 5x in 150 steps."""
+    from simcoflow_amd.learned import Trainer
+    H, W, nb = 32, 128, 4
+    rng = np.random.default_rng(2)
+    x = np.zeros((nb, H + 2, W + 2, 8))
+    xi = rng.standard_normal((nb, H, W, 8))
+    x[:, 1:-1, 1:-1, :] = xi
+    k = np.ones((5, 5)) / 25.0
+    t = np.zeros((nb, H, W), dtype=np.float32)
+    for b in range(nb):
+        p = np.pad(xi[b, :, :, 0], 2)
+        for dy in range(5):
+            for dx in range(5):
+                t[b] += (k[dy, dx] * p[dy:dy + H, dx:dx + W]).astype(np.float32)
+    t = t / t.std()
+    m = np.ones((nb, H, W), dtype=np.float32)
+    xd, td, md = _to_dev(x, t, m)
+    torch.cuda.synchronize()
+    print("[nn] training on a smooth map", flush=True)
+    tr = Trainer(H, W, nb, seed=1)
+    try:
+        l0 = tr.step(xd, td, md, nb, 0.0, want_loss=True)
+        for i in range(150):
+            tr.step(xd, td, md, nb, 2e-3)
+        l1 = tr.step(xd, td, md, nb, 0.0, want_loss=True)
+    finally:
+        tr.close()
+    print("loss", l0, "->", l1)
+    assert l1 < 0.2 * l0
+
+
+def test_learned_warm_start_ablation():
+    """The trained network against the method without it (x0 = 0) and
+    against the untrained network on held-out bubble swarms (512^2, net on
+    the 256^2 level).  Reported: PCG iterations to the same stopping test
+    (relative residual 1e-8) with the learned warm start alone and with the
+    deflation vector, and the share of the solution's energy x'Ax the
+    Galerkin-scaled start captures.  Asserted: every variant converges to
+    the same pressure (1e-6 relative L2), and training raises the captured
+    energy above the untrained network's."""
+    from simcoflow_amd import cases as C
+    from simcoflow_amd import learned as Lr
+    from simcoflow_amd.projection import Projection, params_for
+    import ctypes
+    from simcoflow_amd import _lib
+
+    def mk(s):
+        return C.case_swarm(512, 512, seed=s)
+    print("[nn] ablation: collecting pairs", flush=True)
+    X, T, M, level = Lr.collect_pairs(mk, seeds=[1, 2, 3, 4], per_case=48, progress=print)
+    w, K, hist = Lr.train(X, T, M, epochs=12, batch=16, lr=3e-3, progress=print)
+    assert hist[-1] < 0.5 * hist[0]
+    w0 = Lr.init_weights(0)
+    gen = torch.Generator(device="cpu")
+    gen.manual_seed(99)
+    res = {"plain": [], "untrained": [], "warm": [], "deflated": []}
+    energy = {"untrained": [], "warm": []}
+    for s in (11, 12):
+        case = mk(s)
+        g = case.grid
+        _, _, DpU, DpV = C.predictor_fields(case, seed=s + 7)
+        pj = Projection(g, params_for(case.fluids, tol=1e-8, max_iter=300))
+        try:
+            pj.set_geometry(case.geo)
+            pj.set_interface(case.itf.vof, case.itf.phi)
+            pj.upload("DPU", DpU)
+            pj.upload("DPV", DpV)
+            for r in range(4):
+                u, v = Lr.smooth_velocities(torch, (g.nj + 2, g.ni + 2), case.fluids.g * case.dt, gen)
+                torch.cuda.synchronize()
+                _lib.check(pj.lib.sg_upload_device(pj.ctx, _lib.F["UPRED"], ctypes.c_void_p(u.data_ptr())))
+                _lib.check(pj.lib.sg_upload_device(pj.ctx, _lib.F["VPRED"], ctypes.c_void_p(v.data_ptr())))
+                out = {}
+                for mode, ws, wts, defl in (("plain", 0, None, False), ("untrained", 2, w0, False),
+                                            ("warm", 2, w, False), ("deflated", 2, w, True)):
+                    if ws == 2:
+                        Lr.enable(pj, wts, level, K, deflate=defl)
+                    pj.set_params(params_for(case.fluids, tol=1e-8, max_iter=300, warm_start=ws))
+                    pj.build(0.0)
+                    info = pj.solve()
+                    assert info.relres <= 1.05e-8, (mode, info.relres)
+                    res[mode].append(info.iterations)
+                    out[mode] = pj.download("PP")
+                    if mode == "plain":
+                        rhs = pj.rhs()
+                        xb = float((out[mode][1:-1, 1:-1] * rhs).sum())
+                    if mode in energy:
+                        energy[mode].append(info.warm_energy / xb)
+                    print(f"seed {s} rhs {r} {mode}: {info.iterations} iterations, start relres "
+                          f"{info.warm_relres:.3e}, energy share {info.warm_energy / xb:.4f}, "
+                          f"warm {info.warm_ms:.3f} ms")
+                ref = out["plain"]
+                for mode in ("untrained", "warm", "deflated"):
+                    d = np.linalg.norm(out[mode] - ref) / np.linalg.norm(ref)
+                    assert d < 1e-6, (mode, d)
+        finally:
+            pj.close()
+    mean = {k: float(np.mean(v)) for k, v in res.items()}
+    en = {k: float(np.mean(v)) for k, v in energy.items()}
+    print("mean iterations", mean, "mean energy share", en)
+    assert en["warm"] > en["untrained"]
+    assert en["warm"] > 0.0

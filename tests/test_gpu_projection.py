@@ -154,3 +154,19 @@ def test_fortran_dropin_host():
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "FORTRAN DROP-IN OK" in r.stdout
+
+
+def test_fortran_transport_dropin_host():
+    """The Fortran host calls the drop-in Coupled_LS_VOF, Initial_ClsVofUV and
+    Predictor_UV bodies (volume kept, exact half-cell fractions of a flat
+    surface, gravity increment of liquid at rest)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fortran", "_build",
+                       "test_transport")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (no amdflang)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "FORTRAN TRANSPORT DROP-IN OK" in r.stdout
