@@ -212,6 +212,20 @@ int sg_clsvof_advect_resident(sg_ctx* ctx, double dt, double time, const sg_clsv
 int sg_clsvof_advect(sg_ctx* ctx, double* vof, double* phi, double* nx, double* ny, const double* u,
                      const double* v, double dt, double time, const sg_clsvof_params* p);
 
+/* ---- time-step control (replaces ComputeTimeStep, Solver_Mod.f90:321-344) -
+ * dt = min over the cells of the reference's CFL bounds on the device fields
+ * U, V (cfl; Fr of sg_params; vint, uref = TVar%Vint, TVar%Uref; vs, asy =
+ * the solid's velocity and acceleration, 0 for solids at rest), a device
+ * min-reduction; dt also stays on the device. */
+int sg_time_step_size(sg_ctx* ctx, double cfl, double vint, double uref, double vs, double asy, double* dt);
+/* ghost layer of a closed box on U, V, P: the box form of
+ * Boundary_Condition_Var (StateVariables.f90:118-211) -- pressure and
+ * tangential velocity copied into the ghosts, P = 0 above the open top,
+ * U = 0 on the west/east walls, V = 0 on the bottom wall */
+int sg_closed_box_bc(sg_ctx* ctx);
+/* device copy of field src onto field dst of the same shape (CopyNewCell) */
+int sg_field_copy(sg_ctx* ctx, int dst, int src);
+
 /* ---- learned warm start (no counterpart in the reference: it starts PCG
  * from x0 = 0, ProjectionP_Mod.f90:319) --------------------------------------
  * A 4-level convolutional encoder-decoder (16/32/64/128 channels, 3x3

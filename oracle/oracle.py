@@ -430,3 +430,33 @@ def momentum_solve(matr, rhs, active):
     out = np.zeros((nj, ni))
     out[jj, ii] = x
     return out
+
+
+# ---------------------------------------------------------------------------
+# Time-step control (Solver_Mod.f90:321-344) and the closed-box ghost layer
+_step_declared = False
+
+
+def _step_lib():
+    global _step_declared
+    L = lib()
+    if not _step_declared:
+        dbl = ctypes.c_double
+        L.orc_time_step.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D] + [dbl] * 6
+        L.orc_time_step.restype = dbl
+        L.orc_closed_box_bc.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D]
+        _step_declared = True
+    return L
+
+
+def time_step(g, u, v, cfl, Fr, vint=0.0, uref=1.0, vs=0.0, asy=0.0):
+    c = np.ascontiguousarray
+    return _step_lib().orc_time_step(g.ni, g.nj, _d(c(g.udx, dtype=np.float64)), _d(c(g.vdy, dtype=np.float64)),
+                                     _d(c(u)), _d(c(v)), cfl, Fr, vint, uref, vs, asy)
+
+
+def closed_box_bc(u, v, p):
+    """In place on ghosted (nj+2, ni+2) arrays; returns them."""
+    nj, ni = u.shape[0] - 2, u.shape[1] - 2
+    _step_lib().orc_closed_box_bc(ni, nj, _d(u), _d(v), _d(p))
+    return u, v, p

@@ -196,6 +196,11 @@ void orc_predictor_assemble(const orc_pred_in* in, double* fluxn1, double* matr_
  * (PredictorVelocityBoundaryCondition :2199-2213) */
 void orc_predictor_finish(int ni, int nj, double* DpU, double* DpV, double* vpred);
 
+/* ---- time-step control (Solver_Mod.f90) ---------------------------------- */
+double orc_time_step(int ni, int nj, const double* udx, const double* vdy, const double* u, const double* v,
+                     double cfl, double Fr, double vint, double uref, double vs, double asy);
+void orc_closed_box_bc(int ni, int nj, double* u, double* v, double* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,6 +116,10 @@ PROTOTYPES = {
     "sg_predictor_rows": ([_CTX, _D, _D, _D, _D], ctypes.c_int),
     "sg_interface_to_uv_resident": ([_CTX], ctypes.c_int),
     "sg_upload_device": ([_CTX, ctypes.c_int, _V], ctypes.c_int),
+    "sg_time_step_size": ([_CTX, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                           ctypes.c_double, _D], ctypes.c_int),
+    "sg_closed_box_bc": ([_CTX], ctypes.c_int),
+    "sg_field_copy": ([_CTX, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "sg_download_device": ([_CTX, ctypes.c_int, _V], ctypes.c_int),
     "sg_nn_num_weights": ([], ctypes.c_size_t),
     "sg_nn_init_weights": ([_F, ctypes.c_ulonglong], ctypes.c_int),

@@ -29,12 +29,14 @@ SOURCES = [
     "sg_momentum.hip",
     "sg_nn.hip",
     "sg_learned.hip",
+    "sg_step.hip",
 ]
 EXTRA = {
     "sg_poisson.hip": ["-ffp-contract=off"],
     "sg_clsvof.hip": ["-ffp-contract=off"],
     "sg_predictor.hip": ["-ffp-contract=off"],
     "sg_api.hip": ["-ffp-contract=off"],
+    "sg_step.hip": ["-ffp-contract=off"],
     # FP32 atomic adds of the weight gradients as hardware atomics (device
     # memory, coarse-grained)
     "sg_nn.hip": ["-munsafe-fp-atomics"],

@@ -197,7 +197,8 @@ int sg_ctx_create(int ni, int nj, const double* dx, const double* dy, const doub
     c->red.partials = (double*)dev_alloc(c, sizeof(double) * 8 * NRED * (size_t)c->red.max_blocks);
     c->red.tickets = (unsigned int*)dev_alloc(c, sizeof(unsigned int) * 8);
     c->sc = (PcgScalars*)dev_alloc(c, sizeof(PcgScalars));
-    if (!c->red.partials || !c->red.tickets || !c->sc) return fail_alloc();
+    c->step_dev = (double*)dev_alloc(c, 4 * sizeof(double));
+    if (!c->red.partials || !c->red.tickets || !c->sc || !c->step_dev) return fail_alloc();
     CK(hipHostMalloc((void**)&c->sc_host, sizeof(PcgScalars), hipHostMallocDefault));
     std::memset(c->sc_host, 0, sizeof(PcgScalars));
     // a geometry with no solid: open apertures, fluid everywhere
@@ -769,6 +770,42 @@ int sg_set_learned(sg_ctx* ctx, const float* w, int level, double out_scale, int
     int rc = sg::learned_enable(c, w, level, out_scale, deflate);
     if (rc == SG_OK) (void)hipStreamSynchronize(c->stream);
     return rc;
+}
+
+}  // extern "C"
+
+// ---- time-step control ------------------------------------------------------
+extern "C" {
+
+int sg_time_step_size(sg_ctx* ctx, double cfl, double vint, double uref, double vs, double asy, double* dt) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !dt || !(cfl > 0.0) || uref == 0.0) {
+        sg::set_error("sg_time_step_size: bad argument");
+        return SG_ERR_ARG;
+    }
+    return sg::time_step_size(c, cfl, vint, uref, vs, asy, dt);
+}
+
+int sg_closed_box_bc(sg_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    int rc = sg::closed_box_bc(c);
+    if (rc) return rc;
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_field_copy(sg_ctx* ctx, int dst, int src) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || dst < 0 || src < 0 || dst >= SG_F_COUNT || src >= SG_F_COUNT ||
+        c->field_bytes[dst] != c->field_bytes[src]) {
+        sg::set_error("sg_field_copy: fields of different shape or out of range");
+        return SG_ERR_ARG;
+    }
+    SG_HIP(hipMemcpyAsync(c->field[dst], c->field[src], c->field_bytes[src], hipMemcpyDeviceToDevice, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    c->built = 0;
+    return SG_OK;
 }
 
 }  // extern "C"

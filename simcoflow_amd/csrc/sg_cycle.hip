@@ -22,11 +22,10 @@ namespace {
 
 constexpr int NTB = 256;
 
-constexpr int CW = 32, CH = 16;  // coarse-level (9-point) tile
-constexpr int TAIL_MAX = 256;    // levels with at most this many points run in the tail kernel
+constexpr int TAIL_MAX = 1024;   // levels with at most this many points run in the tail kernel
 constexpr int TAIL_THREADS = 1024;
 constexpr int MAX_TAIL = 10;
-constexpr size_t TAIL_LDS_MAX = 64 * 1024;
+constexpr size_t TAIL_LDS_MAX = 150 * 1024;  // of the 160 KB of LDS a workgroup can hold
 
 struct Slots {
     double* part;
@@ -35,295 +34,26 @@ struct Slots {
 
 __device__ inline bool in_dom(const Dims& D, int i, int j) { return i >= 0 && j >= 0 && i < D.ni && j < D.nj; }
 
-// operator of a tile (+halo) in LDS; local (li,lj) <-> cell (i0+li-h, j0+lj-h)
-template <typename T, int RW, int RH, bool NINE>
-struct Stage {
-    T d[RH][RW], e[RH][RW], n[RH][RW];
-    T ne[NINE ? RH : 1][NINE ? RW : 1], nw[NINE ? RH : 1][NINE ? RW : 1];
-    __device__ void load(const OpV<T>& A, const Dims& F, int i0, int j0, int h, int tid) {
-        constexpr int NL = (RW * RH + NTB - 1) / NTB;
-#pragma unroll
-        for (int m = 0; m < NL; ++m) {
-            const int k = tid + m * NTB;
-            if (k >= RW * RH) continue;
-            const int li = k % RW, lj = k / RW;
-            const int gi = i0 + li - h, gj = j0 + lj - h;
-            T a = T(0), b = T(0), c = T(0), dd = T(0), f = T(0);
-            if (in_dom(F, gi, gj)) {
-                const size_t p = F.off(gi, gj);
-                a = A.diag[p];
-                b = A.cE[p];
-                c = A.cN[p];
-                if (NINE) {
-                    dd = A.cNE[p];
-                    f = A.cNW[p];
-                }
-            }
-            d[lj][li] = a;
-            e[lj][li] = b;
-            n[lj][li] = c;
-            if (NINE) {
-                ne[lj][li] = dd;
-                nw[lj][li] = f;
-            }
-        }
-    }
-    template <typename XF>
-    __device__ T coupled(int li, int lj, XF X) const {
-        T s = e[lj][li - 1] * X(-1, 0) + e[lj][li] * X(1, 0) + n[lj - 1][li] * X(0, -1) + n[lj][li] * X(0, 1);
-        if (NINE)
-            s += ne[lj][li] * X(1, 1) + ne[lj - 1][li - 1] * X(-1, -1) + nw[lj][li] * X(-1, 1) +
-                 nw[lj - 1][li + 1] * X(1, -1);
-        return s;
-    }
-};
-
-// prolongation weights (and optionally the coarse vector) of a coarse window
-// [I0, I0+WW) x [J0, J0+WH); ring cells read as zero
-template <typename T, int WW, int WH>
-struct StageW {
-    T w[9][WH][WW];
-    T ev[WH][WW];
-    __device__ void load(const Prol<T>& P, const T* e, const Dims& C, int I0, int J0, int tid) {
-        for (int k = tid; k < WW * WH; k += NTB) {
-            const int a = k % WW, b = k / WW;
-            const int I = I0 + a, J = J0 + b;
-            const bool ok = I >= -1 && J >= -1 && I <= C.ni && J <= C.nj;
-            const size_t c = ok ? C.off(I, J) : 0;
-#pragma unroll
-            for (int m = 0; m < 9; ++m) w[m][b][a] = ok ? P.w[m][c] : T(0);
-            if (e) ev[b][a] = ok ? e[c] : T(0);
-        }
-    }
-    // P^T r at coarse (I,J) sitting at window (wa,wb); R(a,b) = r at fine (2I+a,2J+b)
-    template <typename RF>
-    __device__ T restrict_at(int wa, int wb, RF R) const {
-        T s = w[W_INJ][wb][wa] * R(0, 0);
-        s += w[W_XW][wb][wa] * R(1, 0) + w[W_XE][wb][wa - 1] * R(-1, 0);
-        s += w[W_YS][wb][wa] * R(0, 1) + w[W_YN][wb - 1][wa] * R(0, -1);
-        s += w[W_SW][wb][wa] * R(1, 1) + w[W_SE][wb][wa - 1] * R(-1, 1) + w[W_NW][wb - 1][wa] * R(1, -1) +
-             w[W_NE][wb - 1][wa - 1] * R(-1, -1);
-        return s;
-    }
-    // P e at a fine cell of parity (pi,pj) whose coarse parent sits at (a,c)
-    __device__ T prolong_at(int pi, int pj, int a, int c) const {
-        if (!pi && !pj) return w[W_INJ][c][a] * ev[c][a];
-        if (pi && !pj) return w[W_XW][c][a] * ev[c][a] + w[W_XE][c][a] * ev[c][a + 1];
-        if (!pi && pj) return w[W_YS][c][a] * ev[c][a] + w[W_YN][c][a] * ev[c + 1][a];
-        return w[W_SW][c][a] * ev[c][a] + w[W_SE][c][a] * ev[c][a + 1] + w[W_NW][c][a] * ev[c + 1][a] +
-               w[W_NE][c][a] * ev[c + 1][a + 1];
-    }
-};
-
-// one colour pass over the local rectangle [lo, RW-lo) x [lo, RH-lo);
-// new values staged in registers, written after a barrier
-template <typename T, int RW, int RH, bool NINE, int LO>
-__device__ inline void colour_pass(const Stage<T, RW, RH, NINE>& S, T (&sb)[RH][RW], T (&sx)[RH][RW], int i0, int j0,
-                                   int h, int col, int tid) {
-    constexpr int W = RW - 2 * LO, H = RH - 2 * LO;
-    constexpr int NB = (W * H + NTB - 1) / NTB;
-    T nv[NB];
-#pragma unroll
-    for (int m = 0; m < NB; ++m) {
-        const int k = tid + m * NTB;
-        nv[m] = T(0);
-        if (k >= W * H) continue;
-        const int li = k % W + LO, lj = k / W + LO;
-        const int gi = i0 + li - h, gj = j0 + lj - h;
-        T v = sx[lj][li];
-        if (((gi + gj) & 1) == col) {
-            const T d = S.d[lj][li];
-            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            v = d > T(0) ? (sb[lj][li] + S.coupled(li, lj, X)) / d : T(0);
-        }
-        nv[m] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < NB; ++m) {
-        const int k = tid + m * NTB;
-        if (k >= W * H) continue;
-        sx[k / W + LO][k % W + LO] = nv[m];
-    }
-    __syncthreads();
-}
-
 // ---------------------------------------------------------------------------
+// 9-point coarse levels that are not in the tail: the operator is read from
+// global memory (through the caches) and only b, x live in LDS, which keeps
+// occupancy high.  Couplings between cells of one colour (the diagonals) are
+// taken Jacobi-wise: a colour pass computes into registers and writes after
+// a barrier.
+
 // down: x = S(0; b) (red from zero, then black), r = b - A x, bc = P^T r
-template <typename T, typename TB, bool NINE, int TW, int TH>
-__global__ void __launch_bounds__(NTB) k_down(Dims F, OpV<T> A, const TB* __restrict__ b, T* __restrict__ xo, Dims C,
-                                              Prol<T> P, T* __restrict__ bc) {
-    constexpr int H = 3;
-    constexpr int RW = TW + 5, RH = TH + 5;  // cells [-3, T+2)
-    constexpr int WW = TW / 2 + 1, WH = TH / 2 + 1;
-    constexpr int NL = (RW * RH + NTB - 1) / NTB;
-    __shared__ Stage<T, RW, RH, NINE> S;
-    __shared__ StageW<T, WW, WH> Wt;
-    __shared__ T sb[RH][RW];
-    __shared__ T sx[RH][RW];
-    const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
-    const int I0 = i0 / 2, J0 = j0 / 2;
-    const int tid = threadIdx.x;
-    S.load(A, F, i0, j0, H, tid);
-    Wt.load(P, nullptr, C, I0 - 1, J0 - 1, tid);
-#pragma unroll
-    for (int m = 0; m < NL; ++m) {
-        const int k = tid + m * NTB;
-        if (k >= RW * RH) continue;
-        const int li = k % RW, lj = k / RW;
-        const int gi = i0 + li - H, gj = j0 + lj - H;
-        sb[lj][li] = in_dom(F, gi, gj) ? (T)b[F.off(gi, gj)] : T(0);
-    }
-    __syncthreads();
-    // red from a zero guess
-#pragma unroll
-    for (int m = 0; m < NL; ++m) {
-        const int k = tid + m * NTB;
-        if (k >= RW * RH) continue;
-        const int li = k % RW, lj = k / RW;
-        const int gi = i0 + li - H, gj = j0 + lj - H;
-        const T d = S.d[lj][li];
-        sx[lj][li] = (((gi + gj) & 1) == 0 && d > T(0)) ? sb[lj][li] / d : T(0);
-    }
-    __syncthreads();
-    colour_pass<T, RW, RH, NINE, 1>(S, sb, sx, i0, j0, H, 1, tid);
-    // residual on cells [-1,TW) x [-1,TH), written over b
-    {
-        constexpr int W = TW + 1, HH = TH + 1;
-        constexpr int NR = (W * HH + NTB - 1) / NTB;
-        T rv[NR];
-#pragma unroll
-        for (int m = 0; m < NR; ++m) {
-            const int k = tid + m * NTB;
-            rv[m] = T(0);
-            if (k >= W * HH) continue;
-            const int li = k % W + 2, lj = k / W + 2;
-            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            rv[m] = sb[lj][li] - S.d[lj][li] * sx[lj][li] + S.coupled(li, lj, X);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < NR; ++m) {
-            const int k = tid + m * NTB;
-            if (k >= W * HH) continue;
-            sb[k / W + 2][k % W + 2] = rv[m];
-        }
-    }
-    __syncthreads();
-    for (int k = tid; k < (TW / 2) * (TH / 2); k += NTB) {
-        const int a = k % (TW / 2), bq = k / (TW / 2);
-        const int I = I0 + a, J = J0 + bq;
-        if (I < C.ni && J < C.nj) {
-            const int li = 2 * a + H, lj = 2 * bq + H;
-            auto R = [&](int aa, int bb) { return sb[lj + bb][li + aa]; };
-            bc[C.off(I, J)] = Wt.restrict_at(a + 1, bq + 1, R);
-        }
-    }
-    for (int k = tid; k < TW * TH; k += NTB) {
-        const int li = k % TW + H, lj = k / TW + H;
-        const int gi = i0 + li - H, gj = j0 + lj - H;
-        if (in_dom(F, gi, gj)) xo[F.off(gi, gj)] = sx[lj][li];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// up: x = xin + P e, then black and red passes; writes xo (TZ).  DOTS: the
-// fine level of the PCG -- partials of (b, x) and (x, q), b being the PCG
-// residual the cycle was applied to.
-template <typename T, typename TB, typename TZ, bool NINE, bool DOTS, int TW, int TH>
-__global__ void __launch_bounds__(NTB) k_up(Dims F, OpV<T> A, const TB* __restrict__ b, const T* __restrict__ xin,
-                                            Dims C, Prol<T> P, const T* __restrict__ e, TZ* __restrict__ xo,
-                                            const double* __restrict__ q, Slots Sl) {
-    constexpr int H = 2;
-    constexpr int RW = TW + 4, RH = TH + 4;  // cells [-2, T+2)
-    constexpr int WW = TW / 2 + 3, WH = TH / 2 + 3;
-    constexpr int NL = (RW * RH + NTB - 1) / NTB;
-    constexpr int NR = (TW * TH + NTB - 1) / NTB;
-    __shared__ Stage<T, RW, RH, NINE> S;
-    __shared__ StageW<T, WW, WH> Wt;
-    __shared__ T sb[RH][RW];
-    __shared__ T sx[RH][RW];
-    const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
-    const int I0 = i0 / 2, J0 = j0 / 2;
-    const int tid = threadIdx.x;
-    S.load(A, F, i0, j0, H, tid);
-    Wt.load(P, e, C, I0 - 1, J0 - 1, tid);
-    T xr[NL];
-#pragma unroll
-    for (int m = 0; m < NL; ++m) {
-        const int k = tid + m * NTB;
-        xr[m] = T(0);
-        if (k >= RW * RH) continue;
-        const int li = k % RW, lj = k / RW;
-        const int gi = i0 + li - H, gj = j0 + lj - H;
-        T bv = T(0);
-        if (in_dom(F, gi, gj)) {
-            const size_t p = F.off(gi, gj);
-            bv = (T)b[p];
-            xr[m] = xin[p];
-        }
-        sb[lj][li] = bv;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < NL; ++m) {
-        const int k = tid + m * NTB;
-        if (k >= RW * RH) continue;
-        const int li = k % RW, lj = k / RW;
-        const int gi = i0 + li - H, gj = j0 + lj - H;
-        T xv = T(0);
-        if (in_dom(F, gi, gj)) xv = xr[m] + Wt.prolong_at(gi & 1, gj & 1, (gi >> 1) - I0 + 1, (gj >> 1) - J0 + 1);
-        sx[lj][li] = xv;
-    }
-    __syncthreads();
-    colour_pass<T, RW, RH, NINE, 1>(S, sb, sx, i0, j0, H, 1, tid);
-    // red on the tile, written straight out
-    double v0 = 0.0, v1 = 0.0;
-#pragma unroll
-    for (int m = 0; m < NR; ++m) {
-        const int k = tid + m * NTB;
-        if (k >= TW * TH) continue;
-        const int li = k % TW + H, lj = k / TW + H;
-        const int gi = i0 + li - H, gj = j0 + lj - H;
-        if (!in_dom(F, gi, gj)) continue;
-        T v = sx[lj][li];
-        if (((gi + gj) & 1) == 0) {
-            const T d = S.d[lj][li];
-            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            v = d > T(0) ? (sb[lj][li] + S.coupled(li, lj, X)) / d : T(0);
-        }
-        const size_t p = F.off(gi, gj);
-        xo[p] = (TZ)v;
-        if (DOTS) {
-            const double zv = (double)(TZ)v;
-            v0 += (double)b[p] * zv;
-            v1 += zv * q[p];
-        }
-    }
-    if (DOTS) {
-        double vv[2] = {v0, v1};
-        block_partials<2, NTB>(vv, Sl.part, Sl.max_blocks);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Fine (5-point) level at full size: many tiles hide the memory latency, so
-// the operator is read straight from global memory (through the caches)
-// rather than staged, which keeps LDS small and occupancy high.
-constexpr int GW = 64, GH = 16;
-
-template <typename T, typename TB>
-__global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __restrict__ b, T* __restrict__ xo,
+template <typename T, int TW, int TH>
+__global__ void __launch_bounds__(NTB) k_down9(Dims F, OpV<T> A, const T* __restrict__ b, T* __restrict__ xo,
                                                Dims C, Prol<T> P, T* __restrict__ bc) {
-    constexpr int RW = GW + 5, RH = GH + 5;  // local (li,lj) <-> cell (i0+li-3, j0+lj-3)
-    constexpr int NB = ((RW - 2) * (RH - 2) + NTB - 1) / NTB;
+    constexpr int RW = TW + 5, RH = TH + 5;  // local (li,lj) <-> cell (i0+li-3, j0+lj-3)
+    constexpr int NL = (RW * RH + NTB - 1) / NTB;
     __shared__ T sb[RH][RW];
     __shared__ T sx[RH][RW];
-    __shared__ T sr[RH][RW];
-    const int i0 = blockIdx.x * GW, j0 = blockIdx.y * GH;
+    int tx, ty;
+    xcd_tile(tx, ty);
+    const int i0 = tx * TW, j0 = ty * TH;
     const int tid = threadIdx.x;
     const size_t ld = F.ld;
-    constexpr int NL = (RW * RH + NTB - 1) / NTB;
 #pragma unroll
     for (int m = 0; m < NL; ++m) {
         const int k = tid + m * NTB;
@@ -333,7 +63,7 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
         T bv = T(0), xv = T(0);
         if (in_dom(F, gi, gj)) {
             const size_t p = F.off(gi, gj);
-            bv = (T)b[p];
+            bv = b[p];
             if (((gi + gj) & 1) == 0) {
                 const T d = A.diag[p];
                 xv = d > T(0) ? bv / d : T(0);
@@ -343,15 +73,17 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
         sx[lj][li] = xv;
     }
     __syncthreads();
-    // black cells on [-2,GW+1) x [-2,GH+1), one thread per black cell; the
-    // 5-point stencil couples black only to red, so the update is in place.
-    // Local parity: black <=> (li + lj) odd.
+    // black on [-2,TW+1) x [-2,TH+1); local parity: black <=> (li + lj) odd
     {
-        constexpr int PR = (RW - 1) / 2;  // black slots per row
+        constexpr int PR = (RW - 1) / 2;
         constexpr int NBK = (PR * (RH - 2) + NTB - 1) / NTB;
+        T nv[NBK];
+        int pos[NBK];
 #pragma unroll
         for (int n = 0; n < NBK; ++n) {
             const int k = tid + n * NTB;
+            pos[n] = -1;
+            nv[n] = T(0);
             if (k >= PR * (RH - 2)) continue;
             const int lj = k / PR + 1;
             const int li = 2 * (k % PR) + 1 + (lj & 1);
@@ -361,17 +93,23 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
             const size_t p = F.off(gi, gj);
             const T d = A.diag[p];
             auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            sx[lj][li] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+            nv[n] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+            pos[n] = lj * RW + li;
         }
+        __syncthreads();
+#pragma unroll
+        for (int n = 0; n < NBK; ++n)
+            if (pos[n] >= 0) (&sx[0][0])[pos[n]] = nv[n];
     }
-    (void)NB;
     __syncthreads();
-    constexpr int NRS = ((GW + 1) * (GH + 1) + NTB - 1) / NTB;
+    // residual on [-1,TW) x [-1,TH), in place over b (each cell reads only
+    // its own b)
+    constexpr int NRS = ((TW + 1) * (TH + 1) + NTB - 1) / NTB;
 #pragma unroll
     for (int m = 0; m < NRS; ++m) {
         const int k = tid + m * NTB;
-        if (k >= (GW + 1) * (GH + 1)) continue;
-        const int li = k % (GW + 1) + 2, lj = k / (GW + 1) + 2;
+        if (k >= (TW + 1) * (TH + 1)) continue;
+        const int li = k % (TW + 1) + 2, lj = k / (TW + 1) + 2;
         const int gi = i0 + li - 3, gj = j0 + lj - 3;
         T r = T(0);
         if (in_dom(F, gi, gj)) {
@@ -379,37 +117,36 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
             auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
             r = sb[lj][li] - A.diag[p] * sx[lj][li] + coupled_sum<T>(A, p, ld, X);
         }
-        sr[lj][li] = r;
+        sb[lj][li] = r;
     }
     __syncthreads();
-    {
-        const int I = i0 / 2 + tid % (GW / 2), J = j0 / 2 + tid / (GW / 2);
+    for (int k = tid; k < (TW / 2) * (TH / 2); k += NTB) {
+        const int I = i0 / 2 + k % (TW / 2), J = j0 / 2 + k / (TW / 2);
         if (I < C.ni && J < C.nj) {
             const int li = 2 * I - i0 + 3, lj = 2 * J - j0 + 3;
-            auto R = [&](int a, int bb) { return sr[lj + bb][li + a]; };
+            auto R = [&](int a, int bb) { return sb[lj + bb][li + a]; };
             bc[C.off(I, J)] = restrict_point<T>(C, P, I, J, R);
         }
     }
-    for (int k = tid; k < GW * GH; k += NTB) {
-        const int li = k % GW + 3, lj = k / GW + 3;
+    for (int k = tid; k < TW * TH; k += NTB) {
+        const int li = k % TW + 3, lj = k / TW + 3;
         const int gi = i0 + li - 3, gj = j0 + lj - 3;
         if (in_dom(F, gi, gj)) xo[F.off(gi, gj)] = sx[lj][li];
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __restrict__ b,
-                                             const T* __restrict__ xin, Dims C, Prol<T> P, const T* __restrict__ e,
-                                             double* __restrict__ z, const double* __restrict__ q, Slots Sl) {
-    constexpr int RW = GW + 4, RH = GH + 4;  // local (li,lj) <-> cell (i0+li-2, j0+lj-2)
-    constexpr int NR = (GW * GH + NTB - 1) / NTB;
+// up: x = xin + P e, then black and red passes; writes xo
+template <typename T, int TW, int TH>
+__global__ void __launch_bounds__(NTB) k_up9(Dims F, OpV<T> A, const T* __restrict__ b, const T* __restrict__ xin,
+                                             Dims C, Prol<T> P, const T* __restrict__ e, T* __restrict__ xo) {
+    constexpr int RW = TW + 4, RH = TH + 4;  // local (li,lj) <-> cell (i0+li-2, j0+lj-2)
     __shared__ T sb[RH][RW];
     __shared__ T sx[RH][RW];
-    const int i0 = blockIdx.x * GW, j0 = blockIdx.y * GH;
+    int tx, ty;
+    xcd_tile(tx, ty);
+    const int i0 = tx * TW, j0 = ty * TH;
     const int tid = threadIdx.x;
     const size_t ld = F.ld;
-    // load b and x + P e per 2x2 block of cells sharing one coarse parent
-    // (the region starts on an even cell, so blocks align with coarse cells)
     {
         constexpr int BW = RW / 2, BH = RH / 2;
         constexpr int NL2 = (BW * BH + NTB - 1) / NTB;
@@ -436,7 +173,7 @@ __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __r
                         const int gi = 2 * I + di, gj = 2 * J + dj;
                         if (gi < F.ni && gj < F.nj) {
                             const size_t p = F.off(gi, gj);
-                            bv[dj][di] = (T)b[p];
+                            bv[dj][di] = b[p];
                             xv[dj][di] = xin[p] + pe[dj][di];
                         }
                     }
@@ -451,58 +188,347 @@ __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __r
         }
     }
     __syncthreads();
-    // black on [-1,GW+1) x [-1,GH+1), one thread per black cell, in place
-    // (no black-black coupling).  Local parity: black <=> (li + lj) odd.
+    // black on [-1,TW+1) x [-1,TH+1), staged (Jacobi among black cells)
     {
-        constexpr int PR = (RW - 2) / 2;  // black cells per row
+        constexpr int PR = (RW - 2) / 2;
         constexpr int NBK = (PR * (RH - 2) + NTB - 1) / NTB;
+        T nv[NBK];
+        int pos[NBK];
 #pragma unroll
         for (int m = 0; m < NBK; ++m) {
             const int k = tid + m * NTB;
+            pos[m] = -1;
+            nv[m] = T(0);
             if (k >= PR * (RH - 2)) continue;
             const int lj = k / PR + 1;
             const int li = 2 * (k % PR) + 1 + (lj & 1);
+            if (li >= RW - 1) continue;
             const int gi = i0 + li - 2, gj = j0 + lj - 2;
             if (!in_dom(F, gi, gj)) continue;
             const size_t p = F.off(gi, gj);
             const T d = A.diag[p];
             auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            sx[lj][li] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+            nv[m] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+            pos[m] = lj * RW + li;
         }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NBK; ++m)
+            if (pos[m] >= 0) (&sx[0][0])[pos[m]] = nv[m];
     }
     __syncthreads();
-    // red on the tile, in place
-    {
-        constexpr int PR = GW / 2;
-        constexpr int NRD = (PR * GH + NTB - 1) / NTB;
-#pragma unroll
-        for (int m = 0; m < NRD; ++m) {
-            const int k = tid + m * NTB;
-            if (k >= PR * GH) continue;
-            const int lj = k / PR + 2;
-            const int li = 2 * (k % PR) + 2 + (lj & 1);
-            const int gi = i0 + li - 2, gj = j0 + lj - 2;
-            if (!in_dom(F, gi, gj)) continue;
-            const size_t p = F.off(gi, gj);
-            const T d = A.diag[p];
-            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            sx[lj][li] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
-        }
-    }
-    __syncthreads();
-    double v0 = 0.0, v1 = 0.0;
-#pragma unroll
-    for (int n = 0; n < NR; ++n) {
-        const int k = tid + n * NTB;
-        if (k >= GW * GH) continue;
-        const int li = k % GW + 2, lj = k / GW + 2;
+    // red on the tile, written straight out (reads only sx)
+    for (int k = tid; k < TW * TH; k += NTB) {
+        const int li = k % TW + 2, lj = k / TW + 2;
         const int gi = i0 + li - 2, gj = j0 + lj - 2;
         if (!in_dom(F, gi, gj)) continue;
         const size_t p = F.off(gi, gj);
-        const double zv = (double)sx[lj][li];
-        z[p] = zv;
-        v0 += b[p] * zv;
-        v1 += zv * q[p];
+        T v = sx[lj][li];
+        if (((gi + gj) & 1) == 0) {
+            const T d = A.diag[p];
+            auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
+            v = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+        }
+        xo[p] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fine (5-point) level at full size: many tiles hide the memory latency, so
+// the operator is read straight from global memory (through the caches)
+// rather than staged, which keeps LDS small and occupancy high.
+constexpr int GW = 64, GH = 16;  // fine-level tile
+
+// down on the fine level: x = S(0; b) (red from zero, then black), r = b - A
+// x, bc = P^T r.  As in the up-sweep, each thread holds the couplings of its
+// pairs of tile cells in registers for all three passes; cells of the halo
+// read theirs from memory.
+template <typename T, typename TB, int TW, int TH>
+__global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __restrict__ b, T* __restrict__ xo,
+                                               Dims C, Prol<T> P, T* __restrict__ bc) {
+    constexpr int RW = TW + 5, RH = TH + 5;  // local (li,lj) <-> cell (i0+li-3, j0+lj-3)
+    constexpr int PPR = TW / 2;
+    constexpr int PPT = PPR * TH / NTB;
+    static_assert(PPR * TH % NTB == 0, "tile pairs must divide among the threads");
+    constexpr int NL = (RW * RH + NTB - 1) / NTB;
+    __shared__ T sb[RH][RW];
+    __shared__ T sx[RH][RW];
+    int tx, ty;
+    xcd_tile(tx, ty);
+    const int i0 = tx * TW, j0 = ty * TH;
+    const int tid = threadIdx.x;
+    const size_t ld = F.ld;
+    T* sxf = &sx[0][0];
+    T* sbf = &sb[0][0];
+    auto in_tile = [&](int li, int lj) { return li >= 3 && li < TW + 3 && lj >= 3 && lj < TH + 3; };
+    auto coef = [&](size_t p, T (&c)[5]) {
+        c[0] = A.diag[p];
+        c[1] = A.cE[p - 1];
+        c[2] = A.cE[p];
+        c[3] = A.cN[p - ld];
+        c[4] = A.cN[p];
+    };
+    auto gs = [&](const T (&c)[5], int pos) {
+        const T s = c[1] * sxf[pos - 1] + c[2] * sxf[pos + 1] + c[3] * sxf[pos - RW] + c[4] * sxf[pos + RW];
+        return c[0] > T(0) ? (sbf[pos] + s) / c[0] : T(0);
+    };
+    // b; red from a zero guess on the halo (the tile's red cells below)
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+        const int k = tid + m * NTB;
+        if (k >= RW * RH) continue;
+        const int li = k % RW, lj = k / RW;
+        const int gi = i0 + li - 3, gj = j0 + lj - 3;
+        T bv = T(0), xv = T(0);
+        if (in_dom(F, gi, gj)) {
+            const size_t p = F.off(gi, gj);
+            bv = (T)b[p];
+            if (((gi + gj) & 1) == 0 && !in_tile(li, lj)) {
+                const T d = A.diag[p];
+                xv = d > T(0) ? bv / d : T(0);
+            }
+        }
+        sb[lj][li] = bv;
+        sx[lj][li] = xv;
+    }
+    // couplings of the thread's pairs: [0] black, [1] red
+    T cf[PPT][2][5];
+    int cl[PPT][2];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const int k = tid + u * NTB;
+        const int lj = k / PPR + 3;
+        const int l0 = 2 * (k % PPR) + 3;
+        // black <=> (li + lj) odd (i0, j0 even; local offset 3 on both axes)
+        const int lb = l0 + (lj & 1), lr = l0 + ((lj + 1) & 1);
+        const int ll[2] = {lb, lr};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int gi = i0 + ll[h] - 3, gj = j0 + lj - 3;
+            cl[u][h] = -1;
+#pragma unroll
+            for (int t = 0; t < 5; ++t) cf[u][h][t] = T(0);
+            if (!in_dom(F, gi, gj)) continue;
+            coef(F.off(gi, gj), cf[u][h]);
+            cl[u][h] = lj * RW + ll[h];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const int pos = cl[u][1];
+        if (pos >= 0) sxf[pos] = cf[u][1][0] > T(0) ? sbf[pos] / cf[u][1][0] : T(0);
+    }
+    __syncthreads();
+    // black on [-2,TW+1) x [-2,TH+1): the tile's from registers, the halo's
+    // from memory (in place: black couples only to red)
+#pragma unroll
+    for (int u = 0; u < PPT; ++u)
+        if (cl[u][0] >= 0) sxf[cl[u][0]] = gs(cf[u][0], cl[u][0]);
+    {
+        constexpr int HW = RW - 2, HH = RH - 2;  // li, lj in [1, RW-1) x [1, RH-1)
+        for (int k = tid; k < HW * HH; k += NTB) {
+            const int li = k % HW + 1, lj = k / HW + 1;
+            if (in_tile(li, lj) || ((li + lj) & 1) == 0) continue;
+            const int gi = i0 + li - 3, gj = j0 + lj - 3;
+            if (!in_dom(F, gi, gj)) continue;
+            T c[5];
+            coef(F.off(gi, gj), c);
+            sxf[lj * RW + li] = gs(c, lj * RW + li);
+        }
+    }
+    __syncthreads();
+    // residual on [-1,TW) x [-1,TH), in place over b (each cell reads only
+    // its own b)
+    auto resid = [&](const T (&c)[5], int pos) {
+        return sbf[pos] - c[0] * sxf[pos] +
+               (c[1] * sxf[pos - 1] + c[2] * sxf[pos + 1] + c[3] * sxf[pos - RW] + c[4] * sxf[pos + RW]);
+    };
+    T rt[PPT][2];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) rt[u][h] = cl[u][h] >= 0 ? resid(cf[u][h], cl[u][h]) : T(0);
+    {
+        constexpr int NRING = TW + 1 + TH;  // row -1 (columns -1..TW-1), column -1 (rows 0..TH-1)
+        for (int k = tid; k < NRING; k += NTB) {
+            const int li = k <= TW ? k + 2 : 2;
+            const int lj = k <= TW ? 2 : k - TW + 2;
+            const int gi = i0 + li - 3, gj = j0 + lj - 3;
+            T r = T(0);
+            if (in_dom(F, gi, gj)) {
+                T c[5];
+                coef(F.off(gi, gj), c);
+                r = resid(c, lj * RW + li);
+            }
+            sbf[lj * RW + li] = r;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PPT; ++u)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (cl[u][h] >= 0) sbf[cl[u][h]] = rt[u][h];
+    __syncthreads();
+    for (int k = tid; k < (TW / 2) * (TH / 2); k += NTB) {
+        const int I = i0 / 2 + k % (TW / 2), J = j0 / 2 + k / (TW / 2);
+        if (I < C.ni && J < C.nj) {
+            const int li = 2 * I - i0 + 3, lj = 2 * J - j0 + 3;
+            auto R = [&](int a, int bb) { return sb[lj + bb][li + a]; };
+            bc[C.off(I, J)] = restrict_point<T>(C, P, I, J, R);
+        }
+    }
+    for (int k = tid; k < TW * TH; k += NTB) {
+        const int li = k % TW + 3, lj = k / TW + 3;
+        const int gi = i0 + li - 3, gj = j0 + lj - 3;
+        if (in_dom(F, gi, gj)) xo[F.off(gi, gj)] = sx[lj][li];
+    }
+}
+
+// up on the fine level: x = xin + P e, black then red, z = x, dots.  The
+// tile's cells are taken in pairs (2m, 2m+1) of a row -- one black, one red
+// -- and each thread loads the couplings of its pairs once into registers:
+// the black pass uses the black cell's, the red pass the red cell's, so no
+// coefficient is read from memory twice.  The one-cell black ring around the
+// tile is smoothed from global reads.  b stays in LDS in FP64 for the dots.
+template <typename T, int TW, int TH>
+__global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __restrict__ b,
+                                             const T* __restrict__ xin, Dims C, Prol<T> P, const T* __restrict__ e,
+                                             double* __restrict__ z, const double* __restrict__ q, Slots Sl) {
+    constexpr int RW = TW + 4, RH = TH + 4;  // local (li,lj) <-> cell (i0+li-2, j0+lj-2)
+    constexpr int PPR = TW / 2;              // pairs per row
+    constexpr int PPT = PPR * TH / NTB;      // pairs per thread
+    static_assert(PPR * TH % NTB == 0, "tile pairs must divide among the threads");
+    __shared__ double sb[RH][RW];
+    __shared__ T sx[RH][RW];
+    int tx, ty;
+    xcd_tile(tx, ty);
+    const int i0 = tx * TW, j0 = ty * TH;
+    const int tid = threadIdx.x;
+    const size_t ld = F.ld;
+    // b and x + P e per 2x2 block of cells sharing one coarse parent (the
+    // region starts on an even cell, so blocks align with coarse cells)
+    {
+        constexpr int BW = RW / 2, BH = RH / 2;
+        constexpr int NL2 = (BW * BH + NTB - 1) / NTB;
+        const int Ib = i0 / 2 - 1, Jb = j0 / 2 - 1;
+#pragma unroll
+        for (int m = 0; m < NL2; ++m) {
+            const int k = tid + m * NTB;
+            if (k >= BW * BH) continue;
+            const int a = k % BW, c = k / BW;
+            const int I = Ib + a, J = Jb + c;
+            T xv[2][2] = {{T(0), T(0)}, {T(0), T(0)}};
+            double bv[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+            if (I >= 0 && J >= 0 && I < C.ni && J < C.nj) {
+                const size_t cc = C.off(I, J);
+                T* const* w = P.w;
+                const T e00 = e[cc], e10 = e[cc + 1], e01 = e[cc + C.ld], e11 = e[cc + C.ld + 1];
+                const T pe[2][2] = {
+                    {w[W_INJ][cc] * e00, w[W_XW][cc] * e00 + w[W_XE][cc] * e10},
+                    {w[W_YS][cc] * e00 + w[W_YN][cc] * e01,
+                     w[W_SW][cc] * e00 + w[W_SE][cc] * e10 + w[W_NW][cc] * e01 + w[W_NE][cc] * e11}};
+#pragma unroll
+                for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                    for (int di = 0; di < 2; ++di) {
+                        const int gi = 2 * I + di, gj = 2 * J + dj;
+                        if (gi < F.ni && gj < F.nj) {
+                            const size_t p = F.off(gi, gj);
+                            bv[dj][di] = b[p];
+                            xv[dj][di] = xin[p] + pe[dj][di];
+                        }
+                    }
+            }
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    sb[2 * c + dj][2 * a + di] = bv[dj][di];
+                    sx[2 * c + dj][2 * a + di] = xv[dj][di];
+                }
+        }
+    }
+    // couplings of the thread's pairs: [0] the black cell, [1] the red one
+    // (d, west, east, south, north)
+    T cf[PPT][2][5];
+    int cl[PPT][2];  // LDS position (lj * RW + li), -1 outside the domain
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const int k = tid + u * NTB;
+        const int lj = k / PPR + 2;
+        const int l0 = 2 * (k % PPR) + 2;
+        const int lb = l0 + ((lj + 1) & 1), lr = l0 + (lj & 1);  // black <=> (li + lj) odd
+        const int ll[2] = {lb, lr};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int gi = i0 + ll[h] - 2, gj = j0 + lj - 2;
+            cl[u][h] = -1;
+#pragma unroll
+            for (int t = 0; t < 5; ++t) cf[u][h][t] = T(0);
+            if (!in_dom(F, gi, gj)) continue;
+            const size_t p = F.off(gi, gj);
+            cf[u][h][0] = A.diag[p];
+            cf[u][h][1] = A.cE[p - 1];
+            cf[u][h][2] = A.cE[p];
+            cf[u][h][3] = A.cN[p - ld];
+            cf[u][h][4] = A.cN[p];
+            cl[u][h] = lj * RW + ll[h];
+        }
+    }
+    __syncthreads();
+    T* sxf = &sx[0][0];
+    const double* sbf = &sb[0][0];
+    auto gs = [&](const T (&c)[5], int pos) {
+        const T s = c[1] * sxf[pos - 1] + c[2] * sxf[pos + 1] + c[3] * sxf[pos - RW] + c[4] * sxf[pos + RW];
+        return c[0] > T(0) ? ((T)sbf[pos] + s) / c[0] : T(0);
+    };
+    // black: the tile's black cells from registers, the ring from memory (in
+    // place: black couples only to red)
+#pragma unroll
+    for (int u = 0; u < PPT; ++u)
+        if (cl[u][0] >= 0) sxf[cl[u][0]] = gs(cf[u][0], cl[u][0]);
+    {
+        // ring cells: rows -1 and TH (columns -1..TW), columns -1 and TW (rows 0..TH-1)
+        constexpr int NRING = 2 * (TW + 2) + 2 * TH;
+        for (int k = tid; k < NRING; k += NTB) {
+            int li, lj;
+            if (k < TW + 2) {
+                li = k + 1;
+                lj = 1;
+            } else if (k < 2 * (TW + 2)) {
+                li = k - (TW + 2) + 1;
+                lj = TH + 2;
+            } else {
+                const int r = k - 2 * (TW + 2);
+                li = (r & 1) ? TW + 2 : 1;
+                lj = r / 2 + 2;
+            }
+            if (((li + lj) & 1) == 0) continue;  // red
+            const int gi = i0 + li - 2, gj = j0 + lj - 2;
+            if (!in_dom(F, gi, gj)) continue;
+            const size_t p = F.off(gi, gj);
+            const T c[5] = {A.diag[p], A.cE[p - 1], A.cE[p], A.cN[p - ld], A.cN[p]};
+            sxf[lj * RW + li] = gs(c, lj * RW + li);
+        }
+    }
+    __syncthreads();
+    // red from registers; z = x, dots (b, z), (z, q)
+    double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int pos = cl[u][h];
+            if (pos < 0) continue;
+            const T v = h ? gs(cf[u][1], pos) : sxf[pos];
+            const int li = pos % RW, lj = pos / RW;
+            const size_t p = F.off(i0 + li - 2, j0 + lj - 2);
+            const double zv = (double)v;
+            z[p] = zv;
+            v0 += sbf[pos] * zv;
+            v1 += zv * q[p];
+        }
     }
     double vv[2] = {v0, v1};
     block_partials<2, NTB>(vv, Sl.part, Sl.max_blocks);
@@ -737,6 +763,9 @@ struct Acc<float> {
 
 inline dim3 tiles(const Dims& d, int tw, int th) { return dim3((d.ni + tw - 1) / tw, (d.nj + th - 1) / th); }
 
+// 64x16 tiles where they give at least two per CU, 32x8 below
+inline bool big(const Dims& d) { return (size_t)d.ni * d.nj >= (size_t)512 * 512; }
+
 inline Dims compact(const Dims& d) {
     Dims c = d;
     c.ld = d.ni + 1;
@@ -763,12 +792,16 @@ int vcycle(Ctx* c, const double* r, double* z, const double* q, Slots S, int* do
         const Level& L = c->lev[l];
         const Level& Cl = c->lev[l + 1];
         if (l == 0)
-            hipLaunchKernelGGL((k_down5<T, double>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
+            hipLaunchKernelGGL((k_down5<T, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
                                Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), Acc<T>::vec(Cl).b);
+        else if (big(L.d))
+            hipLaunchKernelGGL((k_down9<T, 64, 16>), tiles(L.d, 64, 16), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
+                               (const T*)Acc<T>::vec(L).b, Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L),
+                               Acc<T>::vec(Cl).b);
         else
-            hipLaunchKernelGGL((k_down<T, T, true, CW, CH>), tiles(L.d, CW, CH), dim3(NTB), 0, s, L.d,
-                               opv(Acc<T>::op(L)), (const T*)Acc<T>::vec(L).b, Acc<T>::vec(L).x, Cl.d,
-                               Acc<T>::pr(L), Acc<T>::vec(Cl).b);
+            hipLaunchKernelGGL((k_down9<T, 32, 8>), tiles(L.d, 32, 8), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
+                               (const T*)Acc<T>::vec(L).b, Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L),
+                               Acc<T>::vec(Cl).b);
     }
     // tail
     TailArgs<T> ta{};
@@ -808,12 +841,16 @@ int vcycle(Ctx* c, const double* r, double* z, const double* q, Slots S, int* do
         const Level& Cl = c->lev[l + 1];
         const T* e = (l + 1 == lt) ? Acc<T>::vec(Cl).x : Acc<T>::vec(Cl).r;
         if (l == 0)
-            hipLaunchKernelGGL((k_up5<T>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
+            hipLaunchKernelGGL((k_up5<T, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
                                (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e, z, q, S);
+        else if (big(L.d))
+            hipLaunchKernelGGL((k_up9<T, 64, 16>), tiles(L.d, 64, 16), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
+                               (const T*)Acc<T>::vec(L).b, (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e,
+                               Acc<T>::vec(L).r);
         else
-            hipLaunchKernelGGL((k_up<T, T, T, true, false, CW, CH>), tiles(L.d, CW, CH), dim3(NTB), 0, s, L.d,
-                               opv(Acc<T>::op(L)), (const T*)Acc<T>::vec(L).b, (const T*)Acc<T>::vec(L).x, Cl.d,
-                               Acc<T>::pr(L), e, Acc<T>::vec(L).r, q, S);
+            hipLaunchKernelGGL((k_up9<T, 32, 8>), tiles(L.d, 32, 8), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
+                               (const T*)Acc<T>::vec(L).b, (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e,
+                               Acc<T>::vec(L).r);
     }
     SG_HIP(hipGetLastError());
     return SG_OK;
@@ -846,20 +883,20 @@ int mg_launch_fine(Ctx* c, int which) {
     const bool mixed = c->prm.precision == SG_PREC_MIXED;
     if (which == 1) {
         if (mixed)
-            hipLaunchKernelGGL((k_down5<float, double>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->r,
+            hipLaunchKernelGGL((k_down5<float, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->r,
                                L.vf.x, Cl.d, L.Pf, Cl.vf.b);
         else
-            hipLaunchKernelGGL((k_down5<double, double>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
+            hipLaunchKernelGGL((k_down5<double, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
                                L.v.x, Cl.d, L.P, Cl.v.b);
     } else if (which == 2) {
         // the coarse correction sits in x when level 1 is the tail's first level, else in r
         const bool tail1 = c->nlev == 2 || (size_t)Cl.d.ni * Cl.d.nj <= (size_t)TAIL_MAX;
         if (mixed)
-            hipLaunchKernelGGL((k_up5<float>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->r,
+            hipLaunchKernelGGL((k_up5<float, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->r,
                                (const float*)L.vf.x, Cl.d, L.Pf, (const float*)(tail1 ? Cl.vf.x : Cl.vf.r), c->z,
                                c->q, S);
         else
-            hipLaunchKernelGGL((k_up5<double>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
+            hipLaunchKernelGGL((k_up5<double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
                                (const double*)L.v.x, Cl.d, L.P, (const double*)(tail1 ? Cl.v.x : Cl.v.r), c->z,
                                c->q, S);
     } else if (which == 3) {

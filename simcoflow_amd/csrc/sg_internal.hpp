@@ -219,6 +219,7 @@ struct Ctx {
     hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
     int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0}, graph_defl[2] = {0, 0};
     NnState nn;
+    double* step_dev = nullptr;   // device scalars of the time-step control (dt)
     hipEvent_t ev_nn0 = nullptr, ev_nn1 = nullptr;
     int use_graph = 1;
     ClsBuffers cls;
@@ -232,6 +233,19 @@ constexpr int TX = 64;  // one wavefront along x
 constexpr int TY = 4;
 inline dim3 grid2d(int ni, int nj) { return dim3((ni + TX - 1) / TX, (nj + TY - 1) / TY); }
 inline dim3 block2d() { return dim3(TX, TY); }
+
+// Tile of this workgroup with the grid's workgroups dealt out to the chip's 8
+// XCDs in contiguous bands: the hardware hands workgroup b to XCD b % 8, so
+// tile start(b % 8) + b / 8 gives each XCD (and its L2) one band of
+// neighbouring tiles, whose shared halo rows then hit in that L2.
+__device__ inline void xcd_tile(int& tx, int& ty) {
+    const int nb = (int)(gridDim.x * gridDim.y);
+    const int b = (int)(blockIdx.x + blockIdx.y * gridDim.x);
+    const int per = nb / 8, rem = nb % 8, x = b % 8;
+    const int t = x * per + (x < rem ? x : rem) + b / 8;
+    tx = t % (int)gridDim.x;
+    ty = t / (int)gridDim.x;
+}
 
 // error handling
 void set_error(const std::string& msg);
@@ -270,6 +284,9 @@ int learned_warm_start(Ctx* c);   // x = Galerkin-scaled prolongated net output
 int learned_features(Ctx* c, int level, void* X, float* t, float* mask, double* scale);
 int learned_enable(Ctx* c, const float* w, int level, double out_scale, int deflate);
 void learned_release(Ctx* c);
+// time-step control and boundary layer (sg_step.hip)
+int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, double asy, double* dt);
+int closed_box_bc(Ctx* c);
 
 // momentum predictor (sg_predictor.hip, sg_momentum.hip)
 int predictor_allocate(Ctx* c);

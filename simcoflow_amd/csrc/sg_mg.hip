@@ -22,7 +22,7 @@ namespace sg {
 namespace {
 
 // ---- prolongation weights --------------------------------------------------
-__global__ void __launch_bounds__(256) k_prol_edges(Dims F, OpV<double> A, Dims C, Prol<double> P) {
+__global__ void __launch_bounds__(256) k_prol_edges(Dims F, OpV<double> A, Dims C, Prol<double> P, Prol<float> Pf) {
     const int I = blockIdx.x * TX + threadIdx.x;
     const int J = blockIdx.y * TY + threadIdx.y;
     if (I >= C.ni || J >= C.nj) return;
@@ -61,9 +61,16 @@ __global__ void __launch_bounds__(256) k_prol_edges(Dims F, OpV<double> A, Dims 
     P.w[W_XE][cp] = xE;
     P.w[W_YS][cp] = yS;
     P.w[W_YN][cp] = yN;
+    if (Pf.w[0]) {
+        Pf.w[W_INJ][cp] = (float)P.w[W_INJ][cp];
+        Pf.w[W_XW][cp] = (float)xW;
+        Pf.w[W_XE][cp] = (float)xE;
+        Pf.w[W_YS][cp] = (float)yS;
+        Pf.w[W_YN][cp] = (float)yN;
+    }
 }
 
-__global__ void __launch_bounds__(256) k_prol_center(Dims F, OpV<double> A, Dims C, Prol<double> P) {
+__global__ void __launch_bounds__(256) k_prol_center(Dims F, OpV<double> A, Dims C, Prol<double> P, Prol<float> Pf) {
     const int I = blockIdx.x * TX + threadIdx.x;
     const int J = blockIdx.y * TY + threadIdx.y;
     if (I >= C.ni || J >= C.nj) return;
@@ -90,51 +97,107 @@ __global__ void __launch_bounds__(256) k_prol_center(Dims F, OpV<double> A, Dims
     P.w[W_SE][cp] = wse;
     P.w[W_NW][cp] = wnw;
     P.w[W_NE][cp] = wne;
+    if (Pf.w[0]) {
+        Pf.w[W_SW][cp] = (float)wsw;
+        Pf.w[W_SE][cp] = (float)wse;
+        Pf.w[W_NW][cp] = (float)wnw;
+        Pf.w[W_NE][cp] = (float)wne;
+    }
 }
 
 // ---- Galerkin coarse operator ---------------------------------------------
-__global__ void __launch_bounds__(256) k_rap(Dims F, OpV<double> A, Dims C, Prol<double> P, Op<double> Ac) {
+// Ac(C1,C2) = sum_f sum_g P(f,C1) A(f,g) P(g,C2) for C1 = (I,J) and the five
+// stored neighbours C2 - C1 in {(0,0),(1,0),(0,1),(1,1),(-1,1)}.  The fine
+// cells f with P(f,C1) != 0 are (2I+a, 2J+b), |a|,|b| <= 1; g = f + (di,dj)
+// runs over the stencil of f.  Every offset is a compile-time constant once
+// the loops are unrolled, so the parity cases of the prolongation and the
+// target each product lands in are resolved statically: the kernel is a
+// straight sum of products of coalesced loads.  Ring cells of the operator
+// and of the weights are zero, so no bounds tests are needed.
+__device__ __forceinline__ int rap_target(int dI, int dJ) {
+    return (dI == 0 && dJ == 0) ? 0 : (dI == 1 && dJ == 0) ? 1 : (dI == 0 && dJ == 1) ? 2
+         : (dI == 1 && dJ == 1) ? 3 : (dI == -1 && dJ == 1) ? 4 : -1;
+}
+
+template <bool NINE>
+__global__ void __launch_bounds__(256) k_rap(Dims F, OpV<double> A, Dims C, Prol<double> P, Op<double> Ac, Op<float> Af) {
     const int I = blockIdx.x * TX + threadIdx.x;
     const int J = blockIdx.y * TY + threadIdx.y;
     if (I >= C.ni || J >= C.nj) return;
-    // the five coarse neighbours whose coupling this point stores
-    const int DI[5] = {I, I + 1, I, I + 1, I - 1};
-    const int DJ[5] = {J, J, J + 1, J + 1, J + 1};
+    const size_t ldf = F.ld, ldc = C.ld;
+    const size_t c = C.off(I, J);
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double* const* w = P.w;
-    for (int b = -1; b <= 1; ++b) {
-        for (int a = -1; a <= 1; ++a) {
-            const int fi = 2 * I + a, fj = 2 * J + b;
-            if (fi < 0 || fj < 0 || fi >= F.ni || fj >= F.nj) continue;
-            const double pfc = pweight<double>(C, w, fi, fj, I, J);
-            if (pfc == 0.0) continue;
-            for (int dj = -1; dj <= 1; ++dj) {
-                for (int di = -1; di <= 1; ++di) {
-                    const double afg = aentry<double>(F, A, fi, fj, di, dj);
-                    if (afg == 0.0) continue;
-                    const int gi = fi + di, gj = fj + dj;
-                    const double s = pfc * afg;
+    // P(g, parent) for g = (2I+ga, 2J+gb): add s * weight to the targets
+    auto scatter = [&](double s, int ga, int gb) {
+        const bool ox = ga & 1, oy = gb & 1;
+        if (!ox && !oy) {
+            const int pI = ga / 2, pJ = gb / 2, k = rap_target(pI, pJ);
+            if (k >= 0) acc[k] += s * P.w[W_INJ][c + (long)pJ * (long)ldc + pI];
+        } else if (ox && !oy) {
+            const int I0 = (ga - 1) / 2, J0 = gb / 2;
+            const size_t q = c + (long)J0 * (long)ldc + I0;
+            const int k0 = rap_target(I0, J0), k1 = rap_target(I0 + 1, J0);
+            if (k0 >= 0) acc[k0] += s * P.w[W_XW][q];
+            if (k1 >= 0) acc[k1] += s * P.w[W_XE][q];
+        } else if (!ox && oy) {
+            const int I0 = ga / 2, J0 = (gb - 1) / 2;
+            const size_t q = c + (long)J0 * (long)ldc + I0;
+            const int k0 = rap_target(I0, J0), k1 = rap_target(I0, J0 + 1);
+            if (k0 >= 0) acc[k0] += s * P.w[W_YS][q];
+            if (k1 >= 0) acc[k1] += s * P.w[W_YN][q];
+        } else {
+            const int I0 = (ga - 1) / 2, J0 = (gb - 1) / 2;
+            const size_t q = c + (long)J0 * (long)ldc + I0;
+            const int k0 = rap_target(I0, J0), k1 = rap_target(I0 + 1, J0);
+            const int k2 = rap_target(I0, J0 + 1), k3 = rap_target(I0 + 1, J0 + 1);
+            if (k0 >= 0) acc[k0] += s * P.w[W_SW][q];
+            if (k1 >= 0) acc[k1] += s * P.w[W_SE][q];
+            if (k2 >= 0) acc[k2] += s * P.w[W_NW][q];
+            if (k3 >= 0) acc[k3] += s * P.w[W_NE][q];
+        }
+    };
 #pragma unroll
-                    for (int k = 0; k < 5; ++k) acc[k] += s * pweight<double>(C, w, gi, gj, DI[k], DJ[k]);
-                }
+    for (int b = -1; b <= 1; ++b) {
+#pragma unroll
+        for (int a = -1; a <= 1; ++a) {
+            // P(f, (I,J)) for f = (2I+a, 2J+b)
+            double wf;
+            if (a == 0 && b == 0) wf = P.w[W_INJ][c];
+            else if (b == 0) wf = a > 0 ? P.w[W_XW][c] : P.w[W_XE][c - 1];
+            else if (a == 0) wf = b > 0 ? P.w[W_YS][c] : P.w[W_YN][c - ldc];
+            else if (a > 0) wf = b > 0 ? P.w[W_SW][c] : P.w[W_NW][c - ldc];
+            else wf = b > 0 ? P.w[W_SE][c - 1] : P.w[W_NE][c - ldc - 1];
+            const size_t f = F.off(2 * I + a, 2 * J + b);
+            scatter(wf * A.diag[f], a, b);
+            scatter(wf * -A.cE[f], a + 1, b);
+            scatter(wf * -A.cE[f - 1], a - 1, b);
+            scatter(wf * -A.cN[f], a, b + 1);
+            scatter(wf * -A.cN[f - ldf], a, b - 1);
+            if (NINE) {
+                scatter(wf * -A.cNE[f], a + 1, b + 1);
+                scatter(wf * -A.cNE[f - ldf - 1], a - 1, b - 1);
+                scatter(wf * -A.cNW[f], a - 1, b + 1);
+                scatter(wf * -A.cNW[f - ldf + 1], a + 1, b - 1);
             }
         }
     }
-    const size_t cp = C.off(I, J);
-    Ac.diag[cp] = acc[0] > 0.0 ? acc[0] : 0.0;
-    Ac.cE[cp] = (I + 1 < C.ni) ? -acc[1] : 0.0;
-    Ac.cN[cp] = (J + 1 < C.nj) ? -acc[2] : 0.0;
-    Ac.cNE[cp] = (I + 1 < C.ni && J + 1 < C.nj) ? -acc[3] : 0.0;
-    Ac.cNW[cp] = (I > 0 && J + 1 < C.nj) ? -acc[4] : 0.0;
-}
-
-// FP32 copy of an operator / of prolongation weights (interior only)
-__global__ void __launch_bounds__(256) k_to_f32(Dims D, const double* __restrict__ a, float* __restrict__ b) {
-    const int i = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    if (i >= D.ni || j >= D.nj) return;
-    const size_t p = D.off(i, j);
-    b[p] = (float)a[p];
+    const double d = acc[0] > 0.0 ? acc[0] : 0.0;
+    const double e = (I + 1 < C.ni) ? -acc[1] : 0.0;
+    const double n = (J + 1 < C.nj) ? -acc[2] : 0.0;
+    const double ne = (I + 1 < C.ni && J + 1 < C.nj) ? -acc[3] : 0.0;
+    const double nw = (I > 0 && J + 1 < C.nj) ? -acc[4] : 0.0;
+    Ac.diag[c] = d;
+    Ac.cE[c] = e;
+    Ac.cN[c] = n;
+    Ac.cNE[c] = ne;
+    Ac.cNW[c] = nw;
+    if (Af.diag) {
+        Af.diag[c] = (float)d;
+        Af.cE[c] = (float)e;
+        Af.cN[c] = (float)n;
+        Af.cNE[c] = (float)ne;
+        Af.cNW[c] = (float)nw;
+    }
 }
 
 // ---- dense inverse of the coarsest operator (one workgroup) ---------------
@@ -262,19 +325,15 @@ int mg_setup(Ctx* c) {
         Level& F = c->lev[l];
         Level& C = c->lev[l + 1];
         OpV<double> A = opv(F.A);
-        hipLaunchKernelGGL(k_prol_edges, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P);
-        hipLaunchKernelGGL(k_prol_center, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P);
-        hipLaunchKernelGGL(k_rap, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P, C.A);
+        const Prol<float> Pf = mixed ? F.Pf : Prol<float>{};
+        const Op<float> Af = mixed ? C.Af : Op<float>{};
+        hipLaunchKernelGGL(k_prol_edges, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P, Pf);
+        hipLaunchKernelGGL(k_prol_center, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P, Pf);
+        if (l == 0)
+            hipLaunchKernelGGL(k_rap<false>, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P, C.A, Af);
+        else
+            hipLaunchKernelGGL(k_rap<true>, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P, C.A, Af);
         SG_HIP(hipGetLastError());
-        if (mixed) {
-            for (int w = 0; w < 9; ++w)
-                hipLaunchKernelGGL(k_to_f32, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, C.d, F.P.w[w], F.Pf.w[w]);
-            const double* src[5] = {C.A.diag, C.A.cE, C.A.cN, C.A.cNE, C.A.cNW};
-            float* dst[5] = {C.Af.diag, C.Af.cE, C.Af.cN, C.Af.cNE, C.Af.cNW};
-            for (int k = 0; k < 5; ++k)
-                hipLaunchKernelGGL(k_to_f32, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, C.d, src[k], dst[k]);
-            SG_HIP(hipGetLastError());
-        }
     }
     Level& Lc = c->lev[c->nlev - 1];
     hipLaunchKernelGGL(k_coarse_inverse, dim3(1), dim3(256), 0, s, Lc.d, opv(Lc.A), c->coarse_inv, c->coarse_inv_f);

@@ -80,30 +80,51 @@ __global__ void __launch_bounds__(256) k_dot_mu(Dims D, const double* __restrict
 }
 
 // p_new = z + beta p_old (- mu w with deflation) ; q = A p_new ;
-// partials of (p_new, q)
+// partials of (p_new, q).  A 64x16 tile: p_new of the tile and its one-cell
+// halo is formed once in LDS, then the stencil reads it from there.
+constexpr int DW = 64, DH = 16;
 __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restrict__ diag,
                                                    const double* __restrict__ cE, const double* __restrict__ cN,
                                                    const double* __restrict__ z, const double* __restrict__ pold,
                                                    const double* __restrict__ w, double* __restrict__ pnew,
                                                    double* __restrict__ q, Slots S, const PcgScalars* sc) {
     if (!sc->active) return;
+    constexpr int RW = DW + 2, RH = DH + 2;
+    __shared__ double sp[RH][RW];
     const double beta = sc->beta;
     const double mu = w ? sc->mu : 0.0;
-    const int i = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
+    int tx, ty;
+    xcd_tile(tx, ty);
+    const int i0 = tx * DW, j0 = ty * DH;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < RW * RH; k += 256) {
+        const int li = k % RW, lj = k / RW;
+        const int gi = i0 + li - 1, gj = j0 + lj - 1;
+        double v = 0.0;
+        if (gi >= 0 && gj >= 0 && gi < D.ni && gj < D.nj) {
+            const size_t p = D.off(gi, gj);
+            v = w ? z[p] + beta * pold[p] - mu * w[p] : z[p] + beta * pold[p];
+        }
+        sp[lj][li] = v;
+    }
+    __syncthreads();
     double v[1] = {0.0};
-    if (i < D.ni && j < D.nj) {
-        const size_t p = D.off(i, j);
-        const size_t ld = D.ld;
-        auto pn = [&](size_t k) { return w ? z[k] + beta * pold[k] - mu * w[k] : z[k] + beta * pold[k]; };
-        const double pc = pn(p);
-        const double qv = diag[p] * pc - cE[p - 1] * pn(p - 1) - cE[p] * pn(p + 1) - cN[p - ld] * pn(p - ld) -
-                          cN[p] * pn(p + ld);
+    const size_t ld = D.ld;
+#pragma unroll
+    for (int m = 0; m < DW * DH / 256; ++m) {
+        const int k = tid + m * 256;
+        const int li = k % DW + 1, lj = k / DW + 1;
+        const int gi = i0 + li - 1, gj = j0 + lj - 1;
+        if (gi >= D.ni || gj >= D.nj) continue;
+        const size_t p = D.off(gi, gj);
+        const double pc = sp[lj][li];
+        const double qv = diag[p] * pc - cE[p - 1] * sp[lj][li - 1] - cE[p] * sp[lj][li + 1] -
+                          cN[p - ld] * sp[lj - 1][li] - cN[p] * sp[lj + 1][li];
         pnew[p] = pc;
         q[p] = qv;
-        v[0] = pc * qv;
+        v[0] += pc * qv;
     }
-    block_partials<1, NT>(v, S.part, S.max_blocks);
+    block_partials<1, 256>(v, S.part, S.max_blocks);
 }
 
 // x += alpha p ; r -= alpha q ; partials of (r,r)
@@ -258,9 +279,10 @@ int enqueue_iteration(Ctx* c, int parity) {
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S6, nb, (int)FIN_MU, c->sc, 0.0);
         w = c->nn.w;
     }
-    hipLaunchKernelGGL(k_dir_apply, g, b, 0, s, D, L0.A.diag, L0.A.cE, L0.A.cN, (const double*)c->z,
+    const dim3 gd((c->ni + DW - 1) / DW, (c->nj + DH - 1) / DH);
+    hipLaunchKernelGGL(k_dir_apply, gd, dim3(256), 0, s, D, L0.A.diag, L0.A.cE, L0.A.cN, (const double*)c->z,
                        (const double*)pold, w, pnew, c->q, S2, (const PcgScalars*)c->sc);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S2, nb, (int)FIN_PQ, c->sc, 0.0);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S2, (int)(gd.x * gd.y), (int)FIN_PQ, c->sc, 0.0);
     hipLaunchKernelGGL(k_update, g, b, 0, s, D, (const double*)pnew, (const double*)c->q, c->x, c->r, S3,
                        (const PcgScalars*)c->sc);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S3, nb, (int)FIN_RR, c->sc, 0.0);

@@ -1,0 +1,127 @@
+// Time-step control around the hot path: the CFL step of ComputeTimeStep
+// (Solver_Mod.f90:321-344) as a device min-reduction, and the ghost layer of
+// a closed box (the box form of Boundary_Condition_Var,
+// StateVariables.f90:118-211: pressure and tangential velocity copied into
+// the ghosts, p = 0 above the open top, no flow through the walls).
+// Compiled without FP contraction: the bounds round exactly as the
+// reference's expressions, so the step equals the oracle's bit for bit.
+#include <cmath>
+
+#include "sg_internal.hpp"
+
+namespace sg {
+
+namespace {
+
+constexpr int NTS = 256;
+
+struct StepIn {
+    int ni, nj;
+    const double *udx, *vdy, *u, *v;
+    double cfl, Fr, vint, uref, vs, asy;
+};
+
+__device__ inline size_t gh(int ni, int i, int j) { return (size_t)i + (size_t)j * (size_t)(ni + 2); }
+
+// per-block minimum of the step bounds over the cells of a grid-stride range
+__global__ void __launch_bounds__(NTS) k_dt(StepIn a, double* part) {
+    __shared__ double red[NTS / 64];
+    double dt = 1.0;
+    const int n = a.ni * a.nj;
+    for (int e = blockIdx.x * NTS + threadIdx.x; e < n; e += gridDim.x * NTS) {
+        const int i = e % a.ni + 1, j = e / a.ni + 1;
+        const double dx = a.udx[i - 1], dy = a.vdy[j - 1];
+        const double uu = a.u[gh(a.ni, i, j)], vv = a.v[gh(a.ni, i, j)];
+        dt = fmin(dt, a.cfl * dx / fabs(uu));
+        dt = fmin(dt, a.cfl * dy / fabs(vv));
+        dt = fmin(fmin(dt, 2.0 * a.cfl * dy / (fabs(vv) + sqrt(vv * vv + 4.0 * dy / a.Fr))),
+                  a.cfl * dy / (a.vint / a.uref));
+        dt = fmin(dt, a.cfl * dy / (a.vint / a.uref));
+        dt = fmin(dt, a.cfl * dy / fabs(a.vs));
+        dt = fmin(dt, (-a.vs + sqrt(fmax(fabs(a.vs * a.vs + 2.0 * fabs(a.asy) * 0.05 * dy), 1e-30))) /
+                          fabs(a.asy + 1.e-20));
+        if (i == 1) dt = fmin(dt, a.cfl * a.udx[0] / fabs(a.u[gh(a.ni, 0, j)]));
+    }
+    for (int o = 32; o >= 1; o >>= 1) dt = fmin(dt, __shfl_xor(dt, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int w = 1; w < NTS / 64; ++w) m = fmin(m, red[w]);
+        part[blockIdx.x] = m;
+    }
+}
+
+__global__ void __launch_bounds__(NTS) k_dt_fin(const double* part, int nb, double* out) {
+    __shared__ double red[NTS / 64];
+    double m = 1.0;
+    for (int b = threadIdx.x; b < nb; b += NTS) m = fmin(m, part[b]);
+    for (int o = 32; o >= 1; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int w = 1; w < NTS / 64; ++w) r = fmin(r, red[w]);
+        out[0] = r;
+    }
+}
+
+// one thread per boundary position: k < nj the columns, else the rows
+__global__ void __launch_bounds__(NTS) k_box_bc(int ni, int nj, double* __restrict__ u, double* __restrict__ v,
+                                                double* __restrict__ p) {
+    const int k = blockIdx.x * NTS + threadIdx.x;
+    if (k < nj) {
+        const int j = k + 1;
+        p[gh(ni, 0, j)] = p[gh(ni, 1, j)];
+        p[gh(ni, ni + 1, j)] = p[gh(ni, ni, j)];
+        u[gh(ni, 0, j)] = 0.0;
+        u[gh(ni, ni, j)] = 0.0;
+        u[gh(ni, ni + 1, j)] = 0.0;
+        v[gh(ni, 0, j)] = v[gh(ni, 1, j)];
+        v[gh(ni, ni + 1, j)] = v[gh(ni, ni, j)];
+    }
+}
+
+__global__ void __launch_bounds__(NTS) k_box_bc_rows(int ni, int nj, double* __restrict__ u, double* __restrict__ v,
+                                                     double* __restrict__ p) {
+    const int k = blockIdx.x * NTS + threadIdx.x;
+    if (k < ni) {
+        const int i = k + 1;
+        p[gh(ni, i, 0)] = p[gh(ni, i, 1)];
+        u[gh(ni, i, 0)] = u[gh(ni, i, 1)];
+        v[gh(ni, i, 0)] = 0.0;
+        p[gh(ni, i, nj + 1)] = 0.0;
+        u[gh(ni, i, nj + 1)] = u[gh(ni, i, nj)];
+        v[gh(ni, i, nj + 1)] = v[gh(ni, i, nj)];
+    }
+}
+
+}  // namespace
+
+int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, double asy, double* dt) {
+    StepIn a{c->ni, c->nj, c->udx, c->vdy, (const double*)c->field[SG_F_U], (const double*)c->field[SG_F_V],
+             cfl, c->prm.Fr, vint, uref, vs, asy};
+    int nb = (c->ni * c->nj + NTS - 1) / NTS;
+    if (nb > c->red.max_blocks) nb = c->red.max_blocks;
+    if (nb > 2048) nb = 2048;
+    double* part = c->red.partials + (size_t)7 * NRED * c->red.max_blocks;
+    hipLaunchKernelGGL(k_dt, dim3(nb), dim3(NTS), 0, c->stream, a, part);
+    hipLaunchKernelGGL(k_dt_fin, dim3(1), dim3(NTS), 0, c->stream, (const double*)part, nb, c->step_dev);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipMemcpyAsync(dt, c->step_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int closed_box_bc(Ctx* c) {
+    double* u = (double*)c->field[SG_F_U];
+    double* v = (double*)c->field[SG_F_V];
+    double* p = (double*)c->field[SG_F_P];
+    // columns first, then rows (the rows' copies see the corrected columns)
+    hipLaunchKernelGGL(k_box_bc, dim3((c->nj + NTS - 1) / NTS), dim3(NTS), 0, c->stream, c->ni, c->nj, u, v, p);
+    hipLaunchKernelGGL(k_box_bc_rows, dim3((c->ni + NTS - 1) / NTS), dim3(NTS), 0, c->stream, c->ni, c->nj, u, v, p);
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+}  // namespace sg

@@ -1,0 +1,97 @@
+"""Time stepping of the hot path on one device context.
+
+``TimeStepper.step`` follows ``AdamBasforthCrankNicolson``
+(Solver_Mod.f90:246-319) for solids at rest in a closed box:
+
+  CopyNewCell (time-n copies of the P / U / V interface fields)
+  ComputeTimeStep (:321-344)                       -> sg_time_step_size
+  if itt > 1: Coupled_LS_VOF (Clsvof_Mod.f90:730)  -> sg_clsvof_advect_resident
+              Initial_ClsVofUV (:402) for U and V  -> sg_interface_to_uv_resident
+  Boundary_Condition_Var (closed-box form)         -> sg_closed_box_bc
+  UpdatePUV (Solver_Mod.f90:312):
+      Predictor_UV                                 -> sg_predictor_resident
+      PoissonEquationSolver                        -> sg_poisson_build/solve_resident
+      velocity correction                          -> sg_velocity_correct_resident
+
+Every field stays on the device between steps; the host reads dt and the
+solver reports.  Left out, as for the rest of this build: moving solids
+(ObjectMovement / SolidVolumeFraction / Grid_Preprocess are not re-run, the
+cut-cell geometry is the one given at set-up), the particle model and the
+wave channel's inlet/outlet profiles.  Non-dimensional groups use Uref = 1
+m/s and Lref = 1 m (StateVariables.f90:97-98): Re = 1/nu_l, Fr = 1/sqrt(g).
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from . import cases as C
+from ._lib import F, check
+from .projection import Projection, params_for
+
+
+def physics(fluids):
+    nu_l = fluids.mu_l / fluids.rho_l
+    return dict(nu_l=nu_l, nu_g=fluids.mu_g / fluids.rho_g, nu_ref=nu_l, Re=1.0 / nu_l, Fr=1.0 / math.sqrt(fluids.g))
+
+
+class TimeStepper:
+    def __init__(self, case, precision=_lib.SG_PREC_FP64, tol=None, cfl=0.25, vint=0.0, uref=1.0, max_iter=500,
+                 learned=None):
+        self.case = case
+        g = case.grid
+        self.cfl, self.vint, self.uref = cfl, vint, uref
+        ph = physics(case.fluids)
+        self.params = params_for(case.fluids, tol=case.tol if tol is None else tol, precision=precision,
+                                 max_iter=max_iter, bc=case.bc, Re=ph["Re"], Fr=ph["Fr"],
+                                 warm_start=2 if learned else 0)
+        # params_for takes nu from the fluids already
+        self.pj = pj = Projection(g, self.params)
+        pj.set_geometry(case.geo)
+        U, V = C.staggered_geometry(g, case.geo, case.solid)
+        pj.set_staggered_geometry(0, U)
+        pj.set_staggered_geometry(1, V)
+        itf = case.itf
+        pj.upload("VOF", itf.vof)
+        pj.upload("PHI", itf.phi)
+        pj.upload("NX", itf.nx)
+        pj.upload("NY", itf.ny)
+        z = np.zeros((g.nj + 2, g.ni + 2))
+        for name in ("U", "V", "P", "UPRED", "VPRED", "PP", "DPU", "DPV", "FLUXN1U", "FLUXN1V"):
+            pj.upload(name, z)
+        check(pj.lib.sg_interface_to_uv_resident(pj.ctx))
+        if learned:
+            from . import learned as L
+            L.enable(pj, *learned)
+        self.itt = 0
+        self.time = 0.0
+        self.last = {}
+
+    def close(self):
+        self.pj.close()
+
+    def step(self):
+        pj, lib, ctx = self.pj, self.pj.lib, self.pj.ctx
+        self.itt += 1
+        for dst, src in (("VOFO", "VOF"), ("PHIO", "PHI"), ("NXO", "NX"), ("NYO", "NY"), ("UVOFO", "UVOF"),
+                         ("VVOFO", "VVOF")):
+            check(lib.sg_field_copy(ctx, F[dst], F[src]))
+        dt = _lib.ctypes.c_double(0.0)
+        check(lib.sg_time_step_size(ctx, self.cfl, self.vint, self.uref, 0.0, 0.0, _lib.ctypes.byref(dt)))
+        dt = dt.value
+        if self.itt > 1:
+            pj.clsvof_advect_resident(dt, self.time)
+            check(lib.sg_interface_to_uv_resident(ctx))
+        check(lib.sg_closed_box_bc(ctx))
+        pinfo = pj.predictor_resident(dt, vb=0.0, first_step=(self.itt == 1))
+        pj.build(0.0)
+        sinfo = pj.solve()
+        pj.velocity_correct(0.0)
+        self.time += dt
+        self.last = dict(dt=dt, predictor=pinfo.as_dict(), poisson=sinfo.as_dict())
+        return self.last
+
+    def state(self):
+        """Host copies of the interface and the velocity / pressure fields."""
+        return {k: self.pj.download(n) for k, n in (("vof", "VOF"), ("phi", "PHI"), ("nx", "NX"), ("ny", "NY"),
+                                                    ("u", "U"), ("v", "V"), ("p", "P"))}

@@ -1,0 +1,76 @@
+"""The oracle's time step, in the order of simcoflow_amd.solver.TimeStepper
+(AdamBasforthCrankNicolson, Solver_Mod.f90:246-319, solids at rest, closed
+box): copies of the time-n interface, ComputeTimeStep, Coupled_LS_VOF and
+Initial_ClsVofUV (from the second step), the closed-box ghost layer,
+Predictor_UV (direct sparse momentum solves), PoissonEquationSolver (the
+oracle's PCG-multigrid to `tol`) and the velocity correction."""
+from types import SimpleNamespace
+
+import numpy as np
+
+import predictor_cases as P
+from simcoflow_amd import cases as C
+
+
+class OracleStepper:
+    def __init__(self, O, case, cfl=0.25, vint=0.0, uref=1.0, tol=1e-12):
+        self.O, self.case, self.g = O, case, case.grid
+        self.cfl, self.vint, self.uref, self.tol = cfl, vint, uref, tol
+        g, geo = case.grid, case.geo
+        self.cg = O.ClsGeom(g, geo)
+        self.U, self.V = C.staggered_geometry(g, geo, case.solid)
+        itf = case.itf
+        self.vof, self.phi = itf.vof.copy(), itf.phi.copy()
+        self.nx, self.ny = itf.nx.copy(), itf.ny.copy()
+        gz = np.zeros((g.nj + 2, g.ni + 2))
+        self.u, self.v, self.p = gz.copy(), gz.copy(), gz.copy()
+        self.upred, self.vpred, self.DpU, self.DpV = gz.copy(), gz.copy(), gz.copy(), gz.copy()
+        self.fluxn1 = np.zeros((2, g.nj + 2, g.ni + 2))
+        self.uvof = self._to_uv(0)
+        self.vvof = self._to_uv(1)
+        self.phys = P.phys_of(case.fluids)
+        self.itt = 0
+        self.time = 0.0
+
+    def _to_uv(self, which):
+        st = self.U if which == 0 else self.V
+        return self.O.interface_to_uv(self.cg, which, self.vof, self.phi, self.nx, self.ny, st.vofS, st.phiS)[0]
+
+    def step(self):
+        O, g, case = self.O, self.g, self.case
+        geo = case.geo
+        self.itt += 1
+        vofo, phio, nxo, nyo = self.vof.copy(), self.phi.copy(), self.nx.copy(), self.ny.copy()
+        uvofo, vvofo = self.uvof.copy(), self.vvof.copy()
+        dt = O.time_step(g, self.u, self.v, self.cfl, self.phys["Fr"], self.vint, self.uref)
+        if self.itt > 1:
+            self.vof, self.phi, self.nx, self.ny = O.clsvof_advect(self.cg, self.vof, self.phi, self.u, self.v, dt,
+                                                                   time=self.time)
+            self.uvof = self._to_uv(0)
+            self.vvof = self._to_uv(1)
+        O.closed_box_bc(self.u, self.v, self.p)
+        d = dict(grid=g, U=self.U, V=self.V, uvo=uvofo, uvn=self.uvof, vvo=vvofo, vvn=self.vvof,
+                 pcell=dict(vof=vofo, phi=phio, nx=nxo, ny=nyo, vofS=geo.vofS, nxS=geo.nxS, nyS=geo.nyS,
+                            phiS=geo.phiS),
+                 u=self.u, v=self.v, phys=self.phys, dt=dt, fluxn1=self.fluxn1, DpU=self.DpU, DpV=self.DpV,
+                 upred=self.upred, vpred=self.vpred)
+        r = P.oracle_predictor(O, d, first_step=(self.itt == 1))
+        self.fluxn1, self.DpU, self.DpV = r["fluxn1"], r["DpU"], r["DpV"]
+        self.upred, self.vpred = r["upred"], r["vpred"]
+        f = SimpleNamespace(vof=self.vof, phi=self.phi, vofS=geo.vofS)
+        M = O.poisson_matrix(g, geo, O.poisson_coef(g, f, self.DpU, self.DpV, case.fluids), case.bc)
+        b = O.poisson_rhs(g, geo, self.upred, self.vpred)
+        x, it, rr = O.pcg_mg_solve(g, M, b, tol=self.tol, max_iter=3000)
+        Pp = O.pressure_scatter_bc(g, geo, x, case.bc)
+        self.u, self.v, self.p, mres = O.velocity_correct(g, geo, f, case.fluids, self.DpU, self.DpV, self.upred,
+                                                          self.vpred, Pp, self.u, self.v, self.p)
+        self.time += dt
+        return dict(dt=dt, iterations=it, mres=float(np.abs(mres).max()))
+
+
+def gas_centroid(g, vof, vofS):
+    """Centroid of the gas (1 - vof - vofS weighted) in cell units."""
+    X, Y = np.meshgrid(g.xc, g.yc)
+    w = np.clip(1.0 - vof - vofS, 0.0, None) * (g.dx[None, :] * g.dy[:, None])
+    s = w.sum()
+    return float((w * X).sum() / s / g.dx[0]), float((w * Y).sum() / s / g.dy[0])
