@@ -1,17 +1,30 @@
-"""Benchmark of the projection step on one MI355X.
+"""Benchmark of the projection step (and the full time step) on one MI355X.
 
 Workload: the 64-bubble swarm on a 2048x2048 grid (1x1 m box, liquid depth
 0.8 m, density ratio 1000:1), the configuration the solver metric is quoted
 on.  One step = one projection step from device-resident inputs: operator and
 RHS build from the predictor fields (ComputePossionMatrixCoefficient /
-SetPoissonMatrix / SetPoissonVectors), PCG + multigrid solve from x0 = 0 to a
-relative residual of 1e-8, pressure scatter + ghosts, velocity correction and
-mass residual (UpdatePUV's correction).  Every step redoes all of it.
+SetPoissonMatrix / SetPoissonVectors), PCG + multigrid solve to a relative
+residual of 1e-8, pressure scatter + ghosts, velocity correction and mass
+residual (UpdatePUV's correction).  Every step redoes all of it.
 
-Prints one JSON line.  ``value`` is Mcell-steps/s of the headline path (the
-mixed-precision solve); the plain-FP64 path is timed beside it.
+Three solver paths are timed on the same inputs:
+  * plain FP64 PCG-multigrid from x0 = 0 (the reference's start);
+  * mixed precision (FP32 multigrid cycle, FP64 outer PCG, dots and
+    coarsest solve) from x0 = 0;
+  * learned + mixed: the mixed solver started from the learned warm start
+    (the deflation vector needs FP64 directions; the net is trained inside this run, with the
+    library's own kernels, on (rhs, p) pairs the plain solver makes for
+    seeded 512x512 swarms; it runs on the 256x256 level of the hierarchy).
+``value`` is the learned + mixed path's Mcell-steps/s (the headline metric);
+the others are reported beside it.  A full time step of the same swarm
+(time step, transport, interface on U/V cells, predictor, projection,
+correction) is timed after it.
+
+Prints one JSON line.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -28,26 +41,75 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ni", type=int, default=2048)
     ap.add_argument("--nj", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip training and the learned path")
+    ap.add_argument("--no-full-step", action="store_true", help="skip the full time step")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
 
-def run_path(case, fields, precision, steps, warmup, tol):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# Algorithmic bytes per fine cell and PCG iteration (compulsory traffic at
+# the stored precision): operator apply 5 words (FP64 operator, the vector
+# counted at FP64 too), PCG vector work 9 words of the vector precision (+ 2 FP32 words when the
+# vectors are FP32: x stays FP64), one V-cycle 26 words of the multigrid
+# precision; plus 15 FP64 words per solve for the coefficient and RHS build.
+def solve_bytes(n, iters, w):
+    x_extra = 8 if w == 4 else 0
+    return n * (15 * 8 + iters * (5 * 8 + 9 * w + x_extra + 26 * w))
+
+
+# Algorithmic bytes per fine cell of the fine-level up-sweep k_up5 (the
+# kernel with the largest share of a solve): reads b = PCG residual, x after
+# pre-smoothing, diag/cE/cN, the 9 prolongation weights and the coarse
+# correction (both at 1/4 resolution) and q (for the fused dot); writes z.
+# w = word size of the solve (FP64: 8; mixed precision: 4, vectors included).
+def up_sweep_bytes_per_cell(w):
+    return w + w + 3 * w + 9 * w / 4 + w / 4 + w + w
+
+
+def kernel_roofline(pj, n, precision):
+    """Average duration of the fine up-sweep (and, for calibration, a plain
+    copy), timed with HIP events on the library's stream by sg_time_kernel."""
+    from simcoflow_amd import _lib
+    fn = _lib.load().sg_time_kernel
+    w = 4 if precision == 1 else 8
+    out = {}
+    ms = ctypes.c_double(0.0)
+    if fn(pj.ctx, 2, 100, ctypes.byref(ms)) == 0:
+        b = up_sweep_bytes_per_cell(w) * n
+        out["up"] = {"name": f"k_up5 (fine multigrid up-sweep + PCG dots, {'FP32' if w == 4 else 'FP64'})",
+                     "avg_ms": ms.value, "bytes": b, "GBs": b / (ms.value * 1e-3) / 1e9}
+    if fn(pj.ctx, 0, 100, ctypes.byref(ms)) == 0:
+        b = 5 * 8 * n
+        out["apply"] = {"name": "k_apply5 (fine operator apply, FP64)", "avg_ms": ms.value, "bytes": b,
+                        "GBs": b / (ms.value * 1e-3) / 1e9}
+    return out
+
+
+def run_path(case, fields, precision, steps, warmup, tol, learned=None):
+    from simcoflow_amd import learned as Lr
     from simcoflow_amd.projection import Projection, params_for
     u, v, DpU, DpV = fields
-    prm = params_for(case.fluids, tol=tol, precision=precision, max_iter=200)
+    prm = params_for(case.fluids, tol=tol, precision=precision, max_iter=200, warm_start=2 if learned else 0)
     pj = Projection(case.grid, prm)
     pj.set_geometry(case.geo)
     pj.set_interface(case.itf.vof, case.itf.phi)
     pj.set_predictor(u, v, DpU, DpV)
     pj.upload("U", u)
     pj.upload("V", v)
+    if learned:
+        w, K = learned
+        # deflation needs FP64 directions; the mixed solve takes the warm start
+        Lr.enable(pj, w, Lr.pick_level(pj), K, deflate=(precision == 0))
     infos = []
 
     def step():
@@ -66,15 +128,16 @@ def run_path(case, fields, precision, steps, warmup, tol):
     t1 = time.perf_counter()
     Pp = pj.download("PP")
     mres = pj.download("MRES")
-    kern = kernel_roofline(pj, case, precision)
+    kern = kernel_roofline(pj, case.grid.ncells, precision) if not learned else {}
     pj.close()
-    ms = (t1 - t0) * 1e3 / steps
     return {
-        "ms_per_step": ms,
+        "ms_per_step": (t1 - t0) * 1e3 / steps,
         "poisson_ms": float(np.mean([i.ms for i in infos])),
         "setup_ms": float(np.mean([i.setup_ms for i in infos])),
-        "iterations": int(np.mean([i.iterations for i in infos])),
+        "warm_ms": float(np.mean([i.warm_ms for i in infos])),
+        "iterations": float(np.mean([i.iterations for i in infos])),
         "relres": float(max(i.relres for i in infos)),
+        "warm_relres": float(np.mean([i.warm_relres for i in infos])),
         "max_row_residual": float(max(i.max_row_residual for i in infos)),
         "max_mres": float(np.abs(mres).max()),
         "Pp": Pp,
@@ -82,44 +145,65 @@ def run_path(case, fields, precision, steps, warmup, tol):
     }
 
 
-# Algorithmic bytes per fine cell of the fine-level up-sweep k_up5 (the
-# kernel with the largest share of a solve): reads b = PCG residual (FP64),
-# x after pre-smoothing, diag/cE/cN, the 9 prolongation weights and the
-# coarse correction (both at 1/4 resolution) and q (FP64, for the fused
-# dot); writes z (FP64).  w = word size of the multigrid data.
-def up_sweep_bytes_per_cell(w):
-    return 8 + w + 3 * w + 9 * w / 4 + w / 4 + 8 + 8
+def train_warm_start(seed):
+    """The learned warm start, trained in this run on the GPU: pairs from the
+    plain FP64 solver on four seeded 512x512 swarms, 48 right-hand sides
+    each, then Adam with the library's kernels."""
+    from simcoflow_amd import cases as C
+    from simcoflow_amd import learned as Lr
+    t0 = time.perf_counter()
+    X, T, M, level = Lr.collect_pairs(lambda s: C.case_swarm(512, 512, seed=s), seeds=[seed + 101 + k for k in range(4)],
+                                      per_case=48)
+    t1 = time.perf_counter()
+    w, K, hist = Lr.train(X, T, M, epochs=10, batch=16, lr=3e-3)
+    t2 = time.perf_counter()
+    return w, K, {"pairs": int(X.shape[0]), "level_grid": [int(X.shape[1] - 2), int(X.shape[2] - 2)],
+                  "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+                  "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]}
 
 
-def kernel_roofline(pj, case, precision):
-    """Average duration of one kernel, timed with HIP events on the
-    library's stream by sg_time_kernel, against its algorithmic bytes."""
-    import ctypes
+def full_step(case, precision, steps=3, warmup=2):
+    """The whole time step on the swarm (TimeStepper), with the device time
+    of each part from host timers around its synchronous call."""
     from simcoflow_amd import _lib
-    L = _lib.load()
-    fn = L.sg_time_kernel
-    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
-    fn.restype = ctypes.c_int
-    out = {}
-    n = case.grid.ncells
-    w = 4 if precision == 1 else 8
-    for which, name, bpc in [(2, f"k_up5 (fine multigrid up-sweep + PCG dots, {'FP32' if w == 4 else 'FP64'})",
-                              up_sweep_bytes_per_cell(w)),
-                             (0, "k_apply5 (fine operator apply, FP64)", 5 * 8)]:
-        ms = ctypes.c_double(0.0)
-        if fn(pj.ctx, which, 100, ctypes.byref(ms)) != 0:
-            continue
-        b = bpc * n
-        out[which] = {"name": name, "avg_ms": ms.value, "bytes": b, "GBs": b / (ms.value * 1e-3) / 1e9}
-    # a plain copy of one padded fine-level FP64 vector: the calibration
-    # kernel for the FETCH_SIZE / WRITE_SIZE counters
-    ld = ((case.grid.ni + 1 + 31) // 32) * 32
-    ncopy = (case.grid.nj + 3) * ld
-    ms = ctypes.c_double(0.0)
-    if fn(pj.ctx, 3, 20, ctypes.byref(ms)) == 0:
-        out[3] = {"name": "k_copy", "avg_ms": ms.value, "bytes": 16 * ncopy,
-                  "GBs": 16 * ncopy / (ms.value * 1e-3) / 1e9}
-    return out
+    from simcoflow_amd.solver import TimeStepper
+    st = TimeStepper(case, precision=precision, tol=1e-8)
+    pj, lib, ctx = st.pj, st.pj.lib, st.pj.ctx
+    parts = {}
+    try:
+        for k in range(warmup + steps):
+            if k < warmup:
+                st.step()
+                continue
+            # the same sequence as TimeStepper.step, each part timed
+            st.itt += 1
+            t = time.perf_counter
+            a = t()
+            for dst, src in (("VOFO", "VOF"), ("PHIO", "PHI"), ("NXO", "NX"), ("NYO", "NY"), ("UVOFO", "UVOF"),
+                             ("VVOFO", "VVOF")):
+                _lib.check(lib.sg_field_copy(ctx, _lib.F[dst], _lib.F[src]))
+            dt = ctypes.c_double()
+            _lib.check(lib.sg_time_step_size(ctx, st.cfl, st.vint, st.uref, 0.0, 0.0, ctypes.byref(dt)))
+            b = t()
+            pj.clsvof_advect_resident(dt.value, st.time)
+            c = t()
+            _lib.check(lib.sg_interface_to_uv_resident(ctx))
+            _lib.check(lib.sg_closed_box_bc(ctx))
+            d = t()
+            pj.predictor_resident(dt.value, 0.0, False)
+            e = t()
+            pj.build(0.0)
+            info = pj.solve()
+            pj.velocity_correct(0.0)
+            pj.synchronize()
+            f = t()
+            st.time += dt.value
+            for name, ms in (("copies_and_dt", b - a), ("transport", c - b), ("interface_to_uv_and_bc", d - c),
+                             ("predictor", e - d), ("projection", f - e), ("step", f - a)):
+                parts.setdefault(name, []).append(ms * 1e3)
+    finally:
+        st.close()
+    return {k: round(float(np.mean(v)), 4) for k, v in parts.items()}
 
 
 def measured_traffic(kernel_name):
@@ -135,7 +219,7 @@ def measured_traffic(kernel_name):
         return None
 
 
-def cpu_baseline(case, fields, threads):
+def cpu_baseline(threads):
     """The oracle's FP64 PCG + multigrid on the host cores, on a bounded
     sample: solves of the same workload at 512x512 (a 1/16 sub-sample of the
     grid), scaled to Mcell-steps/s."""
@@ -160,9 +244,9 @@ def cpu_baseline(case, fields, threads):
         if time.perf_counter() - t0 > 10.0 or n >= 50:
             break
     dt = (time.perf_counter() - t0) / n
-    return {"value": g.ncells / dt / 1e6, "unit": "Mcell-steps/s", "cores": nthreads, "kind": "port",
-            "sample": f"{n} projection solves (build+PCG-MG to 1e-8, {int(np.mean(its))} iters) of the swarm "
-                      f"at 512x512, {dt * 1e3:.1f} ms each"}
+    return {"value": round(g.ncells / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"{n} projection solves (build + PCG-MG to 1e-8, {int(np.mean(its))} iterations) of the swarm "
+                      f"at 512x512, {dt * 1e3:.1f} ms each, on {nthreads} host threads"}
 
 
 def main():
@@ -171,15 +255,32 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the projection is "
               "not sharded across GPUs in this build.", file=sys.stderr)
         sys.exit(2)
+    if not a.no_learned:
+        # torch (the training data's device buffers) loads its HIP runtime
+        # first; the library then binds to the same one
+        import torch  # noqa: F401
     from simcoflow_amd import cases as C
+    log("building the swarm case")
     case = C.case_swarm(a.ni, a.nj, seed=a.seed)
     fields = C.predictor_fields(case)
     tol = 1e-8
+    n = case.grid.ncells
+    log("FP64 path")
     fp64 = run_path(case, fields, 0, a.steps, a.warmup, tol)
+    log("mixed path")
     mixed = run_path(case, fields, 1, a.steps, a.warmup, tol)
-    err = float(np.linalg.norm(mixed["Pp"] - fp64["Pp"]) / np.linalg.norm(fp64["Pp"]))
-    ncell = case.grid.ncells
-    value = ncell / (mixed["ms_per_step"] * 1e-3) / 1e6
+    learned = None
+    train = None
+    if not a.no_learned:
+        log("training the warm-start net")
+        w, K, train = train_warm_start(a.seed)
+        log("learned + mixed path")
+        learned = run_path(case, fields, 1, a.steps, a.warmup, tol, learned=(w, K))
+    head = learned if learned else mixed
+    err_mixed = float(np.linalg.norm(mixed["Pp"] - fp64["Pp"]) / np.linalg.norm(fp64["Pp"]))
+    value = n / (head["ms_per_step"] * 1e-3) / 1e6
+    fp64_bytes = solve_bytes(n, fp64["iterations"], 8)
+    mixed_bytes = solve_bytes(n, mixed["iterations"], 4)
     out = {
         "metric": "Poisson ms/step & Mcell-steps/s, learned+FP32 vs plain FP64 PCG at 1e-8 residual",
         "value": round(value, 3),
@@ -187,36 +288,56 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(mixed["ms_per_step"], 4),
+        "ms_per_step": round(head["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 outer / f32 multigrid",
+        "dtype": "f32 PCG vectors and multigrid, f64 x / dots / residual recomputation",
         "data": "synthetic (seeded bubble swarm, seeded predictor fields)",
         "config": {"workload": f"bubble swarm {a.ni}x{a.nj}, 64 bubbles seed {a.seed}, projection step "
-                               f"(build + PCG-MG to 1e-8 + correction)", "ni": a.ni, "nj": a.nj},
-        "poisson_ms_per_step": round(mixed["poisson_ms"], 4),
-        "fp64_poisson_ms_per_step": round(fp64["poisson_ms"], 4),
-        "fp64_ms_per_step": round(fp64["ms_per_step"], 4),
-        "fp64_value": round(ncell / (fp64["ms_per_step"] * 1e-3) / 1e6, 3),
-        "speedup_mixed_vs_fp64": round(fp64["poisson_ms"] / mixed["poisson_ms"], 3),
+                               f"(build + PCG-MG to 1e-8 + correction)", "ni": a.ni, "nj": a.nj,
+                   "path": "learned warm start, mixed precision" if learned else "mixed precision"},
+        "poisson_ms_per_step": {"fp64": round(fp64["poisson_ms"], 4), "mixed": round(mixed["poisson_ms"], 4)},
+        "ms_per_step_by_path": {"fp64": round(fp64["ms_per_step"], 4), "mixed": round(mixed["ms_per_step"], 4)},
+        "mcell_steps_per_s": {"fp64": round(n / (fp64["ms_per_step"] * 1e-3) / 1e6, 3),
+                              "mixed": round(n / (mixed["ms_per_step"] * 1e-3) / 1e6, 3)},
         "iterations": {"fp64": fp64["iterations"], "mixed": mixed["iterations"]},
         "relres": {"fp64": fp64["relres"], "mixed": mixed["relres"]},
-        "pressure_rel_l2_mixed_vs_fp64": err,
+        "setup_ms": {"fp64": round(fp64["setup_ms"], 4), "mixed": round(mixed["setup_ms"], 4)},
+        "pressure_rel_l2_vs_fp64": {"mixed": err_mixed},
         "max_mass_residual": {"fp64": fp64["max_mres"], "mixed": mixed["max_mres"]},
+        "solve_hbm": {
+            "fp64": {"bytes_alg": int(fp64_bytes), "GBs": round(fp64_bytes / (fp64["poisson_ms"] * 1e-3) / 1e9, 1),
+                     "frac": round(fp64_bytes / (fp64["poisson_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
+            "mixed": {"bytes_alg": int(mixed_bytes),
+                      "GBs": round(mixed_bytes / (mixed["poisson_ms"] * 1e-3) / 1e9, 1),
+                      "frac": round(mixed_bytes / (mixed["poisson_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}},
+        "speedup_vs_fp64": {"mixed": round(fp64["poisson_ms"] / mixed["poisson_ms"], 3)},
     }
-    k = mixed["kernel"].get(2)
+    if learned:
+        out["poisson_ms_per_step"]["learned_mixed"] = round(learned["poisson_ms"], 4)
+        out["ms_per_step_by_path"]["learned_mixed"] = round(learned["ms_per_step"], 4)
+        out["mcell_steps_per_s"]["learned_mixed"] = round(value, 3)
+        out["iterations"]["learned_mixed"] = learned["iterations"]
+        out["relres"]["learned_mixed"] = learned["relres"]
+        out["pressure_rel_l2_vs_fp64"]["learned_mixed"] = float(
+            np.linalg.norm(learned["Pp"] - fp64["Pp"]) / np.linalg.norm(fp64["Pp"]))
+        out["speedup_vs_fp64"]["learned_mixed"] = round(fp64["poisson_ms"] / learned["poisson_ms"], 3)
+        out["learned"] = dict(train, warm_start_ms=round(learned["warm_ms"], 4),
+                              start_relres=learned["warm_relres"])
+    k = mixed["kernel"].get("up")
     if k:
         out["roofline"] = {"bound": "hbm", "achieved": round(k["GBs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                            "frac": round(k["GBs"] / PEAK_HBM_GBS, 4), "traffic": measured_traffic(k["name"]),
-                           "kernel": k["name"], "avg_ms": round(k["avg_ms"], 5),
-                           "bytes_alg": int(k["bytes"])}
-        out["kernels"] = {
-            "fp64_up_sweep": fp64["kernel"].get(2),
-            "fp64_apply": fp64["kernel"].get(0),
-        }
+                           "kernel": k["name"], "avg_ms": round(k["avg_ms"], 5), "bytes_alg": int(k["bytes"])}
+        out["kernels"] = {"fp64_up_sweep": fp64["kernel"].get("up"), "fp64_apply": fp64["kernel"].get("apply")}
+    if not a.no_full_step:
+        log("full time step")
+        out["full_step_ms"] = {"mixed": full_step(case, 1)}
+        out["full_step_ms"]["mcell_steps_per_s"] = round(n / (out["full_step_ms"]["mixed"]["step"] * 1e-3) / 1e6, 3)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(case, fields, a.cpu_threads)
+        log("CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(a.cpu_threads)
     print(json.dumps(out))
 
 

@@ -169,6 +169,11 @@ int sg_ctx_create(int ni, int nj, const double* dx, const double* dy, const doub
         *v = (double*)dev_alloc(c, n0 * sizeof(double));
         if (!*v) return fail_alloc();
     }
+    float** vecf[] = {&c->rf, &c->zf, &c->pf0, &c->pf1, &c->qf};
+    for (float** v : vecf) {
+        *v = (float*)dev_alloc(c, n0 * sizeof(float));
+        if (!*v) return fail_alloc();
+    }
     if (clsvof_allocate(c) != SG_OK) return fail_alloc();
     if (predictor_allocate(c) != SG_OK) return fail_alloc();
     {

@@ -391,10 +391,10 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
 // the black pass uses the black cell's, the red pass the red cell's, so no
 // coefficient is read from memory twice.  The one-cell black ring around the
 // tile is smoothed from global reads.  b stays in LDS in FP64 for the dots.
-template <typename T, int TW, int TH>
-__global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __restrict__ b,
+template <typename T, typename V, int TW, int TH>
+__global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const V* __restrict__ b,
                                              const T* __restrict__ xin, Dims C, Prol<T> P, const T* __restrict__ e,
-                                             double* __restrict__ z, const double* __restrict__ q, Slots Sl) {
+                                             V* __restrict__ z, const V* __restrict__ q, Slots Sl) {
     constexpr int RW = TW + 4, RH = TH + 4;  // local (li,lj) <-> cell (i0+li-2, j0+lj-2)
     constexpr int PPR = TW / 2;              // pairs per row
     constexpr int PPT = PPR * TH / NTB;      // pairs per thread
@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __r
                         const int gi = 2 * I + di, gj = 2 * J + dj;
                         if (gi < F.ni && gj < F.nj) {
                             const size_t p = F.off(gi, gj);
-                            bv[dj][di] = b[p];
+                            bv[dj][di] = (double)b[p];
                             xv[dj][di] = xin[p] + pe[dj][di];
                         }
                     }
@@ -524,10 +524,11 @@ __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const double* __r
             const T v = h ? gs(cf[u][1], pos) : sxf[pos];
             const int li = pos % RW, lj = pos / RW;
             const size_t p = F.off(i0 + li - 2, j0 + lj - 2);
-            const double zv = (double)v;
-            z[p] = zv;
+            const V zs = (V)v;
+            const double zv = (double)zs;
+            z[p] = zs;
             v0 += sbf[pos] * zv;
-            v1 += zv * q[p];
+            v1 += zv * (double)q[p];
         }
     }
     double vv[2] = {v0, v1};
@@ -772,15 +773,16 @@ inline Dims compact(const Dims& d) {
     return c;
 }
 
-// z = M^{-1} r ; fine-level dots (r,z), (z,q) go to the partials slot S
-template <typename T>
-int vcycle(Ctx* c, const double* r, double* z, const double* q, Slots S, int* dots_done) {
+// z = M^{-1} r ; fine-level dots (r,z), (z,q) go to the partials slot S;
+// V = type of the PCG vectors r, z, q
+template <typename T, typename V>
+int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done) {
     hipStream_t s = c->stream;
     const int nl = c->nlev;
     *dots_done = 0;
     if (nl == 1) {
         const Level& L = c->lev[0];
-        hipLaunchKernelGGL((k_direct<T, double, double>), dim3(1), dim3(256), 0, s, L.d, Acc<T>::inv(c), r, z);
+        hipLaunchKernelGGL((k_direct<T, V, V>), dim3(1), dim3(256), 0, s, L.d, Acc<T>::inv(c), r, z);
         SG_HIP(hipGetLastError());
         return SG_OK;
     }
@@ -792,7 +794,7 @@ int vcycle(Ctx* c, const double* r, double* z, const double* q, Slots S, int* do
         const Level& L = c->lev[l];
         const Level& Cl = c->lev[l + 1];
         if (l == 0)
-            hipLaunchKernelGGL((k_down5<T, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
+            hipLaunchKernelGGL((k_down5<T, V, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
                                Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), Acc<T>::vec(Cl).b);
         else if (big(L.d))
             hipLaunchKernelGGL((k_down9<T, 64, 16>), tiles(L.d, 64, 16), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
@@ -841,7 +843,7 @@ int vcycle(Ctx* c, const double* r, double* z, const double* q, Slots S, int* do
         const Level& Cl = c->lev[l + 1];
         const T* e = (l + 1 == lt) ? Acc<T>::vec(Cl).x : Acc<T>::vec(Cl).r;
         if (l == 0)
-            hipLaunchKernelGGL((k_up5<T, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
+            hipLaunchKernelGGL((k_up5<T, V, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
                                (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e, z, q, S);
         else if (big(L.d))
             hipLaunchKernelGGL((k_up9<T, 64, 16>), tiles(L.d, 64, 16), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
@@ -858,11 +860,11 @@ int vcycle(Ctx* c, const double* r, double* z, const double* q, Slots S, int* do
 
 }  // namespace
 
-int mg_vcycle_dots(Ctx* c, const double* r, double* z, const double* q, double* partials, int max_blocks,
-                   int* dots_done) {
+int mg_vcycle_dots(Ctx* c, double* partials, int max_blocks, int* dots_done) {
     Slots S{partials, max_blocks};
-    if (c->prm.precision == SG_PREC_MIXED) return vcycle<float>(c, r, z, q, S, dots_done);
-    return vcycle<double>(c, r, z, q, S, dots_done);
+    if (c->prm.precision == SG_PREC_MIXED)
+        return vcycle<float, float>(c, c->rf, c->zf, c->qf, S, dots_done);
+    return vcycle<double, double>(c, c->r, c->z, c->q, S, dots_done);
 }
 
 namespace {
@@ -883,7 +885,7 @@ int mg_launch_fine(Ctx* c, int which) {
     const bool mixed = c->prm.precision == SG_PREC_MIXED;
     if (which == 1) {
         if (mixed)
-            hipLaunchKernelGGL((k_down5<float, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->r,
+            hipLaunchKernelGGL((k_down5<float, float, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->rf,
                                L.vf.x, Cl.d, L.Pf, Cl.vf.b);
         else
             hipLaunchKernelGGL((k_down5<double, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
@@ -892,11 +894,11 @@ int mg_launch_fine(Ctx* c, int which) {
         // the coarse correction sits in x when level 1 is the tail's first level, else in r
         const bool tail1 = c->nlev == 2 || (size_t)Cl.d.ni * Cl.d.nj <= (size_t)TAIL_MAX;
         if (mixed)
-            hipLaunchKernelGGL((k_up5<float, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->r,
-                               (const float*)L.vf.x, Cl.d, L.Pf, (const float*)(tail1 ? Cl.vf.x : Cl.vf.r), c->z,
-                               c->q, S);
+            hipLaunchKernelGGL((k_up5<float, float, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af),
+                               c->rf, (const float*)L.vf.x, Cl.d, L.Pf, (const float*)(tail1 ? Cl.vf.x : Cl.vf.r),
+                               c->zf, c->qf, S);
         else
-            hipLaunchKernelGGL((k_up5<double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
+            hipLaunchKernelGGL((k_up5<double, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
                                (const double*)L.v.x, Cl.d, L.P, (const double*)(tail1 ? Cl.v.x : Cl.v.r), c->z,
                                c->q, S);
     } else if (which == 3) {

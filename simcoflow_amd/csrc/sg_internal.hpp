@@ -205,6 +205,8 @@ struct Ctx {
     int coarse_n = 0;
     // PCG vectors on the fine level (FP64)
     double *x = nullptr, *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
+    // FP32 PCG vectors of the mixed-precision solve (x and the RHS stay FP64)
+    float *rf = nullptr, *zf = nullptr, *pf0 = nullptr, *pf1 = nullptr, *qf = nullptr;
     double* rhs = nullptr;
     double* stage = nullptr;  // cell-layout staging, 5*ni*nj
     int pcur = 0;
@@ -269,11 +271,11 @@ int cell_to_padded(Ctx* c, const double* src_cell, double* dst);
 // multigrid (sg_mg.hip)
 int mg_allocate(Ctx* c);
 int mg_setup(Ctx* c);
-// z = M^{-1} r on the fine level (padded FP64 vectors); the per-block
+// z = M^{-1} r on the fine level with the PCG vectors of the context's
+// precision (r, z, q in FP64; rf, zf, qf in mixed precision); the per-block
 // partials of (r,z) and (z,q) land in `partials` (2 planes of max_blocks),
 // *dots_done says whether they did (a one-level hierarchy has no fused dots)
-int mg_vcycle_dots(Ctx* c, const double* r, double* z, const double* q, double* partials, int max_blocks,
-                   int* dots_done);
+int mg_vcycle_dots(Ctx* c, double* partials, int max_blocks, int* dots_done);
 int mg_partial_blocks(Ctx* c);  // blocks that wrote partials in mg_vcycle_dots
 int mg_launch_fine(Ctx* c, int which);  // measurement: 1 down, 2 up, 3 copy
 // PCG (sg_pcg.hip)

@@ -1,9 +1,17 @@
 // Preconditioned conjugate gradients for the projection (stands where
-// HYPRE_ParCSRPCGSolve stands, ProjectionP_Mod.f90:57-58), FP64 vectors and
-// dot products, multigrid V-cycle preconditioner.  All scalars (alpha, beta,
-// norms) stay on the device; the host reads them only to test convergence.
-// In mixed precision the cycle is not exactly symmetric, so beta takes the
-// Polak-Ribiere (flexible) form -alpha*(z_new,q_old)/(r_old,z_old).
+// HYPRE_ParCSRPCGSolve stands, ProjectionP_Mod.f90:57-58) with a multigrid
+// V-cycle preconditioner.  All scalars (alpha, beta, norms) stay on the
+// device; the host reads them only to test convergence.
+//
+// FP64: every vector and the operator in FP64.  Mixed precision: the PCG
+// vectors r, z, p, q and the whole cycle are FP32, q = A p applies the FP64
+// operator; x and b stay FP64, every dot product is accumulated in FP64, and the
+// residual is recomputed from x in FP64 (r = b - A x, "reliable update")
+// whenever the recursive one has fallen by 100x since the last
+// recomputation and again at convergence, so the FP32 recursion never
+// limits the attained FP64 residual.  The FP32 cycle is not exactly
+// symmetric, so beta takes the Polak-Ribiere (flexible) form
+// -alpha*(z_new,q_old)/(r_old,z_old).
 #include "sg_internal.hpp"
 #include "sg_reduce.hpp"
 
@@ -14,7 +22,7 @@ namespace {
 constexpr int NT = TX * TY;
 constexpr int NTF = 1024;  // threads of the finalize kernels
 
-enum FinMode { FIN_INIT = 0, FIN_REPLACE, FIN_RZ, FIN_PQ, FIN_RR, FIN_TRUE, FIN_MU };
+enum FinMode { FIN_INIT = 0, FIN_REPLACE, FIN_RZ, FIN_PQ, FIN_RR, FIN_TRUE, FIN_MU, FIN_RELIABLE };
 
 // one reduction slot: NRED planes of per-block partials
 struct Slots {
@@ -22,10 +30,12 @@ struct Slots {
     int max_blocks;
 };
 
-// r = b - A x (or r = b for a zero start); partials of ||b||^2, ||r||^2
+// r = b - A x in FP64, stored as V (or r = b for a zero start); partials of
+// ||b||^2, ||r||^2
+template <typename V>
 __global__ void __launch_bounds__(256) k_init(Dims D, const double* __restrict__ diag, const double* __restrict__ cE,
                                               const double* __restrict__ cN, const double* __restrict__ b,
-                                              double* __restrict__ x, double* __restrict__ r, int warm, Slots S) {
+                                              double* __restrict__ x, V* __restrict__ r, int warm, Slots S) {
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     double v[2] = {0.0, 0.0};
@@ -42,7 +52,7 @@ __global__ void __launch_bounds__(256) k_init(Dims D, const double* __restrict__
         } else {
             x[p] = 0.0;
         }
-        r[p] = rp;
+        r[p] = (V)rp;
         v[0] = bp * bp;
         v[1] = rp * rp;
     }
@@ -50,23 +60,25 @@ __global__ void __launch_bounds__(256) k_init(Dims D, const double* __restrict__
 }
 
 // partials of rz = (r,z), zq = (z,q_old)
-__global__ void __launch_bounds__(256) k_dots_rz(Dims D, const double* __restrict__ r, const double* __restrict__ z,
-                                                 const double* __restrict__ q, Slots S, const PcgScalars* sc) {
+template <typename V>
+__global__ void __launch_bounds__(256) k_dots_rz(Dims D, const V* __restrict__ r, const V* __restrict__ z,
+                                                 const V* __restrict__ q, Slots S, const PcgScalars* sc) {
     if (!sc->active) return;
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     double v[2] = {0.0, 0.0};
     if (i < D.ni && j < D.nj) {
         const size_t p = D.off(i, j);
-        const double zp = z[p];
-        v[0] = r[p] * zp;
-        v[1] = zp * q[p];
+        const double zp = (double)z[p];
+        v[0] = (double)r[p] * zp;
+        v[1] = zp * (double)q[p];
     }
     block_partials<2, NT>(v, S.part, S.max_blocks);
 }
 
 // partials of (A w, z): the deflation coefficient mu = (A w, z) / (w, A w)
-__global__ void __launch_bounds__(256) k_dot_mu(Dims D, const double* __restrict__ aw, const double* __restrict__ z,
+template <typename V>
+__global__ void __launch_bounds__(256) k_dot_mu(Dims D, const double* __restrict__ aw, const V* __restrict__ z,
                                                 Slots S, const PcgScalars* sc) {
     if (!sc->active) return;
     const int i = blockIdx.x * TX + threadIdx.x;
@@ -74,7 +86,7 @@ __global__ void __launch_bounds__(256) k_dot_mu(Dims D, const double* __restrict
     double v[1] = {0.0};
     if (i < D.ni && j < D.nj) {
         const size_t p = D.off(i, j);
-        v[0] = aw[p] * z[p];
+        v[0] = aw[p] * (double)z[p];
     }
     block_partials<1, NT>(v, S.part, S.max_blocks);
 }
@@ -82,17 +94,23 @@ __global__ void __launch_bounds__(256) k_dot_mu(Dims D, const double* __restrict
 // p_new = z + beta p_old (- mu w with deflation) ; q = A p_new ;
 // partials of (p_new, q).  A 64x16 tile: p_new of the tile and its one-cell
 // halo is formed once in LDS, then the stencil reads it from there.
+// V = type of the vectors (FP64, or FP32 in mixed precision); the operator
+// is the FP64 one in both (an FP32 copy would make the recursion solve a
+// perturbed system, which the FP64 residual recomputation then has to undo
+// at a rate set by the condition number), the stencil sums and (p,q) in
+// FP64.
 constexpr int DW = 64, DH = 16;
+template <typename V>
 __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restrict__ diag,
                                                    const double* __restrict__ cE, const double* __restrict__ cN,
-                                                   const double* __restrict__ z, const double* __restrict__ pold,
-                                                   const double* __restrict__ w, double* __restrict__ pnew,
-                                                   double* __restrict__ q, Slots S, const PcgScalars* sc) {
+                                                   const V* __restrict__ z, const V* __restrict__ pold,
+                                                   const double* __restrict__ w, V* __restrict__ pnew,
+                                                   V* __restrict__ q, Slots S, const PcgScalars* sc) {
     if (!sc->active) return;
     constexpr int RW = DW + 2, RH = DH + 2;
-    __shared__ double sp[RH][RW];
-    const double beta = sc->beta;
-    const double mu = w ? sc->mu : 0.0;
+    __shared__ V sp[RH][RW];
+    const V beta = (V)sc->beta;
+    const V mu = w ? (V)sc->mu : V(0);
     int tx, ty;
     xcd_tile(tx, ty);
     const int i0 = tx * DW, j0 = ty * DH;
@@ -100,10 +118,10 @@ __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restr
     for (int k = tid; k < RW * RH; k += 256) {
         const int li = k % RW, lj = k / RW;
         const int gi = i0 + li - 1, gj = j0 + lj - 1;
-        double v = 0.0;
+        V v = V(0);
         if (gi >= 0 && gj >= 0 && gi < D.ni && gj < D.nj) {
             const size_t p = D.off(gi, gj);
-            v = w ? z[p] + beta * pold[p] - mu * w[p] : z[p] + beta * pold[p];
+            v = w ? z[p] + beta * pold[p] - mu * (V)w[p] : z[p] + beta * pold[p];
         }
         sp[lj][li] = v;
     }
@@ -117,31 +135,35 @@ __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restr
         const int gi = i0 + li - 1, gj = j0 + lj - 1;
         if (gi >= D.ni || gj >= D.nj) continue;
         const size_t p = D.off(gi, gj);
-        const double pc = sp[lj][li];
-        const double qv = diag[p] * pc - cE[p - 1] * sp[lj][li - 1] - cE[p] * sp[lj][li + 1] -
-                          cN[p - ld] * sp[lj - 1][li] - cN[p] * sp[lj + 1][li];
+        const V pc = sp[lj][li];
+        const double qd = diag[p] * (double)pc - cE[p - 1] * (double)sp[lj][li - 1] -
+                          cE[p] * (double)sp[lj][li + 1] - cN[p - ld] * (double)sp[lj - 1][li] -
+                          cN[p] * (double)sp[lj + 1][li];
+        const V qv = (V)qd;
         pnew[p] = pc;
         q[p] = qv;
-        v[0] += pc * qv;
+        v[0] += (double)pc * (double)qv;
     }
     block_partials<1, 256>(v, S.part, S.max_blocks);
 }
 
-// x += alpha p ; r -= alpha q ; partials of (r,r)
-__global__ void __launch_bounds__(256) k_update(Dims D, const double* __restrict__ p, const double* __restrict__ q,
-                                                double* __restrict__ x, double* __restrict__ r, Slots S,
+// x += alpha p (x in FP64) ; r -= alpha q ; partials of (r,r)
+template <typename V>
+__global__ void __launch_bounds__(256) k_update(Dims D, const V* __restrict__ p, const V* __restrict__ q,
+                                                double* __restrict__ x, V* __restrict__ r, Slots S,
                                                 const PcgScalars* sc) {
     if (!sc->active) return;
     const double alpha = sc->alpha;
+    const V alv = (V)alpha;
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     double v[1] = {0.0};
     if (i < D.ni && j < D.nj) {
         const size_t k = D.off(i, j);
-        x[k] += alpha * p[k];
-        const double rn = r[k] - alpha * q[k];
+        x[k] += alpha * (double)p[k];
+        const V rn = r[k] - alv * q[k];
         r[k] = rn;
-        v[0] = rn * rn;
+        v[0] = (double)rn * (double)rn;
     }
     block_partials<1, NT>(v, S.part, S.max_blocks);
 }
@@ -179,7 +201,7 @@ __global__ void __launch_bounds__(256) k_true_resid(Dims D, const double* __rest
 
 // One block: reduce the partials in block order and update the scalars.
 __global__ void __launch_bounds__(NTF) k_finalize(Slots S, int nblocks, int mode, PcgScalars* sc, double tol) {
-    if (mode != FIN_INIT && mode != FIN_REPLACE && mode != FIN_TRUE && !sc->active) return;
+    if (mode != FIN_INIT && mode != FIN_REPLACE && mode != FIN_TRUE && mode != FIN_RELIABLE && !sc->active) return;
     if (mode == FIN_TRUE) {
         __shared__ double mx[NTF / 64];
         double s[1];
@@ -194,6 +216,18 @@ __global__ void __launch_bounds__(NTF) k_finalize(Slots S, int nblocks, int mode
             for (int w = 0; w < NTF / 64; ++w) c = fmax(c, mx[w]);
             sc->true_rr = s[0];
             sc->true_max = c;
+        }
+        return;
+    }
+    if (mode == FIN_RELIABLE) {
+        double s2[2];
+        finalize_sum<2, NTF>(S.part, S.max_blocks, nblocks, s2);
+        if (threadIdx.x != 0) return;
+        sc->rr = s2[1];
+        sc->active = (s2[1] > sc->tol2) ? 1 : 0;
+        if (isnan(s2[1])) {
+            sc->active = 0;
+            sc->guard |= 1;
         }
         return;
     }
@@ -250,53 +284,68 @@ Slots slots(Ctx* c, int s) {
 bool learned(Ctx* c) { return c->prm.warm_start == 2 && c->nn.on; }
 bool deflating(Ctx* c) { return learned(c) && c->nn.deflate; }
 
-// one PCG iteration (V-cycle + direction + update), enqueued on the stream
-int enqueue_iteration(Ctx* c, int parity) {
-    Level& L0 = c->lev[0];
-    const Dims D = L0.d;
+// one PCG iteration (V-cycle + direction + update), enqueued on the stream;
+// V = the vector precision, A the operator of that precision
+template <typename V>
+int enqueue_iteration_t(Ctx* c, int parity, V* r, V* z, V* q, V* p0, V* p1) {
+    const Op<double>& A = c->lev[0].A;
+    const Dims D = c->lev[0].d;
     hipStream_t s = c->stream;
     const dim3 g = grid2d(c->ni, c->nj), b = block2d();
     const int nb = (int)(g.x * g.y);
-    double* pold = parity ? c->p1 : c->p0;
-    double* pnew = parity ? c->p0 : c->p1;
+    V* pold = parity ? p1 : p0;
+    V* pnew = parity ? p0 : p1;
     Slots S1 = slots(c, 1), S2 = slots(c, 2), S3 = slots(c, 3);
     // the cycle's last kernel also leaves the partials of (r,z) and (z,q)
     int dots_done = 0;
-    int rc = mg_vcycle_dots(c, c->r, c->z, c->q, S1.part, S1.max_blocks, &dots_done);
+    int rc = mg_vcycle_dots(c, S1.part, S1.max_blocks, &dots_done);
     if (rc) return rc;
     int nb_rz = mg_partial_blocks(c);
     if (!dots_done) {
-        hipLaunchKernelGGL(k_dots_rz, g, b, 0, s, D, (const double*)c->r, (const double*)c->z,
-                           (const double*)c->q, S1, (const PcgScalars*)c->sc);
+        hipLaunchKernelGGL(k_dots_rz<V>, g, b, 0, s, D, (const V*)r, (const V*)z, (const V*)q, S1,
+                           (const PcgScalars*)c->sc);
         nb_rz = nb;
     }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S1, nb_rz, (int)FIN_RZ, c->sc, 0.0);
     const double* w = nullptr;
     if (deflating(c)) {
         Slots S6 = slots(c, 6);
-        hipLaunchKernelGGL(k_dot_mu, g, b, 0, s, D, (const double*)c->nn.aw, (const double*)c->z, S6,
+        hipLaunchKernelGGL(k_dot_mu<V>, g, b, 0, s, D, (const double*)c->nn.aw, (const V*)z, S6,
                            (const PcgScalars*)c->sc);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S6, nb, (int)FIN_MU, c->sc, 0.0);
         w = c->nn.w;
     }
     const dim3 gd((c->ni + DW - 1) / DW, (c->nj + DH - 1) / DH);
-    hipLaunchKernelGGL(k_dir_apply, gd, dim3(256), 0, s, D, L0.A.diag, L0.A.cE, L0.A.cN, (const double*)c->z,
-                       (const double*)pold, w, pnew, c->q, S2, (const PcgScalars*)c->sc);
+    hipLaunchKernelGGL(k_dir_apply<V>, gd, dim3(256), 0, s, D, (const double*)A.diag, (const double*)A.cE,
+                       (const double*)A.cN, (const V*)z, (const V*)pold, w, pnew, q, S2, (const PcgScalars*)c->sc);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S2, (int)(gd.x * gd.y), (int)FIN_PQ, c->sc, 0.0);
-    hipLaunchKernelGGL(k_update, g, b, 0, s, D, (const double*)pnew, (const double*)c->q, c->x, c->r, S3,
+    hipLaunchKernelGGL(k_update<V>, g, b, 0, s, D, (const V*)pnew, (const V*)q, c->x, r, S3,
                        (const PcgScalars*)c->sc);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S3, nb, (int)FIN_RR, c->sc, 0.0);
     SG_HIP(hipGetLastError());
     return SG_OK;
 }
 
+bool mixed(Ctx* c) { return c->prm.precision == SG_PREC_MIXED; }
+
+int enqueue_iteration(Ctx* c, int parity) {
+    if (mixed(c)) return enqueue_iteration_t<float>(c, parity, c->rf, c->zf, c->qf, c->pf0, c->pf1);
+    return enqueue_iteration_t<double>(c, parity, c->r, c->z, c->q, c->p0, c->p1);
+}
+
+// r = b - A x (warm) or r = b, x = 0 (cold), in FP64, stored at the
+// context's vector precision; then the finalize of `mode`
 int enqueue_init(Ctx* c, int warm, int mode) {
     Level& L0 = c->lev[0];
     const dim3 g = grid2d(c->ni, c->nj);
     const int nb = (int)(g.x * g.y);
     Slots S0 = slots(c, 0);
-    hipLaunchKernelGGL(k_init, g, block2d(), 0, c->stream, L0.d, L0.A.diag, L0.A.cE, L0.A.cN,
-                       (const double*)c->rhs, c->x, c->r, warm, S0);
+    if (mixed(c))
+        hipLaunchKernelGGL(k_init<float>, g, block2d(), 0, c->stream, L0.d, L0.A.diag, L0.A.cE, L0.A.cN,
+                           (const double*)c->rhs, c->x, c->rf, warm, S0);
+    else
+        hipLaunchKernelGGL(k_init<double>, g, block2d(), 0, c->stream, L0.d, L0.A.diag, L0.A.cE, L0.A.cN,
+                           (const double*)c->rhs, c->x, c->r, warm, S0);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, c->stream, S0, nb, mode, c->sc, c->prm.tol);
     SG_HIP(hipGetLastError());
     return SG_OK;
@@ -373,6 +422,13 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
             set_error("warm_start = 2 needs the learned warm start (sg_set_learned)");
             return SG_ERR_ARG;
         }
+        if (deflating(c) && mixed(c)) {
+            // with FP32 directions the A-orthogonality to w that deflation
+            // relies on decays and the recursion diverges
+            set_error("the deflation vector needs FP64 PCG vectors (precision SG_PREC_FP64); use deflate = 0 "
+                      "in mixed precision");
+            return SG_ERR_ARG;
+        }
         rc = learned_warm_start(c);
         if (rc) return rc;
         rc = enqueue_init(c, 1, FIN_INIT);
@@ -399,6 +455,7 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
     }
     int it = 0;
     int replacements = 0;
+    double rr_ref = c->sc_host->rr;  // ||r||^2 at the last FP64 recomputation
     for (;;) {
         while (c->sc_host->active && it < max_iter) {
             if (graph) {
@@ -420,12 +477,22 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
             }
             SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
             SG_HIP(hipStreamSynchronize(s));
+            // mixed precision: recompute the FP32 residual from x in FP64
+            // once it has fallen 100x, keeping the search direction
+            if (mixed(c) && c->sc_host->active && c->sc_host->rr < 1e-4 * rr_ref) {
+                rc = enqueue_init(c, 1, FIN_RELIABLE);
+                if (rc) return rc;
+                SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+                SG_HIP(hipStreamSynchronize(s));
+                rr_ref = c->sc_host->rr;
+            }
         }
-        // FP64 residual replacement: the recursive residual drifts from
-        // b - A x by rounding; restart CG from the true residual until it
-        // meets tol
+        // residual replacement at convergence: the recursive residual drifts
+        // from b - A x by rounding; recompute it in FP64 from x and go on until
+        // the true residual meets tol (FP64: restart CG from it; mixed
+        // precision: keep the direction, as above)
         if (c->sc_host->active || (c->sc_host->guard & 1) || replacements >= 4 || it >= max_iter) break;
-        rc = enqueue_init(c, 1, FIN_REPLACE);
+        rc = enqueue_init(c, 1, mixed(c) ? FIN_RELIABLE : FIN_REPLACE);
         if (rc) return rc;
         SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
         SG_HIP(hipStreamSynchronize(s));
