@@ -92,6 +92,13 @@ def kernel_roofline(pj, n, precision):
         b = 5 * 8 * n
         out["apply"] = {"name": "k_apply5 (fine operator apply, FP64)", "avg_ms": ms.value, "bytes": b,
                         "GBs": b / (ms.value * 1e-3) / 1e9}
+    # a plain copy of one padded fine-level FP64 vector: the calibration
+    # kernel of the FETCH_SIZE / WRITE_SIZE counters
+    ld = ((pj.ni + 1 + 31) // 32) * 32
+    ncopy = (pj.nj + 3) * ld
+    if fn(pj.ctx, 3, 20, ctypes.byref(ms)) == 0:
+        out["copy"] = {"name": "k_copy (FP64 vector copy)", "avg_ms": ms.value, "bytes": 16 * ncopy,
+                       "GBs": 16 * ncopy / (ms.value * 1e-3) / 1e9}
     return out
 
 
@@ -330,7 +337,8 @@ def main():
         out["roofline"] = {"bound": "hbm", "achieved": round(k["GBs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                            "frac": round(k["GBs"] / PEAK_HBM_GBS, 4), "traffic": measured_traffic(k["name"]),
                            "kernel": k["name"], "avg_ms": round(k["avg_ms"], 5), "bytes_alg": int(k["bytes"])}
-        out["kernels"] = {"fp64_up_sweep": fp64["kernel"].get("up"), "fp64_apply": fp64["kernel"].get("apply")}
+        out["kernels"] = {"fp64_up_sweep": fp64["kernel"].get("up"), "fp64_apply": fp64["kernel"].get("apply"),
+                          "copy": fp64["kernel"].get("copy")}
     if not a.no_full_step:
         log("full time step")
         out["full_step_ms"] = {"mixed": full_step(case, 1)}

@@ -14,6 +14,17 @@
  *
  * Layout: cell arrays a[(i-1)+(j-1)*ni]; ghosted arrays a[i+j*(ni+2)];
  * ue (0:ni,1:nj) at i+(j-1)*(ni+1); ve (1:ni,0:nj) at (i-1)+j*ni.
+ *
+ * Pinned by independent checks (tests/test_oracle_clsvof.py): orc_frac,
+ * orc_cellgeocal (polygon clipping), orc_plic_distance and
+ * orc_distance_fluid (round trips), orc_reconstruct on straight lines,
+ * orc_redistance (straight line, circle), the sweeps (exact translation,
+ * uniform states, volume), orc_interface_to_uv (straight lines, solid
+ * bounds).  Parity unpinned (restated line by line, nothing independent fixes
+ * them): the quality-based stencil choice of orc_normal_irre, the stale
+ * normals on the domain edge, the tie order of orc_redistance, the one-sided
+ * level-set fluxes at the first and last faces, orc_inflow_level, and the
+ * stale-value walk-back of orc_interface_to_uv.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -19,6 +19,12 @@
  * Layout: cell arrays a[(i-1)+(j-1)*ni]; ghosted arrays a[i+j*(ni+2)].
  * Face arrays EW (i=1..ni+1, j=1..nj) and NS (i=1..ni, j=1..nj+1) are
  * stored in ghosted arrays at (i,j).
+ *
+ * Pinned by closed forms (tests/test_oracle_predictor.py): steady uniform
+ * stream, gravity increment and Dp = dt at rest, steady Couette shear, the
+ * Adams-Bashforth history.  Parity unpinned (restated line by line): the
+ * MUSCL / kinetic flux details (high_order_x/y, muscl), the mixing weights
+ * (mixing) and the cut-cell interpolation branches (modified_conv).
  */
 #include <math.h>
 #include <stdlib.h>
