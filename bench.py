@@ -1,25 +1,28 @@
-"""Benchmark of the projection step (and the full time step) on one MI355X.
+"""Benchmark of the hot path on one MI355X: full time steps and the
+projection (Poisson) solve.
 
 Workload: the 64-bubble swarm on a 2048x2048 grid (1x1 m box, liquid depth
 0.8 m, density ratio 1000:1), the configuration the solver metric is quoted
-on.  One step = one projection step from device-resident inputs: operator and
-RHS build from the predictor fields (ComputePossionMatrixCoefficient /
-SetPoissonMatrix / SetPoissonVectors), PCG + multigrid solve to a relative
-residual of 1e-8, pressure scatter + ghosts, velocity correction and mass
-residual (UpdatePUV's correction).  Every step redoes all of it.
+on.  One step = one full time step of the hot path, device resident, in the
+order of AdamBasforthCrankNicolson (Solver_Mod.f90:246-319): CFL time step,
+interface transport (Coupled_LS_VOF, 6 sub-steps) and the interface on the
+U/V cells, the box ghost layer, the momentum predictor (Predictor_UV, two
+implicit solves), the projection (operator and RHS build, PCG + multigrid to
+a relative residual of 1e-8, pressure scatter) and the velocity correction.
+The swarm starts at rest; W steps of warm-up, then K timed steps.
 
-Three solver paths are timed on the same inputs:
+Three solver paths run the same steps:
   * plain FP64 PCG-multigrid from x0 = 0 (the reference's start);
-  * mixed precision (FP32 multigrid cycle, FP64 outer PCG, dots and
-    coarsest solve) from x0 = 0;
+  * mixed precision (FP32 PCG vectors and multigrid cycle; FP64 x, dots and
+    residual recomputation) from x0 = 0;
   * learned + mixed: the mixed solver started from the learned warm start
-    (the deflation vector needs FP64 directions; the net is trained inside this run, with the
-    library's own kernels, on (rhs, p) pairs the plain solver makes for
-    seeded 512x512 swarms; it runs on the 256x256 level of the hierarchy).
-``value`` is the learned + mixed path's Mcell-steps/s (the headline metric);
-the others are reported beside it.  A full time step of the same swarm
-(time step, transport, interface on U/V cells, predictor, projection,
-correction) is timed after it.
+    (the net is trained inside this run, with the library's own kernels, on
+    (rhs, p) pairs the plain solver makes for seeded 512x512 swarms; it runs
+    on the 256x256 level of the hierarchy).
+``value`` is the learned + mixed path's Mcell-steps/s over full time steps
+(the headline metric); ``poisson_ms_per_step`` gives the Poisson solve time
+per step of every path, inside those steps and, separately, on a fixed
+projection step from seeded predictor fields (the solver alone, repeated).
 
 Prints one JSON line.
 """
@@ -48,7 +51,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip training and the learned path")
-    ap.add_argument("--no-full-step", action="store_true", help="skip the full time step")
+    ap.add_argument("--no-full-step", action="store_true", help="skip the per-part step timing")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
@@ -169,12 +172,44 @@ def train_warm_start(seed):
                   "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]}
 
 
-def full_step(case, precision, steps=3, warmup=2):
+def stepping(case, stags, precision, steps, warmup, learned=None):
+    """K timed full time steps after W warm-up steps (TimeStepper), with the
+    Poisson device time of each step from its solve report."""
+    from simcoflow_amd.solver import TimeStepper
+    lrn = None
+    if learned:
+        w, K = learned
+        # deflation needs FP64 directions; the mixed solve takes the warm start
+        lrn = (w, None, K, precision == 0)
+    st = TimeStepper(case, precision=precision, tol=1e-8, max_iter=200, learned=lrn, stags=stags)
+    try:
+        for _ in range(warmup):
+            st.step()
+        st.pj.synchronize()
+        reps = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reps.append(st.step())
+        st.pj.synchronize()
+        t1 = time.perf_counter()
+        vof = st.pj.download("VOF")
+    finally:
+        st.close()
+    p = [r["poisson"] for r in reps]
+    return {"ms_per_step": (t1 - t0) * 1e3 / steps,
+            "poisson_ms": float(np.mean([r["ms"] for r in p])),
+            "warm_ms": float(np.mean([r["warm_ms"] for r in p])),
+            "iterations": float(np.mean([r["iterations"] for r in p])),
+            "relres": float(max(r["relres"] for r in p)),
+            "dt_mean": float(np.mean([r["dt"] for r in reps])), "vof": vof}
+
+
+def step_parts(case, stags, precision, steps=3, warmup=2):
     """The whole time step on the swarm (TimeStepper), with the device time
     of each part from host timers around its synchronous call."""
     from simcoflow_amd import _lib
     from simcoflow_amd.solver import TimeStepper
-    st = TimeStepper(case, precision=precision, tol=1e-8)
+    st = TimeStepper(case, precision=precision, tol=1e-8, max_iter=200, stags=stags)
     pj, lib, ctx = st.pj, st.pj.lib, st.pj.ctx
     parts = {}
     try:
@@ -227,39 +262,41 @@ def measured_traffic(kernel_name):
 
 
 def cpu_baseline(threads):
-    """The oracle's FP64 PCG + multigrid on the host cores, on a bounded
-    sample: solves of the same workload at 512x512 (a 1/16 sub-sample of the
-    grid), scaled to Mcell-steps/s."""
+    """The oracle's full time step on the host cores, on a bounded sample:
+    time steps of the same swarm on a 256x256 grid (a 1/64 sub-sample),
+    scaled to Mcell-steps/s.  The oracle's PCG-multigrid uses every host
+    thread (OpenMP); its interface transport and predictor assembly are
+    single-threaded C, its momentum solves scipy's direct sparse solver."""
     from oracle import oracle as O
+    from oracle.step import OracleStepper
     from simcoflow_amd import cases as C
     O.build()
     if threads > 0:
         O.lib().orc_set_num_threads(threads)
     nthreads = O.lib().orc_num_threads()
-    small = C.case_swarm(512, 512, seed=0)
-    u, v, DpU, DpV = C.predictor_fields(small)
-    g = small.grid
+    small = C.case_swarm(256, 256, seed=0)
+    st = OracleStepper(O, small, tol=1e-8)
+    st.step()
+    st.step()
     t0 = time.perf_counter()
     n = 0
-    its = []
     while True:
-        M = O.poisson_matrix(g, small.geo, O.poisson_coef(g, small.itf, DpU, DpV, small.fluids), small.bc)
-        b = O.poisson_rhs(g, small.geo, u, v)
-        x, it, rr = O.pcg_mg_solve(g, M, b, tol=1e-8, max_iter=2000)
-        its.append(abs(it))
+        st.step()
         n += 1
-        if time.perf_counter() - t0 > 10.0 or n >= 50:
+        if time.perf_counter() - t0 > 10.0 or n >= 20:
             break
     dt = (time.perf_counter() - t0) / n
-    return {"value": round(g.ncells / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": nthreads, "kind": "port",
-            "sample": f"{n} projection solves (build + PCG-MG to 1e-8, {int(np.mean(its))} iterations) of the swarm "
-                      f"at 512x512, {dt * 1e3:.1f} ms each, on {nthreads} host threads"}
+    return {"value": round(small.grid.ncells / dt / 1e6, 4), "unit": "Mcell-steps/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"{n} full time steps (transport, U/V interface, predictor with direct momentum solves, "
+                      f"PCG-MG projection to 1e-8, correction) of the swarm at 256x256 after 2 warm-up steps, "
+                      f"{dt * 1e3:.0f} ms each; PCG-MG on {nthreads} host threads, the rest single-threaded"}
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the projection is "
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the hot path is "
               "not sharded across GPUs in this build.", file=sys.stderr)
         sys.exit(2)
     if not a.no_learned:
@@ -267,27 +304,34 @@ def main():
         # first; the library then binds to the same one
         import torch  # noqa: F401
     from simcoflow_amd import cases as C
-    log("building the swarm case")
+    log("building the swarm case and its staggered geometry")
     case = C.case_swarm(a.ni, a.nj, seed=a.seed)
+    stags = C.staggered_geometry(case.grid, case.geo, case.solid)
     fields = C.predictor_fields(case)
     tol = 1e-8
     n = case.grid.ncells
-    log("FP64 path")
-    fp64 = run_path(case, fields, 0, a.steps, a.warmup, tol)
-    log("mixed path")
-    mixed = run_path(case, fields, 1, a.steps, a.warmup, tol)
-    learned = None
-    train = None
+    w = K = train = None
     if not a.no_learned:
         log("training the warm-start net")
         w, K, train = train_warm_start(a.seed)
-        log("learned + mixed path")
-        learned = run_path(case, fields, 1, a.steps, a.warmup, tol, learned=(w, K))
-    head = learned if learned else mixed
-    err_mixed = float(np.linalg.norm(mixed["Pp"] - fp64["Pp"]) / np.linalg.norm(fp64["Pp"]))
+    # full time steps, one per solver path
+    paths = [("fp64", 0, None), ("mixed", 1, None)] + ([("learned_mixed", 1, (w, K))] if w is not None else [])
+    full = {}
+    for name, prec, lrn in paths:
+        log(f"full time steps, {name}")
+        full[name] = stepping(case, stags, prec, a.steps, a.warmup, learned=lrn)
+    # the projection alone on one fixed set of predictor fields
+    proj = {}
+    for name, prec, lrn in paths:
+        log(f"projection step on fixed fields, {name}")
+        proj[name] = run_path(case, fields, prec, a.steps, a.warmup, tol, learned=lrn)
+    head_name = paths[-1][0]
+    head = full[head_name]
     value = n / (head["ms_per_step"] * 1e-3) / 1e6
-    fp64_bytes = solve_bytes(n, fp64["iterations"], 8)
-    mixed_bytes = solve_bytes(n, mixed["iterations"], 4)
+    ref_p = proj["fp64"]["Pp"]
+    ref_vof = full["fp64"]["vof"]
+    cellvol = np.outer(case.grid.dy, case.grid.dx)
+    vol_ref = float((ref_vof * cellvol).sum())
     out = {
         "metric": "Poisson ms/step & Mcell-steps/s, learned+FP32 vs plain FP64 PCG at 1e-8 residual",
         "value": round(value, 3),
@@ -299,50 +343,56 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 PCG vectors and multigrid, f64 x / dots / residual recomputation",
-        "data": "synthetic (seeded bubble swarm, seeded predictor fields)",
-        "config": {"workload": f"bubble swarm {a.ni}x{a.nj}, 64 bubbles seed {a.seed}, projection step "
-                               f"(build + PCG-MG to 1e-8 + correction)", "ni": a.ni, "nj": a.nj,
-                   "path": "learned warm start, mixed precision" if learned else "mixed precision"},
-        "poisson_ms_per_step": {"fp64": round(fp64["poisson_ms"], 4), "mixed": round(mixed["poisson_ms"], 4)},
-        "ms_per_step_by_path": {"fp64": round(fp64["ms_per_step"], 4), "mixed": round(mixed["ms_per_step"], 4)},
-        "mcell_steps_per_s": {"fp64": round(n / (fp64["ms_per_step"] * 1e-3) / 1e6, 3),
-                              "mixed": round(n / (mixed["ms_per_step"] * 1e-3) / 1e6, 3)},
-        "iterations": {"fp64": fp64["iterations"], "mixed": mixed["iterations"]},
-        "relres": {"fp64": fp64["relres"], "mixed": mixed["relres"]},
-        "setup_ms": {"fp64": round(fp64["setup_ms"], 4), "mixed": round(mixed["setup_ms"], 4)},
-        "pressure_rel_l2_vs_fp64": {"mixed": err_mixed},
-        "max_mass_residual": {"fp64": fp64["max_mres"], "mixed": mixed["max_mres"]},
-        "solve_hbm": {
-            "fp64": {"bytes_alg": int(fp64_bytes), "GBs": round(fp64_bytes / (fp64["poisson_ms"] * 1e-3) / 1e9, 1),
-                     "frac": round(fp64_bytes / (fp64["poisson_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
-            "mixed": {"bytes_alg": int(mixed_bytes),
-                      "GBs": round(mixed_bytes / (mixed["poisson_ms"] * 1e-3) / 1e9, 1),
-                      "frac": round(mixed_bytes / (mixed["poisson_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}},
-        "speedup_vs_fp64": {"mixed": round(fp64["poisson_ms"] / mixed["poisson_ms"], 3)},
+        "dtype": "f32 PCG vectors and multigrid, f64 x / dots / residual recomputation / operator in q = A p; "
+                 "f64 transport and predictor",
+        "data": "synthetic (seeded bubble swarm at rest; seeded predictor fields for the fixed projection)",
+        "config": {"workload": f"bubble swarm {a.ni}x{a.nj} (1x1 m, liquid depth 0.8 m, 64 bubbles, seed {a.seed}), "
+                               "full time step: CFL dt, CLSVOF transport (6 sub-steps), U/V interface, box ghost "
+                               "layer, momentum predictor, projection (PCG-multigrid to 1e-8), correction",
+                   "ni": a.ni, "nj": a.nj,
+                   "path": "learned warm start, mixed precision" if w is not None else "mixed precision"},
+        "poisson_ms_per_step": {k: round(v["poisson_ms"], 4) for k, v in full.items()},
+        "ms_per_step_by_path": {k: round(v["ms_per_step"], 4) for k, v in full.items()},
+        "mcell_steps_per_s": {k: round(n / (v["ms_per_step"] * 1e-3) / 1e6, 3) for k, v in full.items()},
+        "iterations": {k: v["iterations"] for k, v in full.items()},
+        "relres": {k: v["relres"] for k, v in full.items()},
+        "poisson_speedup_vs_fp64": {k: round(full["fp64"]["poisson_ms"] / v["poisson_ms"], 3)
+                                    for k, v in full.items() if k != "fp64"},
+        "vof_max_abs_diff_vs_fp64": {k: float(np.abs(v["vof"] - ref_vof).max())
+                                     for k, v in full.items() if k != "fp64"},
+        "liquid_volume_rel_diff_vs_fp64": {k: abs(float((v["vof"] * cellvol).sum()) - vol_ref) / vol_ref
+                                           for k, v in full.items() if k != "fp64"},
     }
-    if learned:
-        out["poisson_ms_per_step"]["learned_mixed"] = round(learned["poisson_ms"], 4)
-        out["ms_per_step_by_path"]["learned_mixed"] = round(learned["ms_per_step"], 4)
-        out["mcell_steps_per_s"]["learned_mixed"] = round(value, 3)
-        out["iterations"]["learned_mixed"] = learned["iterations"]
-        out["relres"]["learned_mixed"] = learned["relres"]
-        out["pressure_rel_l2_vs_fp64"]["learned_mixed"] = float(
-            np.linalg.norm(learned["Pp"] - fp64["Pp"]) / np.linalg.norm(fp64["Pp"]))
-        out["speedup_vs_fp64"]["learned_mixed"] = round(fp64["poisson_ms"] / learned["poisson_ms"], 3)
-        out["learned"] = dict(train, warm_start_ms=round(learned["warm_ms"], 4),
-                              start_relres=learned["warm_relres"])
-    k = mixed["kernel"].get("up")
+    fixed = {"poisson_ms": {}, "setup_ms": {}, "iterations": {}, "relres": {}, "pressure_rel_l2_vs_fp64": {},
+             "max_mass_residual": {}, "speedup_vs_fp64": {}, "solve_hbm": {}}
+    for k, v in proj.items():
+        fixed["poisson_ms"][k] = round(v["poisson_ms"], 4)
+        fixed["setup_ms"][k] = round(v["setup_ms"], 4)
+        fixed["iterations"][k] = v["iterations"]
+        fixed["relres"][k] = v["relres"]
+        fixed["max_mass_residual"][k] = v["max_mres"]
+        if k != "fp64":
+            fixed["pressure_rel_l2_vs_fp64"][k] = float(np.linalg.norm(v["Pp"] - ref_p) / np.linalg.norm(ref_p))
+            fixed["speedup_vs_fp64"][k] = round(proj["fp64"]["poisson_ms"] / v["poisson_ms"], 3)
+        if k != "learned_mixed":
+            b = solve_bytes(n, v["iterations"], 8 if k == "fp64" else 4)
+            gbs = b / (v["poisson_ms"] * 1e-3) / 1e9
+            fixed["solve_hbm"][k] = {"bytes_alg": int(b), "GBs": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4)}
+    out["projection_fixed_fields"] = fixed
+    if w is not None:
+        out["learned"] = dict(train, warm_start_ms=round(proj["learned_mixed"]["warm_ms"], 4),
+                              start_relres=proj["learned_mixed"]["warm_relres"])
+    k = proj["mixed"]["kernel"].get("up")
     if k:
         out["roofline"] = {"bound": "hbm", "achieved": round(k["GBs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                            "frac": round(k["GBs"] / PEAK_HBM_GBS, 4), "traffic": measured_traffic(k["name"]),
                            "kernel": k["name"], "avg_ms": round(k["avg_ms"], 5), "bytes_alg": int(k["bytes"])}
-        out["kernels"] = {"fp64_up_sweep": fp64["kernel"].get("up"), "fp64_apply": fp64["kernel"].get("apply"),
-                          "copy": fp64["kernel"].get("copy")}
+        out["kernels"] = {"fp64_up_sweep": proj["fp64"]["kernel"].get("up"),
+                          "fp64_apply": proj["fp64"]["kernel"].get("apply"),
+                          "copy": proj["fp64"]["kernel"].get("copy")}
     if not a.no_full_step:
-        log("full time step")
-        out["full_step_ms"] = {"mixed": full_step(case, 1)}
-        out["full_step_ms"]["mcell_steps_per_s"] = round(n / (out["full_step_ms"]["mixed"]["step"] * 1e-3) / 1e6, 3)
+        log("time step parts")
+        out["step_parts_ms"] = {"mixed": step_parts(case, stags, 1)}
     if not a.no_cpu:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_threads)

@@ -93,7 +93,7 @@ def loss_and_grads(x, t, m, Ws, bf16=True):
     y = forward(x, Ws, bf16=bf16)
     loss = (m * (y - t) ** 2).sum() / y.numel()
     loss.backward()
-    return float(loss), y.detach(), [w.grad for w in Ws]
+    return float(loss.detach()), y.detach(), [w.grad for w in Ws]
 
 
 def bf16_round(a):

@@ -1,4 +1,8 @@
-"""The oracle's time step, in the order of simcoflow_amd.solver.TimeStepper
+"""The oracle's time step (test infrastructure: the checker of the GPU
+TimeStepper and the CPU-baseline leg of bench.py; nothing in the product path
+imports it).
+
+``OracleStepper.step`` runs, in the order of simcoflow_amd.solver.TimeStepper
 (AdamBasforthCrankNicolson, Solver_Mod.f90:246-319, solids at rest, closed
 box): copies of the time-n interface, ComputeTimeStep, Coupled_LS_VOF and
 Initial_ClsVofUV (from the second step), the closed-box ghost layer,
@@ -8,8 +12,36 @@ from types import SimpleNamespace
 
 import numpy as np
 
-import predictor_cases as P
 from simcoflow_amd import cases as C
+
+
+def phys_of(fl, Re=None, Fr=None):
+    """Non-dimensional groups with Uref = 1 m/s, Lref = 1 m
+    (StateVariables.f90: Re = Uref Lref / nuref, Fr = Uref / sqrt(g Lref))."""
+    nu_l = fl.mu_l / fl.rho_l
+    return dict(rho_l=fl.rho_l, rho_g=fl.rho_g, rho_ref=fl.rho_ref, nu_l=nu_l, nu_g=fl.mu_g / fl.rho_g,
+                nu_ref=nu_l, Re=1.0 / nu_l if Re is None else Re, Fr=1.0 / np.sqrt(fl.g) if Fr is None else Fr)
+
+
+def oracle_predictor(O, d, first_step=False, vb=0.0):
+    """The oracle's step: rows, direct solves, scatter and ghost copies."""
+    g = d["grid"]
+    r = O.predictor_assemble(g, d["U"], d["V"], d["uvo"], d["uvn"], d["vvo"], d["vvn"], d["pcell"], d["u"], d["v"],
+                             d["phys"], d["dt"], vb=vb, first_step=first_step, fluxn1=d["fluxn1"], DpU=d["DpU"],
+                             DpV=d["DpV"])
+    au = d["U"].active.copy()
+    au[:, -1] = 0
+    av = d["V"].active.copy()
+    av[-1, :] = 0
+    xu = O.momentum_solve(r["matr_u"], r["rhs_u"], au)
+    xv = O.momentum_solve(r["matr_v"], r["rhs_v"], av)
+    up = d["upred"].copy()
+    vp = d["vpred"].copy()
+    up[1:-1, 1:-1][:, :-1] = np.where(au[:, :-1] == 1, xu[:, :-1], 0.0)
+    vp[1:-1, 1:-1][:-1, :] = np.where(av[:-1, :] == 1, xv[:-1, :], 0.0)
+    DU, DV, vp = O.predictor_finish(g, r["DpU"], r["DpV"], vp)
+    r.update(upred=up, vpred=vp, DpU=DU, DpV=DV, xu=xu, xv=xv)
+    return r
 
 
 class OracleStepper:
@@ -28,7 +60,7 @@ class OracleStepper:
         self.fluxn1 = np.zeros((2, g.nj + 2, g.ni + 2))
         self.uvof = self._to_uv(0)
         self.vvof = self._to_uv(1)
-        self.phys = P.phys_of(case.fluids)
+        self.phys = phys_of(case.fluids)
         self.itt = 0
         self.time = 0.0
 
@@ -54,7 +86,7 @@ class OracleStepper:
                             phiS=geo.phiS),
                  u=self.u, v=self.v, phys=self.phys, dt=dt, fluxn1=self.fluxn1, DpU=self.DpU, DpV=self.DpV,
                  upred=self.upred, vpred=self.vpred)
-        r = P.oracle_predictor(O, d, first_step=(self.itt == 1))
+        r = oracle_predictor(O, d, first_step=(self.itt == 1))
         self.fluxn1, self.DpU, self.DpV = r["fluxn1"], r["DpU"], r["DpV"]
         self.upred, self.vpred = r["upred"], r["vpred"]
         f = SimpleNamespace(vof=self.vof, phi=self.phi, vofS=geo.vofS)

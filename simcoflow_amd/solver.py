@@ -37,7 +37,10 @@ def physics(fluids):
 
 class TimeStepper:
     def __init__(self, case, precision=_lib.SG_PREC_FP64, tol=None, cfl=0.25, vint=0.0, uref=1.0, max_iter=500,
-                 learned=None):
+                 learned=None, stags=None):
+        """learned: (weights, level, out_scale, deflate) of a trained warm
+        start, level None for the first multigrid level no wider than 256;
+        stags: the (U, V) cases.StagGeometry if already computed."""
         self.case = case
         g = case.grid
         self.cfl, self.vint, self.uref = cfl, vint, uref
@@ -48,7 +51,7 @@ class TimeStepper:
         # params_for takes nu from the fluids already
         self.pj = pj = Projection(g, self.params)
         pj.set_geometry(case.geo)
-        U, V = C.staggered_geometry(g, case.geo, case.solid)
+        U, V = stags if stags is not None else C.staggered_geometry(g, case.geo, case.solid)
         pj.set_staggered_geometry(0, U)
         pj.set_staggered_geometry(1, V)
         itf = case.itf
@@ -62,7 +65,8 @@ class TimeStepper:
         check(pj.lib.sg_interface_to_uv_resident(pj.ctx))
         if learned:
             from . import learned as L
-            L.enable(pj, *learned)
+            w, level, scale, deflate = learned
+            L.enable(pj, w, L.pick_level(pj) if level is None else level, scale, deflate)
         self.itt = 0
         self.time = 0.0
         self.last = {}

@@ -3,15 +3,8 @@ n, U/V cell geometry and liquid fractions, face velocities, Flux_n1 and the
 previous Dp, all seeded."""
 import numpy as np
 
+from oracle.step import oracle_predictor, phys_of  # noqa: F401  (re-exported for the tests)
 from simcoflow_amd import cases as C
-
-
-def phys_of(fl, Re=None, Fr=None):
-    """Non-dimensional groups with Uref = 1 m/s, Lref = 1 m
-    (StateVariables.f90: Re = Uref Lref / nuref, Fr = Uref / sqrt(g Lref))."""
-    nu_l = fl.mu_l / fl.rho_l
-    return dict(rho_l=fl.rho_l, rho_g=fl.rho_g, rho_ref=fl.rho_ref, nu_l=nu_l, nu_g=fl.mu_g / fl.rho_g,
-                nu_ref=nu_l, Re=1.0 / nu_l if Re is None else Re, Fr=1.0 / np.sqrt(fl.g) if Fr is None else Fr)
 
 
 def predictor_inputs(case, seed=0, amp=0.4, Re=None, shift=0.3):
@@ -42,27 +35,6 @@ def predictor_inputs(case, seed=0, amp=0.4, Re=None, shift=0.3):
     return dict(grid=g, U=U, V=V, uvo=uvo, uvn=uvn, vvo=vvo, vvn=vvn, pcell=pcell, u=np.ascontiguousarray(u),
                 v=np.ascontiguousarray(v), phys=phys_of(case.fluids, Re=Re), dt=case.dt, fluxn1=fluxn1,
                 DpU=DpU, DpV=DpV, upred=upred, vpred=vpred, geo=case.geo)
-
-
-def oracle_predictor(O, d, first_step=False, vb=0.0):
-    """The oracle's step: rows, direct solves, scatter and ghost copies."""
-    g = d["grid"]
-    r = O.predictor_assemble(g, d["U"], d["V"], d["uvo"], d["uvn"], d["vvo"], d["vvn"], d["pcell"], d["u"], d["v"],
-                             d["phys"], d["dt"], vb=vb, first_step=first_step, fluxn1=d["fluxn1"], DpU=d["DpU"],
-                             DpV=d["DpV"])
-    au = d["U"].active.copy()
-    au[:, -1] = 0
-    av = d["V"].active.copy()
-    av[-1, :] = 0
-    xu = O.momentum_solve(r["matr_u"], r["rhs_u"], au)
-    xv = O.momentum_solve(r["matr_v"], r["rhs_v"], av)
-    up = d["upred"].copy()
-    vp = d["vpred"].copy()
-    up[1:-1, 1:-1][:, :-1] = np.where(au[:, :-1] == 1, xu[:, :-1], 0.0)
-    vp[1:-1, 1:-1][:-1, :] = np.where(av[:-1, :] == 1, xv[:-1, :], 0.0)
-    DU, DV, vp = O.predictor_finish(g, r["DpU"], r["DpV"], vp)
-    r.update(upred=up, vpred=vp, DpU=DU, DpV=DV, xu=xu, xv=xv)
-    return r
 
 
 def device_predictor(d, first_step=False, vb=0.0):

@@ -71,7 +71,7 @@ def _run_gpu(case, steps, precision, tol):
 
 
 def test_200_steps_match_oracle(oracle_lib):
-    import step_oracle as S
+    from oracle import step as S
     case = C.case_bubble(64, 128)
     g = case.grid
     print("[step] bubble 64x128, 200 steps, GPU FP64", flush=True)
@@ -98,7 +98,7 @@ def test_mixed_precision_interface_error():
     """200 steps with the mixed-precision solver (FP32 multigrid, FP64
     outer) against the FP64 solver, both to a relative residual of 1e-10:
     interface position within 0.01 cell, pressures and volumes reported."""
-    import step_oracle as S
+    from oracle import step as S
     case = C.case_bubble(64, 128)
     g = case.grid
     print("[step] bubble 64x128, 200 steps, FP64 and mixed", flush=True)

@@ -55,10 +55,7 @@ def test_oracle_steps_keep_volume_and_rest(oracle_lib):
     """A few oracle steps of a bubble: liquid volume kept (the sweeps are
     conservative up to the clipping of the implicit sweep), the correction
     leaves no mass residual, and the bubble starts to rise."""
-    import sys
-    import os
-    sys.path.insert(0, os.path.dirname(__file__))
-    from step_oracle import OracleStepper, gas_centroid
+    from oracle.step import OracleStepper, gas_centroid
     case = C.case_bubble(32, 64)
     st = OracleStepper(oracle_lib, case, tol=1e-12)
     v0 = st.vof.sum()
