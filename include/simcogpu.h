@@ -50,7 +50,8 @@ typedef struct sg_params {
     int mg_pre, mg_post; /* smoothing sweeps per level (ProjectionP_Mod.f90:126: 1) */
     int warm_start;  /* 0: x0 = 0 (ProjectionP_Mod.f90:319); 1: x0 = previous solution;
                         2: learned warm start + deflation vector (sg_set_learned) */
-    int check_every; /* PCG iterations between host convergence checks (>= 1) */
+    int check_every; /* PCG iterations between host convergence checks; 0 (default): set from
+                        the observed convergence rate */
 } sg_params;
 
 typedef struct sg_solve_info {

@@ -42,8 +42,9 @@ static bool is_int_field(int f) { return f == SG_F_UFLAG || f == SG_F_VFLAG; }
 static void destroy(Ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (hipGraphExec_t g : c->iter_graph)
-        if (g) (void)hipGraphExecDestroy(g);
+    for (auto& per : c->iter_graph)
+        for (hipGraphExec_t g : per)
+            if (g) (void)hipGraphExecDestroy(g);
     learned_release(c);
     if (c->ev_nn0) (void)hipEventDestroy(c->ev_nn0);
     if (c->ev_nn1) (void)hipEventDestroy(c->ev_nn1);
@@ -103,7 +104,7 @@ int sg_params_default(sg_params* p) {
     p->mg_pre = 1;
     p->mg_post = 1;
     p->warm_start = 0;
-    p->check_every = 2;
+    p->check_every = 0;
     return SG_OK;
 }
 

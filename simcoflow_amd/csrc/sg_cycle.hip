@@ -44,7 +44,8 @@ __device__ inline bool in_dom(const Dims& D, int i, int j) { return i >= 0 && j 
 // down: x = S(0; b) (red from zero, then black), r = b - A x, bc = P^T r
 template <typename T, int TW, int TH>
 __global__ void __launch_bounds__(NTB) k_down9(Dims F, OpV<T> A, const T* __restrict__ b, T* __restrict__ xo,
-                                               Dims C, Prol<T> P, T* __restrict__ bc) {
+                                               Dims C, Prol<T> P, T* __restrict__ bc, const int* on) {
+    if (on && !*on) return;
     constexpr int RW = TW + 5, RH = TH + 5;  // local (li,lj) <-> cell (i0+li-3, j0+lj-3)
     constexpr int NL = (RW * RH + NTB - 1) / NTB;
     __shared__ T sb[RH][RW];
@@ -138,7 +139,9 @@ __global__ void __launch_bounds__(NTB) k_down9(Dims F, OpV<T> A, const T* __rest
 // up: x = xin + P e, then black and red passes; writes xo
 template <typename T, int TW, int TH>
 __global__ void __launch_bounds__(NTB) k_up9(Dims F, OpV<T> A, const T* __restrict__ b, const T* __restrict__ xin,
-                                             Dims C, Prol<T> P, const T* __restrict__ e, T* __restrict__ xo) {
+                                             Dims C, Prol<T> P, const T* __restrict__ e, T* __restrict__ xo,
+                                             const int* on) {
+    if (on && !*on) return;
     constexpr int RW = TW + 4, RH = TH + 4;  // local (li,lj) <-> cell (i0+li-2, j0+lj-2)
     __shared__ T sb[RH][RW];
     __shared__ T sx[RH][RW];
@@ -245,7 +248,8 @@ constexpr int GW = 64, GH = 16;  // fine-level tile
 // read theirs from memory.
 template <typename T, typename TB, int TW, int TH>
 __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __restrict__ b, T* __restrict__ xo,
-                                               Dims C, Prol<T> P, T* __restrict__ bc) {
+                                               Dims C, Prol<T> P, T* __restrict__ bc, const int* on) {
+    if (on && !*on) return;
     constexpr int RW = TW + 5, RH = TH + 5;  // local (li,lj) <-> cell (i0+li-3, j0+lj-3)
     constexpr int PPR = TW / 2;
     constexpr int PPT = PPR * TH / NTB;
@@ -394,7 +398,9 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
 template <typename T, typename V, int TW, int TH>
 __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const V* __restrict__ b,
                                              const T* __restrict__ xin, Dims C, Prol<T> P, const T* __restrict__ e,
-                                             V* __restrict__ z, const V* __restrict__ q, Slots Sl) {
+                                             V* __restrict__ z, const V* __restrict__ q, Slots Sl,
+                                             const int* on) {
+    if (on && !*on) return;
     constexpr int RW = TW + 4, RH = TH + 4;  // local (li,lj) <-> cell (i0+li-2, j0+lj-2)
     constexpr int PPR = TW / 2;              // pairs per row
     constexpr int PPT = PPR * TH / NTB;      // pairs per thread
@@ -557,6 +563,7 @@ struct TailArgs {
     int n;  // number of tail levels (the last is the coarsest)
     TailLevel<T> L[MAX_TAIL];
     const T* inv;  // dense inverse of the coarsest operator
+    const int* on; // the PCG's convergence flag (the cycle runs while *on), or null
 };
 
 template <typename T>
@@ -611,6 +618,7 @@ __device__ void tail_colour(T* lds, const TailLevel<T>& L, int col, bool zero_st
 
 template <typename T>
 __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_elems) {
+    if (a.on && !*a.on) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
     const int tid = threadIdx.x;
@@ -736,7 +744,8 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
 // coarsest-only hierarchy (grids of at most COARSEST_MAX cells)
 template <typename T, typename TB, typename TZ>
 __global__ void __launch_bounds__(256) k_direct(Dims C, const T* __restrict__ inv, const TB* __restrict__ b,
-                                                TZ* __restrict__ x) {
+                                                TZ* __restrict__ x, const int* on) {
+    if (on && !*on) return;
     const int n = C.ni * C.nj;
     const int r = threadIdx.x;
     if (r >= n) return;
@@ -764,8 +773,9 @@ struct Acc<float> {
 
 inline dim3 tiles(const Dims& d, int tw, int th) { return dim3((d.ni + tw - 1) / tw, (d.nj + th - 1) / th); }
 
-// 64x16 tiles where they give at least two per CU, 32x8 below
-inline bool big(const Dims& d) { return (size_t)d.ni * d.nj >= (size_t)512 * 512; }
+// 64x16 tiles on coarse levels of 2048^2 and more, 32x8 below: the latency
+// of a tile's passes is hidden only with several tiles per CU
+inline bool big(const Dims& d) { return (size_t)d.ni * d.nj >= (size_t)2048 * 2048; }
 
 inline Dims compact(const Dims& d) {
     Dims c = d;
@@ -774,15 +784,16 @@ inline Dims compact(const Dims& d) {
 }
 
 // z = M^{-1} r ; fine-level dots (r,z), (z,q) go to the partials slot S;
-// V = type of the PCG vectors r, z, q
+// V = type of the PCG vectors r, z, q; every kernel returns at once while
+// *on == 0 (a converged PCG)
 template <typename T, typename V>
-int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done) {
+int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done, const int* on) {
     hipStream_t s = c->stream;
     const int nl = c->nlev;
     *dots_done = 0;
     if (nl == 1) {
         const Level& L = c->lev[0];
-        hipLaunchKernelGGL((k_direct<T, V, V>), dim3(1), dim3(256), 0, s, L.d, Acc<T>::inv(c), r, z);
+        hipLaunchKernelGGL((k_direct<T, V, V>), dim3(1), dim3(256), 0, s, L.d, Acc<T>::inv(c), r, z, on);
         SG_HIP(hipGetLastError());
         return SG_OK;
     }
@@ -795,15 +806,15 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done) {
         const Level& Cl = c->lev[l + 1];
         if (l == 0)
             hipLaunchKernelGGL((k_down5<T, V, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
-                               Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), Acc<T>::vec(Cl).b);
+                               Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), Acc<T>::vec(Cl).b, on);
         else if (big(L.d))
             hipLaunchKernelGGL((k_down9<T, 64, 16>), tiles(L.d, 64, 16), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
                                (const T*)Acc<T>::vec(L).b, Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L),
-                               Acc<T>::vec(Cl).b);
+                               Acc<T>::vec(Cl).b, on);
         else
             hipLaunchKernelGGL((k_down9<T, 32, 8>), tiles(L.d, 32, 8), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
                                (const T*)Acc<T>::vec(L).b, Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L),
-                               Acc<T>::vec(Cl).b);
+                               Acc<T>::vec(Cl).b, on);
     }
     // tail
     TailArgs<T> ta{};
@@ -835,6 +846,7 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done) {
         return SG_ERR_STATE;
     }
     ta.inv = Acc<T>::inv(c);
+    ta.on = on;
     hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(TAIL_THREADS), lds_bytes, s, ta, off);
     // up: level l reads x_l from the down pass and e_{l+1}; writes its own
     // correction into r_l (its solution for the level above)
@@ -844,15 +856,15 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done) {
         const T* e = (l + 1 == lt) ? Acc<T>::vec(Cl).x : Acc<T>::vec(Cl).r;
         if (l == 0)
             hipLaunchKernelGGL((k_up5<T, V, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
-                               (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e, z, q, S);
+                               (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e, z, q, S, on);
         else if (big(L.d))
             hipLaunchKernelGGL((k_up9<T, 64, 16>), tiles(L.d, 64, 16), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
                                (const T*)Acc<T>::vec(L).b, (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e,
-                               Acc<T>::vec(L).r);
+                               Acc<T>::vec(L).r, on);
         else
             hipLaunchKernelGGL((k_up9<T, 32, 8>), tiles(L.d, 32, 8), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
                                (const T*)Acc<T>::vec(L).b, (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e,
-                               Acc<T>::vec(L).r);
+                               Acc<T>::vec(L).r, on);
     }
     SG_HIP(hipGetLastError());
     return SG_OK;
@@ -863,8 +875,8 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done) {
 int mg_vcycle_dots(Ctx* c, double* partials, int max_blocks, int* dots_done) {
     Slots S{partials, max_blocks};
     if (c->prm.precision == SG_PREC_MIXED)
-        return vcycle<float, float>(c, c->rf, c->zf, c->qf, S, dots_done);
-    return vcycle<double, double>(c, c->r, c->z, c->q, S, dots_done);
+        return vcycle<float, float>(c, c->rf, c->zf, c->qf, S, dots_done, &c->sc->active);
+    return vcycle<double, double>(c, c->r, c->z, c->q, S, dots_done, &c->sc->active);
 }
 
 namespace {
@@ -886,21 +898,21 @@ int mg_launch_fine(Ctx* c, int which) {
     if (which == 1) {
         if (mixed)
             hipLaunchKernelGGL((k_down5<float, float, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->rf,
-                               L.vf.x, Cl.d, L.Pf, Cl.vf.b);
+                               L.vf.x, Cl.d, L.Pf, Cl.vf.b, nullptr);
         else
             hipLaunchKernelGGL((k_down5<double, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
-                               L.v.x, Cl.d, L.P, Cl.v.b);
+                               L.v.x, Cl.d, L.P, Cl.v.b, nullptr);
     } else if (which == 2) {
         // the coarse correction sits in x when level 1 is the tail's first level, else in r
         const bool tail1 = c->nlev == 2 || (size_t)Cl.d.ni * Cl.d.nj <= (size_t)TAIL_MAX;
         if (mixed)
             hipLaunchKernelGGL((k_up5<float, float, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af),
                                c->rf, (const float*)L.vf.x, Cl.d, L.Pf, (const float*)(tail1 ? Cl.vf.x : Cl.vf.r),
-                               c->zf, c->qf, S);
+                               c->zf, c->qf, S, nullptr);
         else
             hipLaunchKernelGGL((k_up5<double, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
                                (const double*)L.v.x, Cl.d, L.P, (const double*)(tail1 ? Cl.v.x : Cl.v.r), c->z,
-                               c->q, S);
+                               c->q, S, nullptr);
     } else if (which == 3) {
         const size_t n = L.d.alloc_elems();
         hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, (const double*)L.v.b,

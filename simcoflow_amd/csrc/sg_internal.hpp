@@ -100,6 +100,7 @@ struct PcgScalars {
     double wb;      // learned warm start: (w, b)
     double walpha;  // learned warm start: x0 = walpha * w
     double nn_scale;// rms of the restricted right-hand side (net input scale)
+    double rr_ref;  // (r,r) at the last FP64 residual recomputation
     int active;     // 1 while not converged
     int iters;      // completed iterations
     int first;      // 1 before the first direction update
@@ -218,7 +219,9 @@ struct Ctx {
     double vb = 0.0;
     // captured pair of PCG iterations, per precision (kernel arguments are
     // fixed for the life of the context, so one capture serves every solve)
-    hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
+    // [precision][0: two iterations from parity 0, 1: one from parity 0,
+    // 2: one from parity 1]
+    hipGraphExec_t iter_graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0}, graph_defl[2] = {0, 0};
     NnState nn;
     double* step_dev = nullptr;   // device scalars of the time-step control (dt)

@@ -11,7 +11,10 @@
 // recomputation and again at convergence, so the FP32 recursion never
 // limits the attained FP64 residual.  The FP32 cycle is not exactly
 // symmetric, so beta takes the Polak-Ribiere (flexible) form
-// -alpha*(z_new,q_old)/(r_old,z_old).
+// -alpha*(z_new,q_old)/(r_old,z_old).  The recomputation test runs on the
+// device, inside the captured iterations, so the host only tests
+// convergence -- after a number of iterations it sets from the observed
+// rate; iterations launched past convergence cost one flag test per kernel.
 #include "sg_internal.hpp"
 #include "sg_reduce.hpp"
 
@@ -22,7 +25,15 @@ namespace {
 constexpr int NT = TX * TY;
 constexpr int NTF = 1024;  // threads of the finalize kernels
 
-enum FinMode { FIN_INIT = 0, FIN_REPLACE, FIN_RZ, FIN_PQ, FIN_RR, FIN_TRUE, FIN_MU, FIN_RELIABLE };
+enum FinMode { FIN_INIT = 0, FIN_REPLACE, FIN_RZ, FIN_PQ, FIN_RR, FIN_TRUE, FIN_MU, FIN_RELIABLE, FIN_RELIABLE_IF };
+
+// mixed precision: the FP32 residual is recomputed from x in FP64 once it
+// has fallen by this factor (in (r,r)) since the last recomputation
+constexpr double RELIABLE_DROP = 1e-4;
+
+__device__ inline bool reliable_due(const PcgScalars* sc) {
+    return sc->active && sc->rr < RELIABLE_DROP * sc->rr_ref;
+}
 
 // one reduction slot: NRED planes of per-block partials
 struct Slots {
@@ -31,11 +42,13 @@ struct Slots {
 };
 
 // r = b - A x in FP64, stored as V (or r = b for a zero start); partials of
-// ||b||^2, ||r||^2
+// ||b||^2, ||r||^2.  With `gate`, only when a recomputation is due.
 template <typename V>
 __global__ void __launch_bounds__(256) k_init(Dims D, const double* __restrict__ diag, const double* __restrict__ cE,
                                               const double* __restrict__ cN, const double* __restrict__ b,
-                                              double* __restrict__ x, V* __restrict__ r, int warm, Slots S) {
+                                              double* __restrict__ x, V* __restrict__ r, int warm, Slots S,
+                                              const PcgScalars* gate) {
+    if (gate && !reliable_due(gate)) return;
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     double v[2] = {0.0, 0.0};
@@ -201,6 +214,10 @@ __global__ void __launch_bounds__(256) k_true_resid(Dims D, const double* __rest
 
 // One block: reduce the partials in block order and update the scalars.
 __global__ void __launch_bounds__(NTF) k_finalize(Slots S, int nblocks, int mode, PcgScalars* sc, double tol) {
+    if (mode == FIN_RELIABLE_IF) {
+        if (!reliable_due(sc)) return;
+        mode = FIN_RELIABLE;
+    }
     if (mode != FIN_INIT && mode != FIN_REPLACE && mode != FIN_TRUE && mode != FIN_RELIABLE && !sc->active) return;
     if (mode == FIN_TRUE) {
         __shared__ double mx[NTF / 64];
@@ -224,6 +241,7 @@ __global__ void __launch_bounds__(NTF) k_finalize(Slots S, int nblocks, int mode
         finalize_sum<2, NTF>(S.part, S.max_blocks, nblocks, s2);
         if (threadIdx.x != 0) return;
         sc->rr = s2[1];
+        sc->rr_ref = s2[1];
         sc->active = (s2[1] > sc->tol2) ? 1 : 0;
         if (isnan(s2[1])) {
             sc->active = 0;
@@ -250,6 +268,7 @@ __global__ void __launch_bounds__(NTF) k_finalize(Slots S, int nblocks, int mode
             sc->iters = 0;
         }
         sc->rr = s[1];
+        sc->rr_ref = s[1];
         sc->first = 1;
         sc->beta = 0.0;
         sc->active = (s[0] > 0.0 && s[1] > sc->tol2) ? 1 : 0;
@@ -322,6 +341,14 @@ int enqueue_iteration_t(Ctx* c, int parity, V* r, V* z, V* q, V* p0, V* p1) {
     hipLaunchKernelGGL(k_update<V>, g, b, 0, s, D, (const V*)pnew, (const V*)q, c->x, r, S3,
                        (const PcgScalars*)c->sc);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S3, nb, (int)FIN_RR, c->sc, 0.0);
+    if (sizeof(V) == 4 && parity == 1) {
+        // FP32 vectors: recompute r from x in FP64 when it is due, keeping
+        // the search direction
+        Slots S0 = slots(c, 0);
+        hipLaunchKernelGGL(k_init<V>, g, b, 0, s, D, (const double*)A.diag, (const double*)A.cE,
+                           (const double*)A.cN, (const double*)c->rhs, c->x, r, 1, S0, (const PcgScalars*)c->sc);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S0, nb, (int)FIN_RELIABLE_IF, c->sc, 0.0);
+    }
     SG_HIP(hipGetLastError());
     return SG_OK;
 }
@@ -342,32 +369,37 @@ int enqueue_init(Ctx* c, int warm, int mode) {
     Slots S0 = slots(c, 0);
     if (mixed(c))
         hipLaunchKernelGGL(k_init<float>, g, block2d(), 0, c->stream, L0.d, L0.A.diag, L0.A.cE, L0.A.cN,
-                           (const double*)c->rhs, c->x, c->rf, warm, S0);
+                           (const double*)c->rhs, c->x, c->rf, warm, S0, nullptr);
     else
         hipLaunchKernelGGL(k_init<double>, g, block2d(), 0, c->stream, L0.d, L0.A.diag, L0.A.cE, L0.A.cN,
-                           (const double*)c->rhs, c->x, c->r, warm, S0);
+                           (const double*)c->rhs, c->x, c->r, warm, S0, nullptr);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, c->stream, S0, nb, mode, c->sc, c->prm.tol);
     SG_HIP(hipGetLastError());
     return SG_OK;
 }
 
-// Two PCG iterations (parities 0 and 1) captured once into a graph.
-int iteration_graph(Ctx* c, hipGraphExec_t* out) {
+// PCG iterations captured once into graphs: variant 0 = two iterations
+// (parities 0 and 1), 1 = one iteration of parity 0, 2 = one of parity 1.
+int iteration_graph(Ctx* c, int variant, hipGraphExec_t* out) {
     const int k = c->prm.precision == SG_PREC_MIXED ? 1 : 0;
     const int defl = deflating(c) ? 1 : 0;
-    if (c->iter_graph[k] && c->graph_pre[k] == c->prm.mg_pre && c->graph_post[k] == c->prm.mg_post &&
-        c->graph_defl[k] == defl) {
-        *out = c->iter_graph[k];
-        return SG_OK;
+    if (c->graph_pre[k] != c->prm.mg_pre || c->graph_post[k] != c->prm.mg_post || c->graph_defl[k] != defl) {
+        for (hipGraphExec_t& g : c->iter_graph[k]) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+        c->graph_pre[k] = c->prm.mg_pre;
+        c->graph_post[k] = c->prm.mg_post;
+        c->graph_defl[k] = defl;
     }
-    if (c->iter_graph[k]) {
-        (void)hipGraphExecDestroy(c->iter_graph[k]);
-        c->iter_graph[k] = nullptr;
+    if (c->iter_graph[k][variant]) {
+        *out = c->iter_graph[k][variant];
+        return SG_OK;
     }
     hipGraph_t graph = nullptr;
     SG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_iteration(c, 0);
-    if (rc == SG_OK) rc = enqueue_iteration(c, 1);
+    int rc = enqueue_iteration(c, variant == 2 ? 1 : 0);
+    if (rc == SG_OK && variant == 0) rc = enqueue_iteration(c, 1);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -378,11 +410,33 @@ int iteration_graph(Ctx* c, hipGraphExec_t* out) {
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-    c->iter_graph[k] = exec;
-    c->graph_pre[k] = c->prm.mg_pre;
-    c->graph_post[k] = c->prm.mg_post;
-    c->graph_defl[k] = defl;
+    c->iter_graph[k][variant] = exec;
     *out = exec;
+    return SG_OK;
+}
+
+// enqueue n PCG iterations, continuing the direction parity c->pcur
+int launch_iterations(Ctx* c, int n) {
+    while (n > 0) {
+        if (c->use_graph) {
+            const int variant = (c->pcur == 0 && n >= 2) ? 0 : (c->pcur == 0 ? 1 : 2);
+            hipGraphExec_t g = nullptr;
+            const int rc = iteration_graph(c, variant, &g);
+            if (rc) return rc;
+            SG_HIP(hipGraphLaunch(g, c->stream));
+            if (variant == 0) {
+                n -= 2;
+            } else {
+                n -= 1;
+                c->pcur ^= 1;
+            }
+        } else {
+            const int rc = enqueue_iteration(c, c->pcur);
+            if (rc) return rc;
+            c->pcur ^= 1;
+            n -= 1;
+        }
+    }
     return SG_OK;
 }
 
@@ -446,51 +500,38 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
         if (c->sc_host->wq > 0.0) warm_energy = c->sc_host->wb * c->sc_host->wb / c->sc_host->wq;
     }
     const int max_iter = c->prm.max_iter > 0 ? c->prm.max_iter : 1000;
-    const int every = c->prm.check_every > 0 ? c->prm.check_every : 1;
-    hipGraphExec_t graph = nullptr;
-    if (c->use_graph) {
-        c->pcur = 0;
-        rc = iteration_graph(c, &graph);
-        if (rc) return rc;
-    }
-    int it = 0;
+    const int every = c->prm.check_every;
+    c->pcur = 0;
+    int it = 0;  // iterations launched (those past convergence do nothing)
     int replacements = 0;
-    double rr_ref = c->sc_host->rr;  // ||r||^2 at the last FP64 recomputation
+    // convergence rate per iteration (log of the (r,r) ratio) between host
+    // checks, 0 while unknown
+    double rate = 0.0;
     for (;;) {
         while (c->sc_host->active && it < max_iter) {
-            if (graph) {
-                // whole graph launches of two iterations; iterations past
-                // convergence leave x and r untouched (kernels test sc->active)
-                int pairs = (every + 1) / 2;
-                const int left = (max_iter - it + 1) / 2;
-                if (pairs > left) pairs = left;
-                for (int k = 0; k < pairs; ++k) SG_HIP(hipGraphLaunch(graph, s));
-                it += 2 * pairs;
-            } else {
-                const int nb = (max_iter - it) < every ? (max_iter - it) : every;
-                for (int k = 0; k < nb; ++k) {
-                    rc = enqueue_iteration(c, c->pcur);
-                    if (rc) return rc;
-                    c->pcur ^= 1;
-                }
-                it += nb;
+            int n = 2;
+            if (every > 0) {
+                n = every;
+            } else if (rate < 0.0) {
+                const double need = log(c->sc_host->tol2 / c->sc_host->rr) / rate;
+                n = need < 1.0 ? 1 : (need > 16.0 ? 16 : (int)ceil(need));
             }
+            if (n > max_iter - it) n = max_iter - it;
+            const double rr0 = c->sc_host->rr;
+            const int k0 = c->sc_host->iters;
+            rc = launch_iterations(c, n);
+            if (rc) return rc;
+            it += n;
             SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
             SG_HIP(hipStreamSynchronize(s));
-            // mixed precision: recompute the FP32 residual from x in FP64
-            // once it has fallen 100x, keeping the search direction
-            if (mixed(c) && c->sc_host->active && c->sc_host->rr < 1e-4 * rr_ref) {
-                rc = enqueue_init(c, 1, FIN_RELIABLE);
-                if (rc) return rc;
-                SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-                SG_HIP(hipStreamSynchronize(s));
-                rr_ref = c->sc_host->rr;
-            }
+            const int done = c->sc_host->iters - k0;
+            if (done > 0 && c->sc_host->rr > 0.0 && c->sc_host->rr < rr0)
+                rate = log(c->sc_host->rr / rr0) / done;
         }
         // residual replacement at convergence: the recursive residual drifts
         // from b - A x by rounding; recompute it in FP64 from x and go on until
         // the true residual meets tol (FP64: restart CG from it; mixed
-        // precision: keep the direction, as above)
+        // precision: keep the direction)
         if (c->sc_host->active || (c->sc_host->guard & 1) || replacements >= 4 || it >= max_iter) break;
         rc = enqueue_init(c, 1, mixed(c) ? FIN_RELIABLE : FIN_REPLACE);
         if (rc) return rc;
