@@ -31,6 +31,7 @@ module simcogpu
     integer(c_int) :: mg_pre, mg_post
     integer(c_int) :: warm_start
     integer(c_int) :: check_every
+    integer(c_int) :: mg_coarse_krylov
   end type sg_params
 
   type, bind(C), public :: sg_solve_info

@@ -52,6 +52,9 @@ typedef struct sg_params {
                         2: learned warm start + deflation vector (sg_set_learned) */
     int check_every; /* PCG iterations between host convergence checks; 0 (default): set from
                         the observed convergence rate */
+    int mg_coarse_krylov; /* flexible-CG iterations on the first level of at most 1024 points,
+                             each preconditioned by the V-cycle below it (0: one V-cycle);
+                             default 4 */
 } sg_params;
 
 typedef struct sg_solve_info {

@@ -72,6 +72,10 @@ static int check_params(const sg_params* p) {
         set_error("tol must be positive");
         return SG_ERR_ARG;
     }
+    if (p->mg_coarse_krylov < 0 || p->mg_coarse_krylov > 64) {
+        set_error("mg_coarse_krylov must be in [0, 64]");
+        return SG_ERR_ARG;
+    }
     return SG_OK;
 }
 
@@ -105,6 +109,7 @@ int sg_params_default(sg_params* p) {
     p->mg_post = 1;
     p->warm_start = 0;
     p->check_every = 0;
+    p->mg_coarse_krylov = 4;
     return SG_OK;
 }
 

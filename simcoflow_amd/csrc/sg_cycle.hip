@@ -564,6 +564,8 @@ struct TailArgs {
     TailLevel<T> L[MAX_TAIL];
     const T* inv;  // dense inverse of the coarsest operator
     const int* on; // the PCG's convergence flag (the cycle runs while *on), or null
+    int krylov;    // inner flexible-CG iterations on the first tail level (0: one V-cycle)
+    int pbase;     // LDS offset of the inner iteration's direction vector
 };
 
 template <typename T>
@@ -616,14 +618,10 @@ __device__ void tail_colour(T* lds, const TailLevel<T>& L, int col, bool zero_st
     __syncthreads();
 }
 
+// stage operators, weights and the first level's b into LDS
 template <typename T>
-__global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_elems) {
-    if (a.on && !*a.on) return;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* lds = reinterpret_cast<T*>(smem_raw);
+__device__ void tail_stage(T* lds, const TailArgs<T>& a) {
     const int tid = threadIdx.x;
-    for (int k = tid; k < lds_elems; k += TAIL_THREADS) lds[k] = T(0);
-    __syncthreads();
     // stage operators, weights and the first level's b
     for (int l = 0; l < a.n; ++l) {
         const TailLevel<T>& L = a.L[l];
@@ -650,7 +648,12 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
             }
         }
     }
-    __syncthreads();
+}
+
+// one V-cycle over the tail levels: level 0's TA_B -> level 0's TA_X
+template <typename T>
+__device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
+    const int tid = threadIdx.x;
     const int last = a.n - 1;
     for (int l = 0; l < last; ++l) {
         const TailLevel<T>& L = a.L[l];
@@ -731,14 +734,108 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
         tail_colour<T>(lds, L, 1, false);
         tail_colour<T>(lds, L, 0, false);
     }
-    {
-        const TailLevel<T>& L = a.L[0];
-        const T* x = tarr(lds, L, TA_X);
-        for (int k = tid; k < L.d.ni * L.d.nj; k += TAIL_THREADS) {
-            const int i = k % L.d.ni, j = k / L.d.ni;
-            L.x[L.d.off(i, j)] = x[L.cd.off(i, j)];
-        }
+}
+
+// Sum of v over the workgroup, returned to every thread (part: 16 slots of
+// LDS per value, not reused before a barrier)
+template <int NV>
+__device__ inline void tail_allreduce(double (&v)[NV], double (*part)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) v[k] += __shfl_xor(v[k], m, 64);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) part[threadIdx.x >> 6][k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < TAIL_THREADS / 64; ++w) s += part[w][k];
+        v[k] = s;
     }
+}
+
+// The tail: one V-cycle over levels lt..nlev-1, or -- with a.krylov > 0 --
+// that many iterations of flexible CG on level lt preconditioned by the
+// V-cycle (x0 = 0; each direction A-orthogonal to the previous one).  The
+// coarse levels of a bubbly flow carry slow modes (the pressure of each
+// bubble against the liquid around it) that one V-cycle resolves poorly;
+// the inner iterations remove them at the cost of a few V-cycles of the
+// smallest levels, all inside one workgroup.
+template <typename T>
+__global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_elems) {
+    if (a.on && !*a.on) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    const int tid = threadIdx.x;
+    for (int k = tid; k < lds_elems; k += TAIL_THREADS) lds[k] = T(0);
+    __syncthreads();
+    tail_stage<T>(lds, a);
+    __syncthreads();
+    const TailLevel<T>& L = a.L[0];
+    const Dims& D = L.cd;
+    const int n = D.ni * D.nj;
+    if (a.krylov <= 0) {
+        tail_vcycle<T>(lds, a);
+        const T* x = tarr(lds, L, TA_X);
+        for (int k = tid; k < n; k += TAIL_THREADS) {
+            const int i = k % D.ni, j = k / D.ni;
+            L.x[L.d.off(i, j)] = x[D.off(i, j)];
+        }
+        return;
+    }
+    // inner flexible CG; level lt has at most TAIL_THREADS points, one per thread
+    __shared__ double part_b[TAIL_THREADS / 64][2];
+    __shared__ double part_a[TAIL_THREADS / 64][2];
+    const size_t ld = D.ld;
+    const bool own = tid < n;
+    const size_t p = own ? D.off(tid % D.ni, tid / D.ni) : 0;
+    T* res = tarr(lds, L, TA_B);  // the V-cycle's right-hand side
+    const T* z = tarr(lds, L, TA_X);
+    T* pv = lds + a.pbase;
+    const T* d = tarr(lds, L, TA_D);
+    const T* e = tarr(lds, L, TA_E);
+    const T* nn = tarr(lds, L, TA_N);
+    const T* ne = tarr(lds, L, TA_NE);
+    const T* nw = tarr(lds, L, TA_NW);
+    T xacc = T(0), qv = T(0);
+    double pq_prev = 0.0, rr0 = 0.0;
+    for (int it = 0; it < a.krylov; ++it) {
+        tail_vcycle<T>(lds, a);  // ends with a barrier
+        const T zp = own ? z[p] : T(0);
+        const double rp = own ? (double)res[p] : 0.0;
+        double v[2] = {(double)zp * (double)qv, rp * rp};
+        tail_allreduce<2>(v, part_b);
+        // every thread holds the same sums, so the exits below are uniform:
+        // stop once the inner residual has fallen 1e4 (further iterations
+        // would work on rounding noise, in FP32 above all)
+        if (it == 0) rr0 = v[1];
+        else if (!(v[1] > 1e-8 * rr0)) break;
+        const T beta = (it > 0 && pq_prev > 0.0) ? (T)(-v[0] / pq_prev) : T(0);
+        if (own) pv[p] = zp + beta * pv[p];
+        __syncthreads();
+        T pp = T(0);
+        if (own) {
+            pp = pv[p];
+            const T s = e[p - 1] * pv[p - 1] + e[p] * pv[p + 1] + nn[p - ld] * pv[p - ld] + nn[p] * pv[p + ld] +
+                        ne[p] * pv[p + ld + 1] + ne[p - ld - 1] * pv[p - ld - 1] + nw[p] * pv[p + ld - 1] +
+                        nw[p - ld + 1] * pv[p - ld + 1];
+            qv = d[p] * pp - s;
+        }
+        double v2[2] = {(double)pp * rp, (double)pp * (double)qv};
+        tail_allreduce<2>(v2, part_a);
+        const double al = v2[0] / v2[1];
+        if (!(v2[1] > 0.0) || !(fabs(al) < 1e30)) break;
+        const T alpha = (T)al;
+        pq_prev = v2[1];
+        if (own) {
+            xacc += alpha * pp;
+            res[p] -= alpha * qv;
+        }
+        __syncthreads();
+    }
+    if (own) L.x[L.d.off(tid % D.ni, tid / D.ni)] = xacc;
 }
 
 // coarsest-only hierarchy (grids of at most COARSEST_MAX cells)
@@ -839,6 +936,15 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done, const 
             t.wbase = off;
             off += 9 * (int)compact(c->lev[lt + k + 1].d).alloc_elems();
         }
+    }
+    ta.krylov = c->prm.mg_coarse_krylov;
+    ta.pbase = off;
+    if (ta.krylov > 0) {
+        if (c->lev[lt].d.ni * c->lev[lt].d.nj > TAIL_THREADS) {
+            set_error("multigrid: the first tail level exceeds one point per thread");
+            return SG_ERR_STATE;
+        }
+        off += (int)compact(c->lev[lt].d).alloc_elems();
     }
     const size_t lds_bytes = (size_t)off * sizeof(T);
     if (lds_bytes > TAIL_LDS_MAX) {

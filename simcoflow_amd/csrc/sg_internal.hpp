@@ -222,7 +222,7 @@ struct Ctx {
     // [precision][0: two iterations from parity 0, 1: one from parity 0,
     // 2: one from parity 1]
     hipGraphExec_t iter_graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-    int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0}, graph_defl[2] = {0, 0};
+    int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0}, graph_defl[2] = {0, 0}, graph_kry[2] = {0, 0};
     NnState nn;
     double* step_dev = nullptr;   // device scalars of the time-step control (dt)
     hipEvent_t ev_nn0 = nullptr, ev_nn1 = nullptr;

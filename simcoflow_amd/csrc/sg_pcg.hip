@@ -383,7 +383,8 @@ int enqueue_init(Ctx* c, int warm, int mode) {
 int iteration_graph(Ctx* c, int variant, hipGraphExec_t* out) {
     const int k = c->prm.precision == SG_PREC_MIXED ? 1 : 0;
     const int defl = deflating(c) ? 1 : 0;
-    if (c->graph_pre[k] != c->prm.mg_pre || c->graph_post[k] != c->prm.mg_post || c->graph_defl[k] != defl) {
+    if (c->graph_pre[k] != c->prm.mg_pre || c->graph_post[k] != c->prm.mg_post || c->graph_defl[k] != defl ||
+        c->graph_kry[k] != c->prm.mg_coarse_krylov) {
         for (hipGraphExec_t& g : c->iter_graph[k]) {
             if (g) (void)hipGraphExecDestroy(g);
             g = nullptr;
@@ -391,6 +392,7 @@ int iteration_graph(Ctx* c, int variant, hipGraphExec_t* out) {
         c->graph_pre[k] = c->prm.mg_pre;
         c->graph_post[k] = c->prm.mg_post;
         c->graph_defl[k] = defl;
+        c->graph_kry[k] = c->prm.mg_coarse_krylov;
     }
     if (c->iter_graph[k][variant]) {
         *out = c->iter_graph[k][variant];
@@ -467,8 +469,9 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
         if (rc) return rc;
     }
     SG_HIP(hipEventRecord(c->ev1, s));
-    // flexible beta in mixed precision
-    c->sc_host->flexible = c->prm.precision == SG_PREC_MIXED ? 1 : 0;
+    // flexible beta in mixed precision and with inner iterations on the
+    // coarse levels (both make the preconditioner vary slightly)
+    c->sc_host->flexible = (c->prm.precision == SG_PREC_MIXED || c->prm.mg_coarse_krylov > 0) ? 1 : 0;
     SG_HIP(hipMemcpyAsync(&c->sc->flexible, &c->sc_host->flexible, sizeof(int), hipMemcpyHostToDevice, s));
     int rc;
     if (c->prm.warm_start == 2) {
@@ -529,11 +532,12 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
                 rate = log(c->sc_host->rr / rr0) / done;
         }
         // residual replacement at convergence: the recursive residual drifts
-        // from b - A x by rounding; recompute it in FP64 from x and go on until
-        // the true residual meets tol (FP64: restart CG from it; mixed
-        // precision: keep the direction)
+        // from b - A x by rounding; recompute it in FP64 from x and restart CG
+        // from it until the true residual meets tol.  (Keeping the direction
+        // here, as the recomputations during the solve do, let the recursion
+        // run away once the preconditioner holds inner iterations.)
         if (c->sc_host->active || (c->sc_host->guard & 1) || replacements >= 4 || it >= max_iter) break;
-        rc = enqueue_init(c, 1, mixed(c) ? FIN_RELIABLE : FIN_REPLACE);
+        rc = enqueue_init(c, 1, FIN_REPLACE);
         if (rc) return rc;
         SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
         SG_HIP(hipStreamSynchronize(s));
