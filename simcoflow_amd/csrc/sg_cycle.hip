@@ -552,6 +552,7 @@ struct TailLevel {
     Dims cd;  // LDS layout
     int base;  // LDS offset (elements) of this level's TA_COUNT arrays
     int wbase; // LDS offset of its 9 prolongation-weight arrays (next level's cd)
+    int sh;    // thread k of a pass takes cell (k & (2^sh - 1), k >> sh), 2^sh >= ni
     OpV<T> A;
     Prol<T> P;  // to the next level
     T* x;       // global x of the level (written back for the first tail level)
@@ -568,52 +569,46 @@ struct TailArgs {
     int pbase;     // LDS offset of the inner iteration's direction vector
 };
 
+// offset of cell (i,j) in a level's LDS arrays (compact layout, 32-bit)
+__device__ inline int toff(const Dims& D, int i, int j) { return (j + 2) * D.ld + i; }
+
 template <typename T>
 __device__ inline T* tarr(T* lds, const TailLevel<T>& L, int k) {
     return lds + L.base + (size_t)k * L.cd.alloc_elems();
 }
 
+// One colour pass of the smoother, double-buffered: the cells of colour
+// `col` get (b + sum of couplings * src) / d, the others keep src; every
+// cell is written to dst, so one barrier ends the pass (same-colour diagonal
+// couplings read src, i.e. Jacobi-wise).  zero_start: src = 0.
 template <typename T>
-__device__ void tail_colour(T* lds, const TailLevel<T>& L, int col, bool zero_start) {
-    constexpr int NP = (TAIL_MAX + TAIL_THREADS - 1) / TAIL_THREADS;
+__device__ void tail_pass(T* lds, const TailLevel<T>& L, const T* src, T* dst, int col, bool zero_start) {
     const Dims& D = L.cd;
-    const int n = D.ni * D.nj;
-    const size_t ld = D.ld;
+    const int ld = D.ld;
     const T* d = tarr(lds, L, TA_D);
     const T* e = tarr(lds, L, TA_E);
     const T* nn = tarr(lds, L, TA_N);
     const T* ne = tarr(lds, L, TA_NE);
     const T* nw = tarr(lds, L, TA_NW);
-    T* x = tarr(lds, L, TA_X);
     const T* b = tarr(lds, L, TA_B);
-    T nv[NP];
-#pragma unroll
-    for (int m = 0; m < NP; ++m) {
-        const int k = threadIdx.x + m * TAIL_THREADS;
-        nv[m] = T(0);
-        if (k >= n) continue;
-        const int i = k % D.ni, j = k / D.ni;
-        const size_t p = D.off(i, j);
-        T v = zero_start ? T(0) : x[p];
+    const int mask = (1 << L.sh) - 1;
+    for (int k = threadIdx.x; k < (D.nj << L.sh); k += TAIL_THREADS) {
+        const int i = k & mask, j = k >> L.sh;
+        if (i >= D.ni) continue;
+        const int p = toff(D, i, j);
+        T v = zero_start ? T(0) : src[p];
         if (((i + j) & 1) == col) {
             const T dd = d[p];
             if (zero_start) {
                 v = dd > T(0) ? b[p] / dd : T(0);
             } else {
-                const T s = e[p - 1] * x[p - 1] + e[p] * x[p + 1] + nn[p - ld] * x[p - ld] + nn[p] * x[p + ld] +
-                            ne[p] * x[p + ld + 1] + ne[p - ld - 1] * x[p - ld - 1] + nw[p] * x[p + ld - 1] +
-                            nw[p - ld + 1] * x[p - ld + 1];
+                const T s = e[p - 1] * src[p - 1] + e[p] * src[p + 1] + nn[p - ld] * src[p - ld] +
+                            nn[p] * src[p + ld] + ne[p] * src[p + ld + 1] + ne[p - ld - 1] * src[p - ld - 1] +
+                            nw[p] * src[p + ld - 1] + nw[p - ld + 1] * src[p - ld + 1];
                 v = dd > T(0) ? (b[p] + s) / dd : T(0);
             }
         }
-        nv[m] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < NP; ++m) {
-        const int k = threadIdx.x + m * TAIL_THREADS;
-        if (k >= n) continue;
-        x[D.off(k % D.ni, k / D.ni)] = nv[m];
+        dst[p] = v;
     }
     __syncthreads();
 }
@@ -650,7 +645,9 @@ __device__ void tail_stage(T* lds, const TailArgs<T>& a) {
     }
 }
 
-// one V-cycle over the tail levels: level 0's TA_B -> level 0's TA_X
+// One V-cycle over the tail levels: level 0's TA_B -> level 0's TA_R.  On
+// every level the smoothed x ends in TA_R; TA_X holds the first pass and then
+// the residual that is restricted.
 template <typename T>
 __device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
     const int tid = threadIdx.x;
@@ -659,19 +656,24 @@ __device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
         const TailLevel<T>& L = a.L[l];
         const TailLevel<T>& Cl = a.L[l + 1];
         const Dims& D = L.cd;
-        const size_t ld = D.ld;
-        tail_colour<T>(lds, L, 0, true);
-        tail_colour<T>(lds, L, 1, false);
+        const int ld = D.ld;
+        T* xa = tarr(lds, L, TA_X);
+        T* xb = tarr(lds, L, TA_R);
+        tail_pass<T>(lds, L, nullptr, xa, 0, true);
+        tail_pass<T>(lds, L, xa, xb, 1, false);
         const T* d = tarr(lds, L, TA_D);
         const T* e = tarr(lds, L, TA_E);
         const T* nn = tarr(lds, L, TA_N);
         const T* ne = tarr(lds, L, TA_NE);
         const T* nw = tarr(lds, L, TA_NW);
-        const T* x = tarr(lds, L, TA_X);
         const T* b = tarr(lds, L, TA_B);
-        T* r = tarr(lds, L, TA_R);
-        for (int k = tid; k < D.ni * D.nj; k += TAIL_THREADS) {
-            const size_t p = D.off(k % D.ni, k / D.ni);
+        const T* x = xb;
+        T* r = xa;
+        const int mask = (1 << L.sh) - 1;
+        for (int k = tid; k < (D.nj << L.sh); k += TAIL_THREADS) {
+            const int i = k & mask;
+            if (i >= D.ni) continue;
+            const int p = toff(D, i, k >> L.sh);
             const T s = e[p - 1] * x[p - 1] + e[p] * x[p + 1] + nn[p - ld] * x[p - ld] + nn[p] * x[p + ld] +
                         ne[p] * x[p + ld + 1] + ne[p - ld - 1] * x[p - ld - 1] + nw[p] * x[p + ld - 1] +
                         nw[p - ld + 1] * x[p - ld + 1];
@@ -679,33 +681,38 @@ __device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
         }
         __syncthreads();
         T* bc = tarr(lds, Cl, TA_B);
-        const size_t wstride = Cl.cd.alloc_elems();
-        T* const w0 = lds + L.wbase;
-        for (int k = tid; k < Cl.cd.ni * Cl.cd.nj; k += TAIL_THREADS) {
-            const int I = k % Cl.cd.ni, J = k / Cl.cd.ni;
-            const Dims& Cd = Cl.cd;
-            auto W = [&](int m, int ii, int jj) { return w0[(size_t)m * wstride + Cd.off(ii, jj)]; };
-            auto R = [&](int aa, int bb) { return r[D.off(2 * I + aa, 2 * J + bb)]; };
-            T s = W(W_INJ, I, J) * R(0, 0);
-            s += W(W_XW, I, J) * R(1, 0) + W(W_XE, I - 1, J) * R(-1, 0);
-            s += W(W_YS, I, J) * R(0, 1) + W(W_YN, I, J - 1) * R(0, -1);
-            s += W(W_SW, I, J) * R(1, 1) + W(W_SE, I - 1, J) * R(-1, 1) + W(W_NW, I, J - 1) * R(1, -1) +
-                 W(W_NE, I - 1, J - 1) * R(-1, -1);
-            bc[Cd.off(I, J)] = s;
+        const Dims& Cd = Cl.cd;
+        const int ws = (int)Cd.alloc_elems(), cld = Cd.ld;
+        const T* w0 = lds + L.wbase;
+        const int cmask = (1 << Cl.sh) - 1;
+        for (int k = tid; k < (Cd.nj << Cl.sh); k += TAIL_THREADS) {
+            const int I = k & cmask, J = k >> Cl.sh;
+            if (I >= Cd.ni) continue;
+            const int c = toff(Cd, I, J), f = toff(D, 2 * I, 2 * J);
+            const T* w = w0 + c;
+            T s = w[W_INJ * ws] * r[f];
+            s += w[W_XW * ws] * r[f + 1] + w[W_XE * ws - 1] * r[f - 1];
+            s += w[W_YS * ws] * r[f + ld] + w[W_YN * ws - cld] * r[f - ld];
+            s += w[W_SW * ws] * r[f + ld + 1] + w[W_SE * ws - 1] * r[f + ld - 1] + w[W_NW * ws - cld] * r[f - ld + 1] +
+                 w[W_NE * ws - cld - 1] * r[f - ld - 1];
+            bc[c] = s;
         }
         __syncthreads();
     }
     {
+        // dense solve on the coarsest level: four threads per row
         const TailLevel<T>& L = a.L[last];
         const Dims& D = L.cd;
         const int n = D.ni * D.nj;
         const T* b = tarr(lds, L, TA_B);
-        T* x = tarr(lds, L, TA_X);
-        for (int r = tid; r < n; r += TAIL_THREADS) {
-            T s = T(0);
-            for (int q = 0; q < n; ++q) s += a.inv[(size_t)r * n + q] * b[D.off(q % D.ni, q / D.ni)];
-            x[D.off(r % D.ni, r / D.ni)] = s;
-        }
+        T* x = tarr(lds, L, TA_R);
+        const int row = tid >> 2, quarter = tid & 3;
+        T s = T(0);
+        if (row < n)
+            for (int q = quarter; q < n; q += 4) s += a.inv[(size_t)row * n + q] * b[D.off(q % D.ni, q / D.ni)];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (row < n && quarter == 0) x[D.off(row % D.ni, row / D.ni)] = s;
         __syncthreads();
     }
     for (int l = last - 1; l >= 0; --l) {
@@ -713,26 +720,29 @@ __device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
         const TailLevel<T>& Cl = a.L[l + 1];
         const Dims& D = L.cd;
         const Dims& Cd = Cl.cd;
-        const size_t wstride = Cd.alloc_elems();
+        const int ws = (int)Cd.alloc_elems(), cld = Cd.ld;
         const T* w0 = lds + L.wbase;
-        const T* ec = tarr(lds, Cl, TA_X);
-        T* x = tarr(lds, L, TA_X);
-        for (int k = tid; k < D.ni * D.nj; k += TAIL_THREADS) {
-            const int i = k % D.ni, j = k / D.ni;
-            const int I = i >> 1, J = j >> 1;
-            const size_t c = Cd.off(I, J);
-            auto W = [&](int m) { return w0[(size_t)m * wstride + c]; };
+        const T* ec = tarr(lds, Cl, TA_R);
+        T* xa = tarr(lds, L, TA_X);
+        T* xb = tarr(lds, L, TA_R);
+        const int mask = (1 << L.sh) - 1;
+        for (int k = tid; k < (D.nj << L.sh); k += TAIL_THREADS) {
+            const int i = k & mask, j = k >> L.sh;
+            if (i >= D.ni) continue;
+            const int c = toff(Cd, i >> 1, j >> 1);
+            const T* w = w0 + c;
             const int pi = i & 1, pj = j & 1;
             T v;
-            if (!pi && !pj) v = W(W_INJ) * ec[c];
-            else if (pi && !pj) v = W(W_XW) * ec[c] + W(W_XE) * ec[c + 1];
-            else if (!pi && pj) v = W(W_YS) * ec[c] + W(W_YN) * ec[c + Cd.ld];
-            else v = W(W_SW) * ec[c] + W(W_SE) * ec[c + 1] + W(W_NW) * ec[c + Cd.ld] + W(W_NE) * ec[c + Cd.ld + 1];
-            x[D.off(i, j)] += v;
+            if (!pi && !pj) v = w[W_INJ * ws] * ec[c];
+            else if (pi && !pj) v = w[W_XW * ws] * ec[c] + w[W_XE * ws] * ec[c + 1];
+            else if (!pi && pj) v = w[W_YS * ws] * ec[c] + w[W_YN * ws] * ec[c + cld];
+            else v = w[W_SW * ws] * ec[c] + w[W_SE * ws] * ec[c + 1] + w[W_NW * ws] * ec[c + cld] +
+                     w[W_NE * ws] * ec[c + cld + 1];
+            xb[toff(D, i, j)] += v;
         }
         __syncthreads();
-        tail_colour<T>(lds, L, 1, false);
-        tail_colour<T>(lds, L, 0, false);
+        tail_pass<T>(lds, L, xb, xa, 1, false);
+        tail_pass<T>(lds, L, xa, xb, 0, false);
     }
 }
 
@@ -778,7 +788,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
     const int n = D.ni * D.nj;
     if (a.krylov <= 0) {
         tail_vcycle<T>(lds, a);
-        const T* x = tarr(lds, L, TA_X);
+        const T* x = tarr(lds, L, TA_R);
         for (int k = tid; k < n; k += TAIL_THREADS) {
             const int i = k % D.ni, j = k / D.ni;
             L.x[L.d.off(i, j)] = x[D.off(i, j)];
@@ -792,7 +802,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
     const bool own = tid < n;
     const size_t p = own ? D.off(tid % D.ni, tid / D.ni) : 0;
     T* res = tarr(lds, L, TA_B);  // the V-cycle's right-hand side
-    const T* z = tarr(lds, L, TA_X);
+    const T* z = tarr(lds, L, TA_R);
     T* pv = lds + a.pbase;
     const T* d = tarr(lds, L, TA_D);
     const T* e = tarr(lds, L, TA_E);
@@ -926,6 +936,8 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done, const 
         TailLevel<T>& t = ta.L[k];
         t.d = L.d;
         t.cd = compact(L.d);
+        t.sh = 0;
+        while ((1 << t.sh) < L.d.ni) ++t.sh;
         t.base = off;
         off += TA_COUNT * (int)t.cd.alloc_elems();
         t.A = opv(Acc<T>::op(L));
