@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="skip training and the learned path")
     ap.add_argument("--no-full-step", action="store_true", help="skip the per-part step timing")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--nn-width", type=int, default=256, help="widest multigrid level the net runs on")
     return ap.parse_args()
 
 
@@ -105,7 +106,7 @@ def kernel_roofline(pj, n, precision):
     return out
 
 
-def run_path(case, fields, precision, steps, warmup, tol, learned=None):
+def run_path(case, fields, precision, steps, warmup, tol, learned=None, width=256):
     from simcoflow_amd import learned as Lr
     from simcoflow_amd.projection import Projection, params_for
     u, v, DpU, DpV = fields
@@ -119,7 +120,7 @@ def run_path(case, fields, precision, steps, warmup, tol, learned=None):
     if learned:
         w, K = learned
         # deflation needs FP64 directions; the mixed solve takes the warm start
-        Lr.enable(pj, w, Lr.pick_level(pj), K, deflate=(precision == 0))
+        Lr.enable(pj, w, Lr.pick_level(pj, width), K, deflate=(precision == 0))
     infos = []
 
     def step():
@@ -155,7 +156,7 @@ def run_path(case, fields, precision, steps, warmup, tol, learned=None):
     }
 
 
-def train_warm_start(seed):
+def train_warm_start(seed, width):
     """The learned warm start, trained in this run on the GPU: pairs from the
     plain FP64 solver on four seeded 512x512 swarms, 48 right-hand sides
     each, then Adam with the library's kernels."""
@@ -163,7 +164,7 @@ def train_warm_start(seed):
     from simcoflow_amd import learned as Lr
     t0 = time.perf_counter()
     X, T, M, level = Lr.collect_pairs(lambda s: C.case_swarm(512, 512, seed=s), seeds=[seed + 101 + k for k in range(4)],
-                                      per_case=48)
+                                      per_case=48, max_width=width)
     t1 = time.perf_counter()
     w, K, hist = Lr.train(X, T, M, epochs=10, batch=16, lr=3e-3)
     t2 = time.perf_counter()
@@ -172,7 +173,7 @@ def train_warm_start(seed):
                   "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]}
 
 
-def stepping(case, stags, precision, steps, warmup, learned=None):
+def stepping(case, stags, precision, steps, warmup, learned=None, width=256):
     """K timed full time steps after W warm-up steps (TimeStepper), with the
     Poisson device time of each step from its solve report."""
     from simcoflow_amd.solver import TimeStepper
@@ -181,7 +182,8 @@ def stepping(case, stags, precision, steps, warmup, learned=None):
         w, K = learned
         # deflation needs FP64 directions; the mixed solve takes the warm start
         lrn = (w, None, K, precision == 0)
-    st = TimeStepper(case, precision=precision, tol=1e-8, max_iter=200, learned=lrn, stags=stags)
+    st = TimeStepper(case, precision=precision, tol=1e-8, max_iter=200, learned=lrn, stags=stags,
+                     learned_width=width)
     try:
         for _ in range(warmup):
             st.step()
@@ -313,18 +315,18 @@ def main():
     w = K = train = None
     if not a.no_learned:
         log("training the warm-start net")
-        w, K, train = train_warm_start(a.seed)
+        w, K, train = train_warm_start(a.seed, a.nn_width)
     # full time steps, one per solver path
     paths = [("fp64", 0, None), ("mixed", 1, None)] + ([("learned_mixed", 1, (w, K))] if w is not None else [])
     full = {}
     for name, prec, lrn in paths:
         log(f"full time steps, {name}")
-        full[name] = stepping(case, stags, prec, a.steps, a.warmup, learned=lrn)
+        full[name] = stepping(case, stags, prec, a.steps, a.warmup, learned=lrn, width=a.nn_width)
     # the projection alone on one fixed set of predictor fields
     proj = {}
     for name, prec, lrn in paths:
         log(f"projection step on fixed fields, {name}")
-        proj[name] = run_path(case, fields, prec, a.steps, a.warmup, tol, learned=lrn)
+        proj[name] = run_path(case, fields, prec, a.steps, a.warmup, tol, learned=lrn, width=a.nn_width)
     head_name = paths[-1][0]
     head = full[head_name]
     value = n / (head["ms_per_step"] * 1e-3) / 1e6

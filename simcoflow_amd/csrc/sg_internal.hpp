@@ -186,6 +186,7 @@ struct NnState {
     double* w = nullptr;       // fine deflation vector (padded FP64)
     double* aw = nullptr;      // A w
     float ms = 0.f;            // device time of the last warm start
+    hipGraphExec_t graph = nullptr;  // the warm start's kernels, captured once
 };
 
 struct Ctx {

@@ -201,6 +201,10 @@ int features(Ctx* c, int level, bf16* X, float* t, float* mask) {
 }  // namespace
 
 void learned_release(Ctx* c) {
+    if (c->nn.graph) {
+        (void)hipGraphExecDestroy(c->nn.graph);
+        c->nn.graph = nullptr;
+    }
     if (c->nn.net) {
         nn::destroy(c->nn.net);
         delete c->nn.net;
@@ -269,10 +273,12 @@ int learned_features(Ctx* c, int level, void* X, float* t, float* mask, double* 
     return SG_OK;
 }
 
-int learned_warm_start(Ctx* c) {
+namespace {
+
+// x0 = (w,b)/(w,Aw) w with w the net's prediction carried to the fine grid
+int enqueue_warm_start(Ctx* c) {
     hipStream_t s = c->stream;
     const int L = c->nn.level;
-    SG_HIP(hipEventRecord(c->ev_nn0, s));
     int rc = features(c, L, (bf16*)c->nn.X, nullptr, nullptr);
     if (rc) return rc;
     rc = nn::forward(c->nn.net, (const bf16*)c->nn.X, 1);
@@ -297,6 +303,35 @@ int learned_warm_start(Ctx* c) {
     hipLaunchKernelGGL(k_galerkin_fin, dim3(1), dim3(NTR), 0, s, (const double*)part, c->red.max_blocks, nb, c->sc);
     hipLaunchKernelGGL(k_start, g, block2d(), 0, s, D, (const double*)c->nn.w, (const PcgScalars*)c->sc, c->x);
     SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+}  // namespace
+
+// The warm start's ~45 kernels (restriction, features, the net, the
+// prolongation, the Galerkin scaling) run as one graph, captured at the first
+// call after sg_set_learned: every buffer it touches lives as long as the net.
+int learned_warm_start(Ctx* c) {
+    hipStream_t s = c->stream;
+    if (!c->nn.graph) {
+        hipGraph_t graph = nullptr;
+        SG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_warm_start(c);
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        const hipError_t e2 = hipGraphInstantiate(&c->nn.graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e2 != hipSuccess) {
+            c->nn.graph = nullptr;
+            return hip_fail(e2, "hipGraphInstantiate");
+        }
+    }
+    SG_HIP(hipEventRecord(c->ev_nn0, s));
+    SG_HIP(hipGraphLaunch(c->nn.graph, s));
     SG_HIP(hipEventRecord(c->ev_nn1, s));
     return SG_OK;
 }

@@ -177,10 +177,11 @@ def smooth_velocities(torch, shape, g_dt, gen, modes=8, amp=0.2):
     return u.contiguous(), v.contiguous()
 
 
-def collect_pairs(make_case, seeds, per_case, level=None, tol=1e-9, progress=None, seed=1234):
+def collect_pairs(make_case, seeds, per_case, level=None, tol=1e-9, progress=None, seed=1234, max_width=256):
     """(x, t, mask) training pairs from the plain FP64 solver: for each case
     seed, `per_case` right-hand sides from seeded smooth predictor
-    velocities.  Returns torch tensors on the GPU and the level."""
+    velocities, on `level` (default: the first level no wider than
+    max_width).  Returns torch tensors on the GPU and the level."""
     import torch
     from . import cases as C
     from .projection import Projection, params_for
@@ -210,7 +211,7 @@ def collect_pairs(make_case, seeds, per_case, level=None, tol=1e-9, progress=Non
                 pj.solve()
                 if X is None:
                     if level is None:
-                        level = pick_level(pj)
+                        level = pick_level(pj, max_width)
                     H, W = level_dims(pj, level)
                     X = torch.zeros((n, H + 2, W + 2, 8), device="cuda", dtype=torch.bfloat16)
                     T = torch.zeros((n, H, W), device="cuda", dtype=torch.float32)

@@ -37,10 +37,11 @@ def physics(fluids):
 
 class TimeStepper:
     def __init__(self, case, precision=_lib.SG_PREC_FP64, tol=None, cfl=0.25, vint=0.0, uref=1.0, max_iter=500,
-                 learned=None, stags=None):
+                 learned=None, stags=None, learned_width=256):
         """learned: (weights, level, out_scale, deflate) of a trained warm
-        start, level None for the first multigrid level no wider than 256;
-        stags: the (U, V) cases.StagGeometry if already computed."""
+        start, level None for the first multigrid level no wider than
+        learned_width; stags: the (U, V) cases.StagGeometry if already
+        computed."""
         self.case = case
         g = case.grid
         self.cfl, self.vint, self.uref = cfl, vint, uref
@@ -66,7 +67,7 @@ class TimeStepper:
         if learned:
             from . import learned as L
             w, level, scale, deflate = learned
-            L.enable(pj, w, L.pick_level(pj) if level is None else level, scale, deflate)
+            L.enable(pj, w, L.pick_level(pj, learned_width) if level is None else level, scale, deflate)
         self.itt = 0
         self.time = 0.0
         self.last = {}
