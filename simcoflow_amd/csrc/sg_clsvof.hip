@@ -3,13 +3,16 @@
 // One call advances vof and phi over dt in nsub sub-steps (Coupled_LS_VOF,
 // Clsvof_Mod.f90:730-959).  Each sub-step runs these launches:
 //
-//   k_recon      PLIC reconstruction (Interface_Reconstruct :1250-1321)
+//   k_recon      PLIC reconstruction (Interface_Reconstruct :1250-1321):
+//                classify the cells and list the interface ones,
+//   k_recon_ifc  then their normals and plane constants, packed one per lane
 //   k_dis_fix    cells whose plane constant the reference leaves stale
+//   k_flux_pre   PLIC face fluxes of the listed cells, packed one per lane
 //   k_sweep      first sweep, implicit divergence (:798-823 / :862-888)
 //   k_bc         boundary copy (Boundary_Condition :961-973)
-//   k_recon, k_dis_fix
+//   k_recon, k_recon_ifc, k_dis_fix, k_flux_pre
 //   k_sweep      second sweep, explicit divergence (:834-858 / :898-929)
-//   k_bc, k_recon (+ interface bit mask and source list), k_dis_fix
+//   k_bc, k_recon (+ interface bit mask and source list), k_recon_ifc, k_dis_fix
 //   k_rd_reset, k_rd_scatter, k_redistance: level set from the PLIC
 //                segments (Redistance :1525-1708)
 //
@@ -593,6 +596,29 @@ __device__ __forceinline__ void append(bool pred, unsigned* count, int* list, in
     if (pred) list[base + (unsigned)__popcll(b & ((1ull << lane) - 1ull))] = value;
 }
 
+// Block-aggregated append (one atomic per workgroup): every thread of the
+// workgroup must call it; NW = wavefronts per workgroup.  Lists grow in
+// arbitrary order -- their users treat every entry on its own.
+template <int NW>
+__device__ __forceinline__ void block_append(bool pred, unsigned* count, int* list, int value) {
+    __shared__ unsigned wcnt[NW], wbase[NW + 1];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    const int lane = t & 63, w = t >> 6;
+    const unsigned long long b = __ballot(pred);
+    if (lane == 0) wcnt[w] = (unsigned)__popcll(b);
+    __syncthreads();
+    if (t == 0) {
+        unsigned s = 0;
+        for (int k = 0; k < NW; ++k) {
+            wbase[k] = s;
+            s += wcnt[k];
+        }
+        wbase[NW] = s ? atomicAdd(count, s) : 0u;
+    }
+    __syncthreads();
+    if (pred) list[wbase[NW] + wbase[w] + (unsigned)__popcll(b & ((1ull << lane) - 1ull))] = value;
+}
+
 // Interface_Reconstruct.  Interface cells get their normal and plane
 // constant; three-phase cells whose Ridders search leaves the plane constant
 // unassigned (the reference then keeps the previous cell's) are flagged in
@@ -601,23 +627,54 @@ __device__ __forceinline__ void append(bool pred, unsigned* count, int* list, in
 // writes the re-distancing source mask (vofeps < vof < 1-vofeps-vofS, one bit
 // per cell, one 64-bit word per wavefront) and lists those cells.
 // cnt[0]: sources listed, cnt[1]: stale cells listed.
-__global__ void __launch_bounds__(256) k_recon(ClsGeo G, const double* __restrict__ vof,
-                                               const double* __restrict__ phi, Planes P,
-                                               unsigned char* __restrict__ stale, int full_out,
+constexpr int RECON_TY = 16;  // rows per k_recon workgroup (64 x 16 cells)
+__global__ void __launch_bounds__(64 * RECON_TY) k_recon(ClsGeo G, const double* __restrict__ vof, Planes P,
+                                                         int full_out,
                                                unsigned long long* __restrict__ mask, int mw,
-                                               int* __restrict__ srcs, int* __restrict__ stales,
+                                               int* __restrict__ srcs, int* __restrict__ ifc,
                                                unsigned* __restrict__ cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const bool in = i <= G.ni && j <= G.nj;
-    bool src = false, st = false;
+    bool src = false, itf = false, heavy = false;
     size_t c = 0;
     if (in) {
         c = cix(G, i, j);
         const double vf = vof[c], vs = G.vofS[c];
         src = (0.0 + VOFEPS) < vf && vf < (1.0 - VOFEPS - vs);
-        const int cl = recon_class(vf, vs);
+        itf = recon_class(vf, vs) != 0;
+        heavy = !(vf >= (1.0 - VOFEPS) || vf <= VOFEPS);  // vof_flux's PLIC branch
+        if (!itf && full_out) {
+            P.nx[c] = 0.0;
+            P.ny[c] = 1.0;
+        }
+    }
+    block_append<RECON_TY>(itf || heavy, cnt + 2, ifc, (int)c);
+    if (mask) {
+        // blockDim.x == 64: one wavefront covers 64 consecutive i of row j
+        const unsigned long long bits = __ballot(src);
+        if (threadIdx.x == 0 && j <= G.nj) mask[(size_t)(j - 1) * mw + blockIdx.x] = bits;
+        block_append<RECON_TY>(src, cnt, srcs, (int)c);
+    }
+}
+
+// The interface cells listed by k_recon, packed one per lane: the normal and
+// plane constant cost the same per cell wherever a cell lies, and packing
+// keeps the wavefronts of bulk liquid and gas out of that work.
+__global__ void __launch_bounds__(256) k_recon_ifc(ClsGeo G, const double* __restrict__ vof,
+                                                   const double* __restrict__ phi, Planes P,
+                                                   unsigned char* __restrict__ stale, const int* __restrict__ ifc,
+                                                   int* __restrict__ stales, unsigned* __restrict__ cnt) {
+    const unsigned n = cnt[2];
+    for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const unsigned t = base + threadIdx.x;
+        bool st = false;
+        int c = 0;
+        if (t < n) c = ifc[t];
+        const double vf = t < n ? vof[c] : 0.0, vs = t < n ? G.vofS[c] : 0.0;
+        const int cl = t < n ? recon_class(vf, vs) : 0;
         if (cl) {
+            const int i = c % G.ni + 1, j = c / G.ni + 1;
             double nx, ny, d = 0.0;
             if (i > 1 && i < G.ni && j > 1 && j < G.nj)
                 final_interior(G, vof, phi, i, j, nx, ny);
@@ -634,17 +691,8 @@ __global__ void __launch_bounds__(256) k_recon(ClsGeo G, const double* __restric
             P.ny[c] = ny;
             P.dis[c] = d;
             if (cl == 2) stale[c] = st ? 1 : 0;
-        } else if (full_out) {
-            P.nx[c] = 0.0;
-            P.ny[c] = 1.0;
         }
-    }
-    append(st, cnt + 1, stales, (int)c);
-    if (mask) {
-        // blockDim.x == 64: one wavefront covers 64 consecutive i of row j
-        const unsigned long long bits = __ballot(src);
-        if (threadIdx.x == 0 && j <= G.nj) mask[(size_t)(j - 1) * mw + blockIdx.x] = bits;
-        append(src, cnt, srcs, (int)c);
+        append(st, cnt + 1, stales, c);
     }
 }
 
@@ -759,7 +807,30 @@ struct SweepIn {
     const double* vof;
     const double* phi;
     Planes pl;
+    // PLIC fluxes precomputed by k_flux_pre: the sweep itself holds none of
+    // the plane geometry, which keeps it free of scratch and light on registers
+    const double* fr;
+    const double* fl;
 };
+
+// the PLIC flux of vof_flux through face a with upwind cell a (far face of
+// cell a) and through face a with upwind cell a+1 (near face of cell a+1)
+template <bool X>
+__device__ double plic_far(const Line<X>& L, const SweepIn& s, int a, size_t c, double f, double w, double dtv) {
+    const ClsGeo& G = L.G;
+    const double hx = X ? G.dx[a - 1] : G.dx[L.b - 1], hy = X ? G.dy[L.b - 1] : G.dy[a - 1];
+    double nx, ny, d;
+    plane_of(G, s.pl, c, f, hy, nx, ny, d);
+    return plic_face_flux<X>(1, nx, ny, d, f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c], hx, hy, w * dtv);
+}
+template <bool X>
+__device__ double plic_near(const Line<X>& L, const SweepIn& s, int a, size_t c, double f, double w, double dtv) {
+    const ClsGeo& G = L.G;
+    const double hx = X ? G.dx[a] : G.dx[L.b - 1], hy = X ? G.dy[L.b - 1] : G.dy[a];
+    double nx, ny, d;
+    plane_of(G, s.pl, c, f, hy, nx, ny, d);
+    return -plic_face_flux<X>(-1, nx, ny, d, f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c], hx, hy, -w * dtv);
+}
 
 // volume flux through face a (between cells a and a+1), X_Sweep :1014-1051
 template <bool X>
@@ -776,10 +847,7 @@ __device__ double vof_flux(const Line<X>& L, const SweepIn& s, int a, double dtv
             if (f >= 1.0) flux = w * dtv;
             if (f < 0.0) flux = 0.0;
         } else {
-            const double hx = X ? G.dx[a - 1] : G.dx[L.b - 1], hy = X ? G.dy[L.b - 1] : G.dy[a - 1];
-            double nx, ny, d;
-            plane_of(G, s.pl, c, f, hy, nx, ny, d);
-            flux = plic_face_flux<X>(1, nx, ny, d, f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c], hx, hy, w * dtv);
+            flux = s.fr[c];
         }
         if (L.ap(a) > 1.0 - VOFEPS && G.vofS[c] > 1.0 - VOFEPS) flux = 1.0 * w * dtv;
         return flux;
@@ -793,10 +861,7 @@ __device__ double vof_flux(const Line<X>& L, const SweepIn& s, int a, double dtv
             if (f >= 1.0) flux = w * dtv;
             if (f < 0.0) flux = 0.0;
         } else {
-            const double hx = X ? G.dx[a] : G.dx[L.b - 1], hy = X ? G.dy[L.b - 1] : G.dy[a];
-            double nx, ny, d;
-            plane_of(G, s.pl, c, f, hy, nx, ny, d);
-            flux = -plic_face_flux<X>(-1, nx, ny, d, f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c], hx, hy, -w * dtv);
+            flux = s.fl[c];
         }
         if (L.am(a + 1) > 1.0 - VOFEPS && G.vofS[c] > 1.0 - VOFEPS) flux = 1.0 * w * dtv;
         return flux;
@@ -843,6 +908,29 @@ __device__ double ls_flux_last(const Line<X>& L, const double* phi, double dtv) 
     else
         l = pn - L.h(n - 1) / 2.0 * (1.0 + w * dtv / L.h(n - 1)) * (pn - pm) / (L.pos(n) - L.pos(n - 1));
     return l * w * dtv;
+}
+
+// The PLIC face fluxes of a sweep for the listed cells (VOFEPS < vof <
+// 1-VOFEPS), packed one per lane: the far-face flux where the cell is upwind
+// of its far face, the near-face flux where it is upwind of its near face --
+// exactly the values vof_flux would compute for them in the sweep.
+template <bool X>
+__global__ void __launch_bounds__(256) k_flux_pre(ClsGeo G, SweepIn s, double dtv, const int* __restrict__ list,
+                                                  const unsigned* __restrict__ cnt, double* __restrict__ fr,
+                                                  double* __restrict__ fl) {
+    const unsigned n = cnt[2];
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int c = list[t];
+        const double f = s.vof[c];
+        if (f >= (1.0 - VOFEPS) || f <= VOFEPS) continue;  // not vof_flux's PLIC branch
+        const int i = c % G.ni + 1, j = c / G.ni + 1;
+        const Line<X> L{G, X ? j : i};
+        const int a = X ? i : j;
+        const double w = L.vel(a);
+        if (X ? (w >= 0.0) : (w > 0.0)) fr[c] = plic_far<X>(L, s, a, c, f, w, dtv);
+        const double w0 = L.vel(a - 1);
+        if (!(X ? (w0 >= 0.0) : (w0 > 0.0))) fl[c] = plic_near<X>(L, s, a - 1, c, f, w0, dtv);
+    }
 }
 
 // One sweep.  IMPL: first sweep of the sub-step (implicit divergence, clip);
@@ -1365,10 +1453,13 @@ int clsvof_allocate(Ctx* c) {
     B.best = (unsigned long long*)dev_alloc(c, n * sizeof(unsigned long long));
     B.srcs = (int*)dev_alloc(c, n * sizeof(int));
     B.stales = (int*)dev_alloc(c, n * sizeof(int));
+    B.ifc = (int*)dev_alloc(c, n * sizeof(int));
+    B.fr = (double*)dev_alloc(c, n * sizeof(double));
+    B.fl = (double*)dev_alloc(c, n * sizeof(double));
     B.cnt = (unsigned*)dev_alloc(c, ClsBuffers::CNT_SLOTS * sizeof(unsigned));
     B.mask = (unsigned long long*)dev_alloc(c, (size_t)B.mw * c->nj * sizeof(unsigned long long));
     if (!B.xc || !B.yc || !B.dis || !B.vt || !B.pt || !B.tv || !B.tl || !B.stale || !B.hit || !B.best || !B.srcs ||
-        !B.stales || !B.cnt || !B.mask)
+        !B.stales || !B.ifc || !B.fr || !B.fl || !B.cnt || !B.mask)
         return SG_ERR_ALLOC;
     return SG_OK;
 }
@@ -1385,10 +1476,11 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
     const int ring = 2 * c->ni + 2 * (c->nj - 2);
     const dim3 rblk(256), rgrd((ring + 255) / 256);
     const dim3 pblk(256), pgrd(1024);  // grid-stride loops over the listed pairs / cells
+    const dim3 rcblk(64, RECON_TY), rcgrd((c->ni + 63) / 64, (c->nj + RECON_TY - 1) / RECON_TY);
     const double dtv = dt / (double)nsub;
     hipStream_t st = c->stream;
-    // two counters per reconstruction, three reconstructions per sub-step
-    const int slots = 6 * nsub;
+    // three counters per reconstruction, three reconstructions per sub-step
+    const int slots = 9 * nsub;
     if (slots > ClsBuffers::CNT_SLOTS) {
         set_error("clsvof_advect: too many sub-steps");
         return SG_ERR_ARG;
@@ -1397,17 +1489,22 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
     int slot = 0;
     auto recon = [&](const double* v, const double* p, bool last) {
         unsigned* cnt = B.cnt + slot;
-        slot += 2;
-        hipLaunchKernelGGL(k_recon, grd, blk, 0, st, G, v, p, P, B.stale, last ? 1 : 0,
-                           last ? B.mask : (unsigned long long*)nullptr, B.mw, B.srcs, B.stales, cnt);
+        slot += 3;
+        hipLaunchKernelGGL(k_recon, rcgrd, rcblk, 0, st, G, v, P, last ? 1 : 0,
+                           last ? B.mask : (unsigned long long*)nullptr, B.mw, B.srcs, B.ifc, cnt);
+        hipLaunchKernelGGL(k_recon_ifc, dim3(256), pblk, 0, st, G, v, p, P, B.stale, B.ifc, B.stales, cnt);
         hipLaunchKernelGGL(k_dis_fix, dim3(64), pblk, 0, st, G, v, B.stale, B.stales, cnt, B.dis);
         return cnt;
     };
     for (int k = 1; k <= nsub; ++k) {
         const double level = levels[k - 1];
         // first sweep: (vof, phi) -> (vt, pt), implicit divergence
-        recon(vof, phi, false);
-        const SweepIn s1{vof, phi, P};
+        const unsigned* c1 = recon(vof, phi, false);
+        const SweepIn s1{vof, phi, P, B.fr, B.fl};
+        if (k % 2 == 1)
+            hipLaunchKernelGGL((k_flux_pre<true>), pgrd, pblk, 0, st, G, s1, dtv, B.ifc, c1, B.fr, B.fl);
+        else
+            hipLaunchKernelGGL((k_flux_pre<false>), pgrd, pblk, 0, st, G, s1, dtv, B.ifc, c1, B.fr, B.fl);
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_sweep<true, true>), grd, blk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
                                B.tl, B.vt, B.pt);
@@ -1416,8 +1513,12 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
                                B.tl, B.vt, B.pt);
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, B.vt, B.pt);
         // second sweep: (vt, pt) -> (vof, phi), explicit divergence
-        recon(B.vt, B.pt, false);
-        const SweepIn s2{B.vt, B.pt, P};
+        const unsigned* c2 = recon(B.vt, B.pt, false);
+        const SweepIn s2{B.vt, B.pt, P, B.fr, B.fl};
+        if (k % 2 == 1)
+            hipLaunchKernelGGL((k_flux_pre<false>), pgrd, pblk, 0, st, G, s2, dtv, B.ifc, c2, B.fr, B.fl);
+        else
+            hipLaunchKernelGGL((k_flux_pre<true>), pgrd, pblk, 0, st, G, s2, dtv, B.ifc, c2, B.fr, B.fl);
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_sweep<false, false>), grd, blk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
                                nullptr, vof, phi);

@@ -128,7 +128,11 @@ struct ClsBuffers {
     unsigned long long* best = nullptr;  // re-distancing: nearest distance (bits)
     int* srcs = nullptr;               // listed interface cells (re-distancing sources)
     int* stales = nullptr;             // listed cells with a stale plane constant
-    unsigned* cnt = nullptr;           // list lengths, two per reconstruction
+    int* ifc = nullptr;                // listed interface cells of a reconstruction and the cells
+                                       // whose face fluxes need the PLIC geometry
+    double* fr = nullptr;              // the latter's PLIC flux through the far / near face of a sweep
+    double* fl = nullptr;
+    unsigned* cnt = nullptr;           // list lengths, three per reconstruction
     static constexpr int CNT_SLOTS = 6 * 1024;
     unsigned long long* mask = nullptr;  // interface cells, 1 bit per cell
     int mw = 0;                        // mask words per row
