@@ -156,7 +156,10 @@ struct StagDev {
 
 // Device scalars of the momentum BiCGStab.
 struct BiScalars {
-    double rho, rho_old, alpha, omega, beta, rv, ts, tt, rr, bb;
+    double rho, rho_old, alpha, omega, beta, rv, ts, tt, rr, bb, tol2;
+    int active;  // 1 while the Jacobi-scaled residual is above tol
+    int iters;   // completed iterations
+    int nan;     // the residual became NaN
 };
 
 // Work arrays of the momentum predictor.

@@ -14,7 +14,9 @@
 //
 // Reductions are the library's two-phase deterministic ones: per-block
 // partials, then one block adds them in block order and updates the
-// scalars.
+// scalars.  The convergence test runs on the device too: once the residual
+// meets tol every kernel of a later iteration returns at once, so the host
+// enqueues several iterations between its checks.
 #include "sg_reduce.hpp"
 
 namespace sg {
@@ -61,7 +63,8 @@ __global__ void __launch_bounds__(NT) k_bi_init(size_t n, Mat5 A, const double* 
 
 // partials of (rh, r)
 __global__ void __launch_bounds__(NT) k_bi_dot(size_t n, const double* __restrict__ a1, const double* __restrict__ b1,
-                                               double* __restrict__ part) {
+                                               double* __restrict__ part, const BiScalars* __restrict__ sc) {
+    if (!sc->active) return;
     double a[1] = {0.0};
     for (size_t c = blockIdx.x * (size_t)NT + threadIdx.x; c < n; c += (size_t)gridDim.x * NT) a[0] += a1[c] * b1[c];
     block_partials<1, NT>(a, part, PredBuffers::BI_BLOCKS);
@@ -72,6 +75,7 @@ __global__ void __launch_bounds__(NT) k_bi_dot(size_t n, const double* __restric
 __global__ void __launch_bounds__(NT) k_bi_dir(size_t n, Mat5 A, const BiScalars* __restrict__ sc,
                                                const double* __restrict__ r, const double* __restrict__ v,
                                                double* __restrict__ p, double* __restrict__ y) {
+    if (!sc->active) return;
     const double beta = sc->beta, omega = sc->omega;
     for (size_t c = blockIdx.x * (size_t)NT + threadIdx.x; c < n; c += (size_t)gridDim.x * NT) {
         const double pc = r[c] + beta * (p[c] - omega * v[c]);
@@ -83,7 +87,8 @@ __global__ void __launch_bounds__(NT) k_bi_dir(size_t n, Mat5 A, const BiScalars
 // out = A in; partials of (rh, out)
 __global__ void __launch_bounds__(NT) k_bi_apply_dot(size_t n, Mat5 A, const double* __restrict__ in,
                                                      const double* __restrict__ rh, double* __restrict__ out,
-                                                     double* __restrict__ part) {
+                                                     double* __restrict__ part, const BiScalars* __restrict__ sc) {
+    if (!sc->active) return;
     double a[1] = {0.0};
     for (size_t c = blockIdx.x * (size_t)NT + threadIdx.x; c < n; c += (size_t)gridDim.x * NT) {
         const double y = apply_row(A, in, c);
@@ -97,6 +102,7 @@ __global__ void __launch_bounds__(NT) k_bi_apply_dot(size_t n, Mat5 A, const dou
 __global__ void __launch_bounds__(NT) k_bi_half(size_t n, Mat5 A, const BiScalars* __restrict__ sc,
                                                 const double* __restrict__ r, const double* __restrict__ v,
                                                 double* __restrict__ s, double* __restrict__ z) {
+    if (!sc->active) return;
     const double alpha = sc->alpha;
     for (size_t c = blockIdx.x * (size_t)NT + threadIdx.x; c < n; c += (size_t)gridDim.x * NT) {
         const double sv = r[c] - alpha * v[c];
@@ -108,7 +114,8 @@ __global__ void __launch_bounds__(NT) k_bi_half(size_t n, Mat5 A, const BiScalar
 // t = A z; partials of (t,s), (t,t)
 __global__ void __launch_bounds__(NT) k_bi_apply_t(size_t n, Mat5 A, const double* __restrict__ z,
                                                    const double* __restrict__ s, double* __restrict__ t,
-                                                   double* __restrict__ part) {
+                                                   double* __restrict__ part, const BiScalars* __restrict__ sc) {
+    if (!sc->active) return;
     double a[2] = {0.0, 0.0};
     for (size_t c = blockIdx.x * (size_t)NT + threadIdx.x; c < n; c += (size_t)gridDim.x * NT) {
         const double tv = apply_row(A, z, c);
@@ -125,6 +132,7 @@ __global__ void __launch_bounds__(NT) k_bi_update(size_t n, Mat5 A, const BiScal
                                                   const double* __restrict__ s, const double* __restrict__ t,
                                                   double* __restrict__ x, double* __restrict__ r,
                                                   double* __restrict__ part) {
+    if (!sc->active) return;
     const double alpha = sc->alpha, omega = sc->omega;
     double a[1] = {0.0};
     for (size_t c = blockIdx.x * (size_t)NT + threadIdx.x; c < n; c += (size_t)gridDim.x * NT) {
@@ -140,7 +148,8 @@ __global__ void __launch_bounds__(NT) k_bi_update(size_t n, Mat5 A, const BiScal
 enum { F_INIT = 0, F_RHO, F_ALPHA, F_OMEGA, F_RR };
 
 __global__ void __launch_bounds__(NT) k_bi_fin(const double* __restrict__ part, int nb, int mode,
-                                               BiScalars* __restrict__ sc) {
+                                               BiScalars* __restrict__ sc, double tol) {
+    if (mode != F_INIT && !sc->active) return;
     double s[2];
     if (mode == F_OMEGA) {
         finalize_sum<2, NT>(part, PredBuffers::BI_BLOCKS, nb, s);
@@ -154,6 +163,10 @@ __global__ void __launch_bounds__(NT) k_bi_fin(const double* __restrict__ part, 
     case F_INIT:
         sc->bb = s[0];
         sc->rr = s[0];
+        sc->tol2 = tol * tol * s[0];
+        sc->active = s[0] > 0.0 ? 1 : 0;
+        sc->iters = 0;
+        sc->nan = 0;
         sc->rho_old = 1.0;
         sc->alpha = 1.0;
         sc->omega = 1.0;
@@ -174,6 +187,12 @@ __global__ void __launch_bounds__(NT) k_bi_fin(const double* __restrict__ part, 
         break;
     case F_RR:
         sc->rr = s[0];
+        sc->iters += 1;
+        if (!(s[0] > sc->tol2)) sc->active = 0;
+        if (s[0] != s[0]) {
+            sc->active = 0;
+            sc->nan = 1;
+        }
         break;
     }
 }
@@ -193,35 +212,39 @@ int momentum_solve(Ctx* c, const double* m, const double* b, double* x, double t
     double* part = B.partials;
     BiScalars* sc = B.sc;
     hipLaunchKernelGGL(k_bi_init, g, blk, 0, st, n, A, b, x, r, rh, p, v, part);
-    hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_INIT, sc);
-    SG_HIP(hipMemcpyAsync(B.sc_host, sc, sizeof(BiScalars), hipMemcpyDeviceToHost, st));
-    SG_HIP(hipStreamSynchronize(st));
-    const double bb = B.sc_host->bb;
+    hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_INIT, sc, tol);
     *iters = 0;
     *relres = 0.0;
-    if (!(bb > 0.0)) return SG_OK;  // x = 0 solves b = 0
-    const double tol2 = tol * tol * bb;
-    for (int it = 1; it <= max_iter; ++it) {
-        hipLaunchKernelGGL(k_bi_dot, g, blk, 0, st, n, rh, r, part);
-        hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_RHO, sc);
-        hipLaunchKernelGGL(k_bi_dir, g, blk, 0, st, n, A, sc, r, v, p, y);
-        hipLaunchKernelGGL(k_bi_apply_dot, g, blk, 0, st, n, A, y, rh, v, part);
-        hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_ALPHA, sc);
-        hipLaunchKernelGGL(k_bi_half, g, blk, 0, st, n, A, sc, r, v, s, z);
-        hipLaunchKernelGGL(k_bi_apply_t, g, blk, 0, st, n, A, z, s, t, part);
-        hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_OMEGA, sc);
-        hipLaunchKernelGGL(k_bi_update, g, blk, 0, st, n, A, sc, y, z, s, t, x, r, part);
-        hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_RR, sc);
+    // these diagonally dominant rows converge in a few iterations: enqueue
+    // four before the first check, then two at a time
+    int it = 0, batch = 4;
+    while (it < max_iter) {
+        const int k = std::min(batch, max_iter - it);
+        for (int q = 0; q < k; ++q) {
+            hipLaunchKernelGGL(k_bi_dot, g, blk, 0, st, n, rh, r, part, (const BiScalars*)sc);
+            hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_RHO, sc, 0.0);
+            hipLaunchKernelGGL(k_bi_dir, g, blk, 0, st, n, A, sc, r, v, p, y);
+            hipLaunchKernelGGL(k_bi_apply_dot, g, blk, 0, st, n, A, y, rh, v, part, (const BiScalars*)sc);
+            hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_ALPHA, sc, 0.0);
+            hipLaunchKernelGGL(k_bi_half, g, blk, 0, st, n, A, sc, r, v, s, z);
+            hipLaunchKernelGGL(k_bi_apply_t, g, blk, 0, st, n, A, z, s, t, part, (const BiScalars*)sc);
+            hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_OMEGA, sc, 0.0);
+            hipLaunchKernelGGL(k_bi_update, g, blk, 0, st, n, A, sc, y, z, s, t, x, r, part);
+            hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_RR, sc, 0.0);
+        }
+        SG_HIP(hipGetLastError());
+        it += k;
+        batch = 2;
         SG_HIP(hipMemcpyAsync(B.sc_host, sc, sizeof(BiScalars), hipMemcpyDeviceToHost, st));
         SG_HIP(hipStreamSynchronize(st));
-        *iters = it;
-        const double rr = B.sc_host->rr;
-        *relres = std::sqrt(rr / bb);
-        if (!(rr == rr)) {
+        const BiScalars& h = *B.sc_host;
+        *iters = h.iters;
+        *relres = h.bb > 0.0 ? std::sqrt(h.rr / h.bb) : 0.0;
+        if (h.nan) {
             set_error("momentum solve: NaN in the BiCGStab residual");
             return SG_NAN;
         }
-        if (rr <= tol2) return SG_OK;
+        if (!h.active) return SG_OK;  // converged (or b = 0: x = 0)
     }
     return SG_NOT_CONVERGED;
 }

@@ -378,8 +378,13 @@ int enqueue_init(Ctx* c, int warm, int mode) {
     return SG_OK;
 }
 
-// PCG iterations captured once into graphs: variant 0 = two iterations
-// (parities 0 and 1), 1 = one iteration of parity 0, 2 = one of parity 1.
+// PCG iterations captured once into graphs: variant 0 = GRAPH_ITERS
+// iterations (parities 0, 1, 0, 1, ...), 1 = one iteration of parity 0, 2 =
+// one of parity 1.  Each graph launch costs a pause of tens of microseconds
+// partway through its first iteration, so the long graph carries several
+// iterations.
+constexpr int GRAPH_ITERS = 4;
+static_assert(GRAPH_ITERS % 2 == 0, "the long graph must end on the parity it starts from");
 int iteration_graph(Ctx* c, int variant, hipGraphExec_t* out) {
     const int k = c->prm.precision == SG_PREC_MIXED ? 1 : 0;
     const int defl = deflating(c) ? 1 : 0;
@@ -401,7 +406,7 @@ int iteration_graph(Ctx* c, int variant, hipGraphExec_t* out) {
     hipGraph_t graph = nullptr;
     SG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_iteration(c, variant == 2 ? 1 : 0);
-    if (rc == SG_OK && variant == 0) rc = enqueue_iteration(c, 1);
+    for (int q = 1; rc == SG_OK && variant == 0 && q < GRAPH_ITERS; ++q) rc = enqueue_iteration(c, q & 1);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -421,13 +426,13 @@ int iteration_graph(Ctx* c, int variant, hipGraphExec_t* out) {
 int launch_iterations(Ctx* c, int n) {
     while (n > 0) {
         if (c->use_graph) {
-            const int variant = (c->pcur == 0 && n >= 2) ? 0 : (c->pcur == 0 ? 1 : 2);
+            const int variant = (c->pcur == 0 && n >= GRAPH_ITERS) ? 0 : (c->pcur == 0 ? 1 : 2);
             hipGraphExec_t g = nullptr;
             const int rc = iteration_graph(c, variant, &g);
             if (rc) return rc;
             SG_HIP(hipGraphLaunch(g, c->stream));
             if (variant == 0) {
-                n -= 2;
+                n -= GRAPH_ITERS;
             } else {
                 n -= 1;
                 c->pcur ^= 1;
@@ -512,7 +517,7 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
     double rate = 0.0;
     for (;;) {
         while (c->sc_host->active && it < max_iter) {
-            int n = 2;
+            int n = 4;  // before the rate is known
             if (every > 0) {
                 n = every;
             } else if (rate < 0.0) {
