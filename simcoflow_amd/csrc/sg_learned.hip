@@ -45,37 +45,50 @@ __global__ void __launch_bounds__(256) k_prolong(Dims F, Dims C, Prol<double> P,
     x[p] = diag[p] > 0.0 ? prolong_point(C, P, e, i, j) : 0.0;
 }
 
-// one block: rms of b over the fluid points of the level -> sc->nn_scale
-__global__ void __launch_bounds__(NTR) k_rms(Dims D, const double* __restrict__ diag, const double* __restrict__ b,
-                                             PcgScalars* sc) {
-    __shared__ double red[2][NTR / 64];
-    double s = 0.0, n = 0.0;
-    for (int e = threadIdx.x; e < D.ni * D.nj; e += NTR) {
-        const int i = e % D.ni, j = e / D.ni;
+// rms of b over the fluid points of the level -> sc->nn_scale, in two
+// phases: per-block partials of (sum b^2, count), then one block
+__global__ void __launch_bounds__(256) k_rms(Dims D, const double* __restrict__ diag, const double* __restrict__ b,
+                                             double* __restrict__ part, int max_blocks) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    double v[2] = {0.0, 0.0};
+    if (i < D.ni && j < D.nj) {
         const size_t p = D.off(i, j);
         if (diag[p] > 0.0) {
-            s += b[p] * b[p];
-            n += 1.0;
+            v[0] = b[p] * b[p];
+            v[1] = 1.0;
         }
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-        s += __shfl_xor(s, o, 64);
-        n += __shfl_xor(n, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = s;
-        red[1][threadIdx.x >> 6] = n;
-    }
-    __syncthreads();
+    block_partials<2, TX * TY>(v, part, max_blocks);
+}
+
+__global__ void __launch_bounds__(NTR) k_rms_fin(const double* __restrict__ part, int max_blocks, int nb,
+                                                 PcgScalars* sc) {
+    double s[2];
+    finalize_sum<2, NTR>(part, max_blocks, nb, s);
     if (threadIdx.x == 0) {
-        double ss = 0.0, nn = 0.0;
-        for (int w = 0; w < NTR / 64; ++w) {
-            ss += red[0][w];
-            nn += red[1][w];
-        }
-        const double r = nn > 0.0 ? sqrt(ss / nn) : 0.0;
+        const double r = s[1] > 0.0 ? sqrt(s[0] / s[1]) : 0.0;
         sc->nn_scale = r > 0.0 ? r : 1.0;
     }
+}
+
+// sum and count of the fine cells of block (I,J), rows outer: all S*S loads
+// are independent, so they are in flight together
+template <int S>
+__device__ inline double block_vof(const double* __restrict__ vof, int I, int J, int ni0, int nj0, int& nv) {
+    double vs = 0.0;
+    nv = 0;
+#pragma unroll
+    for (int a = 0; a < S; ++a)
+#pragma unroll
+        for (int c = 0; c < S; ++c) {
+            const int jj = J * S + a, ii = I * S + c;
+            if (jj < nj0 && ii < ni0) {
+                vs += vof[(size_t)jj * ni0 + ii];
+                ++nv;
+            }
+        }
+    return vs;
 }
 
 // net input at level L (see the file comment); optionally the training
@@ -95,11 +108,19 @@ __global__ void __launch_bounds__(256) k_features(Dims D, Op<double> A, const do
     const int s = 1 << L;
     double vs = 0.0;
     int nv = 0;
-    for (int jj = J * s; jj < (J + 1) * s && jj < nj0; ++jj)
-        for (int ii = I * s; ii < (I + 1) * s && ii < ni0; ++ii) {
-            vs += vof[(size_t)jj * ni0 + ii];
-            ++nv;
-        }
+    switch (L) {
+    case 0: vs = block_vof<1>(vof, I, J, ni0, nj0, nv); break;
+    case 1: vs = block_vof<2>(vof, I, J, ni0, nj0, nv); break;
+    case 2: vs = block_vof<4>(vof, I, J, ni0, nj0, nv); break;
+    case 3: vs = block_vof<8>(vof, I, J, ni0, nj0, nv); break;
+    case 4: vs = block_vof<16>(vof, I, J, ni0, nj0, nv); break;
+    default:
+        for (int jj = J * s; jj < (J + 1) * s && jj < nj0; ++jj)
+            for (int ii = I * s; ii < (I + 1) * s && ii < ni0; ++ii) {
+                vs += vof[(size_t)jj * ni0 + ii];
+                ++nv;
+            }
+    }
     float f[8];
     f[0] = d > 0.0 ? (float)(b[p] * inv) : 0.f;
     f[1] = (float)(A.cE[p] * id);
@@ -190,7 +211,14 @@ int features(Ctx* c, int level, bf16* X, float* t, float* mask) {
     if (rc) return rc;
     Level& L = c->lev[level];
     const double* b = level ? L.v.b : c->rhs;
-    hipLaunchKernelGGL(k_rms, dim3(1), dim3(NTR), 0, c->stream, L.d, (const double*)L.A.diag, b, c->sc);
+    {
+        const dim3 g = grid2d(L.d.ni, L.d.nj);
+        double* part = c->red.partials + (size_t)5 * NRED * c->red.max_blocks;
+        hipLaunchKernelGGL(k_rms, g, block2d(), 0, c->stream, L.d, (const double*)L.A.diag, b, part,
+                           c->red.max_blocks);
+        hipLaunchKernelGGL(k_rms_fin, dim3(1), dim3(NTR), 0, c->stream, (const double*)part, c->red.max_blocks,
+                           (int)(g.x * g.y), c->sc);
+    }
     hipLaunchKernelGGL(k_features, grid2d(L.d.ni, L.d.nj), block2d(), 0, c->stream, L.d, L.A, b, c->lev[0].d,
                        (const double*)c->field[SG_F_VOF], c->ni, c->nj, level, (const double*)c->x,
                        (const PcgScalars*)c->sc, X, t, mask);

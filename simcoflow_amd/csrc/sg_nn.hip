@@ -28,19 +28,26 @@ __device__ __forceinline__ floatx4 mfma(const bf16x8& a, const bf16x8& b, const 
 }
 
 // out(b,y,x,co) = sum_{tap,ci} wt[co][tap*CIN+ci] * in(b, y+dy-1, x+dx-1, ci)
-// One wave: 16*NPB pixels of one row, all COUT channels.  The epilogue
+// One wave: 16*NPB pixels of one row and COUT/CG of the output channels
+// (CG > 1 on small levels, where one wave per pixel group would leave most
+// of the chip idle; each output still sums over (tap, ci) in the same
+// order).  The epilogue
 // applies ReLU (optionally after adding a skip tensor), masks by aux > 0
 // (data gradient through a ReLU), writes plain, or writes channel 0 as FP32
 // (the output layer).
-template <int CIN, int COUT, int NPB>
+template <int CIN, int COUT, int NPB, int CG>
 __global__ void __launch_bounds__(256) k_conv(T4 in, const bf16* __restrict__ wt, T4 out, T4 aux,
                                               float* __restrict__ yo, int nb, int epi) {
     constexpr int KP = kpad(CIN);
-    constexpr int MB = COUT / 16;
+    constexpr int MB = COUT / 16 / CG;
+    static_assert(COUT % (16 * CG) == 0, "channel groups must split the 16-channel blocks");
     const int lane = threadIdx.x & 63, r16 = lane & 15, q4 = lane >> 4;
     const int tiles = in.W / (16 * NPB);
-    const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wave >= (long)nb * in.H * tiles) return;
+    const long wave_all = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave_all >= (long)nb * in.H * tiles * CG) return;
+    const int g = (int)(wave_all % CG);
+    const long wave = wave_all / CG;
+    if (epi == E_OUT && g != 0) return;  // the output layer keeps channel 0 only
     const int tx = (int)(wave % tiles);
     const long t = wave / tiles;
     const int y = (int)(t % in.H), b = (int)(t / in.H);
@@ -50,7 +57,7 @@ __global__ void __launch_bounds__(256) k_conv(T4 in, const bf16* __restrict__ wt
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
         for (int q = 0; q < NPB; ++q) acc[mb][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const bf16* wrow = wt + (size_t)r16 * KP + 8 * q4;
+    const bf16* wrow = wt + ((size_t)g * MB * 16 + r16) * KP + 8 * q4;
     for (int kc = 0; kc < KP; kc += 32) {
         const int k0 = kc + 8 * q4;
         const int tap = k0 / CIN, ci = k0 - tap * CIN;
@@ -83,7 +90,7 @@ __global__ void __launch_bounds__(256) k_conv(T4 in, const bf16* __restrict__ wt
         }
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) {
-            const int co = mb * 16 + 4 * q4;
+            const int co = (g * MB + mb) * 16 + 4 * q4;
             float v[4] = {acc[mb][q][0], acc[mb][q][1], acc[mb][q][2], acc[mb][q][3]};
             if (epi == E_RELU_SKIP || epi == E_MASK) {
                 const bf16x4 a = *(const bf16x4*)(aux.p + t4off(aux, b, y + 1, px + 1) + co);
@@ -333,17 +340,35 @@ __global__ void __launch_bounds__(256) k_adam(float* __restrict__ w, const float
 
 inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
+template <int CIN, int COUT, int NPB, int CG>
+void conv_launch(Net* n, T4 in, const bf16* w, T4 out, T4 aux, float* yo, int nb, int epi, long waves) {
+    hipLaunchKernelGGL((k_conv<CIN, COUT, NPB, CG>), dim3((unsigned)((waves * CG + 3) / 4)), dim3(256), 0, n->stream,
+                       in, w, out, aux, yo, nb, epi);
+}
+
+template <int CIN, int COUT, int NPB>
+void conv_split(Net* n, T4 in, const bf16* w, T4 out, T4 aux, float* yo, int nb, int epi, long waves) {
+    // split the output channels while there are fewer than ~4 waves per CU
+    constexpr int MB = COUT / 16;
+    int cg = 1;
+    while (cg < MB && waves * cg < 1024) cg *= 2;
+    if (cg >= 8 && MB % 8 == 0)
+        conv_launch<CIN, COUT, NPB, (MB % 8 == 0 ? 8 : 1)>(n, in, w, out, aux, yo, nb, epi, waves);
+    else if (cg >= 4 && MB % 4 == 0)
+        conv_launch<CIN, COUT, NPB, (MB % 4 == 0 ? 4 : 1)>(n, in, w, out, aux, yo, nb, epi, waves);
+    else if (cg >= 2 && MB % 2 == 0)
+        conv_launch<CIN, COUT, NPB, (MB % 2 == 0 ? 2 : 1)>(n, in, w, out, aux, yo, nb, epi, waves);
+    else
+        conv_launch<CIN, COUT, NPB, 1>(n, in, w, out, aux, yo, nb, epi, waves);
+}
+
 template <int CIN, int COUT>
 int conv(Net* n, T4 in, const bf16* w, T4 out, T4 aux, float* yo, int nb, int epi) {
-    const long waves4 = (long)nb * in.H * (in.W / 64);
-    if (in.W % 64 == 0) {
-        hipLaunchKernelGGL((k_conv<CIN, COUT, 4>), dim3((unsigned)((waves4 + 3) / 4)), dim3(256), 0, n->stream, in,
-                           w, out, aux, yo, nb, epi);
-    } else {
-        const long waves = (long)nb * in.H * (in.W / 16);
-        hipLaunchKernelGGL((k_conv<CIN, COUT, 1>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, n->stream, in,
-                           w, out, aux, yo, nb, epi);
-    }
+    // four 16-pixel groups per wave where that still leaves ~4 waves per CU
+    if (in.W % 64 == 0 && (long)nb * in.H * (in.W / 64) >= 1024)
+        conv_split<CIN, COUT, 4>(n, in, w, out, aux, yo, nb, epi, (long)nb * in.H * (in.W / 64));
+    else
+        conv_split<CIN, COUT, 1>(n, in, w, out, aux, yo, nb, epi, (long)nb * in.H * (in.W / 16));
     SG_HIP(hipGetLastError());
     return SG_OK;
 }
