@@ -32,6 +32,7 @@ module simcogpu
     integer(c_int) :: warm_start
     integer(c_int) :: check_every
     integer(c_int) :: mg_coarse_krylov
+    integer(c_int) :: momentum_method
   end type sg_params
 
   type, bind(C), public :: sg_solve_info

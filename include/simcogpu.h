@@ -55,6 +55,9 @@ typedef struct sg_params {
     int mg_coarse_krylov; /* flexible-CG iterations on the first level of at most 1024 points,
                              each preconditioned by the V-cycle below it (0: one V-cycle);
                              default 4 */
+    int momentum_method;  /* predictor's momentum systems: 0 (default) Jacobi sweeps, with
+                             Jacobi-BiCGStab if they have not converged in 40 sweeps;
+                             1 Jacobi-BiCGStab */
 } sg_params;
 
 typedef struct sg_solve_info {

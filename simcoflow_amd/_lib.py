@@ -37,7 +37,7 @@ class Params(ctypes.Structure):
         ("bc_w", ctypes.c_int), ("bc_e", ctypes.c_int), ("bc_s", ctypes.c_int), ("bc_n", ctypes.c_int),
         ("precision", ctypes.c_int), ("max_iter", ctypes.c_int), ("tol", ctypes.c_double),
         ("mg_pre", ctypes.c_int), ("mg_post", ctypes.c_int), ("warm_start", ctypes.c_int),
-        ("check_every", ctypes.c_int), ("mg_coarse_krylov", ctypes.c_int),
+        ("check_every", ctypes.c_int), ("mg_coarse_krylov", ctypes.c_int), ("momentum_method", ctypes.c_int),
     ]
 
 

@@ -76,6 +76,10 @@ static int check_params(const sg_params* p) {
         set_error("mg_coarse_krylov must be in [0, 64]");
         return SG_ERR_ARG;
     }
+    if (p->momentum_method != 0 && p->momentum_method != 1) {
+        set_error("momentum_method must be 0 (Jacobi) or 1 (BiCGStab)");
+        return SG_ERR_ARG;
+    }
     return SG_OK;
 }
 
@@ -110,6 +114,7 @@ int sg_params_default(sg_params* p) {
     p->warm_start = 0;
     p->check_every = 0;
     p->mg_coarse_krylov = 4;
+    p->momentum_method = 0;
     return SG_OK;
 }
 

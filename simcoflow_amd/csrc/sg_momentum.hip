@@ -286,7 +286,7 @@ int jacobi_solve(Ctx* c, const Mat5& A, const double* b, double* x, double tol, 
 
 int momentum_solve(Ctx* c, const double* m, const double* b, double* x, double tol, int max_iter, int* iters,
                    double* relres) {
-    {
+    if (c->prm.momentum_method == 0) {
         const size_t n = (size_t)c->ni * c->nj;
         const Mat5 A{m, m + n, m + 2 * n, m + 3 * n, m + 4 * n, c->ni, c->nj};
         int status = SG_OK;

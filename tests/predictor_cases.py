@@ -37,13 +37,14 @@ def predictor_inputs(case, seed=0, amp=0.4, Re=None, shift=0.3):
                 DpU=DpU, DpV=DpV, upred=upred, vpred=vpred, geo=case.geo)
 
 
-def device_predictor(d, first_step=False, vb=0.0):
+def device_predictor(d, first_step=False, vb=0.0, momentum_method=0):
     """The same step through the C ABI; returns the device results."""
     from simcoflow_amd.projection import Projection, default_params
     g = d["grid"]
     ph = d["phys"]
     prm = default_params(rho_l=ph["rho_l"], rho_g=ph["rho_g"], rho_ref=ph["rho_ref"], nu_l=ph["nu_l"],
-                         nu_g=ph["nu_g"], nu_ref=ph["nu_ref"], Re=ph["Re"], Fr=ph["Fr"])
+                         nu_g=ph["nu_g"], nu_ref=ph["nu_ref"], Re=ph["Re"], Fr=ph["Fr"],
+                         momentum_method=momentum_method)
     pj = Projection(g, prm)
     try:
         pj.set_geometry(d["geo"])

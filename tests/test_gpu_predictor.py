@@ -86,3 +86,24 @@ def test_predictor_low_reynolds(oracle_lib):
     for k in ("upred", "vpred"):
         scale = np.abs(ref[k][1:-1, 1:-1]).max()
         assert np.abs(got[k] - ref[k]).max() / scale <= 1e-11
+
+
+@pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
+def test_momentum_methods_agree(oracle_lib, name, mk):
+    """The default Jacobi sweeps and the Jacobi-BiCGStab solver reach the same
+    predicted velocities (both to a Jacobi-scaled residual of 1e-13), and both
+    match the oracle's direct solves."""
+    case = mk()
+    d = P.predictor_inputs(case, seed=4)
+    print(f"[predictor] momentum methods {name}", flush=True)
+    ref = P.oracle_predictor(oracle_lib, d)
+    jac = P.device_predictor(d, momentum_method=0)
+    bic = P.device_predictor(d, momentum_method=1)
+    print("Jacobi sweeps", jac["info"]["iterations_u"], jac["info"]["iterations_v"],
+          "BiCGStab iterations", bic["info"]["iterations_u"], bic["info"]["iterations_v"])
+    for got in (jac, bic):
+        assert got["info"]["relres_u"] <= 1e-13 and got["info"]["relres_v"] <= 1e-13, got["info"]
+    for k in ("upred", "vpred"):
+        scale = max(np.abs(ref[k][1:-1, 1:-1]).max(), 1e-30)
+        assert np.abs(jac[k] - bic[k]).max() / scale <= 1e-11, k
+        assert np.abs(jac[k] - ref[k]).max() / scale <= 1e-11, k

@@ -125,6 +125,31 @@ def test_velocity_correction_bit_exact(oracle_lib, name, mk):
     assert np.abs(mg).max() <= 1e-10
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("name,mk", SMALL, ids=[s[0] for s in SMALL])
+def test_coarse_krylov_against_plain_cycle(oracle_lib, name, mk, precision):
+    """The inner flexible-CG iterations on the coarse levels (default) and
+    the plain V-cycle (mg_coarse_krylov = 0) both reach the direct solution
+    to the 1e-8 bar; the inner iterations never cost outer iterations."""
+    import scipy.sparse.linalg as spl
+    O = oracle_lib
+    case = mk()
+    res = {}
+    for kry in (0, 4):
+        pj, fields = _setup(case, precision=precision, tol=1e-10, mg_coarse_krylov=kry)
+        info = pj.solve()
+        res[kry] = (pj.download("PP")[1:-1, 1:-1], info.iterations, info.relres)
+        pj.close()
+    M, b = _oracle_system(O, case, fields)
+    x = spl.spsolve(O.matr_to_scipy(case.grid, case.geo, M).tocsc(), b.ravel()).reshape(case.grid.shape)
+    for kry, (xg, its, relres) in res.items():
+        rel = np.linalg.norm(xg - x) / np.linalg.norm(x)
+        print(name, "precision", precision, "coarse krylov", kry, "iterations", its, "err", rel)
+        assert relres <= 1e-10 * 1.01
+        assert rel <= 1e-8, (kry, rel)
+    assert res[4][1] <= res[0][1] + 1, (res[0][1], res[4][1])
+
+
 def test_swarm_full_size_fp64_and_mixed():
     """BASELINE-size swarm (2048x2048): both precisions reach 1e-8 and agree."""
     case = C.case_swarm(2048, 2048, seed=0)
