@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip training and the learned path")
     ap.add_argument("--no-full-step", action="store_true", help="skip the per-part step timing")
+    ap.add_argument("--no-particles", action="store_true", help="skip the particle tracker timing")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--nn-width", type=int, default=256, help="widest multigrid level the net runs on")
     return ap.parse_args()
@@ -250,6 +251,46 @@ def step_parts(case, stags, precision, steps=3, warmup=2):
     return {k: round(float(np.mean(v)), 4) for k, v in parts.items()}
 
 
+PARTICLE_STREAM_BYTES = 72  # x, y, u, v, d read + x, y, u, v written, FP64
+PARTICLE_GATHER_BYTES = 48  # vof, vofS, two u faces, two v faces of the particle's cell
+
+
+def particles(case, fields, n=1 << 20, reps=20, seed=0):
+    """The Lagrangian tracker (TrackingParticles, 10 sub-steps of the
+    exponential drag integrator) on n particles spread uniformly over the
+    swarm box, diameters log-normal about 50 um (sigma 0.5), density 2500
+    kg/m^3, in the seeded predictor velocity field; the kernel's average
+    device time over reps launches (HIP events on the context's stream)."""
+    from simcoflow_amd import _lib
+    from simcoflow_amd.projection import Projection, params_for
+    g = case.grid
+    rng = np.random.default_rng(seed)
+    lx, ly = float(g.dx.sum()), float(g.dy.sum())
+    x, y = rng.uniform(0.0, lx, n), rng.uniform(0.0, ly, n)
+    u, v = rng.normal(0.0, 0.05, n), rng.normal(0.0, 0.05, n)
+    d = np.exp(rng.normal(np.log(50e-6), 0.5, n))
+    f = case.fluids
+    prm = dict(rho_l=f.rho_l, rho_g=f.rho_g, nu_l=f.mu_l / f.rho_l, nu_g=f.mu_g / f.rho_g, rho_p=2500.0, gx=0.0,
+               gy=-f.g, lref=1.0, uref=1.0, us=0.0, vs=0.0)
+    pj = Projection(g, params_for(f))
+    try:
+        pj.set_geometry(case.geo)
+        pj.upload("VOF", case.itf.vof)
+        pj.upload("U", fields[0])
+        pj.upload("V", fields[1])
+        pj.particles_set(x, y, u, v, d)
+        flagged = pj.particles_track(case.dt, prm)
+        ms = ctypes.c_double()
+        _lib.check(pj.lib.sg_time_kernel(pj.ctx, 4, reps, ctypes.byref(ms)))
+    finally:
+        pj.close()
+    b = n * (PARTICLE_STREAM_BYTES + PARTICLE_GATHER_BYTES)
+    gbs = b / (ms.value * 1e-3) / 1e9
+    return {"kernel": "k_particles (TrackingParticles, FP64, one thread per particle)", "particles": n,
+            "avg_ms": round(ms.value, 5), "mparticle_steps_per_s": round(n / (ms.value * 1e-3) / 1e6, 1),
+            "bytes_alg": int(b), "GBs": round(gbs, 1), "frac_hbm": round(gbs / PEAK_HBM_GBS, 4), "flagged": flagged}
+
+
 def measured_traffic(kernel_name):
     """HBM bytes per launch of the roofline kernel from the committed rocprof
     counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel),
@@ -392,6 +433,9 @@ def main():
         out["kernels"] = {"fp64_up_sweep": proj["fp64"]["kernel"].get("up"),
                           "fp64_apply": proj["fp64"]["kernel"].get("apply"),
                           "copy": proj["fp64"]["kernel"].get("copy")}
+    if not a.no_particles:
+        log("particle tracking")
+        out["particles"] = particles(case, fields)
     if not a.no_full_step:
         log("time step parts")
         out["step_parts_ms"] = {"mixed": step_parts(case, stags, 1)}

@@ -67,12 +67,18 @@ module simcogpu
     real(c_double) :: depth, amp, kw, omega
   end type sg_clsvof_params
 
+  ! TrackingParticles (Particles.f90:71-205) constants; see sg_particle_params in simcogpu.h
+  type, bind(C), public :: sg_particle_params
+    real(c_double) :: rho_l, rho_g, nu_l, nu_g, rho_p, gx, gy, lref, uref, us, vs
+  end type sg_particle_params
+
   public :: sg_set_staggered_geometry, sg_predictor_resident, sg_predictor_rows, sg_interface_to_uv_resident
   public :: sg_clsvof_params_default, sg_set_centres, sg_clsvof_advect_resident, sg_clsvof_advect
   public :: sg_last_error, sg_params_default, sg_ctx_create, sg_ctx_destroy, sg_set_params
   public :: sg_upload, sg_download, sg_synchronize, sg_set_geometry
   public :: sg_poisson_build_resident, sg_poisson_solve_resident, sg_poisson_solve
   public :: sg_poisson_matrix, sg_poisson_rhs, sg_poisson_apply, sg_velocity_correct_resident
+  public :: sg_particles_set, sg_particles_get, sg_particles_track
   public :: sg_error_message
 
   interface
@@ -215,6 +221,26 @@ module simcogpu
       type(c_ptr), value :: ctx, vof, phi, nx, ny, u, v
       real(c_double), value :: dt, time
       type(sg_clsvof_params), intent(in) :: p
+      integer(c_int) :: rc
+    end function
+    function sg_particles_set(ctx, n, x, y, u, v, d) bind(C, name="sg_particles_set") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, x, y, u, v, d
+      integer(c_int), value :: n
+      integer(c_int) :: rc
+    end function
+    function sg_particles_get(ctx, n, x, y, u, v) bind(C, name="sg_particles_get") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, x, y, u, v
+      integer(c_int), value :: n
+      integer(c_int) :: rc
+    end function
+    function sg_particles_track(ctx, dt, p, flagged) bind(C, name="sg_particles_track") result(rc)
+      import :: c_int, c_ptr, c_double, sg_particle_params
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+      type(sg_particle_params), intent(in) :: p
+      integer(c_int), intent(out) :: flagged
       integer(c_int) :: rc
     end function
   end interface

@@ -275,13 +275,42 @@ int sg_nn_features(sg_ctx* ctx, int level, void* x, float* t, float* mask, doubl
  * disables it */
 int sg_set_learned(sg_ctx* ctx, const float* w, int level, double out_scale, int deflate);
 
+/* ---- Lagrangian particles (replaces TrackingParticles, Particles.f90:71-205,
+ * with Drag :206-223 and ParticlePosition :228-249) -------------------------
+ * A particle set lives on the device with the context.  sg_particles_track
+ * advances it over one time step dt (non-dimensional, as the flow) in
+ * itp = 10 sub-steps of the reference's exponential drag integrator
+ * (Schiller-Naumann drag, added mass Ca = 0.5, the fluid velocity
+ * interpolated in the particle's P-cell at its position at the start of the
+ * step), one-way coupled: it reads the device fields VOF, U, V, the geometry
+ * (VOFS) and the P-cell centres, and changes nothing else.  flagged: the
+ * number of particles that met the reference's failure test (NaN velocity,
+ * |x| > 1e10 or |ax| > 1e10; the reference pauses). */
+typedef struct sg_particle_params {
+    double rho_l, rho_g; /* row, roa (StateVariables.f90:26-27) */
+    double nu_l, nu_g;   /* nuw, nua */
+    double rho_p;        /* rop, the particles' density (Main.f90:150) */
+    double gx, gy;       /* gravity components as the reference's gx, gy (Main.f90:200-201) */
+    double lref, uref;   /* PGrid%Lref, TVar%Uref: positions in metres, velocities in m/s */
+    double us, vs;       /* velocity of the solid (BoomCase%us, %vs) */
+} sg_particle_params;
+
+/* positions x, y, velocities u, v, diameters d of n particles (host) */
+int sg_particles_set(sg_ctx* ctx, int n, const double* x, const double* y, const double* u, const double* v,
+                     const double* d);
+int sg_particles_get(sg_ctx* ctx, int n, double* x, double* y, double* u, double* v);
+int sg_particles_track(sg_ctx* ctx, double dt, const sg_particle_params* p, int* flagged);
+
 /* ---- measurement ----------------------------------------------------------
  * Average device time (HIP events on the context's stream) of `reps`
  * back-to-back launches of one kernel on the current data: which = 0 the
  * fine-level matrix-free apply y = A x, 1 the fine-level multigrid
  * down-sweep, 2 the fine-level up-sweep (with the fused PCG dots), 3 a plain
- * copy of one padded fine-level vector (counter calibration).  1-3 need one
- * solve first.  No counterpart in the reference (it has no timers). */
+ * copy of one padded fine-level vector (counter calibration), 4 the particle
+ * tracker on the current particle set with the last sg_particles_track call's
+ * dt and parameters (the set advances reps + 1 steps).  0 needs the operator
+ * built, 1-3 one solve, 4 one tracking call.  No counterpart in the reference
+ * (it has no timers). */
 int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms);
 
 #ifdef __cplusplus

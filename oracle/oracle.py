@@ -460,3 +460,55 @@ def closed_box_bc(u, v, p):
     nj, ni = u.shape[0] - 2, u.shape[1] - 2
     _step_lib().orc_closed_box_bc(ni, nj, _d(u), _d(v), _d(p))
     return u, v, p
+
+
+# ---------------------------------------------------------------------------
+# Lagrangian particles (Particles.f90:71-249)
+class ParticleParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("rho_l", "rho_g", "nu_l", "nu_g", "rho_p", "gx", "gy", "lref", "uref",
+                                               "us", "vs")]
+
+
+_part_declared = False
+
+
+def _part_lib():
+    global _part_declared
+    L = lib()
+    if not _part_declared:
+        dbl, ci = ctypes.c_double, ctypes.c_int
+        L.orc_particle_drag.argtypes = [dbl]
+        L.orc_particle_drag.restype = dbl
+        L.orc_particle_position.argtypes = [ci, ci, _D, _D, _D, _D, dbl, dbl, dbl, ctypes.POINTER(ci),
+                                            ctypes.POINTER(ci)]
+        L.orc_particle_track.argtypes = [ci, ci, _D, _D, _D, _D, _D, _D, _D, _D, ci, _D, _D, _D, _D, _D,
+                                         ctypes.POINTER(ParticleParams), dbl]
+        L.orc_particle_track.restype = ci
+        _part_declared = True
+    return L
+
+
+def particle_drag(re):
+    return _part_lib().orc_particle_drag(float(re))
+
+
+def particle_position(g, x, y, lref=1.0):
+    """ParticlePosition: 1-based (ii, jj), -1 where the point is outside."""
+    c = np.ascontiguousarray
+    ii, jj = ctypes.c_int(), ctypes.c_int()
+    _part_lib().orc_particle_position(g.ni, g.nj, _d(c(g.xc, dtype=np.float64)), _d(c(g.yc, dtype=np.float64)),
+                                      _d(c(g.dx, dtype=np.float64)), _d(c(g.dy, dtype=np.float64)), float(lref),
+                                      float(x), float(y), ctypes.byref(ii), ctypes.byref(jj))
+    return ii.value, jj.value
+
+
+def particle_track(g, vof, vofS, u, v, px, py, pu, pv, pd, params, dt):
+    """TrackingParticles over one step; returns (px, py, pu, pv, flagged)."""
+    c = np.ascontiguousarray
+    keep = [c(a, dtype=np.float64) for a in (g.xc, g.yc, g.dx, g.dy, vof, vofS, u, v)]
+    out = [np.array(a, dtype=np.float64, order="C") for a in (px, py, pu, pv)]
+    d = c(pd, dtype=np.float64)
+    P = ParticleParams(**{k: float(val) for k, val in params.items()})
+    bad = _part_lib().orc_particle_track(g.ni, g.nj, *[_d(a) for a in keep], len(d), *[_d(a) for a in out], _d(d),
+                                         ctypes.byref(P), float(dt))
+    return (*out, bad)

@@ -201,6 +201,23 @@ double orc_time_step(int ni, int nj, const double* udx, const double* vdy, const
                      double cfl, double Fr, double vint, double uref, double vs, double asy);
 void orc_closed_box_bc(int ni, int nj, double* u, double* v, double* p);
 
+/* sg_oracle_particles.c -- Lagrangian particles (Particles.f90:71-249) */
+typedef struct {
+    double rho_l, rho_g;  /* row, roa */
+    double nu_l, nu_g;    /* nuw, nua */
+    double rho_p;         /* rop */
+    double gx, gy;        /* gravity components */
+    double lref, uref;    /* PGrid%Lref, TVar%Uref */
+    double us, vs;        /* BoomCase%us, %vs */
+} orc_particle_params;
+double orc_particle_drag(double re);
+void orc_particle_position(int ni, int nj, const double* xc, const double* yc, const double* dx, const double* dy,
+                           double lref, double x, double y, int* ii, int* jj);
+int orc_particle_track(int ni, int nj, const double* xc, const double* yc, const double* dx, const double* dy,
+                       const double* vof, const double* vofS, const double* u, const double* v, int n, double* px,
+                       double* py, double* pu, double* pv, const double* pd, const orc_particle_params* P,
+                       double dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,11 @@ class PredictorInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ParticleParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("rho_l", "rho_g", "nu_l", "nu_g", "rho_p", "gx", "gy", "lref", "uref",
+                                               "us", "vs")]
+
+
 class ClsvofParams(ctypes.Structure):
     _fields_ = [("nsub", ctypes.c_int), ("depth", ctypes.c_double), ("amp", ctypes.c_double),
                 ("kw", ctypes.c_double), ("omega", ctypes.c_double)]
@@ -115,6 +120,9 @@ PROTOTYPES = {
                                ctypes.POINTER(PredictorInfo)], ctypes.c_int),
     "sg_predictor_rows": ([_CTX, _D, _D, _D, _D], ctypes.c_int),
     "sg_interface_to_uv_resident": ([_CTX], ctypes.c_int),
+    "sg_particles_set": ([_CTX, ctypes.c_int, _D, _D, _D, _D, _D], ctypes.c_int),
+    "sg_particles_get": ([_CTX, ctypes.c_int, _D, _D, _D, _D], ctypes.c_int),
+    "sg_particles_track": ([_CTX, ctypes.c_double, ctypes.POINTER(ParticleParams), _I], ctypes.c_int),
     "sg_upload_device": ([_CTX, ctypes.c_int, _V], ctypes.c_int),
     "sg_time_step_size": ([_CTX, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                            ctypes.c_double, _D], ctypes.c_int),

@@ -30,6 +30,7 @@ SOURCES = [
     "sg_nn.hip",
     "sg_learned.hip",
     "sg_step.hip",
+    "sg_particles.hip",
 ]
 EXTRA = {
     "sg_poisson.hip": ["-ffp-contract=off"],
@@ -37,6 +38,7 @@ EXTRA = {
     "sg_predictor.hip": ["-ffp-contract=off"],
     "sg_api.hip": ["-ffp-contract=off"],
     "sg_step.hip": ["-ffp-contract=off"],
+    "sg_particles.hip": ["-ffp-contract=off"],
     # FP32 atomic adds of the weight gradients as hardware atomics (device
     # memory, coarse-grained)
     "sg_nn.hip": ["-munsafe-fp-atomics"],

@@ -524,6 +524,64 @@ int sg_predictor_rows(sg_ctx* ctx, double* matr_u, double* rhs_u, double* matr_v
     return SG_OK;
 }
 
+int sg_particles_set(sg_ctx* ctx, int n, const double* x, const double* y, const double* u, const double* v,
+                     const double* d) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || n < 0 || (n > 0 && (!x || !y || !u || !v || !d))) {
+        set_error("sg_particles_set: need n >= 0 and five host arrays of n values");
+        return SG_ERR_ARG;
+    }
+    ParticleState& S = c->part;
+    if (n > S.cap) {
+        // grows only; earlier buffers are released with the context
+        const size_t b = (size_t)n * sizeof(double);
+        S.x = (double*)dev_alloc(c, b);
+        S.y = (double*)dev_alloc(c, b);
+        S.u = (double*)dev_alloc(c, b);
+        S.v = (double*)dev_alloc(c, b);
+        S.d = (double*)dev_alloc(c, b);
+        if (!S.flagged) S.flagged = (unsigned*)dev_alloc(c, sizeof(unsigned));
+        if (!S.x || !S.y || !S.u || !S.v || !S.d || !S.flagged) return SG_ERR_ALLOC;
+        S.cap = n;
+    }
+    S.n = n;
+    if (n == 0) return SG_OK;
+    const size_t b = (size_t)n * sizeof(double);
+    SG_HIP(hipMemcpyAsync(S.x, x, b, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(S.y, y, b, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(S.u, u, b, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(S.v, v, b, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(S.d, d, b, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_particles_get(sg_ctx* ctx, int n, double* x, double* y, double* u, double* v) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || n != c->part.n || (n > 0 && (!x || !y || !u || !v))) {
+        set_error("sg_particles_get: n must be the count given to sg_particles_set, with four host arrays");
+        return SG_ERR_ARG;
+    }
+    if (n == 0) return SG_OK;
+    const ParticleState& S = c->part;
+    const size_t b = (size_t)n * sizeof(double);
+    SG_HIP(hipMemcpyAsync(x, S.x, b, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipMemcpyAsync(y, S.y, b, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipMemcpyAsync(u, S.u, b, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipMemcpyAsync(v, S.v, b, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_particles_track(sg_ctx* ctx, double dt, const sg_particle_params* p, int* flagged) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !p || !(p->lref > 0.0) || !(p->uref > 0.0)) {
+        set_error("sg_particles_track: need parameters with lref, uref > 0");
+        return SG_ERR_ARG;
+    }
+    return particles_track(c, dt, p, flagged);
+}
+
 int sg_interface_to_uv_resident(sg_ctx* ctx) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     if (!c) return SG_ERR_ARG;
@@ -608,24 +666,11 @@ int sg_clsvof_advect(sg_ctx* ctx, double* vof, double* phi, double* nx, double* 
     return SG_OK;
 }
 
-int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms) {
-    Ctx* c = reinterpret_cast<Ctx*>(ctx);
-    if (!c || !avg_ms || reps <= 0) return SG_ERR_ARG;
-    if (!c->built) {
-        set_error("sg_time_kernel: build the operator first");
-        return SG_ERR_STATE;
-    }
-    if (which < 0 || which > 3) {
-        set_error("sg_time_kernel: unknown kernel id");
-        return SG_ERR_ARG;
-    }
-    if (which > 0 && !c->mg_ready) {
-        set_error("sg_time_kernel: solve once before timing the cycle kernels");
-        return SG_ERR_STATE;
-    }
-    // 0: fine operator apply y = A x (PCG x -> level-0 residual buffer);
-    // 1/2: fine down/up sweep of the V-cycle; 3: plain copy (calibration)
-    auto launch = [&]() { return which == 0 ? poisson_apply_fine(c, c->x, c->lev[0].v.r) : mg_launch_fine(c, which); };
+}  // extern "C"
+
+// one untimed launch, then reps launches between two events on the context's stream
+template <class L>
+static int time_launches(sg::Ctx* c, int reps, double* avg_ms, L launch) {
     int rc = launch();
     if (rc) return rc;
     SG_HIP(hipEventRecord(c->ev0, c->stream));
@@ -639,6 +684,39 @@ int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms) {
     SG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *avg_ms = (double)ms / reps;
     return SG_OK;
+}
+
+extern "C" {
+
+int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !avg_ms || reps <= 0) return SG_ERR_ARG;
+    if (which == 4) {
+        // the particle tracker on the current set, with the last call's dt
+        // and parameters (the set advances reps + 1 steps)
+        if (!c->part.tracked) {
+            set_error("sg_time_kernel: track the particles once before timing them");
+            return SG_ERR_STATE;
+        }
+        return time_launches(c, reps, avg_ms, [&]() { return particles_launch(c, c->part.last_dt, &c->part.last); });
+    }
+    if (!c->built) {
+        set_error("sg_time_kernel: build the operator first");
+        return SG_ERR_STATE;
+    }
+    if (which < 0 || which > 4) {
+        set_error("sg_time_kernel: unknown kernel id");
+        return SG_ERR_ARG;
+    }
+    if (which > 0 && !c->mg_ready) {
+        set_error("sg_time_kernel: solve once before timing the cycle kernels");
+        return SG_ERR_STATE;
+    }
+    // 0: fine operator apply y = A x (PCG x -> level-0 residual buffer);
+    // 1/2: fine down/up sweep of the V-cycle; 3: plain copy (calibration)
+    return time_launches(c, reps, avg_ms, [&]() {
+        return which == 0 ? poisson_apply_fine(c, c->x, c->lev[0].v.r) : mg_launch_fine(c, which);
+    });
 }
 
 }  // extern "C"

@@ -183,6 +183,15 @@ struct Net;
 }
 
 // Learned warm start / deflation vector of the projection PCG.
+struct ParticleState {
+    int n = 0, cap = 0;
+    double *x = nullptr, *y = nullptr, *u = nullptr, *v = nullptr, *d = nullptr;
+    unsigned* flagged = nullptr;
+    sg_particle_params last{};  // parameters and dt of the last tracking call
+    double last_dt = 0.0;
+    bool tracked = false;
+};
+
 struct NnState {
     nn::Net* net = nullptr;  // inference net (batch 1) on level `level`
     int level = 0;
@@ -232,6 +241,7 @@ struct Ctx {
     hipGraphExec_t iter_graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     int graph_pre[2] = {0, 0}, graph_post[2] = {0, 0}, graph_defl[2] = {0, 0}, graph_kry[2] = {0, 0};
     NnState nn;
+    ParticleState part;
     double* step_dev = nullptr;   // device scalars of the time-step control (dt)
     hipEvent_t ev_nn0 = nullptr, ev_nn1 = nullptr;
     int use_graph = 1;
@@ -305,6 +315,8 @@ int closed_box_bc(Ctx* c);
 int predictor_allocate(Ctx* c);
 int predictor_assemble(Ctx* c, double dt, double vb, int first_step);
 int predictor_finish(Ctx* c);
+int particles_track(Ctx* c, double dt, const sg_particle_params* p, int* flagged);
+int particles_launch(Ctx* c, double dt, const sg_particle_params* p);  // no host sync
 int momentum_solve(Ctx* c, const double* m, const double* b, double* x, double tol, int max_iter, int* iters,
                    double* relres);
 // interface transport (sg_clsvof.hip)

@@ -243,6 +243,29 @@ class Projection:
                                          mv.ctypes.data_as(_DP), rv.ctypes.data_as(_DP)))
         return mu, ru, mv, rv
 
+    # -- Lagrangian particles (TrackingParticles, Particles.f90:71-205) ----
+    def particles_set(self, x, y, u, v, d):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, u, v, d)]
+        n = len(arrs[0])
+        if any(len(a) != n for a in arrs):
+            raise ValueError("particle arrays must have one length")
+        self._npart = n
+        check(self.lib.sg_particles_set(self.ctx, n, *[_d(a)[1] for a in arrs]))
+
+    def particles_get(self):
+        n = getattr(self, "_npart", 0)
+        out = [np.zeros(n) for _ in range(4)]
+        check(self.lib.sg_particles_get(self.ctx, n, *[_d(a)[1] for a in out]))
+        return tuple(out)
+
+    def particles_track(self, dt, params):
+        """params: dict of sg_particle_params fields; returns the number of
+        particles that met the reference's failure test."""
+        p = _lib.ParticleParams(**{k: float(v) for k, v in params.items()})
+        flagged = ctypes.c_int(0)
+        check(self.lib.sg_particles_track(self.ctx, float(dt), ctypes.byref(p), ctypes.byref(flagged)))
+        return flagged.value
+
     def interface_to_uv(self):
         """Initial_ClsVofUV on the device fields; returns the U and V
         (vof, phi, nx, ny)."""
