@@ -19,6 +19,8 @@ Three solver paths run the same steps:
     (the net is trained inside this run, with the library's own kernels, on
     (rhs, p) pairs the plain solver makes for seeded 512x512 swarms; it runs
     on the 256x256 level of the hierarchy).
+A fourth full-step path, reported beside them, runs the mixed solver from the
+last step's pressure instead of zero.
 ``value`` is the learned + mixed path's Mcell-steps/s over full time steps
 (the headline metric); ``poisson_ms_per_step`` gives the Poisson solve time
 per step of every path, inside those steps and, separately, on a fixed
@@ -174,7 +176,7 @@ def train_warm_start(seed, width):
                   "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]}
 
 
-def stepping(case, stags, precision, steps, warmup, learned=None, width=256):
+def stepping(case, stags, precision, steps, warmup, learned=None, width=256, previous=False):
     """K timed full time steps after W warm-up steps (TimeStepper), with the
     Poisson device time of each step from its solve report."""
     from simcoflow_amd.solver import TimeStepper
@@ -184,7 +186,7 @@ def stepping(case, stags, precision, steps, warmup, learned=None, width=256):
         # deflation needs FP64 directions; the mixed solve takes the warm start
         lrn = (w, None, K, precision == 0)
     st = TimeStepper(case, precision=precision, tol=1e-8, max_iter=200, learned=lrn, stags=stags,
-                     learned_width=width)
+                     learned_width=width, previous_pressure=previous)
     try:
         for _ in range(warmup):
             st.step()
@@ -363,6 +365,11 @@ def main():
     for name, prec, lrn in paths:
         log(f"full time steps, {name}")
         full[name] = stepping(case, stags, prec, a.steps, a.warmup, learned=lrn, width=a.nn_width)
+    # not the metric's path: mixed precision started from the last step's
+    # pressure (full steps only; on fixed fields every repeat would start
+    # from the answer)
+    log("full time steps, mixed from the previous pressure")
+    full["mixed_previous_pressure"] = stepping(case, stags, 1, a.steps, a.warmup, previous=True)
     # the projection alone on one fixed set of predictor fields
     proj = {}
     for name, prec, lrn in paths:

@@ -79,6 +79,7 @@ module simcogpu
   public :: sg_poisson_build_resident, sg_poisson_solve_resident, sg_poisson_solve
   public :: sg_poisson_matrix, sg_poisson_rhs, sg_poisson_apply, sg_velocity_correct_resident
   public :: sg_particles_set, sg_particles_get, sg_particles_track
+  public :: sg_state_bytes, sg_state_save, sg_state_load
   public :: sg_error_message
 
   interface
@@ -233,6 +234,24 @@ module simcogpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, x, y, u, v
       integer(c_int), value :: n
+      integer(c_int) :: rc
+    end function
+    function sg_state_bytes(ctx, bytes) bind(C, name="sg_state_bytes") result(rc)
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_size_t), intent(out) :: bytes
+      integer(c_int) :: rc
+    end function
+    function sg_state_save(ctx, buf, bytes) bind(C, name="sg_state_save") result(rc)
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx, buf
+      integer(c_size_t), value :: bytes
+      integer(c_int) :: rc
+    end function
+    function sg_state_load(ctx, buf, bytes) bind(C, name="sg_state_load") result(rc)
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx, buf
+      integer(c_size_t), value :: bytes
       integer(c_int) :: rc
     end function
     function sg_particles_track(ctx, dt, p, flagged) bind(C, name="sg_particles_track") result(rc)

@@ -301,6 +301,21 @@ int sg_particles_set(sg_ctx* ctx, int n, const double* x, const double* y, const
 int sg_particles_get(sg_ctx* ctx, int n, double* x, double* y, double* u, double* v);
 int sg_particles_track(sg_ctx* ctx, double dt, const sg_particle_params* p, int* flagged);
 
+/* ---- checkpoint (replaces the Tecplot-ASCII restart of RunAgain,
+ * Solver_Mod.f90:164-195 with ReadOldData*, PrintResult.f90:192-270) -------
+ * The state is what a time step reads from the one before it: the P and U/V
+ * interfaces (VOF PHI NX NY, UVOF VVOF UPHI UNX UNY VPHI VNX VNY), U V P,
+ * FLUXN1U FLUXN1V, the Poisson solution and the two momentum solutions (the
+ * starts of the next solves) and the particle set, in FP64 as held, so a run
+ * resumed from it repeats the uninterrupted run bit for bit.  Not included:
+ * the geometry (set it up first, as the reference recomputes it), the
+ * parameters, and the host's step counter and time.  sg_state_bytes gives
+ * the size of the current state; sg_state_load takes a state saved from a
+ * context of the same grid (grows the particle buffers if needed). */
+int sg_state_bytes(sg_ctx* ctx, size_t* bytes);
+int sg_state_save(sg_ctx* ctx, void* buf, size_t bytes);
+int sg_state_load(sg_ctx* ctx, const void* buf, size_t bytes);
+
 /* ---- measurement ----------------------------------------------------------
  * Average device time (HIP events on the context's stream) of `reps`
  * back-to-back launches of one kernel on the current data: which = 0 the

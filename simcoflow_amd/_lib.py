@@ -123,6 +123,9 @@ PROTOTYPES = {
     "sg_particles_set": ([_CTX, ctypes.c_int, _D, _D, _D, _D, _D], ctypes.c_int),
     "sg_particles_get": ([_CTX, ctypes.c_int, _D, _D, _D, _D], ctypes.c_int),
     "sg_particles_track": ([_CTX, ctypes.c_double, ctypes.POINTER(ParticleParams), _I], ctypes.c_int),
+    "sg_state_bytes": ([_CTX, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "sg_state_save": ([_CTX, _V, ctypes.c_size_t], ctypes.c_int),
+    "sg_state_load": ([_CTX, _V, ctypes.c_size_t], ctypes.c_int),
     "sg_upload_device": ([_CTX, ctypes.c_int, _V], ctypes.c_int),
     "sg_time_step_size": ([_CTX, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                            ctypes.c_double, _D], ctypes.c_int),

@@ -31,6 +31,7 @@ SOURCES = [
     "sg_learned.hip",
     "sg_step.hip",
     "sg_particles.hip",
+    "sg_state.hip",
 ]
 EXTRA = {
     "sg_poisson.hip": ["-ffp-contract=off"],

@@ -532,18 +532,8 @@ int sg_particles_set(sg_ctx* ctx, int n, const double* x, const double* y, const
         return SG_ERR_ARG;
     }
     ParticleState& S = c->part;
-    if (n > S.cap) {
-        // grows only; earlier buffers are released with the context
-        const size_t b = (size_t)n * sizeof(double);
-        S.x = (double*)dev_alloc(c, b);
-        S.y = (double*)dev_alloc(c, b);
-        S.u = (double*)dev_alloc(c, b);
-        S.v = (double*)dev_alloc(c, b);
-        S.d = (double*)dev_alloc(c, b);
-        if (!S.flagged) S.flagged = (unsigned*)dev_alloc(c, sizeof(unsigned));
-        if (!S.x || !S.y || !S.u || !S.v || !S.d || !S.flagged) return SG_ERR_ALLOC;
-        S.cap = n;
-    }
+    const int rc = particles_reserve(c, n);
+    if (rc) return rc;
     S.n = n;
     if (n == 0) return SG_OK;
     const size_t b = (size_t)n * sizeof(double);

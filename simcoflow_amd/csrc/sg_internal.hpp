@@ -317,6 +317,7 @@ int predictor_assemble(Ctx* c, double dt, double vb, int first_step);
 int predictor_finish(Ctx* c);
 int particles_track(Ctx* c, double dt, const sg_particle_params* p, int* flagged);
 int particles_launch(Ctx* c, double dt, const sg_particle_params* p);  // no host sync
+int particles_reserve(Ctx* c, int n);  // grows the particle buffers to n
 int momentum_solve(Ctx* c, const double* m, const double* b, double* x, double tol, int max_iter, int* iters,
                    double* relres);
 // interface transport (sg_clsvof.hip)

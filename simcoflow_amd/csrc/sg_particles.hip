@@ -129,6 +129,25 @@ __global__ void __launch_bounds__(256) k_particles(PGeo G, int n, double* __rest
 
 }  // namespace
 
+int particles_reserve(Ctx* c, int n) {
+    ParticleState& S = c->part;
+    if (n <= S.cap) return SG_OK;
+    // grows only; earlier buffers are released with the context
+    const size_t b = (size_t)n * sizeof(double);
+    S.x = (double*)dev_alloc(c, b);
+    S.y = (double*)dev_alloc(c, b);
+    S.u = (double*)dev_alloc(c, b);
+    S.v = (double*)dev_alloc(c, b);
+    S.d = (double*)dev_alloc(c, b);
+    if (!S.flagged) S.flagged = (unsigned*)dev_alloc(c, sizeof(unsigned));
+    if (!S.x || !S.y || !S.u || !S.v || !S.d || !S.flagged) {
+        set_error("particle buffers: device allocation failed");
+        return SG_ERR_ALLOC;
+    }
+    S.cap = n;
+    return SG_OK;
+}
+
 int particles_launch(Ctx* c, double dt, const sg_particle_params* prm) {
     ParticleState& S = c->part;
     if (S.n == 0) return SG_OK;

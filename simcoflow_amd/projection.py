@@ -266,6 +266,21 @@ class Projection:
         check(self.lib.sg_particles_track(self.ctx, float(dt), ctypes.byref(p), ctypes.byref(flagged)))
         return flagged.value
 
+    def state_save(self):
+        """The context's time-stepping state (sg_state_save) as bytes."""
+        n = ctypes.c_size_t(0)
+        check(self.lib.sg_state_bytes(self.ctx, ctypes.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        check(self.lib.sg_state_save(self.ctx, buf.ctypes.data_as(ctypes.c_void_p), n.value))
+        return buf.tobytes()
+
+    def state_load(self, blob):
+        """Restores a state saved by state_save on a context of the same grid."""
+        buf = np.frombuffer(blob, dtype=np.uint8)
+        check(self.lib.sg_state_load(self.ctx, buf.ctypes.data_as(ctypes.c_void_p), buf.size))
+        # the particle count is the header's last field (bytes 168-175)
+        self._npart = int(np.frombuffer(blob[168:176], dtype=np.int64)[0])
+
     def interface_to_uv(self):
         """Initial_ClsVofUV on the device fields; returns the U and V
         (vof, phi, nx, ny)."""

@@ -37,18 +37,23 @@ def physics(fluids):
 
 class TimeStepper:
     def __init__(self, case, precision=_lib.SG_PREC_FP64, tol=None, cfl=0.25, vint=0.0, uref=1.0, max_iter=500,
-                 learned=None, stags=None, learned_width=256):
+                 learned=None, stags=None, learned_width=256, previous_pressure=False):
         """learned: (weights, level, out_scale, deflate) of a trained warm
         start, level None for the first multigrid level no wider than
         learned_width; stags: the (U, V) cases.StagGeometry if already
-        computed."""
+        computed; previous_pressure: start each Poisson solve from the last
+        step's pressure instead of zero (the reference starts from zero,
+        ProjectionP_Mod.f90:319; it solves for the full pressure, so the
+        last one is a close start)."""
+        if learned and previous_pressure:
+            raise ValueError("the learned warm start and the previous-pressure start exclude each other")
         self.case = case
         g = case.grid
         self.cfl, self.vint, self.uref = cfl, vint, uref
         ph = physics(case.fluids)
         self.params = params_for(case.fluids, tol=case.tol if tol is None else tol, precision=precision,
                                  max_iter=max_iter, bc=case.bc, Re=ph["Re"], Fr=ph["Fr"],
-                                 warm_start=2 if learned else 0)
+                                 warm_start=2 if learned else (1 if previous_pressure else 0))
         # params_for takes nu from the fluids already
         self.pj = pj = Projection(g, self.params)
         pj.set_geometry(case.geo)
@@ -95,6 +100,27 @@ class TimeStepper:
         self.time += dt
         self.last = dict(dt=dt, predictor=pinfo.as_dict(), poisson=sinfo.as_dict())
         return self.last
+
+    def save(self, path):
+        """Checkpoint of the run (device state, step counter, time)."""
+        from .output import save_checkpoint
+        save_checkpoint(path, self.pj, self.itt, self.time)
+
+    def load(self, path):
+        """Resumes from a checkpoint written by save on the same case."""
+        from .output import load_checkpoint
+        meta = load_checkpoint(path, self.pj)
+        self.itt, self.time = meta["itt"], meta["time"]
+
+    def write_vtr(self, directory, fmt="ascii"):
+        """Print_Result_VTK_2D's file for the current step
+        (<directory>/Paraview/FlowField_<itt>.vtr); returns its path."""
+        from .output import vtr_name, write_vtr
+        path = vtr_name(directory, self.itt)
+        st = self.state()
+        geo = self.case.geo
+        write_vtr(path, self.case.grid, st["u"], st["v"], st["p"], st["phi"], geo.phiS, st["vof"], geo.vofS, fmt=fmt)
+        return path
 
     def state(self):
         """Host copies of the interface and the velocity / pressure fields."""
