@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip training and the learned path")
     ap.add_argument("--no-full-step", action="store_true", help="skip the per-part step timing")
-    ap.add_argument("--no-particles", action="store_true", help="skip the particle tracker timing")
+    ap.add_argument("--no-particles", action="store_true", help="skip the particle and diagnostics timing")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--nn-width", type=int, default=256, help="widest multigrid level the net runs on")
     return ap.parse_args()
@@ -255,14 +255,16 @@ def step_parts(case, stags, precision, steps=3, warmup=2):
 
 PARTICLE_STREAM_BYTES = 72  # x, y, u, v, d read + x, y, u, v written, FP64
 PARTICLE_GATHER_BYTES = 48  # vof, vofS, two u faces, two v faces of the particle's cell
+DIAG_BYTES_PER_CELL = 40  # vof, vofS, mres, u, v read once, FP64
 
 
 def particles(case, fields, n=1 << 20, reps=20, seed=0):
     """The Lagrangian tracker (TrackingParticles, 10 sub-steps of the
     exponential drag integrator) on n particles spread uniformly over the
     swarm box, diameters log-normal about 50 um (sigma 0.5), density 2500
-    kg/m^3, in the seeded predictor velocity field; the kernel's average
-    device time over reps launches (HIP events on the context's stream)."""
+    kg/m^3, in the seeded predictor velocity field, and one row of the
+    per-step diagnostics on the same fields; average device times over reps
+    launches (HIP events on the context's stream)."""
     from simcoflow_amd import _lib
     from simcoflow_amd.projection import Projection, params_for
     g = case.grid
@@ -284,13 +286,21 @@ def particles(case, fields, n=1 << 20, reps=20, seed=0):
         flagged = pj.particles_track(case.dt, prm)
         ms = ctypes.c_double()
         _lib.check(pj.lib.sg_time_kernel(pj.ctx, 4, reps, ctypes.byref(ms)))
+        dms = ctypes.c_double()
+        _lib.check(pj.lib.sg_time_kernel(pj.ctx, 5, reps, ctypes.byref(dms)))
     finally:
         pj.close()
     b = n * (PARTICLE_STREAM_BYTES + PARTICLE_GATHER_BYTES)
     gbs = b / (ms.value * 1e-3) / 1e9
-    return {"kernel": "k_particles (TrackingParticles, FP64, one thread per particle)", "particles": n,
+    part = {"kernel": "k_particles (TrackingParticles, FP64, one thread per particle)", "particles": n,
             "avg_ms": round(ms.value, 5), "mparticle_steps_per_s": round(n / (ms.value * 1e-3) / 1e6, 1),
             "bytes_alg": int(b), "GBs": round(gbs, 1), "frac_hbm": round(gbs / PEAK_HBM_GBS, 4), "flagged": flagged}
+    db = g.ni * g.nj * DIAG_BYTES_PER_CELL
+    dgbs = db / (dms.value * 1e-3) / 1e9
+    diag = {"kernel": "k_diag + k_diag_fin (one diagnostics row: volumes, gas centroid, max |mres|, |u|, |v|)",
+            "avg_ms": round(dms.value, 5), "bytes_alg": int(db), "GBs": round(dgbs, 1),
+            "frac_hbm": round(dgbs / PEAK_HBM_GBS, 4)}
+    return part, diag
 
 
 def measured_traffic(kernel_name):
@@ -441,8 +451,8 @@ def main():
                           "fp64_apply": proj["fp64"]["kernel"].get("apply"),
                           "copy": proj["fp64"]["kernel"].get("copy")}
     if not a.no_particles:
-        log("particle tracking")
-        out["particles"] = particles(case, fields)
+        log("particle tracking and diagnostics")
+        out["particles"], out["diagnostics"] = particles(case, fields)
     if not a.no_full_step:
         log("time step parts")
         out["step_parts_ms"] = {"mixed": step_parts(case, stags, 1)}

@@ -301,6 +301,28 @@ int sg_particles_set(sg_ctx* ctx, int n, const double* x, const double* y, const
 int sg_particles_get(sg_ctx* ctx, int n, double* x, double* y, double* u, double* v);
 int sg_particles_track(sg_ctx* ctx, double dt, const sg_particle_params* p, int* flagged);
 
+/* ---- per-step diagnostics without host synchronisation -------------------
+ * sg_diagnostics_record enqueues, on the context's stream, two kernels that
+ * reduce the current device fields (deterministic two-phase sums and
+ * maxima) into one row of a device ring of SG_DIAG_RING rows; nothing waits
+ * for them.  Row columns:
+ *   0 dt of the last sg_time_step_size (ComputeTimeStep, Solver_Mod.f90:321)
+ *   1 liquid volume  sum vof dx dy
+ *   2 gas volume     sum (1 - vof - vofS) dx dy
+ *   3, 4 gas centroid (x, y), gas-volume weighted cell centres
+ *   5 max |mres|: the mass residual of the last velocity correction
+ *     (ComputePUV_Mod.f90:184-217)
+ *   6, 7 max |u|, max |v| over the faces U(1:ni,1:nj), V(1:ni,1:nj)
+ * (maxima propagate NaN).  sg_diagnostics_read waits for the stream and
+ * copies the oldest rows not yet read, at most max_rows, in recording
+ * order; n_lost (may be NULL) counts rows overwritten before they were
+ * read.  The reference writes its per-step history from the host
+ * (PrintHistory, Solver_Mod.f90:346-356). */
+#define SG_DIAG_COLS 8
+#define SG_DIAG_RING 4096
+int sg_diagnostics_record(sg_ctx* ctx);
+int sg_diagnostics_read(sg_ctx* ctx, int max_rows, double* rows, int* n_rows, long long* n_lost);
+
 /* ---- checkpoint (replaces the Tecplot-ASCII restart of RunAgain,
  * Solver_Mod.f90:164-195 with ReadOldData*, PrintResult.f90:192-270) -------
  * The state is what a time step reads from the one before it: the P and U/V
@@ -323,9 +345,9 @@ int sg_state_load(sg_ctx* ctx, const void* buf, size_t bytes);
  * down-sweep, 2 the fine-level up-sweep (with the fused PCG dots), 3 a plain
  * copy of one padded fine-level vector (counter calibration), 4 the particle
  * tracker on the current particle set with the last sg_particles_track call's
- * dt and parameters (the set advances reps + 1 steps).  0 needs the operator
- * built, 1-3 one solve, 4 one tracking call.  No counterpart in the reference
- * (it has no timers). */
+ * dt and parameters (the set advances reps + 1 steps), 5 sg_diagnostics_record
+ * (writes reps + 1 rows).  0 needs the operator built, 1-3 one solve, 4 one
+ * tracking call.  No counterpart in the reference (it has no timers). */
 int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms);
 
 #ifdef __cplusplus

@@ -83,6 +83,12 @@ class ClsvofParams(ctypes.Structure):
 
 
 # exported symbol -> (argtypes, restype); kept in step with include/simcogpu.h
+# per-step diagnostics ring (SG_DIAG_COLS, SG_DIAG_RING); the columns
+DIAG_COLS = 8
+DIAG_RING = 4096
+DIAG_NAMES = ("dt", "liquid_volume", "gas_volume", "gas_centroid_x", "gas_centroid_y", "max_abs_mres", "max_abs_u",
+              "max_abs_v")
+
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 _V = ctypes.c_void_p
@@ -123,6 +129,8 @@ PROTOTYPES = {
     "sg_particles_set": ([_CTX, ctypes.c_int, _D, _D, _D, _D, _D], ctypes.c_int),
     "sg_particles_get": ([_CTX, ctypes.c_int, _D, _D, _D, _D], ctypes.c_int),
     "sg_particles_track": ([_CTX, ctypes.c_double, ctypes.POINTER(ParticleParams), _I], ctypes.c_int),
+    "sg_diagnostics_record": ([_CTX], ctypes.c_int),
+    "sg_diagnostics_read": ([_CTX, ctypes.c_int, _D, _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "sg_state_bytes": ([_CTX, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "sg_state_save": ([_CTX, _V, ctypes.c_size_t], ctypes.c_int),
     "sg_state_load": ([_CTX, _V, ctypes.c_size_t], ctypes.c_int),

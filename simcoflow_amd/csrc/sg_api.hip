@@ -215,6 +215,10 @@ int sg_ctx_create(int ni, int nj, const double* dx, const double* dy, const doub
     c->sc = (PcgScalars*)dev_alloc(c, sizeof(PcgScalars));
     c->step_dev = (double*)dev_alloc(c, 4 * sizeof(double));
     if (!c->red.partials || !c->red.tickets || !c->sc || !c->step_dev) return fail_alloc();
+    c->diag.ring.rows = (double*)dev_alloc(c, sizeof(double) * SG_DIAG_RING * SG_DIAG_COLS);
+    c->diag.ring.count = (long long*)dev_alloc(c, sizeof(long long));
+    c->diag.partials = (double*)dev_alloc(c, sizeof(double) * 7 * DIAG_BLOCKS);
+    if (!c->diag.ring.rows || !c->diag.ring.count || !c->diag.partials) return fail_alloc();
     CK(hipHostMalloc((void**)&c->sc_host, sizeof(PcgScalars), hipHostMallocDefault));
     std::memset(c->sc_host, 0, sizeof(PcgScalars));
     // a geometry with no solid: open apertures, fluid everywhere
@@ -563,6 +567,21 @@ int sg_particles_get(sg_ctx* ctx, int n, double* x, double* y, double* u, double
     return SG_OK;
 }
 
+int sg_diagnostics_record(sg_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    return diagnostics_record(c);
+}
+
+int sg_diagnostics_read(sg_ctx* ctx, int max_rows, double* rows, int* n_rows, long long* n_lost) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || max_rows < 0 || (max_rows > 0 && !rows) || !n_rows) {
+        set_error("sg_diagnostics_read: need max_rows >= 0, a row buffer and n_rows");
+        return SG_ERR_ARG;
+    }
+    return diagnostics_read(c, max_rows, rows, n_rows, n_lost);
+}
+
 int sg_particles_track(sg_ctx* ctx, double dt, const sg_particle_params* p, int* flagged) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     if (!c || !p || !(p->lref > 0.0) || !(p->uref > 0.0)) {
@@ -690,11 +709,12 @@ int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms) {
         }
         return time_launches(c, reps, avg_ms, [&]() { return particles_launch(c, c->part.last_dt, &c->part.last); });
     }
+    if (which == 5) return time_launches(c, reps, avg_ms, [&]() { return diagnostics_record(c); });
     if (!c->built) {
         set_error("sg_time_kernel: build the operator first");
         return SG_ERR_STATE;
     }
-    if (which < 0 || which > 4) {
+    if (which < 0 || which > 5) {
         set_error("sg_time_kernel: unknown kernel id");
         return SG_ERR_ARG;
     }

@@ -108,6 +108,18 @@ struct PcgScalars {
     int guard;      // bit 0: NaN, bit 1: diag/rhs guard tripped
 };
 
+// ring of per-step diagnostics rows written on the device
+constexpr int DIAG_BLOCKS = 1024;
+struct DiagRing {
+    double* rows = nullptr;       // [SG_DIAG_RING][SG_DIAG_COLS]
+    long long* count = nullptr;   // rows written since the context was made
+};
+struct DiagState {
+    DiagRing ring;
+    double* partials = nullptr;   // [7][DIAG_BLOCKS]
+    long long read = 0;           // rows handed to the host
+};
+
 struct Reducer {
     double* partials = nullptr;  // [NRED][max_blocks]
     unsigned int* tickets = nullptr;
@@ -243,6 +255,7 @@ struct Ctx {
     NnState nn;
     ParticleState part;
     double* step_dev = nullptr;   // device scalars of the time-step control (dt)
+    DiagState diag;
     hipEvent_t ev_nn0 = nullptr, ev_nn1 = nullptr;
     int use_graph = 1;
     ClsBuffers cls;
@@ -310,6 +323,8 @@ void learned_release(Ctx* c);
 // time-step control and boundary layer (sg_step.hip)
 int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, double asy, double* dt);
 int closed_box_bc(Ctx* c);
+int diagnostics_record(Ctx* c);
+int diagnostics_read(Ctx* c, int max_rows, double* rows, int* n_rows, long long* n_lost);
 
 // momentum predictor (sg_predictor.hip, sg_momentum.hip)
 int predictor_allocate(Ctx* c);

@@ -5,6 +5,7 @@
 // the ghosts, p = 0 above the open top, no flow through the walls).
 // Compiled without FP contraction: the bounds round exactly as the
 // reference's expressions, so the step equals the oracle's bit for bit.
+#include <algorithm>
 #include <cmath>
 
 #include "sg_internal.hpp"
@@ -14,6 +15,7 @@ namespace sg {
 namespace {
 
 constexpr int NTS = 256;
+constexpr int NDP = 7;             // partials per block of the diagnostics
 
 struct StepIn {
     int ni, nj;
@@ -96,7 +98,128 @@ __global__ void __launch_bounds__(NTS) k_box_bc_rows(int ni, int nj, double* __r
     }
 }
 
+// NaN-propagating maximum (a NaN in a field shows in the diagnostics)
+__device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b : a; }
+
+struct DiagIn {
+    int ni, nj;
+    const double *dx, *dy, *xc, *yc;
+    const double *vof, *vofS, *mres;  // cell arrays
+    const double *u, *v;              // ghosted
+};
+
+// per-block partial sums (liquid volume, gas volume, gas first moments in x
+// and y) and maxima (|mres|, |u|, |v|) over a grid-stride range of cells
+__global__ void __launch_bounds__(NTS) k_diag(DiagIn a, double* part, int nb) {
+    __shared__ double red[NDP][NTS / 64];
+    double v[NDP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int n = a.ni * a.nj;
+    for (int e = blockIdx.x * NTS + threadIdx.x; e < n; e += gridDim.x * NTS) {
+        const int i = e % a.ni, j = e / a.ni;
+        const double area = a.dx[i] * a.dy[j];
+        const double f = a.vof[e], gas = 1.0 - f - a.vofS[e];
+        v[0] += f * area;
+        v[1] += gas * area;
+        v[2] += gas * area * a.xc[i];
+        v[3] += gas * area * a.yc[j];
+        const size_t g = (size_t)(i + 1) + (size_t)(j + 1) * (size_t)(a.ni + 2);
+        v[4] = nmax(v[4], fabs(a.mres[e]));
+        v[5] = nmax(v[5], fabs(a.u[g]));
+        v[6] = nmax(v[6], fabs(a.v[g]));
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        for (int k = 0; k < 4; ++k) v[k] += __shfl_xor(v[k], o, 64);
+        for (int k = 4; k < NDP; ++k) v[k] = nmax(v[k], __shfl_xor(v[k], o, 64));
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < NDP; ++k) red[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+    if (threadIdx.x < NDP) {
+        const int k = threadIdx.x;
+        double r = red[k][0];
+        for (int w = 1; w < NTS / 64; ++w) r = k < 4 ? r + red[k][w] : nmax(r, red[k][w]);
+        part[(size_t)k * nb + blockIdx.x] = r;
+    }
+}
+
+// one block: the partials in block order, then one row of the ring
+__global__ void __launch_bounds__(NTS) k_diag_fin(const double* part, int nb, const double* dt, DiagRing ring) {
+    __shared__ double red[NDP][NTS / 64];
+    double v[NDP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += NTS)
+        for (int k = 0; k < NDP; ++k) v[k] = k < 4 ? v[k] + part[(size_t)k * nb + b] : nmax(v[k], part[(size_t)k * nb + b]);
+    for (int o = 32; o >= 1; o >>= 1) {
+        for (int k = 0; k < 4; ++k) v[k] += __shfl_xor(v[k], o, 64);
+        for (int k = 4; k < NDP; ++k) v[k] = nmax(v[k], __shfl_xor(v[k], o, 64));
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < NDP; ++k) red[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r[NDP];
+        for (int k = 0; k < NDP; ++k) {
+            r[k] = red[k][0];
+            for (int w = 1; w < NTS / 64; ++w) r[k] = k < 4 ? r[k] + red[k][w] : nmax(r[k], red[k][w]);
+        }
+        const long long cnt = ring.count[0];
+        double* row = ring.rows + (size_t)(cnt % SG_DIAG_RING) * SG_DIAG_COLS;
+        row[0] = dt[0];
+        row[1] = r[0];
+        row[2] = r[1];
+        row[3] = r[1] > 0.0 ? r[2] / r[1] : 0.0;
+        row[4] = r[1] > 0.0 ? r[3] / r[1] : 0.0;
+        row[5] = r[4];
+        row[6] = r[5];
+        row[7] = r[6];
+        ring.count[0] = cnt + 1;
+    }
+}
+
 }  // namespace
+
+int diagnostics_record(Ctx* c) {
+    const DiagIn a{c->ni,
+                   c->nj,
+                   c->dx,
+                   c->dy,
+                   c->cls.xc,
+                   c->cls.yc,
+                   (const double*)c->field[SG_F_VOF],
+                   (const double*)c->field[SG_F_VOFS],
+                   (const double*)c->field[SG_F_MRES],
+                   (const double*)c->field[SG_F_U],
+                   (const double*)c->field[SG_F_V]};
+    int nb = (c->ni * c->nj + NTS - 1) / NTS;
+    if (nb > DIAG_BLOCKS) nb = DIAG_BLOCKS;
+    hipLaunchKernelGGL(k_diag, dim3(nb), dim3(NTS), 0, c->stream, a, c->diag.partials, nb);
+    hipLaunchKernelGGL(k_diag_fin, dim3(1), dim3(NTS), 0, c->stream, (const double*)c->diag.partials, nb,
+                       (const double*)c->step_dev, c->diag.ring);
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+int diagnostics_read(Ctx* c, int max_rows, double* rows, int* n_rows, long long* n_lost) {
+    long long cnt = 0;
+    SG_HIP(hipMemcpyAsync(&cnt, c->diag.ring.count, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    long long first = c->diag.read;
+    long long lost = 0;
+    if (cnt - first > SG_DIAG_RING) {
+        lost = cnt - first - SG_DIAG_RING;
+        first = cnt - SG_DIAG_RING;
+    }
+    const long long n = std::min<long long>(cnt - first, (long long)max_rows);
+    for (long long r = 0; r < n; ++r) {
+        const long long slot = (first + r) % SG_DIAG_RING;
+        SG_HIP(hipMemcpyAsync(rows + r * SG_DIAG_COLS, c->diag.ring.rows + slot * SG_DIAG_COLS,
+                              SG_DIAG_COLS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    SG_HIP(hipStreamSynchronize(c->stream));
+    c->diag.read = first + n;
+    *n_rows = (int)n;
+    if (n_lost) *n_lost = lost;
+    return SG_OK;
+}
 
 int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, double asy, double* dt) {
     StepIn a{c->ni, c->nj, c->udx, c->vdy, (const double*)c->field[SG_F_U], (const double*)c->field[SG_F_V],

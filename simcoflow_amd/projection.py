@@ -266,6 +266,18 @@ class Projection:
         check(self.lib.sg_particles_track(self.ctx, float(dt), ctypes.byref(p), ctypes.byref(flagged)))
         return flagged.value
 
+    def diagnostics_record(self):
+        """Enqueues one row of per-step diagnostics (no host wait)."""
+        check(self.lib.sg_diagnostics_record(self.ctx))
+
+    def diagnostics_read(self, max_rows=_lib.DIAG_RING):
+        """Rows recorded and not yet read, oldest first: (rows (n, 8), lost)."""
+        out = np.zeros((max(max_rows, 1), _lib.DIAG_COLS))
+        n = ctypes.c_int(0)
+        lost = ctypes.c_longlong(0)
+        check(self.lib.sg_diagnostics_read(self.ctx, max_rows, _d(out)[1], ctypes.byref(n), ctypes.byref(lost)))
+        return out[:n.value].copy(), lost.value
+
     def state_save(self):
         """The context's time-stepping state (sg_state_save) as bytes."""
         n = ctypes.c_size_t(0)

@@ -37,14 +37,15 @@ def physics(fluids):
 
 class TimeStepper:
     def __init__(self, case, precision=_lib.SG_PREC_FP64, tol=None, cfl=0.25, vint=0.0, uref=1.0, max_iter=500,
-                 learned=None, stags=None, learned_width=256, previous_pressure=False):
+                 learned=None, stags=None, learned_width=256, previous_pressure=False, diagnostics=False):
         """learned: (weights, level, out_scale, deflate) of a trained warm
         start, level None for the first multigrid level no wider than
         learned_width; stags: the (U, V) cases.StagGeometry if already
         computed; previous_pressure: start each Poisson solve from the last
         step's pressure instead of zero (the reference starts from zero,
         ProjectionP_Mod.f90:319; it solves for the full pressure, so the
-        last one is a close start)."""
+        last one is a close start); diagnostics: record a row of device
+        diagnostics after every step (read with history())."""
         if learned and previous_pressure:
             raise ValueError("the learned warm start and the previous-pressure start exclude each other")
         self.case = case
@@ -76,6 +77,7 @@ class TimeStepper:
         self.itt = 0
         self.time = 0.0
         self.last = {}
+        self.diagnostics = diagnostics
 
     def close(self):
         self.pj.close()
@@ -97,9 +99,20 @@ class TimeStepper:
         pj.build(0.0)
         sinfo = pj.solve()
         pj.velocity_correct(0.0)
+        if self.diagnostics:
+            pj.diagnostics_record()
         self.time += dt
         self.last = dict(dt=dt, predictor=pinfo.as_dict(), poisson=sinfo.as_dict())
         return self.last
+
+    def history(self):
+        """Diagnostics rows recorded since the last call, as a dict of
+        columns (dt, liquid and gas volume, gas centroid, max |mres|, max
+        |u|, max |v|)."""
+        rows, lost = self.pj.diagnostics_read()
+        if lost:
+            raise RuntimeError(f"{lost} diagnostics rows were overwritten before they were read")
+        return {k: rows[:, c] for c, k in enumerate(_lib.DIAG_NAMES)}
 
     def save(self, path):
         """Checkpoint of the run (device state, step counter, time)."""
