@@ -80,6 +80,7 @@ module simcogpu
   public :: sg_poisson_matrix, sg_poisson_rhs, sg_poisson_apply, sg_velocity_correct_resident
   public :: sg_particles_set, sg_particles_get, sg_particles_track
   public :: sg_state_bytes, sg_state_save, sg_state_load
+  public :: sg_diagnostics_record, sg_diagnostics_read
   public :: sg_error_message
 
   interface
@@ -234,6 +235,20 @@ module simcogpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, x, y, u, v
       integer(c_int), value :: n
+      integer(c_int) :: rc
+    end function
+    function sg_diagnostics_record(ctx) bind(C, name="sg_diagnostics_record") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function sg_diagnostics_read(ctx, max_rows, rows, n_rows, n_lost) bind(C, name="sg_diagnostics_read") &
+        result(rc)
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx, rows
+      integer(c_int), value :: max_rows
+      integer(c_int), intent(out) :: n_rows
+      integer(c_long_long), intent(out) :: n_lost
       integer(c_int) :: rc
     end function
     function sg_state_bytes(ctx, bytes) bind(C, name="sg_state_bytes") result(rc)
