@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-learned", action="store_true", help="skip training and the learned path")
     ap.add_argument("--no-full-step", action="store_true", help="skip the per-part step timing")
     ap.add_argument("--no-particles", action="store_true", help="skip the particle and diagnostics timing")
+    ap.add_argument("--only-particles", action="store_true",
+                    help="time only the particle tracker and the diagnostics (for counter runs); prints their JSON")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--nn-width", type=int, default=256, help="widest multigrid level the net runs on")
     return ap.parse_args()
@@ -303,6 +305,27 @@ def particles(case, fields, n=1 << 20, reps=20, seed=0):
     return part, diag
 
 
+def geometry_rebuild(case, reps=10):
+    """The cut-cell geometry of a moving cylinder (radius 0.1 m) rebuilt on
+    the device on the swarm grid (sg_geometry_rebuild: P, U and V cells),
+    host wall time per call (the call waits for the closing-rule passes)."""
+    from simcoflow_amd.projection import Projection, params_for
+    g = case.grid
+    pj = Projection(g, params_for(case.fluids))
+    try:
+        times = []
+        for k in range(reps + 2):
+            t0 = time.perf_counter()
+            pj.geometry_rebuild("circle", 0.5 + 1e-3 * k, 0.4 + 7e-4 * k, 0.1)
+            if k >= 2:
+                times.append(time.perf_counter() - t0)
+    finally:
+        pj.close()
+    ms = float(np.median(times)) * 1e3
+    return {"what": "sg_geometry_rebuild, circle r = 0.1 m moved between calls", "ms_per_call": round(ms, 4),
+            "mcells_per_s": round(g.ni * g.nj / (ms * 1e-3) / 1e6, 1)}
+
+
 def measured_traffic(kernel_name):
     """HBM bytes per launch of the roofline kernel from the committed rocprof
     counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel),
@@ -361,8 +384,12 @@ def main():
     from simcoflow_amd import cases as C
     log("building the swarm case and its staggered geometry")
     case = C.case_swarm(a.ni, a.nj, seed=a.seed)
-    stags = C.staggered_geometry(case.grid, case.geo, case.solid)
     fields = C.predictor_fields(case)
+    if a.only_particles:
+        part, diag = particles(case, fields)
+        print(json.dumps({"particles": part, "diagnostics": diag, "geometry_rebuild": geometry_rebuild(case)}))
+        return
+    stags = C.staggered_geometry(case.grid, case.geo, case.solid)
     tol = 1e-8
     n = case.grid.ncells
     w = K = train = None
@@ -453,6 +480,7 @@ def main():
     if not a.no_particles:
         log("particle tracking and diagnostics")
         out["particles"], out["diagnostics"] = particles(case, fields)
+        out["geometry_rebuild"] = geometry_rebuild(case)
     if not a.no_full_step:
         log("time step parts")
         out["step_parts_ms"] = {"mixed": step_parts(case, stags, 1)}

@@ -301,6 +301,34 @@ int sg_particles_set(sg_ctx* ctx, int n, const double* x, const double* y, const
 int sg_particles_get(sg_ctx* ctx, int n, double* x, double* y, double* u, double* v);
 int sg_particles_track(sg_ctx* ctx, double dt, const sg_particle_params* p, int* flagged);
 
+/* ---- cut-cell geometry rebuilt on the device (replaces Grid_Preprocess,
+ * Cutcell_Mod.f90:18-725, with SolidVolumeFraction, Clsvof_Mod.f90:2553-2929,
+ * for a solid given as a shape; ObjectMovement, Clsvof_Mod.f90:2384-2398, is
+ * the host's: it moves the shape's centre before each call) -----------------
+ * The solid level set and normal on the P, U and V cells (a signed distance,
+ * positive in the fluid), the solid fractions, apertures (harmonic face
+ * averaging, solid and closed cells shut), wall lengths, the U/V solid
+ * fractions and flags, the momentum-exchange cells and every staggered-cell
+ * array of sg_set_staggered_geometry: the device state sg_set_geometry +
+ * sg_set_staggered_geometry would set, computed on the device.  box: half
+ * width a, half height b; circle: radius a; ellipse: semi-axes a, b; none:
+ * no solid.  Needs ni, nj >= 4. */
+#define SG_SOLID_NONE 0
+#define SG_SOLID_BOX 1
+#define SG_SOLID_CIRCLE 2
+#define SG_SOLID_ELLIPSE 3
+typedef struct sg_solid_shape {
+    int kind;
+    double cx, cy; /* centre */
+    double a, b;
+} sg_solid_shape;
+int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape);
+/* test hook: one staggered-geometry array of U (which = 0) or V cells, in the
+ * order of sg_stag_geometry's double arrays (vofS nxS nyS phiS WlLh delh ccx
+ * ccy: ni*nj; aE aW aN aS EtaE EtaN AlE AlN DAlE DAlN SxE SyN fcex fcey fcnx
+ * fcny: (ni+2)*(nj+2)), then moex and active (int, ni*nj) */
+int sg_stag_download(sg_ctx* ctx, int which, int k, void* host);
+
 /* ---- per-step diagnostics without host synchronisation -------------------
  * sg_diagnostics_record enqueues, on the context's stream, two kernels that
  * reduce the current device fields (deterministic two-phase sums and

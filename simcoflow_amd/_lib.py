@@ -83,6 +83,15 @@ class ClsvofParams(ctypes.Structure):
 
 
 # exported symbol -> (argtypes, restype); kept in step with include/simcogpu.h
+# solid shapes of sg_geometry_rebuild
+SOLID_KINDS = {"none": 0, "box": 1, "circle": 2, "ellipse": 3}
+
+
+class SolidShape(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("cx", ctypes.c_double), ("cy", ctypes.c_double), ("a", ctypes.c_double),
+                ("b", ctypes.c_double)]
+
+
 # per-step diagnostics ring (SG_DIAG_COLS, SG_DIAG_RING); the columns
 DIAG_COLS = 8
 DIAG_RING = 4096
@@ -129,6 +138,8 @@ PROTOTYPES = {
     "sg_particles_set": ([_CTX, ctypes.c_int, _D, _D, _D, _D, _D], ctypes.c_int),
     "sg_particles_get": ([_CTX, ctypes.c_int, _D, _D, _D, _D], ctypes.c_int),
     "sg_particles_track": ([_CTX, ctypes.c_double, ctypes.POINTER(ParticleParams), _I], ctypes.c_int),
+    "sg_geometry_rebuild": ([_CTX, ctypes.POINTER(SolidShape)], ctypes.c_int),
+    "sg_stag_download": ([_CTX, ctypes.c_int, ctypes.c_int, _V], ctypes.c_int),
     "sg_diagnostics_record": ([_CTX], ctypes.c_int),
     "sg_diagnostics_read": ([_CTX, ctypes.c_int, _D, _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "sg_state_bytes": ([_CTX, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),

@@ -32,6 +32,7 @@ SOURCES = [
     "sg_step.hip",
     "sg_particles.hip",
     "sg_state.hip",
+    "sg_geometry.hip",
 ]
 EXTRA = {
     "sg_poisson.hip": ["-ffp-contract=off"],
@@ -40,6 +41,7 @@ EXTRA = {
     "sg_api.hip": ["-ffp-contract=off"],
     "sg_step.hip": ["-ffp-contract=off"],
     "sg_particles.hip": ["-ffp-contract=off"],
+    "sg_geometry.hip": ["-ffp-contract=off"],
     # FP32 atomic adds of the weight gradients as hardware atomics (device
     # memory, coarse-grained)
     "sg_nn.hip": ["-munsafe-fp-atomics"],

@@ -567,6 +567,40 @@ int sg_particles_get(sg_ctx* ctx, int n, double* x, double* y, double* u, double
     return SG_OK;
 }
 
+int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !shape || shape->kind < SG_SOLID_NONE || shape->kind > SG_SOLID_ELLIPSE) {
+        set_error("sg_geometry_rebuild: need a shape of a known kind");
+        return SG_ERR_ARG;
+    }
+    if (shape->kind != SG_SOLID_NONE &&
+        (!(shape->a > 0.0) || (shape->kind != SG_SOLID_CIRCLE && !(shape->b > 0.0)) ||
+         !std::isfinite(shape->cx) || !std::isfinite(shape->cy))) {
+        set_error("sg_geometry_rebuild: the shape needs a finite centre and positive sizes");
+        return SG_ERR_ARG;
+    }
+    if (c->ni < 4 || c->nj < 4) {
+        set_error("sg_geometry_rebuild: needs ni, nj >= 4");
+        return SG_ERR_ARG;
+    }
+    return geometry_rebuild(c, shape);
+}
+
+int sg_stag_download(sg_ctx* ctx, int which, int k, void* host) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !host || (which != 0 && which != 1) || k < 0 || k > STG_COUNT + 1) {
+        set_error("sg_stag_download: bad argument");
+        return SG_ERR_ARG;
+    }
+    const size_t n = (size_t)c->ni * c->nj, ng = (size_t)(c->ni + 2) * (c->nj + 2);
+    const StagDev& S = c->stag[which];
+    const void* src = k < STG_COUNT ? (const void*)S.a[k] : (k == STG_COUNT ? (const void*)S.moex : (const void*)S.act);
+    const size_t bytes = k < STG_NCELL ? n * sizeof(double) : (k < STG_COUNT ? ng * sizeof(double) : n * sizeof(int));
+    SG_HIP(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
 int sg_diagnostics_record(sg_ctx* ctx) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     if (!c) return SG_ERR_ARG;

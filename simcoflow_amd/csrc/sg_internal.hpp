@@ -108,6 +108,15 @@ struct PcgScalars {
     int guard;      // bit 0: NaN, bit 1: diag/rhs guard tripped
 };
 
+// work arrays of the device geometry rebuild (allocated at its first call)
+struct GeoScratch {
+    double* raw[4] = {nullptr, nullptr, nullptr, nullptr};  // raw E, W, N, S apertures (ghosted)
+    int* close[2] = {nullptr, nullptr};                     // closing-rule decisions, two passes
+    int* changed = nullptr;
+    int* moex[2] = {nullptr, nullptr};                      // U, V momentum-exchange cells
+    double* next[2] = {nullptr, nullptr};                   // dx of column i+1, dy of row j+1
+};
+
 // ring of per-step diagnostics rows written on the device
 constexpr int DIAG_BLOCKS = 1024;
 struct DiagRing {
@@ -256,6 +265,7 @@ struct Ctx {
     ParticleState part;
     double* step_dev = nullptr;   // device scalars of the time-step control (dt)
     DiagState diag;
+    GeoScratch geo;
     hipEvent_t ev_nn0 = nullptr, ev_nn1 = nullptr;
     int use_graph = 1;
     ClsBuffers cls;
@@ -323,6 +333,7 @@ void learned_release(Ctx* c);
 // time-step control and boundary layer (sg_step.hip)
 int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, double asy, double* dt);
 int closed_box_bc(Ctx* c);
+int geometry_rebuild(Ctx* c, const sg_solid_shape* shape);
 int diagnostics_record(Ctx* c);
 int diagnostics_read(Ctx* c, int max_rows, double* rows, int* n_rows, long long* n_lost);
 

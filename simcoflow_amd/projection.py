@@ -266,6 +266,29 @@ class Projection:
         check(self.lib.sg_particles_track(self.ctx, float(dt), ctypes.byref(p), ctypes.byref(flagged)))
         return flagged.value
 
+    def geometry_rebuild(self, kind, cx=0.0, cy=0.0, a=0.0, b=0.0):
+        """Cut-cell geometry of a box (half sizes a, b), circle (radius a) or
+        ellipse (semi-axes a, b) centred at (cx, cy), or of no solid, rebuilt
+        on the device (P, U and V cells)."""
+        s = _lib.SolidShape(_lib.SOLID_KINDS[kind], float(cx), float(cy), float(a), float(b))
+        check(self.lib.sg_geometry_rebuild(self.ctx, ctypes.byref(s)))
+
+    def stag_download(self, which):
+        """The device's staggered geometry of U (0) or V (1) cells as a dict
+        of the cases.StagGeometry array names."""
+        out = {}
+        names = _lib.STAG_CELL[:8] + _lib.STAG_GHOST
+        for k, name in enumerate(names):
+            shape = (self.nj, self.ni) if k < 8 else (self.nj + 2, self.ni + 2)
+            a = np.empty(shape)
+            check(self.lib.sg_stag_download(self.ctx, which, k, a.ctypes.data_as(ctypes.c_void_p)))
+            out[name] = a
+        for k, name in ((len(names), "moex"), (len(names) + 1, "active")):
+            a = np.empty((self.nj, self.ni), dtype=np.int32)
+            check(self.lib.sg_stag_download(self.ctx, which, k, a.ctypes.data_as(ctypes.c_void_p)))
+            out[name] = a
+        return out
+
     def diagnostics_record(self):
         """Enqueues one row of per-step diagnostics (no host wait)."""
         check(self.lib.sg_diagnostics_record(self.ctx))
