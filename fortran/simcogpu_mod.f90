@@ -67,6 +67,12 @@ module simcogpu
     real(c_double) :: depth, amp, kw, omega
   end type sg_clsvof_params
 
+  ! a solid shape for sg_geometry_rebuild (kind: 0 none, 1 box, 2 circle, 3 ellipse)
+  type, bind(C), public :: sg_solid_shape
+    integer(c_int) :: kind
+    real(c_double) :: cx, cy, a, b
+  end type sg_solid_shape
+
   ! TrackingParticles (Particles.f90:71-205) constants; see sg_particle_params in simcogpu.h
   type, bind(C), public :: sg_particle_params
     real(c_double) :: rho_l, rho_g, nu_l, nu_g, rho_p, gx, gy, lref, uref, us, vs
@@ -81,6 +87,7 @@ module simcogpu
   public :: sg_particles_set, sg_particles_get, sg_particles_track
   public :: sg_state_bytes, sg_state_save, sg_state_load
   public :: sg_diagnostics_record, sg_diagnostics_read
+  public :: sg_geometry_rebuild, sg_object_force
   public :: sg_error_message
 
   interface
@@ -235,6 +242,19 @@ module simcogpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, x, y, u, v
       integer(c_int), value :: n
+      integer(c_int) :: rc
+    end function
+    function sg_geometry_rebuild(ctx, shape) bind(C, name="sg_geometry_rebuild") result(rc)
+      import :: c_int, c_ptr, sg_solid_shape
+      type(c_ptr), value :: ctx
+      type(sg_solid_shape), intent(in) :: shape
+      integer(c_int) :: rc
+    end function
+    function sg_object_force(ctx, vs, force, clp, clf) bind(C, name="sg_object_force") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: vs
+      real(c_double), intent(out) :: force, clp, clf
       integer(c_int) :: rc
     end function
     function sg_diagnostics_record(ctx) bind(C, name="sg_diagnostics_record") result(rc)

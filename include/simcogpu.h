@@ -323,6 +323,14 @@ typedef struct sg_solid_shape {
     double a, b;
 } sg_solid_shape;
 int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape);
+/* force on the solid (replaces ComputeForceObject, Clsvof_Mod.f90:2400-2549,
+ * with ICorProb = .TRUE., Main.f90:205): the vertical force clf - clp from
+ * the device fields P, V, the P geometry (VOFS NYS WLLH AE AW) and the V-cell
+ * geometry (vofS, delh, moex), with Rey = sg_params.Re and the solid's
+ * vertical velocity vs; clp (pressure part) and clf (shear part) may be
+ * NULL.  The host integrates ObjectMovement with it (asy = (force - M g)/M,
+ * Solver_Mod.f90:277-283). */
+int sg_object_force(sg_ctx* ctx, double vs, double* force, double* clp, double* clf);
 /* test hook: one staggered-geometry array of U (which = 0) or V cells, in the
  * order of sg_stag_geometry's double arrays (vofS nxS nyS phiS WlLh delh ccx
  * ccy: ni*nj; aE aW aN aS EtaE EtaN AlE AlN DAlE DAlN SxE SyN fcex fcey fcnx

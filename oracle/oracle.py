@@ -512,3 +512,28 @@ def particle_track(g, vof, vofS, u, v, px, py, pu, pv, pd, params, dt):
     bad = _part_lib().orc_particle_track(g.ni, g.nj, *[_d(a) for a in keep], len(d), *[_d(a) for a in out], _d(d),
                                          ctypes.byref(P), float(dt))
     return (*out, bad)
+
+
+# ---------------------------------------------------------------------------
+# Force on the solid (Clsvof_Mod.f90:2400-2549)
+_force_declared = False
+
+
+def object_force(g, geo, V, p, v, vs, Re):
+    """ComputeForceObject on host arrays: geo a cases.Geometry, V the V-cell
+    cases.StagGeometry, p and v ghosted; returns (force, clp, clf)."""
+    global _force_declared
+    L = lib()
+    if not _force_declared:
+        dbl, ci = ctypes.c_double, ctypes.c_int
+        L.orc_object_force.argtypes = [ci, ci] + [_D] * 8 + [ctypes.POINTER(ci), _D, _D, dbl, dbl, _D, _D]
+        L.orc_object_force.restype = dbl
+        _force_declared = True
+    c = np.ascontiguousarray
+    arrs = [c(a, dtype=np.float64) for a in (g.dx, geo.vofS, geo.nyS, geo.WlLh, geo.aE, geo.aW, p, V.vofS)]
+    moex = c(V.moex, dtype=np.int32)
+    rest = [c(a, dtype=np.float64) for a in (V.delh, v)]
+    clp, clf = ctypes.c_double(), ctypes.c_double()
+    f = L.orc_object_force(g.ni, g.nj, *[_d(a) for a in arrs], moex.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                           *[_d(a) for a in rest], float(vs), float(Re), ctypes.byref(clp), ctypes.byref(clf))
+    return f, clp.value, clf.value

@@ -201,6 +201,12 @@ double orc_time_step(int ni, int nj, const double* udx, const double* vdy, const
                      double cfl, double Fr, double vint, double uref, double vs, double asy);
 void orc_closed_box_bc(int ni, int nj, double* u, double* v, double* p);
 
+/* sg_oracle_force.c -- ComputeForceObject (Clsvof_Mod.f90:2400-2549) */
+double orc_object_force(int ni, int nj, const double* dx, const double* vofS, const double* nyS, const double* WlLh,
+                        const double* aE, const double* aW, const double* p, const double* vofSV, const int* moexV,
+                        const double* delhV, const double* v, double vs, double Rey, double* clp_out,
+                        double* clf_out);
+
 /* sg_oracle_particles.c -- Lagrangian particles (Particles.f90:71-249) */
 typedef struct {
     double rho_l, rho_g;  /* row, roa */

@@ -586,6 +586,15 @@ int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape) {
     return geometry_rebuild(c, shape);
 }
 
+int sg_object_force(sg_ctx* ctx, double vs, double* force, double* clp, double* clf) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !force) {
+        set_error("sg_object_force: null argument");
+        return SG_ERR_ARG;
+    }
+    return object_force(c, vs, force, clp, clf);
+}
+
 int sg_stag_download(sg_ctx* ctx, int which, int k, void* host) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     if (!c || !host || (which != 0 && which != 1) || k < 0 || k > STG_COUNT + 1) {

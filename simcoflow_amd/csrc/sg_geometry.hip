@@ -466,7 +466,116 @@ __global__ void __launch_bounds__(256) k_stag_ghosts(StagIn s) {
     for (int k : ks) s.a[k][g] = s.a[k][src];
 }
 
+struct ForceIn {
+    int ni, nj;
+    const double *dx, *vofS, *nyS, *wl, *aE, *aW, *p;  // P cells (p, aE, aW ghosted)
+    const double *vofSV, *delhV, *v;                  // V cells (v ghosted)
+    const int* moexV;
+    double vs, Rey;
+};
+
+constexpr int NTF_FORCE = 256;
+
+// per-block partial sums of the pressure (clp) and shear (clf) parts of
+// ComputeForceObject (Clsvof_Mod.f90:2400-2549, ICorProb = .TRUE.)
+__global__ void __launch_bounds__(NTF_FORCE) k_force(ForceIn f, double* part, int nb) {
+#pragma clang fp contract(off)
+    __shared__ double red[2][NTF_FORCE / 64];
+    const int ni = f.ni, nj = f.nj;
+    double clp = 0.0, clf = 0.0;
+    for (int e = blockIdx.x * NTF_FORCE + threadIdx.x; e < ni * nj; e += gridDim.x * NTF_FORCE) {
+        const int i = e % ni, j = e / ni;
+        const size_t c = (size_t)e, g = gix(ni, i, j);
+        const double s = f.vofS[c];
+        if (s < 1.0 - EPSI && s > EPSI) clp += f.p[g] * f.wl[c] * f.nyS[c];
+        if (j <= nj - 2) {
+            const double sn = f.vofS[c + ni];
+            const bool full_above = sn < 1.0 + EPSI && sn > 1.0 - EPSI;
+            if (s < 1.0 - EPSI && f.aE[g] < G_EPSIF && f.aW[g] < G_EPSIF && full_above) {
+                const double pres = f.p[g - (size_t)(ni + 2)], area = f.dx[i], nyfp = -1.0;
+                clp += pres * area * nyfp - f.p[g] * area * nyfp;
+            }
+            if (s > -EPSI && s < EPSI) {
+                if (full_above) clp += f.p[g] * f.dx[i] * -1.0;
+                if (sn > 0.6 - EPSI && sn < 0.6 + EPSI && j < nj - 2) {
+                    const double s2 = f.vofS[c + 2 * (size_t)ni];
+                    if (s2 > 0.6 - EPSI && s2 < 0.6 + EPSI) clp += f.p[g] * (0.6 * f.dx[i]) * -1.0;
+                }
+            }
+        }
+        const double sv = f.vofSV[c];
+        if (sv > EPSI && sv < 1.0 - EPSI && f.moexV[c] != 1) clf += (f.v[g] - f.vs) / f.Rey / f.delhV[c] * f.wl[c];
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        clp += __shfl_xor(clp, o, 64);
+        clf += __shfl_xor(clf, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = clp;
+        red[1][threadIdx.x >> 6] = clf;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double r = red[threadIdx.x][0];
+        for (int w = 1; w < NTF_FORCE / 64; ++w) r += red[threadIdx.x][w];
+        part[(size_t)threadIdx.x * nb + blockIdx.x] = r;
+    }
+}
+
+// one block: the partials in block order; out = {clf - clp, clp, clf}
+__global__ void __launch_bounds__(NTF_FORCE) k_force_fin(const double* part, int nb, double* out) {
+    __shared__ double red[2][NTF_FORCE / 64];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < nb; k += NTF_FORCE) {
+        a += part[k];
+        b += part[(size_t)nb + k];
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double clp = red[0][0], clf = red[1][0];
+        for (int w = 1; w < NTF_FORCE / 64; ++w) {
+            clp += red[0][w];
+            clf += red[1][w];
+        }
+        out[0] = clf - clp;
+        out[1] = clp;
+        out[2] = clf;
+    }
+}
+
 }  // namespace
+
+int object_force(Ctx* c, double vs, double* force, double* clp, double* clf) {
+    const int ni = c->ni, nj = c->nj;
+    auto F = [&](int f) { return (const double*)c->field[f]; };
+    const StagDev& V = c->stag[1];
+    const ForceIn in{ni,        nj,        c->dx,       F(SG_F_VOFS),      F(SG_F_NYS),   F(SG_F_WLLH),
+                     F(SG_F_AE), F(SG_F_AW), F(SG_F_P),   V.a[STG_VOFS],     V.a[STG_DELH], F(SG_F_V),
+                     V.moex,    vs,        c->prm.Re};
+    int nb = (ni * nj + NTF_FORCE - 1) / NTF_FORCE;
+    if (nb > DIAG_BLOCKS) nb = DIAG_BLOCKS;
+    // the diagnostics' partial buffer (same stream, so never in use by both)
+    double* part = c->diag.partials;
+    double* out = c->step_dev + 1;
+    hipLaunchKernelGGL(k_force, dim3(nb), dim3(NTF_FORCE), 0, c->stream, in, part, nb);
+    hipLaunchKernelGGL(k_force_fin, dim3(1), dim3(NTF_FORCE), 0, c->stream, (const double*)part, nb, out);
+    SG_HIP(hipGetLastError());
+    double h[3];
+    SG_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    if (force) *force = h[0];
+    if (clp) *clp = h[1];
+    if (clf) *clf = h[2];
+    return SG_OK;
+}
 
 int geometry_rebuild(Ctx* c, const sg_solid_shape* shape) {
     const int ni = c->ni, nj = c->nj;

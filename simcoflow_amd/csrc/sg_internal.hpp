@@ -334,6 +334,7 @@ void learned_release(Ctx* c);
 int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, double asy, double* dt);
 int closed_box_bc(Ctx* c);
 int geometry_rebuild(Ctx* c, const sg_solid_shape* shape);
+int object_force(Ctx* c, double vs, double* force, double* clp, double* clf);
 int diagnostics_record(Ctx* c);
 int diagnostics_read(Ctx* c, int max_rows, double* rows, int* n_rows, long long* n_lost);
 

@@ -273,6 +273,12 @@ class Projection:
         s = _lib.SolidShape(_lib.SOLID_KINDS[kind], float(cx), float(cy), float(a), float(b))
         check(self.lib.sg_geometry_rebuild(self.ctx, ctypes.byref(s)))
 
+    def object_force(self, vs=0.0):
+        """ComputeForceObject on the device fields: (force, clp, clf)."""
+        f, a, b = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(self.lib.sg_object_force(self.ctx, float(vs), ctypes.byref(f), ctypes.byref(a), ctypes.byref(b)))
+        return f.value, a.value, b.value
+
     def stag_download(self, which):
         """The device's staggered geometry of U (0) or V (1) cells as a dict
         of the cases.StagGeometry array names."""

@@ -114,3 +114,41 @@ def test_rebuilt_geometry_drives_the_projection():
             pj.close()
     assert out[0][1] == out[1][1]
     assert np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("kind,sizes", [("circle", (0.15, 0.0)), ("box", (0.2, 0.07)), ("ellipse", (0.2, 0.1))])
+def test_object_force_matches_oracle(oracle_lib, kind, sizes):
+    """ComputeForceObject on the device against the oracle on the same
+    geometry and fields (a hydrostatic pressure plus noise, a random v):
+    pressure and shear parts to 1e-12 relative (two-phase sums in another
+    order than the reference's loops)."""
+    from simcoflow_amd.projection import Projection, params_for
+    n = 128
+    g = C.uniform_grid(n, n, 1.0, 1.0)
+    cx, cy = 0.5 + 0.3 / n, 0.45 + 0.7 / n
+    solid = _shape(kind, cx, cy, *sizes)
+    geo = C.build_geometry(g, solid)
+    V = C.staggered_geometry(g, geo, solid)[1]
+    rng = np.random.default_rng(9)
+    yg = np.concatenate([[g.yc[0] - g.dy[0]], g.yc, [g.yc[-1] + g.dy[-1]]])
+    p = np.repeat((-1000.0 * 9.81 * yg)[:, None], n + 2, axis=1) + rng.normal(0, 10.0, (n + 2, n + 2))
+    v = rng.normal(0.0, 0.1, (n + 2, n + 2))
+    vs = 0.05
+    fluids = C.Fluids()
+    prm = params_for(fluids)
+    Re = prm.Re
+    pj = Projection(g, prm)
+    try:
+        pj.geometry_rebuild(kind, cx, cy, *sizes)
+        pj.upload("P", p)
+        pj.upload("V", v)
+        print(f"[force] {kind} on {n}^2", flush=True)
+        got = pj.object_force(vs)
+    finally:
+        pj.close()
+    ref = oracle_lib.object_force(g, geo, V, p, v, vs, Re)
+    print(f"force gpu {got} oracle {ref}")
+    for a, b in zip(got[1:], ref[1:]):
+        assert abs(a - b) <= 1e-12 * max(abs(b), 1e-300)
+    assert abs(got[0] - ref[0]) <= 1e-12 * (abs(ref[1]) + abs(ref[2]))
+    assert ref[2] != 0.0
