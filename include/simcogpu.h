@@ -49,7 +49,9 @@ typedef struct sg_params {
     double tol;      /* stop at ||b-Ap||_2 <= tol*||b||_2 */
     int mg_pre, mg_post; /* smoothing sweeps per level (ProjectionP_Mod.f90:126: 1) */
     int warm_start;  /* 0: x0 = 0 (ProjectionP_Mod.f90:319); 1: x0 = previous solution;
-                        2: learned warm start + deflation vector (sg_set_learned) */
+                        2: learned warm start + deflation vector (sg_set_learned);
+                        3: previous solution + the learned correction predicted from
+                        its residual (Galerkin-scaled) */
     int check_every; /* PCG iterations between host convergence checks; 0 (default): set from
                         the observed convergence rate */
     int mg_coarse_krylov; /* flexible-CG iterations on the first level of at most 1024 points,

@@ -76,6 +76,10 @@ static int check_params(const sg_params* p) {
         set_error("mg_coarse_krylov must be in [0, 64]");
         return SG_ERR_ARG;
     }
+    if (p->warm_start < 0 || p->warm_start > 3) {
+        set_error("warm_start must be 0, 1, 2 or 3");
+        return SG_ERR_ARG;
+    }
     if (p->momentum_method != 0 && p->momentum_method != 1) {
         set_error("momentum_method must be 0 (Jacobi) or 1 (BiCGStab)");
         return SG_ERR_ARG;

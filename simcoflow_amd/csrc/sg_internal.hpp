@@ -224,6 +224,7 @@ struct NnState {
     double* aw = nullptr;      // A w
     float ms = 0.f;            // device time of the last warm start
     hipGraphExec_t graph = nullptr;  // the warm start's kernels, captured once
+    hipGraphExec_t graph_prev = nullptr;  // the same on top of the previous solution
 };
 
 struct Ctx {
@@ -326,7 +327,7 @@ int mg_launch_fine(Ctx* c, int which);  // measurement: 1 down, 2 up, 3 copy
 int pcg_solve(Ctx* c, sg_solve_info* info);
 int fine_residual_norms(Ctx* c, double* relres, double* maxrow);
 // learned warm start (sg_learned.hip)
-int learned_warm_start(Ctx* c);   // x = Galerkin-scaled prolongated net output
+int learned_warm_start(Ctx* c, bool prev);   // x = Galerkin-scaled prolongated net output
 int learned_features(Ctx* c, int level, void* X, float* t, float* mask, double* scale);
 int learned_enable(Ctx* c, const float* w, int level, double out_scale, int deflate);
 void learned_release(Ctx* c);

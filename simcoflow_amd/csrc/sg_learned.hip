@@ -181,21 +181,35 @@ __global__ void __launch_bounds__(NTR) k_galerkin_fin(const double* part, int ma
     sc->walpha = s[1] > 0.0 ? s[0] / s[1] : 0.0;
 }
 
-// x = walpha * w
+// x = walpha * w, or x += walpha * w on top of the previous solution
 __global__ void __launch_bounds__(256) k_start(Dims D, const double* __restrict__ w, const PcgScalars* sc,
-                                               double* __restrict__ x) {
+                                               double* __restrict__ x, int add) {
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     if (i >= D.ni || j >= D.nj) return;
     const size_t p = D.off(i, j);
-    x[p] = sc->walpha * w[p];
+    x[p] = add ? x[p] + sc->walpha * w[p] : sc->walpha * w[p];
+}
+
+// r = b - A x on the fine level (the residual of the previous solution)
+__global__ void __launch_bounds__(256) k_resid_prev(Dims D, const double* __restrict__ diag,
+                                                    const double* __restrict__ cE, const double* __restrict__ cN,
+                                                    const double* __restrict__ b, const double* __restrict__ x,
+                                                    double* __restrict__ r) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    if (i >= D.ni || j >= D.nj) return;
+    const size_t p = D.off(i, j);
+    const size_t ld = D.ld;
+    const double ax =
+        diag[p] * x[p] - cE[p - 1] * x[p - 1] - cE[p] * x[p + 1] - cN[p - ld] * x[p - ld] - cN[p] * x[p + ld];
+    r[p] = b[p] - ax;
 }
 
 bool net_fits(const Dims& d) { return d.nj % 8 == 0 && d.ni % 128 == 0; }
 
-int restrict_to(Ctx* c, int level) {
+int restrict_to(Ctx* c, int level, const double* src) {
     hipStream_t s = c->stream;
-    const double* src = c->rhs;
     for (int l = 0; l < level; ++l) {
         const Dims& C = c->lev[l + 1].d;
         hipLaunchKernelGGL(k_restrict, grid2d(C.ni, C.nj), block2d(), 0, s, c->lev[l].d, C, c->lev[l].P, src,
@@ -206,11 +220,11 @@ int restrict_to(Ctx* c, int level) {
     return SG_OK;
 }
 
-int features(Ctx* c, int level, bf16* X, float* t, float* mask) {
-    int rc = restrict_to(c, level);
+int features(Ctx* c, int level, bf16* X, float* t, float* mask, const double* src) {
+    int rc = restrict_to(c, level, src);
     if (rc) return rc;
     Level& L = c->lev[level];
-    const double* b = level ? L.v.b : c->rhs;
+    const double* b = level ? L.v.b : src;
     {
         const dim3 g = grid2d(L.d.ni, L.d.nj);
         double* part = c->red.partials + (size_t)5 * NRED * c->red.max_blocks;
@@ -229,10 +243,11 @@ int features(Ctx* c, int level, bf16* X, float* t, float* mask) {
 }  // namespace
 
 void learned_release(Ctx* c) {
-    if (c->nn.graph) {
-        (void)hipGraphExecDestroy(c->nn.graph);
-        c->nn.graph = nullptr;
-    }
+    for (hipGraphExec_t* ge : {&c->nn.graph, &c->nn.graph_prev})
+        if (*ge) {
+            (void)hipGraphExecDestroy(*ge);
+            *ge = nullptr;
+        }
     if (c->nn.net) {
         nn::destroy(c->nn.net);
         delete c->nn.net;
@@ -293,7 +308,7 @@ int learned_features(Ctx* c, int level, void* X, float* t, float* mask, double* 
         set_error("sg_nn_features: no such level");
         return SG_ERR_ARG;
     }
-    int rc = features(c, level, (bf16*)X, t, mask);
+    int rc = features(c, level, (bf16*)X, t, mask, c->rhs);
     if (rc) return rc;
     SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
@@ -304,10 +319,20 @@ int learned_features(Ctx* c, int level, void* X, float* t, float* mask, double* 
 namespace {
 
 // x0 = (w,b)/(w,Aw) w with w the net's prediction carried to the fine grid
-int enqueue_warm_start(Ctx* c) {
+int enqueue_warm_start(Ctx* c, bool prev) {
     hipStream_t s = c->stream;
     const int L = c->nn.level;
-    int rc = features(c, L, (bf16*)c->nn.X, nullptr, nullptr);
+    // from the previous solution: the net predicts the correction from the
+    // residual that solution leaves
+    const double* src = c->rhs;
+    if (prev) {
+        const Dims D0 = c->lev[0].d;
+        hipLaunchKernelGGL(k_resid_prev, grid2d(c->ni, c->nj), block2d(), 0, s, D0,
+                           (const double*)c->lev[0].A.diag, (const double*)c->lev[0].A.cE,
+                           (const double*)c->lev[0].A.cN, (const double*)c->rhs, (const double*)c->x, c->r);
+        src = c->r;
+    }
+    int rc = features(c, L, (bf16*)c->nn.X, nullptr, nullptr, src);
     if (rc) return rc;
     rc = nn::forward(c->nn.net, (const bf16*)c->nn.X, 1);
     if (rc) return rc;
@@ -327,9 +352,10 @@ int enqueue_warm_start(Ctx* c) {
     double* part = c->red.partials + (size_t)5 * NRED * c->red.max_blocks;
     hipLaunchKernelGGL(k_galerkin, g, block2d(), 0, s, D, (const double*)c->lev[0].A.diag,
                        (const double*)c->lev[0].A.cE, (const double*)c->lev[0].A.cN, (const double*)c->nn.w,
-                       (const double*)c->rhs, c->nn.aw, part, c->red.max_blocks);
+                       src, c->nn.aw, part, c->red.max_blocks);
     hipLaunchKernelGGL(k_galerkin_fin, dim3(1), dim3(NTR), 0, s, (const double*)part, c->red.max_blocks, nb, c->sc);
-    hipLaunchKernelGGL(k_start, g, block2d(), 0, s, D, (const double*)c->nn.w, (const PcgScalars*)c->sc, c->x);
+    hipLaunchKernelGGL(k_start, g, block2d(), 0, s, D, (const double*)c->nn.w, (const PcgScalars*)c->sc, c->x,
+                       prev ? 1 : 0);
     SG_HIP(hipGetLastError());
     return SG_OK;
 }
@@ -339,27 +365,28 @@ int enqueue_warm_start(Ctx* c) {
 // The warm start's ~45 kernels (restriction, features, the net, the
 // prolongation, the Galerkin scaling) run as one graph, captured at the first
 // call after sg_set_learned: every buffer it touches lives as long as the net.
-int learned_warm_start(Ctx* c) {
+int learned_warm_start(Ctx* c, bool prev) {
     hipStream_t s = c->stream;
-    if (!c->nn.graph) {
+    hipGraphExec_t& ge = prev ? c->nn.graph_prev : c->nn.graph;
+    if (!ge) {
         hipGraph_t graph = nullptr;
         SG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_warm_start(c);
+        const int rc = enqueue_warm_start(c, prev);
         const hipError_t e = hipStreamEndCapture(s, &graph);
         if (rc) {
             if (graph) (void)hipGraphDestroy(graph);
             return rc;
         }
         if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-        const hipError_t e2 = hipGraphInstantiate(&c->nn.graph, graph, nullptr, nullptr, 0);
+        const hipError_t e2 = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e2 != hipSuccess) {
-            c->nn.graph = nullptr;
+            ge = nullptr;
             return hip_fail(e2, "hipGraphInstantiate");
         }
     }
     SG_HIP(hipEventRecord(c->ev_nn0, s));
-    SG_HIP(hipGraphLaunch(c->nn.graph, s));
+    SG_HIP(hipGraphLaunch(ge, s));
     SG_HIP(hipEventRecord(c->ev_nn1, s));
     return SG_OK;
 }

@@ -300,7 +300,7 @@ Slots slots(Ctx* c, int s) {
     return Slots{c->red.partials + (size_t)s * NRED * c->red.max_blocks, c->red.max_blocks};
 }
 
-bool learned(Ctx* c) { return c->prm.warm_start == 2 && c->nn.on; }
+bool learned(Ctx* c) { return (c->prm.warm_start == 2 || c->prm.warm_start == 3) && c->nn.on; }
 bool deflating(Ctx* c) { return learned(c) && c->nn.deflate; }
 
 // one PCG iteration (V-cycle + direction + update), enqueued on the stream;
@@ -479,9 +479,9 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
     c->sc_host->flexible = (c->prm.precision == SG_PREC_MIXED || c->prm.mg_coarse_krylov > 0) ? 1 : 0;
     SG_HIP(hipMemcpyAsync(&c->sc->flexible, &c->sc_host->flexible, sizeof(int), hipMemcpyHostToDevice, s));
     int rc;
-    if (c->prm.warm_start == 2) {
+    if (c->prm.warm_start == 2 || c->prm.warm_start == 3) {
         if (!c->nn.on) {
-            set_error("warm_start = 2 needs the learned warm start (sg_set_learned)");
+            set_error("warm_start = 2 or 3 needs the learned warm start (sg_set_learned)");
             return SG_ERR_ARG;
         }
         if (deflating(c) && mixed(c)) {
@@ -491,7 +491,7 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
                       "in mixed precision");
             return SG_ERR_ARG;
         }
-        rc = learned_warm_start(c);
+        rc = learned_warm_start(c, c->prm.warm_start == 3);
         if (rc) return rc;
         rc = enqueue_init(c, 1, FIN_INIT);
     } else {
@@ -503,7 +503,7 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
     const double warm_relres = c->sc_host->bb > 0.0 ? sqrt(c->sc_host->rr / c->sc_host->bb) : 0.0;
     float warm_ms = 0.f;
     double warm_energy = 0.0;
-    if (c->prm.warm_start == 2) {
+    if (c->prm.warm_start == 2 || c->prm.warm_start == 3) {
         SG_HIP(hipEventElapsedTime(&warm_ms, c->ev_nn0, c->ev_nn1));
         if (c->sc_host->wq > 0.0) warm_energy = c->sc_host->wb * c->sc_host->wb / c->sc_host->wq;
     }

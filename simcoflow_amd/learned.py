@@ -226,6 +226,44 @@ def collect_pairs(make_case, seeds, per_case, level=None, tol=1e-9, progress=Non
     return X, T, M, level
 
 
+def collect_step_pairs(make_case, seeds, steps, skip=2, level=None, tol=1e-9, progress=None, max_width=256):
+    """(x, t, mask) training pairs from time stepping: for each case seed,
+    the right-hand side and pressure of every Poisson solve of `steps` full
+    time steps (TimeStepper, FP64 solver) after `skip` warm-up steps, on
+    `level` (default: the first level no wider than max_width).  These are
+    the systems the solver meets in a run, unlike collect_pairs' seeded
+    predictor fields.  Returns torch tensors on the GPU and the level."""
+    import torch
+    from .solver import TimeStepper
+    X = T = M = None
+    k = 0
+    n = len(seeds) * steps
+    t0 = time.time()
+    for s in seeds:
+        st = TimeStepper(make_case(s), precision=_lib.SG_PREC_FP64, tol=tol, max_iter=500)
+        pj = st.pj
+        try:
+            for _ in range(skip):
+                st.step()
+            for _ in range(steps):
+                st.step()
+                if X is None:
+                    if level is None:
+                        level = pick_level(pj, max_width)
+                    H, W = level_dims(pj, level)
+                    X = torch.zeros((n, H + 2, W + 2, 8), device="cuda", dtype=torch.bfloat16)
+                    T = torch.zeros((n, H, W), device="cuda", dtype=torch.float32)
+                    M = torch.zeros((n, H, W), device="cuda", dtype=torch.float32)
+                    torch.cuda.synchronize()
+                features(pj, level, X[k], T[k], M[k])
+                k += 1
+            if progress:
+                progress(f"step pairs {k}/{n} ({time.time() - t0:.1f} s)")
+        finally:
+            st.close()
+    return X, T, M, level
+
+
 def train(X, T, M, epochs=10, batch=16, lr=2e-3, seed=0, weights=None, progress=None, lr_final=2e-4):
     """Adam on the normalised targets T / K (K = rms of T over the fluid
     points); cosine learning-rate decay from lr to lr_final.  Returns (weights,

@@ -44,17 +44,17 @@ class TimeStepper:
         computed; previous_pressure: start each Poisson solve from the last
         step's pressure instead of zero (the reference starts from zero,
         ProjectionP_Mod.f90:319; it solves for the full pressure, so the
-        last one is a close start); diagnostics: record a row of device
-        diagnostics after every step (read with history())."""
-        if learned and previous_pressure:
-            raise ValueError("the learned warm start and the previous-pressure start exclude each other")
+        last one is a close start), with learned: the last pressure plus the
+        net's correction predicted from its residual; diagnostics: record a
+        row of device diagnostics after every step (read with history())."""
         self.case = case
         g = case.grid
         self.cfl, self.vint, self.uref = cfl, vint, uref
         ph = physics(case.fluids)
         self.params = params_for(case.fluids, tol=case.tol if tol is None else tol, precision=precision,
                                  max_iter=max_iter, bc=case.bc, Re=ph["Re"], Fr=ph["Fr"],
-                                 warm_start=2 if learned else (1 if previous_pressure else 0))
+                                 warm_start=(3 if previous_pressure else 2) if learned else
+                                 (1 if previous_pressure else 0))
         # params_for takes nu from the fluids already
         self.pj = pj = Projection(g, self.params)
         pj.set_geometry(case.geo)
