@@ -364,11 +364,30 @@ def cpu_baseline(threads):
         if time.perf_counter() - t0 > 10.0 or n >= 20:
             break
     dt = (time.perf_counter() - t0) / n
+    # the same steps on one host thread (a few of them)
+    O.lib().orc_set_num_threads(1)
+    t1 = time.perf_counter()
+    n1 = 0
+    while True:
+        st.step()
+        n1 += 1
+        if time.perf_counter() - t1 > 5.0 or n1 >= 10:
+            break
+    dt1 = (time.perf_counter() - t1) / n1
+    O.lib().orc_set_num_threads(nthreads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(small.grid.ncells / dt / 1e6, 4), "unit": "Mcell-steps/s", "cores": nthreads,
             "kind": "port",
             "sample": f"{n} full time steps (transport, U/V interface, predictor with direct momentum solves, "
                       f"PCG-MG projection to 1e-8, correction) of the swarm at 256x256 after 2 warm-up steps, "
-                      f"{dt * 1e3:.0f} ms each; PCG-MG on {nthreads} host threads, the rest single-threaded"}
+                      f"{dt * 1e3:.0f} ms each; PCG-MG on {nthreads} host threads, the rest single-threaded",
+            "one_thread": {"value": round(small.grid.ncells / dt1 / 1e6, 4), "steps": n1},
+            "host": {"cpu_model": model, "nproc": os.cpu_count()}}
 
 
 def main():
