@@ -417,6 +417,7 @@ class Case:
     bc: tuple = (1, 1, 1, 0)   # W,E,S,N: Neumann W/E/S, Dirichlet N (ProjectionP_Mod.f90:55)
     solid: object = None       # solid(X, Y) -> (phiS, nxS, nyS), or None
     levelset: object = None    # levelset(X, Y) -> (phi, nx, ny), liquid where phi < 0
+    shape: tuple = ("none",)   # the solid as (kind, cx, cy, a, b) for the device geometry rebuild
 
 
 def _gas_bubble(cx, cy, r):
@@ -446,7 +447,8 @@ def case_dambreak(ni=512, nj=256):
     def ls(X, Y):
         return sdf_box(X, Y, 0.5, 0.5, 0.5, 0.5)  # > 0 outside the water column
     itf = interface_from_levelset(g, geo, ls)
-    return Case("dambreak", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-10, solid=solid, levelset=ls)
+    return Case("dambreak", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-10, solid=solid, levelset=ls,
+                shape=("box", 2.5, 0.6, 0.05, 0.15))
 
 
 def swarm_bubbles(seed=0, n=64, depth=0.8, rmin=0.01, rmax=0.03, ymin=0.1, ymax=0.6):
@@ -520,7 +522,8 @@ def case_waves(ni=1024, nj=1024, seed=0):
         nrm = np.sqrt(1.0 + ds * ds)
         return (Y - s) / nrm, np.broadcast_to(-ds / nrm, X.shape).copy(), np.broadcast_to(1.0 / nrm, X.shape).copy()
     itf = interface_from_levelset(g, geo, ls)
-    return Case("waves", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-8, solid=solid, levelset=ls)
+    return Case("waves", g, Fluids(), geo, itf, dt=0.25 * g.dx[0], tol=1e-8, solid=solid, levelset=ls,
+                shape=("ellipse", 4.0, yc, 0.2, 0.075))
 
 
 CASES = {"bubble": case_bubble, "dambreak": case_dambreak, "swarm": case_swarm, "waves": case_waves}

@@ -37,7 +37,8 @@ def physics(fluids):
 
 class TimeStepper:
     def __init__(self, case, precision=_lib.SG_PREC_FP64, tol=None, cfl=0.25, vint=0.0, uref=1.0, max_iter=500,
-                 learned=None, stags=None, learned_width=256, previous_pressure=False, diagnostics=False):
+                 learned=None, stags=None, learned_width=256, previous_pressure=False, diagnostics=False,
+                 device_geometry=False):
         """learned: (weights, level, out_scale, deflate) of a trained warm
         start, level None for the first multigrid level no wider than
         learned_width; stags: the (U, V) cases.StagGeometry if already
@@ -46,7 +47,10 @@ class TimeStepper:
         ProjectionP_Mod.f90:319; it solves for the full pressure, so the
         last one is a close start), with learned: the last pressure plus the
         net's correction predicted from its residual; diagnostics: record a
-        row of device diagnostics after every step (read with history())."""
+        row of device diagnostics after every step (read with history());
+        device_geometry: build the cut-cell geometry of case.shape on the
+        device (sg_geometry_rebuild, equal to the host builder bit for bit)
+        instead of uploading case.geo and the host's staggered geometry."""
         self.case = case
         g = case.grid
         self.cfl, self.vint, self.uref = cfl, vint, uref
@@ -57,10 +61,13 @@ class TimeStepper:
                                  (1 if previous_pressure else 0))
         # params_for takes nu from the fluids already
         self.pj = pj = Projection(g, self.params)
-        pj.set_geometry(case.geo)
-        U, V = stags if stags is not None else C.staggered_geometry(g, case.geo, case.solid)
-        pj.set_staggered_geometry(0, U)
-        pj.set_staggered_geometry(1, V)
+        if device_geometry:
+            pj.geometry_rebuild(*case.shape)
+        else:
+            pj.set_geometry(case.geo)
+            U, V = stags if stags is not None else C.staggered_geometry(g, case.geo, case.solid)
+            pj.set_staggered_geometry(0, U)
+            pj.set_staggered_geometry(1, V)
         itf = case.itf
         pj.upload("VOF", itf.vof)
         pj.upload("PHI", itf.phi)
