@@ -314,7 +314,9 @@ int sg_particles_track(sg_ctx* ctx, double dt, const sg_particle_params* p, int*
  * array of sg_set_staggered_geometry: the device state sg_set_geometry +
  * sg_set_staggered_geometry would set, computed on the device.  box: half
  * width a, half height b; circle: radius a; ellipse: semi-axes a, b; none:
- * no solid.  Needs ni, nj >= 4. */
+ * no solid.  Needs ni, nj >= 4.  The first call allocates the rebuild's work
+ * arrays (4 ghosted + 2 cell planes); later calls allocate nothing and wait
+ * on the host once per pass of the closing rule (1-3 passes). */
 #define SG_SOLID_NONE 0
 #define SG_SOLID_BOX 1
 #define SG_SOLID_CIRCLE 2
