@@ -47,7 +47,8 @@ typedef struct sg_params {
     int precision;   /* SG_PREC_FP64 or SG_PREC_MIXED */
     int max_iter;    /* PCG iteration cap (ProjectionP_Mod.f90:108 uses 50) */
     double tol;      /* stop at ||b-Ap||_2 <= tol*||b||_2 */
-    int mg_pre, mg_post; /* smoothing sweeps per level (ProjectionP_Mod.f90:126: 1) */
+    int mg_pre, mg_post; /* smoothing sweeps per level (ProjectionP_Mod.f90:126: 1); the
+                            fused cycle kernels carry one each: only 1 is accepted */
     int warm_start;  /* 0: x0 = 0 (ProjectionP_Mod.f90:319); 1: x0 = previous solution;
                         2: learned warm start + deflation vector (sg_set_learned);
                         3: previous solution + the learned correction predicted from

@@ -76,6 +76,10 @@ static int check_params(const sg_params* p) {
         set_error("mg_coarse_krylov must be in [0, 64]");
         return SG_ERR_ARG;
     }
+    if (p->mg_pre != 1 || p->mg_post != 1) {
+        set_error("mg_pre and mg_post must be 1 (the cycle kernels fuse one sweep each)");
+        return SG_ERR_ARG;
+    }
     if (p->warm_start < 0 || p->warm_start > 3) {
         set_error("warm_start must be 0, 1, 2 or 3");
         return SG_ERR_ARG;
