@@ -1477,6 +1477,9 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
     const dim3 rblk(256), rgrd((ring + 255) / 256);
     const dim3 pblk(256), pgrd(1024);  // grid-stride loops over the listed pairs / cells
     const dim3 rcblk(64, RECON_TY), rcgrd((c->ni + 63) / 64, (c->nj + RECON_TY - 1) / RECON_TY);
+    // sweeps: 32x8 tiles (3% faster per transport call than 64x4, 256x1 or
+    // 128x2 at 2048^2: the y stencil's rows stay within fewer tiles)
+    const dim3 sblk(32, 8), sgrd((c->ni + 31) / 32, (c->nj + 7) / 8);
     const double dtv = dt / (double)nsub;
     hipStream_t st = c->stream;
     // three counters per reconstruction, three reconstructions per sub-step
@@ -1506,10 +1509,10 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         else
             hipLaunchKernelGGL((k_flux_pre<false>), pgrd, pblk, 0, st, G, s1, dtv, B.ifc, c1, B.fr, B.fl);
         if (k % 2 == 1)
-            hipLaunchKernelGGL((k_sweep<true, true>), grd, blk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
+            hipLaunchKernelGGL((k_sweep<true, true>), sgrd, sblk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
                                B.tl, B.vt, B.pt);
         else
-            hipLaunchKernelGGL((k_sweep<false, true>), grd, blk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
+            hipLaunchKernelGGL((k_sweep<false, true>), sgrd, sblk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
                                B.tl, B.vt, B.pt);
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, B.vt, B.pt);
         // second sweep: (vt, pt) -> (vof, phi), explicit divergence
@@ -1520,10 +1523,10 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         else
             hipLaunchKernelGGL((k_flux_pre<true>), pgrd, pblk, 0, st, G, s2, dtv, B.ifc, c2, B.fr, B.fl);
         if (k % 2 == 1)
-            hipLaunchKernelGGL((k_sweep<false, false>), grd, blk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
+            hipLaunchKernelGGL((k_sweep<false, false>), sgrd, sblk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
                                nullptr, vof, phi);
         else
-            hipLaunchKernelGGL((k_sweep<true, false>), grd, blk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
+            hipLaunchKernelGGL((k_sweep<true, false>), sgrd, sblk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
                                nullptr, vof, phi);
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, vof, phi);
         // reconstruction (full normals out) and re-distancing
