@@ -15,12 +15,15 @@ Three solver paths run the same steps:
   * plain FP64 PCG-multigrid from x0 = 0 (the reference's start);
   * mixed precision (FP32 PCG vectors and multigrid cycle; FP64 x, dots and
     residual recomputation) from x0 = 0;
-  * learned + mixed: the mixed solver started from the learned warm start
-    (the net is trained inside this run, with the library's own kernels, on
-    (rhs, p) pairs the plain solver makes for seeded 512x512 swarms; it runs
-    on the 256x256 level of the hierarchy).
-A fourth full-step path, reported beside them, runs the mixed solver from the
-last step's pressure instead of zero.
+  * learned + mixed: the mixed solver started from the last step's pressure
+    plus the learned correction (the net predicts it from the residual that
+    pressure leaves, Galerkin-scaled; warm_start = 3).  The net is trained
+    inside this run, with the library's own kernels, on (rhs, p) pairs the
+    plain solver makes for seeded 512x512 swarms; it runs on the 256x256
+    level of the hierarchy.
+Two more full-step paths are reported beside them: the learned start alone
+(from zero, warm_start = 2) and the mixed solver from the last step's
+pressure without the net (warm_start = 1).
 ``value`` is the learned + mixed path's Mcell-steps/s over full time steps
 (the headline metric); ``poisson_ms_per_step`` gives the Poisson solve time
 per step of every path, inside those steps and, separately, on a fixed
@@ -420,10 +423,16 @@ def main():
     full = {}
     for name, prec, lrn in paths:
         log(f"full time steps, {name}")
-        full[name] = stepping(case, stags, prec, a.steps, a.warmup, learned=lrn, width=a.nn_width)
-    # not the metric's path: mixed precision started from the last step's
-    # pressure (full steps only; on fixed fields every repeat would start
-    # from the answer)
+        # the learned path starts from the last pressure plus the net's correction
+        full[name] = stepping(case, stags, prec, a.steps, a.warmup, learned=lrn, width=a.nn_width,
+                              previous=lrn is not None)
+    # beside the metric's path (full steps only; on fixed fields every repeat
+    # would start from the answer): the learned start from zero, and the last
+    # step's pressure without the net
+    if w is not None:
+        log("full time steps, learned start from zero")
+        full["learned_mixed_zero_start"] = stepping(case, stags, 1, a.steps, a.warmup, learned=(w, K),
+                                                    width=a.nn_width)
     log("full time steps, mixed from the previous pressure")
     full["mixed_previous_pressure"] = stepping(case, stags, 1, a.steps, a.warmup, previous=True)
     # the projection alone on one fixed set of predictor fields
@@ -456,7 +465,8 @@ def main():
                                "full time step: CFL dt, CLSVOF transport (6 sub-steps), U/V interface, box ghost "
                                "layer, momentum predictor, projection (PCG-multigrid to 1e-8), correction",
                    "ni": a.ni, "nj": a.nj,
-                   "path": "learned warm start, mixed precision" if w is not None else "mixed precision"},
+                   "path": ("previous pressure + learned correction (warm_start 3), mixed precision" if w is not None
+                            else "mixed precision")},
         "poisson_ms_per_step": {k: round(v["poisson_ms"], 4) for k, v in full.items()},
         "ms_per_step_by_path": {k: round(v["ms_per_step"], 4) for k, v in full.items()},
         "mcell_steps_per_s": {k: round(n / (v["ms_per_step"] * 1e-3) / 1e6, 3) for k, v in full.items()},
