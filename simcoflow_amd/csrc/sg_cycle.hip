@@ -567,6 +567,7 @@ struct TailArgs {
     const int* on; // the PCG's convergence flag (the cycle runs while *on), or null
     int krylov;    // inner flexible-CG iterations on the first tail level (0: one V-cycle)
     int pbase;     // LDS offset of the inner iteration's direction vector
+    int invbase;   // LDS offset of the staged dense inverse
 };
 
 // offset of cell (i,j) in a level's LDS arrays (compact layout, 32-bit)
@@ -700,16 +701,18 @@ __device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
         __syncthreads();
     }
     {
-        // dense solve on the coarsest level: four threads per row
+        // dense solve on the coarsest level: four threads per row, the
+        // inverse staged in LDS
         const TailLevel<T>& L = a.L[last];
         const Dims& D = L.cd;
         const int n = D.ni * D.nj;
         const T* b = tarr(lds, L, TA_B);
+        const T* inv = lds + a.invbase;
         T* x = tarr(lds, L, TA_R);
         const int row = tid >> 2, quarter = tid & 3;
         T s = T(0);
         if (row < n)
-            for (int q = quarter; q < n; q += 4) s += a.inv[(size_t)row * n + q] * b[D.off(q % D.ni, q / D.ni)];
+            for (int q = quarter; q < n; q += 4) s += inv[row * n + q] * b[D.off(q % D.ni, q / D.ni)];
         s += __shfl_xor(s, 1, 64);
         s += __shfl_xor(s, 2, 64);
         if (row < n && quarter == 0) x[D.off(row % D.ni, row / D.ni)] = s;
@@ -782,6 +785,11 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
     for (int k = tid; k < lds_elems; k += TAIL_THREADS) lds[k] = T(0);
     __syncthreads();
     tail_stage<T>(lds, a);
+    {
+        const Dims& Dc = a.L[a.n - 1].d;
+        const int nc = Dc.ni * Dc.nj;
+        for (int k = tid; k < nc * nc; k += TAIL_THREADS) lds[a.invbase + k] = a.inv[k];
+    }
     __syncthreads();
     const TailLevel<T>& L = a.L[0];
     const Dims& D = L.cd;
@@ -958,6 +966,8 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done, const 
         }
         off += (int)compact(c->lev[lt].d).alloc_elems();
     }
+    ta.invbase = off;
+    off += (int)(c->lev[nl - 1].d.ni * c->lev[nl - 1].d.nj) * (int)(c->lev[nl - 1].d.ni * c->lev[nl - 1].d.nj);
     const size_t lds_bytes = (size_t)off * sizeof(T);
     if (lds_bytes > TAIL_LDS_MAX) {
         set_error("multigrid: tail levels exceed the LDS budget");
