@@ -62,6 +62,7 @@ CASES = [
     ("dambreak-rotation", lambda: C.case_dambreak(128, 64), "rotation"),
     ("waves-vortex", lambda: C.case_waves(96, 48, seed=2), "vortex"),
     ("swarm-vortex", lambda: C.case_swarm(128, 128, seed=3), "vortex"),
+    ("dambreak-odd-vortex", lambda: C.case_dambreak(131, 67), "vortex"),
 ]
 
 
@@ -228,6 +229,7 @@ UV_CASES = [
     ("bubble", lambda: C.case_bubble(64, 128)),
     ("dambreak", lambda: C.case_dambreak(128, 64)),
     ("waves", lambda: C.case_waves(96, 48, seed=2)),
+    ("dambreak-odd", lambda: C.case_dambreak(131, 67)),
 ]
 
 

@@ -23,6 +23,7 @@ CASES = [
     ("dambreak", lambda: C.case_dambreak(128, 64)),
     ("waves", lambda: C.case_waves(96, 48, seed=2)),
     ("swarm", lambda: C.case_swarm(128, 128, seed=3)),
+    ("dambreak-odd", lambda: C.case_dambreak(131, 67)),
 ]
 
 

@@ -17,6 +17,9 @@ SMALL = [
     ("dambreak", lambda: C.case_dambreak(128, 64)),
     ("swarm", lambda: C.case_swarm(128, 128, seed=3)),
     ("waves", lambda: C.case_waves(96, 48, seed=2)),
+    # odd sizes: the last coarse cell of each level has one fine child per axis
+    ("bubble-odd", lambda: C.case_bubble(37, 75)),
+    ("dambreak-odd", lambda: C.case_dambreak(131, 67)),
 ]
 
 
@@ -151,7 +154,7 @@ def test_coarse_krylov_against_plain_cycle(oracle_lib, name, mk, precision):
 
 
 def test_swarm_full_size_fp64_and_mixed():
-    """BASELINE-size swarm (2048x2048): both precisions reach 1e-8 and agree."""
+    """The 2048x2048 swarm: both precisions reach 1e-8 and agree."""
     case = C.case_swarm(2048, 2048, seed=0)
     out = {}
     for prec in (0, 1):
