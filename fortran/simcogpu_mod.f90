@@ -87,7 +87,7 @@ module simcogpu
   public :: sg_particles_set, sg_particles_get, sg_particles_track
   public :: sg_state_bytes, sg_state_save, sg_state_load
   public :: sg_diagnostics_record, sg_diagnostics_read
-  public :: sg_geometry_rebuild, sg_object_force
+  public :: sg_geometry_rebuild, sg_object_force, sg_geometry_snapshot, sg_inter_new_var
   public :: sg_error_message
 
   interface
@@ -255,6 +255,17 @@ module simcogpu
       type(c_ptr), value :: ctx
       real(c_double), value :: vs
       real(c_double), intent(out) :: force, clp, clf
+      integer(c_int) :: rc
+    end function
+    function sg_geometry_snapshot(ctx) bind(C, name="sg_geometry_snapshot") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function sg_inter_new_var(ctx, vb) bind(C, name="sg_inter_new_var") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: vb
       integer(c_int) :: rc
     end function
     function sg_diagnostics_record(ctx) bind(C, name="sg_diagnostics_record") result(rc)

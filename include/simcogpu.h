@@ -336,6 +336,17 @@ int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape);
  * NULL.  The host integrates ObjectMovement with it (asy = (force - M g)/M,
  * Solver_Mod.f90:277-283). */
 int sg_object_force(sg_ctx* ctx, double vs, double* force, double* clp, double* clf);
+/* the velocities after the solid moved (replaces InterNewVar,
+ * Solver_Mod.f90:360-495): sg_geometry_snapshot keeps the geometry of the
+ * step's start (the part CopyNewCell's PCellO/UCellO/VCellO give it: the P
+ * apertures AE, AN and the U/V solid fractions; its first call allocates
+ * them); after sg_geometry_rebuild, sg_inter_new_var sets u = 0 on U faces
+ * whose cell the solid uncovered, v = vb on such V faces, and scales u / v
+ * on P faces whose open fraction shrank below 0.6 of its old value (either
+ * above 0.5) by old/new.  The reference's other computations there are
+ * unused and left out. */
+int sg_geometry_snapshot(sg_ctx* ctx);
+int sg_inter_new_var(sg_ctx* ctx, double vb);
 /* test hook: one staggered-geometry array of U (which = 0) or V cells, in the
  * order of sg_stag_geometry's double arrays (vofS nxS nyS phiS WlLh delh ccx
  * ccy: ni*nj; aE aW aN aS EtaE EtaN AlE AlN DAlE DAlN SxE SyN fcex fcey fcnx

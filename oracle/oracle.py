@@ -537,3 +537,16 @@ def object_force(g, geo, V, p, v, vs, Re):
     f = L.orc_object_force(g.ni, g.nj, *[_d(a) for a in arrs], moex.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                            *[_d(a) for a in rest], float(vs), float(Re), ctypes.byref(clp), ctypes.byref(clf))
     return f, clp.value, clf.value
+
+
+def inter_new_var(g, geo_old, geo_new, u, v, vb):
+    """InterNewVar (Solver_Mod.f90:360-495) on host arrays (cases.Geometry at
+    the step's start and now); returns the new (u, v)."""
+    L = lib()
+    c = np.ascontiguousarray
+    L.orc_inter_new_var.argtypes = [ctypes.c_int, ctypes.c_int] + [_D] * 8 + [ctypes.c_double, _D, _D]
+    arrs = [c(a, dtype=np.float64) for a in (geo_old.vofSU, geo_new.vofSU, geo_old.vofSV, geo_new.vofSV, geo_old.aE,
+                                             geo_new.aE, geo_old.aN, geo_new.aN)]
+    uo, vo = np.array(u, dtype=np.float64, order="C"), np.array(v, dtype=np.float64, order="C")
+    L.orc_inter_new_var(g.ni, g.nj, *[_d(a) for a in arrs], float(vb), _d(uo), _d(vo))
+    return uo, vo

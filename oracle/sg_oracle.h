@@ -207,6 +207,10 @@ double orc_object_force(int ni, int nj, const double* dx, const double* vofS, co
                         const double* delhV, const double* v, double vs, double Rey, double* clp_out,
                         double* clf_out);
 
+void orc_inter_new_var(int ni, int nj, const double* vofSUo, const double* vofSU, const double* vofSVo,
+                       const double* vofSV, const double* aEo, const double* aE, const double* aNo,
+                       const double* aN, double vb, double* u, double* v);
+
 /* sg_oracle_particles.c -- Lagrangian particles (Particles.f90:71-249) */
 typedef struct {
     double rho_l, rho_g;  /* row, roa */

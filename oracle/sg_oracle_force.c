@@ -70,3 +70,24 @@ double orc_object_force(int ni, int nj, const double* dx, const double* vofS, co
     if (clf_out) *clf_out = clf;
     return clf - clp;
 }
+
+/* InterNewVar (Solver_Mod.f90:360-495) in its loop order: U faces whose U
+ * cell was full solid at time n and is not now get u = 0, V faces likewise
+ * v = vb; a P cell's north / east face whose open fraction shrank by more
+ * than 0.6 (new > old / 0.6, either above 0.5) scales v / u by old/new.  The
+ * reference's uleft/uright/vbot/vtop and the MsCe lookups are computed but
+ * never used, so they are left out.  u, v ghosted (i + j*(ni+2)), in/out;
+ * aE/aN (new) and aEo/aNo (time n) ghosted; vofSU/V, vofSUo/Vo cell arrays. */
+void orc_inter_new_var(int ni, int nj, const double* vofSUo, const double* vofSU, const double* vofSVo,
+                       const double* vofSV, const double* aEo, const double* aE, const double* aNo,
+                       const double* aN, double vb, double* u, double* v) {
+    for (int i = 1; i <= ni; ++i)
+        for (int j = 1; j <= nj; ++j) {
+            if (CL(vofSUo, i, j) > 1.0 - EPSI && CL(vofSU, i, j) < 1.0 - EPSI) GH(u, i, j) = 0.0;
+            if (CL(vofSVo, i, j) > 1.0 - EPSI && CL(vofSV, i, j) < 1.0 - EPSI) GH(v, i, j) = vb;
+            const double nN = GH(aN, i, j), oN = GH(aNo, i, j);
+            if (nN > EPSI && oN > EPSI && (nN > 0.5 || oN > 0.5) && oN / nN < 0.6) GH(v, i, j) = GH(v, i, j) * oN / nN;
+            const double nE = GH(aE, i, j), oE = GH(aEo, i, j);
+            if (nE > EPSI && oE > EPSI && (nE > 0.5 || oE > 0.5) && oE / nE < 0.6) GH(u, i, j) = GH(u, i, j) * oE / nE;
+        }
+}

@@ -603,6 +603,18 @@ int sg_object_force(sg_ctx* ctx, double vs, double* force, double* clp, double* 
     return object_force(c, vs, force, clp, clf);
 }
 
+int sg_geometry_snapshot(sg_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    return geometry_snapshot(c);
+}
+
+int sg_inter_new_var(sg_ctx* ctx, double vb) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    return inter_new_var(c, vb);
+}
+
 int sg_stag_download(sg_ctx* ctx, int which, int k, void* host) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     if (!c || !host || (which != 0 && which != 1) || k < 0 || k > STG_COUNT + 1) {

@@ -551,7 +551,71 @@ __global__ void __launch_bounds__(NTF_FORCE) k_force_fin(const double* part, int
     }
 }
 
+struct NewVarIn {
+    int ni, nj;
+    const double *vofSUo, *vofSU, *vofSVo, *vofSV;  // cell arrays
+    const double *aEo, *aE, *aNo, *aN;              // ghosted P apertures
+    double vb;
+    double *u, *v;                                   // ghosted
+};
+
+// InterNewVar (Solver_Mod.f90:360-495): every (i,j) reads and writes only
+// its own u and v (the reference's uleft/uright/vbot/vtop and mass-cell
+// lookups are computed but unused), so the cells are independent
+__global__ void __launch_bounds__(256) k_inter_new_var(NewVarIn a) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    if (i >= a.ni || j >= a.nj) return;
+    const size_t c = (size_t)i + (size_t)j * a.ni, g = gix(a.ni, i, j);
+    double u = a.u[g], v = a.v[g];
+    if (a.vofSUo[c] > 1.0 - EPSI && a.vofSU[c] < 1.0 - EPSI) u = 0.0;
+    if (a.vofSVo[c] > 1.0 - EPSI && a.vofSV[c] < 1.0 - EPSI) v = a.vb;
+    const double nN = a.aN[g], oN = a.aNo[g];
+    if (nN > EPSI && oN > EPSI && (nN > 0.5 || oN > 0.5) && oN / nN < 0.6) v = v * oN / nN;
+    const double nE = a.aE[g], oE = a.aEo[g];
+    if (nE > EPSI && oE > EPSI && (nE > 0.5 || oE > 0.5) && oE / nE < 0.6) u = u * oE / nE;
+    a.u[g] = u;
+    a.v[g] = v;
+}
+
 }  // namespace
+
+int geometry_snapshot(Ctx* c) {
+    GeoScratch& W = c->geo;
+    const size_t ncell = (size_t)c->ni * c->nj, nghost = (size_t)(c->ni + 2) * (c->nj + 2);
+    if (!W.old[0]) {
+        W.old[0] = (double*)dev_alloc(c, nghost * sizeof(double));
+        W.old[1] = (double*)dev_alloc(c, nghost * sizeof(double));
+        W.old[2] = (double*)dev_alloc(c, ncell * sizeof(double));
+        W.old[3] = (double*)dev_alloc(c, ncell * sizeof(double));
+        if (!W.old[0] || !W.old[1] || !W.old[2] || !W.old[3]) {
+            set_error("sg_geometry_snapshot: device allocation failed");
+            return SG_ERR_ALLOC;
+        }
+    }
+    const int src[4] = {SG_F_AE, SG_F_AN, SG_F_VOFSU, SG_F_VOFSV};
+    for (int k = 0; k < 4; ++k)
+        SG_HIP(hipMemcpyAsync(W.old[k], c->field[src[k]], c->field_bytes[src[k]], hipMemcpyDeviceToDevice, c->stream));
+    W.have_old = true;
+    return SG_OK;
+}
+
+int inter_new_var(Ctx* c, double vb) {
+    GeoScratch& W = c->geo;
+    if (!W.have_old) {
+        set_error("sg_inter_new_var: no geometry snapshot (call sg_geometry_snapshot at the start of the step)");
+        return SG_ERR_STATE;
+    }
+    auto F = [&](int f) { return (double*)c->field[f]; };
+    const NewVarIn a{c->ni,     c->nj,        W.old[2],       F(SG_F_VOFSU), W.old[3], F(SG_F_VOFSV),
+                     W.old[0],  F(SG_F_AE),   W.old[1],       F(SG_F_AN),    vb,       F(SG_F_U),
+                     F(SG_F_V)};
+    hipLaunchKernelGGL(k_inter_new_var, grid2d(c->ni, c->nj), block2d(), 0, c->stream, a);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
 
 int object_force(Ctx* c, double vs, double* force, double* clp, double* clf) {
     const int ni = c->ni, nj = c->nj;

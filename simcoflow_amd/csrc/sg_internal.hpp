@@ -115,6 +115,8 @@ struct GeoScratch {
     int* changed = nullptr;
     int* moex[2] = {nullptr, nullptr};                      // U, V momentum-exchange cells
     double* next[2] = {nullptr, nullptr};                   // dx of column i+1, dy of row j+1
+    double* old[4] = {nullptr, nullptr, nullptr, nullptr};  // aE, aN, vofSU, vofSV at the step's start
+    bool have_old = false;
 };
 
 // ring of per-step diagnostics rows written on the device
@@ -336,6 +338,8 @@ int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, doub
 int closed_box_bc(Ctx* c);
 int geometry_rebuild(Ctx* c, const sg_solid_shape* shape);
 int object_force(Ctx* c, double vs, double* force, double* clp, double* clf);
+int geometry_snapshot(Ctx* c);
+int inter_new_var(Ctx* c, double vb);
 int diagnostics_record(Ctx* c);
 int diagnostics_read(Ctx* c, int max_rows, double* rows, int* n_rows, long long* n_lost);
 

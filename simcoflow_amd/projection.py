@@ -279,6 +279,14 @@ class Projection:
         check(self.lib.sg_object_force(self.ctx, float(vs), ctypes.byref(f), ctypes.byref(a), ctypes.byref(b)))
         return f.value, a.value, b.value
 
+    def geometry_snapshot(self):
+        """Keeps the geometry of the step's start for inter_new_var."""
+        check(self.lib.sg_geometry_snapshot(self.ctx))
+
+    def inter_new_var(self, vb=0.0):
+        """InterNewVar on the device velocities after a geometry rebuild."""
+        check(self.lib.sg_inter_new_var(self.ctx, float(vb)))
+
     def stag_download(self, which):
         """The device's staggered geometry of U (0) or V (1) cells as a dict
         of the cases.StagGeometry array names."""

@@ -89,3 +89,43 @@ def test_boom_bar_corner_case(O):
     f, clp, _ = O.object_force(g, geo, V, p, np.zeros_like(p), 0.0, 1e6)
     assert clp == pytest.approx(-7.0 * 0.6 * g.dx[2], rel=1e-15)
     assert f == -clp
+
+
+def _geo_arrays(n, aE=None, aN=None, su=None, sv=None):
+    from types import SimpleNamespace
+    gh = np.ones((n + 2, n + 2))
+    return SimpleNamespace(aE=gh.copy() if aE is None else aE, aN=gh.copy() if aN is None else aN,
+                           vofSU=np.zeros((n, n)) if su is None else su, vofSV=np.zeros((n, n)) if sv is None else sv)
+
+
+def test_inter_new_var_known_answers(O):
+    """InterNewVar (Solver_Mod.f90:360-495) on hand-made geometries: an
+    uncovered U cell gets u = 0, an uncovered V cell v = vb; a face whose
+    open fraction grew from 0.4 to 0.8 scales its velocity by 0.5, one that
+    grew from 0.5 to 0.8 (ratio 0.625), from 0.3 to 0.45 (neither above 0.5)
+    or that is shut stays; nothing else changes."""
+    n = 6
+    g = C.uniform_grid(n, n, 1.0, 1.0)
+    su_o = np.zeros((n, n)); su_o[2, 3] = 1.0
+    su_n = np.zeros((n, n)); su_n[2, 3] = 0.3
+    sv_o = np.zeros((n, n)); sv_o[4, 1] = 1.0
+    sv_n = np.zeros((n, n)); sv_n[4, 1] = 0.0
+    aN_o = np.ones((n + 2, n + 2)); aN_n = np.ones((n + 2, n + 2))
+    aN_o[1 + 1, 1 + 1], aN_n[1 + 1, 1 + 1] = 0.4, 0.8   # scaled by 0.5
+    aN_o[1 + 3, 1 + 2], aN_n[1 + 3, 1 + 2] = 0.5, 0.8   # ratio 0.625: kept
+    aN_o[1 + 0, 1 + 4], aN_n[1 + 0, 1 + 4] = 0.3, 0.45  # neither above 0.5: kept
+    aN_o[1 + 5, 1 + 5], aN_n[1 + 5, 1 + 5] = 0.4, 0.0   # shut now: kept
+    aE_o = np.ones((n + 2, n + 2)); aE_n = np.ones((n + 2, n + 2))
+    aE_o[1 + 3, 1 + 0], aE_n[1 + 3, 1 + 0] = 0.2, 0.9   # scaled by 0.2/0.9
+    old = _geo_arrays(n, aE=aE_o, aN=aN_o, su=su_o, sv=sv_o)
+    new = _geo_arrays(n, aE=aE_n, aN=aN_n, su=su_n, sv=sv_n)
+    rng = np.random.default_rng(0)
+    u = rng.standard_normal((n + 2, n + 2))
+    v = rng.standard_normal((n + 2, n + 2))
+    uo, vo = O.inter_new_var(g, old, new, u, v, vb=0.7)
+    eu, ev = u.copy(), v.copy()
+    eu[1 + 2, 1 + 3] = 0.0
+    ev[1 + 4, 1 + 1] = 0.7
+    ev[1 + 1, 1 + 1] = v[1 + 1, 1 + 1] * 0.4 / 0.8
+    eu[1 + 3, 1 + 0] = u[1 + 3, 1 + 0] * 0.2 / 0.9
+    assert np.array_equal(uo, eu) and np.array_equal(vo, ev)
