@@ -209,10 +209,14 @@ int diagnostics_read(Ctx* c, int max_rows, double* rows, int* n_rows, long long*
         first = cnt - SG_DIAG_RING;
     }
     const long long n = std::min<long long>(cnt - first, (long long)max_rows);
-    for (long long r = 0; r < n; ++r) {
-        const long long slot = (first + r) % SG_DIAG_RING;
-        SG_HIP(hipMemcpyAsync(rows + r * SG_DIAG_COLS, c->diag.ring.rows + slot * SG_DIAG_COLS,
-                              SG_DIAG_COLS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // the rows are contiguous in the ring up to its end, then from its start
+    long long done = 0;
+    while (done < n) {
+        const long long slot = (first + done) % SG_DIAG_RING;
+        const long long len = std::min<long long>(n - done, SG_DIAG_RING - slot);
+        SG_HIP(hipMemcpyAsync(rows + done * SG_DIAG_COLS, c->diag.ring.rows + slot * SG_DIAG_COLS,
+                              (size_t)len * SG_DIAG_COLS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        done += len;
     }
     SG_HIP(hipStreamSynchronize(c->stream));
     c->diag.read = first + n;
