@@ -250,7 +250,9 @@ int jacobi_solve(Ctx* c, const Mat5& A, const double* b, double* x, double tol, 
     // sweep k (1-based) writes x when k is odd, y when even
     hipLaunchKernelGGL(k_jac, g, blk, 0, st, n, A, b, (const double*)y, x, part, (const BiScalars*)sc, 1);
     hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_JINIT, sc, tol);
-    int k = 1, batch = 5;
+    // the first host check after the sweep count these rows typically need
+    // (7 at 2048^2; sweeps past convergence return at once)
+    int k = 1, batch = 7;
     for (;;) {
         for (int q = 0; q < batch && k < JACOBI_MAX; ++q, ++k) {
             const bool odd = ((k + 1) & 1) != 0;  // sweep k+1
