@@ -88,6 +88,7 @@ module simcogpu
   public :: sg_state_bytes, sg_state_save, sg_state_load
   public :: sg_diagnostics_record, sg_diagnostics_read
   public :: sg_geometry_rebuild, sg_object_force, sg_geometry_snapshot, sg_inter_new_var
+  public :: sg_set_moving_solid, sg_get_moving_solid
   public :: sg_error_message
 
   interface
@@ -255,6 +256,20 @@ module simcogpu
       type(c_ptr), value :: ctx
       real(c_double), value :: vs
       real(c_double), intent(out) :: force, clp, clf
+      integer(c_int) :: rc
+    end function
+    function sg_set_moving_solid(ctx, shape, us, vs, asx, asy) bind(C, name="sg_set_moving_solid") result(rc)
+      import :: c_int, c_ptr, c_double, sg_solid_shape
+      type(c_ptr), value :: ctx
+      type(sg_solid_shape), intent(in) :: shape
+      real(c_double), value :: us, vs, asx, asy
+      integer(c_int) :: rc
+    end function
+    function sg_get_moving_solid(ctx, shape, us, vs) bind(C, name="sg_get_moving_solid") result(rc)
+      import :: c_int, c_ptr, c_double, sg_solid_shape
+      type(c_ptr), value :: ctx
+      type(sg_solid_shape), intent(out) :: shape
+      real(c_double), intent(out) :: us, vs
       integer(c_int) :: rc
     end function
     function sg_geometry_snapshot(ctx) bind(C, name="sg_geometry_snapshot") result(rc)

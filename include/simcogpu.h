@@ -336,6 +336,18 @@ int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape);
  * NULL.  The host integrates ObjectMovement with it (asy = (force - M g)/M,
  * Solver_Mod.f90:277-283). */
 int sg_object_force(sg_ctx* ctx, double vs, double* force, double* clp, double* clf);
+/* a solid that moves inside the transport (ObjectMovement + SolidVolumeFraction
+ * on the P cells after each sub-step's sweeps, Clsvof_Mod.f90:920-921): once
+ * armed with a shape (kind != NONE), velocity (us, vs) and acceleration
+ * (asx, asy), every sub-step of sg_clsvof_advect_resident / sg_clsvof_advect
+ * moves the shape by ObjectMovement's kinematics (:2384-2398) and recomputes
+ * the P cells' PHIS, NXS, NYS, VOFS for it (snapped below epsi and above
+ * 1 - epsi) and carries VOF onto the new fluid part of each cell
+ * (:2906-2917).  sg_get_moving_solid reads the moved shape and velocity
+ * back; kind NONE disarms.  The U/V cells and apertures follow with
+ * sg_geometry_rebuild after the transport, as Grid_Preprocess does. */
+int sg_set_moving_solid(sg_ctx* ctx, const sg_solid_shape* shape, double us, double vs, double asx, double asy);
+int sg_get_moving_solid(sg_ctx* ctx, sg_solid_shape* shape, double* us, double* vs);
 /* the velocities after the solid moved (replaces InterNewVar,
  * Solver_Mod.f90:360-495): sg_geometry_snapshot keeps the geometry of the
  * step's start (the part CopyNewCell's PCellO/UCellO/VCellO give it: the P

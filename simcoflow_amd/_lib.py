@@ -141,6 +141,9 @@ PROTOTYPES = {
     "sg_geometry_rebuild": ([_CTX, ctypes.POINTER(SolidShape)], ctypes.c_int),
     "sg_stag_download": ([_CTX, ctypes.c_int, ctypes.c_int, _V], ctypes.c_int),
     "sg_object_force": ([_CTX, ctypes.c_double, _D, _D, _D], ctypes.c_int),
+    "sg_set_moving_solid": ([_CTX, ctypes.POINTER(SolidShape), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double], ctypes.c_int),
+    "sg_get_moving_solid": ([_CTX, ctypes.POINTER(SolidShape), _D, _D], ctypes.c_int),
     "sg_geometry_snapshot": ([_CTX], ctypes.c_int),
     "sg_inter_new_var": ([_CTX, ctypes.c_double], ctypes.c_int),
     "sg_diagnostics_record": ([_CTX], ctypes.c_int),

@@ -603,6 +603,32 @@ int sg_object_force(sg_ctx* ctx, double vs, double* force, double* clp, double* 
     return object_force(c, vs, force, clp, clf);
 }
 
+int sg_set_moving_solid(sg_ctx* ctx, const sg_solid_shape* shape, double us, double vs, double asx, double asy) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !shape || shape->kind < SG_SOLID_NONE || shape->kind > SG_SOLID_ELLIPSE) {
+        set_error("sg_set_moving_solid: need a shape of a known kind");
+        return SG_ERR_ARG;
+    }
+    c->moving.shape = *shape;
+    c->moving.us = us;
+    c->moving.vs = vs;
+    c->moving.asx = asx;
+    c->moving.asy = asy;
+    return SG_OK;
+}
+
+int sg_get_moving_solid(sg_ctx* ctx, sg_solid_shape* shape, double* us, double* vs) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !shape) {
+        set_error("sg_get_moving_solid: null argument");
+        return SG_ERR_ARG;
+    }
+    *shape = c->moving.shape;
+    if (us) *us = c->moving.us;
+    if (vs) *vs = c->moving.vs;
+    return SG_OK;
+}
+
 int sg_geometry_snapshot(sg_ctx* ctx) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     if (!c) return SG_ERR_ARG;

@@ -1528,6 +1528,17 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         else
             hipLaunchKernelGGL((k_sweep<true, false>), sgrd, sblk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
                                nullptr, vof, phi);
+        if (c->moving.shape.kind != SG_SOLID_NONE) {
+            // ObjectMovement (Clsvof_Mod.f90:2384-2398) over the sub-step,
+            // then SolidVolumeFraction on the P cells (:920-921)
+            auto& m = c->moving;
+            m.shape.cy = m.shape.cy + 0.5 * m.asy * (dtv * dtv) + m.vs * dtv;
+            m.shape.cx = m.shape.cx + 0.5 * m.asx * (dtv * dtv) + m.us * dtv;
+            m.us = m.us + m.asx * dtv;
+            m.vs = m.vs + m.asy * dtv;
+            const int rc = solid_fraction_step(c, &m.shape);
+            if (rc) return rc;
+        }
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, vof, phi);
         // reconstruction (full normals out) and re-distancing
         unsigned* cnt = recon(vof, phi, true);

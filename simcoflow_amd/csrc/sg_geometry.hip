@@ -551,6 +551,34 @@ __global__ void __launch_bounds__(NTF_FORCE) k_force_fin(const double* part, int
     }
 }
 
+// SolidVolumeFraction (Clsvof_Mod.f90:2553-2929) on the P cells for a shape:
+// the solid level set, normal and fraction (snapped below epsi and above
+// 1 - epsi, :2906-2907), and the liquid fraction carried onto the new fluid
+// part of the cell (:2908-2917)
+__global__ void __launch_bounds__(256) k_solid_fraction(GridSpec G, Shape S, double* __restrict__ phiS,
+                                                        double* __restrict__ nxS, double* __restrict__ nyS,
+                                                        double* __restrict__ vofS, double* __restrict__ vof) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    if (i >= G.ni || j >= G.nj) return;
+    const size_t c = (size_t)i + (size_t)j * G.ni;
+    double phi, nx, ny;
+    plane(S, G.xc[i], G.yc[j], phi, nx, ny);
+    double vs = geo_frac(nx, ny, phi, G.hx[i], G.hy[j]);
+    if (vs < EPSI) vs = 0.0;
+    if (vs >= 1.0 - EPSI) vs = 1.0;
+    const double old = vofS[c];
+    double f = vof[c];
+    if (f + old >= 1.0 - EPSI || f + vs >= 1.0 - EPSI) f = 1.0 - vs;
+    if (old < 1.0 - EPSI && f + old < 1.0 - EPSI && f + vs < 1.0 - EPSI) f = f / (1.0 - old + G_ROUNDOFF) * (1.0 - vs);
+    phiS[c] = phi;
+    nxS[c] = nx;
+    nyS[c] = ny;
+    vofS[c] = vs;
+    vof[c] = f;
+}
+
 struct NewVarIn {
     int ni, nj;
     const double *vofSUo, *vofSU, *vofSVo, *vofSV;  // cell arrays
@@ -580,6 +608,16 @@ __global__ void __launch_bounds__(256) k_inter_new_var(NewVarIn a) {
 }
 
 }  // namespace
+
+int solid_fraction_step(Ctx* c, const sg_solid_shape* shape) {
+    const GridSpec GP{c->ni, c->nj, c->cls.xc, c->cls.yc, c->dx, c->dy, c->dx, c->dy, 0, 0};
+    const Shape S{shape->kind, shape->cx, shape->cy, shape->a, shape->b};
+    auto F = [&](int f) { return (double*)c->field[f]; };
+    hipLaunchKernelGGL(k_solid_fraction, grid2d(c->ni, c->nj), block2d(), 0, c->stream, GP, S, F(SG_F_PHIS),
+                       F(SG_F_NXS), F(SG_F_NYS), F(SG_F_VOFS), F(SG_F_VOF));
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
 
 int geometry_snapshot(Ctx* c) {
     GeoScratch& W = c->geo;

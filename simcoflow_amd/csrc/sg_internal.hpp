@@ -269,6 +269,11 @@ struct Ctx {
     double* step_dev = nullptr;   // device scalars of the time-step control (dt)
     DiagState diag;
     GeoScratch geo;
+    // a solid moved inside the transport's sub-steps (kind NONE: static)
+    struct {
+        sg_solid_shape shape{};
+        double us = 0.0, vs = 0.0, asx = 0.0, asy = 0.0;
+    } moving;
     hipEvent_t ev_nn0 = nullptr, ev_nn1 = nullptr;
     int use_graph = 1;
     ClsBuffers cls;
@@ -339,6 +344,7 @@ int closed_box_bc(Ctx* c);
 int geometry_rebuild(Ctx* c, const sg_solid_shape* shape);
 int object_force(Ctx* c, double vs, double* force, double* clp, double* clf);
 int geometry_snapshot(Ctx* c);
+int solid_fraction_step(Ctx* c, const sg_solid_shape* shape);
 int inter_new_var(Ctx* c, double vb);
 int diagnostics_record(Ctx* c);
 int diagnostics_read(Ctx* c, int max_rows, double* rows, int* n_rows, long long* n_lost);

@@ -279,6 +279,19 @@ class Projection:
         check(self.lib.sg_object_force(self.ctx, float(vs), ctypes.byref(f), ctypes.byref(a), ctypes.byref(b)))
         return f.value, a.value, b.value
 
+    def set_moving_solid(self, kind, cx=0.0, cy=0.0, a=0.0, b=0.0, us=0.0, vs=0.0, asx=0.0, asy=0.0):
+        """Arms the solid motion inside the transport's sub-steps ("none"
+        disarms)."""
+        s = _lib.SolidShape(_lib.SOLID_KINDS[kind], float(cx), float(cy), float(a), float(b))
+        check(self.lib.sg_set_moving_solid(self.ctx, ctypes.byref(s), float(us), float(vs), float(asx), float(asy)))
+
+    def get_moving_solid(self):
+        """(kind, cx, cy, a, b), us, vs of the moved solid."""
+        s = _lib.SolidShape()
+        us, vs = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.sg_get_moving_solid(self.ctx, ctypes.byref(s), ctypes.byref(us), ctypes.byref(vs)))
+        return (s.kind, s.cx, s.cy, s.a, s.b), us.value, vs.value
+
     def geometry_snapshot(self):
         """Keeps the geometry of the step's start for inter_new_var."""
         check(self.lib.sg_geometry_snapshot(self.ctx))

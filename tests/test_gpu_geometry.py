@@ -184,3 +184,60 @@ def test_inter_new_var_matches_oracle(oracle_lib):
     ou, ov = oracle_lib.inter_new_var(g, geo0, geo1, u, v, -0.3)
     assert np.array_equal(gu, ou) and np.array_equal(gv, ov)
     assert (ou != u).sum() > 0 and (ov != v).sum() > 0
+
+
+def test_solid_moving_inside_the_transport():
+    """A submerged cylinder moving up at 0.3 m/s (accelerating at -2 m/s^2)
+    through one transport call of 6 sub-steps: the moved centre and velocity
+    follow ObjectMovement's kinematics bit for bit, the P cells' solid level
+    set, normal and fraction equal the host's for the final centre (fraction
+    snapped below epsi and above 1 - epsi), and the liquid fraction stays in
+    [0, 1 - vofS] with full solid cells empty."""
+    from simcoflow_amd.projection import Projection, params_for
+    n = 96
+    g = C.uniform_grid(n, n, 1.0, 1.0)
+    c0 = (0.5 + 0.3 / n, 0.4)
+    r = 0.15
+    geo = C.build_geometry(g, _shape("circle", *c0, r, 0.0))
+
+    def ls(X, Y):
+        return Y - 0.8, np.zeros(X.shape), np.ones(X.shape)
+    itf = C.interface_from_levelset(g, geo, ls)
+    dt = 0.25 / n
+    vs0, asy = 0.3, -2.0
+    pj = Projection(g, params_for(C.Fluids()))
+    try:
+        pj.set_geometry(geo)
+        pj.upload("VOF", itf.vof)
+        pj.upload("PHI", itf.phi)
+        pj.upload("NX", itf.nx)
+        pj.upload("NY", itf.ny)
+        z = np.zeros((n + 2, n + 2))
+        pj.upload("U", z)
+        pj.upload("V", z)
+        pj.set_moving_solid("circle", *c0, r, 0.0, us=0.0, vs=vs0, asx=0.0, asy=asy)
+        print("[geometry] moving cylinder inside one transport call", flush=True)
+        pj.clsvof_advect_resident(dt)
+        shape, us, vs = pj.get_moving_solid()
+        got = {k: pj.download(k) for k in ("VOF", "VOFS", "PHIS", "NXS", "NYS")}
+        pj.set_moving_solid("none")
+    finally:
+        pj.close()
+    cx, cy, v = c0[0], c0[1], vs0
+    dtv = dt / 6.0
+    for _ in range(6):
+        cy = cy + 0.5 * asy * (dtv * dtv) + v * dtv
+        cx = cx + 0.5 * 0.0 * (dtv * dtv) + 0.0 * dtv
+        v = v + asy * dtv
+    assert shape[1:3] == (cx, cy) and vs == v and us == 0.0
+    Xp, Yp = np.meshgrid(g.xc, g.yc)
+    phi, nx, ny = C.sdf_circle(Xp, Yp, cx, cy, r)
+    vofS = C.frac(nx, ny, phi, np.broadcast_to(g.dx[None, :], (n, n)), np.broadcast_to(g.dy[:, None], (n, n)))
+    vofS = np.where(vofS < 1e-3, 0.0, vofS)
+    vofS = np.where(vofS >= 1.0 - 1e-3, 1.0, vofS)
+    assert np.array_equal(got["PHIS"], phi) and np.array_equal(got["NXS"], nx) and np.array_equal(got["NYS"], ny)
+    assert np.array_equal(got["VOFS"], vofS)
+    f = got["VOF"]
+    assert np.all(f >= 0.0) and np.all(f <= 1.0 - vofS + 1e-12)
+    assert np.all(f[vofS == 1.0] == 0.0)
+    assert cy > c0[1] and not np.array_equal(vofS, geo.vofS)
