@@ -38,7 +38,7 @@ def physics(fluids):
 class TimeStepper:
     def __init__(self, case, precision=_lib.SG_PREC_FP64, tol=None, cfl=0.25, vint=0.0, uref=1.0, max_iter=500,
                  learned=None, stags=None, learned_width=256, previous_pressure=False, diagnostics=False,
-                 device_geometry=False):
+                 device_geometry=False, moving=None):
         """learned: (weights, level, out_scale, deflate) of a trained warm
         start, level None for the first multigrid level no wider than
         learned_width; stags: the (U, V) cases.StagGeometry if already
@@ -50,7 +50,17 @@ class TimeStepper:
         row of device diagnostics after every step (read with history());
         device_geometry: build the cut-cell geometry of case.shape on the
         device (sg_geometry_rebuild, equal to the host builder bit for bit)
-        instead of uploading case.geo and the host's staggered geometry."""
+        instead of uploading case.geo and the host's staggered geometry;
+        moving: dict(mass=M, g=gravity, start=itt) to let case.shape move
+        as a rigid body under gravity and the fluid force (ComputeForceObject
+        -> asy = (F - M g)/M, held at rest before step `start`,
+        Solver_Mod.f90:277-290), moved inside the transport's sub-steps
+        (ObjectMovement + SolidVolumeFraction), its cut cells rebuilt after
+        the transport (Grid_Preprocess) and the velocities of uncovered cells
+        set by InterNewVar (Solver_Mod.f90:296-307); needs device_geometry."""
+        if moving and (not device_geometry or case.shape[0] == "none"):
+            raise ValueError("a moving solid needs device_geometry and a case.shape")
+        self.moving = dict(moving, shape=tuple(case.shape), us=0.0, vs=0.0, asy=0.0) if moving else None
         self.case = case
         g = case.grid
         self.cfl, self.vint, self.uref = cfl, vint, uref
@@ -95,17 +105,37 @@ class TimeStepper:
         for dst, src in (("VOFO", "VOF"), ("PHIO", "PHI"), ("NXO", "NX"), ("NYO", "NY"), ("UVOFO", "UVOF"),
                          ("VVOFO", "VVOF")):
             check(lib.sg_field_copy(ctx, F[dst], F[src]))
+        mv = self.moving
+        if mv:
+            pj.geometry_snapshot()
+            force = pj.object_force(mv["vs"])[0]
+            mv["asy"] = (force - mv["mass"] * mv["g"]) / mv["mass"]
+            if self.itt < mv.get("start", 10):
+                mv["us"] = mv["vs"] = mv["asy"] = 0.0
+            mv["force"] = force
+        vs, asy = (mv["vs"], mv["asy"]) if mv else (0.0, 0.0)
         dt = _lib.ctypes.c_double(0.0)
-        check(lib.sg_time_step_size(ctx, self.cfl, self.vint, self.uref, 0.0, 0.0, _lib.ctypes.byref(dt)))
+        check(lib.sg_time_step_size(ctx, self.cfl, self.vint, self.uref, vs, asy, _lib.ctypes.byref(dt)))
         dt = dt.value
         if self.itt > 1:
+            if mv:
+                kind, cx, cy, a, b = mv["shape"]
+                pj.set_moving_solid(kind, cx, cy, a, b, us=mv["us"], vs=mv["vs"], asx=0.0, asy=mv["asy"])
             pj.clsvof_advect_resident(dt, self.time)
             check(lib.sg_interface_to_uv_resident(ctx))
+            if mv:
+                (_, cx, cy, a, b), mv["us"], mv["vs"] = pj.get_moving_solid()
+                pj.set_moving_solid("none")
+                mv["shape"] = (kind, cx, cy, a, b)
+                pj.geometry_rebuild(kind, cx, cy, a, b)
         check(lib.sg_closed_box_bc(ctx))
-        pinfo = pj.predictor_resident(dt, vb=0.0, first_step=(self.itt == 1))
-        pj.build(0.0)
+        vb = mv["vs"] if mv else 0.0
+        if mv and self.itt > 1:
+            pj.inter_new_var(vb)
+        pinfo = pj.predictor_resident(dt, vb=vb, first_step=(self.itt == 1))
+        pj.build(vb)
         sinfo = pj.solve()
-        pj.velocity_correct(0.0)
+        pj.velocity_correct(vb)
         if self.diagnostics:
             pj.diagnostics_record()
         self.time += dt
