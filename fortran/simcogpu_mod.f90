@@ -67,10 +67,10 @@ module simcogpu
     real(c_double) :: depth, amp, kw, omega
   end type sg_clsvof_params
 
-  ! a solid shape for sg_geometry_rebuild (kind: 0 none, 1 box, 2 circle, 3 ellipse)
+  ! a solid shape for sg_geometry_rebuild (kind: 0 none, 1 box, 2 circle, 3 ellipse, 4 boom)
   type, bind(C), public :: sg_solid_shape
     integer(c_int) :: kind
-    real(c_double) :: cx, cy, a, b
+    real(c_double) :: cx, cy, a, b, c
   end type sg_solid_shape
 
   ! TrackingParticles (Particles.f90:71-205) constants; see sg_particle_params in simcogpu.h

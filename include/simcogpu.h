@@ -314,18 +314,22 @@ int sg_particles_track(sg_ctx* ctx, double dt, const sg_particle_params* p, int*
  * fractions and flags, the momentum-exchange cells and every staggered-cell
  * array of sg_set_staggered_geometry: the device state sg_set_geometry +
  * sg_set_staggered_geometry would set, computed on the device.  box: half
- * width a, half height b; circle: radius a; ellipse: semi-axes a, b; none:
- * no solid.  Needs ni, nj >= 4.  The first call allocates the rebuild's work
+ * width a, half height b; circle: radius a; ellipse: semi-axes a, b; boom
+ * (the oil boom BoomCase of Main.f90:173-182, moved by ObjectMovement,
+ * Clsvof_Mod.f90:2388-2398): a cylinder of radius a (Dobj/2) with a bar of
+ * half width b (Wobj/2 < a) reaching c (LBar) below where it leaves the
+ * cylinder, as the distance to the union of the two; none: no solid.  Needs ni, nj >= 4.  The first call allocates the rebuild's work
  * arrays (4 ghosted + 2 cell planes); later calls allocate nothing and wait
  * on the host once per pass of the closing rule (1-3 passes). */
 #define SG_SOLID_NONE 0
 #define SG_SOLID_BOX 1
 #define SG_SOLID_CIRCLE 2
 #define SG_SOLID_ELLIPSE 3
+#define SG_SOLID_BOOM 4
 typedef struct sg_solid_shape {
     int kind;
     double cx, cy; /* centre */
-    double a, b;
+    double a, b, c;
 } sg_solid_shape;
 int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape);
 /* force on the solid (replaces ComputeForceObject, Clsvof_Mod.f90:2400-2549,

@@ -84,12 +84,12 @@ class ClsvofParams(ctypes.Structure):
 
 # exported symbol -> (argtypes, restype); kept in step with include/simcogpu.h
 # solid shapes of sg_geometry_rebuild
-SOLID_KINDS = {"none": 0, "box": 1, "circle": 2, "ellipse": 3}
+SOLID_KINDS = {"none": 0, "box": 1, "circle": 2, "ellipse": 3, "boom": 4}
 
 
 class SolidShape(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("cx", ctypes.c_double), ("cy", ctypes.c_double), ("a", ctypes.c_double),
-                ("b", ctypes.c_double)]
+                ("b", ctypes.c_double), ("c", ctypes.c_double)]
 
 
 # per-step diagnostics ring (SG_DIAG_COLS, SG_DIAG_RING); the columns

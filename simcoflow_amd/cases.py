@@ -145,6 +145,18 @@ def sdf_circle(X, Y, cx, cy, r):
     return d - r, np.where(d > 0, rx / safe, 1.0), np.where(d > 0, ry / safe, 0.0)
 
 
+def sdf_boom(X, Y, cx, cy, r, hw, length):
+    """The oil boom (BoomCase, Main.f90:173-182: r = Dobj/2, hw = Wobj/2,
+    length = LBar): a cylinder of radius r with a bar of half width hw
+    reaching `length` below where it leaves the cylinder, as the distance to
+    their union (the smaller distance, the cylinder's on ties)."""
+    p1, x1, y1 = sdf_circle(X, Y, cx, cy, r)
+    bottom = cy - np.sqrt(r * r - hw * hw) - length
+    p2, x2, y2 = sdf_box(X, Y, cx, 0.5 * (cy + bottom), hw, 0.5 * (cy - bottom))
+    bar = p2 < p1
+    return np.where(bar, p2, p1), np.where(bar, x2, x1), np.where(bar, y2, y1)
+
+
 def sdf_ellipse(X, Y, cx, cy, a, b):
     """First-order distance to an ellipse with semi-axes a, b."""
     rx = (X - cx) / a
@@ -417,7 +429,7 @@ class Case:
     bc: tuple = (1, 1, 1, 0)   # W,E,S,N: Neumann W/E/S, Dirichlet N (ProjectionP_Mod.f90:55)
     solid: object = None       # solid(X, Y) -> (phiS, nxS, nyS), or None
     levelset: object = None    # levelset(X, Y) -> (phi, nx, ny), liquid where phi < 0
-    shape: tuple = ("none",)   # the solid as (kind, cx, cy, a, b) for the device geometry rebuild
+    shape: tuple = ("none",)   # the solid as (kind, cx, cy, a, b[, c]) for the device geometry rebuild
 
 
 def _gas_bubble(cx, cy, r):

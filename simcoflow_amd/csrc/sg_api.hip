@@ -577,7 +577,7 @@ int sg_particles_get(sg_ctx* ctx, int n, double* x, double* y, double* u, double
 
 int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
-    if (!c || !shape || shape->kind < SG_SOLID_NONE || shape->kind > SG_SOLID_ELLIPSE) {
+    if (!c || !shape || shape->kind < SG_SOLID_NONE || shape->kind > SG_SOLID_BOOM) {
         set_error("sg_geometry_rebuild: need a shape of a known kind");
         return SG_ERR_ARG;
     }
@@ -585,6 +585,10 @@ int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape) {
         (!(shape->a > 0.0) || (shape->kind != SG_SOLID_CIRCLE && !(shape->b > 0.0)) ||
          !std::isfinite(shape->cx) || !std::isfinite(shape->cy))) {
         set_error("sg_geometry_rebuild: the shape needs a finite centre and positive sizes");
+        return SG_ERR_ARG;
+    }
+    if (shape->kind == SG_SOLID_BOOM && (!(shape->b < shape->a) || !(shape->c >= 0.0))) {
+        set_error("sg_geometry_rebuild: a boom needs bar half width b < radius a and bar length c >= 0");
         return SG_ERR_ARG;
     }
     if (c->ni < 4 || c->nj < 4) {
@@ -605,7 +609,7 @@ int sg_object_force(sg_ctx* ctx, double vs, double* force, double* clp, double* 
 
 int sg_set_moving_solid(sg_ctx* ctx, const sg_solid_shape* shape, double us, double vs, double asx, double asy) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
-    if (!c || !shape || shape->kind < SG_SOLID_NONE || shape->kind > SG_SOLID_ELLIPSE) {
+    if (!c || !shape || shape->kind < SG_SOLID_NONE || shape->kind > SG_SOLID_BOOM) {
         set_error("sg_set_moving_solid: need a shape of a known kind");
         return SG_ERR_ARG;
     }

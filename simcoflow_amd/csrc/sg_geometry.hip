@@ -1,7 +1,8 @@
 // Cut-cell geometry rebuilt on the device for a solid given as a shape: the
 // solid level set and normal on the P, U and V cells, the solid fractions
-// (SolidVolumeFraction's role, Clsvof_Mod.f90:2553-2929, for a box, circle
-// or ellipse), the cell geometry of Cell_Geo_Cal / Edge_Geo_Cal
+// (SolidVolumeFraction's role, Clsvof_Mod.f90:2553-2929, for a box,
+// circle, ellipse or the oil boom's cylinder with its bar, Main.f90:173-182),
+// the cell geometry of Cell_Geo_Cal / Edge_Geo_Cal
 // (Cutcell_Mod.f90:34-337: edge solid fractions, wall length, apertures with
 // the harmonic face averaging, solid cells closing their faces, cells closed
 // on both sides of an axis becoming solid), DefineMomentumExchangeCell
@@ -41,37 +42,58 @@ __device__ inline double np_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -
 
 struct Shape {
     int kind;
-    double cx, cy, a, b;
+    double cx, cy, a, b, c;
 };
+
+__device__ void sdf_box(double X, double Y, double cx, double cy, double hw, double hh, double& phi, double& gx,
+                        double& gy) {
+    const double qx = fabs(X - cx) - hw;
+    const double qy = fabs(Y - cy) - hh;
+    const double ox = np_max(qx, 0.0);
+    const double oy = np_max(qy, 0.0);
+    const double outside = sqrt(ox * ox + oy * oy);
+    const double inside = np_min(np_max(qx, qy), 0.0);
+    phi = outside + inside;
+    double sx = np_sign(X - cx);
+    double sy = np_sign(Y - cy);
+    if (sx == 0.0) sx = 1.0;
+    if (sy == 0.0) sy = 1.0;
+    if (outside > 0.0) {
+        gx = sx * ox / outside;
+        gy = sy * oy / outside;
+    } else {
+        gx = qx >= qy ? sx : 0.0;
+        gy = qx >= qy ? 0.0 : sy;
+    }
+}
+
+__device__ void sdf_circle(double X, double Y, double cx, double cy, double r, double& phi, double& gx,
+                           double& gy) {
+    const double rx = X - cx;
+    const double ry = Y - cy;
+    const double d = sqrt(rx * rx + ry * ry);
+    phi = d - r;
+    gx = d > 0.0 ? rx / d : 1.0;
+    gy = d > 0.0 ? ry / d : 0.0;
+}
 
 // signed distance (> 0 in the fluid) and its unit gradient
 __device__ void plane(const Shape& s, double X, double Y, double& phi, double& gx, double& gy) {
-    if (s.kind == SG_SOLID_BOX) {
-        const double qx = fabs(X - s.cx) - s.a;
-        const double qy = fabs(Y - s.cy) - s.b;
-        const double ox = np_max(qx, 0.0);
-        const double oy = np_max(qy, 0.0);
-        const double outside = sqrt(ox * ox + oy * oy);
-        const double inside = np_min(np_max(qx, qy), 0.0);
-        phi = outside + inside;
-        double sx = np_sign(X - s.cx);
-        double sy = np_sign(Y - s.cy);
-        if (sx == 0.0) sx = 1.0;
-        if (sy == 0.0) sy = 1.0;
-        if (outside > 0.0) {
-            gx = sx * ox / outside;
-            gy = sy * oy / outside;
-        } else {
-            gx = qx >= qy ? sx : 0.0;
-            gy = qx >= qy ? 0.0 : sy;
-        }
+    if (s.kind == SG_SOLID_BOOM) {
+        // union of the cylinder and the bar: the smaller distance (the
+        // cylinder's on ties)
+        double p1, x1, y1, p2, x2, y2;
+        sdf_circle(X, Y, s.cx, s.cy, s.a, p1, x1, y1);
+        const double bottom = s.cy - sqrt(s.a * s.a - s.b * s.b) - s.c;
+        sdf_box(X, Y, s.cx, 0.5 * (s.cy + bottom), s.b, 0.5 * (s.cy - bottom), p2, x2, y2);
+        const bool bar = p2 < p1;
+        phi = bar ? p2 : p1;
+        gx = bar ? x2 : x1;
+        gy = bar ? y2 : y1;
+    } else if (s.kind == SG_SOLID_BOX) {
+        sdf_box(X, Y, s.cx, s.cy, s.a, s.b, phi, gx, gy);
     } else if (s.kind == SG_SOLID_CIRCLE) {
-        const double rx = X - s.cx;
-        const double ry = Y - s.cy;
-        const double d = sqrt(rx * rx + ry * ry);
-        phi = d - s.a;
-        gx = d > 0.0 ? rx / d : 1.0;
-        gy = d > 0.0 ? ry / d : 0.0;
+        sdf_circle(X, Y, s.cx, s.cy, s.a, phi, gx, gy);
     } else if (s.kind == SG_SOLID_ELLIPSE) {
         const double rx = (X - s.cx) / s.a;
         const double ry = (Y - s.cy) / s.b;
@@ -611,7 +633,7 @@ __global__ void __launch_bounds__(256) k_inter_new_var(NewVarIn a) {
 
 int solid_fraction_step(Ctx* c, const sg_solid_shape* shape) {
     const GridSpec GP{c->ni, c->nj, c->cls.xc, c->cls.yc, c->dx, c->dy, c->dx, c->dy, 0, 0};
-    const Shape S{shape->kind, shape->cx, shape->cy, shape->a, shape->b};
+    const Shape S{shape->kind, shape->cx, shape->cy, shape->a, shape->b, shape->c};
     auto F = [&](int f) { return (double*)c->field[f]; };
     hipLaunchKernelGGL(k_solid_fraction, grid2d(c->ni, c->nj), block2d(), 0, c->stream, GP, S, F(SG_F_PHIS),
                        F(SG_F_NXS), F(SG_F_NYS), F(SG_F_VOFS), F(SG_F_VOF));
@@ -695,7 +717,7 @@ int geometry_rebuild(Ctx* c, const sg_solid_shape* shape) {
         }
     }
     hipStream_t st = c->stream;
-    const Shape S{shape->kind, shape->cx, shape->cy, shape->a, shape->b};
+    const Shape S{shape->kind, shape->cx, shape->cy, shape->a, shape->b, shape->c};
     const RawAp R{W.raw[0], W.raw[1], W.raw[2], W.raw[3]};
     const dim3 gc = grid2d(ni, nj), gg = grid2d(ni + 2, nj + 2), b = block2d();
     auto F = [&](int f) { return (double*)c->field[f]; };

@@ -266,11 +266,12 @@ class Projection:
         check(self.lib.sg_particles_track(self.ctx, float(dt), ctypes.byref(p), ctypes.byref(flagged)))
         return flagged.value
 
-    def geometry_rebuild(self, kind, cx=0.0, cy=0.0, a=0.0, b=0.0):
-        """Cut-cell geometry of a box (half sizes a, b), circle (radius a) or
-        ellipse (semi-axes a, b) centred at (cx, cy), or of no solid, rebuilt
-        on the device (P, U and V cells)."""
-        s = _lib.SolidShape(_lib.SOLID_KINDS[kind], float(cx), float(cy), float(a), float(b))
+    def geometry_rebuild(self, kind, cx=0.0, cy=0.0, a=0.0, b=0.0, c=0.0):
+        """Cut-cell geometry of a box (half sizes a, b), circle (radius a),
+        ellipse (semi-axes a, b) or boom (cylinder radius a, bar half width b,
+        bar length c) centred at (cx, cy), or of no solid, rebuilt on the
+        device (P, U and V cells)."""
+        s = _lib.SolidShape(_lib.SOLID_KINDS[kind], float(cx), float(cy), float(a), float(b), float(c))
         check(self.lib.sg_geometry_rebuild(self.ctx, ctypes.byref(s)))
 
     def object_force(self, vs=0.0):
@@ -279,18 +280,18 @@ class Projection:
         check(self.lib.sg_object_force(self.ctx, float(vs), ctypes.byref(f), ctypes.byref(a), ctypes.byref(b)))
         return f.value, a.value, b.value
 
-    def set_moving_solid(self, kind, cx=0.0, cy=0.0, a=0.0, b=0.0, us=0.0, vs=0.0, asx=0.0, asy=0.0):
+    def set_moving_solid(self, kind, cx=0.0, cy=0.0, a=0.0, b=0.0, c=0.0, us=0.0, vs=0.0, asx=0.0, asy=0.0):
         """Arms the solid motion inside the transport's sub-steps ("none"
         disarms)."""
-        s = _lib.SolidShape(_lib.SOLID_KINDS[kind], float(cx), float(cy), float(a), float(b))
+        s = _lib.SolidShape(_lib.SOLID_KINDS[kind], float(cx), float(cy), float(a), float(b), float(c))
         check(self.lib.sg_set_moving_solid(self.ctx, ctypes.byref(s), float(us), float(vs), float(asx), float(asy)))
 
     def get_moving_solid(self):
-        """(kind, cx, cy, a, b), us, vs of the moved solid."""
+        """(kind, cx, cy, a, b, c), us, vs of the moved solid."""
         s = _lib.SolidShape()
         us, vs = ctypes.c_double(), ctypes.c_double()
         check(self.lib.sg_get_moving_solid(self.ctx, ctypes.byref(s), ctypes.byref(us), ctypes.byref(vs)))
-        return (s.kind, s.cx, s.cy, s.a, s.b), us.value, vs.value
+        return (s.kind, s.cx, s.cy, s.a, s.b, s.c), us.value, vs.value
 
     def geometry_snapshot(self):
         """Keeps the geometry of the step's start for inter_new_var."""

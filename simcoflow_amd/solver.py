@@ -60,7 +60,8 @@ class TimeStepper:
         set by InterNewVar (Solver_Mod.f90:296-307); needs device_geometry."""
         if moving and (not device_geometry or case.shape[0] == "none"):
             raise ValueError("a moving solid needs device_geometry and a case.shape")
-        self.moving = dict(moving, shape=tuple(case.shape), us=0.0, vs=0.0, asy=0.0) if moving else None
+        self.moving = dict(moving, shape=(tuple(case.shape) + (0.0,) * 6)[:6], us=0.0, vs=0.0,
+                           asy=0.0) if moving else None
         self.case = case
         g = case.grid
         self.cfl, self.vint, self.uref = cfl, vint, uref
@@ -119,15 +120,15 @@ class TimeStepper:
         dt = dt.value
         if self.itt > 1:
             if mv:
-                kind, cx, cy, a, b = mv["shape"]
-                pj.set_moving_solid(kind, cx, cy, a, b, us=mv["us"], vs=mv["vs"], asx=0.0, asy=mv["asy"])
+                kind, cx, cy, a, b, cc = mv["shape"]
+                pj.set_moving_solid(kind, cx, cy, a, b, cc, us=mv["us"], vs=mv["vs"], asx=0.0, asy=mv["asy"])
             pj.clsvof_advect_resident(dt, self.time)
             check(lib.sg_interface_to_uv_resident(ctx))
             if mv:
-                (_, cx, cy, a, b), mv["us"], mv["vs"] = pj.get_moving_solid()
+                (_, cx, cy, a, b, cc), mv["us"], mv["vs"] = pj.get_moving_solid()
                 pj.set_moving_solid("none")
-                mv["shape"] = (kind, cx, cy, a, b)
-                pj.geometry_rebuild(kind, cx, cy, a, b)
+                mv["shape"] = (kind, cx, cy, a, b, cc)
+                pj.geometry_rebuild(kind, cx, cy, a, b, cc)
         check(lib.sg_closed_box_bc(ctx))
         vb = mv["vs"] if mv else 0.0
         if mv and self.itt > 1:

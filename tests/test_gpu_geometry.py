@@ -30,7 +30,9 @@ def _stretched(ni, nj, seed):
     return C.Grid(ni, nj, 0.0, 0.0, dx, dy, udx, vdy, xc, yc)
 
 
-def _shape(kind, cx, cy, a, b):
+def _shape(kind, cx, cy, a, b, c=0.0):
+    if kind == "boom":
+        return lambda X, Y: C.sdf_boom(X, Y, cx, cy, a, b, c)
     if kind == "box":
         return lambda X, Y: C.sdf_box(X, Y, cx, cy, a, b)
     if kind == "circle":
@@ -57,6 +59,51 @@ def _compare(pj, g, solid):
         for name, arr in got.items():
             errs.append(_diff(f"{'UV'[which]}.{name}", arr, getattr(st, name)))
     return [e for e in errs if e], geo
+
+
+def _boom_area(a, b, c):
+    """Area of the union of a disc of radius a and the bar |x| <= b,
+    -sqrt(a^2 - b^2) - c <= y <= 0: the disc plus the bar less the part of
+    the bar's lower end inside the disc's cap."""
+    x = np.linspace(-b, b, 200001)
+    top = -np.sqrt(a * a - b * b)
+    low = np.maximum(top - c, -np.sqrt(a * a - x * x))
+    overlap = np.maximum(top - low, 0.0)
+    return np.pi * a * a + 2.0 * b * c - float(np.sum(0.5 * (overlap[1:] + overlap[:-1]) * np.diff(x)))
+
+
+@pytest.mark.parametrize("grid", ["uniform64", "stretched40x56"])
+def test_boom_rebuild_matches_host_builder(grid):
+    """The oil boom (a cylinder with a bar below it) rebuilt on the device
+    equals the host builder bit for bit, for the reference's proportions
+    (Dobj 0.8, Wobj 0.16, LBar 1.5, scaled into the box) and ten random
+    ones; the solid area is the union's to within the cut-cell error."""
+    from simcoflow_amd.projection import Projection, params_for
+    g = {"uniform64": lambda: C.uniform_grid(64, 64, 1.0, 1.0),
+         "stretched40x56": lambda: _stretched(40, 56, 3)}[grid]()
+    lx, ly = float(g.dx.sum()), float(g.dy.sum())
+    rng = np.random.default_rng({"uniform64": 21, "stretched40x56": 22}[grid])
+    s = 0.25 * ly / 1.9
+    booms = [(0.5 * lx, 0.8 * ly, 0.4 * s, 0.08 * s, 1.5 * s)]
+    for _ in range(10):
+        a = rng.uniform(0.05, 0.15) * lx
+        booms.append((rng.uniform(0.3, 0.7) * lx, rng.uniform(0.55, 0.75) * ly, a, rng.uniform(0.1, 0.6) * a,
+                      rng.uniform(0.0, 0.3) * ly))
+    pj = Projection(g, params_for(C.Fluids()))
+    DX = np.broadcast_to(g.dx[None, :], (g.nj, g.ni))
+    DY = np.broadcast_to(g.dy[:, None], (g.nj, g.ni))
+    try:
+        for t, (cx, cy, a, b, c) in enumerate(booms):
+            print(f"[geometry] {grid} boom #{t}: at ({cx:.4f}, {cy:.4f}) r {a:.4f} half width {b:.4f} "
+                  f"bar {c:.4f}", flush=True)
+            pj.geometry_rebuild("boom", cx, cy, a, b, c)
+            errs, geo = _compare(pj, g, _shape("boom", cx, cy, a, b, c))
+            assert not errs, "\n".join(errs)
+            area = _boom_area(a, b, c)
+            got = float((geo.vofS * DX * DY).sum())
+            assert abs(got - area) < 4.0 * float(g.dx.max() * g.dy.max()) + 0.02 * area, (got, area)
+    finally:
+        pj.close()
 
 
 @pytest.mark.parametrize("grid", ["uniform48", "stretched40x56", "uniform96x64"])
