@@ -228,6 +228,19 @@ int sg_clsvof_advect(sg_ctx* ctx, double* vof, double* phi, double* nx, double* 
  * the solid's velocity and acceleration, 0 for solids at rest), a device
  * min-reduction; dt also stays on the device. */
 int sg_time_step_size(sg_ctx* ctx, double cfl, double vint, double uref, double vs, double asy, double* dt);
+/* The same CFL step without a context, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  No counterpart in the reference.  Every pointer
+ * is DEVICE memory the caller owns: udx (ni), vdy (nj) = UGrid%dx(:,1),
+ * VGrid%dy(1,:); u, v ghosted (ni+2)*(nj+2) as TVar%u, TVar%v; dt (one
+ * double) receives the step.  Launches on `stream` (a hipStream_t, NULL =
+ * the null stream) and returns without waiting.  Allocates nothing: the
+ * per-block partial minima go to a 16 KiB array of the library in device
+ * memory (one per GPU, held until the process ends), so calls that overlap
+ * on one GPU race.  Rejects ni*nj outside [1, 2^31 - 1 - 256*2048]. */
+int sg_time_step_size_noalloc(int ni, int nj, const double* udx, const double* vdy, const double* u,
+                              const double* v, double cfl, double Fr, double vint, double uref, double vs,
+                              double asy, double* dt, void* stream);
 /* ghost layer of a closed box on U, V, P: the box form of
  * Boundary_Condition_Var (StateVariables.f90:118-211) -- pressure and
  * tangential velocity copied into the ghosts, P = 0 above the open top,

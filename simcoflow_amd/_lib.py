@@ -13,6 +13,10 @@ SG_OK = 0
 SG_NOT_CONVERGED = 1
 SG_NAN = 2
 SG_GUARD = 3
+SG_ERR_ARG = -1
+SG_ERR_HIP = -2
+SG_ERR_ALLOC = -3
+SG_ERR_STATE = -4
 SG_PREC_FP64 = 0
 SG_PREC_MIXED = 1
 
@@ -154,6 +158,8 @@ PROTOTYPES = {
     "sg_upload_device": ([_CTX, ctypes.c_int, _V], ctypes.c_int),
     "sg_time_step_size": ([_CTX, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                            ctypes.c_double, _D], ctypes.c_int),
+    "sg_time_step_size_noalloc": ([ctypes.c_int, ctypes.c_int, _V, _V, _V, _V] + [ctypes.c_double] * 6 + [_V, _V],
+                                  ctypes.c_int),
     "sg_closed_box_bc": ([_CTX], ctypes.c_int),
     "sg_field_copy": ([_CTX, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "sg_download_device": ([_CTX, ctypes.c_int, _V], ctypes.c_int),

@@ -983,6 +983,17 @@ int sg_time_step_size(sg_ctx* ctx, double cfl, double vint, double uref, double 
     return sg::time_step_size(c, cfl, vint, uref, vs, asy, dt);
 }
 
+int sg_time_step_size_noalloc(int ni, int nj, const double* udx, const double* vdy, const double* u, const double* v,
+                              double cfl, double Fr, double vint, double uref, double vs, double asy, double* dt,
+                              void* stream) {
+    if (!udx || !vdy || !u || !v || !dt || !(cfl > 0.0) || uref == 0.0) {
+        sg::set_error("sg_time_step_size_noalloc: bad argument");
+        return SG_ERR_ARG;
+    }
+    return sg::time_step_size_noalloc(ni, nj, udx, vdy, u, v, cfl, Fr, vint, uref, vs, asy, dt,
+                                      reinterpret_cast<hipStream_t>(stream));
+}
+
 int sg_closed_box_bc(sg_ctx* ctx) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     if (!c) return SG_ERR_ARG;

@@ -341,6 +341,9 @@ void learned_release(Ctx* c);
 // time-step control and boundary layer (sg_step.hip)
 int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, double asy, double* dt);
 int closed_box_bc(Ctx* c);
+int time_step_size_noalloc(int ni, int nj, const double* udx, const double* vdy, const double* u, const double* v,
+                           double cfl, double Fr, double vint, double uref, double vs, double asy, double* dt_dev,
+                           hipStream_t stream);
 int geometry_rebuild(Ctx* c, const sg_solid_shape* shape);
 int object_force(Ctx* c, double vs, double* force, double* clp, double* clf);
 int geometry_snapshot(Ctx* c);

@@ -6,7 +6,10 @@
 // Compiled without FP contraction: the bounds round exactly as the
 // reference's expressions, so the step equals the oracle's bit for bit.
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdint>
+#include <mutex>
 
 #include "sg_internal.hpp"
 
@@ -237,6 +240,63 @@ int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, doub
     SG_HIP(hipGetLastError());
     SG_HIP(hipMemcpyAsync(dt, c->step_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+// ---- the CFL step without a context and without allocation ---------------
+// The per-block partial minima live in a file-scope device array instead of
+// the context's reduction buffer.  Host code never passes the array's name
+// to a kernel (in host code it names a host-side stand-in): the device
+// address is looked up once per device with hipGetSymbolAddress and kept.
+constexpr int NOALLOC_PARTIALS = 2048;  // blocks of k_dt, 16 KiB
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {nullptr};
+
+int scratch_for_current_device(double** out) {
+    int dev = -1;
+    SG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        set_error("sg_time_step_size_noalloc: device index beyond the library's table of 64");
+        return SG_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("sg_time_step_size_noalloc: no device address for the scratch array");
+            return SG_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SG_OK;
+}
+}  // namespace
+
+int time_step_size_noalloc(int ni, int nj, const double* udx, const double* vdy, const double* u, const double* v,
+                           double cfl, double Fr, double vint, double uref, double vs, double asy, double* dt_dev,
+                           hipStream_t stream) {
+    // grid-stride loops cover any grid; the bound keeps e + stride in int
+    const long long n = (long long)ni * (long long)nj;
+    if (ni < 1 || nj < 1 || n > (long long)INT32_MAX - (long long)NTS * NOALLOC_PARTIALS) {
+        set_error("sg_time_step_size_noalloc: ni*nj must be in [1, 2^31 - 1 - 256*2048]");
+        return SG_ERR_ARG;
+    }
+    double* part = nullptr;
+    const int rc = scratch_for_current_device(&part);
+    if (rc) return rc;
+    const StepIn a{ni, nj, udx, vdy, u, v, cfl, Fr, vint, uref, vs, asy};
+    long long nb = (n + NTS - 1) / NTS;
+    if (nb > NOALLOC_PARTIALS) nb = NOALLOC_PARTIALS;
+    // every partial k_dt_fin reads is written by k_dt first: nothing to clear
+    hipLaunchKernelGGL(k_dt, dim3((unsigned)nb), dim3(NTS), 0, stream, a, part);
+    hipLaunchKernelGGL(k_dt_fin, dim3(1), dim3(NTS), 0, stream, (const double*)part, (int)nb, dt_dev);
+    SG_HIP(hipGetLastError());
     return SG_OK;
 }
 
