@@ -398,6 +398,54 @@ def predictor_assemble(g, Ust, Vst, uvof_old, uvof_new, vvof_old, vvof_new, pcel
     return dict(fluxn1=f1, matr_u=mu, rhs_u=ru, matr_v=mv, rhs_v=rv, DpU=DU, DpV=DV)
 
 
+def _pred_struct(g, Ust, Vst, uvof_old, uvof_new, vvof_old, vvof_new, pcell, u, v, phys, dt, vb, first_step, keep):
+    c = np.ascontiguousarray
+    ni, nj = g.ni, g.nj
+    gx = [c(a, dtype=np.float64) for a in (g.dx, g.dy, g.udx, g.vdy)]
+    keep += gx
+    Us = _stag_struct(Ust, uvof_old, uvof_new, keep)
+    Vs = _stag_struct(Vst, vvof_old, vvof_new, keep)
+    pk = [c(pcell[k], dtype=np.float64) for k in ("vof", "phi", "nx", "ny", "vofS", "nxS", "nyS", "phiS")]
+    uv = [c(u, dtype=np.float64), c(v, dtype=np.float64)]
+    keep += pk + uv
+    return _PredIn(ni, nj, *[_d(a) for a in gx], Us, Vs, *[_d(a) for a in pk], *[_d(a) for a in uv],
+                   *[float(phys[k]) for k in ("rho_l", "rho_g", "rho_ref", "nu_l", "nu_g", "nu_ref", "Re", "Fr")],
+                   float(dt), float(vb), int(bool(first_step)))
+
+
+def muscl_limiter(r, tolim):
+    L = _pred_lib()
+    L.orc_muscl_limiter.argtypes = [ctypes.c_double, ctypes.c_double]
+    L.orc_muscl_limiter.restype = ctypes.c_double
+    return L.orc_muscl_limiter(r, tolim)
+
+
+def conv_fluxes(g, Ust, Vst, uvof, vvof, pcell, u, v, phys, dt):
+    """The high-order convective face fluxes: (CFluxEW, CFluxNS), each of
+    shape (2, nj+2, ni+2) (components U, V)."""
+    keep = []
+    s = _pred_struct(g, Ust, Vst, uvof, uvof, vvof, vvof, pcell, u, v, phys, dt, 0.0, False, keep)
+    L = _pred_lib()
+    L.orc_conv_fluxes.argtypes = [ctypes.POINTER(_PredIn), _D, _D]
+    F1 = np.zeros((2, g.nj + 2, g.ni + 2))
+    F2 = np.zeros((2, g.nj + 2, g.ni + 2))
+    L.orc_conv_fluxes(ctypes.byref(s), _d(F1), _d(F2))
+    return F1, F2
+
+
+def mixing(g, Ust, Vst, comp, flux, pcell, phys):
+    """Mixing_Procedure of the U (comp 0) or V cells on the cell array flux."""
+    keep = []
+    z = np.zeros(g.shape)
+    zg = np.zeros((g.nj + 2, g.ni + 2))
+    s = _pred_struct(g, Ust, Vst, z, z, z, z, pcell, zg, zg, phys, 1.0, 0.0, False, keep)
+    L = _pred_lib()
+    L.orc_mixing.argtypes = [ctypes.POINTER(_PredIn), ctypes.c_int, _D]
+    f = np.array(flux, dtype=np.float64, order="C")
+    L.orc_mixing(ctypes.byref(s), comp, _d(f))
+    return f
+
+
 def predictor_finish(g, DpU, DpV, vpred):
     DU = np.array(DpU, dtype=np.float64, order="C")
     DV = np.array(DpV, dtype=np.float64, order="C")
@@ -550,3 +598,120 @@ def inter_new_var(g, geo_old, geo_new, u, v, vb):
     uo, vo = np.array(u, dtype=np.float64, order="C"), np.array(v, dtype=np.float64, order="C")
     L.orc_inter_new_var(g.ni, g.nj, *[_d(a) for a in arrs], float(vb), _d(uo), _d(vo))
     return uo, vo
+
+
+# ---------------------------------------------------------------------------
+# Cut-cell geometry (Cutcell_Mod.f90: Grid_Preprocess + NewCellFace)
+class _CGrid(ctypes.Structure):
+    _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("x", _D), ("y", _D), ("dx", _D), ("dy", _D)]
+
+
+_CCELL_D1 = ["vofS", "phiS", "nxS", "nyS", "WlLh", "delh", "ccx", "ccy"]
+_CCELL_I = ["moex", "posnu"]
+_CCELL_D2 = ["aE", "aW", "aN", "aS", "fcex", "fcey", "fcnx", "fcny",
+             "EtaE", "EtaN", "AlE", "AlN", "DAlE", "DAlN", "SxE", "SyN"]
+
+
+class _CCell(ctypes.Structure):
+    _fields_ = ([(k, _D) for k in _CCELL_D1] + [(k, _I) for k in _CCELL_I] + [(k, _D) for k in _CCELL_D2])
+
+
+_geo_declared = False
+
+
+def _geo_lib():
+    global _geo_declared
+    L = lib()
+    if not _geo_declared:
+        dbl = ctypes.c_double
+        G = ctypes.POINTER(_CGrid)
+        C = ctypes.POINTER(_CCell)
+        L.orc_edge_geo_cal.argtypes = [dbl] * 7 + [_D, _D, _D]
+        L.orc_solid_fraction.argtypes = [ctypes.c_int, ctypes.c_int] + [_D] * 6
+        L.orc_cell_geo_cal.argtypes = [G, C, ctypes.c_int]
+        L.orc_cut_cell_geometry.argtypes = [G, G, G, C, C, C, ctypes.c_int]
+        _geo_declared = True
+    return L
+
+
+def edge_geo_cal(p1, p2, nx, ny, d):
+    """Edge_Geo_Cal: (solid fraction, face centre x, face centre y)."""
+    a, fx, fy = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _geo_lib().orc_edge_geo_cal(p1[0], p1[1], p2[0], p2[1], nx, ny, d, ctypes.byref(a), ctypes.byref(fx),
+                                ctypes.byref(fy))
+    return a.value, fx.value, fy.value
+
+
+def grid_spacings(g):
+    """(x, y, dx, dy) per i / per j of the P, U and V grids (Mesh_Mod.f90:123-175)."""
+    P = (g.xc, g.yc, g.dx, g.dy)
+    U = (g.xc + 0.5 * g.dx, g.yc, g.udx, g.dy)
+    V = (g.xc, g.yc + 0.5 * g.dy, g.dx, g.vdy)
+    return {"P": P, "U": U, "V": V}
+
+
+def cut_cell_geometry(g, planes, prev_AlN=None, icorprob=True):
+    """Grid_Preprocess + NewCellFace for solid lines `planes` = {"P"|"U"|"V":
+    (phiS, nxS, nyS)} (cell arrays, phiS > 0 in the fluid).  The solid
+    fractions are formed from the lines (frac + clipping).  prev_AlN = {"U":
+    AlN, "V": AlN} of the previous call (ghosted); None runs the set-up twice
+    so that AlE(0,j) = AlN(1,j) reads the same geometry's north factor.
+    Returns {"P": dict, "U": dict, "V": dict} of numpy arrays."""
+    L = _geo_lib()
+    ni, nj = g.ni, g.nj
+    sp = grid_spacings(g)
+    keep = []
+
+    def arr(v):
+        a = np.ascontiguousarray(v, dtype=np.float64)
+        keep.append(a)
+        return a
+
+    grids = {}
+    for k in "PUV":
+        x, y, dx, dy = (arr(a) for a in sp[k])
+        grids[k] = _CGrid(ni, nj, _d(x), _d(y), _d(dx), _d(dy))
+
+    def run(alN):
+        out = {}
+        cells = {}
+        for k in "PUV":
+            phiS, nxS, nyS = (np.array(a, dtype=np.float64, order="C") for a in planes[k])
+            d = {"phiS": phiS, "nxS": nxS, "nyS": nyS, "vofS": np.zeros((nj, ni)), "WlLh": np.zeros((nj, ni)),
+                 "posnu": np.zeros((nj, ni), dtype=np.int32)}
+            x, y, dx, dy = sp[k]
+            L.orc_solid_fraction(ni, nj, _d(arr(dx)), _d(arr(dy)), _d(phiS), _d(nxS), _d(nyS), _d(d["vofS"]))
+            for n in ("aE", "aW", "aN", "aS", "fcex", "fcey", "fcnx", "fcny"):
+                d[n] = np.zeros((nj + 2, ni + 2))
+            if k != "P":
+                for n in ("delh", "ccx", "ccy"):
+                    d[n] = np.zeros((nj, ni))
+                d["moex"] = np.zeros((nj, ni), dtype=np.int32)
+                for n in ("EtaE", "EtaN", "AlE", "DAlE", "DAlN", "SxE", "SyN"):
+                    d[n] = np.zeros((nj + 2, ni + 2))
+                d["AlN"] = np.ascontiguousarray(alN[k], dtype=np.float64).copy()
+            c = _CCell()
+            for n in _CCELL_D1 + _CCELL_D2:
+                setattr(c, n, _d(d[n]) if n in d else _D())
+            for n in _CCELL_I:
+                setattr(c, n, _i(d[n]) if n in d else _I())
+            cells[k] = c
+            out[k] = d
+        L.orc_cut_cell_geometry(ctypes.byref(grids["P"]), ctypes.byref(grids["U"]), ctypes.byref(grids["V"]),
+                                ctypes.byref(cells["P"]), ctypes.byref(cells["U"]), ctypes.byref(cells["V"]),
+                                1 if icorprob else 0)
+        # a row of the momentum system: numbered by NumberExternalCell
+        # (the last column of U cells and last row of V cells never are)
+        for k, (iu, iv) in (("U", (1, 0)), ("V", (0, 1))):
+            act = (out[k]["posnu"] != -1).astype(np.int32)
+            if iu:
+                act[:, ni - 1] = 0
+            if iv:
+                act[nj - 1, :] = 0
+            out[k]["active"] = act
+        return out
+
+    if prev_AlN is None:
+        first = run({"U": np.ones((nj + 2, ni + 2)), "V": np.ones((nj + 2, ni + 2))})
+        prev_AlN = {"U": first["U"]["AlN"], "V": first["V"]["AlN"]}
+    return run(prev_AlN)

@@ -192,6 +192,12 @@ typedef struct orc_pred_in {
  * and Flux_n1 (ghosted, 2 planes, in/out). */
 void orc_predictor_assemble(const orc_pred_in* in, double* fluxn1, double* matr_u, double* rhs_u,
                             double* matr_v, double* rhs_v, double* DpU, double* DpV);
+/* test hooks: the superbee limiter (:2354-2367), the high-order convective
+ * face fluxes (:1186-1429; CF1 = CFluxEW, CF2 = CFluxNS, 2 ghosted planes
+ * each) and Mixing_Procedure (:2127-2197) of U (comp 0) or V cells */
+double orc_muscl_limiter(double r, double tolim);
+void orc_conv_fluxes(const orc_pred_in* in, double* CF1, double* CF2);
+void orc_mixing(const orc_pred_in* in, int comp, double* flux);
 /* ghost copies of PU%Dp / PV%Dp (:530-540) and Pred%v(:,nj) = Pred%v(:,nj-1)
  * (PredictorVelocityBoundaryCondition :2199-2213) */
 void orc_predictor_finish(int ni, int nj, double* DpU, double* DpV, double* vpred);
@@ -227,6 +233,34 @@ int orc_particle_track(int ni, int nj, const double* xc, const double* yc, const
                        const double* vof, const double* vofS, const double* u, const double* v, int n, double* px,
                        double* py, double* pu, double* pv, const double* pd, const orc_particle_params* P,
                        double dt);
+
+/* sg_oracle_cutcell.c -- cut-cell geometry (Cutcell_Mod.f90) */
+typedef struct orc_cgrid {
+    int ni, nj;
+    const double *x, *y;   /* TGrid%x(i,.) per i, TGrid%y(.,j) per j */
+    const double *dx, *dy; /* TGrid%dx per i, TGrid%dy per j */
+} orc_cgrid;
+/* One grid's Cell fields (Mesh_Mod.f90:11-41).  Cell arrays: vofS, phiS,
+ * nxS, nyS, WlLh, delh, Cell_Cent (ccx, ccy), MoExCell (moex), Posnu.
+ * Ghosted: apertures, FCE (fcex, fcey), FCN (fcnx, fcny) and the face
+ * factors.  P cells: moex, delh, ccx/ccy and the face factors NULL. */
+typedef struct orc_ccell {
+    double *vofS, *phiS, *nxS, *nyS, *WlLh, *delh, *ccx, *ccy;
+    int *moex, *posnu;
+    double *aE, *aW, *aN, *aS, *fcex, *fcey, *fcnx, *fcny;
+    double *EtaE, *EtaN, *AlE, *AlN, *DAlE, *DAlN, *SxE, *SyN;
+} orc_ccell;
+void orc_edge_geo_cal(double p1x, double p1y, double p2x, double p2y, double nxx, double nyy, double diss,
+                      double* area, double* fx, double* fy);
+void orc_solid_fraction(int ni, int nj, const double* dx, const double* dy, const double* phiS, const double* nxS,
+                        const double* nyS, double* vofS);
+void orc_cell_geo_cal(const orc_cgrid* G, orc_ccell* C, int icorprob);
+void orc_define_moex(int ni, int nj, orc_ccell* P, orc_ccell* U, orc_ccell* V);
+void orc_number_external(int ni, int nj, orc_ccell* C, int iu, int iv);
+void orc_new_cell_face(const orc_cgrid* PG, const orc_cgrid* UG, const orc_cgrid* VG, orc_ccell* P, orc_ccell* U,
+                       orc_ccell* V);
+void orc_cut_cell_geometry(const orc_cgrid* PG, const orc_cgrid* UG, const orc_cgrid* VG, orc_ccell* P,
+                           orc_ccell* U, orc_ccell* V, int icorprob);
 
 #ifdef __cplusplus
 }

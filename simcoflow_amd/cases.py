@@ -18,8 +18,6 @@ Arrays use the reference's Fortran layout: a cell array has numpy shape
 the (0,0) ghost at [0, 0].
 """
 from dataclasses import dataclass, field
-import heapq
-
 import numpy as np
 
 EPSI = 1.0e-3      # StateVariables.f90:28
@@ -197,188 +195,48 @@ class Geometry:
 
 def _solid_cells(phiS, nxS, nyS, dxc, dyc):
     """Solid volume fraction from the solid plane (Clsvof_Mod.f90:273-277)."""
-    vofS = frac(nxS, nyS, phiS, dxc, dyc)
-    vofS = np.where(vofS < VOFEPS, 0.0, vofS)
-    vofS = np.where(vofS >= 1.0 - VOFEPS, 1.0, vofS)
-    return vofS
+    from .cutcell import solid_fraction
+    return solid_fraction(phiS, nxS, nyS, dxc, dyc)
 
 
-def _edge_areas(phiS, nxS, nyS, dxc, dyc):
-    """Edge_Geo_Cal on the four edges (Cutcell_Mod.f90:52-73,302-337):
-    returns the *solid* fraction of each edge (W, N, E, S)."""
-    hx = 0.5 * dxc
-    hy = 0.5 * dyc
-    eps = 1e-14
-    # corner level-set values: SW, NW, NE, SE
-    L = {}
-    for key, (sx, sy) in {"sw": (-1, -1), "nw": (-1, 1), "ne": (1, 1), "se": (1, -1)}.items():
-        L[key] = nxS * (sx * hx) + nyS * (sy * hy) + phiS
-
-    def edge(l1, l2):
-        a = np.zeros(np.broadcast(l1, l2).shape)
-        both_solid = (l1 < eps) & (l2 < eps)
-        a[both_solid] = 1.0
-        m1 = (l1 >= eps) & (l2 < eps)
-        m2 = (l1 < eps) & (l2 >= eps)
-        den = np.abs(l1) + np.abs(l2)
-        den = np.where(den > 0, den, 1.0)
-        a = np.where(m1, np.abs(l2) / den, a)
-        a = np.where(m2, np.abs(l1) / den, a)
-        return a
-
-    return (edge(L["sw"], L["nw"]), edge(L["nw"], L["ne"]),
-            edge(L["ne"], L["se"]), edge(L["se"], L["sw"]))
+def solid_planes(g: Grid, solid=None):
+    """Solid line of every P, U and V cell: {"P"|"U"|"V": (phiS, nxS, nyS)}
+    from ``solid(X, Y) -> (phiS, nxS, nyS)`` (a signed distance, positive in
+    the fluid) at the cell centres, or a far plane where there is no solid."""
+    from .cutcell import grid_axes
+    out = {}
+    for k, (x, y, _, _) in grid_axes(g).items():
+        X, Y = np.meshgrid(x, y)
+        if solid is None:
+            out[k] = (np.full(X.shape, 1.0e3), np.zeros(X.shape), np.ones(X.shape))
+        else:
+            out[k] = tuple(np.ascontiguousarray(a, dtype=np.float64) for a in solid(X, Y))
+    return out
 
 
-def _cell_geo(phiS, nxS, nyS, vofS, dxc, dyc, closed_rule):
-    """Aperture part of Cell_Geo_Cal (Cutcell_Mod.f90:34-300) for one grid.
-    Returns ghosted apertures, the wall length and the modified vofS."""
-    nj, ni = phiS.shape
-    sW, sN, sE, sS = _edge_areas(phiS, nxS, nyS, dxc, dyc)
-    # wall length from the raw edge values (Cutcell_Mod.f90:79-89)
-    WlLh = np.sqrt(((sE - sW) * dxc) ** 2 + ((sN - sS) * dyc) ** 2)
-    aW = np.zeros((nj + 2, ni + 2)); aE = np.zeros_like(aW)
-    aN = np.zeros_like(aW); aS = np.zeros_like(aW)
-    aW[1:-1, 1:-1] = 1.0 - sW
-    aE[1:-1, 1:-1] = 1.0 - sE
-    aN[1:-1, 1:-1] = 1.0 - sN
-    aS[1:-1, 1:-1] = 1.0 - sS
-    # harmonic face averaging for i=2..Isize-1, j=2..Jsize-1 (:161-204)
-    if ni >= 3 and nj >= 3:
-        I = slice(2, ni)      # ghosted index of i = 2..ni-1
-        J = slice(2, nj)
-        Ip = slice(3, ni + 1)
-        Jp = slice(3, nj + 1)
-        n_ = aN[J, I]; s_ = aS[Jp, I]
-        avg = 2.0 * n_ * s_ / (n_ + s_ + ROUNDOFF)
-        aN[J, I] = avg
-        aS[Jp, I] = avg
-        e_ = aE[J, I]; w_ = aW[J, Ip]
-        avg = 2.0 * e_ * w_ / (e_ + w_ + ROUNDOFF)
-        aE[J, I] = avg
-        aW[J, Ip] = avg
-    vofS = vofS.copy()
-    WlLh = WlLh.copy()
-    # solid cells close their faces and their neighbours' (:215-228)
-    solid = vofS > 1.0 - EPSI
-    vofS[solid] = 1.0
-    js, is_ = np.nonzero(solid)
-    gj, gi = js + 1, is_ + 1
-    for arr in (aW, aE, aN, aS):
-        arr[gj, gi] = 0.0
-    aE[gj, gi - 1] = 0.0
-    aN[gj - 1, gi] = 0.0
-    aW[gj, gi + 1] = 0.0
-    aS[gj + 1, gi] = 0.0
-    if closed_rule:
-        # cells closed on both sides of an axis become solid, visited in the
-        # reference's loop order (i outer, j inner; :229-242)
-        def closed(j, i):
-            g_j, g_i = j + 1, i + 1
-            return ((aW[g_j, g_i] + aE[g_j, g_i] < ROUNDOFF or aS[g_j, g_i] + aN[g_j, g_i] < ROUNDOFF)
-                    and vofS[j, i] < 1.0 - EPSI)
-        anyzero = ((aW[1:-1, 1:-1] == 0) | (aE[1:-1, 1:-1] == 0) | (aN[1:-1, 1:-1] == 0) |
-                   (aS[1:-1, 1:-1] == 0)) & (vofS < 1.0 - EPSI)
-        cj, ci = np.nonzero(anyzero)
-        heap = [(int(i), int(j)) for j, i in zip(cj, ci)]
-        heapq.heapify(heap)
-        seen = set(heap)
-        while heap:
-            i, j = heapq.heappop(heap)
-            if closed(j, i):
-                vofS[j, i] = 1.0
-                g_j, g_i = j + 1, i + 1
-                for arr in (aW, aE, aN, aS):
-                    arr[g_j, g_i] = 0.0
-                aE[g_j, g_i - 1] = 0.0
-                aN[g_j - 1, g_i] = 0.0
-                aW[g_j, g_i + 1] = 0.0
-                aS[g_j + 1, g_i] = 0.0
-                for (ii, jj) in ((i + 1, j), (i, j + 1)):
-                    if ii < ni and jj < nj and (ii, jj) not in seen:
-                        seen.add((ii, jj))
-                        heapq.heappush(heap, (ii, jj))
-    # wall length of solid cells (:245-258)
-    full = np.abs(vofS - 1.0) < ROUNDOFF
-    ew0 = (aW[1:-1, 1:-1] == 0.0) | (aE[1:-1, 1:-1] == 0.0)
-    ns0 = (aN[1:-1, 1:-1] == 0.0) | (aS[1:-1, 1:-1] == 0.0)
-    WlLh = np.where(full & ew0, np.broadcast_to(dyc, WlLh.shape), WlLh)
-    WlLh = np.where(full & ns0, np.broadcast_to(dxc, WlLh.shape), WlLh)
-    return aE, aW, aN, aS, WlLh, vofS
+def _geometry_from(cc):
+    P, U, V = cc["P"], cc["U"], cc["V"]
+    uflag = ((U["numbered"] != 0) & (U["moex"] != 1)).astype(np.int32)
+    vflag = ((V["numbered"] != 0) & (V["moex"] != 1)).astype(np.int32)
+    c = np.ascontiguousarray
+    return Geometry(vofS=c(P["vofS"]), phiS=c(P["phiS"]), nxS=c(P["nxS"]), nyS=c(P["nyS"]), WlLh=c(P["WlLh"]),
+                    aE=c(P["aE"]), aW=c(P["aW"]), aN=c(P["aN"]), aS=c(P["aS"]), vofSU=c(U["vofS"]),
+                    vofSV=c(V["vofS"]), moexU=c(U["moex"].astype(np.int32)), moexV=c(V["moex"].astype(np.int32)),
+                    uflag=uflag, vflag=vflag)
+
+
+def cut_cells(g: Grid, solid=None, prev_AlN=None):
+    """The whole cut-cell geometry (Grid_Preprocess + NewCellFace) for a
+    solid given as in solid_planes: returns (Geometry, (U, V))."""
+    from .cutcell import cut_cell_geometry
+    cc = cut_cell_geometry(g, solid_planes(g, solid), prev_AlN=prev_AlN)
+    return _geometry_from(cc), tuple(_stag_from(cc[k]) for k in "UV")
 
 
 def build_geometry(g: Grid, solid=None):
-    """Cut-cell geometry for a solid given as ``solid(X, Y) -> (phiS, nxS, nyS)``
-    (a signed distance, positive in the fluid), or no solid at all."""
-    ni, nj = g.ni, g.nj
-    DX = np.broadcast_to(g.dx[None, :], (nj, ni))
-    DY = np.broadcast_to(g.dy[:, None], (nj, ni))
-    Xp, Yp = np.meshgrid(g.xc, g.yc)
-    # U cells centred on east faces, V cells on north faces (Mesh_Mod.f90:151,165)
-    Xu = Xp + 0.5 * DX
-    Yv = Yp + 0.5 * DY
-    UDX = np.broadcast_to(g.udx[None, :], (nj, ni))
-    VDY = np.broadcast_to(g.vdy[:, None], (nj, ni))
-
-    if solid is None:
-        # no solid: every aperture open (Edge_Geo_Cal gives 0 solid on every
-        # edge), no wall, no momentum-exchange cell
-        zc = np.zeros((nj, ni))
-        ones_g = np.zeros((nj + 2, ni + 2))
-        ones_g[1:-1, 1:-1] = 1.0
-        return Geometry(vofS=zc.copy(), phiS=np.full((nj, ni), 1.0e3), nxS=zc.copy(), nyS=np.ones((nj, ni)),
-                        WlLh=zc.copy(), aE=ones_g.copy(), aW=ones_g.copy(), aN=ones_g.copy(), aS=ones_g.copy(),
-                        vofSU=zc.copy(), vofSV=zc.copy(), moexU=np.zeros((nj, ni), np.int32),
-                        moexV=np.zeros((nj, ni), np.int32), uflag=np.ones((nj, ni), np.int32),
-                        vflag=np.ones((nj, ni), np.int32))
-
-    def plane(X, Y):
-        return solid(X, Y)
-
-    phiS, nxS, nyS = plane(Xp, Yp)
-    vofS = _solid_cells(phiS, nxS, nyS, DX, DY)
-    aE, aW, aN, aS, WlLh, vofS = _cell_geo(phiS, nxS, nyS, vofS, DX, DY, closed_rule=True)
-
-    phiU, nxU, nyU = plane(Xu, Yp)
-    vofSU = _solid_cells(phiU, nxU, nyU, UDX, DY)
-    _, _, _, _, _, vofSU = _cell_geo(phiU, nxU, nyU, vofSU, UDX, DY, closed_rule=False)
-    phiV, nxV, nyV = plane(Xp, Yv)
-    vofSV = _solid_cells(phiV, nxV, nyV, DX, VDY)
-    _, _, _, _, _, vofSV = _cell_geo(phiV, nxV, nyV, vofSV, DX, VDY, closed_rule=False)
-
-    # DefineMomentumExchangeCell (Cutcell_Mod.f90:360-397), in loop order
-    moexU = np.zeros((nj, ni), np.int32)
-    moexV = np.zeros((nj, ni), np.int32)
-    candU = (vofSU >= EPSI) & (vofSU < 1.0 - EPSI)
-    candV = (vofSV >= EPSI) & (vofSV < 1.0 - EPSI)
-    cj, ci = np.nonzero(candU | candV)
-    order = np.lexsort((cj, ci))
-    for k in order:
-        j, i = int(cj[k]), int(ci[k])
-        gj, gi = j + 1, i + 1
-        if candU[j, i]:
-            if i < ni - 1:
-                if vofS[j, i] >= 1 - EPSI or vofS[j, i + 1] >= 1 - EPSI or aE[gj, gi] < EPSIF:
-                    moexU[j, i] = 1
-                    aE[gj, gi] = 0.0
-                    aW[gj, gi + 1] = 0.0
-            elif vofS[j, i] >= 1 - EPSI:
-                moexU[j, i] = 1
-        if candV[j, i]:
-            if j < nj - 1:
-                if vofS[j, i] >= 1 - EPSI or vofS[j + 1, i] >= 1 - EPSI or aN[gj, gi] < EPSIF:
-                    moexV[j, i] = 1
-                    aN[gj, gi] = 0.0
-                    aS[gj + 1, gi] = 0.0
-            elif vofS[j, i] >= 1 - EPSI:
-                moexV[j, i] = 1
-    uflag = ((vofSU < 1.0 - EPSI) & (moexU != 1)).astype(np.int32)
-    vflag = ((vofSV < 1.0 - EPSI) & (moexV != 1)).astype(np.int32)
-    return Geometry(vofS=np.ascontiguousarray(vofS), phiS=np.ascontiguousarray(phiS),
-                    nxS=np.ascontiguousarray(nxS), nyS=np.ascontiguousarray(nyS),
-                    WlLh=np.ascontiguousarray(WlLh), aE=aE, aW=aW, aN=aN, aS=aS,
-                    vofSU=np.ascontiguousarray(vofSU), vofSV=np.ascontiguousarray(vofSV),
-                    moexU=moexU, moexV=moexV, uflag=uflag, vflag=vflag)
+    """Cut-cell geometry of the P cells (+ the U/V flags the correction
+    needs) for a solid given as in solid_planes, or no solid at all."""
+    return cut_cells(g, solid)[0]
 
 
 # ---------------------------------------------------------------------------
@@ -659,134 +517,21 @@ class StagGeometry:
     fcny: np.ndarray
 
 
-def _fluid_centroid(nxS, nyS, phiS, hx, hy):
-    """Centroid of {nxS x + nyS y + phiS >= 0} inside the hx*hy box centred
-    at 0 (polygon clipping); (0,0) when the box is all fluid or all solid."""
-    poly = [(-hx / 2, -hy / 2), (hx / 2, -hy / 2), (hx / 2, hy / 2), (-hx / 2, hy / 2)]
-    out = []
-    for k in range(4):
-        p, q = poly[k], poly[(k + 1) % 4]
-        fp = nxS * p[0] + nyS * p[1] + phiS
-        fq = nxS * q[0] + nyS * q[1] + phiS
-        if fp >= 0:
-            out.append(p)
-        if (fp < 0 <= fq) or (fq < 0 <= fp):
-            t = fp / (fp - fq)
-            out.append((p[0] + t * (q[0] - p[0]), p[1] + t * (q[1] - p[1])))
-    if len(out) < 3:
-        return 0.0, 0.0
-    a = cx = cy = 0.0
-    for k in range(len(out)):
-        p, q = out[k], out[(k + 1) % len(out)]
-        cr = p[0] * q[1] - q[0] * p[1]
-        a += cr
-        cx += (p[0] + q[0]) * cr
-        cy += (p[1] + q[1]) * cr
-    if abs(a) < 1e-300:
-        return 0.0, 0.0
-    return cx / (3.0 * a), cy / (3.0 * a)
-
-
-def _stag(g, hx, hy, phiS, nxS, nyS, vofS, moex, which, spacing_next):
-    """One staggered grid.  hx, hy: (nj, ni) cell sizes; spacing_next: the
-    centre distance to the next cell along x (for SxE) and along y (SyN)."""
-    nj, ni = phiS.shape
-    aE, aW, aN, aS, WlLh, vofS2 = _cell_geo(phiS, nxS, nyS, vofS, hx, hy, closed_rule=False)
-    cut = (vofS2 > EPSI) & (vofS2 < 1.0 - EPSI)
-    ccx = np.zeros((nj, ni))
-    ccy = np.zeros((nj, ni))
-    for j, i in zip(*np.nonzero(cut)):
-        ccx[j, i], ccy[j, i] = _fluid_centroid(nxS[j, i], nyS[j, i], phiS[j, i], hx[j, i], hy[j, i])
-    G = lambda: np.zeros((nj + 2, ni + 2))
-    fcex, fcey, fcnx, fcny = G(), G(), G(), G()
-    fcex[1:-1, 1:-1] = hx / 2
-    fcny[1:-1, 1:-1] = hy / 2
-    # open segment of a cut east / north edge: its centre sits at half the open
-    # length from the fluid end
-    e_open = aE[1:-1, 1:-1]
-    n_open = aN[1:-1, 1:-1]
-    fcey[1:-1, 1:-1] = np.where(cut, (0.5 - e_open / 2.0) * np.sign(nyS) * hy, 0.0)
-    fcnx[1:-1, 1:-1] = np.where(cut, (0.5 - n_open / 2.0) * np.sign(nxS) * hx, 0.0)
-    for a in (fcex, fcey, fcnx, fcny):
-        a[0, :] = a[1, :]
-        a[-1, :] = a[-2, :]
-        a[:, 0] = a[:, 1]
-        a[:, -1] = a[:, -2]
-    active = (vofS2 < 1.0 - EPSI).astype(np.int32)
-    if which == 0:
-        active[:, -1] = 0
-    else:
-        active[-1, :] = 0
-    SxE, EtaE, AlE, DAlE = G(), G(), G(), G()
-    SyN, EtaN, AlN, DAlN = G(), G(), G(), G()
-    sx_next, sy_next = spacing_next
-    SxE[1:-1, 1:-1] = sx_next
-    SyN[1:-1, 1:-1] = sy_next
-    EtaE[:] = 0.5
-    EtaN[:] = 0.5
-    AlE[:] = 1.0
-    AlN[:] = 1.0
-    # cut faces: interpolation factor from the centroids, the shift of the
-    # interpolation point to the face centre (EastFaceInterpolationInf,
-    # NorthFaceInterpolationInf of Cutcell_Mod.f90 in simplified form)
-    for j in range(nj):
-        for i in range(ni):
-            if i + 1 < ni and active[j, i] and active[j, i + 1] and (cut[j, i] or cut[j, i + 1]):
-                sx = ccx[j, i + 1] + sx_next[j, i] - ccx[j, i]
-                sy = ccy[j, i + 1] - ccy[j, i]
-                SxE[j + 1, i + 1] = sx
-                eta = abs(fcex[j + 1, i + 1] - ccx[j, i]) / sx
-                EtaE[j + 1, i + 1] = 0.5 if eta >= 1.0 else eta
-                if abs(sy) > 1e-4 * hy[j, i]:
-                    AlE[j + 1, i + 1] = min(2.0, 1.0 + abs(sy) / max(hy[j, i], 1e-300))
-                    DAlE[j + 1, i + 1] = sy * 0.5 * (nyS[j, i] + nyS[j, i + 1]) / (sx * hy[j, i])
-            if j + 1 < nj and active[j, i] and active[j + 1, i] and (cut[j, i] or cut[j + 1, i]):
-                sy = ccy[j + 1, i] + sy_next[j, i] - ccy[j, i]
-                sx = ccx[j + 1, i] - ccx[j, i]
-                SyN[j + 1, i + 1] = sy
-                eta = abs(fcny[j + 1, i + 1] - ccy[j, i]) / sy
-                EtaN[j + 1, i + 1] = 0.5 if eta >= 1.0 else eta
-                if abs(sx) >= 1e-4 * hx[j, i]:
-                    AlN[j + 1, i + 1] = min(2.0, 1.0 + abs(sx) / max(hx[j, i], 1e-300))
-                    DAlN[j + 1, i + 1] = sx * 0.5 * (nxS[j, i] + nxS[j + 1, i]) / (sy * hx[j, i])
-    for a in (SxE, EtaE, AlE, DAlE, SyN, EtaN, AlN, DAlN):
-        a[0, :] = a[1, :]
-        a[-1, :] = a[-2, :]
-        a[:, 0] = a[:, 1]
-        a[:, -1] = a[:, -2]
-    delh = np.abs(ccx * nxS + ccy * nyS + phiS) + 1.0e-24
-    delh = np.where(moex == 1, 1.0e-20, delh)
+def _stag_from(T):
     c = np.ascontiguousarray
-    return StagGeometry(vofS=c(vofS2), nxS=c(nxS), nyS=c(nyS), phiS=c(phiS), WlLh=c(WlLh), delh=c(delh),
-                        ccx=c(ccx), ccy=c(ccy), moex=c(moex.astype(np.int32)), active=c(active),
-                        aE=aE, aW=aW, aN=aN, aS=aS, EtaE=EtaE, EtaN=EtaN, AlE=AlE, AlN=AlN, DAlE=DAlE,
-                        DAlN=DAlN, SxE=SxE, SyN=SyN, fcex=fcex, fcey=fcey, fcnx=fcnx, fcny=fcny)
+    kw = {k: c(T[k]) for k in ("vofS", "nxS", "nyS", "phiS", "WlLh", "delh", "ccx", "ccy", "aE", "aW", "aN", "aS",
+                               "EtaE", "EtaN", "AlE", "AlN", "DAlE", "DAlN", "SxE", "SyN", "fcex", "fcey", "fcnx",
+                               "fcny")}
+    kw["moex"] = c(T["moex"].astype(np.int32))
+    kw["active"] = c(T["active"].astype(np.int32))
+    return StagGeometry(**kw)
 
 
-def staggered_geometry(g: Grid, geo: Geometry, solid=None):
-    """U and V cell geometry for the predictor; `solid` as in build_geometry."""
-    ni, nj = g.ni, g.nj
-    DX = np.broadcast_to(g.dx[None, :], (nj, ni))
-    DY = np.broadcast_to(g.dy[:, None], (nj, ni))
-    UDX = np.ascontiguousarray(np.broadcast_to(g.udx[None, :], (nj, ni)))
-    VDY = np.ascontiguousarray(np.broadcast_to(g.vdy[:, None], (nj, ni)))
-    Xp, Yp = np.meshgrid(g.xc, g.yc)
-    Xu, Yv = Xp + 0.5 * DX, Yp + 0.5 * DY
-    if solid is None:
-        zero = np.zeros((nj, ni))
-        planes = [(np.full((nj, ni), 1.0e3), zero.copy(), np.ones((nj, ni)))] * 2
-    else:
-        planes = [solid(Xu, Yp), solid(Xp, Yv)]
-    # centre distance to the next cell: U cells along x = the P spacing of
-    # cell i+1, along y = dy; V cells along x = the U spacing, along y = dy
-    # of cell j+1 (Cutcell_Mod.f90 EastFaceInterpolationInf BGrid%dx)
-    dxn = np.append(g.dx[1:], g.dx[-1])[None, :].repeat(nj, 0)
-    dyn = np.append(g.dy[1:], g.dy[-1])[:, None].repeat(ni, 1)
-    U = _stag(g, UDX, np.ascontiguousarray(DY), planes[0][0], planes[0][1], planes[0][2], geo.vofSU.copy(),
-              geo.moexU, 0, (dxn, np.ascontiguousarray(VDY)))
-    V = _stag(g, np.ascontiguousarray(DX), VDY, planes[1][0], planes[1][1], planes[1][2], geo.vofSV.copy(),
-              geo.moexV, 1, (UDX, dyn))
-    return U, V
+def staggered_geometry(g: Grid, geo: Geometry = None, solid=None, prev_AlN=None):
+    """U and V cell geometry for the predictor (NewCellFace); `solid` as in
+    solid_planes.  prev_AlN = {"U": AlN, "V": AlN} of the geometry before
+    (AlE(0,j) reads it); by default the same geometry's own."""
+    return cut_cells(g, solid, prev_AlN)[1]
 
 
 def staggered_vof(g: Grid, stag: StagGeometry, which, levelset):

@@ -96,6 +96,8 @@ struct PcgScalars {
     double true_rr; // ||b - A x||^2 after the solve
     double true_max;// max |b - A x| over fluid rows after the solve
     double mu;      // deflation: (A w, z) / (w, A w)
+    double gamma;   // deflation: (w, A p) / (w, A w) of the last direction (re-projection)
+    double mu_eff;  // deflation: coefficient of w in the next direction, mu + beta * gamma
     double wq;      // deflation: (w, A w)
     double wb;      // learned warm start: (w, b)
     double walpha;  // learned warm start: x0 = walpha * w
@@ -110,7 +112,7 @@ struct PcgScalars {
 
 // work arrays of the device geometry rebuild (allocated at its first call)
 struct GeoScratch {
-    double* raw[4] = {nullptr, nullptr, nullptr, nullptr};  // raw E, W, N, S apertures (ghosted)
+    double* pfc[4] = {nullptr, nullptr, nullptr, nullptr};  // P-cell FCE (x, y), FCN (x, y), ghosted
     int* close[2] = {nullptr, nullptr};                     // closing-rule decisions, two passes
     int* changed = nullptr;
     int* moex[2] = {nullptr, nullptr};                      // U, V momentum-exchange cells
@@ -317,6 +319,7 @@ int poisson_rhs_only(Ctx* c);
 int poisson_apply_fine(Ctx* c, const double* x, double* y);  // padded layout
 int poisson_scatter(Ctx* c);
 int poisson_matrix_rows(Ctx* c, double* d_matr);
+int poisson_rhs_cells(Ctx* c, double* d_out);  // SetPoissonVectors' b, cell layout
 int velocity_correct(Ctx* c);
 int padded_to_cell(Ctx* c, const double* src, double* dst_cell);
 int cell_to_padded(Ctx* c, const double* src_cell, double* dst);
