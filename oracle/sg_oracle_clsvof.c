@@ -19,9 +19,11 @@
  * orc_cellgeocal (polygon clipping), orc_plic_distance and
  * orc_distance_fluid (round trips), orc_reconstruct on straight lines,
  * orc_redistance (straight line, circle), the sweeps (exact translation,
- * uniform states, volume), orc_interface_to_uv (straight lines, solid
- * bounds).  Parity unpinned (restated line by line, nothing independent fixes
- * them): the quality-based stencil choice of orc_normal_irre, the stale
+ * uniform states, volume; the level-set face value by exact translation of a
+ * linear profile and first-order upwind differences on a quadratic one, both
+ * sweeps and both flow directions), orc_interface_to_uv (straight lines,
+ * solid bounds).  Parity unpinned (restated line by line, nothing independent
+ * fixes them): the quality-based stencil choice of orc_normal_irre, the stale
  * normals on the domain edge, the tie order of orc_redistance, the one-sided
  * level-set fluxes at the first and last faces, orc_inflow_level, and the
  * stale-value walk-back of orc_interface_to_uv.

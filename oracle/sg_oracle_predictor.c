@@ -22,9 +22,15 @@
  *
  * Pinned by closed forms (tests/test_oracle_predictor.py): steady uniform
  * stream, gravity increment and Dp = dt at rest, steady Couette shear, the
- * Adams-Bashforth history.  Parity unpinned (restated line by line): the
- * MUSCL / kinetic flux details (high_order_x/y, muscl), the mixing weights
- * (mixing) and the cut-cell interpolation branches (modified_conv).
+ * Adams-Bashforth history; the limiter against Sweby's superbee table, its
+ * TVD region and symmetry; the MUSCL / kinetic face fluxes exact on linear
+ * profiles; the mixing of a cell with one target (the volume-weighted mean),
+ * its conservation of sum (1 - vofS) flux, and the reference's overwrite of
+ * a small cell that an earlier one mixed into.  Parity unpinned (restated
+ * line by line): the first-order face fluxes of the first and last faces,
+ * the Beta weights of a cell with several targets, and the cut-cell
+ * interpolation branches (modified_conv and the near-solid branches of
+ * high_order_x/y, whose geometry inputs are pinned, not their use).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -652,18 +658,28 @@ static void mixing(const PC* p, const orc_stag* T, int idir, int jdir, double* f
             } else {
                 Fm[2] = 0.0; Be[2] = 0.0; Vt[2] = 1.0;
             }
+            /* A fully solid target (Vt = 0, so Be = 0) makes the reference's
+             * Fmix*Beta/(Beta*(1-vofS)+Vtgt) 0/0, and a cell with no fluid
+             * target makes Beta/sum(Beta) 0/0 (NaN in the reference): both
+             * take the limit 0 here, the share such a target can hold.  Every
+             * other case is the reference's expression unchanged. */
             double temp = (Be[0] + Be[1]) + Be[2];
             for (int k = 0; k < 3; k++) {
-                Be[k] = Be[k] / temp;
-                Fm[k] = Fm[k] * Be[k] / (Be[k] * (1.0 - T->vofS[c]) + Vt[k]);
+                if (temp > 0.0 && Vt[k] > 0.0) {
+                    Be[k] = Be[k] / temp;
+                    Fm[k] = Fm[k] * Be[k] / (Be[k] * (1.0 - T->vofS[c]) + Vt[k]);
+                } else {
+                    Fm[k] = 0.0;
+                }
             }
             fm[c] = f[c] + 1.0 / (1.0 - T->vofS[c]) * ((Fm[0] + Fm[1]) + Fm[2]);
             ii = i + sx; jj = j;
-            if (ii <= imax && ii >= 1) fm[CI(ii, jj)] = fm[CI(ii, jj)] - 1.0 / Vt[0] * Fm[0];
+            if (ii <= imax && ii >= 1 && Vt[0] > 0.0) fm[CI(ii, jj)] = fm[CI(ii, jj)] - 1.0 / Vt[0] * Fm[0];
             ii = i; jj = j + sy;
-            if (jj <= jmax && jj >= 1) fm[CI(ii, jj)] = fm[CI(ii, jj)] - 1.0 / Vt[1] * Fm[1];
+            if (jj <= jmax && jj >= 1 && Vt[1] > 0.0) fm[CI(ii, jj)] = fm[CI(ii, jj)] - 1.0 / Vt[1] * Fm[1];
             ii = i + sx; jj = j + sy;
-            if (ii <= imax && ii >= 1 && jj <= jmax && jj >= 1) fm[CI(ii, jj)] = fm[CI(ii, jj)] - 1.0 / Vt[2] * Fm[2];
+            if (ii <= imax && ii >= 1 && jj <= jmax && jj >= 1 && Vt[2] > 0.0)
+                fm[CI(ii, jj)] = fm[CI(ii, jj)] - 1.0 / Vt[2] * Fm[2];
         }
     }
     free(f);
@@ -938,4 +954,25 @@ void orc_predictor_finish(int ni, int nj, double* DpU, double* DpV, double* vpre
     for (int i = 1; i <= NI; i++) vpred[GI(i, NJ)] = vpred[GI(i, NJ - 1)];
 #undef DU
 #undef DV
+}
+
+/* ---- test hooks: single pieces of the predictor for the closed-form checks
+ * (no counterpart in the reference) ---------------------------------------- */
+double orc_muscl_limiter(double r, double tolim) { return muscl(r, tolim); }
+
+/* HighOrderConvectiveFluxForXDir / YDir (itt > 1): CFluxEW (2 ghosted
+ * planes, comps 1,2) and CFluxNS */
+void orc_conv_fluxes(const orc_pred_in* in, double* CF1, double* CF2)
+{
+    PC pc = {in, in->ni, in->nj}, *p = &pc;
+    size_t ng = (size_t)(NI + 2) * (NJ + 2);
+    high_order_x(p, CF1, CF1 + ng);
+    high_order_y(p, CF2, CF2 + ng);
+}
+
+/* Mixing_Procedure of the U (comp 0) or V (comp 1) cells on a cell array */
+void orc_mixing(const orc_pred_in* in, int comp, double* flux)
+{
+    PC pc = {in, in->ni, in->nj}, *p = &pc;
+    mixing(p, comp == 0 ? &in->U : &in->V, comp == 0, comp == 1, flux);
 }

@@ -248,13 +248,22 @@ static void alloc_level(level_t* L, int ni, int nj, int nine) {
     L->r = dalloc(n);
 }
 
+/* A fluid row whose faces are all closed keeps only the regularisation
+ * D = +1e-24 of ProjectionP_Mod.f90:227 and no coupling: it is a 1x1 system
+ * of its own, solved exactly as b / D after the PCG; the hierarchy treats it
+ * as a solid row. */
+static int decoupled(const double* matr, size_t k, size_t N) {
+    return matr[k] == 0.0 && matr[k + N] == 0.0 && matr[k + 3 * N] == 0.0 && matr[k + 4 * N] == 0.0 &&
+           matr[k + 2 * N] != 0.0 && fabs(matr[k + 2 * N]) <= 1.0e-24;
+}
+
 static void build_hier(hier_t* h, int ni, int nj, const double* matr) {
     memset(h, 0, sizeof(*h));
     const size_t N = (size_t)ni * nj;
     level_t* f = &h->lv[0];
     alloc_level(f, ni, nj, 0);
     for (size_t k = 0; k < N; ++k) {
-        f->diag[k] = matr[k + 2 * N] > 0.0 ? matr[k + 2 * N] : 0.0;
+        f->diag[k] = (matr[k + 2 * N] > 0.0 && !decoupled(matr, k, N)) ? matr[k + 2 * N] : 0.0;
         f->cE[k] = -matr[k + 3 * N];
         f->cN[k] = -matr[k + 4 * N];
     }
@@ -405,6 +414,8 @@ int orc_pcg_mg_solve(int ni, int nj, const double* matr, const double* b,
     int it = 0;
     if (bn == 0.0) {
         memset(x, 0, N * sizeof(double));
+        for (size_t k = 0; k < N; ++k)
+            if (decoupled(matr, k, N)) x[k] = b[k] / matr[k + 2 * N];
         if (relres_out) *relres_out = 0.0;
         free_hier(&h); free(r); free(z); free(p); free(q);
         return 0;
@@ -441,6 +452,8 @@ int orc_pcg_mg_solve(int ni, int nj, const double* matr, const double* b,
         first = 1;
         if (rn <= tol * bn || it >= max_iter) break;
     }
+    for (size_t k = 0; k < N; ++k)
+        if (decoupled(matr, k, N)) x[k] = b[k] / matr[k + 2 * N];
     if (relres_out) *relres_out = rn / bn;
     free_hier(&h); free(r); free(z); free(p); free(q);
     return (rn > tol * bn) ? -it : it;

@@ -413,7 +413,7 @@ int sg_poisson_rhs(sg_ctx* ctx, double* rhs_out) {
         set_error("sg_poisson_rhs: call sg_poisson_build_resident first");
         return SG_ERR_STATE;
     }
-    int rc = padded_to_cell(c, c->rhs, c->stage);
+    int rc = poisson_rhs_cells(c, c->stage);
     if (rc) return rc;
     SG_HIP(hipMemcpyAsync(rhs_out, c->stage, (size_t)c->ni * c->nj * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
@@ -517,6 +517,7 @@ int sg_predictor_resident(sg_ctx* ctx, double dt, double vb, int first_step, sg_
     loc.ms = ms;
     if (info) *info = loc;
     c->built = 0;
+    if (rc == SG_NAN || rc2 == SG_NAN) return SG_NAN;  // the solve's message stands
     if (rc || rc2) {
         set_error("sg_predictor_resident: a momentum system did not converge");
         return SG_NOT_CONVERGED;

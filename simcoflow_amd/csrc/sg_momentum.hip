@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(NT) k_bi_fin(const double* __restrict__ part, 
         sc->tol2 = tol * tol * s[0];
         sc->active = s[0] > sc->tol2 ? 1 : 0;
         sc->iters = 1;
-        sc->nan = 0;
+        sc->nan = s[0] != s[0];  // a NaN in b is not b = 0
         break;
     case F_INIT:
         sc->bb = s[0];
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(NT) k_bi_fin(const double* __restrict__ part, 
         sc->tol2 = tol * tol * s[0];
         sc->active = s[0] > 0.0 ? 1 : 0;
         sc->iters = 0;
-        sc->nan = 0;
+        sc->nan = s[0] != s[0];  // a NaN in b is not b = 0
         sc->rho_old = 1.0;
         sc->alpha = 1.0;
         sc->omega = 1.0;

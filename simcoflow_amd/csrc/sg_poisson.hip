@@ -130,6 +130,10 @@ __device__ inline RowOut cell_row(const CoefIn& in, int ic, int jc, int bcW, int
     return o;
 }
 
+__device__ inline bool decoupled_row(const RowOut& o) {
+    return o.fluid && o.S == 0.0 && o.W == 0.0 && o.E == 0.0 && o.N == 0.0 && fabs(o.D) <= TOLB;
+}
+
 struct RhsIn {
     FLay L;
     const double *u, *v, *aE, *aW, *aN, *aS, *nyS, *WlLh, *vofS, *dx, *dy;
@@ -165,8 +169,13 @@ __global__ void __launch_bounds__(256) k_build(CoefIn cin, RhsIn rin, int bcW, i
         if (gbits) atomicOr(guard, gbits);
     }
     const size_t p = D.off(ic, jc);
+    // a fluid cell whose four faces the cut-cell rules all closed keeps the
+    // reference's row D = +1e-24 with no coupling (ProjectionP_Mod.f90:227):
+    // it is decoupled, its value is b / D, set by the scatter; the solver
+    // sees it as a solid row
+    if (decoupled_row(o)) b = 0.0;
     // symmetric storage: the E/N entries of this row are the face couplings
-    const double dd = o.fluid ? o.D : 0.0;
+    const double dd = (o.fluid && !decoupled_row(o)) ? o.D : 0.0;
     const double ce = -o.E, cn = -o.N;
     diag[p] = dd;
     cE[p] = ce;
@@ -179,12 +188,26 @@ __global__ void __launch_bounds__(256) k_build(CoefIn cin, RhsIn rin, int bcW, i
     }
 }
 
-__global__ void __launch_bounds__(256) k_rhs(RhsIn rin, Dims D, double* __restrict__ rhs) {
+// right-hand side on the rows of the built operator (decoupled rows, whose
+// diagonal is stored as 0, keep b = 0 as in k_build)
+__global__ void __launch_bounds__(256) k_rhs(RhsIn rin, Dims D, const double* __restrict__ diag,
+                                             double* __restrict__ rhs) {
     const int ic = blockIdx.x * TX + threadIdx.x;
     const int jc = blockIdx.y * TY + threadIdx.y;
     if (ic >= D.ni || jc >= D.nj) return;
     const bool fluid = rin.vofS[rin.L.c(ic, jc)] < 1.0 - EPSI;
-    rhs[D.off(ic, jc)] = fluid ? cell_rhs(rin, ic, jc) : 0.0;
+    const size_t p = D.off(ic, jc);
+    rhs[p] = (fluid && diag[p] != 0.0) ? cell_rhs(rin, ic, jc) : 0.0;
+}
+
+// SetPoissonVectors' b of every fluid row in the cell layout, decoupled
+// rows included (the reference's vector; the solver's copy keeps 0 there)
+__global__ void __launch_bounds__(256) k_rhs_cells(RhsIn rin, double* __restrict__ out) {
+    const int ic = blockIdx.x * TX + threadIdx.x;
+    const int jc = blockIdx.y * TY + threadIdx.y;
+    if (ic >= rin.L.ni || jc >= rin.L.nj) return;
+    const size_t k = rin.L.c(ic, jc);
+    out[k] = rin.vofS[k] < 1.0 - EPSI ? cell_rhs(rin, ic, jc) : 0.0;
 }
 
 __global__ void __launch_bounds__(256) k_rows(CoefIn cin, int bcW, int bcE, int bcS, int bcN, double* __restrict__ matr) {
@@ -214,15 +237,23 @@ __global__ void __launch_bounds__(256) k_apply5(Dims D, const double* __restrict
 
 // DeltaPressureGetValues + DeltaPressureBoundaryCondition
 // (ProjectionP_Mod.f90:377-391,582-595) on the ghosted Pp.
+// A decoupled fluid row (all faces closed) takes the exact solution of its
+// own equation, b / D.
 __global__ void __launch_bounds__(256) k_scatter(Dims D, FLay L, const double* __restrict__ x,
-                                                 const double* __restrict__ vofS, int bcW, int bcE, int bcS,
-                                                 int bcN, double* __restrict__ Pp) {
+                                                 const double* __restrict__ vofS, const double* __restrict__ diag,
+                                                 CoefIn cin, RhsIn rin, int bcW, int bcE, int bcS, int bcN,
+                                                 double* __restrict__ Pp) {
+#pragma clang fp contract(off)
     const int gi = blockIdx.x * TX + threadIdx.x;
     const int gj = blockIdx.y * TY + threadIdx.y;
     const int ni = D.ni, nj = D.nj;
     if (gi > ni + 1 || gj > nj + 1) return;
     auto val = [&](int ic, int jc) -> double {
-        return vofS[L.c(ic, jc)] < 1.0 - EPSI ? x[D.off(ic, jc)] : 0.0;
+        if (!(vofS[L.c(ic, jc)] < 1.0 - EPSI)) return 0.0;
+        const size_t p = D.off(ic, jc);
+        if (diag[p] != 0.0) return x[p];
+        const RowOut o = cell_row(cin, ic, jc, bcW, bcE, bcS, bcN);
+        return decoupled_row(o) ? cell_rhs(rin, ic, jc) / o.D : x[p];
     };
     double out = 0.0;
     const bool in_i = gi >= 1 && gi <= ni, in_j = gj >= 1 && gj <= nj;
@@ -387,7 +418,14 @@ int poisson_build(Ctx* c) {
 }
 
 int poisson_rhs_only(Ctx* c) {
-    hipLaunchKernelGGL(k_rhs, grid2d(c->ni, c->nj), block2d(), 0, c->stream, make_rhs_in(c), c->lev[0].d, c->rhs);
+    hipLaunchKernelGGL(k_rhs, grid2d(c->ni, c->nj), block2d(), 0, c->stream, make_rhs_in(c), c->lev[0].d,
+                       (const double*)c->lev[0].A.diag, c->rhs);
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+int poisson_rhs_cells(Ctx* c, double* d_out) {
+    hipLaunchKernelGGL(k_rhs_cells, grid2d(c->ni, c->nj), block2d(), 0, c->stream, make_rhs_in(c), d_out);
     SG_HIP(hipGetLastError());
     return SG_OK;
 }
@@ -409,8 +447,9 @@ int poisson_apply_fine(Ctx* c, const double* x, double* y) {
 
 int poisson_scatter(Ctx* c) {
     hipLaunchKernelGGL(k_scatter, grid2d(c->ni + 2, c->nj + 2), block2d(), 0, c->stream, c->lev[0].d,
-                       FLay{c->ni, c->nj}, c->x, (const double*)c->field[SG_F_VOFS], c->prm.bc_w, c->prm.bc_e,
-                       c->prm.bc_s, c->prm.bc_n, (double*)c->field[SG_F_PP]);
+                       FLay{c->ni, c->nj}, c->x, (const double*)c->field[SG_F_VOFS],
+                       (const double*)c->lev[0].A.diag, make_coef_in(c), make_rhs_in(c), c->prm.bc_w,
+                       c->prm.bc_e, c->prm.bc_s, c->prm.bc_n, (double*)c->field[SG_F_PP]);
     SG_HIP(hipGetLastError());
     return SG_OK;
 }

@@ -803,10 +803,18 @@ __device__ bool mix_source(const PredG& G, const Comp& K, const double* f, int s
             Vt[k] = 1.0;
         }
     }
+    // a fully solid target (Vt = 0) or a source with no fluid target (all
+    // Be = 0) is 0/0 in the reference's expression: its share is 0 here and
+    // the target is skipped; other cases are the reference's expression
     const double temp = (Be[0] + Be[1]) + Be[2];
     for (int k = 0; k < 3; ++k) {
-        Be[k] = Be[k] / temp;
-        Fm[k] = Fm[k] * Be[k] / (Be[k] * (1.0 - vs) + Vt[k]);
+        if (temp > 0.0 && Vt[k] > 0.0) {
+            Be[k] = Be[k] / temp;
+            Fm[k] = Fm[k] * Be[k] / (Be[k] * (1.0 - vs) + Vt[k]);
+        } else {
+            Fm[k] = 0.0;
+            ok[k] = false;
+        }
     }
     return true;
 }

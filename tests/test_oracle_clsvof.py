@@ -13,7 +13,9 @@ checks that do not restate its formulas:
 * the sweeps: a straight interface translated by a uniform velocity stays
   exact to round-off over 18 sub-steps; states uniform along a divergent
   flow stay put (the implicit / explicit divergence terms);
-  a discretely divergence-free vortex keeps the liquid volume.
+  a discretely divergence-free vortex keeps the liquid volume;
+* the level-set face value of the sweeps: exact translation of a linear
+  profile, first-order upwind differences on a quadratic one.
 
 What none of these fixes (the quality-based stencil choice of
 Normal_Vector_Irre, the stale normals on the domain edge, the tie order of
@@ -351,3 +353,68 @@ def test_interface_to_uv_solid_bounds(oracle_lib):
         assert np.abs(tv[deep] - (1.0 - st.vofS[deep])).max() < 1e-12
         dry = case.itf.phi > 3 * h
         assert dry.sum() > 50 and np.all(tv[dry] == 0.0)
+
+
+# ---- the level-set face flux of the sweeps ----------------------------------
+@pytest.mark.parametrize("ue", [0.8, -0.6])
+@pytest.mark.parametrize("profile", ["linear", "quadratic"])
+def test_level_set_sweep_flux(oracle_lib, ue, profile):
+    """X_Sweep's level-set face value is the average of the upwind cell's two
+    neighbours (Clsvof_Mod.f90:1073-1087).  Between interior faces its
+    differences equal those of the upwind cell values whenever the second
+    difference is constant, so for a quadratic profile the update is exactly
+    first-order upwind, phi - u dt/dx (phi_i - phi_i-1) (u > 0), and for a
+    linear one an exact translation, phi - u dt b.  A central face average
+    (phi_i + phi_i+1)/2 would fail the quadratic case by u dt dx."""
+    O = oracle_lib
+    ni, nj = 24, 6
+    g = C.uniform_grid(ni, nj, 1.2, 0.3)
+    geo = C.build_geometry(g, None)
+    cg = O.ClsGeom(g, geo)
+    X, _ = np.meshgrid(g.xc, g.yc)
+    phi = np.ascontiguousarray(0.3 + 1.7 * X if profile == "linear" else (X - 0.35) ** 2)
+    vof = np.ones(g.shape)
+    nx, ny, dis = np.zeros(g.shape), np.ones(g.shape), np.full(g.shape, -1.0)
+    uearr = np.full((nj, ni + 1), ue)
+    dtv = 0.2 * g.dx[0] / abs(ue)
+    _, tl = O.x_sweep(cg, vof, phi, nx, ny, dis, uearr, dtv, 1.0e30, vof.copy(), phi.copy())
+    h = g.dx[0]
+    body = (slice(None), slice(2, ni - 2))
+    if profile == "linear":
+        want = phi - ue * dtv * 1.7
+    else:
+        up = np.zeros_like(phi)
+        if ue > 0:
+            up[:, 1:] = phi[:, 1:] - phi[:, :-1]
+        else:
+            up[:, :-1] = phi[:, 1:] - phi[:, :-1]
+        want = phi - ue * dtv / h * up
+        central = np.zeros_like(phi)
+        central[:, 1:-1] = 0.5 * (phi[:, 2:] - phi[:, :-2])
+        assert np.abs((phi - ue * dtv / h * central) - want)[body].min() > 0.1 * abs(ue) * dtv * h
+    assert np.abs(tl - want)[body].max() < 1e-14
+
+
+@pytest.mark.parametrize("ve", [0.7, -0.5])
+def test_level_set_sweep_flux_y(oracle_lib, ve):
+    """Y_Sweep's level-set face value (Clsvof_Mod.f90:1195-1210), the same
+    upwind-neighbour average along y: first-order upwind on a quadratic
+    profile between interior faces."""
+    O = oracle_lib
+    ni, nj = 6, 24
+    g = C.uniform_grid(ni, nj, 0.3, 1.2)
+    cg = O.ClsGeom(g, C.build_geometry(g, None))
+    _, Y = np.meshgrid(g.xc, g.yc)
+    phi = np.ascontiguousarray((Y - 0.55) ** 2 - 0.01)
+    vof = np.ones(g.shape)
+    nx, ny, dis = np.zeros(g.shape), np.ones(g.shape), np.full(g.shape, -1.0)
+    vearr = np.full((nj + 1, ni), ve)
+    dtv = 0.2 * g.dy[0] / abs(ve)
+    _, tl = O.y_sweep(cg, vof, phi, nx, ny, dis, vearr, dtv, vof.copy(), phi.copy())
+    up = np.zeros_like(phi)
+    if ve > 0:
+        up[1:, :] = phi[1:, :] - phi[:-1, :]
+    else:
+        up[:-1, :] = phi[1:, :] - phi[:-1, :]
+    want = phi - ve * dtv / g.dy[0] * up
+    assert np.abs(tl - want)[2:nj - 2, :].max() < 1e-14

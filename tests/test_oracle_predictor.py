@@ -13,9 +13,14 @@ by closed forms that do not restate its formulas:
 * the two-phase, cut-cell and moving-wall paths are exercised for finite,
   deterministic output.
 
-The MUSCL/kinetic flux details, the mixing weights and the cut-cell
-interpolation branches are parity-unpinned: the oracle follows the source
-line by line there.
+and piece by piece: the superbee limiter (Sweby's table, TVD region,
+symmetry), the MUSCL / kinetic face flux on linear profiles, the mixing of a
+small cell (one target: the volume-weighted mean; several: conservation of
+sum (1 - vofS) flux; the reference's overwrite of a small target).
+
+Parity-unpinned (the oracle follows the source line by line): the one-sided
+first/last face fluxes, the Beta weights of several targets, and the
+near-solid interpolation branches.
 """
 import numpy as np
 import pytest
@@ -115,3 +120,181 @@ def test_cut_cell_paths_finite_and_deterministic(oracle_lib, name, mk):
     for k in ("matr_u", "rhs_u", "matr_v", "rhs_v", "DpU", "DpV", "fluxn1", "upred", "vpred"):
         assert np.all(np.isfinite(a[k])), k
         assert np.array_equal(a[k], b[k]), k
+
+
+# ---- the pieces the end-to-end checks above do not reach ------------------------
+def test_superbee_limiter_known_values(oracle_lib):
+    """MUSCLLimiter option 2 (PredictorUV_mod.f90:2354-2367) on a uniform
+    grid (tolim = 2) is Roe's superbee: Sweby's table, zero for r <= 0,
+    inside the second-order TVD region min(r, 1) <= phi <= min(2r, 2) (r > 0)
+    and symmetric, phi(r) / r = phi(1 / r)."""
+    O = oracle_lib
+    table = {-1.0: 0.0, 0.0: 0.0, 0.25: 0.5, 0.5: 1.0, 0.75: 1.0, 1.0: 1.0, 1.5: 1.5, 2.0: 2.0, 3.0: 2.0,
+             10.0: 2.0}
+    for r, phi in table.items():
+        assert O.muscl_limiter(r, 2.0) == phi, r
+    rng = np.random.default_rng(0)
+    for r in rng.uniform(1e-3, 20.0, 400):
+        f = O.muscl_limiter(r, 2.0)
+        assert min(r, 1.0) - 1e-15 <= f <= min(2.0 * r, 2.0) + 1e-15
+        assert f / r == pytest.approx(O.muscl_limiter(1.0 / r, 2.0), rel=1e-14)
+
+
+def _stream(ni=24, nj=20):
+    g, geo, U, V, pcell, uv = _single_phase(ni, nj)
+    return g, U, V, pcell, uv
+
+
+def test_muscl_flux_exact_on_linear_profiles(oracle_lib):
+    """On a linear profile the slope ratio is 1, the limiter 1, and the
+    reconstruction reaches the face exactly from both sides, so the
+    kinetic flux 0.5 (ur^2 + ul^2 - |a| (ur - ul)) is u_face^2 per unit
+    face (PredictorUV_mod.f90:1186-1305): the U momentum flux through the
+    faces between U cells, and the V momentum flux v_face * a carried by a
+    uniform u."""
+    O = oracle_lib
+    g, U, V, pcell, uv = _stream()
+    phys = P.phys_of(C.Fluids())
+    xu = np.concatenate([[g.xc[0] - 0.5 * g.dx[0] - g.dx[0]], g.xc + 0.5 * g.dx, [g.xc[-1] + 1.5 * g.dx[-1]]])
+    ug = np.ascontiguousarray(np.broadcast_to(0.3 + 1.7 * xu[None, :], (g.nj + 2, g.ni + 2)))
+    xv = np.concatenate([[g.xc[0] - g.dx[0]], g.xc, [g.xc[-1] + g.dx[-1]]])
+    vg = np.ascontiguousarray(np.broadcast_to(-0.2 + 0.9 * xv[None, :], (g.nj + 2, g.ni + 2)))
+    ug2 = np.full_like(ug, 0.45)
+    F1, _ = O.conv_fluxes(g, U, V, uv, uv, pcell, ug, vg, phys, 0.01)
+    F1b, _ = O.conv_fluxes(g, U, V, uv, uv, pcell, ug2, vg, phys, 0.01)
+    for i in range(3, g.ni):         # faces away from the one-sided first/last
+        for j in range(1, g.nj + 1):
+            uf = 0.5 * (ug[j, i - 1] + ug[j, i])
+            assert F1[0, j, i] == pytest.approx(uf * uf * g.dy[j - 1], rel=1e-13)
+            vf = 0.5 * (vg[j, i - 1] + vg[j, i])
+            assert F1b[1, j, i] == pytest.approx(0.45 * vf * g.vdy[j - 1], rel=1e-13)
+    # a first-order upwind face value would not be exact: the check has teeth
+    uf_up = ug[5, 6]
+    assert abs(F1[0, 5, 7] - uf_up * uf_up * g.dy[4]) > 1e-3 * abs(F1[0, 5, 7])
+
+
+def _mixing_setup(n=16):
+    g, geo, U, V, pcell, uv = _single_phase(n, n)
+    rng = np.random.default_rng(3)
+    vofS = np.zeros(g.shape)
+    nxS = np.zeros(g.shape)
+    nyS = np.ones(g.shape)
+    act = np.ones(g.shape, np.int32)
+    act[:, -1] = 0
+    return g, U, V, pcell, vofS, nxS, nyS, act, rng
+
+
+def _with(st, **kw):
+    import dataclasses
+    return dataclasses.replace(st, **{k: np.ascontiguousarray(v) for k, v in kw.items()})
+
+
+def test_mixing_single_target_is_volume_weighted_mean(oracle_lib):
+    """A small cell (vofS >= 0.5 + epsi) whose solid normal points along x
+    mixes with that one neighbour only: both end at the volume-weighted mean
+    ((1 - s) f_s + V f_t) / ((1 - s) + V) (Mixing_Procedure,
+    PredictorUV_mod.f90:2127-2197)."""
+    O = oracle_lib
+    g, U, V, pcell, vofS, nxS, nyS, act, rng = _mixing_setup()
+    j, i = 7, 5
+    vofS[j, i] = 0.8
+    vofS[j, i - 1] = 0.3
+    nxS[j, i], nyS[j, i] = -1.0, 0.0
+    Um = _with(U, vofS=vofS, nxS=nxS, nyS=nyS, active=act)
+    f = rng.standard_normal(g.shape)
+    out = O.mixing(g, Um, V, 0, f, pcell, P.phys_of(C.Fluids()))
+    s, Vt = 0.8, 0.7
+    mean = ((1 - s) * f[j, i] + Vt * f[j, i - 1]) / ((1 - s) + Vt)
+    assert out[j, i] == pytest.approx(mean, rel=1e-14)
+    assert out[j, i - 1] == pytest.approx(mean, rel=1e-14)
+    other = np.ones(g.shape, bool)
+    other[j, i] = other[j, i - 1] = False
+    assert np.array_equal(out[other], f[other])
+
+
+@pytest.mark.parametrize("comp", [0, 1])
+def test_mixing_conserves_volume_weighted_sum(oracle_lib, comp):
+    """Mixing_Procedure moves (1 - s) Fmix out of each small cell and
+    Vtgt * (Fmix / Vtgt) into its targets, so the sum of (1 - vofS) * flux
+    over the cells it works on is unchanged -- here with several small cells
+    and oblique solid normals, two of them sharing targets (no small cell is
+    another's target: see the next test)."""
+    O = oracle_lib
+    g, U, V, pcell, vofS, nxS, nyS, act, rng = _mixing_setup(20)
+    if comp == 1:
+        act = np.ones(g.shape, np.int32)
+        act[-1, :] = 0
+    cells = [(5, 5), (5, 7), (9, 12), (14, 3), (10, 16)]
+    for (j, i) in cells:
+        vofS[j, i] = rng.uniform(0.55, 0.95)
+    for (j, i) in cells:
+        th = rng.uniform(0, 2 * np.pi)
+        nxS[j, i], nyS[j, i] = np.cos(th), np.sin(th)
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                if (j + dj, i + di) not in cells:
+                    vofS[j + dj, i + di] = max(vofS[j + dj, i + di], rng.uniform(0.0, 0.4))
+    # the first two point at the cell between them
+    nxS[5, 5], nyS[5, 5] = 0.8, 0.6
+    nxS[5, 7], nyS[5, 7] = -0.8, 0.6
+    kw = dict(vofS=vofS, nxS=nxS, nyS=nyS, active=act)
+    Um, Vm = (_with(U, **kw), V) if comp == 0 else (U, _with(V, **kw))
+    f = rng.standard_normal(g.shape)
+    out = O.mixing(g, Um, Vm, comp, f, pcell, P.phys_of(C.Fluids()))
+    w = 1.0 - vofS
+    rng_ = (slice(None), slice(0, -1)) if comp == 0 else (slice(0, -1), slice(None))
+    assert (w * out)[rng_].sum() == pytest.approx((w * f)[rng_].sum(), rel=1e-13, abs=1e-13)
+    assert not np.array_equal(out, f)
+
+
+def test_mixing_overwrites_a_small_target(oracle_lib):
+    """The reference sets each small cell's mixed value from the unmixed
+    copy (:2165), so what an earlier small cell moved into it is dropped:
+    with A (visited first) mixing into its small neighbour B, the volume-
+    weighted sum grows by exactly A's Fmix = (1 - s_A) V_B (f_B - f_A) /
+    ((1 - s_A) + V_B)."""
+    O = oracle_lib
+    g, U, V, pcell, vofS, nxS, nyS, act, rng = _mixing_setup()
+    j = 7
+    vofS[j, 4], vofS[j, 5] = 0.8, 0.7      # A = (4, 7), B = (5, 7): B is A's target
+    nxS[j, 4], nyS[j, 4] = 1.0, 0.0
+    nxS[j, 5], nyS[j, 5] = 1.0, 0.0       # B mixes with (6, 7)
+    Um = _with(U, vofS=vofS, nxS=nxS, nyS=nyS, active=act)
+    f = rng.standard_normal(g.shape)
+    out = O.mixing(g, Um, V, 0, f, pcell, P.phys_of(C.Fluids()))
+    w = 1.0 - vofS
+    sA, VB = 0.8, 0.3
+    fmA = (1 - sA) * VB * (f[j, 5] - f[j, 4]) / ((1 - sA) + VB)
+    body = (slice(None), slice(0, -1))
+    assert (w * out)[body].sum() - (w * f)[body].sum() == pytest.approx(fmA, rel=1e-10)
+
+
+def test_mixing_fully_solid_target_takes_no_share(oracle_lib):
+    """A small cell beside a vertical wall (solid normal (-1, +0): the sign of
+    +0 picks the north target) under a fully solid cell: the reference's
+    Fmix * Beta / (Beta (1 - s) + Vtgt) is 0/0 for that target (Vtgt = 0,
+    Beta = 0).  The oracle takes the limit 0 there, so the cell mixes with
+    its west neighbour alone (the volume-weighted mean) and the solid cell's
+    flux is left as it was; with every target solid the cell keeps its own
+    flux."""
+    O = oracle_lib
+    g, U, V, pcell, vofS, nxS, nyS, act, rng = _mixing_setup()
+    j, i = 7, 5
+    vofS[j, i], vofS[j, i - 1], vofS[j + 1, i] = 0.8, 0.3, 1.0
+    nxS[j, i], nyS[j, i] = -1.0, 0.0
+    act[j + 1, i] = 0
+    f = rng.standard_normal(g.shape)
+    out = O.mixing(g, _with(U, vofS=vofS, nxS=nxS, nyS=nyS, active=act), V, 0, f, pcell, P.phys_of(C.Fluids()))
+    assert np.isfinite(out).all()
+    s, Vt = 0.8, 0.7
+    mean = ((1 - s) * f[j, i] + Vt * f[j, i - 1]) / ((1 - s) + Vt)
+    assert out[j, i] == pytest.approx(mean, rel=1e-14)
+    assert out[j, i - 1] == pytest.approx(mean, rel=1e-14)
+    other = np.ones(g.shape, bool)
+    other[j, i] = other[j, i - 1] = False
+    assert np.array_equal(out[other], f[other])
+    # every target solid: no fluid to mix with, the flux stays
+    vofS[j, i - 1] = vofS[j + 1, i - 1] = 1.0
+    act[j, i - 1] = act[j + 1, i - 1] = 0
+    out = O.mixing(g, _with(U, vofS=vofS, nxS=nxS, nyS=nyS, active=act), V, 0, f, pcell, P.phys_of(C.Fluids()))
+    assert np.array_equal(out, f)
