@@ -258,9 +258,9 @@ int sg_field_copy(sg_ctx* ctx, int dst, int src);
  * restricted right-hand side / its rms, the Galerkin couplings E, N, NE, NW
  * / the diagonal, the liquid fraction of the fine block, y and x in [0,1].
  * Output: the level-L pressure / (out_scale * rms).  PCG starts from the
- * prolongated, Galerkin-scaled prediction w and (deflate = 1, FP64
- * precision only) keeps its directions A-orthogonal to w; the stopping test
- * is unchanged. */
+ * prolongated, Galerkin-scaled prediction w and (deflate = 1, FP64 and
+ * mixed precision) keeps its directions A-orthogonal to w; the stopping
+ * test is unchanged. */
 typedef struct sg_nn sg_nn;
 size_t sg_nn_num_weights(void);
 /* He-normal initial weights (host array of sg_nn_num_weights floats) */

@@ -563,11 +563,11 @@ template <typename T>
 struct TailArgs {
     int n;  // number of tail levels (the last is the coarsest)
     TailLevel<T> L[MAX_TAIL];
-    const T* inv;  // dense inverse of the coarsest operator
+    const double* inv;  // dense inverse of the coarsest operator (FP64 in both precisions)
     const int* on; // the PCG's convergence flag (the cycle runs while *on), or null
     int krylov;    // inner flexible-CG iterations on the first tail level (0: one V-cycle)
     int pbase;     // LDS offset of the inner iteration's direction vector
-    int invbase;   // LDS offset of the staged dense inverse
+    int invbase;   // LDS offset (in T) of the staged FP64 dense inverse, 8-byte aligned
 };
 
 // offset of cell (i,j) in a level's LDS arrays (compact layout, 32-bit)
@@ -701,21 +701,21 @@ __device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
         __syncthreads();
     }
     {
-        // dense solve on the coarsest level: four threads per row, the
-        // inverse staged in LDS
+        // dense solve on the coarsest level in FP64 (in both precisions):
+        // four threads per row, the inverse staged in LDS
         const TailLevel<T>& L = a.L[last];
         const Dims& D = L.cd;
         const int n = D.ni * D.nj;
         const T* b = tarr(lds, L, TA_B);
-        const T* inv = lds + a.invbase;
+        const double* inv = reinterpret_cast<const double*>(lds + a.invbase);
         T* x = tarr(lds, L, TA_R);
         const int row = tid >> 2, quarter = tid & 3;
-        T s = T(0);
+        double s = 0.0;
         if (row < n)
-            for (int q = quarter; q < n; q += 4) s += inv[row * n + q] * b[D.off(q % D.ni, q / D.ni)];
+            for (int q = quarter; q < n; q += 4) s += inv[row * n + q] * (double)b[D.off(q % D.ni, q / D.ni)];
         s += __shfl_xor(s, 1, 64);
         s += __shfl_xor(s, 2, 64);
-        if (row < n && quarter == 0) x[D.off(row % D.ni, row / D.ni)] = s;
+        if (row < n && quarter == 0) x[D.off(row % D.ni, row / D.ni)] = (T)s;
         __syncthreads();
     }
     for (int l = last - 1; l >= 0; --l) {
@@ -788,7 +788,8 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
     {
         const Dims& Dc = a.L[a.n - 1].d;
         const int nc = Dc.ni * Dc.nj;
-        for (int k = tid; k < nc * nc; k += TAIL_THREADS) lds[a.invbase + k] = a.inv[k];
+        double* inv = reinterpret_cast<double*>(lds + a.invbase);
+        for (int k = tid; k < nc * nc; k += TAIL_THREADS) inv[k] = a.inv[k];
     }
     __syncthreads();
     const TailLevel<T>& L = a.L[0];
@@ -856,16 +857,17 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
     if (own) L.x[L.d.off(tid % D.ni, tid / D.ni)] = xacc;
 }
 
-// coarsest-only hierarchy (grids of at most COARSEST_MAX cells)
-template <typename T, typename TB, typename TZ>
-__global__ void __launch_bounds__(256) k_direct(Dims C, const T* __restrict__ inv, const TB* __restrict__ b,
+// coarsest-only hierarchy (grids of at most COARSEST_MAX cells): the dense
+// FP64 solve
+template <typename TB, typename TZ>
+__global__ void __launch_bounds__(256) k_direct(Dims C, const double* __restrict__ inv, const TB* __restrict__ b,
                                                 TZ* __restrict__ x, const int* on) {
     if (on && !*on) return;
     const int n = C.ni * C.nj;
     const int r = threadIdx.x;
     if (r >= n) return;
-    T s = T(0);
-    for (int q = 0; q < n; ++q) s += inv[(size_t)r * n + q] * (T)b[C.off(q % C.ni, q / C.ni)];
+    double s = 0.0;
+    for (int q = 0; q < n; ++q) s += inv[(size_t)r * n + q] * (double)b[C.off(q % C.ni, q / C.ni)];
     x[C.off(r % C.ni, r / C.ni)] = (TZ)s;
 }
 
@@ -876,14 +878,12 @@ struct Acc<double> {
     static const Op<double>& op(const Level& L) { return L.A; }
     static const Prol<double>& pr(const Level& L) { return L.P; }
     static const LevelVecs<double>& vec(const Level& L) { return L.v; }
-    static const double* inv(const Ctx* c) { return c->coarse_inv; }
 };
 template <>
 struct Acc<float> {
     static const Op<float>& op(const Level& L) { return L.Af; }
     static const Prol<float>& pr(const Level& L) { return L.Pf; }
     static const LevelVecs<float>& vec(const Level& L) { return L.vf; }
-    static const float* inv(const Ctx* c) { return c->coarse_inv_f; }
 };
 
 inline dim3 tiles(const Dims& d, int tw, int th) { return dim3((d.ni + tw - 1) / tw, (d.nj + th - 1) / th); }
@@ -908,7 +908,7 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done, const 
     *dots_done = 0;
     if (nl == 1) {
         const Level& L = c->lev[0];
-        hipLaunchKernelGGL((k_direct<T, V, V>), dim3(1), dim3(256), 0, s, L.d, Acc<T>::inv(c), r, z, on);
+        hipLaunchKernelGGL((k_direct<V, V>), dim3(1), dim3(256), 0, s, L.d, (const double*)c->coarse_inv, r, z, on);
         SG_HIP(hipGetLastError());
         return SG_OK;
     }
@@ -966,14 +966,17 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done, const 
         }
         off += (int)compact(c->lev[lt].d).alloc_elems();
     }
+    // the FP64 inverse: 8-byte aligned, sizeof(double) / sizeof(T) slots per entry
+    constexpr int per = (int)(sizeof(double) / sizeof(T));
+    off = (off + per - 1) / per * per;
     ta.invbase = off;
-    off += (int)(c->lev[nl - 1].d.ni * c->lev[nl - 1].d.nj) * (int)(c->lev[nl - 1].d.ni * c->lev[nl - 1].d.nj);
+    off += per * (int)(c->lev[nl - 1].d.ni * c->lev[nl - 1].d.nj) * (int)(c->lev[nl - 1].d.ni * c->lev[nl - 1].d.nj);
     const size_t lds_bytes = (size_t)off * sizeof(T);
     if (lds_bytes > TAIL_LDS_MAX) {
         set_error("multigrid: tail levels exceed the LDS budget");
         return SG_ERR_STATE;
     }
-    ta.inv = Acc<T>::inv(c);
+    ta.inv = c->coarse_inv;
     ta.on = on;
     hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(TAIL_THREADS), lds_bytes, s, ta, off);
     // up: level l reads x_l from the down pass and e_{l+1}; writes its own

@@ -15,6 +15,20 @@
 // device, inside the captured iterations, so the host only tests
 // convergence -- after a number of iterations it sets from the observed
 // rate; iterations launched past convergence cost one flag test per kernel.
+//
+// Deflation (a learned vector w): every new direction p = z + beta p_old -
+// mu w is re-projected A-orthogonal to w after q = A p is formed, p' = p -
+// gamma w, q' = q - gamma A w with gamma = (w, A p) / (w, A w); the
+// correction is folded into the update (x += alpha p', r -= alpha q') and
+// into the next direction (its w coefficient mu + beta gamma), so an FP32
+// direction cannot drift away from A-orthogonality over the iterations.
+// gamma takes A p as the FP64 stencil sums, before q is rounded to the
+// vector precision: then (A w, p') vanishes to FP64 rounding, x gains no
+// component along w, and the FP64 residual recomputed from x stays
+// orthogonal to w.  (With the rounded q, (w, A p') is of the order of the
+// FP32 rounding of q at every step; it accumulates in x, no A-orthogonal
+// direction can remove it, and the recomputed residual then carries a w
+// part the FP32 recursion stalls and diverges on.)
 #include "sg_internal.hpp"
 #include "sg_reduce.hpp"
 
@@ -25,7 +39,9 @@ namespace {
 constexpr int NT = TX * TY;
 constexpr int NTF = 1024;  // threads of the finalize kernels
 
-enum FinMode { FIN_INIT = 0, FIN_REPLACE, FIN_RZ, FIN_PQ, FIN_RR, FIN_TRUE, FIN_MU, FIN_RELIABLE, FIN_RELIABLE_IF };
+enum FinMode {
+    FIN_INIT = 0, FIN_REPLACE, FIN_RZ, FIN_PQ, FIN_RR, FIN_TRUE, FIN_MU, FIN_RELIABLE, FIN_RELIABLE_IF, FIN_PQ_DEFL
+};
 
 // mixed precision: the FP32 residual is recomputed from x in FP64 once it
 // has fallen by this factor (in (r,r)) since the last recomputation
@@ -123,7 +139,10 @@ __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restr
     constexpr int RW = DW + 2, RH = DH + 2;
     __shared__ V sp[RH][RW];
     const V beta = (V)sc->beta;
-    const V mu = w ? (V)sc->mu : V(0);
+    const V mu = w ? (V)sc->mu_eff : V(0);
+    // the first direction reads no old one: p_old still holds the last
+    // solve's, which may not be finite after a failed solve (0 * NaN)
+    const bool first = sc->first != 0;
     int tx, ty;
     xcd_tile(tx, ty);
     const int i0 = tx * DW, j0 = ty * DH;
@@ -134,12 +153,13 @@ __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restr
         V v = V(0);
         if (gi >= 0 && gj >= 0 && gi < D.ni && gj < D.nj) {
             const size_t p = D.off(gi, gj);
-            v = w ? z[p] + beta * pold[p] - mu * (V)w[p] : z[p] + beta * pold[p];
+            const V zb = first ? z[p] : z[p] + beta * pold[p];
+            v = w ? zb - mu * (V)w[p] : zb;
         }
         sp[lj][li] = v;
     }
     __syncthreads();
-    double v[1] = {0.0};
+    double v[2] = {0.0, 0.0};
     const size_t ld = D.ld;
 #pragma unroll
     for (int m = 0; m < DW * DH / 256; ++m) {
@@ -156,13 +176,16 @@ __global__ void __launch_bounds__(256) k_dir_apply(Dims D, const double* __restr
         pnew[p] = pc;
         q[p] = qv;
         v[0] += (double)pc * (double)qv;
+        if (w) v[1] += w[p] * qd;  // (w, A p) before rounding
     }
-    block_partials<1, 256>(v, S.part, S.max_blocks);
+    block_partials<2, 256>(v, S.part, S.max_blocks);
 }
 
 // x += alpha p (x in FP64) ; r -= alpha q ; partials of (r,r)
+// with deflation (w, aw = A w): the re-projected p - gamma w, q - gamma A w
 template <typename V>
 __global__ void __launch_bounds__(256) k_update(Dims D, const V* __restrict__ p, const V* __restrict__ q,
+                                                const double* __restrict__ w, const double* __restrict__ aw,
                                                 double* __restrict__ x, V* __restrict__ r, Slots S,
                                                 const PcgScalars* sc) {
     if (!sc->active) return;
@@ -173,10 +196,18 @@ __global__ void __launch_bounds__(256) k_update(Dims D, const V* __restrict__ p,
     double v[1] = {0.0};
     if (i < D.ni && j < D.nj) {
         const size_t k = D.off(i, j);
-        x[k] += alpha * (double)p[k];
-        const V rn = r[k] - alv * q[k];
-        r[k] = rn;
-        v[0] = (double)rn * (double)rn;
+        if (w) {
+            const double gm = sc->gamma;
+            x[k] += alpha * ((double)p[k] - gm * w[k]);
+            const V rn = (V)((double)r[k] - alpha * ((double)q[k] - gm * aw[k]));
+            r[k] = rn;
+            v[0] = (double)rn * (double)rn;
+        } else {
+            x[k] += alpha * (double)p[k];
+            const V rn = r[k] - alv * q[k];
+            r[k] = rn;
+            v[0] = (double)rn * (double)rn;
+        }
     }
     block_partials<1, NT>(v, S.part, S.max_blocks);
 }
@@ -271,19 +302,39 @@ __global__ void __launch_bounds__(NTF) k_finalize(Slots S, int nblocks, int mode
         sc->rr_ref = s[1];
         sc->first = 1;
         sc->beta = 0.0;
+        sc->gamma = 0.0;
         sc->active = (s[0] > 0.0 && s[1] > sc->tol2) ? 1 : 0;
         sc->rz = sc->rz_old = sc->zq = sc->pq = 0.0;
+        return;
+    }
+    if (mode == FIN_PQ || mode == FIN_PQ_DEFL) {
+        double s2[2];
+        finalize_sum<2, NTF>(S.part, S.max_blocks, nblocks, s2);
+        if (threadIdx.x != 0) return;
+        double pq = s2[0];
+        if (mode == FIN_PQ_DEFL) {
+            // re-projection of the direction A-orthogonal to w: (p', q') =
+            // (p, q) - gamma (w, A p)
+            const double gm = sc->wq > 0.0 ? s2[1] / sc->wq : 0.0;
+            sc->gamma = gm;
+            pq = s2[0] - gm * s2[1];
+        }
+        sc->pq = pq;
+        sc->alpha = sc->rz / pq;
+        sc->first = 0;
         return;
     }
     double s[1];
     finalize_sum<1, NTF>(S.part, S.max_blocks, nblocks, s);
     if (threadIdx.x != 0) return;
     if (mode == FIN_MU) {
+        // s[0] = (A w, z).  The last direction was used as p' = p - gamma w,
+        // q' = q - gamma A w: the flexible beta reads (z, q') = (z, q) -
+        // gamma (A w, z), and the new direction's w coefficient takes
+        // beta gamma beside mu.
         sc->mu = sc->wq > 0.0 ? s[0] / sc->wq : 0.0;
-    } else if (mode == FIN_PQ) {
-        sc->pq = s[0];
-        sc->alpha = sc->rz / s[0];
-        sc->first = 0;
+        if (!sc->first && sc->flexible) sc->beta = -sc->alpha * (sc->zq - sc->gamma * s[0]) / sc->rz_old;
+        sc->mu_eff = sc->mu + (sc->first ? 0.0 : sc->beta * sc->gamma);
     } else {  // FIN_RR
         sc->rr = s[0];
         sc->rz_old = sc->rz;
@@ -337,9 +388,10 @@ int enqueue_iteration_t(Ctx* c, int parity, V* r, V* z, V* q, V* p0, V* p1) {
     const dim3 gd((c->ni + DW - 1) / DW, (c->nj + DH - 1) / DH);
     hipLaunchKernelGGL(k_dir_apply<V>, gd, dim3(256), 0, s, D, (const double*)A.diag, (const double*)A.cE,
                        (const double*)A.cN, (const V*)z, (const V*)pold, w, pnew, q, S2, (const PcgScalars*)c->sc);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S2, (int)(gd.x * gd.y), (int)FIN_PQ, c->sc, 0.0);
-    hipLaunchKernelGGL(k_update<V>, g, b, 0, s, D, (const V*)pnew, (const V*)q, c->x, r, S3,
-                       (const PcgScalars*)c->sc);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S2, (int)(gd.x * gd.y), (int)(w ? FIN_PQ_DEFL : FIN_PQ),
+                       c->sc, 0.0);
+    hipLaunchKernelGGL(k_update<V>, g, b, 0, s, D, (const V*)pnew, (const V*)q, w, w ? (const double*)c->nn.aw : nullptr,
+                       c->x, r, S3, (const PcgScalars*)c->sc);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NTF), 0, s, S3, nb, (int)FIN_RR, c->sc, 0.0);
     if (sizeof(V) == 4 && parity == 1) {
         // FP32 vectors: recompute r from x in FP64 when it is due, keeping
@@ -484,13 +536,6 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
             set_error("warm_start = 2 or 3 needs the learned warm start (sg_set_learned)");
             return SG_ERR_ARG;
         }
-        if (deflating(c) && mixed(c)) {
-            // with FP32 directions the A-orthogonality to w that deflation
-            // relies on decays and the recursion diverges
-            set_error("the deflation vector needs FP64 PCG vectors (precision SG_PREC_FP64); use deflate = 0 "
-                      "in mixed precision");
-            return SG_ERR_ARG;
-        }
         rc = learned_warm_start(c, c->prm.warm_start == 3);
         if (rc) return rc;
         rc = enqueue_init(c, 1, FIN_INIT);
@@ -559,8 +604,13 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
     SG_HIP(hipEventElapsedTime(&ms_all, c->ev0, c->ev2));
     SG_HIP(hipEventElapsedTime(&ms_setup, c->ev0, c->ev1));
     int status = SG_OK;
-    if (c->sc_host->guard & 1) status = SG_NAN;
-    else if (c->sc_host->active) status = SG_NOT_CONVERGED;
+    if (c->sc_host->guard & 1) {
+        status = SG_NAN;
+        set_error("poisson solve: NaN in the PCG scalars");
+    } else if (c->sc_host->active) {
+        status = SG_NOT_CONVERGED;
+        set_error("poisson solve: not converged within max_iter");
+    }
     if (info) {
         info->iterations = c->sc_host->iters;
         info->relres = relres;

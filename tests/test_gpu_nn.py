@@ -120,8 +120,10 @@ def test_learned_warm_start_ablation():
     (relative residual 1e-8) with the learned warm start alone and with the
     deflation vector, and the share of the solution's energy x'Ax the
     Galerkin-scaled start captures.  Asserted: every variant converges to
-    the same pressure (1e-6 relative L2), and training raises the captured
-    energy above the untrained network's."""
+    the same pressure (1e-6 relative L2) -- the deflated solve in mixed
+    precision (FP32 directions re-projected A-orthogonal to w each
+    iteration) included -- and training raises the captured energy above the
+    untrained network's."""
     from simcoflow_amd import cases as C
     from simcoflow_amd import learned as Lr
     from simcoflow_amd.projection import Projection, params_for
@@ -137,7 +139,7 @@ def test_learned_warm_start_ablation():
     w0 = Lr.init_weights(0)
     gen = torch.Generator(device="cpu")
     gen.manual_seed(99)
-    res = {"plain": [], "untrained": [], "warm": [], "deflated": []}
+    res = {"plain": [], "untrained": [], "warm": [], "deflated": [], "deflated_mixed": []}
     energy = {"untrained": [], "warm": []}
     for s in (11, 12):
         case = mk(s)
@@ -155,11 +157,12 @@ def test_learned_warm_start_ablation():
                 _lib.check(pj.lib.sg_upload_device(pj.ctx, _lib.F["UPRED"], ctypes.c_void_p(u.data_ptr())))
                 _lib.check(pj.lib.sg_upload_device(pj.ctx, _lib.F["VPRED"], ctypes.c_void_p(v.data_ptr())))
                 out = {}
-                for mode, ws, wts, defl in (("plain", 0, None, False), ("untrained", 2, w0, False),
-                                            ("warm", 2, w, False), ("deflated", 2, w, True)):
+                for mode, ws, wts, defl, prec in (("plain", 0, None, False, 0), ("untrained", 2, w0, False, 0),
+                                                  ("warm", 2, w, False, 0), ("deflated", 2, w, True, 0),
+                                                  ("deflated_mixed", 2, w, True, 1)):
                     if ws == 2:
                         Lr.enable(pj, wts, level, K, deflate=defl)
-                    pj.set_params(params_for(case.fluids, tol=1e-8, max_iter=300, warm_start=ws))
+                    pj.set_params(params_for(case.fluids, tol=1e-8, max_iter=300, warm_start=ws, precision=prec))
                     pj.build(0.0)
                     info = pj.solve()
                     assert info.relres <= 1.05e-8, (mode, info.relres)
@@ -174,7 +177,7 @@ def test_learned_warm_start_ablation():
                           f"{info.warm_relres:.3e}, energy share {info.warm_energy / xb:.4f}, "
                           f"warm {info.warm_ms:.3f} ms")
                 ref = out["plain"]
-                for mode in ("untrained", "warm", "deflated"):
+                for mode in ("untrained", "warm", "deflated", "deflated_mixed"):
                     d = np.linalg.norm(out[mode] - ref) / np.linalg.norm(ref)
                     assert d < 1e-6, (mode, d)
         finally:
