@@ -64,6 +64,25 @@ class OracleStepper:
         self.itt = 0
         self.time = 0.0
 
+    # the stepper's state under the device's field names (ghosted u, v, p,
+    # predictor outputs, Flux_n1; cell-sized interface fields)
+    STATE = {"VOF": "vof", "PHI": "phi", "NX": "nx", "NY": "ny", "U": "u", "V": "v", "P": "p", "UPRED": "upred",
+             "VPRED": "vpred", "DPU": "DpU", "DPV": "DpV", "UVOF": "uvof", "VVOF": "vvof"}
+
+    def set_state(self, fields, itt, time):
+        """Continue from another run's state (fields keyed as STATE, plus
+        FLUXN1U / FLUXN1V): the next step() is that run's step itt + 1."""
+        for key, attr in self.STATE.items():
+            setattr(self, attr, np.array(fields[key], dtype=np.float64))
+        self.fluxn1 = np.stack([np.array(fields["FLUXN1U"], dtype=np.float64),
+                                np.array(fields["FLUXN1V"], dtype=np.float64)])
+        self.itt, self.time = int(itt), float(time)
+
+    def state(self):
+        out = {key: getattr(self, attr) for key, attr in self.STATE.items()}
+        out["FLUXN1U"], out["FLUXN1V"] = self.fluxn1[0], self.fluxn1[1]
+        return out
+
     def _to_uv(self, which):
         st = self.U if which == 0 else self.V
         return self.O.interface_to_uv(self.cg, which, self.vof, self.phi, self.nx, self.ny, st.vofS, st.phiS)[0]

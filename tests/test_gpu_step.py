@@ -140,3 +140,101 @@ def test_previous_pressure_start(precision):
     assert dvol < 1e-12
     assert dp < 1e-6
     assert sum(ib) < sum(ia)
+
+
+STATE_FIELDS = ("VOF", "PHI", "NX", "NY", "U", "V", "P", "UPRED", "VPRED", "DPU", "DPV", "FLUXN1U", "FLUXN1V",
+                "UVOF", "VVOF")
+# transport, U/V interface, Dp and Flux_n1: bit for bit from the same state
+EXACT_FIELDS = ("VOF", "PHI", "NX", "NY", "UVOF", "VVOF", "DPU", "DPV", "FLUXN1U", "FLUXN1V")
+
+
+def _c1_diagnostics(g, vof, vofS, gas_centroid):
+    area = g.dx[None, :] * g.dy[:, None]
+    cx, cy = gas_centroid(g, vof, vofS)
+    return float((vof * area).sum()), cx, cy, float(vof[0, :].sum())
+
+
+def test_dambreak_500_steps_match_oracle(oracle_lib):
+    """The 512 x 256 dam break with its rectangular boom, 500 FP64 steps to
+    a relative residual of 1e-10 (AdamBasforthCrankNicolson,
+    Solver_Mod.f90:196-242,246-319), against the oracle's own 500 steps
+    (fixtures of tests/golden/make_dambreak_c1.py).
+
+    * Step by step: at six points of the run the oracle takes one step from
+      the device's state; the transport, the U/V interface, Dp and Flux_n1
+      come out bit for bit, the predictor within 1e-12 and the pressure and
+      velocities within 1e-9 relative (the two PCG runs round differently).
+    * Trajectory: the first 200 steps agree to dt 1e-10, liquid volume
+      1e-12 relative, gas centroid and bottom-row liquid length 1e-9 cell.
+      The step holds discrete decisions (the sweeps' clipping, the
+      density-ratio branch of the convective term) that a 1e-12 difference
+      can flip: the oracle's own run splits from itself at step 213 when
+      only its solve tolerance changes (1e-10 -> 1e-12, the second fixture),
+      the device's splits from the oracle's at step 276.  After a split the
+      runs stay within dt 1e-3 relative, volume 1e-5 relative, centroid
+      1e-3 cell and bottom-row length 0.1 cell of each other."""
+    import os
+    from oracle import step as S
+    O = oracle_lib
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    ref = np.load(os.path.join(here, "dambreak_512x256_500.npz"))
+    ref12 = np.load(os.path.join(here, "dambreak_512x256_500_tol1e-12.npz"))
+    case = C.case_dambreak(512, 256)
+    g = case.grid
+    from simcoflow_amd.solver import TimeStepper
+    st = TimeStepper(case, precision=0, tol=1e-10)
+    ost = S.OracleStepper(O, case, tol=1e-10)
+    checks = (0, 99, 199, 274, 399, 498)   # one oracle step from the state before these steps
+    rows = []
+    try:
+        for k in range(500):
+            if k in checks:
+                before = {n: st.pj.download(n) for n in STATE_FIELDS}
+                itt, t = st.itt, st.time
+            rep = st.step()
+            assert rep["poisson"]["status"] == 0, (k, rep)
+            s = st.state()
+            rows.append((rep["dt"],) + _c1_diagnostics(g, s["vof"], case.geo.vofS, S.gas_centroid))
+            if k in checks:
+                ost.set_state(before, itt, t)
+                r = ost.step()
+                after = {n: st.pj.download(n) for n in STATE_FIELDS}
+                mine = ost.state()
+                worst = {}
+                for n in STATE_FIELDS:
+                    d = np.abs(after[n] - mine[n]).max()
+                    worst[n] = d / max(np.abs(mine[n]).max(), 1e-300)
+                print(f"[step] dam break step {k + 1}: one oracle step from the device state: dt "
+                      f"{abs(r['dt'] - rep['dt']) / r['dt']:.1e}, max rel diffs "
+                      + ", ".join(f"{n} {v:.1e}" for n, v in worst.items() if v > 0), flush=True)
+                assert r["dt"] == rep["dt"], k
+                for n in EXACT_FIELDS:
+                    assert np.array_equal(after[n], mine[n]), (k, n)
+                assert worst["UPRED"] < 1e-12 and worst["VPRED"] < 1e-12, (k, worst)
+                assert worst["U"] < 1e-9 and worst["V"] < 1e-9 and worst["P"] < 1e-9, (k, worst)
+            if k % 50 == 49:
+                print(f"[step] dam break 512x256, step {k + 1}/500", flush=True)
+    finally:
+        st.close()
+    rows = np.array(rows)
+    keys = ("dt", "volume", "centroid_x", "centroid_y", "bottom_length")
+    gpu = {key: rows[:, c] for c, key in enumerate(keys)}
+
+    def dev(a, b, key):
+        d = np.abs(a[key] - b[key])
+        return d / np.abs(b[key]) if key in ("dt", "volume") else d
+    dg = {key: dev(gpu, ref, key) for key in keys}
+    do = {key: dev(ref12, ref, key) for key in keys}
+    def split(d):
+        over = d["dt"] > 1e-9
+        return int(np.argmax(over)) + 1 if over.any() else None
+    print(f"[step] dam break: device vs oracle (tol 1e-10), worst over steps 1-200 / 1-500; oracle tol 1e-12 vs "
+          f"1e-10 over 1-500: " + "; ".join(f"{key} {dg[key][:200].max():.1e} / {dg[key].max():.1e} / "
+                                           f"{do[key].max():.1e}" for key in keys)
+          + f"; first step with dt apart by 1e-9: device {split(dg)}, oracle {split(do)}", flush=True)
+    assert ref["bottom_length"][-1] - ref["bottom_length"][0] > 20.0   # the water ran out along the floor
+    assert dg["dt"][:200].max() < 1e-10 and dg["volume"][:200].max() < 1e-12
+    for key in ("centroid_x", "centroid_y", "bottom_length"):
+        assert dg[key][:200].max() < 1e-9, key
+    assert dg["dt"].max() < 1e-3 and dg["volume"].max() < 1e-5
+    assert dg["centroid_x"].max() < 1e-3 and dg["centroid_y"].max() < 1e-3 and dg["bottom_length"].max() < 0.1
