@@ -1,6 +1,7 @@
 // C ABI of the simcogpu library: context, device fields, projection entry
 // points.  Every device buffer is allocated in sg_ctx_create; the solve and
 // correction entry points allocate nothing.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -180,6 +181,13 @@ int sg_ctx_create(int ni, int nj, const double* dx, const double* dy, const doub
         c->field_bytes[f] = bytes;
         c->field[f] = dev_alloc(c, bytes);
         if (!c->field[f]) return fail_alloc();
+    }
+    {
+        double dxmin = dx[0], dymax = dy[0];
+        for (int i = 1; i < ni; ++i) dxmin = std::min(dxmin, dx[i]);
+        for (int j = 1; j < nj; ++j) dymax = std::max(dymax, dy[j]);
+        c->semi0 = (nj >= 4 && dxmin >= SEMI_ASPECT * dymax) ? 1 : 0;
+        if (const char* ev = std::getenv("SG_MG_SEMI")) c->semi0 = (ev[0] == '1') ? c->semi0 : 0;
     }
     if ((rc = mg_allocate(c)) != SG_OK) return fail_alloc();
     const size_t n0 = c->lev[0].d.alloc_elems();

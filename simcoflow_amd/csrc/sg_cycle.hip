@@ -245,8 +245,9 @@ constexpr int GW = 64, GH = 16;  // fine-level tile
 // down on the fine level: x = S(0; b) (red from zero, then black), r = b - A
 // x, bc = P^T r.  As in the up-sweep, each thread holds the couplings of its
 // pairs of tile cells in registers for all three passes; cells of the halo
-// read theirs from memory.
-template <typename T, typename TB, int TW, int TH>
+// read theirs from memory.  SEMI: level 1 coarsens in y only (coarse (I,J)
+// on fine (I,2J)).
+template <typename T, typename TB, int TW, int TH, bool SEMI>
 __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __restrict__ b, T* __restrict__ xo,
                                                Dims C, Prol<T> P, T* __restrict__ bc, const int* on) {
     if (on && !*on) return;
@@ -374,12 +375,26 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
         for (int h = 0; h < 2; ++h)
             if (cl[u][h] >= 0) sbf[cl[u][h]] = rt[u][h];
     __syncthreads();
-    for (int k = tid; k < (TW / 2) * (TH / 2); k += NTB) {
-        const int I = i0 / 2 + k % (TW / 2), J = j0 / 2 + k / (TW / 2);
-        if (I < C.ni && J < C.nj) {
-            const int li = 2 * I - i0 + 3, lj = 2 * J - j0 + 3;
-            auto R = [&](int a, int bb) { return sb[lj + bb][li + a]; };
-            bc[C.off(I, J)] = restrict_point<T>(C, P, I, J, R);
+    if (SEMI) {
+        // P^T r over the fine column: (I,2J) injected, (I,2J+1) by YS(I,J),
+        // (I,2J-1) by YN(I,J-1)
+        for (int k = tid; k < TW * (TH / 2); k += NTB) {
+            const int I = i0 + k % TW, J = j0 / 2 + k / TW;
+            if (I < C.ni && J < C.nj) {
+                const int li = I - i0 + 3, lj = 2 * J - j0 + 3;
+                const size_t cc = C.off(I, J);
+                bc[cc] = P.w[W_INJ][cc] * sb[lj][li] + P.w[W_YS][cc] * sb[lj + 1][li] +
+                         P.w[W_YN][cc - C.ld] * sb[lj - 1][li];
+            }
+        }
+    } else {
+        for (int k = tid; k < (TW / 2) * (TH / 2); k += NTB) {
+            const int I = i0 / 2 + k % (TW / 2), J = j0 / 2 + k / (TW / 2);
+            if (I < C.ni && J < C.nj) {
+                const int li = 2 * I - i0 + 3, lj = 2 * J - j0 + 3;
+                auto R = [&](int a, int bb) { return sb[lj + bb][li + a]; };
+                bc[C.off(I, J)] = restrict_point<T>(C, P, I, J, R);
+            }
         }
     }
     for (int k = tid; k < TW * TH; k += NTB) {
@@ -395,7 +410,8 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
 // the black pass uses the black cell's, the red pass the red cell's, so no
 // coefficient is read from memory twice.  The one-cell black ring around the
 // tile is smoothed from global reads.  b stays in LDS in FP64 for the dots.
-template <typename T, typename V, int TW, int TH>
+// SEMI: level 1 coarsens in y only.
+template <typename T, typename V, int TW, int TH, bool SEMI>
 __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const V* __restrict__ b,
                                              const T* __restrict__ xin, Dims C, Prol<T> P, const T* __restrict__ e,
                                              V* __restrict__ z, const V* __restrict__ q, Slots Sl,
@@ -413,8 +429,42 @@ __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const V* __restri
     const int tid = threadIdx.x;
     const size_t ld = F.ld;
     // b and x + P e per 2x2 block of cells sharing one coarse parent (the
-    // region starts on an even cell, so blocks align with coarse cells)
-    {
+    // region starts on an even cell, so blocks align with coarse cells); with
+    // SEMI per 1x2 block (fine (I,2J) and (I,2J+1) of coarse (I,J))
+    if (SEMI) {
+        constexpr int BW = RW, BH = RH / 2;
+        constexpr int NL2 = (BW * BH + NTB - 1) / NTB;
+        const int Ib = i0 - 2, Jb = j0 / 2 - 1;
+#pragma unroll
+        for (int m = 0; m < NL2; ++m) {
+            const int k = tid + m * NTB;
+            if (k >= BW * BH) continue;
+            const int a = k % BW, cr = k / BW;
+            const int I = Ib + a, J = Jb + cr;
+            T xv[2] = {T(0), T(0)};
+            double bv[2] = {0.0, 0.0};
+            if (I >= 0 && J >= 0 && I < C.ni && J < C.nj) {
+                const size_t cc = C.off(I, J);
+                T* const* w = P.w;
+                const T e00 = e[cc], e01 = e[cc + C.ld];
+                const T pe[2] = {w[W_INJ][cc] * e00, w[W_YS][cc] * e00 + w[W_YN][cc] * e01};
+#pragma unroll
+                for (int dj = 0; dj < 2; ++dj) {
+                    const int gj = 2 * J + dj;
+                    if (gj < F.nj) {
+                        const size_t p = F.off(I, gj);
+                        bv[dj] = (double)b[p];
+                        xv[dj] = xin[p] + pe[dj];
+                    }
+                }
+            }
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj) {
+                sb[2 * cr + dj][a] = bv[dj];
+                sx[2 * cr + dj][a] = xv[dj];
+            }
+        }
+    } else {
         constexpr int BW = RW / 2, BH = RH / 2;
         constexpr int NL2 = (BW * BH + NTB - 1) / NTB;
         const int Ib = i0 / 2 - 1, Jb = j0 / 2 - 1;
@@ -539,6 +589,27 @@ __global__ void __launch_bounds__(NTB) k_up5(Dims F, OpV<T> A, const V* __restri
     }
     double vv[2] = {v0, v1};
     block_partials<2, NTB>(vv, Sl.part, Sl.max_blocks);
+}
+
+// the fine-level kernels with the level-1 coarsening chosen at run time
+template <typename T, typename TB>
+void launch_down5(bool semi, hipStream_t s, Dims F, OpV<T> A, const TB* b, T* xo, Dims C, Prol<T> P, T* bc,
+                  const int* on) {
+    const dim3 g((F.ni + GW - 1) / GW, (F.nj + GH - 1) / GH);
+    if (semi)
+        hipLaunchKernelGGL((k_down5<T, TB, GW, GH, true>), g, dim3(NTB), 0, s, F, A, b, xo, C, P, bc, on);
+    else
+        hipLaunchKernelGGL((k_down5<T, TB, GW, GH, false>), g, dim3(NTB), 0, s, F, A, b, xo, C, P, bc, on);
+}
+
+template <typename T, typename V>
+void launch_up5(bool semi, hipStream_t s, Dims F, OpV<T> A, const V* b, const T* xin, Dims C, Prol<T> P,
+                const T* e, V* z, const V* q, Slots S, const int* on) {
+    const dim3 g((F.ni + GW - 1) / GW, (F.nj + GH - 1) / GH);
+    if (semi)
+        hipLaunchKernelGGL((k_up5<T, V, GW, GH, true>), g, dim3(NTB), 0, s, F, A, b, xin, C, P, e, z, q, S, on);
+    else
+        hipLaunchKernelGGL((k_up5<T, V, GW, GH, false>), g, dim3(NTB), 0, s, F, A, b, xin, C, P, e, z, q, S, on);
 }
 
 // ---------------------------------------------------------------------------
@@ -920,8 +991,8 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done, const 
         const Level& L = c->lev[l];
         const Level& Cl = c->lev[l + 1];
         if (l == 0)
-            hipLaunchKernelGGL((k_down5<T, V, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
-                               Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), Acc<T>::vec(Cl).b, on);
+            launch_down5<T, V>(c->semi0 != 0, s, L.d, opv(Acc<T>::op(L)), r, Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L),
+                               Acc<T>::vec(Cl).b, on);
         else if (big(L.d))
             hipLaunchKernelGGL((k_down9<T, 64, 16>), tiles(L.d, 64, 16), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
                                (const T*)Acc<T>::vec(L).b, Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L),
@@ -986,8 +1057,8 @@ int vcycle(Ctx* c, const V* r, V* z, const V* q, Slots S, int* dots_done, const 
         const Level& Cl = c->lev[l + 1];
         const T* e = (l + 1 == lt) ? Acc<T>::vec(Cl).x : Acc<T>::vec(Cl).r;
         if (l == 0)
-            hipLaunchKernelGGL((k_up5<T, V, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)), r,
-                               (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e, z, q, S, on);
+            launch_up5<T, V>(c->semi0 != 0, s, L.d, opv(Acc<T>::op(L)), r, (const T*)Acc<T>::vec(L).x, Cl.d,
+                             Acc<T>::pr(L), e, z, q, S, on);
         else if (big(L.d))
             hipLaunchKernelGGL((k_up9<T, 64, 16>), tiles(L.d, 64, 16), dim3(NTB), 0, s, L.d, opv(Acc<T>::op(L)),
                                (const T*)Acc<T>::vec(L).b, (const T*)Acc<T>::vec(L).x, Cl.d, Acc<T>::pr(L), e,
@@ -1028,22 +1099,22 @@ int mg_launch_fine(Ctx* c, int which) {
     const bool mixed = c->prm.precision == SG_PREC_MIXED;
     if (which == 1) {
         if (mixed)
-            hipLaunchKernelGGL((k_down5<float, float, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af), c->rf,
-                               L.vf.x, Cl.d, L.Pf, Cl.vf.b, nullptr);
+            launch_down5<float, float>(c->semi0 != 0, s, L.d, opv(L.Af), (const float*)c->rf, L.vf.x, Cl.d, L.Pf,
+                                       Cl.vf.b, nullptr);
         else
-            hipLaunchKernelGGL((k_down5<double, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
-                               L.v.x, Cl.d, L.P, Cl.v.b, nullptr);
+            launch_down5<double, double>(c->semi0 != 0, s, L.d, opv(L.A), (const double*)c->r, L.v.x, Cl.d, L.P,
+                                         Cl.v.b, nullptr);
     } else if (which == 2) {
         // the coarse correction sits in x when level 1 is the tail's first level, else in r
         const bool tail1 = c->nlev == 2 || (size_t)Cl.d.ni * Cl.d.nj <= (size_t)TAIL_MAX;
         if (mixed)
-            hipLaunchKernelGGL((k_up5<float, float, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.Af),
-                               c->rf, (const float*)L.vf.x, Cl.d, L.Pf, (const float*)(tail1 ? Cl.vf.x : Cl.vf.r),
-                               c->zf, c->qf, S, nullptr);
+            launch_up5<float, float>(c->semi0 != 0, s, L.d, opv(L.Af), (const float*)c->rf, (const float*)L.vf.x,
+                                     Cl.d, L.Pf, (const float*)(tail1 ? Cl.vf.x : Cl.vf.r), c->zf,
+                                     (const float*)c->qf, S, nullptr);
         else
-            hipLaunchKernelGGL((k_up5<double, double, GW, GH>), tiles(L.d, GW, GH), dim3(NTB), 0, s, L.d, opv(L.A), c->r,
-                               (const double*)L.v.x, Cl.d, L.P, (const double*)(tail1 ? Cl.v.x : Cl.v.r), c->z,
-                               c->q, S, nullptr);
+            launch_up5<double, double>(c->semi0 != 0, s, L.d, opv(L.A), (const double*)c->r, (const double*)L.v.x,
+                                       Cl.d, L.P, (const double*)(tail1 ? Cl.v.x : Cl.v.r), c->z,
+                                       (const double*)c->q, S, nullptr);
     } else if (which == 3) {
         const size_t n = L.d.alloc_elems();
         hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, (const double*)L.v.b,

@@ -25,6 +25,9 @@ constexpr int COARSEST_MAX = 16; // unknowns of the dense coarsest solve
 constexpr int MAX_LEVELS = 24;
 constexpr int NRED = 4;          // values a reduction slot can carry
 
+// cells at least this much wider than tall coarsen in y first
+constexpr double SEMI_ASPECT = 1.4;
+
 struct Dims {
     int ni = 0, nj = 0, ld = 0;
     __host__ __device__ size_t alloc_elems() const { return (size_t)(nj + 3) * (size_t)ld; }
@@ -243,6 +246,10 @@ struct Ctx {
     size_t field_bytes[SG_F_COUNT] = {0};
     // solver
     int nlev = 0;
+    // level 1 coarsens level 0 in y only (cells at least 1.4x wider than
+    // tall: the N couplings are 2x the E ones or more, which point
+    // smoothing leaves to the coarse grid); all other levels coarsen 2x2
+    int semi0 = 0;
     Level lev[MAX_LEVELS];
     double* coarse_inv = nullptr;  // dense inverse of the coarsest operator
     float* coarse_inv_f = nullptr;

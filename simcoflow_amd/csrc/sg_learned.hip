@@ -25,24 +25,40 @@ typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int NTR = 1024;
 
-// b_{l+1} = P^T b_l
+// b_{l+1} = P^T b_l (semi: the level coarsens in y only, coarse (I,J) on
+// fine (I,2J))
 __global__ void __launch_bounds__(256) k_restrict(Dims F, Dims C, Prol<double> P, const double* __restrict__ bf,
-                                                  double* __restrict__ bc) {
+                                                  double* __restrict__ bc, int semi) {
     const int I = blockIdx.x * TX + threadIdx.x;
     const int J = blockIdx.y * TY + threadIdx.y;
     if (I >= C.ni || J >= C.nj) return;
+    const size_t c = C.off(I, J);
+    if (semi) {
+        bc[c] = P.w[W_INJ][c] * bf[F.off(I, 2 * J)] + P.w[W_YS][c] * bf[F.off(I, 2 * J + 1)] +
+                P.w[W_YN][c - C.ld] * bf[F.off(I, 2 * J - 1)];
+        return;
+    }
     auto R = [&](int a, int b) { return bf[F.off(2 * I + a, 2 * J + b)]; };
-    bc[C.off(I, J)] = restrict_point(C, P, I, J, R);
+    bc[c] = restrict_point(C, P, I, J, R);
 }
 
 // x_l = P e_{l+1}; solid points of the target level (diag = 0) get 0
 __global__ void __launch_bounds__(256) k_prolong(Dims F, Dims C, Prol<double> P, const double* __restrict__ diag,
-                                                 const double* __restrict__ e, double* __restrict__ x) {
+                                                 const double* __restrict__ e, double* __restrict__ x, int semi) {
     const int i = blockIdx.x * TX + threadIdx.x;
     const int j = blockIdx.y * TY + threadIdx.y;
     if (i >= F.ni || j >= F.nj) return;
     const size_t p = F.off(i, j);
-    x[p] = diag[p] > 0.0 ? prolong_point(C, P, e, i, j) : 0.0;
+    double v = 0.0;
+    if (diag[p] > 0.0) {
+        if (semi) {
+            const size_t c = C.off(i, j >> 1);
+            v = (j & 1) ? P.w[W_YS][c] * e[c] + P.w[W_YN][c] * e[c + C.ld] : P.w[W_INJ][c] * e[c];
+        } else {
+            v = prolong_point(C, P, e, i, j);
+        }
+    }
+    x[p] = v;
 }
 
 // rms of b over the fluid points of the level -> sc->nn_scale, in two
@@ -92,9 +108,11 @@ __device__ inline double block_vof(const double* __restrict__ vof, int I, int J,
 }
 
 // net input at level L (see the file comment); optionally the training
-// target t = x(2^L I, 2^L J) / scale and the fluid mask
+// target t = x(sx I, 2^L J) / scale and the fluid mask.  A level-L point
+// covers sx x 2^L fine cells: sx = 2^L, or 2^(L-1) when level 1 coarsens in
+// y only (semi)
 __global__ void __launch_bounds__(256) k_features(Dims D, Op<double> A, const double* __restrict__ b, Dims F0,
-                                                  const double* __restrict__ vof, int ni0, int nj0, int L,
+                                                  const double* __restrict__ vof, int ni0, int nj0, int L, int semi,
                                                   const double* __restrict__ x0, const PcgScalars* sc,
                                                   bf16* __restrict__ X, float* __restrict__ t,
                                                   float* __restrict__ mask) {
@@ -106,9 +124,10 @@ __global__ void __launch_bounds__(256) k_features(Dims D, Op<double> A, const do
     const double d = A.diag[p];
     const double id = d > 0.0 ? 1.0 / d : 0.0;
     const int s = 1 << L;
+    const int sx = (semi && L > 0) ? s / 2 : s;
     double vs = 0.0;
     int nv = 0;
-    switch (L) {
+    switch (sx == s ? L : -1) {
     case 0: vs = block_vof<1>(vof, I, J, ni0, nj0, nv); break;
     case 1: vs = block_vof<2>(vof, I, J, ni0, nj0, nv); break;
     case 2: vs = block_vof<4>(vof, I, J, ni0, nj0, nv); break;
@@ -116,7 +135,7 @@ __global__ void __launch_bounds__(256) k_features(Dims D, Op<double> A, const do
     case 4: vs = block_vof<16>(vof, I, J, ni0, nj0, nv); break;
     default:
         for (int jj = J * s; jj < (J + 1) * s && jj < nj0; ++jj)
-            for (int ii = I * s; ii < (I + 1) * s && ii < ni0; ++ii) {
+            for (int ii = I * sx; ii < (I + 1) * sx && ii < ni0; ++ii) {
                 vs += vof[(size_t)jj * ni0 + ii];
                 ++nv;
             }
@@ -136,7 +155,7 @@ __global__ void __launch_bounds__(256) k_features(Dims D, Op<double> A, const do
     *(bf16x8*)(X + ((size_t)(J + 1) * (D.ni + 2) + (I + 1)) * 8) = o;
     if (t) {
         const size_t q = (size_t)J * D.ni + I;
-        t[q] = d > 0.0 ? (float)(x0[F0.off(I * s, J * s)] * inv) : 0.f;
+        t[q] = d > 0.0 ? (float)(x0[F0.off(I * sx, J * s)] * inv) : 0.f;
         mask[q] = d > 0.0 ? 1.f : 0.f;
     }
 }
@@ -213,7 +232,7 @@ int restrict_to(Ctx* c, int level, const double* src) {
     for (int l = 0; l < level; ++l) {
         const Dims& C = c->lev[l + 1].d;
         hipLaunchKernelGGL(k_restrict, grid2d(C.ni, C.nj), block2d(), 0, s, c->lev[l].d, C, c->lev[l].P, src,
-                           c->lev[l + 1].v.b);
+                           c->lev[l + 1].v.b, (l == 0 && c->semi0) ? 1 : 0);
         src = c->lev[l + 1].v.b;
     }
     SG_HIP(hipGetLastError());
@@ -234,7 +253,7 @@ int features(Ctx* c, int level, bf16* X, float* t, float* mask, const double* sr
                            (int)(g.x * g.y), c->sc);
     }
     hipLaunchKernelGGL(k_features, grid2d(L.d.ni, L.d.nj), block2d(), 0, c->stream, L.d, L.A, b, c->lev[0].d,
-                       (const double*)c->field[SG_F_VOF], c->ni, c->nj, level, (const double*)c->x,
+                       (const double*)c->field[SG_F_VOF], c->ni, c->nj, level, c->semi0, (const double*)c->x,
                        (const PcgScalars*)c->sc, X, t, mask);
     SG_HIP(hipGetLastError());
     return SG_OK;
@@ -344,7 +363,7 @@ int enqueue_warm_start(Ctx* c, bool prev) {
         double* dst = l ? c->lev[l].v.x : c->nn.w;
         hipLaunchKernelGGL(k_prolong, grid2d(c->lev[l].d.ni, c->lev[l].d.nj), block2d(), 0, s, c->lev[l].d,
                            c->lev[l + 1].d, c->lev[l].P, (const double*)c->lev[l].A.diag,
-                           (const double*)c->lev[l + 1].v.x, dst);
+                           (const double*)c->lev[l + 1].v.x, dst, (l == 0 && c->semi0) ? 1 : 0);
     }
     const Dims D = c->lev[0].d;
     const dim3 g = grid2d(c->ni, c->nj);

@@ -2,7 +2,12 @@
 // operator (stands where BoomerAMG stands in ProjectionP_Mod.f90:117-133).
 //
 //  * coarsening: every other cell in x and y (coarse point (I,J) sits on fine
-//    cell (2I,2J)); levels down to <= 64 unknowns;
+//    cell (2I,2J)); levels down to <= 64 unknowns.  On grids of cells at
+//    least 1.4x wider than tall the first coarsening is in y only (coarse
+//    (I,J) on fine (I,2J)): the N couplings there are twice the E ones or
+//    more, and red-black point smoothing reduces the error along the strong
+//    direction only -- coarsening that direction alone leaves an isotropic
+//    level 1 (semi-coarsening);
 //  * prolongation: operator-dependent (black-box) interpolation -- edge
 //    points lump the stencil column/row onto their two coarse neighbours,
 //    centre points average their eight interpolated neighbours with the
@@ -105,6 +110,37 @@ __global__ void __launch_bounds__(256) k_prol_center(Dims F, OpV<double> A, Dims
     }
 }
 
+// y-only coarsening of the 5-point fine level: coarse (I,J) on fine (I,2J)
+// (injection); the odd row (I,2J+1) from (I,J) and (I,J+1) with the row's x
+// couplings lumped onto its diagonal, as the edge points above
+__global__ void __launch_bounds__(256) k_prol_semi(Dims F, OpV<double> A, Dims C, Prol<double> P, Prol<float> Pf) {
+    const int I = blockIdx.x * TX + threadIdx.x;
+    const int J = blockIdx.y * TY + threadIdx.y;
+    if (I >= C.ni || J >= C.nj) return;
+    const size_t cp = C.off(I, J);
+    const size_t ld = F.ld;
+    const double inj = (A.diag[F.off(I, 2 * J)] > 0.0) ? 1.0 : 0.0;
+    double yS = 0.0, yN = 0.0;
+    if (2 * J + 1 < F.nj) {
+        const size_t f = F.off(I, 2 * J + 1);
+        const double d = A.diag[f];
+        if (d > 0.0) {
+            const double den = d - A.cE[f] - A.cE[f - 1];
+            if (den > 0.0) {
+                yS = A.cN[f - ld] / den;
+                yN = A.cN[f] / den;
+            }
+        }
+    }
+    const double w[9] = {inj, 0.0, 0.0, yS, yN, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P.w[k][cp] = w[k];
+    if (Pf.w[0]) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Pf.w[k][cp] = (float)w[k];
+    }
+}
+
 // ---- Galerkin coarse operator ---------------------------------------------
 // Ac(C1,C2) = sum_f sum_g P(f,C1) A(f,g) P(g,C2) for C1 = (I,J) and the five
 // stored neighbours C2 - C1 in {(0,0),(1,0),(0,1),(1,1),(-1,1)}.  The fine
@@ -200,6 +236,60 @@ __global__ void __launch_bounds__(256) k_rap(Dims F, OpV<double> A, Dims C, Prol
     }
 }
 
+// Galerkin operator of the y-coarsened level 1 from the 5-point level 0:
+// the fine cells carrying coarse (I,J) are (I,2J+b), b in {-1,0,1}; a fine
+// cell g = (I+ga, 2J+gb) belongs to coarse (I+ga, J+gb/2) by injection (gb
+// even) or to (I+ga, J0) and (I+ga, J0+1), J0 = J+(gb-1)/2, by the odd-row
+// weights (gb odd).  The result is 9-point, stored like k_rap's.
+__global__ void __launch_bounds__(256) k_rap_semi(Dims F, OpV<double> A, Dims C, Prol<double> P, Op<double> Ac,
+                                                  Op<float> Af) {
+    const int I = blockIdx.x * TX + threadIdx.x;
+    const int J = blockIdx.y * TY + threadIdx.y;
+    if (I >= C.ni || J >= C.nj) return;
+    const size_t ldf = F.ld, ldc = C.ld;
+    const size_t c = C.off(I, J);
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    auto scatter = [&](double s, int ga, int gb) {
+        if ((gb & 1) == 0) {
+            const int k = rap_target(ga, gb / 2);
+            if (k >= 0) acc[k] += s * P.w[W_INJ][c + (long)(gb / 2) * (long)ldc + ga];
+        } else {
+            const int J0 = (gb - 1) / 2;  // exact: gb - 1 is even
+            const size_t q = c + (long)J0 * (long)ldc + ga;
+            const int k0 = rap_target(ga, J0), k1 = rap_target(ga, J0 + 1);
+            if (k0 >= 0) acc[k0] += s * P.w[W_YS][q];
+            if (k1 >= 0) acc[k1] += s * P.w[W_YN][q];
+        }
+    };
+#pragma unroll
+    for (int b = -1; b <= 1; ++b) {
+        const double wf = b == 0 ? P.w[W_INJ][c] : (b > 0 ? P.w[W_YS][c] : P.w[W_YN][c - ldc]);
+        const size_t f = F.off(I, 2 * J + b);
+        scatter(wf * A.diag[f], 0, b);
+        scatter(wf * -A.cE[f], 1, b);
+        scatter(wf * -A.cE[f - 1], -1, b);
+        scatter(wf * -A.cN[f], 0, b + 1);
+        scatter(wf * -A.cN[f - ldf], 0, b - 1);
+    }
+    const double d = acc[0] > 0.0 ? acc[0] : 0.0;
+    const double e = (I + 1 < C.ni) ? -acc[1] : 0.0;
+    const double n = (J + 1 < C.nj) ? -acc[2] : 0.0;
+    const double ne = (I + 1 < C.ni && J + 1 < C.nj) ? -acc[3] : 0.0;
+    const double nw = (I > 0 && J + 1 < C.nj) ? -acc[4] : 0.0;
+    Ac.diag[c] = d;
+    Ac.cE[c] = e;
+    Ac.cN[c] = n;
+    Ac.cNE[c] = ne;
+    Ac.cNW[c] = nw;
+    if (Af.diag) {
+        Af.diag[c] = (float)d;
+        Af.cE[c] = (float)e;
+        Af.cN[c] = (float)n;
+        Af.cNE[c] = (float)ne;
+        Af.cNW[c] = (float)nw;
+    }
+}
+
 // ---- dense inverse of the coarsest operator (one workgroup) ---------------
 __global__ void __launch_bounds__(256) k_coarse_inverse(Dims C, OpV<double> A, double* inv, float* invf) {
     __shared__ double a[COARSEST_MAX][COARSEST_MAX];
@@ -271,7 +361,10 @@ int mg_allocate(Ctx* c) {
     c->lev[0].d = make_dims(c->ni, c->nj);
     while ((size_t)c->lev[l].d.ni * c->lev[l].d.nj > (size_t)COARSEST_MAX && l + 1 < MAX_LEVELS) {
         const Dims& f = c->lev[l].d;
-        c->lev[l + 1].d = make_dims((f.ni + 1) / 2, (f.nj + 1) / 2);
+        if (l == 0 && c->semi0)
+            c->lev[1].d = make_dims(f.ni, (f.nj + 1) / 2);
+        else
+            c->lev[l + 1].d = make_dims((f.ni + 1) / 2, (f.nj + 1) / 2);
         ++l;
     }
     c->nlev = l + 1;
@@ -327,6 +420,12 @@ int mg_setup(Ctx* c) {
         OpV<double> A = opv(F.A);
         const Prol<float> Pf = mixed ? F.Pf : Prol<float>{};
         const Op<float> Af = mixed ? C.Af : Op<float>{};
+        if (l == 0 && c->semi0) {
+            hipLaunchKernelGGL(k_prol_semi, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P, Pf);
+            hipLaunchKernelGGL(k_rap_semi, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P, C.A, Af);
+            SG_HIP(hipGetLastError());
+            continue;
+        }
         hipLaunchKernelGGL(k_prol_edges, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P, Pf);
         hipLaunchKernelGGL(k_prol_center, grid2d(C.d.ni, C.d.nj), block2d(), 0, s, F.d, A, C.d, F.P, Pf);
         if (l == 0)

@@ -20,6 +20,9 @@ SMALL = [
     # odd sizes: the last coarse cell of each level has one fine child per axis
     ("bubble-odd", lambda: C.case_bubble(37, 75)),
     ("dambreak-odd", lambda: C.case_dambreak(131, 67)),
+    # cells twice as wide as tall: the first coarsening is in y only
+    ("waves-aspect2", lambda: C.case_waves(96, 96, seed=2)),
+    ("waves-aspect2-odd", lambda: C.case_waves(97, 99, seed=4)),
 ]
 
 
