@@ -13,12 +13,15 @@
       PoissonEquationSolver                        -> sg_poisson_build/solve_resident
       velocity correction                          -> sg_velocity_correct_resident
 
-Every field stays on the device between steps; the host reads dt and the
-solver reports.  Left out, as for the rest of this build: moving solids
-(ObjectMovement / SolidVolumeFraction / Grid_Preprocess are not re-run, the
-cut-cell geometry is the one given at set-up), the particle model and the
-wave channel's inlet/outlet profiles.  Non-dimensional groups use Uref = 1
-m/s and Lref = 1 m (StateVariables.f90:97-98): Re = 1/nu_l, Fr = 1/sqrt(g).
+With ``moving`` the solid moves as a rigid body (ComputeForceObject ->
+acceleration, ObjectMovement + SolidVolumeFraction inside the transport's
+sub-steps, Grid_Preprocess + NewCellFace on the device, InterNewVar,
+moving-wall terms); particles are tracked with ``Projection.particles_*``
+beside the step.  Every field stays on the device between steps; the host
+reads dt and the solver reports.  Left out: the wave channel's
+inlet/outlet profiles (a closed box instead).  Non-dimensional groups use
+Uref = 1 m/s and Lref = 1 m (StateVariables.f90:97-98): Re = 1/nu_l,
+Fr = 1/sqrt(g).
 """
 import math
 
@@ -96,6 +99,7 @@ class TimeStepper:
         self.time = 0.0
         self.last = {}
         self.diagnostics = diagnostics
+        self.device_geometry = device_geometry
 
     def close(self):
         self.pj.close()
@@ -123,12 +127,16 @@ class TimeStepper:
                 kind, cx, cy, a, b, cc = mv["shape"]
                 pj.set_moving_solid(kind, cx, cy, a, b, cc, us=mv["us"], vs=mv["vs"], asx=0.0, asy=mv["asy"])
             pj.clsvof_advect_resident(dt, self.time)
-            check(lib.sg_interface_to_uv_resident(ctx))
             if mv:
+                # Initial_ClsVofUV updates the U / V solid fractions for the
+                # moved solid before it computes and clips the U / V volume
+                # fractions (SolidVolumeFraction, Clsvof_Mod.f90:416,681-684):
+                # rebuild first, so the clip sees the new solid
                 (_, cx, cy, a, b, cc), mv["us"], mv["vs"] = pj.get_moving_solid()
                 pj.set_moving_solid("none")
                 mv["shape"] = (kind, cx, cy, a, b, cc)
                 pj.geometry_rebuild(kind, cx, cy, a, b, cc)
+            check(lib.sg_interface_to_uv_resident(ctx))
         check(lib.sg_closed_box_bc(ctx))
         vb = mv["vs"] if mv else 0.0
         if mv and self.itt > 1:
@@ -153,15 +161,29 @@ class TimeStepper:
         return {k: rows[:, c] for c, k in enumerate(_lib.DIAG_NAMES)}
 
     def save(self, path):
-        """Checkpoint of the run (device state, step counter, time)."""
+        """Checkpoint of the run (device state, step counter, time; a moving
+        solid's shape, velocity, acceleration and mass)."""
         from .output import save_checkpoint
-        save_checkpoint(path, self.pj, self.itt, self.time)
+        extra = {}
+        if self.moving:
+            extra["moving"] = {k: (list(v) if isinstance(v, tuple) else v) for k, v in self.moving.items()}
+        save_checkpoint(path, self.pj, self.itt, self.time, extra=extra)
 
     def load(self, path):
-        """Resumes from a checkpoint written by save on the same case."""
+        """Resumes from a checkpoint written by save on the same case; a
+        moving solid is put back where it was and its cut-cell geometry is
+        rebuilt on the device there (the state holds the fields, not the
+        geometry)."""
         from .output import load_checkpoint
         meta = load_checkpoint(path, self.pj)
         self.itt, self.time = meta["itt"], meta["time"]
+        mv = meta.get("extra", {}).get("moving")
+        if mv:
+            if not self.moving:
+                raise ValueError(f"{path}: saved with a moving solid; make the stepper with moving=...")
+            mv["shape"] = tuple(mv["shape"])
+            self.moving = mv
+            self.pj.geometry_rebuild(*mv["shape"])
 
     def write_vtr(self, directory, fmt="ascii"):
         """Print_Result_VTK_2D's file for the current step
@@ -169,8 +191,12 @@ class TimeStepper:
         from .output import vtr_name, write_vtr
         path = vtr_name(directory, self.itt)
         st = self.state()
-        geo = self.case.geo
-        write_vtr(path, self.case.grid, st["u"], st["v"], st["p"], st["phi"], geo.phiS, st["vof"], geo.vofS, fmt=fmt)
+        if self.device_geometry:
+            # the geometry lives on the device (it moves with a moving solid)
+            phiS, vofS = self.pj.download("PHIS"), self.pj.download("VOFS")
+        else:
+            phiS, vofS = self.case.geo.phiS, self.case.geo.vofS
+        write_vtr(path, self.case.grid, st["u"], st["v"], st["p"], st["phi"], phiS, st["vof"], vofS, fmt=fmt)
         return path
 
     def state(self):
