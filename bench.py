@@ -164,6 +164,55 @@ def run_path(case, fields, precision, steps, warmup, tol, learned=None, width=25
     }
 
 
+def learned_bound(case, fields, width):
+    """What any predictor on the net's level could gain: the exact pressure
+    (zero start) and the exact correction of a previous pressure (warm_start
+    3, a second system whose predictor velocities differ by 10 %), sampled
+    at the level's points as the training targets are and handed to the
+    solve in place of the net's output.  Iterations to 1e-8, FP64, on the
+    net's level and on the first coarse level."""
+    from simcoflow_amd import learned as Lr
+    from simcoflow_amd.projection import Projection, params_for
+    u, v, DpU, DpV = fields
+    rng = np.random.default_rng(7)
+    # a smooth second predictor field: the first plus 10 % of a shifted copy
+    u2 = u + 0.1 * np.roll(u, 37, axis=1) * rng.uniform(0.5, 1.5)
+    v2 = v + 0.1 * np.roll(v, 53, axis=0) * rng.uniform(0.5, 1.5)
+    pj = Projection(case.grid, params_for(case.fluids, tol=1e-8, max_iter=300))
+    try:
+        pj.set_geometry(case.geo)
+        pj.set_interface(case.itf.vof, case.itf.phi)
+
+        def solve(uu, vv, tol=1e-8, ws=0):
+            pj.set_predictor(uu, vv, DpU, DpV)
+            pj.set_params(params_for(case.fluids, tol=tol, max_iter=300, warm_start=ws))
+            pj.build(0.0)
+            info = pj.solve()
+            return info, pj.download("PP")[1:-1, 1:-1].copy()
+        net_level = Lr.pick_level(pj, width)
+        _, x2 = solve(u2, v2, 1e-12)
+        _, x1 = solve(u, v, 1e-12)
+        zero, _ = solve(u, v)
+        solve(u, v, 1e-12)
+        prev, _ = solve(u2, v2, ws=1)
+        out = {"net_level": net_level, "net_level_grid": list(Lr.level_dims(pj, net_level)),
+               "zero_start_iterations": zero.iterations, "zero_start_ms": round(zero.ms, 4),
+               "previous_pressure_iterations": prev.iterations, "previous_pressure_ms": round(prev.ms, 4)}
+        for lv in sorted({net_level, 1}):
+            Lr.set_prediction(pj, lv, Lr.sample_at_level(pj, lv, x1))
+            a, _ = solve(u, v, ws=2)
+            Lr.set_prediction(pj, lv, Lr.sample_at_level(pj, lv, x2 - x1))
+            solve(u, v, 1e-12)
+            b, _ = solve(u2, v2, ws=3)
+            Lr.set_prediction(pj, lv, None)
+            out[f"level_{lv}"] = {"grid": list(Lr.level_dims(pj, lv)),
+                                  "exact_pressure_iterations": a.iterations,
+                                  "exact_correction_iterations": b.iterations}
+    finally:
+        pj.close()
+    return out
+
+
 def train_warm_start(seed, width):
     """The learned warm start, trained in this run on the GPU: pairs from the
     plain FP64 solver on four seeded 512x512 swarms, 48 right-hand sides
@@ -498,6 +547,8 @@ def main():
     if w is not None:
         out["learned"] = dict(train, warm_start_ms=round(proj["learned_mixed"]["warm_ms"], 4),
                               start_relres=proj["learned_mixed"]["warm_relres"])
+        log("upper bound of a learned start on the net's level")
+        out["learned"]["upper_bound"] = learned_bound(case, fields, a.nn_width)
     k = proj["mixed"]["kernel"].get("up")
     if k:
         out["roofline"] = {"bound": "hbm", "achieved": round(k["GBs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

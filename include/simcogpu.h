@@ -290,6 +290,16 @@ int sg_nn_features(sg_ctx* ctx, int level, void* x, float* t, float* mask, doubl
  * (host FP32), the level it runs on, out_scale, deflate 0/1; w = NULL
  * disables it */
 int sg_set_learned(sg_ctx* ctx, const float* w, int level, double out_scale, int deflate);
+/* A level-L prediction from the caller in place of the net's output: the
+ * pressure at the level's points (warm_start = 2), or the correction of the
+ * previous pressure (warm_start = 3), which the solve prolongates, Galerkin-
+ * scales and (deflate = 1) keeps its directions A-orthogonal to, exactly as
+ * it does the net's.  No counterpart in the reference (its solves start from
+ * zero); for callers with their own predictor, and to measure what any
+ * predictor on that level could gain.  e: host array of nj_L x ni_L doubles,
+ * row-major (sg_nn_level_dims gives the level's grid); NULL switches it
+ * off.  Replaces a net set by sg_set_learned, and is replaced by one. */
+int sg_set_learned_prediction(sg_ctx* ctx, int level, const double* e, int deflate);
 
 /* ---- Lagrangian particles (replaces TrackingParticles, Particles.f90:71-205,
  * with Drag :206-223 and ParticlePosition :228-249) -------------------------

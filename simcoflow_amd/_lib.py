@@ -176,6 +176,7 @@ PROTOTYPES = {
                          ctypes.c_int),
     "sg_nn_features": ([_CTX, ctypes.c_int, _V, _V, _V, _D], ctypes.c_int),
     "sg_set_learned": ([_CTX, _F, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    "sg_set_learned_prediction": ([_CTX, ctypes.c_int, _D, ctypes.c_int], ctypes.c_int),
 }
 
 _lib = None

@@ -978,6 +978,12 @@ int sg_set_learned(sg_ctx* ctx, const float* w, int level, double out_scale, int
     return rc;
 }
 
+int sg_set_learned_prediction(sg_ctx* ctx, int level, const double* e, int deflate) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    return sg::learned_set_prediction(c, level, e, deflate);
+}
+
 }  // extern "C"
 
 // ---- time-step control ------------------------------------------------------

@@ -232,6 +232,10 @@ struct NnState {
     float ms = 0.f;            // device time of the last warm start
     hipGraphExec_t graph = nullptr;  // the warm start's kernels, captured once
     hipGraphExec_t graph_prev = nullptr;  // the same on top of the previous solution
+    // a caller's level-L prediction used in place of the net's output
+    // (sg_set_learned_prediction): padded FP64 at the level's layout
+    double* ext = nullptr;
+    int ext_on = 0;
 };
 
 struct Ctx {
@@ -347,6 +351,7 @@ int fine_residual_norms(Ctx* c, double* relres, double* maxrow);
 int learned_warm_start(Ctx* c, bool prev);   // x = Galerkin-scaled prolongated net output
 int learned_features(Ctx* c, int level, void* X, float* t, float* mask, double* scale);
 int learned_enable(Ctx* c, const float* w, int level, double out_scale, int deflate);
+int learned_set_prediction(Ctx* c, int level, const double* e_host, int deflate);
 void learned_release(Ctx* c);
 // time-step control and boundary layer (sg_step.hip)
 int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, double asy, double* dt);

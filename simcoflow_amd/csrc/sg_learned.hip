@@ -170,6 +170,16 @@ __global__ void __launch_bounds__(256) k_from_net(Dims D, const double* __restri
     e[p] = diag[p] > 0.0 ? k * sc->nn_scale * (double)y[(size_t)J * D.ni + I] : 0.0;
 }
 
+// level-L prediction supplied by the caller; solid points of the level get 0
+__global__ void __launch_bounds__(256) k_take_prediction(Dims D, const double* __restrict__ diag,
+                                                         const double* __restrict__ ext, double* __restrict__ e) {
+    const int I = blockIdx.x * TX + threadIdx.x;
+    const int J = blockIdx.y * TY + threadIdx.y;
+    if (I >= D.ni || J >= D.nj) return;
+    const size_t p = D.off(I, J);
+    e[p] = diag[p] > 0.0 ? ext[p] : 0.0;
+}
+
 // q = A w; partials of (w,b), (w,q)
 __global__ void __launch_bounds__(256) k_galerkin(Dims D, const double* __restrict__ diag,
                                                   const double* __restrict__ cE, const double* __restrict__ cN,
@@ -275,9 +285,63 @@ void learned_release(Ctx* c) {
     c->nn.on = 0;
 }
 
+namespace {
+// the fine-level buffers both the net and a caller's prediction carry to
+int learned_buffers(Ctx* c) {
+    if (!c->nn.X) {
+        // input buffers sized for the largest admissible level (the fine grid)
+        const size_t xb = (size_t)(c->nj + 2) * (c->ni + 2) * 8 * sizeof(bf16);
+        const size_t fb = c->lev[0].d.alloc_elems() * sizeof(double);
+        c->nn.X = dev_alloc(c, xb);
+        c->nn.w = (double*)dev_alloc(c, fb);
+        c->nn.aw = (double*)dev_alloc(c, fb);
+        if (!c->nn.X || !c->nn.w || !c->nn.aw) return SG_ERR_ALLOC;
+        SG_HIP(hipMemset(c->nn.X, 0, xb));
+        SG_HIP(hipMemset(c->nn.w, 0, fb));
+        SG_HIP(hipMemset(c->nn.aw, 0, fb));
+    }
+    if (!c->ev_nn0) {
+        SG_HIP(hipEventCreate(&c->ev_nn0));
+        SG_HIP(hipEventCreate(&c->ev_nn1));
+    }
+    return SG_OK;
+}
+}  // namespace
+
+int learned_set_prediction(Ctx* c, int level, const double* e_host, int deflate) {
+    if (!e_host) {
+        c->nn.ext_on = 0;
+        return SG_OK;
+    }
+    if (level < 0 || level >= c->nlev) {
+        set_error("sg_set_learned_prediction: no such level");
+        return SG_ERR_ARG;
+    }
+    int rc = learned_buffers(c);
+    if (rc) return rc;
+    if (!c->nn.ext) {
+        // sized for the fine level, so any level's prediction fits
+        const size_t fb = c->lev[0].d.alloc_elems() * sizeof(double);
+        c->nn.ext = (double*)dev_alloc(c, fb);
+        if (!c->nn.ext) return SG_ERR_ALLOC;
+    }
+    const Dims& d = c->lev[level].d;
+    SG_HIP(hipMemsetAsync(c->nn.ext, 0, d.alloc_elems() * sizeof(double), c->stream));
+    SG_HIP(hipMemcpy2DAsync(c->nn.ext + d.off(0, 0), d.ld * sizeof(double), e_host, d.ni * sizeof(double),
+                            d.ni * sizeof(double), d.nj, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    learned_release(c);  // the prediction replaces a net
+    c->nn.level = level;
+    c->nn.out_scale = 1.0;
+    c->nn.deflate = deflate ? 1 : 0;
+    c->nn.ext_on = 1;
+    return SG_OK;
+}
+
 int learned_enable(Ctx* c, const float* w, int level, double out_scale, int deflate) {
     learned_release(c);
     if (!w) return SG_OK;
+    c->nn.ext_on = 0;
     if (level < 0 || level >= c->nlev || !net_fits(c->lev[level].d)) {
         set_error("sg_set_learned: the level's grid must exist and have nj % 8 == 0, ni % 128 == 0");
         return SG_ERR_ARG;
@@ -295,21 +359,10 @@ int learned_enable(Ctx* c, const float* w, int level, double out_scale, int defl
         learned_release(c);
         return rc;
     }
-    if (!c->nn.X) {
-        // input buffers sized for the largest admissible level (the fine grid)
-        const size_t xb = (size_t)(c->nj + 2) * (c->ni + 2) * 8 * sizeof(bf16);
-        const size_t fb = c->lev[0].d.alloc_elems() * sizeof(double);
-        c->nn.X = dev_alloc(c, xb);
-        c->nn.w = (double*)dev_alloc(c, fb);
-        c->nn.aw = (double*)dev_alloc(c, fb);
-        if (!c->nn.X || !c->nn.w || !c->nn.aw) return SG_ERR_ALLOC;
-        SG_HIP(hipMemset(c->nn.X, 0, xb));
-        SG_HIP(hipMemset(c->nn.w, 0, fb));
-        SG_HIP(hipMemset(c->nn.aw, 0, fb));
-    }
-    if (!c->ev_nn0) {
-        SG_HIP(hipEventCreate(&c->ev_nn0));
-        SG_HIP(hipEventCreate(&c->ev_nn1));
+    rc = learned_buffers(c);
+    if (rc) {
+        learned_release(c);
+        return rc;
     }
     c->nn.level = level;
     c->nn.out_scale = out_scale;
@@ -351,14 +404,20 @@ int enqueue_warm_start(Ctx* c, bool prev) {
                            (const double*)c->lev[0].A.cN, (const double*)c->rhs, (const double*)c->x, c->r);
         src = c->r;
     }
-    int rc = features(c, L, (bf16*)c->nn.X, nullptr, nullptr, src);
-    if (rc) return rc;
-    rc = nn::forward(c->nn.net, (const bf16*)c->nn.X, 1);
-    if (rc) return rc;
     Level& Lv = c->lev[L];
     double* e = L ? Lv.v.x : c->nn.w;
-    hipLaunchKernelGGL(k_from_net, grid2d(Lv.d.ni, Lv.d.nj), block2d(), 0, s, Lv.d, (const double*)Lv.A.diag,
-                       (const float*)c->nn.net->y, c->nn.out_scale, (const PcgScalars*)c->sc, e);
+    if (c->nn.ext_on) {
+        // the caller's level-L prediction in place of the net's
+        hipLaunchKernelGGL(k_take_prediction, grid2d(Lv.d.ni, Lv.d.nj), block2d(), 0, s, Lv.d,
+                           (const double*)Lv.A.diag, (const double*)c->nn.ext, e);
+    } else {
+        int rc = features(c, L, (bf16*)c->nn.X, nullptr, nullptr, src);
+        if (rc) return rc;
+        rc = nn::forward(c->nn.net, (const bf16*)c->nn.X, 1);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_from_net, grid2d(Lv.d.ni, Lv.d.nj), block2d(), 0, s, Lv.d, (const double*)Lv.A.diag,
+                           (const float*)c->nn.net->y, c->nn.out_scale, (const PcgScalars*)c->sc, e);
+    }
     for (int l = L - 1; l >= 0; --l) {
         double* dst = l ? c->lev[l].v.x : c->nn.w;
         hipLaunchKernelGGL(k_prolong, grid2d(c->lev[l].d.ni, c->lev[l].d.nj), block2d(), 0, s, c->lev[l].d,
@@ -386,6 +445,13 @@ int enqueue_warm_start(Ctx* c, bool prev) {
 // call after sg_set_learned: every buffer it touches lives as long as the net.
 int learned_warm_start(Ctx* c, bool prev) {
     hipStream_t s = c->stream;
+    if (c->nn.ext_on) {
+        SG_HIP(hipEventRecord(c->ev_nn0, s));
+        const int rc = enqueue_warm_start(c, prev);
+        if (rc) return rc;
+        SG_HIP(hipEventRecord(c->ev_nn1, s));
+        return SG_OK;
+    }
     hipGraphExec_t& ge = prev ? c->nn.graph_prev : c->nn.graph;
     if (!ge) {
         hipGraph_t graph = nullptr;

@@ -351,7 +351,7 @@ Slots slots(Ctx* c, int s) {
     return Slots{c->red.partials + (size_t)s * NRED * c->red.max_blocks, c->red.max_blocks};
 }
 
-bool learned(Ctx* c) { return (c->prm.warm_start == 2 || c->prm.warm_start == 3) && c->nn.on; }
+bool learned(Ctx* c) { return (c->prm.warm_start == 2 || c->prm.warm_start == 3) && (c->nn.on || c->nn.ext_on); }
 bool deflating(Ctx* c) { return learned(c) && c->nn.deflate; }
 
 // one PCG iteration (V-cycle + direction + update), enqueued on the stream;
@@ -532,8 +532,8 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
     SG_HIP(hipMemcpyAsync(&c->sc->flexible, &c->sc_host->flexible, sizeof(int), hipMemcpyHostToDevice, s));
     int rc;
     if (c->prm.warm_start == 2 || c->prm.warm_start == 3) {
-        if (!c->nn.on) {
-            set_error("warm_start = 2 or 3 needs the learned warm start (sg_set_learned)");
+        if (!c->nn.on && !c->nn.ext_on) {
+            set_error("warm_start = 2 or 3 needs the learned warm start (sg_set_learned or sg_set_learned_prediction)");
             return SG_ERR_ARG;
         }
         rc = learned_warm_start(c, c->prm.warm_start == 3);

@@ -314,6 +314,30 @@ def load(path):
         return d["weights"].astype(np.float32), int(d["level"]), float(d["out_scale"])
 
 
+def set_prediction(pj, level, e, deflate=False):
+    """A level-`level` prediction (array of the level's (H, W) points) in
+    place of the net's output for warm_start 2 / 3; None switches it off."""
+    if e is None:
+        check(pj.lib.sg_set_learned_prediction(pj.ctx, int(level), None, 0))
+        return
+    H, W = level_dims(pj, level)
+    a = np.ascontiguousarray(e, dtype=np.float64)
+    if a.shape != (H, W):
+        raise ValueError(f"prediction of shape {a.shape}, level {level} has {(H, W)}")
+    check(pj.lib.sg_set_learned_prediction(pj.ctx, int(level), a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           int(bool(deflate))))
+
+
+def sample_at_level(pj, level, fine):
+    """The fine cell array sampled at a level's points the way the training
+    targets are: the point covering the block's first cell."""
+    H, W = level_dims(pj, level)
+    nj, ni = fine.shape
+    sy = nj // H if H else 1
+    sx = max(1, ni // W)
+    return np.ascontiguousarray(fine[::sy, ::sx][:H, :W])
+
+
 def enable(pj, weights, level, out_scale, deflate=True):
     """Switch a Projection to the learned warm start (solves with
     warm_start = 2 use it)."""

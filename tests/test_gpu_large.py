@@ -7,8 +7,9 @@
   1/512 m, 8.4 M cells, an ellipse at the surface), mixed precision,
   geometry built on the device: 4 full steps with 10^5 tracked particles
   (log-normal d about 50 um) and a diagnostics row per step; every solve
-  converges, the liquid volume holds to 1e-5 relative (1.7e-6 absolute on
-  15.9 m^2 measured), the mass residual stays small, no particle fails."""
+  converges, the liquid volume holds to 1e-6 relative (1e-7 measured), the
+  mass residual stays small, no particle fails.
+* C4 for 100 steps, learned start in mixed precision against plain FP64."""
 import numpy as np
 import pytest
 
@@ -65,7 +66,7 @@ def test_largest_grid_with_particles():
     # solid-cell quirks of X_Sweep / Y_Sweep (kept bit for bit) move the
     # liquid volume by ~1e-7 relative over these steps; no mass is lost on a
     # larger scale
-    assert np.abs(vol - vol[0]).max() <= 1e-5 * vol[0]
+    assert np.abs(vol - vol[0]).max() <= 1e-6 * vol[0]
     assert np.all(h["max_abs_mres"] < 1e-6)
     assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
 
@@ -77,7 +78,10 @@ def test_largest_grid_100_steps_learned_vs_fp64():
     pressure: warm_start = 3), and plain FP64 from x0 = 0.  Every solve
     converges to 1e-8; the two runs' liquid volumes agree step by step and the
     final interfaces agree, within bounds a little above the measured
-    differences; no particle fails in either run."""
+    differences (volumes 8e-13 relative apart, final vof 7.5e-10 apart on
+    average and 2.5e-3 at the worst cell, particles 7e-8 m; each run's own
+    volume drifts by 1.9e-7 over the 100 steps); no particle fails in either
+    run."""
     from simcoflow_amd import learned as Lr
     from simcoflow_amd.solver import TimeStepper
     print("[large] C4: training pairs from the 4096x2048 tank's first steps", flush=True)
@@ -123,7 +127,8 @@ def test_largest_grid_100_steps_learned_vs_fp64():
           f"{dvof.max():.2e} / mean {dvof.mean():.2e}, particle positions {dpart:.2e} m", flush=True)
     assert a["vol"].size == 100 and b["vol"].size == 100
     for r in (a, b):
-        assert np.abs(r["vol"] - r["vol"][0]).max() <= 1e-5 * r["vol"][0]
+        assert np.abs(r["vol"] - r["vol"][0]).max() <= 1e-6 * r["vol"][0]
         assert np.all(r["mres"] < 1e-6)
-    assert dvol < 1e-6
-    assert dvof.max() < 0.5 and dvof.mean() < 1e-4
+    assert dvol < 1e-10
+    assert dvof.max() < 0.05 and dvof.mean() < 1e-8
+    assert dpart < 1e-5

@@ -221,3 +221,79 @@ def test_learned_correction_of_previous_pressure():
     print(f"iterations zero {out['zero'][1]}, learned+previous {out['learned_prev'][1]}; pressure rel L2 {dp:.2e}")
     assert dp < 1e-6
     assert np.abs(a["vof"] - b["vof"]).max() < 1e-6
+
+
+def test_learned_start_upper_bound():
+    """What any predictor on the net's level could gain, on the 2048^2
+    swarm the bench runs: the exact pressure (zero start) or the exact
+    correction of the previous pressure (warm_start 3), sampled at the
+    level's points the way the training targets are and handed to the solve
+    in place of the net's output (sg_set_learned_prediction).  Prolongated
+    and Galerkin-scaled, even these save at most one PCG iteration on the
+    256^2 level the net runs on -- the V-cycle's coarse correction already
+    carries what a coarse-level prediction holds -- so a net there cannot
+    pay for its own run.  The first coarse level (1024^2) is reported: there
+    the exact prediction saves a few iterations, at 16x the net's cost."""
+    import ctypes
+    from simcoflow_amd import _lib
+    from simcoflow_amd import cases as C
+    from simcoflow_amd import learned as Lr
+    from simcoflow_amd.projection import Projection, params_for
+    case = C.case_swarm(2048, 2048, seed=11)
+    g = case.grid
+    _, _, DpU, DpV = C.predictor_fields(case, seed=18)
+    gen = torch.Generator(device="cpu")
+    gen.manual_seed(5)
+    u1, v1 = Lr.smooth_velocities(torch, (g.nj + 2, g.ni + 2), case.fluids.g * case.dt, gen)
+    du, dv = Lr.smooth_velocities(torch, (g.nj + 2, g.ni + 2), case.fluids.g * case.dt, gen)
+    u2, v2 = u1 + 0.1 * du, v1 + 0.1 * dv
+    pj = Projection(g, params_for(case.fluids, tol=1e-8, max_iter=300))
+    try:
+        pj.set_geometry(case.geo)
+        pj.set_interface(case.itf.vof, case.itf.phi)
+        pj.upload("DPU", DpU)
+        pj.upload("DPV", DpV)
+
+        def upload(u, v):
+            torch.cuda.synchronize()
+            _lib.check(pj.lib.sg_upload_device(pj.ctx, _lib.F["UPRED"], ctypes.c_void_p(u.data_ptr())))
+            _lib.check(pj.lib.sg_upload_device(pj.ctx, _lib.F["VPRED"], ctypes.c_void_p(v.data_ptr())))
+
+        def solve(tol=1e-8, ws=0):
+            pj.set_params(params_for(case.fluids, tol=tol, max_iter=300, warm_start=ws))
+            pj.build(0.0)
+            info = pj.solve()
+            assert info.status == 0
+            return info.iterations, pj.download("PP")[1:-1, 1:-1].copy()
+        net_level = Lr.pick_level(pj, 256)
+        assert Lr.level_dims(pj, net_level) == (256, 256)
+        upload(u2, v2)
+        _, x2 = solve(1e-12)
+        upload(u1, v1)
+        _, x1 = solve(1e-12)
+        zero, _ = solve()
+        out = {}
+        for lv in (net_level, 1):
+            Lr.set_prediction(pj, lv, Lr.sample_at_level(pj, lv, x1))
+            out[("pressure", lv)], _ = solve(ws=2)
+            # previous pressure x1 (in the solver's start vector), then the
+            # exact correction x2 - x1 for the second system
+            Lr.set_prediction(pj, lv, Lr.sample_at_level(pj, lv, x2 - x1))
+            upload(u1, v1)
+            solve(1e-12)
+            upload(u2, v2)
+            out[("correction", lv)], _ = solve(ws=3)
+            Lr.set_prediction(pj, lv, None)
+        upload(u1, v1)
+        solve(1e-12)
+        upload(u2, v2)
+        prev, _ = solve(ws=1)
+    finally:
+        pj.close()
+    print(f"[nn] bound on the 2048^2 swarm: zero start {zero}, exact pressure at level {net_level} "
+          f"{out[('pressure', net_level)]}, at level 1 {out[('pressure', 1)]}; previous pressure {prev}, exact "
+          f"correction at level {net_level} {out[('correction', net_level)]}, at level 1 {out[('correction', 1)]}",
+          flush=True)
+    assert out[("pressure", net_level)] >= zero - 1
+    assert out[("correction", net_level)] >= prev - 1
+    assert out[("pressure", 1)] < zero
