@@ -664,8 +664,9 @@ __global__ void __launch_bounds__(64 * RECON_TY) k_recon(ClsGeo G, const double*
 __global__ void __launch_bounds__(256) k_recon_ifc(ClsGeo G, const double* __restrict__ vof,
                                                    const double* __restrict__ phi, Planes P,
                                                    unsigned char* __restrict__ stale, const int* __restrict__ ifc,
-                                                   int* __restrict__ stales, unsigned* __restrict__ cnt) {
-    const unsigned n = cnt[2];
+                                                   const unsigned* __restrict__ nifc, int* __restrict__ stales,
+                                                   unsigned* __restrict__ cnt) {
+    const unsigned n = *nifc;
     for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const unsigned t = base + threadIdx.x;
         bool st = false;
@@ -1193,18 +1194,35 @@ __global__ void __launch_bounds__(256) k_rd_scatter(ClsGeo G, const double* __re
     }
 }
 
+// The band bitmap of k_redistance: word (w, j) is nonzero when an interface
+// cell lies in rows j-BAND..j+BAND of mask words w-1..w+1, a superset of the
+// windows of the 64 cells of word (w, j) (BAND < 64)
+__global__ void __launch_bounds__(256) k_rd_dilate(int nj, int mw, const unsigned long long* __restrict__ mask,
+                                                   unsigned long long* __restrict__ dil, int* __restrict__ band,
+                                                   unsigned* __restrict__ nband) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= mw * nj) return;
+    const int w = k % mw, j = k / mw;
+    unsigned long long any = 0ull;
+    for (int r = max(j - BAND, 0); r <= min(j + BAND, nj - 1); ++r) {
+        const unsigned long long* row = mask + (size_t)r * mw;
+        any |= row[w];
+        if (w > 0) any |= row[w - 1];
+        if (w + 1 < mw) any |= row[w + 1];
+    }
+    dil[k] = any;
+    if (any) band[atomicAdd(nband, 1u)] = k;
+}
+
 // Redistance, per cell: interface cells take their plane constant, cells
 // reached from an interface cell the nearest distance, the rest the band
 // value.  Full-solid targets replay their visits in the reference's source
-// order (i ascending, then j ascending).
-__global__ void __launch_bounds__(256) k_redistance(ClsGeo G, const double* __restrict__ vof, Planes P,
-                                                    const unsigned long long* __restrict__ mask, int mw,
-                                                    const unsigned long long* __restrict__ best,
-                                                    const unsigned char* __restrict__ hit,
-                                                    double* __restrict__ phi) {
-    const int ti = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int tj = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (ti > G.ni || tj > G.nj) return;
+// order (i ascending, then j ascending).  `near`: the cell's band bitmap
+// word is nonzero (else no interface cell lies in its window).
+__device__ void redistance_cell(const ClsGeo& G, const double* __restrict__ vof, const Planes& P,
+                                const unsigned long long* __restrict__ mask, int mw,
+                                const unsigned long long* __restrict__ best, const unsigned char* __restrict__ hit,
+                                double* __restrict__ phi, int ti, int tj, bool near) {
     const size_t c = cix(G, ti, tj);
     Target T;
     T.v = vof[c];
@@ -1218,7 +1236,7 @@ __global__ void __launch_bounds__(256) k_redistance(ClsGeo G, const double* __re
     }
     double pa = 1.0e4;
     bool visited = false;
-    if (rd_target(T.v, T.s)) {
+    if (rd_target(T.v, T.s) && near) {
         const int ilo = max(ti - BAND, 1), ihi = min(ti + BAND, G.ni);
         const int jlo = max(tj - BAND, 1), jhi = min(tj + BAND, G.nj);
         // window bits per row: bit k <-> source column ilo+k
@@ -1266,6 +1284,42 @@ __global__ void __launch_bounds__(256) k_redistance(ClsGeo G, const double* __re
     if (T.v <= VOFEPS)
         phi[c] = (double)(BAND + 1) * sqrt(hx * hx + hy * hy);
     else if (T.v >= (1.0 - VOFEPS - T.s))
+        phi[c] = -(double)(BAND + 1) * sqrt(hx * hx + hy * hy);
+}
+
+// the cells of the listed band words, one wavefront per word (the window
+// scans and nearest-distance gathers; few words, many registers)
+__global__ void __launch_bounds__(256) k_redistance_band(ClsGeo G, const double* __restrict__ vof, Planes P,
+                                                         const unsigned long long* __restrict__ mask, int mw,
+                                                         const int* __restrict__ band,
+                                                         const unsigned* __restrict__ nband,
+                                                         const unsigned long long* __restrict__ best,
+                                                         const unsigned char* __restrict__ hit,
+                                                         double* __restrict__ phi) {
+    const unsigned n = *nband;
+    const int lane = threadIdx.x & 63;
+    for (unsigned e = blockIdx.x * 4 + (threadIdx.x >> 6); e < n; e += gridDim.x * 4) {
+        const int k = band[e];
+        const int ti = (k % mw) * 64 + lane + 1, tj = k / mw + 1;
+        if (ti <= G.ni) redistance_cell(G, vof, P, mask, mw, best, hit, phi, ti, tj, true);
+    }
+}
+
+// every other cell: no interface cell in its window, so it takes the band
+// value (or keeps its own level set), with the occupancy of a light kernel
+__global__ void __launch_bounds__(256) k_redistance_far(ClsGeo G, const double* __restrict__ vof,
+                                                        const unsigned long long* __restrict__ dil, int mw,
+                                                        double* __restrict__ phi) {
+    const int ti = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int tj = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (ti > G.ni || tj > G.nj) return;
+    if (dil[(size_t)(tj - 1) * mw + ((ti - 1) >> 6)] != 0ull) return;
+    const size_t c = cix(G, ti, tj);
+    const double v = vof[c], s = G.vofS[c];
+    const double hx = G.dx[ti - 1], hy = G.dy[tj - 1];
+    if (v <= VOFEPS)
+        phi[c] = (double)(BAND + 1) * sqrt(hx * hx + hy * hy);
+    else if (v >= (1.0 - VOFEPS - s))
         phi[c] = -(double)(BAND + 1) * sqrt(hx * hx + hy * hy);
 }
 
@@ -1458,8 +1512,10 @@ int clsvof_allocate(Ctx* c) {
     B.fl = (double*)dev_alloc(c, n * sizeof(double));
     B.cnt = (unsigned*)dev_alloc(c, ClsBuffers::CNT_SLOTS * sizeof(unsigned));
     B.mask = (unsigned long long*)dev_alloc(c, (size_t)B.mw * c->nj * sizeof(unsigned long long));
+    B.dil = (unsigned long long*)dev_alloc(c, (size_t)B.mw * c->nj * sizeof(unsigned long long));
+    B.band = (int*)dev_alloc(c, (size_t)B.mw * c->nj * sizeof(int));
     if (!B.xc || !B.yc || !B.dis || !B.vt || !B.pt || !B.tv || !B.tl || !B.stale || !B.hit || !B.best || !B.srcs ||
-        !B.stales || !B.ifc || !B.fr || !B.fl || !B.cnt || !B.mask)
+        !B.stales || !B.ifc || !B.fr || !B.fl || !B.cnt || !B.mask || !B.dil || !B.band)
         return SG_ERR_ALLOC;
     return SG_OK;
 }
@@ -1482,27 +1538,35 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
     const dim3 sblk(32, 8), sgrd((c->ni + 31) / 32, (c->nj + 7) / 8);
     const double dtv = dt / (double)nsub;
     hipStream_t st = c->stream;
-    // three counters per reconstruction, three reconstructions per sub-step
-    const int slots = 9 * nsub;
+    // three counters per reconstruction, three reconstructions per sub-step,
+    // then one band-word count per sub-step
+    const int slots = 10 * nsub;
     if (slots > ClsBuffers::CNT_SLOTS) {
         set_error("clsvof_advect: too many sub-steps");
         return SG_ERR_ARG;
     }
     hipLaunchKernelGGL(k_zero, dim3((slots + 255) / 256), dim3(256), 0, st, B.cnt, slots);
     int slot = 0;
-    auto recon = [&](const double* v, const double* p, bool last) {
+    // `scan`: the interface list of the last full-grid scan, when vof and
+    // vofS have not changed since (re-distancing changes phi only)
+    auto recon = [&](const double* v, const double* p, bool last, const unsigned* scan) {
         unsigned* cnt = B.cnt + slot;
         slot += 3;
-        hipLaunchKernelGGL(k_recon, rcgrd, rcblk, 0, st, G, v, P, last ? 1 : 0,
-                           last ? B.mask : (unsigned long long*)nullptr, B.mw, B.srcs, B.ifc, cnt);
-        hipLaunchKernelGGL(k_recon_ifc, dim3(256), pblk, 0, st, G, v, p, P, B.stale, B.ifc, B.stales, cnt);
+        if (!scan)
+            hipLaunchKernelGGL(k_recon, rcgrd, rcblk, 0, st, G, v, P, last ? 1 : 0,
+                               last ? B.mask : (unsigned long long*)nullptr, B.mw, B.srcs, B.ifc, cnt);
+        const unsigned* nifc = scan ? scan + 2 : cnt + 2;
+        hipLaunchKernelGGL(k_recon_ifc, dim3(256), pblk, 0, st, G, v, p, P, B.stale, B.ifc, nifc, B.stales, cnt);
         hipLaunchKernelGGL(k_dis_fix, dim3(64), pblk, 0, st, G, v, B.stale, B.stales, cnt, B.dis);
-        return cnt;
+        // the interface count (entry 2) the flux precomputation reads
+        return scan ? scan : (const unsigned*)cnt;
     };
+    const unsigned* last_scan = nullptr;  // the previous sub-step's final scan
     for (int k = 1; k <= nsub; ++k) {
         const double level = levels[k - 1];
-        // first sweep: (vof, phi) -> (vt, pt), implicit divergence
-        const unsigned* c1 = recon(vof, phi, false);
+        // first sweep: (vof, phi) -> (vt, pt), implicit divergence; vof is
+        // the one the previous sub-step's last reconstruction scanned
+        const unsigned* c1 = recon(vof, phi, false, last_scan);
         const SweepIn s1{vof, phi, P, B.fr, B.fl};
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_flux_pre<true>), pgrd, pblk, 0, st, G, s1, dtv, B.ifc, c1, B.fr, B.fl);
@@ -1516,7 +1580,7 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
                                B.tl, B.vt, B.pt);
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, B.vt, B.pt);
         // second sweep: (vt, pt) -> (vof, phi), explicit divergence
-        const unsigned* c2 = recon(B.vt, B.pt, false);
+        const unsigned* c2 = recon(B.vt, B.pt, false, nullptr);
         const SweepIn s2{B.vt, B.pt, P, B.fr, B.fl};
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_flux_pre<false>), pgrd, pblk, 0, st, G, s2, dtv, B.ifc, c2, B.fr, B.fl);
@@ -1541,11 +1605,17 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         }
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, vof, phi);
         // reconstruction (full normals out) and re-distancing
-        unsigned* cnt = recon(vof, phi, true);
+        const unsigned* cnt = recon(vof, phi, true, nullptr);
+        last_scan = cnt;
         const PairView pv{B.srcs, cnt};
         hipLaunchKernelGGL(k_rd_reset, pgrd, pblk, 0, st, G, vof, pv, B.best, B.hit);
         hipLaunchKernelGGL(k_rd_scatter, pgrd, pblk, 0, st, G, vof, phi, P, pv, B.best, B.hit);
-        hipLaunchKernelGGL(k_redistance, grd, blk, 0, st, G, vof, P, B.mask, B.mw, B.best, B.hit, phi);
+        unsigned* nband = B.cnt + 9 * nsub + (k - 1);
+        hipLaunchKernelGGL(k_rd_dilate, dim3((unsigned)((B.mw * c->nj + 255) / 256)), dim3(256), 0, st, c->nj, B.mw,
+                           (const unsigned long long*)B.mask, B.dil, B.band, nband);
+        hipLaunchKernelGGL(k_redistance_far, grd, blk, 0, st, G, vof, (const unsigned long long*)B.dil, B.mw, phi);
+        hipLaunchKernelGGL(k_redistance_band, dim3(2048), dim3(256), 0, st, G, vof, P, B.mask, B.mw,
+                           (const int*)B.band, (const unsigned*)nband, B.best, B.hit, phi);
     }
     SG_HIP(hipGetLastError());
     return SG_OK;

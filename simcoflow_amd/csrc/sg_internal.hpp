@@ -163,6 +163,10 @@ struct ClsBuffers {
     unsigned* cnt = nullptr;           // list lengths, three per reconstruction
     static constexpr int CNT_SLOTS = 6 * 1024;
     unsigned long long* mask = nullptr;  // interface cells, 1 bit per cell
+    // per mask word: any interface cell within the band of its 64 cells
+    // (the OR over rows j-BAND..j+BAND of words w-1..w+1)
+    unsigned long long* dil = nullptr;
+    int* band = nullptr;               // the nonzero words of dil (w + j * mw)
     int mw = 0;                        // mask words per row
     double xw = 0.0;                   // west edge x of the grid (host copy)
     double dx0 = 0.0;                  // width of the first column (host copy)

@@ -2,9 +2,9 @@
 // SetMatrix, PredictorUV_mod.f90:622-768) on gfx950, FP64 throughout, all
 // scalars on the device.  The rows are dominated by their dx dy / dt
 // diagonals (off-diagonal sums ~1e-4..1e-2 of the diagonal), so plain Jacobi
-// sweeps -- one pass over the rows each, no other vectors -- reach the
-// tolerance in a handful of sweeps; BiCGStab with the Jacobi preconditioner
-// takes over if they have not after JACOBI_MAX sweeps.
+// sweeps reach the tolerance in a handful of sweeps -- four per pass over
+// the rows (temporally blocked in LDS), no other vectors; BiCGStab with the
+// Jacobi preconditioner takes over if they have not after JACOBI_MAX sweeps.
 //
 // The rows are diagonally dominant (dx dy / dt on the diagonal) but not
 // symmetric where the cut-cell cross-diffusion terms enter, so the
@@ -149,22 +149,136 @@ __global__ void __launch_bounds__(NT) k_bi_update(size_t n, Mat5 A, const BiScal
     block_partials<1, NT>(a, part, PredBuffers::BI_BLOCKS);
 }
 
-enum { F_INIT = 0, F_RHO, F_ALPHA, F_OMEGA, F_RR, F_JINIT };
+enum { F_INIT = 0, F_RHO, F_ALPHA, F_OMEGA, F_RR };
 
-// One Jacobi sweep xout = xin + D^-1 (b - A xin) (xin = 0 on the first);
-// partials of ||D^-1 (b - A xin)||^2, the Jacobi-scaled residual of xin.
-__global__ void __launch_bounds__(NT) k_jac(size_t n, Mat5 A, const double* __restrict__ b,
-                                            const double* __restrict__ xin, double* __restrict__ xout,
-                                            double* __restrict__ part, const BiScalars* __restrict__ sc, int first) {
+// JB Jacobi sweeps x_{k+1} = x_k + D^-1 (b - A x_k) per launch (x_0 = 0 on
+// the first launch), temporally blocked: a 64 x 16 tile stages x_k on the
+// tile plus a JB-cell halo in LDS and sweeps it JB times, each sweep on a
+// ring narrower by one, so the rows' coefficients are read from HBM once per
+// JB sweeps instead of once per sweep.  Each thread holds the coefficients of
+// its cells in registers.  Every sweep evaluates the row as apply_row does
+// (the same terms in the same order), so the iterates equal single sweeps'
+// bit for bit.  Partials: sweep q's ||D^-1 (b - A x_k)||^2 over the tiles'
+// own cells, in plane q.
+constexpr int JB = 4;                                       // sweeps per launch
+constexpr int JTW = 64, JTH = 16;                           // tile
+constexpr int JRW = JTW + 2 * JB, JRH = JTH + 2 * JB;       // x_k staged: tile + JB halo
+constexpr int JCW = JRW - 2, JCH = JRH - 2;                 // cells the first sweep updates
+constexpr int JM = (JCW * JCH + NT - 1) / NT;               // of them per thread
+static_assert(JB <= 4, "the momentum partials buffer holds 4 planes");
+
+__global__ void __launch_bounds__(NT) k_jac_blk(Mat5 A, const double* __restrict__ b, const double* __restrict__ xin,
+                                                double* __restrict__ xout, double* __restrict__ part,
+                                                const BiScalars* __restrict__ sc, int first) {
     if (!first && !sc->active) return;
-    double a[1] = {0.0};
-    for (size_t c = blockIdx.x * (size_t)NT + threadIdx.x; c < n; c += (size_t)gridDim.x * NT) {
-        const double r = first ? b[c] : b[c] - apply_row(A, xin, c);
-        const double dx = r * dinv(A, c);
-        xout[c] = first ? dx : xin[c] + dx;
-        a[0] += dx * dx;
+    __shared__ double sx[2][JRH * JRW];
+    const int ni = A.ni, nj = A.nj;
+    const int tnx = (ni + JTW - 1) / JTW, ntiles = tnx * ((nj + JTH - 1) / JTH);
+    const int tid = threadIdx.x;
+    double acc[JB];
+#pragma unroll
+    for (int q = 0; q < JB; ++q) acc[q] = 0.0;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int i0 = (tile % tnx) * JTW, j0 = (tile / tnx) * JTH;
+        for (int k = tid; k < JRW * JRH; k += NT) {
+            const int gi = i0 + k % JRW - JB, gj = j0 + k / JRW - JB;
+            double v = 0.0;
+            if (!first && gi >= 0 && gj >= 0 && gi < ni && gj < nj) v = xin[(size_t)gj * ni + gi];
+            sx[0][k] = v;
+            sx[1][k] = v;
+        }
+        double cd[JM], cs[JM], cw[JM], ce[JM], cn[JM], cb[JM], ci[JM];
+        int pos[JM], gij[JM];
+#pragma unroll
+        for (int m = 0; m < JM; ++m) {
+            const int k = tid + m * NT;
+            pos[m] = -1;
+            cd[m] = cs[m] = cw[m] = ce[m] = cn[m] = cb[m] = ci[m] = 0.0;
+            gij[m] = 0;
+            if (k >= JCW * JCH) continue;
+            const int li = k % JCW + 1, lj = k / JCW + 1;
+            const int gi = i0 + li - JB, gj = j0 + lj - JB;
+            if (gi < 0 || gj < 0 || gi >= ni || gj >= nj) continue;
+            const size_t c = (size_t)gj * ni + gi;
+            cd[m] = A.d[c];
+            cs[m] = A.s[c];
+            cw[m] = A.w[c];
+            ce[m] = A.e[c];
+            cn[m] = A.n[c];
+            cb[m] = b[c];
+            ci[m] = dinv(A, c);
+            pos[m] = lj * JRW + li;
+            // domain-edge flags of apply_row: bit 0 j > 0, 1 i > 0, 2 i < ni-1, 3 j < nj-1
+            gij[m] = (gj > 0 ? 1 : 0) | (gi > 0 ? 2 : 0) | (gi < ni - 1 ? 4 : 0) | (gj < nj - 1 ? 8 : 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < JB; ++q) {
+            const double* xo = sx[q & 1];
+            double* xn = sx[(q + 1) & 1];
+            const int ring = q + 1;  // sweep q updates [ring, JRW - ring) x [ring, JRH - ring)
+            const bool zero_start = first && q == 0;
+#pragma unroll
+            for (int m = 0; m < JM; ++m) {
+                const int p = pos[m];
+                if (p < 0) continue;
+                const int li = p % JRW, lj = p / JRW;
+                if (li < ring || li >= JRW - ring || lj < ring || lj >= JRH - ring) continue;
+                double r = cb[m];
+                if (!zero_start) {
+                    double y = cd[m] * xo[p];
+                    if (gij[m] & 1) y += cs[m] * xo[p - JRW];
+                    if (gij[m] & 2) y += cw[m] * xo[p - 1];
+                    if (gij[m] & 4) y += ce[m] * xo[p + 1];
+                    if (gij[m] & 8) y += cn[m] * xo[p + JRW];
+                    r = cb[m] - y;
+                }
+                const double dx = r * ci[m];
+                xn[p] = zero_start ? dx : xo[p] + dx;
+                if (li >= JB && li < JB + JTW && lj >= JB && lj < JB + JTH) acc[q] += dx * dx;
+            }
+            __syncthreads();
+        }
+        // x after the JB sweeps: the tile's cells, from buffer JB & 1
+        for (int k = tid; k < JTW * JTH; k += NT) {
+            const int li = k % JTW + JB, lj = k / JTW + JB;
+            const int gi = i0 + li - JB, gj = j0 + lj - JB;
+            if (gi < ni && gj < nj) xout[(size_t)gj * ni + gi] = sx[JB & 1][lj * JRW + li];
+        }
+        __syncthreads();
     }
-    block_partials<1, NT>(a, part, PredBuffers::BI_BLOCKS);
+    block_partials<JB, NT>(acc, part, PredBuffers::BI_BLOCKS);
+}
+
+// the JB residual partials of one k_jac_blk launch: sweep by sweep, the first
+// of the first launch sets ||D^-1 b||^2; the sweep that meets tol ends the
+// solve (the launch's later sweeps have run: x is that much closer)
+__global__ void __launch_bounds__(NT) k_jac_fin(const double* __restrict__ part, int nb, BiScalars* __restrict__ sc,
+                                                double tol, int first) {
+    if (!first && !sc->active) return;
+    double s[JB];
+    finalize_sum<JB, NT>(part, PredBuffers::BI_BLOCKS, nb, s);
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < JB; ++q) {
+        if (first && q == 0) {
+            sc->bb = s[0];
+            sc->rr = s[0];
+            sc->tol2 = tol * tol * s[0];
+            sc->active = s[0] > sc->tol2 ? 1 : 0;
+            sc->iters = 1;
+            sc->nan = s[0] != s[0];  // a NaN in b is not b = 0
+            if (sc->nan) sc->active = 0;
+            continue;
+        }
+        if (!sc->active) break;
+        sc->rr = s[q];
+        sc->iters += 1;
+        if (!(s[q] > sc->tol2)) sc->active = 0;
+        if (s[q] != s[q]) {
+            sc->active = 0;
+            sc->nan = 1;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(NT) k_copy_vec(size_t n, const double* __restrict__ a, double* __restrict__ b) {
@@ -173,7 +287,7 @@ __global__ void __launch_bounds__(NT) k_copy_vec(size_t n, const double* __restr
 
 __global__ void __launch_bounds__(NT) k_bi_fin(const double* __restrict__ part, int nb, int mode,
                                                BiScalars* __restrict__ sc, double tol) {
-    if (mode != F_INIT && mode != F_JINIT && !sc->active) return;
+    if (mode != F_INIT && !sc->active) return;
     double s[2];
     if (mode == F_OMEGA) {
         finalize_sum<2, NT>(part, PredBuffers::BI_BLOCKS, nb, s);
@@ -184,14 +298,6 @@ __global__ void __launch_bounds__(NT) k_bi_fin(const double* __restrict__ part, 
     }
     if (threadIdx.x != 0) return;
     switch (mode) {
-    case F_JINIT:  // the first Jacobi sweep: ||D^-1 b||^2
-        sc->bb = s[0];
-        sc->rr = s[0];
-        sc->tol2 = tol * tol * s[0];
-        sc->active = s[0] > sc->tol2 ? 1 : 0;
-        sc->iters = 1;
-        sc->nan = s[0] != s[0];  // a NaN in b is not b = 0
-        break;
     case F_INIT:
         sc->bb = s[0];
         sc->rr = s[0];
@@ -235,33 +341,36 @@ namespace {
 
 constexpr int JACOBI_MAX = 40;
 
-// Jacobi sweeps until ||D^-1 (b - A x_k)|| <= tol ||D^-1 b|| (the result is
-// x_{k+1}); 1 when they have not converged within JACOBI_MAX sweeps
+// Jacobi sweeps until ||D^-1 (b - A x_k)|| <= tol ||D^-1 b||, JB per launch
+// (the result is x after the launch in which that held); 1 when they have
+// not converged within JACOBI_MAX sweeps
 int jacobi_solve(Ctx* c, const Mat5& A, const double* b, double* x, double tol, int* iters, double* relres,
                  int* status) {
     const size_t n = (size_t)c->ni * c->nj;
     PredBuffers& B = c->pred;
     double* y = B.vec[0];  // the other buffer of the ping-pong with x
-    const int nb = (int)std::min<size_t>((n + NT - 1) / NT, (size_t)PredBuffers::BI_BLOCKS);
+    const int ntiles = ((c->ni + JTW - 1) / JTW) * ((c->nj + JTH - 1) / JTH);
+    const int nb = std::min(ntiles, PredBuffers::BI_BLOCKS);
     const dim3 g(nb), blk(NT);
     hipStream_t st = c->stream;
     double* part = B.partials;
     BiScalars* sc = B.sc;
-    // sweep k (1-based) writes x when k is odd, y when even
-    hipLaunchKernelGGL(k_jac, g, blk, 0, st, n, A, b, (const double*)y, x, part, (const BiScalars*)sc, 1);
-    hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_JINIT, sc, tol);
-    // the first host check after the sweep count these rows typically need
-    // (7 at 2048^2; sweeps past convergence return at once)
-    int k = 1, batch = 7;
+    // launch L (0-based) writes x when L is even, y when odd
+    hipLaunchKernelGGL(k_jac_blk, g, blk, 0, st, A, b, (const double*)y, x, part, (const BiScalars*)sc, 1);
+    hipLaunchKernelGGL(k_jac_fin, dim3(1), blk, 0, st, (const double*)part, nb, sc, tol, 1);
+    // the first host check after the sweeps these rows typically need (7-8
+    // at 2048^2); launches past convergence return at once
+    int L = 1, batch = 1;
+    constexpr int MAX_LAUNCHES = JACOBI_MAX / JB;
     for (;;) {
-        for (int q = 0; q < batch && k < JACOBI_MAX; ++q, ++k) {
-            const bool odd = ((k + 1) & 1) != 0;  // sweep k+1
-            hipLaunchKernelGGL(k_jac, g, blk, 0, st, n, A, b, (const double*)(odd ? y : x), odd ? x : y, part,
+        for (int q = 0; q < batch && L < MAX_LAUNCHES; ++q, ++L) {
+            const bool odd = (L & 1) != 0;
+            hipLaunchKernelGGL(k_jac_blk, g, blk, 0, st, A, b, (const double*)(odd ? x : y), odd ? y : x, part,
                                (const BiScalars*)sc, 0);
-            hipLaunchKernelGGL(k_bi_fin, dim3(1), blk, 0, st, part, nb, (int)F_RR, sc, 0.0);
+            hipLaunchKernelGGL(k_jac_fin, dim3(1), blk, 0, st, (const double*)part, nb, sc, 0.0, 0);
         }
         SG_HIP(hipGetLastError());
-        batch = 2;
+        batch = 1;
         SG_HIP(hipMemcpyAsync(B.sc_host, sc, sizeof(BiScalars), hipMemcpyDeviceToHost, st));
         SG_HIP(hipStreamSynchronize(st));
         const BiScalars& h = *B.sc_host;
@@ -271,16 +380,18 @@ int jacobi_solve(Ctx* c, const Mat5& A, const double* b, double* x, double tol, 
             return 0;
         }
         if (!h.active) {
-            // converged at sweep h.iters (its result is in x when odd)
-            if ((h.iters & 1) == 0)
-                hipLaunchKernelGGL(k_copy_vec, g, blk, 0, st, n, (const double*)y, x);
+            // met tol in launch (iters - 1) / JB, whose x is in y when that is odd
+            if ((((h.iters - 1) / JB) & 1) != 0)
+                hipLaunchKernelGGL(k_copy_vec, dim3((unsigned)std::min<size_t>((n + NT - 1) / NT,
+                                                                               (size_t)PredBuffers::BI_BLOCKS)),
+                                   blk, 0, st, n, (const double*)y, x);
             SG_HIP(hipGetLastError());
-            *iters = h.iters;
+            *iters = (((h.iters - 1) / JB) + 1) * JB;
             *relres = h.bb > 0.0 ? std::sqrt(h.rr / h.bb) : 0.0;
             *status = SG_OK;
             return 0;
         }
-        if (k >= JACOBI_MAX) return 1;
+        if (L >= MAX_LAUNCHES) return 1;
     }
 }
 
