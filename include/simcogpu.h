@@ -440,6 +440,12 @@ int sg_state_load(sg_ctx* ctx, const void* buf, size_t bytes);
  * (writes reps + 1 rows).  0 needs the operator built, 1-3 one solve, 4 one
  * tracking call.  No counterpart in the reference (it has no timers). */
 int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms);
+/* Live timing of the interface transport's sweep launches (X_Sweep /
+ * Y_Sweep of Coupled_LS_VOF): on = 1 clears the totals and brackets every
+ * sweep launch of the calls that follow with HIP events on the context's
+ * stream; on = 0 waits for the recorded launches and returns their number and
+ * summed duration.  No counterpart in the reference (measurement). */
+int sg_sweep_timing(sg_ctx* ctx, int on, long long* launches, double* total_ms);
 
 #ifdef __cplusplus
 }

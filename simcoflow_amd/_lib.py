@@ -177,6 +177,7 @@ PROTOTYPES = {
     "sg_nn_features": ([_CTX, ctypes.c_int, _V, _V, _V, _D], ctypes.c_int),
     "sg_set_learned": ([_CTX, _F, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
     "sg_set_learned_prediction": ([_CTX, ctypes.c_int, _D, ctypes.c_int], ctypes.c_int),
+    "sg_sweep_timing": ([_CTX, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _D], ctypes.c_int),
 }
 
 _lib = None

@@ -49,6 +49,8 @@ static void destroy(Ctx* c) {
     learned_release(c);
     if (c->ev_nn0) (void)hipEventDestroy(c->ev_nn0);
     if (c->ev_nn1) (void)hipEventDestroy(c->ev_nn1);
+    for (hipEvent_t e : c->sweep_timing.ev)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : c->allocations) (void)hipFree(p);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
     if (c->pred.sc_host) (void)hipHostFree(c->pred.sc_host);
@@ -798,6 +800,26 @@ static int time_launches(sg::Ctx* c, int reps, double* avg_ms, L launch) {
 }
 
 extern "C" {
+
+int sg_sweep_timing(sg_ctx* ctx, int on, long long* launches, double* total_ms) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return SG_ERR_ARG;
+    auto& T = c->sweep_timing;
+    if (on) {
+        if (!T.ev[0])
+            for (hipEvent_t& e : T.ev) SG_HIP(hipEventCreate(&e));
+        T.on = 1;
+        T.used = 0;
+        T.launches = 0;
+        T.ms = 0.0;
+        return SG_OK;
+    }
+    int rc = T.on ? sweep_timing_drain(c) : SG_OK;
+    T.on = 0;
+    if (launches) *launches = T.launches;
+    if (total_ms) *total_ms = T.ms;
+    return rc;
+}
 
 int sg_time_kernel(sg_ctx* ctx, int which, int reps, double* avg_ms) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);

@@ -1520,6 +1520,20 @@ int clsvof_allocate(Ctx* c) {
     return SG_OK;
 }
 
+// the recorded sweep launches' durations into the totals (waits for them)
+int sweep_timing_drain(Ctx* c) {
+    auto& T = c->sweep_timing;
+    for (int k = 0; k < T.used; ++k) {
+        SG_HIP(hipEventSynchronize(T.ev[2 * k + 1]));
+        float ms = 0.f;
+        SG_HIP(hipEventElapsedTime(&ms, T.ev[2 * k], T.ev[2 * k + 1]));
+        T.ms += ms;
+        T.launches += 1;
+    }
+    T.used = 0;
+    return SG_OK;
+}
+
 // Coupled_LS_VOF on the device fields VOF, PHI (in/out), NX, NY (out),
 // U, V (in).  `levels` holds Depthw+etau of each sub-step.
 int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
@@ -1562,6 +1576,18 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         return scan ? scan : (const unsigned*)cnt;
     };
     const unsigned* last_scan = nullptr;  // the previous sub-step's final scan
+    // sg_sweep_timing: an event before (end = 0) and after (end = 1) each sweep
+    auto& TM = c->sweep_timing;
+    if (TM.on && TM.used + 2 * nsub > Ctx::SweepTiming::POOL) {
+        const int rc = sweep_timing_drain(c);
+        if (rc) return rc;
+    }
+    auto sweep_mark = [&](int end) -> hipError_t {
+        if (!TM.on) return hipSuccess;
+        const hipError_t e = hipEventRecord(TM.ev[2 * TM.used + end], st);
+        if (end) ++TM.used;
+        return e;
+    };
     for (int k = 1; k <= nsub; ++k) {
         const double level = levels[k - 1];
         // first sweep: (vof, phi) -> (vt, pt), implicit divergence; vof is
@@ -1572,12 +1598,14 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
             hipLaunchKernelGGL((k_flux_pre<true>), pgrd, pblk, 0, st, G, s1, dtv, B.ifc, c1, B.fr, B.fl);
         else
             hipLaunchKernelGGL((k_flux_pre<false>), pgrd, pblk, 0, st, G, s1, dtv, B.ifc, c1, B.fr, B.fl);
+        SG_HIP(sweep_mark(0));
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_sweep<true, true>), sgrd, sblk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
                                B.tl, B.vt, B.pt);
         else
             hipLaunchKernelGGL((k_sweep<false, true>), sgrd, sblk, 0, st, G, s1, dtv, level, nullptr, nullptr, B.tv,
                                B.tl, B.vt, B.pt);
+        SG_HIP(sweep_mark(1));
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, B.vt, B.pt);
         // second sweep: (vt, pt) -> (vof, phi), explicit divergence
         const unsigned* c2 = recon(B.vt, B.pt, false, nullptr);
@@ -1586,12 +1614,14 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
             hipLaunchKernelGGL((k_flux_pre<false>), pgrd, pblk, 0, st, G, s2, dtv, B.ifc, c2, B.fr, B.fl);
         else
             hipLaunchKernelGGL((k_flux_pre<true>), pgrd, pblk, 0, st, G, s2, dtv, B.ifc, c2, B.fr, B.fl);
+        SG_HIP(sweep_mark(0));
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_sweep<false, false>), sgrd, sblk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
                                nullptr, vof, phi);
         else
             hipLaunchKernelGGL((k_sweep<true, false>), sgrd, sblk, 0, st, G, s2, dtv, level, B.tv, B.tl, nullptr,
                                nullptr, vof, phi);
+        SG_HIP(sweep_mark(1));
         if (c->moving.shape.kind != SG_SOLID_NONE) {
             // ObjectMovement (Clsvof_Mod.f90:2384-2398) over the sub-step,
             // then SolidVolumeFraction on the P cells (:920-921)

@@ -292,6 +292,16 @@ struct Ctx {
         double us = 0.0, vs = 0.0, asx = 0.0, asy = 0.0;
     } moving;
     hipEvent_t ev_nn0 = nullptr, ev_nn1 = nullptr;
+    // live timing of the transport's sweep launches (sg_sweep_timing): an
+    // event pair per launch, drained into the totals when the pool fills
+    struct SweepTiming {
+        static constexpr int POOL = 512;  // event pairs
+        int on = 0;
+        int used = 0;
+        hipEvent_t ev[2 * POOL] = {};
+        long long launches = 0;
+        double ms = 0.0;
+    } sweep_timing;
     int use_graph = 1;
     ClsBuffers cls;
     StagDev stag[2];  // U, V cells
@@ -356,6 +366,7 @@ int learned_warm_start(Ctx* c, bool prev);   // x = Galerkin-scaled prolongated 
 int learned_features(Ctx* c, int level, void* X, float* t, float* mask, double* scale);
 int learned_enable(Ctx* c, const float* w, int level, double out_scale, int deflate);
 int learned_set_prediction(Ctx* c, int level, const double* e_host, int deflate);
+int sweep_timing_drain(Ctx* c);  // sg_clsvof.hip
 void learned_release(Ctx* c);
 // time-step control and boundary layer (sg_step.hip)
 int time_step_size(Ctx* c, double cfl, double vint, double uref, double vs, double asy, double* dt);
