@@ -30,6 +30,7 @@
 // This file is compiled with FP contraction off: every sum and product
 // rounds as the reference writes it.
 #include "sg_internal.hpp"
+#include "sg_poly.hpp"
 
 namespace sg {
 
@@ -114,98 +115,70 @@ __device__ double plic_distance(double vf, double nxx, double nyy, double dx, do
 }
 
 // Fluid part of a cell cut by the solid line, counter-clockwise from the SE
-// corner (CellGeoCal :2170-2219, DistanceFluidCalculate :1966-2015).
-// Returns the number of vertices (<= 5); nd[n] repeats nd[0].
+// corner (CellGeoCal :2170-2219, DistanceFluidCalculate :1966-2015): n <= 5
+// vertices, x[n] repeats x[0], the rest are zero.  Every loop over the
+// vertices is unrolled with a guard in place of a run-time bound, so each
+// index is a compile-time constant and the polygon stays in registers.
 struct Poly {
     double x[6], y[6];
     int n;
 };
 
-__device__ void solid_poly(double nxs, double nys, double phis, double dx, double dy, Poly& P) {
-    const double dx2 = dx / 2.0, dy2 = dy / 2.0;
-    const double d0 = dx2 * nxs - dy2 * nys + phis;
-    const double d1 = -dx2 * nxs - dy2 * nys + phis;
-    const double d2 = -dx2 * nxs + dy2 * nys + phis;
-    const double d3 = dx2 * nxs + dy2 * nys + phis;
-    int n = 0;
+__device__ __forceinline__ void solid_poly(double nxs, double nys, double phis, double dx, double dy, Poly& P) {
+#pragma unroll
     for (int k = 0; k < 6; ++k) P.x[k] = P.y[k] = 0.0;
-    auto put = [&](double a, double b) {
+    int n = 0;
+    fluid_vertices(nxs, nys, phis, dx, dy, [&](double a, double b) {
         if (n < 5) {
-            P.x[n] = a;
-            P.y[n] = b;
+#pragma unroll
+            for (int s = 0; s < 5; ++s)
+                if (s == n) {
+                    P.x[s] = a;
+                    P.y[s] = b;
+                }
             ++n;
         }
-    };
-    if (d0 >= 0.0) put(dx2, -dy2);
-    if (d0 * d1 < 0.0) put((dy2 * nys - phis) / nxs, -dy2);
-    if (d1 >= 0.0) put(-dx2, -dy2);
-    if (d1 * d2 < 0.0) put(-dx2, (dx2 * nxs - phis) / nys);
-    if (d2 >= 0.0) put(-dx2, dy2);
-    if (d2 * d3 < 0.0) put((-dy2 * nys - phis) / nxs, dy2);
-    if (d3 >= 0.0) put(dx2, dy2);
-    if (d3 * d0 < 0.0) put(dx2, (-dx2 * nxs - phis) / nys);
-    P.x[n] = P.x[0];
-    P.y[n] = P.y[0];
+    });
+    const double x0 = P.x[0], y0 = P.y[0];
+#pragma unroll
+    for (int s = 1; s < 6; ++s)
+        if (s == n) {
+            P.x[s] = x0;
+            P.y[s] = y0;
+        }
     P.n = n;
 }
 
-// signed shoelace area of the part of polygon P where pdt <= 0
-// (pdt[n] = pdt[0]); CellGeoCal :2227-2251
-__device__ double liquid_area(const Poly& P, const double* pdt) {
-    double lx[7], ly[7];
-    int m = 0;
-    for (int k = 0; k < P.n; ++k) {
-        if (pdt[k] <= 0.0) {
-            lx[m] = P.x[k];
-            ly[m] = P.y[k];
-            ++m;
-        }
-        if (pdt[k] * pdt[k + 1] < 0.0) {
-            const double a = fabs(pdt[k]), b = fabs(pdt[k + 1]);
-            lx[m] = P.x[k] + (P.x[k + 1] - P.x[k]) * a / (a + b);
-            ly[m] = P.y[k] + (P.y[k + 1] - P.y[k]) * a / (a + b);
-            ++m;
-        }
-    }
-    if (m == 0) return 0.0;
-    lx[m] = lx[0];
-    ly[m] = ly[0];
-    double vol = 0.0;
-    for (int k = 0; k < m; ++k) vol = vol + 0.5 * (lx[k] * ly[k + 1] - lx[k + 1] * ly[k]);
-    return vol;
-}
-
 // CellGeoCal (:2159-2253)
-__device__ void cell_geo(double nxs, double nys, double phis, double nxl, double nyl, double phil, double dx,
-                         double dy, double& vols, double& volf) {
-    Poly P;
-    solid_poly(nxs, nys, phis, dx, dy, P);
-    double pdt[7];
-    double vol = 0.0;
-    for (int k = 0; k < P.n; ++k) {
-        vol = vol + 0.5 * (P.x[k] * P.y[k + 1] - P.x[k + 1] * P.y[k]);
-        pdt[k] = P.x[k] * nxl + P.y[k] * nyl + phil;
-    }
-    vols = 1.0 - fabs(vol / (dx * dy));
-    pdt[P.n] = P.n > 0 ? pdt[0] : 0.0;
-    volf = fabs(liquid_area(P, pdt) / (dx * dy));
+__device__ __forceinline__ void cell_geo(double nxs, double nys, double phis, double nxl, double nyl, double phil,
+                                         double dx, double dy, double& vols, double& volf) {
+    cut_cell_fractions(nxs, nys, phis, nxl, nyl, phil, dx, dy, vols, volf);
 }
 
-// LiqFuncVol (:2107-2155) on the fluid polygon
-__device__ double liq_func(double nxf, double nyf, double volf, double dx, double dy, const Poly& P, double xf,
+// LiqFuncVol (:2107-2155): volf less the fraction of the fluid polygon on
+// the liquid side of the line through (xf, yf) with normal (nxf, nyf).  The
+// reference takes the vertex count from the last non-zero vertex, nz - 1
+// (the repeat of the first vertex makes that n), and closes the polygon at
+// vertex nz - 1 with the first vertex's distance.  Inlined, so P stays in
+// registers.
+__device__ __forceinline__ double liq_func(double nxf, double nyf, double volf, double dx, double dy, const Poly& P, double xf,
                            double yf) {
     const double phil = -(nxf * xf + nyf * yf);
     // the reference counts vertices as the last non-zero node
     int nz = 0;
+#pragma unroll
     for (int k = 0; k < 6; ++k)
         if (fabs(P.x[k]) > 1.0e-14 || fabs(P.y[k]) > 1.0e-14) nz = k + 1;
     if (nz == 0) return volf;
-    Poly Q = P;
-    Q.n = nz - 1;
-    double pdt[7];
-    for (int k = 0; k < Q.n; ++k) pdt[k] = Q.x[k] * nxf + Q.y[k] * nyf + phil;
-    pdt[Q.n] = pdt[0];
-    return volf - fabs(liquid_area(Q, pdt) / (dx * dy));
+    const int n = nz - 1;
+    double pdt[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pdt[k] = P.x[k] * nxf + P.y[k] * nyf + phil;
+    ShoelaceClip c;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        if (k < n) c.edge(P.x[k], P.y[k], pdt[k], P.x[k + 1], P.y[k + 1], k + 1 == n ? pdt[0] : pdt[k + 1]);
+    return volf - fabs(c.close() / (dx * dy));
 }
 
 // DistanceFluidCalculate + zriddr (:1957-2105).  Returns false when the
@@ -217,7 +190,9 @@ __device__ bool distance_fluid(double nxs, double nys, double phis, double nxf, 
     const double diag = sqrt(dx * dx + dy * dy);
     double maxd = -2.0 * diag, mind = 2.0 * diag;
     double hx = 0.0, hy = 0.0, lx = 0.0, ly = 0.0;
-    for (int k = 0; k < P.n; ++k) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (k >= P.n) break;
         const double t = nxf * P.x[k] + nyf * P.y[k];
         if (t > maxd) {
             maxd = t;

@@ -22,6 +22,7 @@
 // j = 1..nj+1.  Compiled with FP contraction off: every operation rounds as
 // the reference writes it.
 #include "sg_internal.hpp"
+#include "sg_poly.hpp"
 
 namespace sg {
 
@@ -91,65 +92,6 @@ __device__ double pfrac(double nx, double ny, double d, double dx, double dy) {
         right = t * t;
     }
     return area * (1.0 - top - right);
-}
-
-// CellGeoCal (:2258-2352): solid and liquid fractions of a box
-__device__ void pcellgeo(double nxs, double nys, double phis, double nxl, double nyl, double phil, double dx,
-                         double dy, double& vols, double& volf) {
-    const double dx2 = dx / 2.0, dy2 = dy / 2.0;
-    const double d0 = dx2 * nxs - dy2 * nys + phis;
-    const double d1 = -dx2 * nxs - dy2 * nys + phis;
-    const double d2 = -dx2 * nxs + dy2 * nys + phis;
-    const double d3 = dx2 * nxs + dy2 * nys + phis;
-    double px[6], py[6];
-    int n = 0;
-    auto put = [&](double a, double b) {
-        if (n < 5) {
-            px[n] = a;
-            py[n] = b;
-            ++n;
-        }
-    };
-    if (d0 >= 0.0) put(dx2, -dy2);
-    if (d0 * d1 < 0.0) put((dy2 * nys - phis) / nxs, -dy2);
-    if (d1 >= 0.0) put(-dx2, -dy2);
-    if (d1 * d2 < 0.0) put(-dx2, (dx2 * nxs - phis) / nys);
-    if (d2 >= 0.0) put(-dx2, dy2);
-    if (d2 * d3 < 0.0) put((-dy2 * nys - phis) / nxs, dy2);
-    if (d3 >= 0.0) put(dx2, dy2);
-    if (d3 * d0 < 0.0) put(dx2, (-dx2 * nxs - phis) / nys);
-    px[n] = n ? px[0] : 0.0;
-    py[n] = n ? py[0] : 0.0;
-    double pdt[7];
-    double vol = 0.0;
-    for (int k = 0; k < n; ++k) {
-        vol = vol + 0.5 * (px[k] * py[k + 1] - px[k + 1] * py[k]);
-        pdt[k] = px[k] * nxl + py[k] * nyl + phil;
-    }
-    vols = 1.0 - fabs(vol / (dx * dy));
-    pdt[n] = n ? pdt[0] : 0.0;
-    double lx[7], ly[7];
-    int m = 0;
-    for (int k = 0; k < n; ++k) {
-        if (pdt[k] <= 0.0) {
-            lx[m] = px[k];
-            ly[m] = py[k];
-            ++m;
-        }
-        if (pdt[k] * pdt[k + 1] < 0.0) {
-            const double a = fabs(pdt[k]), b = fabs(pdt[k + 1]);
-            lx[m] = px[k] + (px[k + 1] - px[k]) * a / (a + b);
-            ly[m] = py[k] + (py[k + 1] - py[k]) * a / (a + b);
-            ++m;
-        }
-    }
-    double lv = 0.0;
-    if (m > 0) {
-        lx[m] = lx[0];
-        ly[m] = ly[0];
-        for (int k = 0; k < m; ++k) lv = lv + 0.5 * (lx[k] * ly[k + 1] - lx[k + 1] * ly[k]);
-    }
-    volf = fabs(lv / (dx * dy));
 }
 
 // densities of FaceDensityFlux / DensityBasedConvectiveFlux in their two
@@ -448,8 +390,9 @@ __device__ void dflux_ew(const PredG& G, int i, int j, double& m1, double& m2, d
                 w1 = (volf * G.rl / G.rr + (1.0 - volf - 0.0) * G.rg / G.rr);
             } else {
                 double vols, volf;
-                pcellgeo(G.pnxS[c], G.pnyS[c], G.pphiS[c] + uw * dt / 2.0 * G.pnxS[c], G.pnx[c], G.pny[c],
-                         G.pphi[c] + uw * dt / 2.0 * G.pnx[c], fabs(uw * dt), PDY(j), vols, volf);
+                cut_cell_fractions(G.pnxS[c], G.pnyS[c], G.pphiS[c] + uw * dt / 2.0 * G.pnxS[c], G.pnx[c],
+                                   G.pny[c], G.pphi[c] + uw * dt / 2.0 * G.pnx[c], fabs(uw * dt), PDY(j), vols,
+                                   volf);
                 w1 = (volf * G.rl / G.rr + (1.0 - volf - vols) * G.rg / G.rr);
             }
         }
@@ -559,8 +502,8 @@ __device__ void dflux_ns(const PredG& G, int i, int j, double& m1, double& m2, d
             w2 = (volf * G.rl / G.rr + (1.0 - volf - 0.0) * G.rg / G.rr);
         } else {
             double vols, volf;
-            pcellgeo(G.pnxS[c], G.pnyS[c], G.pphiS[c] + w * dt / 2.0 * G.pnyS[c], G.pnx[c], G.pny[c],
-                     G.pphi[c] + w * dt / 2.0 * G.pny[c], PDX(i), fabs(w * dt), vols, volf);
+            cut_cell_fractions(G.pnxS[c], G.pnyS[c], G.pphiS[c] + w * dt / 2.0 * G.pnyS[c], G.pnx[c], G.pny[c],
+                               G.pphi[c] + w * dt / 2.0 * G.pny[c], PDX(i), fabs(w * dt), vols, volf);
             w2 = (volf * G.rl / G.rr + (1.0 - volf - vols) * G.rg / G.rr);
         }
         m2 = w * w2 * VDX(i);
