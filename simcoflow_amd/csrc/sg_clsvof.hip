@@ -736,13 +736,17 @@ struct Line {
     __device__ int na() const { return X ? G.ni : G.nj; }
     __device__ size_t c(int a) const { return X ? cix(G, a, b) : cix(G, b, a); }
     // masked face velocity ue(a,j) / ve(i,a) (Coupled_LS_VOF :778-797)
+    // (both loads are issued before the flag selects: no dependent round trip)
     __device__ double vel(int a) const {
+        const int af = a > 0 ? a : 1;
         if (X) {
-            if (a == 0) return G.u[gix(G, 0, b)];
-            return G.uflag[cix(G, a, b)] ? G.u[gix(G, a, b)] : 0.0;
+            const double w = G.u[gix(G, a, b)];
+            const bool on = G.uflag[cix(G, af, b)];
+            return a == 0 || on ? w : 0.0;
         }
-        if (a == 0) return G.v[gix(G, b, 0)];
-        return G.vflag[cix(G, b, a)] ? G.v[gix(G, b, a)] : 0.0;
+        const double w = G.v[gix(G, b, a)];
+        const bool on = G.vflag[cix(G, b, af)];
+        return a == 0 || on ? w : 0.0;
     }
     __device__ double ap(int a) const { return X ? G.aE[gix(G, a, b)] : G.aN[gix(G, b, a)]; }
     __device__ double am(int a) const { return X ? G.aW[gix(G, a, b)] : G.aS[gix(G, b, a)]; }
@@ -808,54 +812,35 @@ __device__ double plic_near(const Line<X>& L, const SweepIn& s, int a, size_t c,
     return -plic_face_flux<X>(-1, nx, ny, d, f, G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c], hx, hy, -w * dtv);
 }
 
-// volume flux through face a (between cells a and a+1), X_Sweep :1014-1051
-template <bool X>
-__device__ double vof_flux(const Line<X>& L, const SweepIn& s, int a, double dtv) {
-    const ClsGeo& G = L.G;
-    const double w = L.vel(a);
-    const int n = L.na();
-    if (X ? (w >= 0.0) : (w > 0.0)) {
-        const size_t c = L.c(a);
-        const double f = s.vof[c];
-        double flux;
-        if (f >= (1.0 - VOFEPS) || f <= VOFEPS) {
-            flux = f * w * dtv;
-            if (f >= 1.0) flux = w * dtv;
-            if (f < 0.0) flux = 0.0;
-        } else {
-            flux = s.fr[c];
-        }
-        if (L.ap(a) > 1.0 - VOFEPS && G.vofS[c] > 1.0 - VOFEPS) flux = 1.0 * w * dtv;
-        return flux;
+// vof_flux (X_Sweep :1014-1051) once the upwind cell of the face is known:
+// f, vsU its vof and vofS, apU its aperture of the face, plic the PLIC flux
+// precomputed for that side (read only for an interface cell, where
+// VOFEPS < f < 1-VOFEPS)
+__device__ __forceinline__ double upwind_flux(double w, double f, double vsU, double apU, const double* plic,
+                                              size_t cU, double dtv) {
+    double flux;
+    if (f >= (1.0 - VOFEPS) || f <= VOFEPS) {
+        flux = f * w * dtv;
+        if (f >= 1.0) flux = w * dtv;
+        if (f < 0.0) flux = 0.0;
+    } else {
+        flux = plic[cU];
     }
-    if (a < n) {
-        const size_t c = L.c(a + 1);
-        const double f = s.vof[c];
-        double flux;
-        if (f >= (1.0 - VOFEPS) || f <= VOFEPS) {
-            flux = f * w * dtv;
-            if (f >= 1.0) flux = w * dtv;
-            if (f < 0.0) flux = 0.0;
-        } else {
-            flux = s.fl[c];
-        }
-        if (L.am(a + 1) > 1.0 - VOFEPS && G.vofS[c] > 1.0 - VOFEPS) flux = 1.0 * w * dtv;
-        return flux;
-    }
-    return s.vof[L.c(a)] * w * dtv;
+    if (apU > 1.0 - VOFEPS && vsU > 1.0 - VOFEPS) flux = 1.0 * w * dtv;
+    return flux;
 }
 
-// level-set flux through interior face a, 2 <= a <= na-1 (:1073-1096)
-template <bool X>
-__device__ double ls_flux(const Line<X>& L, const double* phi, int a, double dtv) {
-    const double w = L.vel(a);
+// level-set flux through interior face a (:1073-1096) from the level set
+// at a-1, a, a+1, a+2 (pa1 = phi(a+1), pa2 = phi(a+2), ...)
+__device__ __forceinline__ double ls_face(double w, int a, int n, double pm1, double p0, double p1, double p2,
+                                          double dtv) {
     double l;
     if (w >= 0.0)
-        l = (phi[L.c(a + 1)] + phi[L.c(a - 1)]) / 2.0;
-    else if (a <= L.na() - 2)
-        l = (phi[L.c(a + 2)] + phi[L.c(a)]) / 2.0;
+        l = (p1 + pm1) / 2.0;
+    else if (a <= n - 2)
+        l = (p2 + p0) / 2.0;
     else
-        l = (phi[L.c(a + 1)] + phi[L.c(a - 1)]) / 2.0;
+        l = (p1 + pm1) / 2.0;
     return w * l * dtv;
 }
 
@@ -911,7 +896,10 @@ __global__ void __launch_bounds__(256) k_flux_pre(ClsGeo G, SweepIn s, double dt
 
 // One sweep.  IMPL: first sweep of the sub-step (implicit divergence, clip);
 // otherwise second sweep (explicit divergence with the first sweep's
-// tv/tl).  level: Depthw+etau of the west inflow (x sweeps).
+// tv/tl).  level: Depthw+etau of the west inflow (x sweeps).  Every operand
+// a cell may need is loaded up front, with neighbour indices clamped to the
+// line, before any of them decides which one is used: the upwind choice
+// then costs no second round trip to memory.
 template <bool X, bool IMPL>
 __global__ void __launch_bounds__(256) k_sweep(ClsGeo G, SweepIn s, double dtv, double level,
                                                const double* __restrict__ tv_in, const double* __restrict__ tl_in,
@@ -923,38 +911,66 @@ __global__ void __launch_bounds__(256) k_sweep(ClsGeo G, SweepIn s, double dtv, 
     const Line<X> L{G, X ? j : i};
     const int a = X ? i : j;
     const int n = L.na();
-    const size_t c = L.c(a);
-    const double vs = G.vofS[c];
+    const int am1 = a > 1 ? a - 1 : a, am2 = a > 2 ? a - 2 : am1;
+    const int ap1 = a < n ? a + 1 : a, ap2 = a + 1 < n ? a + 2 : ap1;
+    const size_t c = L.c(a), cm = L.c(am1), cp = L.c(ap1);
+    const double vs = G.vofS[c], vsm = G.vofS[cm], vsp = G.vofS[cp];
+    const double f0 = s.vof[c], fm = s.vof[cm], fp = s.vof[cp];
+    const double wm = L.vel(a - 1), w0 = L.vel(a);
+    const double am0 = L.am(a), ap0 = L.ap(a), apm = L.ap(am1), amp = L.am(ap1);
+    const double ph0 = s.phi[c], phm1 = s.phi[cm], php1 = s.phi[cp];
+    const double phm2 = s.phi[L.c(am2)], php2 = s.phi[L.c(ap2)];
+    const double h = L.h(a);
+    double tvi = 0.0, tli = 0.0;
+    if (!IMPL) {
+        tvi = tv_in[c];
+        tli = tl_in[c];
+    }
     // volume of fluid: temvf(a) after the sweep's in-place updates
-    double t = s.vof[c];
+    double t = f0;
     if (vs < 1.0 - VOFEPS) t = t / (1.0 - vs);
-    if (a >= 2) t = t * (1.0 - vs) + vof_flux<X>(L, s, a - 1, dtv) * L.am(a) / L.h(a);
-    t = t - vof_flux<X>(L, s, a, dtv) * L.ap(a) / L.h(a);
+    if (a >= 2) {
+        // face a-1: upwind cell a-1 or a
+        const double fx = (X ? (wm >= 0.0) : (wm > 0.0)) ? upwind_flux(wm, fm, vsm, apm, s.fr, cm, dtv)
+                                                         : upwind_flux(wm, f0, vs, am0, s.fl, c, dtv);
+        t = t * (1.0 - vs) + fx * am0 / h;
+    }
+    {
+        // face a: upwind cell a or a+1; the last face takes cell a's vof
+        double fx;
+        if (X ? (w0 >= 0.0) : (w0 > 0.0))
+            fx = upwind_flux(w0, f0, vs, ap0, s.fr, c, dtv);
+        else if (a < n)
+            fx = upwind_flux(w0, fp, vsp, amp, s.fl, cp, dtv);
+        else
+            fx = f0 * w0 * dtv;
+        t = t - fx * ap0 / h;
+    }
     if (a == 1) {
         double fin = 0.0;
         if (X) {
-            if (G.yc[j - 1] - G.dy[j - 1] / 2.0 < level) fin = s.vof[c] * L.vel(0) * dtv;
+            if (G.yc[j - 1] - G.dy[j - 1] / 2.0 < level) fin = f0 * wm * dtv;
         } else {
-            fin = s.vof[c] * L.vel(0) * dtv;
+            fin = f0 * wm * dtv;
         }
         t = t + fin * L.am(1) / L.h(1);
     }
     // level set: temls(a)
-    double q = s.phi[c];
-    if (a >= 3) q = q * (1.0 - vs) + ls_flux<X>(L, s.phi, a - 1, dtv) * L.am(a) / L.h(a);
-    if (a >= 2 && a <= n - 1) q = q - ls_flux<X>(L, s.phi, a, dtv) * L.ap(a) / L.h(a);
+    double q = ph0;
+    if (a >= 3) q = q * (1.0 - vs) + ls_face(wm, a - 1, n, phm2, phm1, ph0, php1, dtv) * am0 / h;
+    if (a >= 2 && a <= n - 1) q = q - ls_face(w0, a, n, phm1, ph0, php1, php2, dtv) * ap0 / h;
     if (a == 1) {
         q = q - ls_flux_first<X>(L, s.phi, dtv) * L.ap(1) / L.h(1);
-        q = q + s.phi[c] * L.vel(0) * dtv * L.am(1) / L.h(1);
+        q = q + ph0 * wm * dtv * L.am(1) / L.h(1);
     }
     if (a == 2) q = q + ls_flux_first<X>(L, s.phi, dtv) * L.ap(1) / L.h(2);
     if (a == n) {
         const double fl = ls_flux_last<X>(L, s.phi, dtv);
         q = X ? q - fl * L.ap(n) / L.h(n) : q - fl / L.h(n);
     }
-    const double dvel = L.vel(a) * L.ap(a) - L.vel(a - 1) * L.am(a);
+    const double dvel = w0 * ap0 - wm * am0;
     if (IMPL) {
-        const double den = (1.0 - vs + TOLDENO) - dtv / L.h(a) * dvel;
+        const double den = (1.0 - vs + TOLDENO) - dtv / h * dvel;
         t = t / den;
         q = q / den;
         tv_out[c] = t;
@@ -965,10 +981,10 @@ __global__ void __launch_bounds__(256) k_sweep(ClsGeo G, SweepIn s, double dtv, 
         vof_out[c] = f;
         phi_out[c] = q;
     } else {
-        t = t + dtv / L.h(a) * tv_in[c] * dvel;
-        q = q + dtv / L.h(a) * tl_in[c] * dvel;
+        t = t + dtv / h * tvi * dvel;
+        q = q + dtv / h * tli * dvel;
         vof_out[c] = t;
-        phi_out[c] = vs < 1.0 - EPSI ? q / (1.0 - vs) : s.phi[c];
+        phi_out[c] = vs < 1.0 - EPSI ? q / (1.0 - vs) : ph0;
     }
 }
 
