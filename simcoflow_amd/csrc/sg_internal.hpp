@@ -196,8 +196,6 @@ struct BiScalars {
 
 // Work arrays of the momentum predictor.
 struct PredBuffers {
-    static constexpr int NFACE = 22;
-    double* face[NFACE] = {nullptr};  // face fluxes (ghosted planes)
     double *gu = nullptr, *gv = nullptr;   // FluxDiv before mixing
     double *fdu = nullptr, *fdv = nullptr; // FluxDiv
     double *mu = nullptr, *mv = nullptr;   // momentum rows (5 planes)

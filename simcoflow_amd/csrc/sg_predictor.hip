@@ -1,26 +1,27 @@
 // Momentum predictor on gfx950 (PredictorUV_mod.f90, Predictor_UV :28-497).
 //
-//   k_faces<old>  every EW / NS face of the U and V cells: convective
-//                 fluxes (MUSCL + kinetic flux, or the first-step upwind
-//                 form), face densities from the half-step PLIC volume and
-//                 the density-weighted convective fluxes, viscous
-//                 conductances and cross-diffusion terms at time n
-//   k_conv        per U / V cell: density-corrected convective divergence,
-//                 Adams-Bashforth extrapolation with Flux_n1, closed-cell
-//                 zeroing, the u^n dx dy / dt term
-//   k_mix_visc    small-cell mixing (Mixing_Procedure :2127-2197) gathered
-//                 in the reference's scatter order, explicit half of the
-//                 viscous term, moving-wall friction
-//   k_faces<new>  viscous conductances with the vof after the transport
-//   k_rows        the implicit momentum rows, right-hand sides and Dp
+//   k_conv        per U / V cell: convective fluxes of its faces (MUSCL +
+//                 kinetic flux, or the first-step upwind form), face
+//                 densities from the half-step PLIC volume and the
+//                 density-weighted convective fluxes; then the
+//                 density-corrected convective divergence, Adams-Bashforth
+//                 extrapolation with Flux_n1, closed-cell zeroing, the
+//                 u^n dx dy / dt term
+//   k_mix_visc    viscous conductances and cross-diffusion terms of its
+//                 faces at time n; small-cell mixing (Mixing_Procedure
+//                 :2127-2197) gathered in the reference's scatter order,
+//                 explicit half of the viscous term, moving-wall friction
+//   k_rows        viscous conductances with the vof after the transport;
+//                 the implicit momentum rows, right-hand sides and Dp
 //                 (SetMatrix :622-768, SetVectors :770-845)
 //   BiCGStab      the two momentum systems (sg_momentum.hip)
 //   k_scatter, k_finish  Pred%u/v, ghost copies of Dp, open-top v
 //
-// Face arrays use the ghosted layout (ni+2)*(nj+2): EW face (i,j) is the
-// west face of U/V cell (i,j), i = 1..ni+1; NS face (i,j) the south face,
-// j = 1..nj+1.  Compiled with FP contraction off: every operation rounds as
-// the reference writes it.
+// Each of the three cell kernels computes the face values its tile of U/V
+// cells reads into LDS: no face flux goes through memory.  Face (i,j) of
+// direction EW is the west face of U/V cell (i,j), i = 1..ni+1; NS face
+// (i,j) the south face, j = 1..nj+1.  Compiled with FP contraction off:
+// every operation rounds as the reference writes it.
 #include "sg_internal.hpp"
 #include "sg_poly.hpp"
 
@@ -597,65 +598,66 @@ __device__ void vflux_ns(const PredG& G, int i, int j, bool newvof, double& d1, 
 }
 
 // ---- kernels ----------------------------------------------------------------------
+//
+// The face fluxes are not stored in memory: each of the three cell kernels
+// computes the faces its tile of FTW x FTH U/V cells reads into LDS, then
+// runs its cell update out of LDS.  A tile reads the west and south faces of
+// its cells -- face point (i,j) of cell (i,j), one per thread -- and the east
+// faces of its last column and north faces of its last row, a halo of
+// FTW + FTH faces computed by the first two wavefronts in a second round.
 
-// face arrays, one ghosted plane each; [0] U component, [1] V component
-struct Faces {
-    double *cf_ew[2], *cf_ns[2];  // CFluxEW / CFluxNS
-    double *ma_ew[2], *ma_ns[2];  // MaFluxEW / MaFluxNS
-    double *co_ew[2], *co_ns[2];  // ConFluxEW / ConFluxNS
-    double *di_ew[2], *di_ns[2];  // DifFluxEW / DifFluxNS
-    double *ed_ew[2], *ed_ns[2];  // EDFluxEW / EDFluxNS
-    double *ex_ew, *ex_ns;        // ExEDFlux, V component (the U one is 0)
+constexpr int FTW = 64, FTH = 8, FNT = FTW * FTH;
+
+// N face quantities of a tile: e[k][lj][li] is EW face (i0+li+1, j0+lj+1),
+// li = 0..FTW; n[k][lj][li] is NS face (i0+li+1, j0+lj+1), lj = 0..FTH
+template <int N>
+struct TilePlanes {
+    double e[N][FTH][FTW + 1];
+    double n[N][FTH + 1][FTW];
 };
 
-// one thread per (i,j) in [1,ni+1]x[1,nj+1]: EW face (i,j) when j <= nj,
-// NS face (i,j) when i <= ni
-template <bool OLD>
-__global__ void __launch_bounds__(256) k_faces(PredG G, Faces F) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > G.ni + 1 || j > G.nj + 1) return;
-    const size_t g = gi(G, i, j);
-    if (j <= G.nj) {
-        double d1, d2, e1, e2, x2;
-        vflux_ew(G, i, j, !OLD, d1, d2, e1, e2, x2);
-        F.di_ew[0][g] = d1;
-        F.di_ew[1][g] = d2;
-        F.ed_ew[0][g] = e1;
-        F.ed_ew[1][g] = e2;
-        if (OLD) {
-            F.ex_ew[g] = x2;
-            double f1, f2, m1, m2, k1, k2;
-            cflux_ew(G, i, j, f1, f2);
-            dflux_ew(G, i, j, m1, m2, k1, k2);
-            F.cf_ew[0][g] = f1;
-            F.cf_ew[1][g] = f2;
-            F.ma_ew[0][g] = m1;
-            F.ma_ew[1][g] = m2;
-            F.co_ew[0][g] = k1;
-            F.co_ew[1][g] = k2;
+// calls ew(i, j, lj, li) for every EW face of the tile that exists
+// (j <= nj, i <= ni+1) and ns(i, j, lj, li) for every NS face (i <= ni,
+// j <= nj+1); each is called from one place, so its code is emitted once.
+// Ends with a barrier.
+template <class EW, class NS>
+__device__ __forceinline__ void tile_faces(const PredG& G, int i0, int j0, EW&& ew, NS&& ns) {
+    const int t = threadIdx.x + threadIdx.y * FTW;
+#pragma unroll 1
+    for (int r = 0; r < 2; ++r) {
+        int li, lj;
+        bool e, n;
+        if (r == 0) {
+            li = threadIdx.x;
+            lj = threadIdx.y;
+            e = n = true;
+        } else if (t < FTW) {
+            li = t;  // north faces of the last row
+            lj = FTH;
+            e = false;
+            n = true;
+        } else if (t < FTW + FTH) {
+            li = FTW;  // east faces of the last column
+            lj = t - FTW;
+            e = true;
+            n = false;
+        } else {
+            break;
         }
+        const int i = i0 + li + 1, j = j0 + lj + 1;
+        if (e && j <= G.nj && i <= G.ni + 1) ew(i, j, lj, li);
+        if (n && i <= G.ni && j <= G.nj + 1) ns(i, j, lj, li);
     }
-    if (i <= G.ni) {
-        double d1, d2, e1, e2, x2;
-        vflux_ns(G, i, j, !OLD, d1, d2, e1, e2, x2);
-        F.di_ns[0][g] = d1;
-        F.di_ns[1][g] = d2;
-        F.ed_ns[0][g] = e1;
-        F.ed_ns[1][g] = e2;
-        if (OLD) {
-            F.ex_ns[g] = x2;
-            double f1, f2, m1, m2, k1, k2;
-            cflux_ns(G, i, j, f1, f2);
-            dflux_ns(G, i, j, m1, m2, k1, k2);
-            F.cf_ns[0][g] = f1;
-            F.cf_ns[1][g] = f2;
-            F.ma_ns[0][g] = m1;
-            F.ma_ns[1][g] = m2;
-            F.co_ns[0][g] = k1;
-            F.co_ns[1][g] = k2;
-        }
-    }
+    __syncthreads();
+}
+
+// the four faces of cell (i0+li+1, j0+lj+1) in plane k
+struct Quad {
+    double e, w, n, s;
+};
+template <int N>
+__device__ __forceinline__ Quad quad(const TilePlanes<N>& P, int k, int li, int lj) {
+    return Quad{P.e[k][lj][li + 1], P.e[k][lj][li], P.n[k][lj + 1][li], P.n[k][lj][li]};
 }
 
 struct Comp {  // one momentum component
@@ -669,12 +671,38 @@ __device__ __forceinline__ Comp comp_of(const PredG& G, int c) {
 }
 
 // convective part of the assembly (:175-229 / :320-385): FluxDiv before
-// mixing into fd[c]; Flux_n1 updated on active cells
-__global__ void __launch_bounds__(256) k_conv(PredG G, Faces F, double* __restrict__ fn1u,
-                                              double* __restrict__ fn1v, double* __restrict__ fdu,
-                                              double* __restrict__ fdv) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+// mixing into fd[c]; Flux_n1 updated on active cells.  Faces: CFlux,
+// MaFlux and ConFlux (planes 0-1, 2-3, 4-5; U then V) at time n.
+__global__ void __launch_bounds__(FNT) k_conv(PredG G, double* __restrict__ fn1u, double* __restrict__ fn1v,
+                                              double* __restrict__ fdu, double* __restrict__ fdv) {
+    __shared__ TilePlanes<6> P;
+    const int i0 = blockIdx.x * FTW, j0 = blockIdx.y * FTH;
+    tile_faces(
+        G, i0, j0,
+        [&](int i, int j, int lj, int li) {
+            double f1, f2, m1, m2, k1, k2;
+            cflux_ew(G, i, j, f1, f2);
+            dflux_ew(G, i, j, m1, m2, k1, k2);
+            P.e[0][lj][li] = f1;
+            P.e[1][lj][li] = f2;
+            P.e[2][lj][li] = m1;
+            P.e[3][lj][li] = m2;
+            P.e[4][lj][li] = k1;
+            P.e[5][lj][li] = k2;
+        },
+        [&](int i, int j, int lj, int li) {
+            double f1, f2, m1, m2, k1, k2;
+            cflux_ns(G, i, j, f1, f2);
+            dflux_ns(G, i, j, m1, m2, k1, k2);
+            P.n[0][lj][li] = f1;
+            P.n[1][lj][li] = f2;
+            P.n[2][lj][li] = m1;
+            P.n[3][lj][li] = m2;
+            P.n[4][lj][li] = k1;
+            P.n[5][lj][li] = k2;
+        });
+    const int li = threadIdx.x, lj = threadIdx.y;
+    const int i = i0 + li + 1, j = j0 + lj + 1;
     for (int cp = 0; cp < 2; ++cp) {
         const Comp K = comp_of(G, cp);
         if (i > K.imax || j > K.jmax) continue;
@@ -687,26 +715,17 @@ __global__ void __launch_bounds__(256) k_conv(PredG G, Faces F, double* __restri
             continue;
         }
         const double hx = cp == 0 ? UDX(i) : VDX(i), hy = cp == 0 ? UDY(j) : VDY(j);
-        const double* co_e = F.co_ew[cp];
-        const double* co_n = F.co_ns[cp];
-        const double* ma_e = F.ma_ew[cp];
-        const double* ma_n = F.ma_ns[cp];
-        const double* cf_e = F.cf_ew[cp];
-        const double* cf_n = F.cf_ns[cp];
-        const double Fe = co_e[gi(G, i + 1, j)], Fw = co_e[gi(G, i, j)], Fn = co_n[gi(G, i, j + 1)],
-                     Fs = co_n[gi(G, i, j)];
+        const Quad cf = quad(P, cp, li, lj), ma = quad(P, 2 + cp, li, lj), co = quad(P, 4 + cp, li, lj);
+        const double Fe = co.e, Fw = co.w, Fn = co.n, Fs = co.s;
         const double vf = T.vofo[c], vs = T.vofS[c];
         const double RoCeO = (1.0 - vf - vs) * G.rg / G.rr + vf * G.rl / G.rr;
-        const double RoCeN =
-            RoCeO - G.dt * (ma_e[gi(G, i + 1, j)] - ma_e[gi(G, i, j)] + ma_n[gi(G, i, j + 1)] - ma_n[gi(G, i, j)]) /
-                        (hx * hy);
+        const double RoCeN = RoCeO - G.dt * (ma.e - ma.w + ma.n - ma.s) / (hx * hy);
         const double lo = (1.0 + EPSI - vs) * G.rg / G.rr, hi = (1.0 - EPSI - vs) * G.rl / G.rr;
         double fluxn0;
         if (RoCeN > lo && RoCeN < hi && RoCeO > lo && RoCeO < hi && vs < EPSI)
             fluxn0 = (Fe - Fw + Fn - Fs) / RoCeN + K.w[gi(G, i, j)] * (1.0 - RoCeO / RoCeN) / G.dt * hx * hy;
         else
-            fluxn0 = (cf_e[gi(G, i + 1, j)] - cf_e[gi(G, i, j)] + cf_n[gi(G, i, j + 1)] - cf_n[gi(G, i, j)]) /
-                     (1.0 - vs);
+            fluxn0 = (cf.e - cf.w + cf.n - cf.s) / (1.0 - vs);
         double v0 = G.first ? fluxn0 : 1.5 * fluxn0 - 0.5 * fn1[gi(G, i, j)];
         int closed = 0;
         if (T.aW[gi(G, i, j)] < EPSI) closed++;
@@ -764,12 +783,31 @@ __device__ bool mix_source(const PredG& G, const Comp& K, const double* f, int s
 
 // mixing (gathered in the reference's scatter order) + explicit viscous
 // half + moving-wall friction (:231-249, :387-420).  fpre: FluxDiv before
-// mixing; out: final FluxDiv
-__global__ void __launch_bounds__(256) k_mix_visc(PredG G, Faces F, const double* __restrict__ fpre_u,
+// mixing; out: final FluxDiv.  Faces: DifFlux (planes 0-1, U then V) and
+// the V component's ExEDFlux (plane 2) with the time-n volume fractions.
+__global__ void __launch_bounds__(FNT) k_mix_visc(PredG G, const double* __restrict__ fpre_u,
                                                   const double* __restrict__ fpre_v, double* __restrict__ out_u,
                                                   double* __restrict__ out_v) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    __shared__ TilePlanes<3> P;
+    const int i0 = blockIdx.x * FTW, j0 = blockIdx.y * FTH;
+    tile_faces(
+        G, i0, j0,
+        [&](int i, int j, int lj, int li) {
+            double d1, d2, e1, e2, x2;
+            vflux_ew(G, i, j, false, d1, d2, e1, e2, x2);
+            P.e[0][lj][li] = d1;
+            P.e[1][lj][li] = d2;
+            P.e[2][lj][li] = x2;
+        },
+        [&](int i, int j, int lj, int li) {
+            double d1, d2, e1, e2, x2;
+            vflux_ns(G, i, j, false, d1, d2, e1, e2, x2);
+            P.n[0][lj][li] = d1;
+            P.n[1][lj][li] = d2;
+            P.n[2][lj][li] = x2;
+        });
+    const int li = threadIdx.x, lj = threadIdx.y;
+    const int i = i0 + li + 1, j = j0 + lj + 1;
     for (int cp = 0; cp < 2; ++cp) {
         const Comp K = comp_of(G, cp);
         if (i > K.imax || j > K.jmax) continue;
@@ -793,21 +831,21 @@ __global__ void __launch_bounds__(256) k_mix_visc(PredG G, Faces F, const double
             }
         }
         if (T.act[c]) {
-            const double* de = F.di_ew[cp];
-            const double* dn = F.di_ns[cp];
+            const Quad d = quad(P, cp, li, lj);
             const double* w = K.w;
             double ExDe = 0.0, ExDw = 0.0, ExDn = 0.0, ExDs = 0.0;
             if (cp == 1) {
-                ExDe = F.ex_ew[gi(G, i + 1, j)];
-                ExDw = F.ex_ew[gi(G, i, j)];
-                ExDn = F.ex_ns[gi(G, i, j + 1)];
-                ExDs = F.ex_ns[gi(G, i, j)];
+                const Quad x = quad(P, 2, li, lj);
+                ExDe = x.e;
+                ExDw = x.w;
+                ExDn = x.n;
+                ExDs = x.s;
             }
             const double vs = T.vofS[c];
 #define W(a, b) w[gi(G, a, b)]
             val = val -
-                  BETAVIS * ((de[gi(G, i + 1, j)] * (W(i + 1, j) - W(i, j)) - de[gi(G, i, j)] * (W(i, j) - W(i - 1, j))) -
-                             (dn[gi(G, i, j + 1)] * (W(i, j + 1) - W(i, j)) - dn[gi(G, i, j)] * (W(i, j) - W(i, j - 1)))) /
+                  BETAVIS * ((d.e * (W(i + 1, j) - W(i, j)) - d.w * (W(i, j) - W(i - 1, j))) -
+                             (d.n * (W(i, j + 1) - W(i, j)) - d.s * (W(i, j) - W(i, j - 1)))) /
                       (1.0 - vs) +
                   (-ExDe + ExDw - ExDn + ExDs) / (1.0 - vs);
 #undef W
@@ -822,14 +860,36 @@ __global__ void __launch_bounds__(256) k_mix_visc(PredG G, Faces F, const double
 }
 
 // momentum rows, right-hand sides and Dp (SetMatrix :639-710, SetVectors
-// :797-822) for both components; rows in the matr layout (5 planes)
-__global__ void __launch_bounds__(256) k_rows(PredG G, Faces F, const double* __restrict__ fdu,
+// :797-822) for both components; rows in the matr layout (5 planes).
+// Faces: DifFlux (planes 0-1) and EDFlux (planes 2-3), U then V, with the
+// volume fractions after the transport.
+__global__ void __launch_bounds__(FNT) k_rows(PredG G, const double* __restrict__ fdu,
                                               const double* __restrict__ fdv, double* __restrict__ mu,
                                               double* __restrict__ ru, double* __restrict__ mv,
                                               double* __restrict__ rv, double* __restrict__ DpU,
                                               double* __restrict__ DpV) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    __shared__ TilePlanes<4> P;
+    const int i0 = blockIdx.x * FTW, j0 = blockIdx.y * FTH;
+    tile_faces(
+        G, i0, j0,
+        [&](int i, int j, int lj, int li) {
+            double d1, d2, e1, e2, x2;
+            vflux_ew(G, i, j, true, d1, d2, e1, e2, x2);
+            P.e[0][lj][li] = d1;
+            P.e[1][lj][li] = d2;
+            P.e[2][lj][li] = e1;
+            P.e[3][lj][li] = e2;
+        },
+        [&](int i, int j, int lj, int li) {
+            double d1, d2, e1, e2, x2;
+            vflux_ns(G, i, j, true, d1, d2, e1, e2, x2);
+            P.n[0][lj][li] = d1;
+            P.n[1][lj][li] = d2;
+            P.n[2][lj][li] = e1;
+            P.n[3][lj][li] = e2;
+        });
+    const int li = threadIdx.x, lj = threadIdx.y;
+    const int i = i0 + li + 1, j = j0 + lj + 1;
     const size_t n = (size_t)G.ni * G.nj;
     for (int cp = 0; cp < 2; ++cp) {
         const Comp K = comp_of(G, cp);
@@ -845,23 +905,20 @@ __global__ void __launch_bounds__(256) k_rows(PredG G, Faces F, const double* __
         }
         const int iu = cp == 0, iv = cp == 1;
         const double hx = iu ? UDX(i) : VDX(i), hy = iu ? UDY(j) : VDY(j);
-        const double* de = F.di_ew[cp];
-        const double* dn = F.di_ns[cp];
-        const double* ee = F.ed_ew[cp];
-        const double* en = F.ed_ns[cp];
+        const Quad d = quad(P, cp, li, lj), e = quad(P, 2 + cp, li, lj);
         const double VofAF = 1.0 - T.vofS[c];
-        double De = (1.0 - BETAVIS) * de[gi(G, i + 1, j)] / VofAF;
-        double Dw = (1.0 - BETAVIS) * de[gi(G, i, j)] / VofAF;
-        double Dn = (1.0 - BETAVIS) * dn[gi(G, i, j + 1)] / VofAF;
-        const double Ds = (1.0 - BETAVIS) * dn[gi(G, i, j)] / VofAF;
-        const double Fep = ee[gi(G, i + 1, j)] * (1.0 - T.EtaE[gi(G, i, j)]) / VofAF;
-        const double Fem = ee[gi(G, i + 1, j)] * T.EtaE[gi(G, i, j)] / VofAF;
-        const double Fwp = ee[gi(G, i, j)] * T.EtaE[gi(G, i - 1, j)] / VofAF;
-        const double Fwm = ee[gi(G, i, j)] * (1.0 - T.EtaE[gi(G, i - 1, j)]) / VofAF;
-        const double Fnp = en[gi(G, i, j + 1)] * (1.0 - T.EtaN[gi(G, i, j)]) / VofAF;
-        const double Fnm = en[gi(G, i, j + 1)] * T.EtaN[gi(G, i, j)] / VofAF;
-        const double Fsp = en[gi(G, i, j)] * T.EtaN[gi(G, i, j - 1)] / VofAF;
-        const double Fsm = en[gi(G, i, j)] * (1.0 - T.EtaN[gi(G, i, j - 1)]) / VofAF;
+        double De = (1.0 - BETAVIS) * d.e / VofAF;
+        double Dw = (1.0 - BETAVIS) * d.w / VofAF;
+        double Dn = (1.0 - BETAVIS) * d.n / VofAF;
+        const double Ds = (1.0 - BETAVIS) * d.s / VofAF;
+        const double Fep = e.e * (1.0 - T.EtaE[gi(G, i, j)]) / VofAF;
+        const double Fem = e.e * T.EtaE[gi(G, i, j)] / VofAF;
+        const double Fwp = e.w * T.EtaE[gi(G, i - 1, j)] / VofAF;
+        const double Fwm = e.w * (1.0 - T.EtaE[gi(G, i - 1, j)]) / VofAF;
+        const double Fnp = e.n * (1.0 - T.EtaN[gi(G, i, j)]) / VofAF;
+        const double Fnm = e.n * T.EtaN[gi(G, i, j)] / VofAF;
+        const double Fsp = e.s * T.EtaN[gi(G, i, j - 1)] / VofAF;
+        const double Fsm = e.s * (1.0 - T.EtaN[gi(G, i, j - 1)]) / VofAF;
         if (iu) {
             if (j == G.nj) Dn = 0.0;
             if (i == G.ni - iu) De = 0.0;
@@ -978,10 +1035,6 @@ int predictor_allocate(Ctx* c) {
         S.act = (int*)dev_alloc(c, n * sizeof(int));
         if (!S.moex || !S.act) return SG_ERR_ALLOC;
     }
-    for (int k = 0; k < PredBuffers::NFACE; ++k) {
-        B.face[k] = (double*)dev_alloc(c, ng * sizeof(double));
-        if (!B.face[k]) return SG_ERR_ALLOC;
-    }
     double** cellv[] = {&B.fdu, &B.fdv, &B.gu, &B.gv, &B.ru, &B.rv, &B.xu, &B.xv};
     for (double** p : cellv) {
         *p = (double*)dev_alloc(c, n * sizeof(double));
@@ -1036,27 +1089,12 @@ int predictor_assemble(Ctx* c, double dt, double vb, int first_step) {
     G.vb = vb;
     G.first = first_step ? 1 : 0;
     PredBuffers& B = c->pred;
-    Faces F;
-    int k = 0;
-    for (int cp = 0; cp < 2; ++cp) {
-        F.cf_ew[cp] = B.face[k++]; F.cf_ns[cp] = B.face[k++];
-        F.ma_ew[cp] = B.face[k++]; F.ma_ns[cp] = B.face[k++];
-        F.co_ew[cp] = B.face[k++]; F.co_ns[cp] = B.face[k++];
-        F.di_ew[cp] = B.face[k++]; F.di_ns[cp] = B.face[k++];
-        F.ed_ew[cp] = B.face[k++]; F.ed_ns[cp] = B.face[k++];
-    }
-    F.ex_ew = B.face[k++];
-    F.ex_ns = B.face[k++];
-    const dim3 blk(64, 4);
-    const dim3 gf((c->ni + 1 + 63) / 64, (c->nj + 1 + 3) / 4);
-    const dim3 gc((c->ni + 63) / 64, (c->nj + 3) / 4);
+    const dim3 blk(FTW, FTH), grd((c->ni + FTW - 1) / FTW, (c->nj + FTH - 1) / FTH);
     hipStream_t st = c->stream;
-    hipLaunchKernelGGL(k_faces<true>, gf, blk, 0, st, G, F);
-    hipLaunchKernelGGL(k_conv, gc, blk, 0, st, G, F, (double*)c->field[SG_F_FLUXN1U], (double*)c->field[SG_F_FLUXN1V],
+    hipLaunchKernelGGL(k_conv, grd, blk, 0, st, G, (double*)c->field[SG_F_FLUXN1U], (double*)c->field[SG_F_FLUXN1V],
                        B.gu, B.gv);
-    hipLaunchKernelGGL(k_mix_visc, gc, blk, 0, st, G, F, B.gu, B.gv, B.fdu, B.fdv);
-    hipLaunchKernelGGL(k_faces<false>, gf, blk, 0, st, G, F);
-    hipLaunchKernelGGL(k_rows, gc, blk, 0, st, G, F, B.fdu, B.fdv, B.mu, B.ru, B.mv, B.rv,
+    hipLaunchKernelGGL(k_mix_visc, grd, blk, 0, st, G, B.gu, B.gv, B.fdu, B.fdv);
+    hipLaunchKernelGGL(k_rows, grd, blk, 0, st, G, B.fdu, B.fdv, B.mu, B.ru, B.mv, B.rv,
                        (double*)c->field[SG_F_DPU], (double*)c->field[SG_F_DPV]);
     SG_HIP(hipGetLastError());
     return SG_OK;
