@@ -13,8 +13,8 @@
 //   k_recon, k_recon_ifc, k_dis_fix, k_flux_pre
 //   k_sweep      second sweep, explicit divergence (:834-858 / :898-929)
 //   k_bc, k_recon (+ interface bit mask and source list), k_recon_ifc, k_dis_fix
-//   k_rd_reset, k_rd_scatter, k_redistance: level set from the PLIC
-//                segments (Redistance :1525-1708)
+//   k_rd_scatter, k_rd_dilate, k_redistance_far, k_redistance_band: level
+//                set from the PLIC segments (Redistance :1525-1708)
 //
 // The reference's sweeps walk each row in order and update cells in place;
 // every face flux they use depends only on data from before the sweep, so a
@@ -1124,37 +1124,25 @@ __device__ __forceinline__ void apply(const Visit& w, const Target& T, double& p
     }
 }
 
+constexpr unsigned long long BEST_NONE = 0x7FF0000000000000ull;  // +inf: no proposal yet
+
 __device__ __forceinline__ bool rd_source(double v, double s) { return (0.0 + VOFEPS) < v && v < (1.0 - VOFEPS - s); }
 __device__ __forceinline__ bool rd_target(double v, double s) { return v < 0.0 + VOFEPS || v >= 1.0 - VOFEPS - s; }
 
 // (source, offset) pairs of the listed interface cells; pair p is source
-// p/81 and window offset p%81
+// p/81 and window offset o = p%81: column offset o%9 - BAND, row offset
+// o/9 - BAND (consecutive lanes walk along a row of the window)
 struct PairView {
     const int* srcs;
     const unsigned* cnt;
 };
 
-// Pass 1 over the pairs: reset the minimum of every scattered-to target.
-__global__ void __launch_bounds__(256) k_rd_reset(ClsGeo G, const double* __restrict__ vof, PairView pv,
-                                                  unsigned long long* __restrict__ best,
-                                                  unsigned char* __restrict__ hit) {
-    const unsigned long long np = (unsigned long long)pv.cnt[0] * 81ull;
-    for (unsigned long long p = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; p < np;
-         p += (unsigned long long)gridDim.x * blockDim.x) {
-        const int c0 = pv.srcs[p / 81];
-        const int o = (int)(p % 81);
-        const int ti = c0 % G.ni + 1 + o / 9 - BAND, tj = c0 / G.ni + 1 + o % 9 - BAND;
-        if (ti < 1 || ti > G.ni || tj < 1 || tj > G.nj) continue;
-        const size_t t = cix(G, ti, tj);
-        best[t] = 0x7FF0000000000000ull;  // +inf
-        hit[t] = 0;
-    }
-}
-
-// Pass 2: each pair proposes a distance to its target; two- and
-// three-phase targets keep the minimum (the bit pattern of a non-negative
-// double orders like the value).  Full-solid targets are left to the
-// ordered gather of k_redistance.
+// Each pair proposes a distance to its target; two- and three-phase
+// targets keep the minimum (the bit pattern of a non-negative double orders
+// like the value).  Full-solid targets are left to the ordered gather of
+// k_redistance_band.  `best` and `hit` hold +inf and 0 on entry: every cell
+// a pair can reach lies in the band, and k_redistance_band puts them back
+// after reading them.
 __global__ void __launch_bounds__(256) k_rd_scatter(ClsGeo G, const double* __restrict__ vof,
                                                     const double* __restrict__ phi, Planes P, PairView pv,
                                                     unsigned long long* __restrict__ best,
@@ -1165,7 +1153,7 @@ __global__ void __launch_bounds__(256) k_rd_scatter(ClsGeo G, const double* __re
         const int c0 = pv.srcs[p / 81];
         const int o = (int)(p % 81);
         const int si = c0 % G.ni + 1, sj = c0 / G.ni + 1;
-        const int ii = o / 9 - BAND, jj = o % 9 - BAND;
+        const int ii = o % 9 - BAND, jj = o / 9 - BAND;
         const int ti = si + ii, tj = sj + jj;
         if (ti < 1 || ti > G.ni || tj < 1 || tj > G.nj) continue;
         const size_t t = cix(G, ti, tj);
@@ -1279,20 +1267,27 @@ __device__ void redistance_cell(const ClsGeo& G, const double* __restrict__ vof,
 }
 
 // the cells of the listed band words, one wavefront per word (the window
-// scans and nearest-distance gathers; few words, many registers)
+// scans and nearest-distance gathers; few words, many registers).  Each
+// cell's minimum and hit flag go back to +inf and 0 once read, ready for
+// the next k_rd_scatter: the band holds every cell a pair can reach.
 __global__ void __launch_bounds__(256) k_redistance_band(ClsGeo G, const double* __restrict__ vof, Planes P,
                                                          const unsigned long long* __restrict__ mask, int mw,
                                                          const int* __restrict__ band,
                                                          const unsigned* __restrict__ nband,
-                                                         const unsigned long long* __restrict__ best,
-                                                         const unsigned char* __restrict__ hit,
+                                                         unsigned long long* __restrict__ best,
+                                                         unsigned char* __restrict__ hit,
                                                          double* __restrict__ phi) {
     const unsigned n = *nband;
     const int lane = threadIdx.x & 63;
     for (unsigned e = blockIdx.x * 4 + (threadIdx.x >> 6); e < n; e += gridDim.x * 4) {
         const int k = band[e];
         const int ti = (k % mw) * 64 + lane + 1, tj = k / mw + 1;
-        if (ti <= G.ni) redistance_cell(G, vof, P, mask, mw, best, hit, phi, ti, tj, true);
+        if (ti <= G.ni) {
+            redistance_cell(G, vof, P, mask, mw, best, hit, phi, ti, tj, true);
+            const size_t c = cix(G, ti, tj);
+            best[c] = BEST_NONE;
+            hit[c] = 0;
+        }
     }
 }
 
@@ -1312,6 +1307,11 @@ __global__ void __launch_bounds__(256) k_redistance_far(ClsGeo G, const double* 
         phi[c] = (double)(BAND + 1) * sqrt(hx * hx + hy * hy);
     else if (v >= (1.0 - VOFEPS - s))
         phi[c] = -(double)(BAND + 1) * sqrt(hx * hx + hy * hy);
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ a, size_t n, unsigned long long v) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
 }
 
 __global__ void k_zero(unsigned* __restrict__ a, int n) {
@@ -1496,6 +1496,11 @@ int clsvof_allocate(Ctx* c) {
     B.stale = (unsigned char*)dev_alloc(c, n);
     B.hit = (unsigned char*)dev_alloc(c, n);
     B.best = (unsigned long long*)dev_alloc(c, n * sizeof(unsigned long long));
+    if (B.best) {
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, B.best, n,
+                           BEST_NONE);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) return SG_ERR_ALLOC;
+    }
     B.srcs = (int*)dev_alloc(c, n * sizeof(int));
     B.stales = (int*)dev_alloc(c, n * sizeof(int));
     B.ifc = (int*)dev_alloc(c, n * sizeof(int));
@@ -1629,7 +1634,6 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         const unsigned* cnt = recon(vof, phi, true, nullptr);
         last_scan = cnt;
         const PairView pv{B.srcs, cnt};
-        hipLaunchKernelGGL(k_rd_reset, pgrd, pblk, 0, st, G, vof, pv, B.best, B.hit);
         hipLaunchKernelGGL(k_rd_scatter, pgrd, pblk, 0, st, G, vof, phi, P, pv, B.best, B.hit);
         unsigned* nband = B.cnt + 9 * nsub + (k - 1);
         hipLaunchKernelGGL(k_rd_dilate, dim3((unsigned)((B.mw * c->nj + 255) / 256)), dim3(256), 0, st, c->nj, B.mw,
