@@ -110,6 +110,11 @@ __device__ __forceinline__ double edge_mix(const PredG& G, double vf, double vs)
     const double tol = 1.0e-24;
     return ((1.0 - vf / (1.0 - vs + tol)) * G.nug / G.nur + vf / (1.0 - vs + tol) * G.nul / G.nur);
 }
+// vofS > EPSI in any of four cells; the four loads issue together
+__device__ __forceinline__ bool any_solid4(const double* vofS, size_t a, size_t b, size_t c, size_t d) {
+    const double s0 = vofS[a], s1 = vofS[b], s2 = vofS[c], s3 = vofS[d];
+    return (s0 > EPSI) | (s1 > EPSI) | (s2 > EPSI) | (s3 > EPSI);
+}
 __device__ __forceinline__ void upwind(double w, double& p, double& m) {
     p = 0.5 * (w + fabs(w));
     m = 0.5 * (w - fabs(w));
@@ -182,8 +187,7 @@ __device__ void cflux_ew(const PredG& G, int i, int j, double& f1, double& f2) {
     double alr = fmax(fabs(ur), fabs(ul));
     f1 = 0.5 * (sq(ur) + sq(ul) - fabs(alr) * (ur - ul)) * Uc.aW[gi(G, i, j)] * UDY(j);
     if (i > 2 && i < ni) {
-        if (Uc.vofS[ci(G, i - 2, j)] > EPSI || Uc.vofS[ci(G, i - 1, j)] > EPSI || Uc.vofS[ci(G, i, j)] > EPSI ||
-            Uc.vofS[ci(G, i + 1, j)] > EPSI) {
+        if (any_solid4(Uc.vofS, ci(G, i - 2, j), ci(G, i - 1, j), ci(G, i, j), ci(G, i + 1, j))) {
             const double eta = Uc.EtaE[gi(G, i - 1, j)];
             const double uw = (1 - eta) * UU(i - 1, j) + eta * UU(i, j);
             double p, m;
@@ -202,8 +206,7 @@ __device__ void cflux_ew(const PredG& G, int i, int j, double& f1, double& f2) {
     alr = 0.5 * (UU(i - 1, j) + UU(i - 1, j + 1));
     f2 = 0.5 * (alr * vr + alr * vl - fabs(alr) * (vr - vl)) * Vc.aW[gi(G, i, j)] * VDY(j);
     if (i > 2 && i < ni) {
-        if (Vc.vofS[ci(G, i - 2, j)] > EPSI || Vc.vofS[ci(G, i - 1, j)] > EPSI || Vc.vofS[ci(G, i, j)] > EPSI ||
-            Vc.vofS[ci(G, i + 1, j)] > EPSI) {
+        if (any_solid4(Vc.vofS, ci(G, i - 2, j), ci(G, i - 1, j), ci(G, i, j), ci(G, i + 1, j))) {
             if (Vc.aW[gi(G, i, j)] < 0.5) {
                 const size_t c1 = ci(G, i - 1, j);
                 const double delhec =
@@ -305,8 +308,7 @@ __device__ void cflux_ns(const PredG& G, int i, int j, double& f1, double& f2) {
     double alr = 0.5 * (VV(i, j - 1) + VV(i + 1, j - 1));
     f1 = 0.5 * (alr * ur + alr * ul - fabs(alr) * (ur - ul)) * Uc.aS[gi(G, i, j)] * UDX(i);
     if (j > 2 && j < nj) {
-        if (Uc.vofS[ci(G, i, j - 2)] > EPSI || Uc.vofS[ci(G, i, j - 1)] > EPSI || Uc.vofS[ci(G, i, j)] > EPSI ||
-            Uc.vofS[ci(G, i, j + 1)] > EPSI) {
+        if (any_solid4(Uc.vofS, ci(G, i, j - 2), ci(G, i, j - 1), ci(G, i, j), ci(G, i, j + 1))) {
             if (Uc.aS[gi(G, i, j)] < 0.5) {
                 const size_t c1 = ci(G, i, j - 1);
                 const double delhec =
@@ -348,8 +350,7 @@ __device__ void cflux_ns(const PredG& G, int i, int j, double& f1, double& f2) {
     alr = fmax(fabs(vl), fabs(vr));
     f2 = 0.5 * (vr * vr + vl * vl - fabs(alr) * (vr - vl)) * Vc.aS[gi(G, i, j)] * VDX(i);
     if (j > 2 && j < nj) {
-        if (Vc.vofS[ci(G, i, j - 2)] > EPSI || Vc.vofS[ci(G, i, j - 1)] > EPSI || Vc.vofS[ci(G, i, j)] > EPSI ||
-            Vc.vofS[ci(G, i, j + 1)] > EPSI) {
+        if (any_solid4(Vc.vofS, ci(G, i, j - 2), ci(G, i, j - 1), ci(G, i, j), ci(G, i, j + 1))) {
             const double eta = Vc.EtaN[gi(G, i, j - 1)];
             double vs = (1.0 - eta) * VV(i, j - 1) + eta * VV(i, j);
             vs = (vb + (vs - vb) * Vc.AlN[gi(G, i, j - 1)]);
@@ -379,14 +380,15 @@ __device__ void dflux_ew(const PredG& G, int i, int j, double& m1, double& m2, d
         if (Uc.aE[gi(G, ni, j)] >= EPSI) w1 = mix_rho(G, G.pvof[ci(G, ni, j)], G.pvofS[ci(G, ni, j)]);
         m1 = uw * w1 * UDY(j);
     } else {
-        if (Uc.aW[gi(G, i, j)] >= EPSI) {
-            const double eta = Uc.EtaE[gi(G, i - 1, j)];
+        // operands of every branch below, loaded before the first test
+        const size_t c = ci(G, i, j);
+        const double aw = Uc.aW[gi(G, i, j)], eta = Uc.EtaE[gi(G, i - 1, j)], al = Uc.AlE[gi(G, i - 1, j)];
+        const double pv = G.pvof[c], pvs = G.pvofS[c];
+        if (aw >= EPSI) {
             uw = (1.0 - eta) * UU(i - 1, j) + eta * UU(i, j);
-            const size_t c = ci(G, i, j);
-            const double pv = G.pvof[c];
             if (pv > 1.0 - EPSI || pv < EPSI) {
-                w1 = mix_rho(G, pv, G.pvofS[c]);
-            } else if (G.pvofS[c] < EPSI) {
+                w1 = mix_rho(G, pv, pvs);
+            } else if (pvs < EPSI) {
                 const double volf = pfrac(G.pnx[c], G.pny[c], G.pphi[c] + uw * dt / 2.0 * G.pnx[c], fabs(uw * dt), PDY(j));
                 w1 = (volf * G.rl / G.rr + (1.0 - volf - 0.0) * G.rg / G.rr);
             } else {
@@ -397,7 +399,7 @@ __device__ void dflux_ew(const PredG& G, int i, int j, double& m1, double& m2, d
                 w1 = (volf * G.rl / G.rr + (1.0 - volf - vols) * G.rg / G.rr);
             }
         }
-        m1 = uw * Uc.AlE[gi(G, i - 1, j)] * w1 * UDY(j);
+        m1 = uw * al * w1 * UDY(j);
     }
     // density-weighted U momentum flux
     {
@@ -435,9 +437,11 @@ __device__ void dflux_ew(const PredG& G, int i, int j, double& m1, double& m2, d
         const double w = (1.0 - eta) * UU(i - 1, j) + eta * UU(i - 1, j + 1);
         double p, m;
         upwind(w, p, m);
+        const size_t ca = ci(G, i - 1, j), cb = ci(G, i, j);
+        const double fa = Vc.vofo[ca], sa = Vc.vofS[ca], fb = Vc.vofo[cb], sb = Vc.vofS[cb];
         if (Vc.aW[gi(G, i, j)] >= EPSI) {
-            const double ra = rho_mix(G, Vc.vofo[ci(G, i - 1, j)], Vc.vofS[ci(G, i - 1, j)]);
-            const double rb = rho_mix(G, Vc.vofo[ci(G, i, j)], Vc.vofS[ci(G, i, j)]);
+            const double ra = rho_mix(G, fa, sa);
+            const double rb = rho_mix(G, fb, sb);
             m2 = (p * ra + m * rb) * VDY(j);
             k2 = (p * VV(i - 1, j) * ra + m * VV(i, j) * rb) * VDY(j);
         }
@@ -473,9 +477,11 @@ __device__ void dflux_ns(const PredG& G, int i, int j, double& m1, double& m2, d
         const double w = (1.0 - eta) * VV(i, j - 1) + eta * VV(i + 1, j - 1);
         double p, m;
         upwind(w, p, m);
+        const size_t ca = ci(G, i, j - 1), cb = ci(G, i, j);
+        const double fa = Uc.vofo[ca], sa = Uc.vofS[ca], fb = Uc.vofo[cb], sb = Uc.vofS[cb];
         if (Uc.aS[gi(G, i, j)] >= EPSI) {
-            const double ra = rho_mix(G, Uc.vofo[ci(G, i, j - 1)], Uc.vofS[ci(G, i, j - 1)]);
-            const double rb = rho_mix(G, Uc.vofo[ci(G, i, j)], Uc.vofS[ci(G, i, j)]);
+            const double ra = rho_mix(G, fa, sa);
+            const double rb = rho_mix(G, fb, sb);
             m1 = (p * ra + m * rb) * UDX(i);
             k1 = (p * UU(i, j - 1) * ra + m * UU(i, j) * rb) * UDX(i);
         }
@@ -490,24 +496,28 @@ __device__ void dflux_ns(const PredG& G, int i, int j, double& m1, double& m2, d
     } else if (j == 1) {
         if (Vc.aS[gi(G, i, j)] >= EPSI) w2 = mix_rho(G, G.pvof[ci(G, i, j)], G.pvofS[ci(G, i, j)]);
         m2 = vs * w2 * VDX(i);
-    } else if (Vc.aS[gi(G, i, j)] >= EPSI) {
-        const double eta = Vc.EtaN[gi(G, i, j - 1)];
-        double w = (1.0 - eta) * VV(i, j - 1) + eta * VV(i, j);
-        w = ((w - vb) * Vc.AlN[gi(G, i, j - 1)] + vb);
+    } else {
+        // operands of every branch below, loaded before the first test
         const size_t c = ci(G, i, j);
-        const double pv = G.pvof[c];
-        if (pv > 1.0 - EPSI || pv < EPSI) {
-            w2 = rho_mix(G, pv, G.pvofS[c]);
-        } else if (G.pvofS[c] < EPSI) {
-            const double volf = pfrac(G.pnx[c], G.pny[c], G.pphi[c] + w * dt / 2.0 * G.pny[c], PDX(i), fabs(w * dt));
-            w2 = (volf * G.rl / G.rr + (1.0 - volf - 0.0) * G.rg / G.rr);
-        } else {
-            double vols, volf;
-            cut_cell_fractions(G.pnxS[c], G.pnyS[c], G.pphiS[c] + w * dt / 2.0 * G.pnyS[c], G.pnx[c], G.pny[c],
-                               G.pphi[c] + w * dt / 2.0 * G.pny[c], PDX(i), fabs(w * dt), vols, volf);
-            w2 = (volf * G.rl / G.rr + (1.0 - volf - vols) * G.rg / G.rr);
+        const double as = Vc.aS[gi(G, i, j)], eta = Vc.EtaN[gi(G, i, j - 1)], al = Vc.AlN[gi(G, i, j - 1)];
+        const double pv = G.pvof[c], pvs = G.pvofS[c];
+        if (as >= EPSI) {
+            double w = (1.0 - eta) * VV(i, j - 1) + eta * VV(i, j);
+            w = ((w - vb) * al + vb);
+            if (pv > 1.0 - EPSI || pv < EPSI) {
+                w2 = rho_mix(G, pv, pvs);
+            } else if (pvs < EPSI) {
+                const double volf =
+                    pfrac(G.pnx[c], G.pny[c], G.pphi[c] + w * dt / 2.0 * G.pny[c], PDX(i), fabs(w * dt));
+                w2 = (volf * G.rl / G.rr + (1.0 - volf - 0.0) * G.rg / G.rr);
+            } else {
+                double vols, volf;
+                cut_cell_fractions(G.pnxS[c], G.pnyS[c], G.pphiS[c] + w * dt / 2.0 * G.pnyS[c], G.pnx[c], G.pny[c],
+                                   G.pphi[c] + w * dt / 2.0 * G.pny[c], PDX(i), fabs(w * dt), vols, volf);
+                w2 = (volf * G.rl / G.rr + (1.0 - volf - vols) * G.rg / G.rr);
+            }
+            m2 = w * w2 * VDX(i);
         }
-        m2 = w * w2 * VDX(i);
     }
     // density-weighted V momentum flux
     if (j >= nj || j == 1) {
@@ -548,15 +558,17 @@ __device__ void vflux_ew(const PredG& G, int i, int j, bool newvof, double& d1, 
     double Sy = Uc.ccy[ci(G, i, j)] - Uc.ccy[ci(G, i - 1, j)];
     double vf = 0.5 * (uv[ci(G, i, j)] / (1.0 - Uc.vofS[ci(G, i, j)] + tol) +
                        uv[ci(G, i - 1, j)] / (1.0 - Uc.vofS[ci(G, i - 1, j)] + tol));
+    const double dal1 = Uc.DAlE[gi(G, i - 1, j)];
     d1 = nu_face(G, vf) / Rey * Uc.aW[gi(G, i, j)] * UDY(j) / Sx;
-    if (fabs(Sy) >= 1.0e-3 * UDY(j)) e1 = nu_face(G, vf) / Rey * Uc.aW[gi(G, i, j)] * Uc.DAlE[gi(G, i - 1, j)] * UDY(j);
+    if (fabs(Sy) >= 1.0e-3 * UDY(j)) e1 = nu_face(G, vf) / Rey * Uc.aW[gi(G, i, j)] * dal1 * UDY(j);
     Sx = Vc.SxE[gi(G, i - 1, j)];
     Sy = Vc.ccy[ci(G, i, j)] - Vc.ccy[ci(G, i - 1, j)];
     vf = 0.5 * (vv[ci(G, i, j)] / (1.0 - Vc.vofS[ci(G, i, j)] + tol) +
                 vv[ci(G, i - 1, j)] / (1.0 - Vc.vofS[ci(G, i - 1, j)] + tol));
+    const double dal2 = Vc.DAlE[gi(G, i - 1, j)];
     d2 = nu_face(G, vf) / Rey * Vc.aW[gi(G, i, j)] * VDY(j) / Sx;
     if (fabs(Sy) >= 1.0e-3 * VDY(j)) {
-        e2 = nu_face(G, vf) / Rey * Vc.aW[gi(G, i, j)] * Vc.DAlE[gi(G, i - 1, j)] * VDY(j);
+        e2 = nu_face(G, vf) / Rey * Vc.aW[gi(G, i, j)] * dal2 * VDY(j);
         x2 = G.vb * e2;
     }
 }
@@ -584,15 +596,17 @@ __device__ void vflux_ns(const PredG& G, int i, int j, bool newvof, double& d1, 
     double Sy = Uc.SyN[gi(G, i, j - 1)];
     double vf = 0.5 * (uv[ci(G, i, j - 1)] / (1.0 - Uc.vofS[ci(G, i, j - 1)] + tol) +
                        uv[ci(G, i, j)] / (1.0 - Uc.vofS[ci(G, i, j)] + tol));
+    const double dal1 = Uc.DAlN[gi(G, i, j - 1)];
     d1 = nu_face(G, vf) / Rey * Uc.aS[gi(G, i, j)] * UDX(i) / Sy;
-    if (fabs(Sx) >= 1.0e-3 * UDX(i)) e1 = nu_face(G, vf) / Rey * Uc.aS[gi(G, i, j)] * Uc.DAlN[gi(G, i, j - 1)] * UDX(i);
+    if (fabs(Sx) >= 1.0e-3 * UDX(i)) e1 = nu_face(G, vf) / Rey * Uc.aS[gi(G, i, j)] * dal1 * UDX(i);
     Sx = Vc.ccx[ci(G, i, j)] - Vc.ccx[ci(G, i, j - 1)];
     Sy = Vc.SyN[gi(G, i, j - 1)];
     vf = 0.5 * (vv[ci(G, i, j - 1)] / (1.0 - Vc.vofS[ci(G, i, j - 1)] + tol) +
                 vv[ci(G, i, j)] / (1.0 - Vc.vofS[ci(G, i, j)] + tol));
+    const double dal2 = Vc.DAlN[gi(G, i, j - 1)];
     d2 = nu_face(G, vf) / Rey * Vc.aS[gi(G, i, j)] * VDX(i) / Sy;
     if (fabs(Sx) >= 1.0e-3 * VDX(i)) {
-        e2 = nu_face(G, vf) / Rey * Vc.aS[gi(G, i, j)] * Vc.DAlN[gi(G, i, j - 1)] * VDX(i);
+        e2 = nu_face(G, vf) / Rey * Vc.aS[gi(G, i, j)] * dal2 * VDX(i);
         x2 = G.vb * e2;
     }
 }
