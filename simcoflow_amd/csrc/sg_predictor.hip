@@ -752,14 +752,19 @@ __global__ void __launch_bounds__(FNT) k_conv(PredG G, double* __restrict__ fn1u
     }
 }
 
-// mixing of source cell (si,sj) of component K (Mixing_Procedure :2139-2171):
-// Fm[k], 1/Vt targets; returns false when (si,sj) is not a mixing source
-__device__ bool mix_source(const PredG& G, const Comp& K, const double* f, int si, int sj, double Fm[3], double Vt[3],
+// whether cell (si,sj) of component K is a mixing source (inside the grid,
+// vofS >= 0.5 + EPSI and active; Mixing_Procedure :2139-2141); vs, act are
+// its vofS and activity
+__device__ __forceinline__ bool is_mix_source(const Comp& K, int si, int sj, double vs, int act) {
+    return si >= 1 && si <= K.imax && sj >= 1 && sj <= K.jmax && vs >= 0.5 + EPSI && act;
+}
+
+// mixing of source cell (si,sj) of component K (Mixing_Procedure :2139-2171),
+// a mixing source: Fm[k], 1/Vt targets
+__device__ void mix_source(const PredG& G, const Comp& K, const double* f, int si, int sj, double Fm[3], double Vt[3],
                            int tx[3], int ty[3], bool ok[3]) {
     const StagV& T = *K.T;
-    if (si < 1 || si > K.imax || sj < 1 || sj > K.jmax) return false;
     const size_t c = ci(G, si, sj);
-    if (!(T.vofS[c] >= 0.5 + EPSI && T.act[c])) return false;
     const int sx = (int)fsign(1.0, T.nxS[c]), sy = (int)fsign(1.0, T.nyS[c]);
     tx[0] = si + sx; ty[0] = sj;
     tx[1] = si;      ty[1] = sj + sy;
@@ -792,7 +797,6 @@ __device__ bool mix_source(const PredG& G, const Comp& K, const double* f, int s
             ok[k] = false;
         }
     }
-    return true;
 }
 
 // mixing (gathered in the reference's scatter order) + explicit viscous
@@ -829,13 +833,27 @@ __global__ void __launch_bounds__(FNT) k_mix_visc(PredG G, const double* __restr
         const double* f = cp == 0 ? fpre_u : fpre_v;
         const size_t c = ci(G, i, j);
         double val = f[c];
-        // sources (si,sj) in loop order i outer, j inner over the 3x3 window
-        for (int si = i - 1; si <= i + 1; ++si) {
-            for (int sj = j - 1; sj <= j + 1; ++sj) {
+        // the mixing sources of the 3x3 window as bits 3a+b of source
+        // (i-1+a, j-1+b), their solid fractions and activity loaded
+        // together (indices clamped to the grid; the range is in the test)
+        unsigned src = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const int si = i - 1 + a, sj = j - 1 + b;
+                const size_t cw = ci(G, min(max(si, 1), K.imax), min(max(sj, 1), K.jmax));
+                if (is_mix_source(K, si, sj, T.vofS[cw], T.act[cw])) src |= 1u << (3 * a + b);
+            }
+        // sources (si,sj) in loop order i outer, j inner: ascending bits
+        for (; src; src &= src - 1u) {
+            const int bit = __builtin_ctz(src);
+            const int si = i - 1 + bit / 3, sj = j - 1 + bit % 3;
+            {
                 double Fm[3], Vt[3];
                 int tx[3], ty[3];
                 bool ok[3];
-                if (!mix_source(G, K, f, si, sj, Fm, Vt, tx, ty, ok)) continue;
+                mix_source(G, K, f, si, sj, Fm, Vt, tx, ty, ok);
                 if (si == i && sj == j) {
                     val = f[c] + 1.0 / (1.0 - T.vofS[c]) * ((Fm[0] + Fm[1]) + Fm[2]);
                     continue;
@@ -912,7 +930,27 @@ __global__ void __launch_bounds__(FNT) k_rows(PredG G, const double* __restrict_
         const size_t c = ci(G, i, j);
         double* m = cp == 0 ? mu : mv;
         double* r = cp == 0 ? ru : rv;
-        if (i > K.imax || j > K.jmax || !T.act[c]) {
+        bool act = false;
+        double vs = 0.0, etaE = 0.0, etaW = 0.0, etaN = 0.0, etaS = 0.0, vf = 0.0, wl = 0.0, dh = 0.0, fdc = 0.0;
+        bool actS = false, actW = false, actE = false, actN = false;
+        if (i <= K.imax && j <= K.jmax) {
+            // every operand of the row, loaded before the activity test
+            act = T.act[c] != 0;
+            vs = T.vofS[c];
+            etaE = T.EtaE[gi(G, i, j)];
+            etaW = T.EtaE[gi(G, i - 1, j)];
+            etaN = T.EtaN[gi(G, i, j)];
+            etaS = T.EtaN[gi(G, i, j - 1)];
+            vf = T.vofn[c];
+            wl = T.WlLh[c];
+            dh = T.delh[c];
+            fdc = (cp == 0 ? fdu : fdv)[c];
+            actS = T.act[ci(G, i, max(j - 1, 1))] != 0;
+            actW = T.act[ci(G, max(i - 1, 1), j)] != 0;
+            actE = T.act[ci(G, min(i + 1, G.ni), j)] != 0;
+            actN = T.act[ci(G, i, min(j + 1, G.nj))] != 0;
+        }
+        if (!act) {
             for (int k = 0; k < 5; ++k) m[k * n + c] = 0.0;
             r[c] = 0.0;
             continue;
@@ -920,19 +958,19 @@ __global__ void __launch_bounds__(FNT) k_rows(PredG G, const double* __restrict_
         const int iu = cp == 0, iv = cp == 1;
         const double hx = iu ? UDX(i) : VDX(i), hy = iu ? UDY(j) : VDY(j);
         const Quad d = quad(P, cp, li, lj), e = quad(P, 2 + cp, li, lj);
-        const double VofAF = 1.0 - T.vofS[c];
+        const double VofAF = 1.0 - vs;
         double De = (1.0 - BETAVIS) * d.e / VofAF;
         double Dw = (1.0 - BETAVIS) * d.w / VofAF;
         double Dn = (1.0 - BETAVIS) * d.n / VofAF;
         const double Ds = (1.0 - BETAVIS) * d.s / VofAF;
-        const double Fep = e.e * (1.0 - T.EtaE[gi(G, i, j)]) / VofAF;
-        const double Fem = e.e * T.EtaE[gi(G, i, j)] / VofAF;
-        const double Fwp = e.w * T.EtaE[gi(G, i - 1, j)] / VofAF;
-        const double Fwm = e.w * (1.0 - T.EtaE[gi(G, i - 1, j)]) / VofAF;
-        const double Fnp = e.n * (1.0 - T.EtaN[gi(G, i, j)]) / VofAF;
-        const double Fnm = e.n * T.EtaN[gi(G, i, j)] / VofAF;
-        const double Fsp = e.s * T.EtaN[gi(G, i, j - 1)] / VofAF;
-        const double Fsm = e.s * (1.0 - T.EtaN[gi(G, i, j - 1)]) / VofAF;
+        const double Fep = e.e * (1.0 - etaE) / VofAF;
+        const double Fem = e.e * etaE / VofAF;
+        const double Fwp = e.w * etaW / VofAF;
+        const double Fwm = e.w * (1.0 - etaW) / VofAF;
+        const double Fnp = e.n * (1.0 - etaN) / VofAF;
+        const double Fnm = e.n * etaN / VofAF;
+        const double Fsp = e.s * etaS / VofAF;
+        const double Fsm = e.s * (1.0 - etaS) / VofAF;
         if (iu) {
             if (j == G.nj) Dn = 0.0;
             if (i == G.ni - iu) De = 0.0;
@@ -942,23 +980,22 @@ __global__ void __launch_bounds__(FNT) k_rows(PredG G, const double* __restrict_
         }
         double aE = De, aW = Dw, aN = Dn, aS = Ds;
         const double cwe1 = aW, cwe2 = aE, csn1 = aS, csn2 = aN;
-        const double vf = T.vofn[c], vs = T.vofS[c];
         double aP = hx * hy / G.dt + aE + aW + aN + aS +
-                    ((1.0 - vf / (1.0 - vs)) * G.nug / G.nur + vf / (1.0 - vs) * G.nul / G.nur) / G.Re * T.WlLh[c] /
-                        T.delh[c] / VofAF;
+                    ((1.0 - vf / (1.0 - vs)) * G.nug / G.nur + vf / (1.0 - vs) * G.nul / G.nur) / G.Re * wl / dh /
+                        VofAF;
         aP = aP + Fep - Fwp + Fnp - Fsp;
         aE = aE - Fem;
         aW = aW + Fwm;
         aN = aN - Fnm;
         aS = aS + Fsm;
         (cp == 0 ? DpU : DpV)[gi(G, i, j)] = hx * hy / (aP - aE - aW - aN - aS);
-        m[0 * n + c] = (j > 1 && T.act[ci(G, i, j - 1)]) ? -aS : 0.0;
-        m[1 * n + c] = (i > 1 && T.act[ci(G, i - 1, j)]) ? -aW : 0.0;
+        m[0 * n + c] = (j > 1 && actS) ? -aS : 0.0;
+        m[1 * n + c] = (i > 1 && actW) ? -aW : 0.0;
         m[2 * n + c] = aP;
-        m[3 * n + c] = (i < G.ni && T.act[ci(G, i + 1, j)]) ? -aE : 0.0;
-        m[4 * n + c] = (j < G.nj && T.act[ci(G, i, j + 1)]) ? -aN : 0.0;
+        m[3 * n + c] = (i < G.ni && actE) ? -aE : 0.0;
+        m[4 * n + c] = (j < G.nj && actN) ? -aN : 0.0;
         double b = 0.0;
-        b = b - (cp == 0 ? fdu : fdv)[c];
+        b = b - fdc;
         if (iv) b = b - 1.0 / (G.Fr * G.Fr) * hx * hy;
         const double* uu = G.u;
         const double* vv = G.v;
