@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(NT) k_bi_update(size_t n, Mat5 A, const BiScal
 enum { F_INIT = 0, F_RHO, F_ALPHA, F_OMEGA, F_RR };
 
 // JB Jacobi sweeps x_{k+1} = x_k + D^-1 (b - A x_k) per launch (x_0 = 0 on
-// the first launch), temporally blocked: a 64 x 16 tile stages x_k on the
+// the first launch), temporally blocked: a 64 x 15 tile stages x_k on the
 // tile plus a JB-cell halo in LDS and sweeps it JB times, each sweep on a
 // ring narrower by one, so the rows' coefficients are read from HBM once per
 // JB sweeps instead of once per sweep.  Each thread holds the coefficients of
@@ -161,13 +161,16 @@ enum { F_INIT = 0, F_RHO, F_ALPHA, F_OMEGA, F_RR };
 // bit for bit.  Partials: sweep q's ||D^-1 (b - A x_k)||^2 over the tiles'
 // own cells, in plane q.
 constexpr int JB = 4;                                       // sweeps per launch
-constexpr int JTW = 64, JTH = 16;                           // tile
+constexpr int JTW = 64, JTH = 15;                           // tile
+constexpr int JNT = 512;                                    // threads per tile
 constexpr int JRW = JTW + 2 * JB, JRH = JTH + 2 * JB;       // x_k staged: tile + JB halo
 constexpr int JCW = JRW - 2, JCH = JRH - 2;                 // cells the first sweep updates
-constexpr int JM = (JCW * JCH + NT - 1) / NT;               // of them per thread
+constexpr int JM = (JCW * JCH + JNT - 1) / JNT;             // of them per thread (3: the
+                                                            // coefficients stay in registers
+                                                            // at 6 waves per SIMD)
 static_assert(JB <= 4, "the momentum partials buffer holds 4 planes");
 
-__global__ void __launch_bounds__(NT) k_jac_blk(Mat5 A, const double* __restrict__ b, const double* __restrict__ xin,
+__global__ void __launch_bounds__(JNT) k_jac_blk(Mat5 A, const double* __restrict__ b, const double* __restrict__ xin,
                                                 double* __restrict__ xout, double* __restrict__ part,
                                                 const BiScalars* __restrict__ sc, int first) {
     if (!first && !sc->active) return;
@@ -180,7 +183,7 @@ __global__ void __launch_bounds__(NT) k_jac_blk(Mat5 A, const double* __restrict
     for (int q = 0; q < JB; ++q) acc[q] = 0.0;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int i0 = (tile % tnx) * JTW, j0 = (tile / tnx) * JTH;
-        for (int k = tid; k < JRW * JRH; k += NT) {
+        for (int k = tid; k < JRW * JRH; k += JNT) {
             const int gi = i0 + k % JRW - JB, gj = j0 + k / JRW - JB;
             double v = 0.0;
             if (!first && gi >= 0 && gj >= 0 && gi < ni && gj < nj) v = xin[(size_t)gj * ni + gi];
@@ -191,7 +194,7 @@ __global__ void __launch_bounds__(NT) k_jac_blk(Mat5 A, const double* __restrict
         int pos[JM], gij[JM];
 #pragma unroll
         for (int m = 0; m < JM; ++m) {
-            const int k = tid + m * NT;
+            const int k = tid + m * JNT;
             pos[m] = -1;
             cd[m] = cs[m] = cw[m] = ce[m] = cn[m] = cb[m] = ci[m] = 0.0;
             gij[m] = 0;
@@ -240,14 +243,14 @@ __global__ void __launch_bounds__(NT) k_jac_blk(Mat5 A, const double* __restrict
             __syncthreads();
         }
         // x after the JB sweeps: the tile's cells, from buffer JB & 1
-        for (int k = tid; k < JTW * JTH; k += NT) {
+        for (int k = tid; k < JTW * JTH; k += JNT) {
             const int li = k % JTW + JB, lj = k / JTW + JB;
             const int gi = i0 + li - JB, gj = j0 + lj - JB;
             if (gi < ni && gj < nj) xout[(size_t)gj * ni + gi] = sx[JB & 1][lj * JRW + li];
         }
         __syncthreads();
     }
-    block_partials<JB, NT>(acc, part, PredBuffers::BI_BLOCKS);
+    block_partials<JB, JNT>(acc, part, PredBuffers::BI_BLOCKS);
 }
 
 // the JB residual partials of one k_jac_blk launch: sweep by sweep, the first
@@ -351,12 +354,12 @@ int jacobi_solve(Ctx* c, const Mat5& A, const double* b, double* x, double tol, 
     double* y = B.vec[0];  // the other buffer of the ping-pong with x
     const int ntiles = ((c->ni + JTW - 1) / JTW) * ((c->nj + JTH - 1) / JTH);
     const int nb = std::min(ntiles, PredBuffers::BI_BLOCKS);
-    const dim3 g(nb), blk(NT);
+    const dim3 g(nb), blk(NT), jblk(JNT);
     hipStream_t st = c->stream;
     double* part = B.partials;
     BiScalars* sc = B.sc;
     // launch L (0-based) writes x when L is even, y when odd
-    hipLaunchKernelGGL(k_jac_blk, g, blk, 0, st, A, b, (const double*)y, x, part, (const BiScalars*)sc, 1);
+    hipLaunchKernelGGL(k_jac_blk, g, jblk, 0, st, A, b, (const double*)y, x, part, (const BiScalars*)sc, 1);
     hipLaunchKernelGGL(k_jac_fin, dim3(1), blk, 0, st, (const double*)part, nb, sc, tol, 1);
     // the first host check after the sweeps these rows typically need (7-8
     // at 2048^2); launches past convergence return at once
@@ -365,7 +368,7 @@ int jacobi_solve(Ctx* c, const Mat5& A, const double* b, double* x, double tol, 
     for (;;) {
         for (int q = 0; q < batch && L < MAX_LAUNCHES; ++q, ++L) {
             const bool odd = (L & 1) != 0;
-            hipLaunchKernelGGL(k_jac_blk, g, blk, 0, st, A, b, (const double*)(odd ? x : y), odd ? y : x, part,
+            hipLaunchKernelGGL(k_jac_blk, g, jblk, 0, st, A, b, (const double*)(odd ? x : y), odd ? y : x, part,
                                (const BiScalars*)sc, 0);
             hipLaunchKernelGGL(k_jac_fin, dim3(1), blk, 0, st, (const double*)part, nb, sc, 0.0, 0);
         }

@@ -724,31 +724,35 @@ __global__ void __launch_bounds__(FNT) k_conv(PredG G, double* __restrict__ fn1u
         const size_t c = ci(G, i, j);
         double* fd = cp == 0 ? fdu : fdv;
         double* fn1 = cp == 0 ? fn1u : fn1v;
-        if (!T.act[c]) {
+        // every operand of the cell, loaded before the activity test
+        const size_t g = gi(G, i, j);
+        const bool act = T.act[c] != 0;
+        const double vf = T.vofo[c], vs = T.vofS[c], wc = K.w[g], f1o = fn1[g];
+        const double aw = T.aW[g], ae = T.aE[g], as = T.aS[g], an = T.aN[g];
+        if (!act) {
             fd[c] = 0.0;
             continue;
         }
         const double hx = cp == 0 ? UDX(i) : VDX(i), hy = cp == 0 ? UDY(j) : VDY(j);
         const Quad cf = quad(P, cp, li, lj), ma = quad(P, 2 + cp, li, lj), co = quad(P, 4 + cp, li, lj);
         const double Fe = co.e, Fw = co.w, Fn = co.n, Fs = co.s;
-        const double vf = T.vofo[c], vs = T.vofS[c];
         const double RoCeO = (1.0 - vf - vs) * G.rg / G.rr + vf * G.rl / G.rr;
         const double RoCeN = RoCeO - G.dt * (ma.e - ma.w + ma.n - ma.s) / (hx * hy);
         const double lo = (1.0 + EPSI - vs) * G.rg / G.rr, hi = (1.0 - EPSI - vs) * G.rl / G.rr;
         double fluxn0;
         if (RoCeN > lo && RoCeN < hi && RoCeO > lo && RoCeO < hi && vs < EPSI)
-            fluxn0 = (Fe - Fw + Fn - Fs) / RoCeN + K.w[gi(G, i, j)] * (1.0 - RoCeO / RoCeN) / G.dt * hx * hy;
+            fluxn0 = (Fe - Fw + Fn - Fs) / RoCeN + wc * (1.0 - RoCeO / RoCeN) / G.dt * hx * hy;
         else
             fluxn0 = (cf.e - cf.w + cf.n - cf.s) / (1.0 - vs);
-        double v0 = G.first ? fluxn0 : 1.5 * fluxn0 - 0.5 * fn1[gi(G, i, j)];
+        double v0 = G.first ? fluxn0 : 1.5 * fluxn0 - 0.5 * f1o;
         int closed = 0;
-        if (T.aW[gi(G, i, j)] < EPSI) closed++;
-        if (T.aE[gi(G, i, j)] < EPSI) closed++;
-        if (T.aS[gi(G, i, j)] < EPSI) closed++;
-        if (T.aN[gi(G, i, j)] < EPSI) closed++;
+        if (aw < EPSI) closed++;
+        if (ae < EPSI) closed++;
+        if (as < EPSI) closed++;
+        if (an < EPSI) closed++;
         if (closed >= 3) v0 = 0.0;
-        fd[c] = v0 - K.w[gi(G, i, j)] * (1.0 - vs) / (1.0 - vs) * hx * hy / G.dt;
-        fn1[gi(G, i, j)] = fluxn0;
+        fd[c] = v0 - wc * (1.0 - vs) / (1.0 - vs) * hx * hy / G.dt;
+        fn1[g] = fluxn0;
     }
 }
 
