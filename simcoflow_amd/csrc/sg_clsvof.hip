@@ -905,8 +905,13 @@ __global__ void __launch_bounds__(256) k_sweep(ClsGeo G, SweepIn s, double dtv, 
                                                const double* __restrict__ tv_in, const double* __restrict__ tl_in,
                                                double* __restrict__ tv_out, double* __restrict__ tl_out,
                                                double* __restrict__ vof_out, double* __restrict__ phi_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    // x sweeps: XCD-aware tiles, so the tiles either side of a tile (whose
+    // cells its stencil reads) share its L2; y sweeps: launch order, which
+    // keeps vertically adjacent tiles on one XCD already
+    int tx = blockIdx.x, ty = blockIdx.y;
+    if (X) xcd_tile(tx, ty);
+    const int i = tx * blockDim.x + threadIdx.x + 1;
+    const int j = ty * blockDim.y + threadIdx.y + 1;
     if (i > G.ni || j > G.nj) return;
     const Line<X> L{G, X ? j : i};
     const int a = X ? i : j;
@@ -1443,8 +1448,11 @@ template <int W>
 __global__ void __launch_bounds__(256) k_to_uv(ClsGeo G, UVIn in, double* __restrict__ tvof,
                                                double* __restrict__ tphi, double* __restrict__ tnx,
                                                double* __restrict__ tny) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    // U: XCD-aware tiles (the stencil reaches along x); V: launch order
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (W == 0) xcd_tile(bx, by);
+    const int i = bx * blockDim.x + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
     if (i > G.ni || j > G.nj) return;
     const bool last = W == 0 ? i == G.ni : j == G.nj;
     const int si = last && W == 0 ? i - 1 : i, sj = last && W == 1 ? j - 1 : j;

@@ -690,7 +690,9 @@ __device__ __forceinline__ Comp comp_of(const PredG& G, int c) {
 __global__ void __launch_bounds__(FNT) k_conv(PredG G, double* __restrict__ fn1u, double* __restrict__ fn1v,
                                               double* __restrict__ fdu, double* __restrict__ fdv) {
     __shared__ TilePlanes<6> P;
-    const int i0 = blockIdx.x * FTW, j0 = blockIdx.y * FTH;
+    int bx, by;  // the workgroup's tile, XCD-aware
+    xcd_tile(bx, by);
+    const int i0 = bx * FTW, j0 = by * FTH;
     tile_faces(
         G, i0, j0,
         [&](int i, int j, int lj, int li) {
@@ -811,7 +813,9 @@ __global__ void __launch_bounds__(FNT) k_mix_visc(PredG G, const double* __restr
                                                   const double* __restrict__ fpre_v, double* __restrict__ out_u,
                                                   double* __restrict__ out_v) {
     __shared__ TilePlanes<3> P;
-    const int i0 = blockIdx.x * FTW, j0 = blockIdx.y * FTH;
+    int bx, by;  // the workgroup's tile, XCD-aware
+    xcd_tile(bx, by);
+    const int i0 = bx * FTW, j0 = by * FTH;
     tile_faces(
         G, i0, j0,
         [&](int i, int j, int lj, int li) {
@@ -905,7 +909,9 @@ __global__ void __launch_bounds__(FNT) k_rows(PredG G, const double* __restrict_
                                               double* __restrict__ rv, double* __restrict__ DpU,
                                               double* __restrict__ DpV) {
     __shared__ TilePlanes<4> P;
-    const int i0 = blockIdx.x * FTW, j0 = blockIdx.y * FTH;
+    int bx, by;  // the workgroup's tile, XCD-aware
+    xcd_tile(bx, by);
+    const int i0 = bx * FTW, j0 = by * FTH;
     tile_faces(
         G, i0, j0,
         [&](int i, int j, int lj, int li) {
