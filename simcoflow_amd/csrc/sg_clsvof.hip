@@ -557,6 +557,19 @@ __device__ __forceinline__ void plane_of(const ClsGeo& G, const Planes& P, size_
         d = default_dis(vf, vs, hy);
     }
 }
+// plane_of from values already loaded (cnx, cny, cdis: the stored plane)
+__device__ __forceinline__ void plane_pick(double vf, double vs, double hy, double cnx, double cny, double cdis,
+                                           double& nx, double& ny, double& d) {
+    if (recon_class(vf, vs)) {
+        nx = cnx;
+        ny = cny;
+        d = cdis;
+    } else {
+        nx = 0.0;
+        ny = 1.0;
+        d = default_dis(vf, vs, hy);
+    }
+}
 
 // wavefront-aggregated append of `value` to list where pred holds (one
 // atomic per wavefront); every lane of the wavefront must call it
@@ -1029,11 +1042,18 @@ struct Visit {
     double sg;  // RS sign
 };
 
-__device__ Visit propose(const ClsGeo& G, int i, int j, int ii, int jj, double n1, double n2, double d1,
-                         const Target& T) {
+// the source cell's data a visit reads (its size, centre and solid line)
+struct SrcCell {
+    double dxs, dys, xc, yc, sxn, syn, sph, vs;
+};
+__device__ __forceinline__ SrcCell src_cell(const ClsGeo& G, int i, int j) {
     const size_t c = cix(G, i, j);
-    const double dxs = G.dx[i - 1], dys = G.dy[j - 1];
-    const double sxn = G.nxS[c], syn = G.nyS[c], sph = G.phiS[c];
+    return SrcCell{G.dx[i - 1], G.dy[j - 1], G.xc[i - 1], G.yc[j - 1], G.nxS[c], G.nyS[c], G.phiS[c], G.vofS[c]};
+}
+
+__device__ Visit propose(const SrcCell& S, int ii, int jj, double n1, double n2, double d1, const Target& T) {
+    const double dxs = S.dxs, dys = S.dys;
+    const double sxn = S.sxn, syn = S.syn, sph = S.sph;
     const int l = ii < -1 ? -1 : (ii > 1 ? 1 : ii);
     const int m = jj < -1 ? -1 : (jj > 1 ? 1 : jj);
     // point of the source cell's edge facing the target, moved off the solid
@@ -1076,7 +1096,7 @@ __device__ Visit propose(const ClsGeo& G, int i, int j, int ii, int jj, double n
             }
         }
     }
-    const double ddx = T.x - G.xc[i - 1], ddy = T.y - G.yc[j - 1];
+    const double ddx = T.x - S.xc, ddy = T.y - S.yc;
     const int rule = T.s < EPSI ? V_R2 : V_R3;
     const double dv = n1 * xv + n2 * yv + d1;
     if (dv * fsign(1.0, T.phi) <= 0.0) {
@@ -1092,7 +1112,7 @@ __device__ Visit propose(const ClsGeo& G, int i, int j, int ii, int jj, double n
         return {rule, sqrt(ax * ax + ay * ay), 0.0};
     }
     if (T.s >= 1.0 - EPSI) {
-        if (G.vofS[c] < EPSI) {
+        if (S.vs < EPSI) {
             const double ax = ddx - xp, ay = ddy - yp;
             return {V_RS, sqrt(ax * ax + ay * ay), fsign(1.0, dij)};
         }
@@ -1162,16 +1182,20 @@ __global__ void __launch_bounds__(256) k_rd_scatter(ClsGeo G, const double* __re
         const int ti = si + ii, tj = sj + jj;
         if (ti < 1 || ti > G.ni || tj < 1 || tj > G.nj) continue;
         const size_t t = cix(G, ti, tj);
+        // the target's and the source's data, loaded before the tests that
+        // decide whether they are used
         Target T;
         T.v = vof[t];
         T.s = G.vofS[t];
-        if (!rd_target(T.v, T.s) || T.s >= 1.0 - EPSI) continue;
         T.phi = phi[t];
         T.x = G.xc[ti - 1];
         T.y = G.yc[tj - 1];
+        const SrcCell S = src_cell(G, si, sj);
+        const double sv = vof[c0], snx = P.nx[c0], sny = P.ny[c0], sdis = P.dis[c0];
+        if (!rd_target(T.v, T.s) || T.s >= 1.0 - EPSI) continue;
         double n1, n2, d1;
-        plane_of(G, P, (size_t)c0, vof[c0], G.dy[sj - 1], n1, n2, d1);
-        const Visit w = propose(G, si, sj, ii, jj, n1, n2, d1, T);
+        plane_pick(sv, S.vs, S.dys, snx, sny, sdis, n1, n2, d1);
+        const Visit w = propose(S, ii, jj, n1, n2, d1, T);
         if (w.kind == V_NONE) continue;
         if (w.kind == V_R2) hit[t] = 1;
         if (!(w.d != w.d)) atomicMin(best + t, (unsigned long long)__double_as_longlong(w.d));
@@ -1211,6 +1235,10 @@ __device__ void redistance_cell(const ClsGeo& G, const double* __restrict__ vof,
     Target T;
     T.v = vof[c];
     T.s = G.vofS[c];
+    // the scatter's minimum and hit flag, loaded with the cell's own data
+    // rather than after the window scan that decides whether they are used
+    const unsigned long long bc = best[c];
+    const bool hc = hit[c] != 0;
     const double hy = G.dy[tj - 1];
     if (rd_source(T.v, T.s)) {
         double n1, n2, d1;
@@ -1247,13 +1275,13 @@ __device__ void redistance_cell(const ClsGeo& G, const double* __restrict__ vof,
                         const size_t sc = cix(G, si, sj);
                         double n1, n2, d1;
                         plane_of(G, P, sc, vof[sc], G.dy[sj - 1], n1, n2, d1);
-                        apply(propose(G, si, sj, ti - si, tj - sj, n1, n2, d1, T), T, pa);
+                        apply(propose(src_cell(G, si, sj), ti - si, tj - sj, n1, n2, d1, T), T, pa);
                     }
                 }
             } else {
-                const double b = __longlong_as_double((long long)best[c]);
+                const double b = __longlong_as_double((long long)bc);
                 if (T.s < EPSI) {
-                    if (hit[c]) pa = fsign(1.0, 0.5 - T.v) * fmin(b, 1.0e4);
+                    if (hc) pa = fsign(1.0, 0.5 - T.v) * fmin(b, 1.0e4);
                 } else if (b < 1.0e4) {
                     pa = T.v >= 1.0 - VOFEPS - T.s ? -b : b;
                 }
