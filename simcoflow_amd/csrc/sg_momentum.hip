@@ -358,8 +358,9 @@ int jacobi_solve(Ctx* c, const Mat5& A, const double* b, double* x, double tol, 
     hipStream_t st = c->stream;
     double* part = B.partials;
     BiScalars* sc = B.sc;
-    // launch L (0-based) writes x when L is even, y when odd
-    hipLaunchKernelGGL(k_jac_blk, g, jblk, 0, st, A, b, (const double*)y, x, part, (const BiScalars*)sc, 1);
+    // launch L (0-based) writes y when L is even, x when odd: the usual two
+    // launches leave the solution in x
+    hipLaunchKernelGGL(k_jac_blk, g, jblk, 0, st, A, b, (const double*)x, y, part, (const BiScalars*)sc, 1);
     hipLaunchKernelGGL(k_jac_fin, dim3(1), blk, 0, st, (const double*)part, nb, sc, tol, 1);
     // the first host check after the sweeps these rows typically need (7-8
     // at 2048^2); launches past convergence return at once
@@ -368,7 +369,7 @@ int jacobi_solve(Ctx* c, const Mat5& A, const double* b, double* x, double tol, 
     for (;;) {
         for (int q = 0; q < batch && L < MAX_LAUNCHES; ++q, ++L) {
             const bool odd = (L & 1) != 0;
-            hipLaunchKernelGGL(k_jac_blk, g, jblk, 0, st, A, b, (const double*)(odd ? x : y), odd ? y : x, part,
+            hipLaunchKernelGGL(k_jac_blk, g, jblk, 0, st, A, b, (const double*)(odd ? y : x), odd ? x : y, part,
                                (const BiScalars*)sc, 0);
             hipLaunchKernelGGL(k_jac_fin, dim3(1), blk, 0, st, (const double*)part, nb, sc, 0.0, 0);
         }
@@ -383,8 +384,8 @@ int jacobi_solve(Ctx* c, const Mat5& A, const double* b, double* x, double tol, 
             return 0;
         }
         if (!h.active) {
-            // met tol in launch (iters - 1) / JB, whose x is in y when that is odd
-            if ((((h.iters - 1) / JB) & 1) != 0)
+            // met tol in launch (iters - 1) / JB, whose x is in y when that is even
+            if ((((h.iters - 1) / JB) & 1) == 0)
                 hipLaunchKernelGGL(k_copy_vec, dim3((unsigned)std::min<size_t>((n + NT - 1) / NT,
                                                                                (size_t)PredBuffers::BI_BLOCKS)),
                                    blk, 0, st, n, (const double*)y, x);
