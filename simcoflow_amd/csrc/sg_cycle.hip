@@ -685,6 +685,69 @@ __device__ void tail_pass(T* lds, const TailLevel<T>& L, const T* src, T* dst, i
     __syncthreads();
 }
 
+// The first tail level's couplings of the cell a thread owns in every pass,
+// held in registers for the whole kernel (used when each thread has at most
+// one cell of that level: (nj << sh) <= TAIL_THREADS)
+template <typename T>
+struct OwnRow {
+    bool own;  // the thread has a cell
+    int col;   // its colour, (i + j) & 1
+    int p;     // its LDS offset
+    T d, ew, ee, ns, nn, ne, nesw, nw, nwse;  // d[p], e[p-1], e[p], n[p-ld], n[p], ne[p], ne[p-ld-1], nw[p], nw[p-ld+1]
+};
+
+template <typename T>
+__device__ OwnRow<T> own_row(T* lds, const TailLevel<T>& L) {
+    OwnRow<T> R{};
+    const Dims& D = L.cd;
+    const int k = threadIdx.x, i = k & ((1 << L.sh) - 1), j = k >> L.sh;
+    R.own = k < (D.nj << L.sh) && i < D.ni;
+    if (!R.own) return R;
+    const int ld = D.ld, p = toff(D, i, j);
+    const T* d = tarr(lds, L, TA_D);
+    const T* e = tarr(lds, L, TA_E);
+    const T* nn = tarr(lds, L, TA_N);
+    const T* ne = tarr(lds, L, TA_NE);
+    const T* nw = tarr(lds, L, TA_NW);
+    R.col = (i + j) & 1;
+    R.p = p;
+    R.d = d[p];
+    R.ew = e[p - 1];
+    R.ee = e[p];
+    R.ns = nn[p - ld];
+    R.nn = nn[p];
+    R.ne = ne[p];
+    R.nesw = ne[p - ld - 1];
+    R.nw = nw[p];
+    R.nwse = nw[p - ld + 1];
+    return R;
+}
+
+// tail_pass on the first level with the couplings from registers (same
+// terms in the same order)
+template <typename T>
+__device__ void tail_pass_own(T* lds, const TailLevel<T>& L, const OwnRow<T>& R, const T* src, T* dst, int col,
+                              bool zero_start) {
+    if (R.own) {
+        const int ld = L.cd.ld, p = R.p;
+        const T* b = tarr(lds, L, TA_B);
+        T v = zero_start ? T(0) : src[p];
+        if (R.col == col) {
+            const T dd = R.d;
+            if (zero_start) {
+                v = dd > T(0) ? b[p] / dd : T(0);
+            } else {
+                const T s = R.ew * src[p - 1] + R.ee * src[p + 1] + R.ns * src[p - ld] + R.nn * src[p + ld] +
+                            R.ne * src[p + ld + 1] + R.nesw * src[p - ld - 1] + R.nw * src[p + ld - 1] +
+                            R.nwse * src[p - ld + 1];
+                v = dd > T(0) ? (b[p] + s) / dd : T(0);
+            }
+        }
+        dst[p] = v;
+    }
+    __syncthreads();
+}
+
 // stage operators, weights and the first level's b into LDS
 template <typename T>
 __device__ void tail_stage(T* lds, const TailArgs<T>& a) {
@@ -720,8 +783,9 @@ __device__ void tail_stage(T* lds, const TailArgs<T>& a) {
 // One V-cycle over the tail levels: level 0's TA_B -> level 0's TA_R.  On
 // every level the smoothed x ends in TA_R; TA_X holds the first pass and then
 // the residual that is restricted.
+// With `regs`, level 0 takes its couplings from R (each thread's cell).
 template <typename T>
-__device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
+__device__ void tail_vcycle(T* lds, const TailArgs<T>& a, const OwnRow<T>& R, bool regs) {
     const int tid = threadIdx.x;
     const int last = a.n - 1;
     for (int l = 0; l < last; ++l) {
@@ -731,25 +795,37 @@ __device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
         const int ld = D.ld;
         T* xa = tarr(lds, L, TA_X);
         T* xb = tarr(lds, L, TA_R);
-        tail_pass<T>(lds, L, nullptr, xa, 0, true);
-        tail_pass<T>(lds, L, xa, xb, 1, false);
-        const T* d = tarr(lds, L, TA_D);
-        const T* e = tarr(lds, L, TA_E);
-        const T* nn = tarr(lds, L, TA_N);
-        const T* ne = tarr(lds, L, TA_NE);
-        const T* nw = tarr(lds, L, TA_NW);
         const T* b = tarr(lds, L, TA_B);
         const T* x = xb;
         T* r = xa;
-        const int mask = (1 << L.sh) - 1;
-        for (int k = tid; k < (D.nj << L.sh); k += TAIL_THREADS) {
-            const int i = k & mask;
-            if (i >= D.ni) continue;
-            const int p = toff(D, i, k >> L.sh);
-            const T s = e[p - 1] * x[p - 1] + e[p] * x[p + 1] + nn[p - ld] * x[p - ld] + nn[p] * x[p + ld] +
-                        ne[p] * x[p + ld + 1] + ne[p - ld - 1] * x[p - ld - 1] + nw[p] * x[p + ld - 1] +
-                        nw[p - ld + 1] * x[p - ld + 1];
-            r[p] = b[p] - d[p] * x[p] + s;
+        if (l == 0 && regs) {
+            tail_pass_own<T>(lds, L, R, nullptr, xa, 0, true);
+            tail_pass_own<T>(lds, L, R, xa, xb, 1, false);
+            if (R.own) {
+                const int p = R.p;
+                const T s = R.ew * x[p - 1] + R.ee * x[p + 1] + R.ns * x[p - ld] + R.nn * x[p + ld] +
+                            R.ne * x[p + ld + 1] + R.nesw * x[p - ld - 1] + R.nw * x[p + ld - 1] +
+                            R.nwse * x[p - ld + 1];
+                r[p] = b[p] - R.d * x[p] + s;
+            }
+        } else {
+            tail_pass<T>(lds, L, nullptr, xa, 0, true);
+            tail_pass<T>(lds, L, xa, xb, 1, false);
+            const T* d = tarr(lds, L, TA_D);
+            const T* e = tarr(lds, L, TA_E);
+            const T* nn = tarr(lds, L, TA_N);
+            const T* ne = tarr(lds, L, TA_NE);
+            const T* nw = tarr(lds, L, TA_NW);
+            const int mask = (1 << L.sh) - 1;
+            for (int k = tid; k < (D.nj << L.sh); k += TAIL_THREADS) {
+                const int i = k & mask;
+                if (i >= D.ni) continue;
+                const int p = toff(D, i, k >> L.sh);
+                const T s = e[p - 1] * x[p - 1] + e[p] * x[p + 1] + nn[p - ld] * x[p - ld] + nn[p] * x[p + ld] +
+                            ne[p] * x[p + ld + 1] + ne[p - ld - 1] * x[p - ld - 1] + nw[p] * x[p + ld - 1] +
+                            nw[p - ld + 1] * x[p - ld + 1];
+                r[p] = b[p] - d[p] * x[p] + s;
+            }
         }
         __syncthreads();
         T* bc = tarr(lds, Cl, TA_B);
@@ -815,8 +891,13 @@ __device__ void tail_vcycle(T* lds, const TailArgs<T>& a) {
             xb[toff(D, i, j)] += v;
         }
         __syncthreads();
-        tail_pass<T>(lds, L, xb, xa, 1, false);
-        tail_pass<T>(lds, L, xa, xb, 0, false);
+        if (l == 0 && regs) {
+            tail_pass_own<T>(lds, L, R, xb, xa, 1, false);
+            tail_pass_own<T>(lds, L, R, xa, xb, 0, false);
+        } else {
+            tail_pass<T>(lds, L, xb, xa, 1, false);
+            tail_pass<T>(lds, L, xa, xb, 0, false);
+        }
     }
 }
 
@@ -866,8 +947,11 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
     const TailLevel<T>& L = a.L[0];
     const Dims& D = L.cd;
     const int n = D.ni * D.nj;
+    // level 0's couplings in registers when each thread has at most one cell
+    const bool regs = (D.nj << L.sh) <= TAIL_THREADS;
+    const OwnRow<T> R = regs ? own_row<T>(lds, L) : OwnRow<T>{};
     if (a.krylov <= 0) {
-        tail_vcycle<T>(lds, a);
+        tail_vcycle<T>(lds, a, R, regs);
         const T* x = tarr(lds, L, TA_R);
         for (int k = tid; k < n; k += TAIL_THREADS) {
             const int i = k % D.ni, j = k / D.ni;
@@ -892,7 +976,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a, int lds_el
     T xacc = T(0), qv = T(0);
     double pq_prev = 0.0, rr0 = 0.0;
     for (int it = 0; it < a.krylov; ++it) {
-        tail_vcycle<T>(lds, a);  // ends with a barrier
+        tail_vcycle<T>(lds, a, R, regs);  // ends with a barrier
         const T zp = own ? z[p] : T(0);
         const double rp = own ? (double)res[p] : 0.0;
         double v[2] = {(double)zp * (double)qv, rp * rp};
