@@ -196,6 +196,15 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m simcoflow_amd.build`")
+    # torch carries its own HIP runtime beside the system one this library
+    # links; two runtimes share a process only if torch's starts first (one
+    # started after the library's sees no GPU), so start it before any
+    # library call when torch is installed
+    try:
+        import torch
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     for name, (argt, rest) in PROTOTYPES.items():
         fn = getattr(L, name)

@@ -296,6 +296,62 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
         sb[lj][li] = bv;
         sx[lj][li] = xv;
     }
+    // every operand the passes below read from memory is loaded here, before
+    // the first barrier: the ring cells' couplings and the restriction
+    // weights of the thread's coarse point, beside the pairs' couplings
+    constexpr int HW = RW - 2, HH = RH - 2;  // the black ring: li, lj in [1, RW-1) x [1, RH-1) off the tile
+    constexpr int NBR = 3 * HW + 3 * TH;     // rows 1, 2, TH+3; columns 1, 2, TW+3 of rows 3..TH+2
+    static_assert(NBR <= NTB && TW + 1 + TH <= NTB, "one ring cell per thread");
+    int bpos = -1;  // this thread's black ring cell (LDS position), -1 if none
+    T bcf[5] = {T(0), T(0), T(0), T(0), T(0)};
+    if (tid < NBR) {
+        int li, lj;
+        if (tid < 3 * HW) {
+            li = 1 + tid % HW;
+            lj = tid < 2 * HW ? 1 + tid / HW : TH + 3;
+        } else {
+            const int k2 = tid - 3 * HW;
+            li = k2 < 2 * TH ? 1 + k2 % 2 : TW + 3;
+            lj = k2 < 2 * TH ? 3 + k2 / 2 : 3 + (k2 - 2 * TH);
+        }
+        const int gi = i0 + li - 3, gj = j0 + lj - 3;
+        if (((li + lj) & 1) != 0 && in_dom(F, gi, gj)) {
+            coef(F.off(gi, gj), bcf);
+            bpos = lj * RW + li;
+        }
+    }
+    int rpos = -1;  // this thread's residual ring cell: row -1 (columns -1..TW-1), column -1 (rows 0..TH-1)
+    bool rin = false;
+    T rcf[5] = {T(0), T(0), T(0), T(0), T(0)};
+    if (tid < TW + 1 + TH) {
+        const int li = tid <= TW ? tid + 2 : 2;
+        const int lj = tid <= TW ? 2 : tid - TW + 2;
+        const int gi = i0 + li - 3, gj = j0 + lj - 3;
+        rpos = lj * RW + li;
+        rin = in_dom(F, gi, gj);
+        if (rin) coef(F.off(gi, gj), rcf);
+    }
+    static_assert(SEMI || (TW / 2) * (TH / 2) == NTB, "one coarse point per thread");
+    T rw[9];
+    int rI = -1, rJ = 0;
+    if (!SEMI) {
+        const int I = i0 / 2 + tid % (TW / 2), J = j0 / 2 + tid / (TW / 2);
+        if (I < C.ni && J < C.nj) {
+            T* const* w = P.w;
+            const size_t c = C.off(I, J), cw = C.off(I - 1, J), cs = C.off(I, J - 1), csw = C.off(I - 1, J - 1);
+            rw[0] = w[W_INJ][c];
+            rw[1] = w[W_XW][c];
+            rw[2] = w[W_XE][cw];
+            rw[3] = w[W_YS][c];
+            rw[4] = w[W_YN][cs];
+            rw[5] = w[W_SW][c];
+            rw[6] = w[W_SE][cw];
+            rw[7] = w[W_NW][cs];
+            rw[8] = w[W_NE][csw];
+            rI = I;
+            rJ = J;
+        }
+    }
     // couplings of the thread's pairs: [0] black, [1] red
     T cf[PPT][2][5];
     int cl[PPT][2];
@@ -330,18 +386,7 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
 #pragma unroll
     for (int u = 0; u < PPT; ++u)
         if (cl[u][0] >= 0) sxf[cl[u][0]] = gs(cf[u][0], cl[u][0]);
-    {
-        constexpr int HW = RW - 2, HH = RH - 2;  // li, lj in [1, RW-1) x [1, RH-1)
-        for (int k = tid; k < HW * HH; k += NTB) {
-            const int li = k % HW + 1, lj = k / HW + 1;
-            if (in_tile(li, lj) || ((li + lj) & 1) == 0) continue;
-            const int gi = i0 + li - 3, gj = j0 + lj - 3;
-            if (!in_dom(F, gi, gj)) continue;
-            T c[5];
-            coef(F.off(gi, gj), c);
-            sxf[lj * RW + li] = gs(c, lj * RW + li);
-        }
-    }
+    if (bpos >= 0) sxf[bpos] = gs(bcf, bpos);
     __syncthreads();
     // residual on [-1,TW) x [-1,TH), in place over b (each cell reads only
     // its own b)
@@ -354,21 +399,7 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
     for (int u = 0; u < PPT; ++u)
 #pragma unroll
         for (int h = 0; h < 2; ++h) rt[u][h] = cl[u][h] >= 0 ? resid(cf[u][h], cl[u][h]) : T(0);
-    {
-        constexpr int NRING = TW + 1 + TH;  // row -1 (columns -1..TW-1), column -1 (rows 0..TH-1)
-        for (int k = tid; k < NRING; k += NTB) {
-            const int li = k <= TW ? k + 2 : 2;
-            const int lj = k <= TW ? 2 : k - TW + 2;
-            const int gi = i0 + li - 3, gj = j0 + lj - 3;
-            T r = T(0);
-            if (in_dom(F, gi, gj)) {
-                T c[5];
-                coef(F.off(gi, gj), c);
-                r = resid(c, lj * RW + li);
-            }
-            sbf[lj * RW + li] = r;
-        }
-    }
+    if (rpos >= 0) sbf[rpos] = rin ? resid(rcf, rpos) : T(0);
 #pragma unroll
     for (int u = 0; u < PPT; ++u)
 #pragma unroll
@@ -387,15 +418,15 @@ __global__ void __launch_bounds__(NTB) k_down5(Dims F, OpV<T> A, const TB* __res
                          P.w[W_YN][cc - C.ld] * sb[lj - 1][li];
             }
         }
-    } else {
-        for (int k = tid; k < (TW / 2) * (TH / 2); k += NTB) {
-            const int I = i0 / 2 + k % (TW / 2), J = j0 / 2 + k / (TW / 2);
-            if (I < C.ni && J < C.nj) {
-                const int li = 2 * I - i0 + 3, lj = 2 * J - j0 + 3;
-                auto R = [&](int a, int bb) { return sb[lj + bb][li + a]; };
-                bc[C.off(I, J)] = restrict_point<T>(C, P, I, J, R);
-            }
-        }
+    } else if (rI >= 0) {
+        // restrict_point with the weights loaded above (same terms, same order)
+        const int li = 2 * rI - i0 + 3, lj = 2 * rJ - j0 + 3;
+        auto R = [&](int a, int bb) { return sb[lj + bb][li + a]; };
+        T s = rw[0] * R(0, 0);
+        s += rw[1] * R(1, 0) + rw[2] * R(-1, 0);
+        s += rw[3] * R(0, 1) + rw[4] * R(0, -1);
+        s += rw[5] * R(1, 1) + rw[6] * R(-1, 1) + rw[7] * R(1, -1) + rw[8] * R(-1, -1);
+        bc[C.off(rI, rJ)] = s;
     }
     for (int k = tid; k < TW * TH; k += NTB) {
         const int li = k % TW + 3, lj = k / TW + 3;
