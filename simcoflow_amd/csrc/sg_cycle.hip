@@ -41,7 +41,43 @@ __device__ inline bool in_dom(const Dims& D, int i, int j) { return i >= 0 && j 
 // taken Jacobi-wise: a colour pass computes into registers and writes after
 // a barrier.
 
-// down: x = S(0; b) (red from zero, then black), r = b - A x, bc = P^T r
+// A 9-point row's couplings (d, west, east, south, north, NE, SW, NW, SE),
+// loaded ahead of the passes that use them
+template <typename T>
+struct Row9 {
+    T d, w, e, s, n, ne, sw, nw, se;
+};
+template <typename T>
+__device__ __forceinline__ Row9<T> load_row9(const OpV<T>& A, size_t p, size_t ld) {
+    Row9<T> r;
+    r.d = A.diag[p];
+    r.w = A.cE[p - 1];
+    r.e = A.cE[p];
+    r.s = A.cN[p - ld];
+    r.n = A.cN[p];
+    if (A.cNE) {
+        r.ne = A.cNE[p];
+        r.sw = A.cNE[p - ld - 1];
+        r.nw = A.cNW[p];
+        r.se = A.cNW[p - ld + 1];
+    } else {
+        r.ne = r.sw = r.nw = r.se = T(0);
+    }
+    return r;
+}
+// coupled_sum over a loaded row (same terms, same order)
+template <typename T, typename XF>
+__device__ __forceinline__ T row9_sum(const Row9<T>& r, bool nine, XF X) {
+    T s = r.w * X(-1, 0) + r.e * X(1, 0) + r.s * X(0, -1) + r.n * X(0, 1);
+    if (nine) s += r.ne * X(1, 1) + r.sw * X(-1, -1) + r.nw * X(-1, 1) + r.se * X(1, -1);
+    return s;
+}
+
+// down: x = S(0; b) (red from zero, then black), r = b - A x, bc = P^T r.
+// With tiles of at most NTB cells every coupling and restriction weight the
+// passes read is loaded before the first barrier (a few cells per thread):
+// the passes then wait on LDS only, which matters on the small levels, where
+// a kernel is a chain of memory latencies.
 template <typename T, int TW, int TH>
 __global__ void __launch_bounds__(NTB) k_down9(Dims F, OpV<T> A, const T* __restrict__ b, T* __restrict__ xo,
                                                Dims C, Prol<T> P, T* __restrict__ bc, const int* on) {
@@ -73,28 +109,68 @@ __global__ void __launch_bounds__(NTB) k_down9(Dims F, OpV<T> A, const T* __rest
         sb[lj][li] = bv;
         sx[lj][li] = xv;
     }
+    constexpr bool PRE = TW * TH <= NTB;
+    const bool nine = A.cNE != nullptr;
+    constexpr int PR = (RW - 1) / 2;
+    constexpr int NBK = (PR * (RH - 2) + NTB - 1) / NTB;
+    constexpr int NRS = ((TW + 1) * (TH + 1) + NTB - 1) / NTB;
+    // black cell n of this thread: local (li, lj), or li < 0
+    auto black_cell = [&](int n, int& li, int& lj) {
+        const int k = tid + n * NTB;
+        li = -1;
+        lj = 0;
+        if (k >= PR * (RH - 2)) return;
+        lj = k / PR + 1;
+        const int l = 2 * (k % PR) + 1 + (lj & 1);
+        if (l >= RW - 1 || !in_dom(F, i0 + l - 3, j0 + lj - 3)) return;
+        li = l;
+    };
+    Row9<T> rbk[PRE ? NBK : 1], rrs[PRE ? NRS : 1];
+    T rw[9];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int n = 0; n < NBK; ++n) {
+            int li, lj;
+            black_cell(n, li, lj);
+            if (li >= 0) rbk[n] = load_row9(A, F.off(i0 + li - 3, j0 + lj - 3), ld);
+        }
+#pragma unroll
+        for (int m = 0; m < NRS; ++m) {
+            const int k = tid + m * NTB;
+            const int li = k % (TW + 1) + 2, lj = k / (TW + 1) + 2;
+            if (k < (TW + 1) * (TH + 1) && in_dom(F, i0 + li - 3, j0 + lj - 3))
+                rrs[m] = load_row9(A, F.off(i0 + li - 3, j0 + lj - 3), ld);
+        }
+        const int I = i0 / 2 + tid % (TW / 2), J = j0 / 2 + tid / (TW / 2);
+        if (tid < (TW / 2) * (TH / 2) && I < C.ni && J < C.nj) {
+            T* const* w = P.w;
+            const size_t c = C.off(I, J), cw = C.off(I - 1, J), cs = C.off(I, J - 1), csw = C.off(I - 1, J - 1);
+            rw[0] = w[W_INJ][c];
+            rw[1] = w[W_XW][c];
+            rw[2] = w[W_XE][cw];
+            rw[3] = w[W_YS][c];
+            rw[4] = w[W_YN][cs];
+            rw[5] = w[W_SW][c];
+            rw[6] = w[W_SE][cw];
+            rw[7] = w[W_NW][cs];
+            rw[8] = w[W_NE][csw];
+        }
+    }
     __syncthreads();
     // black on [-2,TW+1) x [-2,TH+1); local parity: black <=> (li + lj) odd
     {
-        constexpr int PR = (RW - 1) / 2;
-        constexpr int NBK = (PR * (RH - 2) + NTB - 1) / NTB;
         T nv[NBK];
         int pos[NBK];
 #pragma unroll
         for (int n = 0; n < NBK; ++n) {
-            const int k = tid + n * NTB;
+            int li, lj;
+            black_cell(n, li, lj);
             pos[n] = -1;
             nv[n] = T(0);
-            if (k >= PR * (RH - 2)) continue;
-            const int lj = k / PR + 1;
-            const int li = 2 * (k % PR) + 1 + (lj & 1);
-            if (li >= RW - 1) continue;
-            const int gi = i0 + li - 3, gj = j0 + lj - 3;
-            if (!in_dom(F, gi, gj)) continue;
-            const size_t p = F.off(gi, gj);
-            const T d = A.diag[p];
+            if (li < 0) continue;
+            const Row9<T> R = PRE ? rbk[n] : load_row9(A, F.off(i0 + li - 3, j0 + lj - 3), ld);
             auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            nv[n] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+            nv[n] = R.d > T(0) ? (sb[lj][li] + row9_sum(R, nine, X)) / R.d : T(0);
             pos[n] = lj * RW + li;
         }
         __syncthreads();
@@ -105,7 +181,6 @@ __global__ void __launch_bounds__(NTB) k_down9(Dims F, OpV<T> A, const T* __rest
     __syncthreads();
     // residual on [-1,TW) x [-1,TH), in place over b (each cell reads only
     // its own b)
-    constexpr int NRS = ((TW + 1) * (TH + 1) + NTB - 1) / NTB;
 #pragma unroll
     for (int m = 0; m < NRS; ++m) {
         const int k = tid + m * NTB;
@@ -114,9 +189,9 @@ __global__ void __launch_bounds__(NTB) k_down9(Dims F, OpV<T> A, const T* __rest
         const int gi = i0 + li - 3, gj = j0 + lj - 3;
         T r = T(0);
         if (in_dom(F, gi, gj)) {
-            const size_t p = F.off(gi, gj);
+            const Row9<T> R = PRE ? rrs[m] : load_row9(A, F.off(gi, gj), ld);
             auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            r = sb[lj][li] - A.diag[p] * sx[lj][li] + coupled_sum<T>(A, p, ld, X);
+            r = sb[lj][li] - R.d * sx[lj][li] + row9_sum(R, nine, X);
         }
         sb[lj][li] = r;
     }
@@ -126,7 +201,16 @@ __global__ void __launch_bounds__(NTB) k_down9(Dims F, OpV<T> A, const T* __rest
         if (I < C.ni && J < C.nj) {
             const int li = 2 * I - i0 + 3, lj = 2 * J - j0 + 3;
             auto R = [&](int a, int bb) { return sb[lj + bb][li + a]; };
-            bc[C.off(I, J)] = restrict_point<T>(C, P, I, J, R);
+            if constexpr (PRE) {
+                // restrict_point with the weights loaded above (same terms, same order)
+                T s = rw[0] * R(0, 0);
+                s += rw[1] * R(1, 0) + rw[2] * R(-1, 0);
+                s += rw[3] * R(0, 1) + rw[4] * R(0, -1);
+                s += rw[5] * R(1, 1) + rw[6] * R(-1, 1) + rw[7] * R(1, -1) + rw[8] * R(-1, -1);
+                bc[C.off(I, J)] = s;
+            } else {
+                bc[C.off(I, J)] = restrict_point<T>(C, P, I, J, R);
+            }
         }
     }
     for (int k = tid; k < TW * TH; k += NTB) {
@@ -190,28 +274,48 @@ __global__ void __launch_bounds__(NTB) k_up9(Dims F, OpV<T> A, const T* __restri
                 }
         }
     }
+    // with tiles of at most NTB cells, the couplings of the black cells and
+    // of the red tile cell of this thread are loaded before the first barrier
+    constexpr bool PRE = TW * TH <= NTB;
+    const bool nine = A.cNE != nullptr;
+    constexpr int PR = (RW - 2) / 2;
+    constexpr int NBK = (PR * (RH - 2) + NTB - 1) / NTB;
+    auto black_cell = [&](int m, int& li, int& lj) {
+        const int k = tid + m * NTB;
+        li = -1;
+        lj = 0;
+        if (k >= PR * (RH - 2)) return;
+        lj = k / PR + 1;
+        const int l = 2 * (k % PR) + 1 + (lj & 1);
+        if (l >= RW - 1 || !in_dom(F, i0 + l - 2, j0 + lj - 2)) return;
+        li = l;
+    };
+    Row9<T> rbk[PRE ? NBK : 1], rred;
+    if constexpr (PRE) {
+#pragma unroll
+        for (int m = 0; m < NBK; ++m) {
+            int li, lj;
+            black_cell(m, li, lj);
+            if (li >= 0) rbk[m] = load_row9(A, F.off(i0 + li - 2, j0 + lj - 2), ld);
+        }
+        const int gi = i0 + tid % TW, gj = j0 + tid / TW;
+        if (tid < TW * TH && in_dom(F, gi, gj) && ((gi + gj) & 1) == 0) rred = load_row9(A, F.off(gi, gj), ld);
+    }
     __syncthreads();
     // black on [-1,TW+1) x [-1,TH+1), staged (Jacobi among black cells)
     {
-        constexpr int PR = (RW - 2) / 2;
-        constexpr int NBK = (PR * (RH - 2) + NTB - 1) / NTB;
         T nv[NBK];
         int pos[NBK];
 #pragma unroll
         for (int m = 0; m < NBK; ++m) {
-            const int k = tid + m * NTB;
+            int li, lj;
+            black_cell(m, li, lj);
             pos[m] = -1;
             nv[m] = T(0);
-            if (k >= PR * (RH - 2)) continue;
-            const int lj = k / PR + 1;
-            const int li = 2 * (k % PR) + 1 + (lj & 1);
-            if (li >= RW - 1) continue;
-            const int gi = i0 + li - 2, gj = j0 + lj - 2;
-            if (!in_dom(F, gi, gj)) continue;
-            const size_t p = F.off(gi, gj);
-            const T d = A.diag[p];
+            if (li < 0) continue;
+            const Row9<T> R = PRE ? rbk[m] : load_row9(A, F.off(i0 + li - 2, j0 + lj - 2), ld);
             auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            nv[m] = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+            nv[m] = R.d > T(0) ? (sb[lj][li] + row9_sum(R, nine, X)) / R.d : T(0);
             pos[m] = lj * RW + li;
         }
         __syncthreads();
@@ -228,9 +332,9 @@ __global__ void __launch_bounds__(NTB) k_up9(Dims F, OpV<T> A, const T* __restri
         const size_t p = F.off(gi, gj);
         T v = sx[lj][li];
         if (((gi + gj) & 1) == 0) {
-            const T d = A.diag[p];
+            const Row9<T> R = PRE ? rred : load_row9(A, p, ld);
             auto X = [&](int di, int dj) { return sx[lj + dj][li + di]; };
-            v = d > T(0) ? (sb[lj][li] + coupled_sum<T>(A, p, ld, X)) / d : T(0);
+            v = R.d > T(0) ? (sb[lj][li] + row9_sum(R, nine, X)) / R.d : T(0);
         }
         xo[p] = v;
     }
