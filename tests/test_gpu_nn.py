@@ -297,3 +297,27 @@ def test_learned_start_upper_bound():
     assert out[("pressure", net_level)] >= zero - 1
     assert out[("correction", net_level)] >= prev - 1
     assert out[("pressure", 1)] < zero
+
+
+def test_torch_after_library_context():
+    """torch ships its own HIP runtime beside the system one the library
+    links; once the library had initialised HIP, torch in the same process
+    saw no GPU.  Loading the library starts torch's runtime first, so a
+    fresh process that creates a context before touching torch still gets
+    torch's GPU (run in a child process: this one has both started)."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from simcoflow_amd import cases as C\n"
+            "from simcoflow_amd.projection import Projection, params_for\n"
+            "case = C.case_bubble(32, 64)\n"
+            "pj = Projection(case.grid, params_for(case.fluids))\n"
+            "import torch\n"
+            "x = torch.ones(8, device='cuda')\n"
+            "print(float(x.sum()))\n"
+            "pj.close()\n") % root
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip().splitlines()[-1] == "8.0", r.stdout
