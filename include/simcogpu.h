@@ -372,7 +372,12 @@ int sg_object_force(sg_ctx* ctx, double vs, double* force, double* clp, double* 
  * 1 - epsi) and carries VOF onto the new fluid part of each cell
  * (:2906-2917).  sg_get_moving_solid reads the moved shape and velocity
  * back; kind NONE disarms.  The U/V cells and apertures follow with
- * sg_geometry_rebuild after the transport, as Grid_Preprocess does. */
+ * sg_geometry_rebuild after the transport, as Grid_Preprocess does.  Both
+ * calls reject (SG_ERR_ARG) an unknown kind, a centre that is not finite,
+ * sizes that are not positive and a boom whose bar is not narrower than its
+ * float (b >= a) or has c < 0; sg_set_moving_solid also a velocity or
+ * acceleration that is not finite.  The shape (and motion) checks come before
+ * the context is used. */
 int sg_set_moving_solid(sg_ctx* ctx, const sg_solid_shape* shape, double us, double vs, double asx, double asy);
 int sg_get_moving_solid(sg_ctx* ctx, sg_solid_shape* shape, double* us, double* vs);
 /* the velocities after the solid moved (replaces InterNewVar,

@@ -586,20 +586,32 @@ int sg_particles_get(sg_ctx* ctx, int n, double* x, double* y, double* u, double
     return SG_OK;
 }
 
-int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape) {
-    Ctx* c = reinterpret_cast<Ctx*>(ctx);
-    if (!c || !shape || shape->kind < SG_SOLID_NONE || shape->kind > SG_SOLID_BOOM) {
-        set_error("sg_geometry_rebuild: need a shape of a known kind");
-        return SG_ERR_ARG;
+// the checks a solid shape passes before any kernel draws it: a known kind,
+// a finite centre, positive sizes, and for a boom a bar narrower than its
+// float (b < a) of non-negative length; who names the caller in the error
+static bool validate_shape(const sg_solid_shape* shape, const char* who) {
+    if (!shape || shape->kind < SG_SOLID_NONE || shape->kind > SG_SOLID_BOOM) {
+        set_error(std::string(who) + ": need a shape of a known kind");
+        return false;
     }
     if (shape->kind != SG_SOLID_NONE &&
         (!(shape->a > 0.0) || (shape->kind != SG_SOLID_CIRCLE && !(shape->b > 0.0)) ||
          !std::isfinite(shape->cx) || !std::isfinite(shape->cy))) {
-        set_error("sg_geometry_rebuild: the shape needs a finite centre and positive sizes");
-        return SG_ERR_ARG;
+        set_error(std::string(who) + ": the shape needs a finite centre and positive sizes");
+        return false;
     }
     if (shape->kind == SG_SOLID_BOOM && (!(shape->b < shape->a) || !(shape->c >= 0.0))) {
-        set_error("sg_geometry_rebuild: a boom needs bar half width b < radius a and bar length c >= 0");
+        set_error(std::string(who) + ": a boom needs bar half width b < radius a and bar length c >= 0");
+        return false;
+    }
+    return true;
+}
+
+int sg_geometry_rebuild(sg_ctx* ctx, const sg_solid_shape* shape) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!validate_shape(shape, "sg_geometry_rebuild")) return SG_ERR_ARG;
+    if (!c) {
+        set_error("sg_geometry_rebuild: null context");
         return SG_ERR_ARG;
     }
     if (c->ni < 4 || c->nj < 4) {
@@ -620,8 +632,13 @@ int sg_object_force(sg_ctx* ctx, double vs, double* force, double* clp, double* 
 
 int sg_set_moving_solid(sg_ctx* ctx, const sg_solid_shape* shape, double us, double vs, double asx, double asy) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
-    if (!c || !shape || shape->kind < SG_SOLID_NONE || shape->kind > SG_SOLID_BOOM) {
-        set_error("sg_set_moving_solid: need a shape of a known kind");
+    if (!validate_shape(shape, "sg_set_moving_solid")) return SG_ERR_ARG;
+    if (!std::isfinite(us) || !std::isfinite(vs) || !std::isfinite(asx) || !std::isfinite(asy)) {
+        set_error("sg_set_moving_solid: velocities and accelerations must be finite");
+        return SG_ERR_ARG;
+    }
+    if (!c) {
+        set_error("sg_set_moving_solid: null context");
         return SG_ERR_ARG;
     }
     c->moving.shape = *shape;
