@@ -297,6 +297,12 @@ def redistance(cg, vof, nx, ny, dis, phi):
     return phi
 
 
+def inflow_level(cg, wave, time, k, dtv):
+    """Depthw + etau of X_Sweep for sub-step k (wave = depth, amp, kw, omega)."""
+    w = _Wave(*wave)
+    return _cls_lib().orc_inflow_level(cg.ref, ctypes.byref(w), time, k, dtv)
+
+
 def clsvof_advect(cg, vof, phi, u, v, dt, time=0.0, nsub=6, wave=(1.0e30, 0.0, 0.0, 0.0)):
     """Coupled_LS_VOF: returns (vof, phi, nx, ny) after nsub sub-steps."""
     vof = np.ascontiguousarray(vof, dtype=np.float64).copy()

@@ -21,12 +21,15 @@
  * orc_redistance (straight line, circle), the sweeps (exact translation,
  * uniform states, volume; the level-set face value by exact translation of a
  * linear profile and first-order upwind differences on a quadratic one, both
- * sweeps and both flow directions), orc_interface_to_uv (straight lines,
- * solid bounds).  Parity unpinned (restated line by line, nothing independent
- * fixes them): the quality-based stencil choice of orc_normal_irre, the stale
- * normals on the domain edge, the tie order of orc_redistance, the one-sided
- * level-set fluxes at the first and last faces, orc_inflow_level, and the
- * stale-value walk-back of orc_interface_to_uv.
+ * sweeps and both flow directions; the one-sided level-set fluxes of the
+ * first and last faces and the inflow at face 0 by the positions they sample
+ * on a linear profile, uniform and stretched grids), orc_inflow_level (the
+ * sine's known values at the west edge) and the rows the west inflow enters,
+ * orc_interface_to_uv (straight lines, solid bounds).  Parity unpinned
+ * (restated line by line, nothing independent fixes them): the
+ * quality-based stencil choice of orc_normal_irre, the stale normals on the
+ * domain edge, the tie order of orc_redistance, and the stale-value
+ * walk-back of orc_interface_to_uv.
  */
 #include <math.h>
 #include <stdlib.h>

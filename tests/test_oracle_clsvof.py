@@ -418,3 +418,140 @@ def test_level_set_sweep_flux_y(oracle_lib, ve):
         up[:-1, :] = phi[1:, :] - phi[:-1, :]
     want = phi - ve * dtv / g.dy[0] * up
     assert np.abs(tl - want)[2:nj - 2, :].max() < 1e-14
+
+
+# ---- the sweeps' boundary faces (X_Sweep :1098-1123, Y_Sweep :1218-1243) ------
+def _stretched_axis(n, length, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(0.6, 1.4, n)
+    d = length * d / d.sum()
+    nodes = np.concatenate([[0.0], np.cumsum(d)])
+    return d, 0.5 * (nodes[1:] + nodes[:-1]), nodes
+
+
+def _boundary_face_values(lin, c, h, f, w, s):
+    """Face values of a level set linear along the sweep, lin(position),
+    stated as the positions they sample: c, h, f are the cell centres, widths
+    and faces 0..n, w the (uniform) velocity, s = w dtv.  Face 0 (inflow)
+    takes cell 1's value; face 1 and an outflow last face the value at the
+    face half a time step upstream, f - s/2 (exact for a translation); an
+    interior face the midpoint of the upwind cell's two neighbours (face n-1
+    with w < 0 has no second neighbour east and takes cell n-1's); the last
+    face with inflow (w <= 0) steps from cell n's centre by half of cell
+    n-1's width, then half a time step."""
+    n = len(c)
+    l = np.empty(n + 1)
+    l[0] = lin(c[0])
+    l[1] = lin(f[1] - s / 2)
+    for a in range(2, n):  # face a between cells a and a+1 (1-based)
+        if w >= 0 or a == n - 1:
+            l[a] = lin(0.5 * (c[a] + c[a - 2]))
+        else:
+            l[a] = lin(0.5 * (c[a + 1] + c[a - 1]))
+    l[n] = lin(f[n] - s / 2) if w > 0 else lin(c[n - 1] - h[n - 2] / 2 - s / 2)
+    return l
+
+
+@pytest.mark.parametrize("ue", [0.8, -0.6])
+@pytest.mark.parametrize("stretched", [False, True])
+def test_level_set_boundary_faces_x(oracle_lib, ue, stretched):
+    """X_Sweep's one-sided level-set fluxes at the first and last faces and
+    the first-order inflow at face 0, on a level set linear in x: every
+    cell's update is the difference of the face values sampled where
+    _boundary_face_values says.  On a stretched grid with u < 0 cells 1
+    and 2 are left out: X_Sweep's inflow form at face 1 divides by 2 dx(2),
+    which is the centre distance only on a uniform grid."""
+    O = oracle_lib
+    ni, nj = 20, 5
+    if stretched:
+        dx, xc, xn = _stretched_axis(ni, 1.2, 7)
+        dy = np.full(nj, 0.3 / nj)
+        yn = np.concatenate([[0.0], np.cumsum(dy)])
+        yc = 0.5 * (yn[1:] + yn[:-1])
+        g = C.Grid(ni, nj, 0.0, 0.0, dx, dy, np.append(xc[1:] - xc[:-1], dx[-1]),
+                   np.append(yc[1:] - yc[:-1], dy[-1]), xc, yc)
+    else:
+        g = C.uniform_grid(ni, nj, 1.2, 0.3)
+        xn = np.concatenate([[0.0], np.cumsum(g.dx)])
+    cg = O.ClsGeom(g, C.build_geometry(g, None))
+    lin = lambda x: 0.3 + 1.7 * x
+    phi = np.ascontiguousarray(np.broadcast_to(lin(g.xc), g.shape))
+    vof = np.ones(g.shape)
+    nx, ny, dis = np.zeros(g.shape), np.ones(g.shape), np.full(g.shape, -1.0)
+    dtv = 0.2 * g.dx.min() / abs(ue)
+    _, tl = O.x_sweep(cg, vof, phi, nx, ny, dis, np.full((nj, ni + 1), ue), dtv, 1.0e30, vof.copy(), phi.copy())
+    l = _boundary_face_values(lin, g.xc, g.dx, xn, ue, ue * dtv)
+    want = lin(g.xc) - ue * dtv * (l[1:] - l[:-1]) / g.dx
+    cells = slice(2, None) if (stretched and ue < 0) else slice(None)
+    assert np.abs(tl - want[None, :])[:, cells].max() < 1e-13
+    # the two ends move as the closed forms say, not as an exact translation
+    # (a dropped half step or a wrong neighbour breaks one of them)
+    s, h = ue * dtv, g.dx[0]
+    if not stretched:
+        assert abs(tl[0, 0] - (lin(g.xc[0]) - s * 1.7 * (h - s) / (2 * h))) < 1e-13
+        last = lin(g.xc[-1]) - s * 1.7 * ((3 * h - s) if ue > 0 else (h - s)) / (2 * h)
+        assert abs(tl[0, -1] - last) < 1e-13
+
+
+@pytest.mark.parametrize("ve", [0.7, -0.5])
+def test_level_set_boundary_faces_y(oracle_lib, ve):
+    """Y_Sweep's boundary faces on a stretched grid: its face-1 inflow form
+    divides by the centre distance y(3) - y(1), so both signs sample
+    f1 - s/2 exactly on any grid; the rest as in x."""
+    O = oracle_lib
+    ni, nj = 5, 20
+    dy, yc, yn = _stretched_axis(nj, 1.2, 11)
+    dx = np.full(ni, 0.3 / ni)
+    xn = np.concatenate([[0.0], np.cumsum(dx)])
+    xc = 0.5 * (xn[1:] + xn[:-1])
+    g = C.Grid(ni, nj, 0.0, 0.0, dx, dy, np.append(xc[1:] - xc[:-1], dx[-1]),
+               np.append(yc[1:] - yc[:-1], dy[-1]), xc, yc)
+    cg = O.ClsGeom(g, C.build_geometry(g, None))
+    lin = lambda y: -0.4 + 2.3 * y
+    phi = np.ascontiguousarray(np.broadcast_to(lin(yc)[:, None], g.shape))
+    vof = np.ones(g.shape)
+    nx, ny, dis = np.zeros(g.shape), np.ones(g.shape), np.full(g.shape, -1.0)
+    dtv = 0.2 * dy.min() / abs(ve)
+    _, tl = O.y_sweep(cg, vof, phi, nx, ny, dis, np.full((nj + 1, ni), ve), dtv, vof.copy(), phi.copy())
+    l = _boundary_face_values(lin, yc, dy, yn, ve, ve * dtv)
+    want = lin(yc) - ve * dtv * (l[1:] - l[:-1]) / dy
+    assert np.abs(tl - want[:, None]).max() < 1e-13
+
+
+def test_inflow_level_known_answers(oracle_lib):
+    """X_Sweep's inflow level Depthw + Amp0 sin(kw xwu - omega (t + k dtv))
+    (Clsvof_Mod.f90:1005,1057) at the grid's west edge xwu: the sine's
+    extremes and zero where its argument is +-pi/2 and 0 -- the west edge,
+    not the first centre, and the time of the sub-step's end."""
+    O = oracle_lib
+    g = C.uniform_grid(8, 4, 2.0, 1.0, x0=1.0)
+    cg = O.ClsGeom(g, C.build_geometry(g, None))
+    depth, amp = 0.6, 0.05
+    kw = np.pi / 2  # kw * xwu = pi/2 at the west edge x = 1
+    assert O.inflow_level(cg, (depth, 0.0, kw, 3.0), 0.7, 2, 0.1) == depth
+    assert abs(O.inflow_level(cg, (depth, amp, kw, 0.0), 0.0, 1, 0.1) - (depth + amp)) < 1e-15
+    # omega (t + k dtv) = pi/2: the argument is 0
+    assert abs(O.inflow_level(cg, (depth, amp, kw, np.pi / 2), 0.5, 5, 0.1) - depth) < 1e-15
+    # omega (t + k dtv) = pi: the argument is -pi/2
+    assert abs(O.inflow_level(cg, (depth, amp, kw, np.pi), 0.8, 2, 0.1) - (depth - amp)) < 1e-15
+
+
+def test_inflow_below_level_only(oracle_lib):
+    """The west inflow of X_Sweep's volume fraction (:1057-1069) enters the
+    rows whose bottom edge lies strictly below the level: a full column
+    moving east keeps vof 1 in those first cells and loses u dtv / dx in
+    the rows above (a row whose bottom edge is the level gets none)."""
+    O = oracle_lib
+    ni, nj = 8, 6
+    g = C.uniform_grid(ni, nj, 2.0, 1.5)  # dy = 0.25: edges exact in binary
+    cg = O.ClsGeom(g, C.build_geometry(g, None))
+    vof = np.ones(g.shape)
+    phi = np.ascontiguousarray(np.broadcast_to(-1.0 - g.yc[:, None], g.shape))
+    nx, ny, dis = np.zeros(g.shape), np.ones(g.shape), np.full(g.shape, -1.0)
+    ue, dtv = 0.5, 0.1
+    level = 0.75  # the bottom edge of row 4
+    tv, _ = O.x_sweep(cg, vof, phi, nx, ny, dis, np.full((nj, ni + 1), ue), dtv, level, vof.copy(), phi.copy())
+    loss = ue * dtv / g.dx[0]
+    assert np.all(np.abs(tv[:3, 0] - 1.0) < 1e-15)
+    assert np.all(np.abs(tv[3:, 0] - (1.0 - loss)) < 1e-15)
+    assert np.all(np.abs(tv[:, 1:] - 1.0) < 1e-15)
