@@ -26,11 +26,14 @@
  * TVD region and symmetry; the MUSCL / kinetic face fluxes exact on linear
  * profiles; the mixing of a cell with one target (the volume-weighted mean),
  * its conservation of sum (1 - vofS) flux, and the reference's overwrite of
- * a small cell that an earlier one mixed into.  Parity unpinned (restated
- * line by line): the first-order face fluxes of the first and last faces,
- * the Beta weights of a cell with several targets, and the cut-cell
- * interpolation branches (modified_conv and the near-solid branches of
- * high_order_x/y, whose geometry inputs are pinned, not their use).
+ * a small cell that an earlier one mixed into; the first and last faces'
+ * fluxes (local Lax-Friedrichs on expansions, upwind for the carried
+ * component) on stretched grids; a source split equally among three
+ * targets.  Parity unpinned (restated line by line): how Beta weighs the
+ * targets' fluid volumes (the fifth power), the first NS face's U momentum
+ * as gfortran parses it, and the cut-cell interpolation branches
+ * (modified_conv and the near-solid branches of high_order_x/y, whose
+ * geometry inputs are pinned, not their use).
  */
 #include <math.h>
 #include <stdlib.h>

@@ -14,13 +14,16 @@ by closed forms that do not restate its formulas:
   deterministic output.
 
 and piece by piece: the superbee limiter (Sweby's table, TVD region,
-symmetry), the MUSCL / kinetic face flux on linear profiles, the mixing of a
-small cell (one target: the volume-weighted mean; several: conservation of
-sum (1 - vofS) flux; the reference's overwrite of a small target).
+symmetry), the MUSCL / kinetic face flux on linear profiles, the first and
+last faces (local Lax-Friedrichs on expansions, upwind for the carried
+component), the mixing of a small cell (one target: the volume-weighted
+mean; three equal targets: the source split in thirds; several:
+conservation of sum (1 - vofS) flux; the reference's overwrite of a small
+target).
 
-Parity-unpinned (the oracle follows the source line by line): the one-sided
-first/last face fluxes, the Beta weights of several targets, and the
-near-solid interpolation branches.
+Parity-unpinned (the oracle follows the source line by line): how Beta
+weighs the targets' fluid volumes, the first NS face's U momentum as
+gfortran parses it, and the near-solid interpolation branches.
 """
 import numpy as np
 import pytest
@@ -298,3 +301,129 @@ def test_mixing_fully_solid_target_takes_no_share(oracle_lib):
     act[j, i - 1] = act[j + 1, i - 1] = 0
     out = O.mixing(g, _with(U, vofS=vofS, nxS=nxS, nyS=nyS, active=act), V, 0, f, pcell, P.phys_of(C.Fluids()))
     assert np.array_equal(out, f)
+
+
+# ---- the first and last faces (HighOrderConvectiveFluxForXDir/YDir) ---------
+_PAIRS = [(0.3, 0.7), (-0.7, -0.2), (0.5, 0.5), (-0.4, -0.4), (0.0, 0.6), (-0.9, 0.0), (0.25, 1.1)]
+
+
+def _stream_stretched(ni, nj, seed):
+    """_stream on a grid stretched in both directions, so that the P, U and
+    V cell widths all differ."""
+    rng = np.random.default_rng(seed)
+    dx, dy = rng.uniform(0.6, 1.4, ni), rng.uniform(0.6, 1.4, nj)
+    dx, dy = dx / dx.sum(), (nj / ni) * dy / dy.sum()
+    xn, yn = np.concatenate([[0.0], np.cumsum(dx)]), np.concatenate([[0.0], np.cumsum(dy)])
+    xc, yc = 0.5 * (xn[1:] + xn[:-1]), 0.5 * (yn[1:] + yn[:-1])
+    g = C.Grid(ni, nj, 0.0, 0.0, dx, dy, np.append(xc[1:] - xc[:-1], dx[-1]), np.append(yc[1:] - yc[:-1], dy[-1]),
+               xc, yc)
+    geo = C.build_geometry(g, None)
+    U, V = C.staggered_geometry(g, geo, None)
+    z = np.zeros(g.shape)
+    pcell = dict(vof=np.ones(g.shape), phi=np.full(g.shape, -1.0), nx=z, ny=z + 1.0,
+                 vofS=geo.vofS, nxS=geo.nxS, nyS=geo.nyS, phiS=geo.phiS)
+    return g, U, V, pcell, np.ones(g.shape)
+
+
+def _lf_expansion(ul, ur):
+    """The local Lax-Friedrichs flux 0.5 (ul^2 + ur^2 - max(|ul|,|ur|)
+    (ur - ul)) of the momentum u^2 on an expansion (ul <= ur) of one sign:
+    the upwind momentum carried by the mean velocity.  Derived from the
+    wave-speed bound, not from the reference's expression: with 0 <= ul <= ur
+    the bound is ur and the flux ul (ul + ur) / 2; with ul <= ur <= 0 it is
+    -ul and the flux ur (ul + ur) / 2."""
+    assert ul <= ur and (ul >= 0.0 or ur <= 0.0)
+    return ul * (ul + ur) / 2 if ul >= 0.0 else ur * (ul + ur) / 2
+
+
+def test_boundary_face_fluxes_x(oracle_lib):
+    """The first and last EW faces (PredictorUV_mod.f90:1198-1206,1294-1303)
+    take the ghost velocities, not the MUSCL reconstruction: the U momentum
+    is the local Lax-Friedrichs flux of u^2 (closed form on expansions), the
+    V momentum the upwind v carried by the face's u, the mean of the two U
+    faces on the boundary column.  Stretched grid: the face lengths are the
+    U and V cells' own heights."""
+    O = oracle_lib
+    g, U, V, pcell, uv = _stream_stretched(12, 10, 1)
+    ni, nj = g.ni, g.nj
+    phys = P.phys_of(C.Fluids())
+    rng = np.random.default_rng(5)
+    ug = rng.uniform(-0.5, 0.5, (nj + 2, ni + 2))
+    vg = rng.uniform(-0.5, 0.5, (nj + 2, ni + 2))
+    for j in range(1, nj + 1):
+        ug[j, 0], ug[j, 1] = _PAIRS[j % len(_PAIRS)]
+        ug[j, ni], ug[j, ni + 1] = _PAIRS[(j + 3) % len(_PAIRS)]
+    F1, _ = O.conv_fluxes(g, U, V, uv, uv, pcell, ug, vg, phys, 0.01)
+    for j in range(1, nj + 1):
+        for i, (a, b) in ((1, (0, 1)), (ni + 1, (ni, ni + 1))):
+            ap = U.aW[j, 1] if i == 1 else U.aE[j, ni]
+            assert ap > 0.5
+            want = _lf_expansion(ug[j, a], ug[j, b]) * ap * g.dy[j - 1]
+            assert F1[0, j, i] == pytest.approx(want, rel=1e-14, abs=1e-16), (i, j)
+            c = 0 if i == 1 else ni
+            w = 0.5 * (ug[j, c] + ug[j + 1, c])
+            apv = V.aW[j, 1] if i == 1 else V.aE[j, ni]
+            vup = vg[j, a] if w > 0 else vg[j, b]
+            assert F1[1, j, i] == pytest.approx(w * vup * apv * g.vdy[j - 1], rel=1e-14, abs=1e-16), (i, j)
+
+
+def test_boundary_face_fluxes_y(oracle_lib):
+    """The first and last NS faces (:1319-1330,1417-1428): the V momentum as
+    the local Lax-Friedrichs flux of v^2 at both ends, the U momentum at the
+    last face as the upwind u carried by the mean of the two V faces on the
+    boundary row.  (The first face's U momentum is the reference's
+    alr*ur*+alr*ul as gfortran parses it, a recorded deviation left out
+    here.)"""
+    O = oracle_lib
+    g, U, V, pcell, uv = _stream_stretched(10, 12, 2)
+    ni, nj = g.ni, g.nj
+    phys = P.phys_of(C.Fluids())
+    rng = np.random.default_rng(6)
+    ug = rng.uniform(-0.5, 0.5, (nj + 2, ni + 2))
+    vg = rng.uniform(-0.5, 0.5, (nj + 2, ni + 2))
+    for i in range(1, ni + 1):
+        vg[0, i], vg[1, i] = _PAIRS[i % len(_PAIRS)]
+        vg[nj, i], vg[nj + 1, i] = _PAIRS[(i + 2) % len(_PAIRS)]
+    _, F2 = O.conv_fluxes(g, U, V, uv, uv, pcell, ug, vg, phys, 0.01)
+    for i in range(1, ni + 1):
+        for j, (a, b) in ((1, (0, 1)), (nj + 1, (nj, nj + 1))):
+            ap = V.aS[1, i] if j == 1 else V.aN[nj, i]
+            assert ap > 0.5
+            want = _lf_expansion(vg[a, i], vg[b, i]) * ap * g.dx[i - 1]
+            assert F2[1, j, i] == pytest.approx(want, rel=1e-14, abs=1e-16), (i, j)
+        w = 0.5 * (vg[nj, i] + vg[nj, i + 1])
+        uup = ug[nj, i] if w > 0 else ug[nj + 1, i]
+        assert F2[0, nj + 1, i] == pytest.approx(w * uup * U.aN[nj, i] * g.udx[i - 1], rel=1e-14, abs=1e-16), i
+
+
+def test_mixing_three_equal_targets_split_the_source(oracle_lib):
+    """Mixing_Procedure (PredictorUV_mod.f90:2139-2183) read as a split: the
+    source's fluid volume (1 - s) is shared among its targets in proportion
+    to Beta, each share is mixed with its target to their volume-weighted
+    mean, and the source ends at the Beta-weighted mean of those means.  A
+    source whose solid normal is at 45 degrees, with three targets of equal
+    fluid volume, gives them equal Beta by symmetry (nx^2 = ny^2 = |nx ny|),
+    so each share is (1 - s) / 3 -- without using how Beta weighs volumes."""
+    O = oracle_lib
+    g, U, V, pcell, vofS, nxS, nyS, act, rng = _mixing_setup()
+    j, i = 7, 5
+    s, vs_t = 0.8, 0.2
+    vofS[j, i] = s
+    nxS[j, i] = nyS[j, i] = np.cos(np.pi / 4)
+    targets = [(j, i + 1), (j + 1, i), (j + 1, i + 1)]
+    for t in targets:
+        vofS[t] = vs_t
+    f = rng.standard_normal(g.shape)
+    out = O.mixing(g, _with(U, vofS=vofS, nxS=nxS, nyS=nyS, active=act), V, 0, f, pcell, P.phys_of(C.Fluids()))
+    share, Vt = (1 - s) / 3, 1 - vs_t
+    means = [(share * f[j, i] + Vt * f[t]) / (share + Vt) for t in targets]
+    for t, m in zip(targets, means):
+        assert out[t] == pytest.approx(m, rel=1e-13)
+    assert out[j, i] == pytest.approx(sum(means) / 3, rel=1e-13)
+    other = np.ones(g.shape, bool)
+    for c in targets + [(j, i)]:
+        other[c] = False
+    assert np.array_equal(out[other], f[other])
+    # conservation of sum (1 - vofS) flux follows from the split
+    w = 1.0 - vofS
+    assert (w * out).sum() == pytest.approx((w * f).sum(), rel=1e-13, abs=1e-13)
