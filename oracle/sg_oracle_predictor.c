@@ -29,7 +29,10 @@
  * a small cell that an earlier one mixed into; the first and last faces'
  * fluxes (local Lax-Friedrichs on expansions, upwind for the carried
  * component) on stretched grids; a source split equally among three
- * targets.  Parity unpinned (restated line by line): how Beta weighs the
+ * targets; the mass-momentum consistency of the density-based faces (a
+ * uniform stream across an inclined interface stays uniform).  Parity
+ * unpinned (restated line by line): the swept-region density those faces
+ * share (pfrac / CellGeoCal of the swept box), how Beta weighs the
  * targets' fluid volumes (the fifth power), the first NS face's U momentum
  * as gfortran parses it, and the cut-cell interpolation branches
  * (modified_conv and the near-solid branches of high_order_x/y, whose

@@ -21,9 +21,14 @@ mean; three equal targets: the source split in thirds; several:
 conservation of sum (1 - vofS) flux; the reference's overwrite of a small
 target).
 
-Parity-unpinned (the oracle follows the source line by line): how Beta
-weighs the targets' fluid volumes, the first NS face's U momentum as
-gfortran parses it, and the near-solid interpolation branches.
+The density-based faces of the mixture cells are pinned by their
+mass-momentum consistency (a uniform stream across an inclined interface
+stays uniform).
+
+Parity-unpinned (the oracle follows the source line by line): the
+swept-region density those faces share, how Beta weighs the targets' fluid
+volumes, the first NS face's U momentum as gfortran parses it, and the
+near-solid interpolation branches.
 """
 import numpy as np
 import pytest
@@ -427,3 +432,45 @@ def test_mixing_three_equal_targets_split_the_source(oracle_lib):
     # conservation of sum (1 - vofS) flux follows from the split
     w = 1.0 - vofS
     assert (w * out).sum() == pytest.approx((w * f).sum(), rel=1e-13, abs=1e-13)
+
+
+@pytest.mark.parametrize("flow", ["x", "y"])
+def test_two_phase_uniform_stream_is_steady(oracle_lib, flow):
+    """Mass-momentum consistency of the density-based convective update
+    (FaceDensityFlux / DensityBasedConvectiveFlux, PredictorUV_mod.f90:
+    1751-2027, used in the mixture cells): when each face's momentum flux is
+    the face velocity times the mass flux that updates the cell's density,
+    a uniform velocity stays uniform across a density jump of any shape
+    (the requirement of consistent mass and momentum transport for large
+    density ratios).  An inclined water-air interface makes the mixture
+    densities differ from face to face, so a momentum flux taking another
+    density than its mass flux breaks the steady state."""
+    O = oracle_lib
+    g = C.uniform_grid(32, 24, 1.0, 0.75)
+    geo = C.build_geometry(g, None)
+    U, V = C.staggered_geometry(g, geo, None)
+    th = np.deg2rad(30.0)
+    nx, ny = np.sin(th), np.cos(th)
+    X, Y = np.meshgrid(g.xc, g.yc)
+    phi = nx * X + ny * Y - 0.45
+    h = g.dx[0]
+    vof = C.frac(nx, ny, phi, g.dx[None, :], g.dy[:, None])
+    uvof = C.frac(nx, ny, phi + nx * 0.5 * h, g.udx[None, :], g.dy[:, None])
+    vvof = C.frac(nx, ny, phi + ny * 0.5 * h, g.dx[None, :], g.vdy[:, None])
+    pcell = dict(vof=vof, phi=phi, nx=np.full(g.shape, nx), ny=np.full(g.shape, ny), vofS=geo.vofS, nxS=geo.nxS,
+                 nyS=geo.nyS, phiS=geo.phiS)
+    ng = (g.nj + 2, g.ni + 2)
+    uc, vc = (0.37, 0.0) if flow == "x" else (0.0, 0.29)
+    d = dict(grid=g, U=U, V=V, uvo=uvof, uvn=uvof, vvo=vvof, vvn=vvof, pcell=pcell, u=np.full(ng, uc),
+             v=np.full(ng, vc), phys=P.phys_of(C.Fluids(), Fr=1e30), dt=0.004, fluxn1=np.zeros((2,) + ng),
+             DpU=np.zeros(ng), DpV=np.zeros(ng), upred=np.zeros(ng), vpred=np.zeros(ng))
+    r = P.oracle_predictor(O, d)
+    mixed = ((uvof > 0.01) & (uvof < 0.99)).sum() if flow == "x" else ((vvof > 0.01) & (vvof < 0.99)).sum()
+    assert mixed > 30
+    if flow == "x":
+        up = r["upred"][1:-1, 1:-1][:, :-1]
+        assert np.abs(up - uc).max() < 1e-12
+        assert np.abs(r["vpred"][1:-1, 1:-1][:-1, :]).max() < 1e-12
+    else:
+        vp = r["vpred"][1:-1, 1:-1][1:-2, :]  # away from the walls' rows
+        assert np.abs(vp - vc).max() < 1e-12
