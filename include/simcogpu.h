@@ -423,8 +423,10 @@ int sg_diagnostics_read(sg_ctx* ctx, int max_rows, double* rows, int* n_rows, lo
  * Solver_Mod.f90:164-195 with ReadOldData*, PrintResult.f90:192-270) -------
  * The state is what a time step reads from the one before it: the P and U/V
  * interfaces (VOF PHI NX NY, UVOF VVOF UPHI UNX UNY VPHI VNX VNY), U V P,
- * FLUXN1U FLUXN1V, the Poisson solution and the two momentum solutions (the
- * starts of the next solves) and the particle set, in FP64 as held, so a run
+ * FLUXN1U FLUXN1V, DPU DPV (an inactive U/V cell keeps its last Dp, which
+ * the next Poisson assembly can read), the Poisson solution and the two
+ * momentum solutions (the starts of the next solves) and the particle set,
+ * in FP64 as held, so a run
  * resumed from it repeats the uninterrupted run bit for bit.  Not included:
  * the geometry (set it up first, as the reference recomputes it), the
  * parameters, and the host's step counter and time.  sg_state_bytes gives

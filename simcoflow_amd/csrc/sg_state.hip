@@ -5,7 +5,10 @@
 //
 // The state is everything a time step reads from the step before it: the P
 // interface (vof, phi, nx, ny), the U/V interface (vof, phi, nx, ny), the
-// velocity and pressure, the Adams-Bashforth history Flux_n1, the Poisson
+// velocity and pressure, the Adams-Bashforth history Flux_n1, the
+// predictor's Dp of the U and V cells (a cell that is not active keeps the
+// last value written to it, and the next Poisson assembly reads Dp of every
+// face between wet cells, ProjectionP_Mod.f90:418-449), the Poisson
 // solution in the solver's layout (the start of a previous-pressure solve),
 // the last momentum solutions (the start of the next momentum solves) and
 // the particle set.  The geometry is not part of it: a restart sets the
@@ -23,10 +26,11 @@ namespace sg {
 namespace {
 
 constexpr char MAGIC[8] = {'S', 'G', 'S', 'T', 'A', 'T', 'E', '1'};
-constexpr int32_t VERSION = 1;
+constexpr int32_t VERSION = 2;
 constexpr int STATE_FIELDS[] = {SG_F_VOF,  SG_F_PHI,  SG_F_NX,  SG_F_NY,  SG_F_UVOF,    SG_F_VVOF,
                                 SG_F_UPHI, SG_F_UNX,  SG_F_UNY, SG_F_VPHI, SG_F_VNX,    SG_F_VNY,
-                                SG_F_U,    SG_F_V,    SG_F_P,   SG_F_FLUXN1U, SG_F_FLUXN1V};
+                                SG_F_U,    SG_F_V,    SG_F_P,   SG_F_FLUXN1U, SG_F_FLUXN1V,
+                                SG_F_DPU,  SG_F_DPV};
 constexpr int NSTATE = (int)(sizeof(STATE_FIELDS) / sizeof(STATE_FIELDS[0]));
 
 struct Header {

@@ -109,15 +109,22 @@ def test_restart_of_a_moving_solid(tmp_path):
     def make():
         return TimeStepper(case, precision=0, tol=1e-10, device_geometry=True,
                            moving=dict(mass=2.0 * 1000.0 * np.pi * 0.12 ** 2, g=case.fluids.g, start=3))
+    # every field a step leaves, the predictor's Dp included: a U/V cell the
+    # solid made inactive keeps its last Dp, which a later Poisson assembly
+    # reads, so the state must carry it
+    def fields(st):
+        return {f: st.pj.download(f) for f in ("U", "V", "P", "VOF", "PHI", "DPU", "DPV", "UPRED", "VPRED")}
     ck = str(tmp_path / "moving.ckpt")
     print("[restart] sinking cylinder 64^2: 12 steps straight through, checkpoint at 6", flush=True)
     a = make()
+    steps_a = []
     try:
         for _ in range(6):
             a.step()
         a.save(ck)
         for _ in range(6):
             a.step()
+            steps_a.append(fields(a))
         sa, ma = a.state(), dict(a.moving)
     finally:
         a.close()
@@ -126,8 +133,11 @@ def test_restart_of_a_moving_solid(tmp_path):
     try:
         b.load(ck)
         assert b.itt == 6
-        for _ in range(6):
+        for k in range(6):
             b.step()
+            fb = fields(b)
+            for f in fb:
+                assert np.array_equal(steps_a[k][f], fb[f]), (7 + k, f)
         sb, mb = b.state(), dict(b.moving)
         vtr = b.write_vtr(str(tmp_path))
         vofS = b.pj.download("VOFS")
