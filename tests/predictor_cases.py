@@ -37,6 +37,30 @@ def predictor_inputs(case, seed=0, amp=0.4, Re=None, shift=0.3):
                 DpU=DpU, DpV=DpV, upred=upred, vpred=vpred, geo=case.geo)
 
 
+def two_phase_stream(flow):
+    """A uniform stream (along x or y) in a 32x24 tank across a water-air
+    interface inclined at 30 degrees, no gravity: the input of the
+    mass-momentum consistency checks (the stream must stay uniform)."""
+    g = C.uniform_grid(32, 24, 1.0, 0.75)
+    geo = C.build_geometry(g, None)
+    U, V = C.staggered_geometry(g, geo, None)
+    th = np.deg2rad(30.0)
+    nx, ny = np.sin(th), np.cos(th)
+    X, Y = np.meshgrid(g.xc, g.yc)
+    phi = nx * X + ny * Y - 0.45
+    h = g.dx[0]
+    vof = C.frac(nx, ny, phi, g.dx[None, :], g.dy[:, None])
+    uvof = C.frac(nx, ny, phi + nx * 0.5 * h, g.udx[None, :], g.dy[:, None])
+    vvof = C.frac(nx, ny, phi + ny * 0.5 * h, g.dx[None, :], g.vdy[:, None])
+    pcell = dict(vof=vof, phi=phi, nx=np.full(g.shape, nx), ny=np.full(g.shape, ny), vofS=geo.vofS, nxS=geo.nxS,
+                 nyS=geo.nyS, phiS=geo.phiS)
+    ng = (g.nj + 2, g.ni + 2)
+    uc, vc = (0.37, 0.0) if flow == "x" else (0.0, 0.29)
+    return dict(grid=g, U=U, V=V, uvo=uvof, uvn=uvof, vvo=vvof, vvn=vvof, pcell=pcell, u=np.full(ng, uc),
+                v=np.full(ng, vc), phys=phys_of(C.Fluids(), Fr=1e30), dt=0.004, fluxn1=np.zeros((2,) + ng),
+                DpU=np.zeros(ng), DpV=np.zeros(ng), upred=np.zeros(ng), vpred=np.zeros(ng), geo=geo, uc=uc, vc=vc)
+
+
 def device_predictor(d, first_step=False, vb=0.0, momentum_method=0):
     """The same step through the C ABI; returns the device results."""
     from simcoflow_amd.projection import Projection, default_params

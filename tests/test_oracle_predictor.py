@@ -446,24 +446,8 @@ def test_two_phase_uniform_stream_is_steady(oracle_lib, flow):
     densities differ from face to face, so a momentum flux taking another
     density than its mass flux breaks the steady state."""
     O = oracle_lib
-    g = C.uniform_grid(32, 24, 1.0, 0.75)
-    geo = C.build_geometry(g, None)
-    U, V = C.staggered_geometry(g, geo, None)
-    th = np.deg2rad(30.0)
-    nx, ny = np.sin(th), np.cos(th)
-    X, Y = np.meshgrid(g.xc, g.yc)
-    phi = nx * X + ny * Y - 0.45
-    h = g.dx[0]
-    vof = C.frac(nx, ny, phi, g.dx[None, :], g.dy[:, None])
-    uvof = C.frac(nx, ny, phi + nx * 0.5 * h, g.udx[None, :], g.dy[:, None])
-    vvof = C.frac(nx, ny, phi + ny * 0.5 * h, g.dx[None, :], g.vdy[:, None])
-    pcell = dict(vof=vof, phi=phi, nx=np.full(g.shape, nx), ny=np.full(g.shape, ny), vofS=geo.vofS, nxS=geo.nxS,
-                 nyS=geo.nyS, phiS=geo.phiS)
-    ng = (g.nj + 2, g.ni + 2)
-    uc, vc = (0.37, 0.0) if flow == "x" else (0.0, 0.29)
-    d = dict(grid=g, U=U, V=V, uvo=uvof, uvn=uvof, vvo=vvof, vvn=vvof, pcell=pcell, u=np.full(ng, uc),
-             v=np.full(ng, vc), phys=P.phys_of(C.Fluids(), Fr=1e30), dt=0.004, fluxn1=np.zeros((2,) + ng),
-             DpU=np.zeros(ng), DpV=np.zeros(ng), upred=np.zeros(ng), vpred=np.zeros(ng))
+    d = P.two_phase_stream(flow)
+    uvof, vvof, uc, vc = d["uvo"], d["vvo"], d["uc"], d["vc"]
     r = P.oracle_predictor(O, d)
     mixed = ((uvof > 0.01) & (uvof < 0.99)).sum() if flow == "x" else ((vvof > 0.01) & (vvof < 0.99)).sum()
     assert mixed > 30
