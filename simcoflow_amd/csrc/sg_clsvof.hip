@@ -584,65 +584,96 @@ __device__ __forceinline__ void append(bool pred, unsigned* count, int* list, in
     if (pred) list[base + (unsigned)__popcll(b & ((1ull << lane) - 1ull))] = value;
 }
 
-// Block-aggregated append (one atomic per workgroup): every thread of the
-// workgroup must call it; NW = wavefronts per workgroup.  Lists grow in
-// arbitrary order -- their users treat every entry on its own.
-template <int NW>
-__device__ __forceinline__ void block_append(bool pred, unsigned* count, int* list, int value) {
-    __shared__ unsigned wcnt[NW], wbase[NW + 1];
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    const int lane = t & 63, w = t >> 6;
-    const unsigned long long b = __ballot(pred);
-    if (lane == 0) wcnt[w] = (unsigned)__popcll(b);
-    __syncthreads();
-    if (t == 0) {
-        unsigned s = 0;
-        for (int k = 0; k < NW; ++k) {
-            wbase[k] = s;
-            s += wcnt[k];
-        }
-        wbase[NW] = s ? atomicAdd(count, s) : 0u;
-    }
-    __syncthreads();
-    if (pred) list[wbase[NW] + wbase[w] + (unsigned)__popcll(b & ((1ull << lane) - 1ull))] = value;
-}
-
 // Interface_Reconstruct.  Interface cells get their normal and plane
 // constant; three-phase cells whose Ridders search leaves the plane constant
 // unassigned (the reference then keeps the previous cell's) are flagged in
 // `stale` and listed for k_dis_fix.  full_out: also write n = (0,1) for the
 // other cells (the PCell%nx/ny output).  With mask != null the kernel also
 // writes the re-distancing source mask (vofeps < vof < 1-vofeps-vofS, one bit
-// per cell, one 64-bit word per wavefront) and lists those cells.
-// cnt[0]: sources listed, cnt[1]: stale cells listed.
-constexpr int RECON_TY = 16;  // rows per k_recon workgroup (64 x 16 cells)
+// per cell, one 64-bit word per wavefront and row) and lists those cells.
+// cnt[0]: sources listed, cnt[1]: stale cells listed, cnt[2]: interface
+// cells listed.
+//
+// This scan is bound by its appends, not its bytes: every workgroup that
+// lists a cell takes one atomic on a counter all workgroups share.  So a
+// workgroup covers 64 x 64 cells, each thread RECON_R rows, and lists all of
+// them with one scan and one atomic per list (the two per-thread counts
+// packed in one word, low and high 16 bits: at most 4096 of each per
+// workgroup).
+constexpr int RECON_TY = 16;  // wavefronts per k_recon workgroup
+constexpr int RECON_R = 4;    // rows per thread: a workgroup covers 64 x (RECON_TY * RECON_R) cells
 __global__ void __launch_bounds__(64 * RECON_TY) k_recon(ClsGeo G, const double* __restrict__ vof, Planes P,
                                                          int full_out,
                                                unsigned long long* __restrict__ mask, int mw,
                                                int* __restrict__ srcs, int* __restrict__ ifc,
                                                unsigned* __restrict__ cnt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const bool in = i <= G.ni && j <= G.nj;
-    bool src = false, itf = false, heavy = false;
-    size_t c = 0;
-    if (in) {
-        c = cix(G, i, j);
-        const double vf = vof[c], vs = G.vofS[c];
-        src = (0.0 + VOFEPS) < vf && vf < (1.0 - VOFEPS - vs);
-        itf = recon_class(vf, vs) != 0;
-        heavy = !(vf >= (1.0 - VOFEPS) || vf <= VOFEPS);  // vof_flux's PLIC branch
-        if (!itf && full_out) {
+    __shared__ unsigned wsum[RECON_TY], wbase_i[RECON_TY], wbase_s[RECON_TY], gbase[2];
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int i = blockIdx.x * 64 + lane + 1;
+    const int j0 = blockIdx.y * (RECON_TY * RECON_R) + w + 1;
+    // every row's operands loaded before any test
+    double vf[RECON_R], vs[RECON_R];
+#pragma unroll
+    for (int r = 0; r < RECON_R; ++r) {
+        const int j = j0 + r * RECON_TY;
+        const bool in = i <= G.ni && j <= G.nj;
+        const size_t c = in ? cix(G, i, j) : 0;
+        vf[r] = in ? vof[c] : 0.0;
+        vs[r] = in ? G.vofS[c] : 0.0;
+    }
+    unsigned list_i = 0, list_s = 0;  // bit r: row r's cell goes to ifc / srcs
+#pragma unroll
+    for (int r = 0; r < RECON_R; ++r) {
+        const int j = j0 + r * RECON_TY;
+        const bool in = i <= G.ni && j <= G.nj;
+        const bool src = in && (0.0 + VOFEPS) < vf[r] && vf[r] < (1.0 - VOFEPS - vs[r]);
+        const bool itf = in && recon_class(vf[r], vs[r]) != 0;
+        const bool heavy = in && !(vf[r] >= (1.0 - VOFEPS) || vf[r] <= VOFEPS);  // vof_flux's PLIC branch
+        if (in && !itf && full_out) {
+            const size_t c = cix(G, i, j);
             P.nx[c] = 0.0;
             P.ny[c] = 1.0;
         }
+        if (itf || heavy) list_i |= 1u << r;
+        if (mask) {
+            // one wavefront covers 64 consecutive i of row j
+            const unsigned long long bits = __ballot(src);
+            if (lane == 0 && j <= G.nj) mask[(size_t)(j - 1) * mw + blockIdx.x] = bits;
+            if (src) list_s |= 1u << r;
+        }
     }
-    block_append<RECON_TY>(itf || heavy, cnt + 2, ifc, (int)c);
-    if (mask) {
-        // blockDim.x == 64: one wavefront covers 64 consecutive i of row j
-        const unsigned long long bits = __ballot(src);
-        if (threadIdx.x == 0 && j <= G.nj) mask[(size_t)(j - 1) * mw + blockIdx.x] = bits;
-        block_append<RECON_TY>(src, cnt, srcs, (int)c);
+    // exclusive offsets of this thread's entries in the workgroup's share of
+    // each list: an inclusive scan of the packed counts over the wavefront,
+    // then over the wavefronts
+    const unsigned own = (unsigned)__popc(list_i) | ((unsigned)__popc(list_s) << 16);
+    unsigned incl = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (w == 0 && lane == 0) {
+        unsigned si = 0, ss = 0;
+        for (int k = 0; k < RECON_TY; ++k) {
+            wbase_i[k] = si;
+            wbase_s[k] = ss;
+            si += wsum[k] & 0xFFFFu;
+            ss += wsum[k] >> 16;
+        }
+        gbase[0] = si ? atomicAdd(cnt + 2, si) : 0u;
+        gbase[1] = ss ? atomicAdd(cnt, ss) : 0u;
+    }
+    __syncthreads();
+    const unsigned excl = incl - own;
+    unsigned oi = gbase[0] + wbase_i[w] + (excl & 0xFFFFu);
+    unsigned os = gbase[1] + wbase_s[w] + (excl >> 16);
+#pragma unroll
+    for (int r = 0; r < RECON_R; ++r) {
+        const int c = (int)cix(G, i, j0 + r * RECON_TY);
+        if (list_i >> r & 1u) ifc[oi++] = c;
+        if (list_s >> r & 1u) srcs[os++] = c;
     }
 }
 
@@ -1578,7 +1609,8 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
     const int ring = 2 * c->ni + 2 * (c->nj - 2);
     const dim3 rblk(256), rgrd((ring + 255) / 256);
     const dim3 pblk(256), pgrd(1024);  // grid-stride loops over the listed pairs / cells
-    const dim3 rcblk(64, RECON_TY), rcgrd((c->ni + 63) / 64, (c->nj + RECON_TY - 1) / RECON_TY);
+    const dim3 rcblk(64, RECON_TY),
+        rcgrd((c->ni + 63) / 64, (c->nj + RECON_TY * RECON_R - 1) / (RECON_TY * RECON_R));
     // sweeps: 32x8 tiles (3% faster per transport call than 64x4, 256x1 or
     // 128x2 at 2048^2: the y stencil's rows stay within fewer tiles)
     const dim3 sblk(32, 8), sgrd((c->ni + 31) / 32, (c->nj + 7) / 8);
@@ -1595,11 +1627,16 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
     int slot = 0;
     // `scan`: the interface list of the last full-grid scan, when vof and
     // vofS have not changed since (re-distancing changes phi only)
-    auto recon = [&](const double* v, const double* p, bool last, const unsigned* scan) {
+    // last: the sub-step's final reconstruction (re-distancing sources);
+    // full: also n = (0,1) on the other cells -- only the transport's very
+    // last one, since within the transport a stored plane is read only
+    // where the cell is an interface cell of the current vof (plane_of,
+    // plane_pick), and every such plane is rewritten by k_recon_ifc
+    auto recon = [&](const double* v, const double* p, bool last, const unsigned* scan, bool full) {
         unsigned* cnt = B.cnt + slot;
         slot += 3;
         if (!scan)
-            hipLaunchKernelGGL(k_recon, rcgrd, rcblk, 0, st, G, v, P, last ? 1 : 0,
+            hipLaunchKernelGGL(k_recon, rcgrd, rcblk, 0, st, G, v, P, full ? 1 : 0,
                                last ? B.mask : (unsigned long long*)nullptr, B.mw, B.srcs, B.ifc, cnt);
         const unsigned* nifc = scan ? scan + 2 : cnt + 2;
         hipLaunchKernelGGL(k_recon_ifc, dim3(256), pblk, 0, st, G, v, p, P, B.stale, B.ifc, nifc, B.stales, cnt);
@@ -1624,7 +1661,7 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         const double level = levels[k - 1];
         // first sweep: (vof, phi) -> (vt, pt), implicit divergence; vof is
         // the one the previous sub-step's last reconstruction scanned
-        const unsigned* c1 = recon(vof, phi, false, last_scan);
+        const unsigned* c1 = recon(vof, phi, false, last_scan, false);
         const SweepIn s1{vof, phi, P, B.fr, B.fl};
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_flux_pre<true>), pgrd, pblk, 0, st, G, s1, dtv, B.ifc, c1, B.fr, B.fl);
@@ -1640,7 +1677,7 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         SG_HIP(sweep_mark(1));
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, B.vt, B.pt);
         // second sweep: (vt, pt) -> (vof, phi), explicit divergence
-        const unsigned* c2 = recon(B.vt, B.pt, false, nullptr);
+        const unsigned* c2 = recon(B.vt, B.pt, false, nullptr, false);
         const SweepIn s2{B.vt, B.pt, P, B.fr, B.fl};
         if (k % 2 == 1)
             hipLaunchKernelGGL((k_flux_pre<false>), pgrd, pblk, 0, st, G, s2, dtv, B.ifc, c2, B.fr, B.fl);
@@ -1667,7 +1704,7 @@ int clsvof_advect(Ctx* c, double dt, int nsub, const double* levels) {
         }
         hipLaunchKernelGGL(k_bc, rgrd, rblk, 0, st, c->ni, c->nj, vof, phi);
         // reconstruction (full normals out) and re-distancing
-        const unsigned* cnt = recon(vof, phi, true, nullptr);
+        const unsigned* cnt = recon(vof, phi, true, nullptr, k == nsub);
         last_scan = cnt;
         const PairView pv{B.srcs, cnt};
         hipLaunchKernelGGL(k_rd_scatter, pgrd, pblk, 0, st, G, vof, phi, P, pv, B.best, B.hit);
