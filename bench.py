@@ -282,7 +282,7 @@ def stepping(case, stags, precision, steps, warmup, learned=None, width=256, pre
 
 def step_parts(case, stags, precision, steps=3, warmup=2):
     """The whole time step on the swarm (TimeStepper), with the device time
-    of each part from host timers around its synchronous call."""
+    of each part from host timers, the stream drained at each boundary."""
     from simcoflow_amd import _lib
     from simcoflow_amd.solver import TimeStepper
     st = TimeStepper(case, precision=precision, tol=1e-8, max_iter=200, stags=stags)
@@ -304,9 +304,11 @@ def step_parts(case, stags, precision, steps=3, warmup=2):
             _lib.check(lib.sg_time_step_size(ctx, st.cfl, st.vint, st.uref, 0.0, 0.0, ctypes.byref(dt)))
             b = t()
             pj.clsvof_advect_resident(dt.value, st.time)
+            pj.synchronize()
             c = t()
             _lib.check(lib.sg_interface_to_uv_resident(ctx))
             _lib.check(lib.sg_closed_box_bc(ctx))
+            pj.synchronize()
             d = t()
             pj.predictor_resident(dt.value, 0.0, False)
             e = t()

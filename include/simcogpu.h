@@ -9,6 +9,14 @@
  *   ghosted array   (0:ni+1,0:nj+1)    (ni+2)*(nj+2) doubles, element (i,j) at i+j*(ni+2)
  * Host-pointer entry points copy in/out and return after the device work has
  * finished; the *_resident entry points work on the context's device fields.
+ * sg_clsvof_advect_resident, sg_interface_to_uv_resident,
+ * sg_velocity_correct_resident, sg_closed_box_bc and sg_field_copy enqueue
+ * their work on the context's stream and return without waiting for it
+ * (every call of a context uses that one stream, so they stay in order);
+ * sg_synchronize, the downloads and the calls that return device results
+ * (solves, predictor, time step) wait, and an error of the enqueued device
+ * work surfaces there.  A caller that reads the fields through
+ * sg_field_ptr on another stream calls sg_synchronize first.
  *
  * Return codes: 0 OK; negative = argument / HIP / allocation error;
  * positive = numerical (1 no convergence within max_iter, 2 NaN found,

@@ -459,7 +459,7 @@ int sg_velocity_correct_resident(sg_ctx* ctx, double vb) {
     c->vb = vb;
     int rc = velocity_correct(c);
     if (rc) return rc;
-    SG_HIP(hipStreamSynchronize(c->stream));
+    SG_HIP(hipGetLastError());
     return SG_OK;
 }
 
@@ -721,7 +721,7 @@ int sg_interface_to_uv_resident(sg_ctx* ctx) {
     }
     int rc = interface_to_uv(c);
     if (rc) return rc;
-    SG_HIP(hipStreamSynchronize(c->stream));
+    SG_HIP(hipGetLastError());
     return SG_OK;
 }
 
@@ -770,7 +770,7 @@ int sg_clsvof_advect_resident(sg_ctx* ctx, double dt, double time, const sg_clsv
     }
     int rc = clsvof_advect(c, dt, p->nsub, levels.data());
     if (rc) return rc;
-    SG_HIP(hipStreamSynchronize(c->stream));
+    SG_HIP(hipGetLastError());
     c->built = 0;
     return SG_OK;
 }
@@ -1053,7 +1053,7 @@ int sg_closed_box_bc(sg_ctx* ctx) {
     if (!c) return SG_ERR_ARG;
     int rc = sg::closed_box_bc(c);
     if (rc) return rc;
-    SG_HIP(hipStreamSynchronize(c->stream));
+    SG_HIP(hipGetLastError());
     return SG_OK;
 }
 
@@ -1065,7 +1065,6 @@ int sg_field_copy(sg_ctx* ctx, int dst, int src) {
         return SG_ERR_ARG;
     }
     SG_HIP(hipMemcpyAsync(c->field[dst], c->field[src], c->field_bytes[src], hipMemcpyDeviceToDevice, c->stream));
-    SG_HIP(hipStreamSynchronize(c->stream));
     c->built = 0;
     return SG_OK;
 }
