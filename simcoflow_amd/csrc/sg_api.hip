@@ -368,19 +368,21 @@ int sg_poisson_solve_resident(sg_ctx* ctx, sg_solve_info* info) {
         set_error("sg_poisson_solve_resident: call sg_poisson_build_resident first");
         return SG_ERR_STATE;
     }
+    // the build's value guard is read after the solve, which runs either
+    // way (its own NaN test stops it early on bit 1), so no wait comes
+    // between the build and the solve
+    int rc = pcg_solve(c, info);
     int guard = 0;
     SG_HIP(hipMemcpyAsync(&guard, &c->sc->guard, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    // clear the guard bits for the next build
+    const int zero = 0;
+    SG_HIP(hipMemcpyAsync(&c->sc->guard, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
     SG_HIP(hipStreamSynchronize(c->stream));
-    int rc = pcg_solve(c, info);
     if (guard && rc == SG_OK) {
         rc = (guard & 1) ? SG_NAN : SG_GUARD;
         if (info) info->status = rc;
         set_error("projection: NaN or out-of-range matrix/RHS value (ProjectionP_Mod.f90:209-213,322-325)");
     }
-    // clear the guard bits for the next build
-    const int zero = 0;
-    SG_HIP(hipMemcpyAsync(&c->sc->guard, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    SG_HIP(hipStreamSynchronize(c->stream));
     if (rc == SG_NOT_CONVERGED) set_error("projection: PCG did not reach the tolerance within max_iter");
     return rc;
 }
