@@ -370,14 +370,13 @@ int sg_poisson_solve_resident(sg_ctx* ctx, sg_solve_info* info) {
     }
     // the build's value guard is read after the solve, which runs either
     // way (its own NaN test stops it early on bit 1), so no wait comes
-    // between the build and the solve
+    // between the build and the solve.  A solve that returns SG_OK has
+    // ended with the true-residual read-back, which copied every scalar,
+    // the guard included, to the host mirror.
     int rc = pcg_solve(c, info);
-    int guard = 0;
-    SG_HIP(hipMemcpyAsync(&guard, &c->sc->guard, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    // clear the guard bits for the next build
-    const int zero = 0;
-    SG_HIP(hipMemcpyAsync(&c->sc->guard, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    SG_HIP(hipStreamSynchronize(c->stream));
+    const int guard = rc == SG_OK ? c->sc_host->guard : 0;
+    // clear the guard bits for the next build (stream-ordered)
+    SG_HIP(hipMemsetAsync(&c->sc->guard, 0, sizeof(int), c->stream));
     if (guard && rc == SG_OK) {
         rc = (guard & 1) ? SG_NAN : SG_GUARD;
         if (info) info->status = rc;
