@@ -39,13 +39,17 @@ __global__ void __launch_bounds__(NTS) k_dt(StepIn a, double* part) {
         const double uu = a.u[gh(a.ni, i, j)], vv = a.v[gh(a.ni, i, j)];
         dt = fmin(dt, a.cfl * dx / fabs(uu));
         dt = fmin(dt, a.cfl * dy / fabs(vv));
-        dt = fmin(fmin(dt, 2.0 * a.cfl * dy / (fabs(vv) + sqrt(vv * vv + 4.0 * dy / a.Fr))),
-                  a.cfl * dy / (a.vint / a.uref));
-        dt = fmin(dt, a.cfl * dy / (a.vint / a.uref));
-        dt = fmin(dt, a.cfl * dy / fabs(a.vs));
-        dt = fmin(dt, (-a.vs + sqrt(fmax(fabs(a.vs * a.vs + 2.0 * fabs(a.asy) * 0.05 * dy), 1e-30))) /
-                          fabs(a.asy + 1.e-20));
-        if (i == 1) dt = fmin(dt, a.cfl * a.udx[0] / fabs(a.u[gh(a.ni, 0, j)]));
+        dt = fmin(dt, 2.0 * a.cfl * dy / (fabs(vv) + sqrt(vv * vv + 4.0 * dy / a.Fr)));
+        if (i == 1) {
+            // the terms that depend on the row alone, once per row: the
+            // minimum is exact and fmin drops a NaN in any order, so where
+            // they are taken does not change dt
+            dt = fmin(dt, a.cfl * dy / (a.vint / a.uref));
+            dt = fmin(dt, a.cfl * dy / fabs(a.vs));
+            dt = fmin(dt, (-a.vs + sqrt(fmax(fabs(a.vs * a.vs + 2.0 * fabs(a.asy) * 0.05 * dy), 1e-30))) /
+                              fabs(a.asy + 1.e-20));
+            dt = fmin(dt, a.cfl * a.udx[0] / fabs(a.u[gh(a.ni, 0, j)]));
+        }
     }
     for (int o = 32; o >= 1; o >>= 1) dt = fmin(dt, __shfl_xor(dt, o, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dt;
