@@ -180,7 +180,8 @@ typedef struct sg_stag_geometry {
 typedef struct sg_predictor_info {
     int iterations_u, iterations_v;  /* BiCGStab iterations of the two systems */
     double relres_u, relres_v;       /* final ||D^-1(b-Ax)||/||D^-1 b||, D = diag(A) */
-    double ms;                       /* device time of the whole call */
+    double ms;                       /* device time of the assembly and the two solves (the
+                                        scatter that ends the call is enqueued, not waited for) */
 } sg_predictor_info;
 
 int sg_set_staggered_geometry(sg_ctx* ctx, int which, const sg_stag_geometry* g);
