@@ -178,7 +178,8 @@ typedef struct sg_stag_geometry {
 } sg_stag_geometry;
 
 typedef struct sg_predictor_info {
-    int iterations_u, iterations_v;  /* BiCGStab iterations of the two systems */
+    int iterations_u, iterations_v;  /* iterations of the two systems: Jacobi sweeps, or BiCGStab
+                                        iterations with momentum_method = 1 */
     double relres_u, relres_v;       /* final ||D^-1(b-Ax)||/||D^-1 b||, D = diag(A) */
     double ms;                       /* device time of the assembly and the two solves (the
                                         scatter that ends the call is enqueued, not waited for) */
