@@ -271,6 +271,7 @@ struct Ctx {
     PcgScalars* sc_host = nullptr; // pinned host mirror
     Reducer red;
     int built = 0;      // operator + rhs built
+    double pcg_rate = 0.0;  // log (r,r) ratio per PCG iteration seen by the last solve (0: none yet)
     int mg_ready = 0;   // hierarchy set up for the current operator
     double vb = 0.0;
     // captured pair of PCG iterations, per precision (kernel arguments are

@@ -558,8 +558,11 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
     int it = 0;  // iterations launched (those past convergence do nothing)
     int replacements = 0;
     // convergence rate per iteration (log of the (r,r) ratio) between host
-    // checks, 0 while unknown
-    double rate = 0.0;
+    // checks; the last solve's to begin with (the systems of consecutive
+    // time steps converge alike, so the first launch usually reaches the
+    // tolerance), 0 while unknown.  Iterations past convergence do nothing,
+    // so the launch sizes never change the result.
+    double rate = c->pcg_rate;
     for (;;) {
         while (c->sc_host->active && it < max_iter) {
             int n = 4;  // before the rate is known
@@ -578,8 +581,10 @@ int pcg_solve(Ctx* c, sg_solve_info* info) {
             SG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
             SG_HIP(hipStreamSynchronize(s));
             const int done = c->sc_host->iters - k0;
-            if (done > 0 && c->sc_host->rr > 0.0 && c->sc_host->rr < rr0)
+            if (done > 0 && c->sc_host->rr > 0.0 && c->sc_host->rr < rr0) {
                 rate = log(c->sc_host->rr / rr0) / done;
+                c->pcg_rate = rate;
+            }
         }
         // residual replacement at convergence: the recursive residual drifts
         // from b - A x by rounding; recompute it in FP64 from x and restart CG
